@@ -1,0 +1,88 @@
+"""ctypes binding of libfem25hip.so (C ABI in include/fem25hip.h).
+
+The library is loaded from the package directory; there is no fallback: if
+it is missing the import of this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfem25hip.so")
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+fp = C.POINTER(C.c_float)
+i64p = C.POINTER(C.c_int64)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("nord", C.c_int), ("nelem", C.c_int), ("nnpg", C.c_int), ("ieng", ip),
+                ("xlocs", dp), ("zlocs", dp), ("xd", dp), ("zd", dp), ("cdomain", C.c_char_p),
+                ("cnnpg", C.c_char_p), ("xwidth", C.c_double), ("zwidth", C.c_double),
+                ("pml", C.c_double * 4), ("pml_srf", C.c_double * 4)]
+
+
+F25_C64, F25_C128 = 0, 1
+F25_APPLY_F64, F25_APPLY_F32, F25_APPLY_F32_REFINED = 0, 1, 2
+
+EXPORTS = {
+    "f25_mesh_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
+    "f25_mesh_create_with_graph": (C.c_int, [C.POINTER(MeshDesc), C.c_int, ip, ip, ip,
+                                             C.POINTER(C.c_void_p)]),
+    "f25_mesh_destroy": (C.c_int, [C.c_void_p]),
+    "f25_mesh_sizes": (C.c_int, [C.c_void_p, i64p]),
+    "f25_mesh_graph": (C.c_int, [C.c_void_p, ip, ip, ip, ip, C.c_char_p]),
+    "f25_set_model": (C.c_int, [C.c_void_p, dp, dp]),
+    "f25_asmble": (C.c_int, [C.c_void_p, C.c_char, C.c_double, C.c_double, C.c_int, C.c_void_p]),
+    "f25_asmble_dev": (C.c_int, [C.c_void_p, C.c_char, C.c_double, C.c_double, C.c_int, C.c_void_p]),
+    "f25_csr_apply_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "f25_asmb_peff": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "f25_apply_batch_dev": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_int, C.c_void_p,
+                                      C.c_void_p]),
+    "f25_apply_batch": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_int, C.c_void_p, C.c_void_p]),
+    "f25_gradient": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, dp]),
+    "f25_gradient_dev": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]),
+    "f25_inv_setup": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_int, C.c_int, C.c_int,
+                                ip, ip]),
+    "f25_inv_get": (C.c_int, [C.c_void_p, ip, ip, ip, ip, dp]),
+    "f25_cfmat_dist": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
+    "f25_adjflam_dist": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, fp]),
+    "f25_gn_fv": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp, C.c_void_p]),
+    "f25_gn_fhw": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, dp]),
+    "f25_gn_diag": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp]),
+    "f25_last_error": (C.c_char_p, []),
+    "f25_last_kernel_ms": (C.c_double, [C.c_char_p]),
+    "f25_selftest_mfma": (C.c_int, [dp]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads and types the library (raises OSError if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is not built; run python -m fem25_amd.build")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class Fem25Error(RuntimeError):
+    def __init__(self, code: int, where: str):
+        msg = load().f25_last_error().decode(errors="replace")
+        super().__init__(f"{where}: error {code}: {msg}")
+        self.code = code
+
+
+def check(code: int, where: str) -> None:
+    if code:
+        raise Fem25Error(code, where)

@@ -1,0 +1,294 @@
+"""Host-side interface of the MI355X path, mirroring the reference's calls.
+
+`MeshInfo` holds what the reference keeps in TYPE(MESH_INFO) for this path
+(fwd_struc.h:10-141) plus the device-resident copies the kernels read.  The
+module-level functions keep the reference routines' names and argument
+meaning:
+
+    gen_graph25(mesh)                               GEN_GRAPH25 numbering (graph25_driver.f90:2)
+    asmble_driver(nzloc, cftype, freq, py, msh)     ASMBLE_DRIVER (asmble.f90:1)  -> (cdist, ierr)
+    asmb_peff(msh, cdist, ue)                       ASMB_PEFF (asmble.f90:318)
+    cfmat_dist(nwork, n, freq, py, wave, msh, inv)  CFMAT_DIST (hessloc.f90:1)     -> (fmat, ierr)
+    adjflam_dist(msh, inv, fmat, adjlam)            ADJFLAM_DIST (adjgrad.f90:88) -> gprod
+
+Every call goes to libfem25hip.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from . import _lib
+from .configs import PML, Mesh, Model
+from ._lib import check, dp, fp, ip
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+@dataclasses.dataclass
+class Graph:
+    """GLL numbering in the reference's conventions (1-based)."""
+    nnp: int
+    ndof: int
+    nz: int
+    ien: np.ndarray    # (nelem, nen)      IEN(NEN,NELEM)
+    lm: np.ndarray     # (nelem, nen, 3)   LM(3,NEN,NELEM), 0 = fixed
+    irptr: np.ndarray  # (ndof+1,)
+    jcptr: np.ndarray  # (nz,)
+    cnp: bytes
+
+
+@dataclasses.dataclass
+class InvInfo:
+    """The INV_INFO fields of the virtual-force path (fwd_struc.h:249-368)."""
+    nnpinv: int
+    ncon: int
+    nnpgl: int
+    nz_fdist: int
+    mbufrhs: int
+    na35: int
+    maskg: np.ndarray
+    mygrad: np.ndarray
+    jcsc_fdist: np.ndarray
+    icsc_fdist: np.ndarray
+    elem_wts: np.ndarray
+    cinvtype: bytes = b"PP"
+    vpvs: float = 3 ** 0.5
+
+
+class MeshInfo:
+    """Mesh + model on the current HIP device."""
+
+    def __init__(self, mesh: Mesh, pml: PML | None = None, pml_srf: PML | None = None,
+                 graph: Graph | None = None):
+        L = _lib.load()
+        self.mesh = mesh
+        pml = pml or PML()
+        pml_srf = pml_srf or pml
+        self._keep = [np.ascontiguousarray(mesh.ieng, np.int32), np.ascontiguousarray(mesh.xlocs),
+                      np.ascontiguousarray(mesh.zlocs), np.ascontiguousarray(mesh.xd),
+                      np.ascontiguousarray(mesh.zd)]
+        d = _lib.MeshDesc()
+        d.nord, d.nelem, d.nnpg = mesh.nord, mesh.nelem, mesh.nnpg
+        d.ieng = _p(self._keep[0], ip)
+        d.xlocs, d.zlocs = _p(self._keep[1], dp), _p(self._keep[2], dp)
+        d.xd, d.zd = _p(self._keep[3], dp), _p(self._keep[4], dp)
+        d.cdomain, d.cnnpg = mesh.cdomain, mesh.cnnpg
+        d.xwidth, d.zwidth = mesh.xwidth, mesh.zwidth
+        d.pml[:] = [pml.rcoeff, pml.dmax, pml.kappa, pml.fcent]
+        d.pml_srf[:] = [pml_srf.rcoeff, pml_srf.dmax, pml_srf.kappa, pml_srf.fcent]
+        self._desc = d
+        h = C.c_void_p()
+        if graph is None:
+            check(L.f25_mesh_create(C.byref(d), C.byref(h)), "f25_mesh_create")
+        else:
+            lm = np.ascontiguousarray(graph.lm, np.int32)
+            irp = np.ascontiguousarray(graph.irptr, np.int32)
+            jcp = np.ascontiguousarray(graph.jcptr, np.int32)
+            check(L.f25_mesh_create_with_graph(C.byref(d), graph.ndof, _p(lm, ip), _p(irp, ip),
+                                               _p(jcp, ip), C.byref(h)),
+                  "f25_mesh_create_with_graph")
+        self.h = h
+        s = np.zeros(6, np.int64)
+        check(L.f25_mesh_sizes(h, s.ctypes.data_as(_lib.i64p)), "f25_mesh_sizes")
+        self.nnp, self.ndof, self.nz, self.ncolour, self.nen, self.nq1 = (int(v) for v in s)
+        self.inv: InvInfo | None = None
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().f25_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- numbering ----------------------------------------------------------
+    def graph(self) -> Graph:
+        m = self.mesh
+        ien = np.zeros((m.nelem, self.nen), np.int32)
+        lm = np.zeros((m.nelem, self.nen, 3), np.int32)
+        irptr = np.zeros(self.ndof + 1, np.int32)
+        jcptr = np.zeros(self.nz, np.int32)
+        cnp = C.create_string_buffer(self.ndof)
+        check(_lib.load().f25_mesh_graph(self.h, _p(ien, ip), _p(lm, ip), _p(irptr, ip),
+                                         _p(jcptr, ip), cnp), "f25_mesh_graph")
+        return Graph(self.nnp, self.ndof, self.nz, ien, lm, irptr, jcptr, cnp.raw[:self.ndof])
+
+    # -- model --------------------------------------------------------------
+    def set_model(self, model: Model):
+        self.model = model
+        ec = np.asfortranarray(model.ecoeff)
+        dens = np.ascontiguousarray(model.dens)
+        check(_lib.load().f25_set_model(self.h, _p(dens, dp), ec.ctypes.data_as(dp)),
+              "f25_set_model")
+
+    # -- assembled operator -------------------------------------------------
+    def asmble(self, cftype: bytes, freq: float, py: float, complex128: bool = False) -> np.ndarray:
+        out = np.zeros(self.nz, np.complex128 if complex128 else np.complex64)
+        check(_lib.load().f25_asmble(self.h, cftype, freq, py,
+                                     _lib.F25_C128 if complex128 else _lib.F25_C64,
+                                     out.ctypes.data_as(C.c_void_p)), "f25_asmble")
+        return out
+
+    def asmble_dev(self, cftype: bytes, freq: float, py: float, out) -> None:
+        """out: torch complex64/complex128 tensor of NZ values on the GPU."""
+        import torch
+        prec = _lib.F25_C128 if out.dtype == torch.complex128 else _lib.F25_C64
+        check(_lib.load().f25_asmble_dev(self.h, cftype, freq, py, prec, C.c_void_p(out.data_ptr())),
+              "f25_asmble_dev")
+
+    def csr_apply_dev(self, s, x, y) -> None:
+        import torch
+        prec = _lib.F25_C128 if s.dtype == torch.complex128 else _lib.F25_C64
+        check(_lib.load().f25_csr_apply_dev(self.h, prec, C.c_void_p(s.data_ptr()),
+                                            C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())),
+              "f25_csr_apply_dev")
+
+    # -- batched element-block product ----------------------------------------
+    def apply_batch_dev(self, omega, ky, x, y, mode: int = _lib.F25_APPLY_F64) -> None:
+        """x, y: torch tensors [ndof, npair] (complex128, or complex64 for F32)."""
+        om = np.ascontiguousarray(omega, np.float64)
+        kv = np.ascontiguousarray(ky, np.float64)
+        check(_lib.load().f25_apply_batch_dev(self.h, om.size, _p(om, dp), _p(kv, dp), mode,
+                                              C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())),
+              "f25_apply_batch_dev")
+
+    def apply_batch(self, omega, ky, x: np.ndarray, mode: int = _lib.F25_APPLY_F64) -> np.ndarray:
+        """x: [npair, ndof] complex128 host array; returns S_b x_b per row."""
+        om = np.ascontiguousarray(omega, np.float64)
+        kv = np.ascontiguousarray(ky, np.float64)
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.zeros_like(x)
+        check(_lib.load().f25_apply_batch(self.h, om.size, _p(om, dp), _p(kv, dp), mode,
+                                          x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p)),
+              "f25_apply_batch")
+        return y
+
+    # -- gradient -----------------------------------------------------------
+    def gradient(self, omega, ky, u: np.ndarray, lam: np.ndarray) -> np.ndarray:
+        """sum_t Re(lam_t^H dS_t/dm u_t) for m = (Vp, Vs, rho) at anchors: (nnpg, 3)."""
+        om = np.ascontiguousarray(np.atleast_1d(omega), np.float64)
+        kv = np.ascontiguousarray(np.atleast_1d(ky), np.float64)
+        u = np.ascontiguousarray(u, np.complex128).reshape(om.size, -1)
+        lam = np.ascontiguousarray(lam, np.complex128).reshape(om.size, -1)
+        g = np.zeros((self.mesh.nnpg, 3))
+        check(_lib.load().f25_gradient(self.h, om.size, _p(om, dp), _p(kv, dp),
+                                       u.ctypes.data_as(C.c_void_p), lam.ctypes.data_as(C.c_void_p),
+                                       _p(g, dp)), "f25_gradient")
+        return g
+
+    def gradient_dev(self, omega, ky, u, lam, g) -> None:
+        """u, lam: torch complex128 [ntrip, ndof] on the GPU; g: torch float64 (nnpg, 3), accumulated."""
+        om = np.ascontiguousarray(omega, np.float64)
+        kv = np.ascontiguousarray(ky, np.float64)
+        check(_lib.load().f25_gradient_dev(self.h, om.size, _p(om, dp), _p(kv, dp),
+                                           C.c_void_p(u.data_ptr()), C.c_void_p(lam.data_ptr()),
+                                           C.c_void_p(g.data_ptr())), "f25_gradient_dev")
+
+    # -- Gauss-Newton element products -----------------------------------------
+    def gn_fv(self, omega, ky, u: np.ndarray, v: np.ndarray) -> np.ndarray:
+        om = np.ascontiguousarray(np.atleast_1d(omega), np.float64)
+        kv = np.ascontiguousarray(np.atleast_1d(ky), np.float64)
+        u = np.ascontiguousarray(u, np.complex128).reshape(om.size, -1)
+        v = np.ascontiguousarray(v, np.float64)
+        out = np.zeros_like(u)
+        check(_lib.load().f25_gn_fv(self.h, om.size, _p(om, dp), _p(kv, dp),
+                                    u.ctypes.data_as(C.c_void_p), _p(v, dp),
+                                    out.ctypes.data_as(C.c_void_p)), "f25_gn_fv")
+        return out
+
+    def gn_fhw(self, omega, ky, u, w) -> np.ndarray:
+        om = np.ascontiguousarray(np.atleast_1d(omega), np.float64)
+        kv = np.ascontiguousarray(np.atleast_1d(ky), np.float64)
+        u = np.ascontiguousarray(u, np.complex128).reshape(om.size, -1)
+        w = np.ascontiguousarray(w, np.complex128).reshape(om.size, -1)
+        g = np.zeros((self.mesh.nnpg, 3))
+        check(_lib.load().f25_gn_fhw(self.h, om.size, _p(om, dp), _p(kv, dp),
+                                     u.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p),
+                                     _p(g, dp)), "f25_gn_fhw")
+        return g
+
+    # -- inversion bookkeeping -------------------------------------------------
+    def inv_setup(self, cinvtype: bytes = b"PP", vpvs: float = 3 ** 0.5, linrec: bool = True,
+                  lkbdry: bool = True, mrdof1=None) -> InvInfo:
+        sizes = np.zeros(6, np.int32)
+        nrec = 0 if mrdof1 is None else len(mrdof1)
+        mr = None if mrdof1 is None else np.ascontiguousarray(mrdof1, np.int32)
+        check(_lib.load().f25_inv_setup(self.h, cinvtype, vpvs, int(linrec), int(lkbdry), nrec,
+                                        None if mr is None else _p(mr, ip), _p(sizes, ip)),
+              "f25_inv_setup")
+        nnpinv, ncon, nnpgl, nzf, mbuf, na35 = (int(v) for v in sizes)
+        maskg = np.zeros(self.mesh.nnpg, np.int32)
+        mygrad = np.zeros(nnpgl, np.int32)
+        jcsc = np.zeros(nnpgl + 1, np.int32)
+        icsc = np.zeros(nzf, np.int32)
+        ew = np.zeros(self.mesh.nelem)
+        check(_lib.load().f25_inv_get(self.h, _p(maskg, ip), _p(mygrad, ip), _p(jcsc, ip),
+                                      _p(icsc, ip), _p(ew, dp)), "f25_inv_get")
+        self.inv = InvInfo(nnpinv, ncon, nnpgl, nzf, mbuf, na35, maskg, mygrad, jcsc, icsc, ew,
+                           cinvtype, vpvs)
+        return self.inv
+
+
+# ---------------------------------------------------------------------------
+# Reference-named entry points.
+
+def gen_graph25(mesh: Mesh, pml: PML | None = None) -> Graph:
+    """GLL numbering and CSR pattern with GEN_GRAPH25's semantics."""
+    mi = MeshInfo(mesh, pml)
+    try:
+        return mi.graph()
+    finally:
+        mi.close()
+
+
+def asmble_driver(nzloc: int, cftype: bytes, freq: float, py: float, msh: MeshInfo):
+    """ASMBLE_DRIVER(NZLOC,CFTYPE,FREQ,PY,MSH,CDIST,IERR): returns (CDIST COMPLEX*8, IERR)."""
+    if nzloc != msh.nz:
+        return np.zeros(max(nzloc, 0), np.complex64), 1
+    try:
+        return msh.asmble(cftype, freq, py), 0
+    except _lib.Fem25Error as e:
+        return np.zeros(nzloc, np.complex64), e.code
+
+
+def asmb_peff(msh: MeshInfo, cdist: np.ndarray, ue: np.ndarray) -> np.ndarray:
+    """ASMB_PEFF for one rank: the Bielak effective force, COMPLEX*8."""
+    cdist = np.ascontiguousarray(cdist, np.complex64)
+    ue = np.ascontiguousarray(ue, np.complex64)
+    out = np.zeros(msh.ndof, np.complex64)
+    check(_lib.load().f25_asmb_peff(msh.h, cdist.ctypes.data_as(C.c_void_p),
+                                    ue.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p)),
+          "f25_asmb_peff")
+    return out
+
+
+def cfmat_dist(nwork: int, n: int, freq: float, py: float, wave: np.ndarray, msh: MeshInfo,
+               inv: InvInfo):
+    """CFMAT_DIST(NWORK,N,FREQ,PY,WAVE,MSH,INV,FMAT_DIST,IERR) -> (FMAT_DIST, IERR)."""
+    if nwork != inv.nz_fdist or n != msh.ndof:
+        return np.zeros(max(nwork, 0), np.complex64), 1
+    wave = np.ascontiguousarray(wave, np.complex64)
+    out = np.zeros(nwork, np.complex64)
+    code = _lib.load().f25_cfmat_dist(msh.h, freq, py, wave.ctypes.data_as(C.c_void_p),
+                                      out.ctypes.data_as(C.c_void_p))
+    return out, int(code)
+
+
+def adjflam_dist(msh: MeshInfo, inv: InvInfo, fmat: np.ndarray, adjlam: np.ndarray) -> np.ndarray:
+    """ADJFLAM_DIST for one rank: GPROD(NA35) = -Re(F^H lambda), REAL*4."""
+    fmat = np.ascontiguousarray(fmat, np.complex64)
+    adjlam = np.ascontiguousarray(adjlam, np.complex64)
+    g = np.zeros(inv.na35, np.float32)
+    check(_lib.load().f25_adjflam_dist(msh.h, fmat.ctypes.data_as(C.c_void_p),
+                                       adjlam.ctypes.data_as(C.c_void_p), _p(g, fp)),
+          "f25_adjflam_dist")
+    return g

@@ -1,0 +1,901 @@
+// C ABI (include/fem25hip.h): mesh handles, device residency and launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "f25_internal.h"
+#include "kernels.hip"
+
+using namespace f25;
+using namespace f25k;
+
+static thread_local std::string g_err;
+static std::map<std::string, double> g_kernel_ms;
+static std::mutex g_ms_mu;
+
+#define HIPCHK(x)                                                                   \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess) {                                                     \
+            g_err = std::string(#x) + ": " + hipGetErrorString(e_);                 \
+            return F25_ERR_HIP;                                                     \
+        }                                                                           \
+    } while (0)
+#define FAIL(code, msg)     \
+    do {                    \
+        g_err = (msg);      \
+        return (code);      \
+    } while (0)
+#define TRY(x)                  \
+    do {                        \
+        int r_ = (x);           \
+        if (r_) return r_;      \
+    } while (0)
+
+template <typename T>
+static int dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc((void **)p, n * sizeof(T)));
+    return 0;
+}
+template <typename T>
+static int upload(T **p, const T *src, size_t n, hipStream_t s) {
+    TRY(dalloc(p, n));
+    if (n) HIPCHK(hipMemcpyAsync(*p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return 0;
+}
+template <typename T>
+static void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+template <typename T>
+static void fill_async(T *p, size_t n, T v, hipStream_t s) {
+    if (!n) return;
+    int nb = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(256), 0, s, p, n, v);
+}
+
+struct f25_mesh_s {
+    // host copies of the description
+    f25_mesh_desc d{};
+    std::vector<int> ieng;
+    std::vector<double> xl, zl, xd, zd;
+    std::string cdom, cnnpg;
+    Tables tab;
+    Graph g;
+    InvInfo inv;
+    bool have_inv = false;
+    std::vector<int> order, cptr, pml_order, pml_cptr;
+    std::vector<double> rho, lam, mu;
+    bool have_model = false;
+    hipStream_t st = nullptr;
+    int dev = 0;
+    // device
+    int *d_ieng = nullptr, *d_lm = nullptr, *d_order = nullptr, *d_pml_order = nullptr, *d_err = nullptr;
+    double *d_xl = nullptr, *d_zl = nullptr, *d_xd = nullptr, *d_zd = nullptr;
+    double *d_shl = nullptr, *d_wq = nullptr, *d_xq = nullptr;
+    char *d_cdom = nullptr, *d_cnp = nullptr;
+    double *d_rho = nullptr, *d_lam = nullptr, *d_mu = nullptr;
+    double *d_blk = nullptr;
+    float *d_blk_hi = nullptr, *d_blk_lo = nullptr;
+    double *d_dblk = nullptr;   // blocks of a model perturbation (GN products)
+    int64_t *d_irptr = nullptr;
+    int *d_jcptr = nullptr;
+    signed char *d_comp = nullptr;
+    double *d_v0 = nullptr, *d_v1 = nullptr, *d_v2 = nullptr;
+    bool basis_ok = false;
+    double *d_part = nullptr;
+    int *d_mconn = nullptr;
+    int ncon_anchor = 0;
+    // inversion (CFMAT/ADJFLAM)
+    int *d_col_elem = nullptr, *d_col_corner = nullptr, *d_mygrad = nullptr, *d_jcsc = nullptr, *d_icsc = nullptr;
+    double *d_col_cp = nullptr, *d_col_cs = nullptr, *d_ew = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+extern "C" const char *f25_last_error(void) { return g_err.c_str(); }
+
+extern "C" double f25_last_kernel_ms(const char *family) {
+    std::lock_guard<std::mutex> lk(g_ms_mu);
+    auto it = g_kernel_ms.find(family ? family : "");
+    return it == g_kernel_ms.end() ? -1.0 : it->second;
+}
+
+static void record_ms(const char *fam, double ms) {
+    std::lock_guard<std::mutex> lk(g_ms_mu);
+    g_kernel_ms[fam] = ms;
+}
+
+static int check_err(f25_mesh *m) {
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, m->d_err, sizeof(int), hipMemcpyDeviceToHost, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    if (h) {
+        int z = 0;
+        HIPCHK(hipMemcpy(m->d_err, &z, sizeof(int), hipMemcpyHostToDevice));
+        if (h == 2) FAIL(F25_ERR_JACOBIAN, "Jacobian undefined or negative");
+        if (h == 3) FAIL(F25_ERR_PATTERN, "element (row, col) pair missing from the CSR pattern");
+        FAIL(F25_ERR_HIP, "device error flag");
+    }
+    return 0;
+}
+
+static int upload_mesh(f25_mesh *m) {
+    const f25_mesh_desc &d = m->d;
+    hipStream_t s = m->st;
+    const int nen = m->tab.nen;
+    TRY(upload(&m->d_ieng, m->ieng.data(), m->ieng.size(), s));
+    TRY(upload(&m->d_xl, m->xl.data(), m->xl.size(), s));
+    TRY(upload(&m->d_zl, m->zl.data(), m->zl.size(), s));
+    TRY(upload(&m->d_xd, m->xd.data(), m->xd.size(), s));
+    TRY(upload(&m->d_zd, m->zd.data(), m->zd.size(), s));
+    TRY(upload(&m->d_cdom, m->cdom.data(), m->cdom.size(), s));
+    TRY(upload(&m->d_shl, m->tab.shl.data(), m->tab.shl.size(), s));
+    TRY(upload(&m->d_wq, m->tab.wq.data(), m->tab.wq.size(), s));
+    TRY(upload(&m->d_xq, m->tab.xq.data(), m->tab.xq.size(), s));
+    std::vector<int> lm0(m->g.lm.size());
+    for (size_t i = 0; i < lm0.size(); ++i) lm0[i] = m->g.lm[i] - 1;   // 0-based, -1 fixed
+    TRY(upload(&m->d_lm, lm0.data(), lm0.size(), s));
+    TRY(upload(&m->d_order, m->order.data(), m->order.size(), s));
+    TRY(upload(&m->d_pml_order, m->pml_order.data(), m->pml_order.size(), s));
+    TRY(dalloc(&m->d_err, 1));
+    HIPCHK(hipMemsetAsync(m->d_err, 0, sizeof(int), s));
+    // anchor connectivity for the gradient gather
+    std::vector<int> cnt(d.nnpg, 0);
+    for (int e = 0; e < d.nelem; ++e)
+        for (int k = 0; k < 4; ++k) cnt[m->ieng[4 * e + k] - 1]++;
+    m->ncon_anchor = *std::max_element(cnt.begin(), cnt.end());
+    std::vector<int> mc((size_t)m->ncon_anchor * d.nnpg, 0), f(d.nnpg, 0);
+    for (int e = 0; e < d.nelem; ++e)
+        for (int k = 0; k < 4; ++k) {
+            int a = m->ieng[4 * e + k] - 1;
+            mc[(size_t)f[a]++ * d.nnpg + a] = e + 1;
+        }
+    TRY(upload(&m->d_mconn, mc.data(), mc.size(), s));
+    TRY(dalloc(&m->d_part, (size_t)d.nelem * 12));
+    (void)nen;
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+static int init_handle(f25_mesh *m, const f25_mesh_desc *desc) {
+    if (!desc || desc->nord < 2 || desc->nord > 6 || desc->nelem <= 0 || desc->nnpg <= 0 ||
+        !desc->ieng || !desc->xlocs || !desc->zlocs || !desc->cdomain || !desc->cnnpg)
+        FAIL(F25_ERR_ARG, "invalid mesh description (NORD must be 2..6)");
+    m->d = *desc;
+    m->ieng.assign(desc->ieng, desc->ieng + 4 * (size_t)desc->nelem);
+    for (int v : m->ieng)
+        if (v < 1 || v > desc->nnpg) FAIL(F25_ERR_ARG, "IENG entry out of range");
+    m->xl.assign(desc->xlocs, desc->xlocs + desc->nnpg);
+    m->zl.assign(desc->zlocs, desc->zlocs + desc->nnpg);
+    m->xd.assign(desc->nnpg, 0.0);
+    m->zd.assign(desc->nnpg, 0.0);
+    if (desc->xd) m->xd.assign(desc->xd, desc->xd + desc->nnpg);
+    if (desc->zd) m->zd.assign(desc->zd, desc->zd + desc->nnpg);
+    m->cdom.assign(desc->cdomain, desc->nelem);
+    m->cnnpg.assign(desc->cnnpg, 2 * (size_t)desc->nnpg);
+    m->d.ieng = m->ieng.data();
+    m->d.xlocs = m->xl.data();
+    m->d.zlocs = m->zl.data();
+    m->d.xd = m->xd.data();
+    m->d.zd = m->zd.data();
+    m->d.cdomain = m->cdom.data();
+    m->d.cnnpg = m->cnnpg.data();
+    make_tables(desc->nord, m->tab);
+    HIPCHK(hipGetDevice(&m->dev));
+    HIPCHK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&m->ev0));
+    HIPCHK(hipEventCreate(&m->ev1));
+    // colouring: all elements, and the PML elements on their own
+    colour_elements(desc->nelem, desc->nnpg, m->ieng.data(), m->order, m->cptr);
+    std::vector<int> pmle, pieng;
+    for (int e = 0; e < desc->nelem; ++e)
+        if (m->cdom[e] == 'A') pmle.push_back(e);
+    if (!pmle.empty()) {
+        for (int e : pmle)
+            for (int k = 0; k < 4; ++k) pieng.push_back(m->ieng[4 * e + k]);
+        std::vector<int> ord;
+        colour_elements((int)pmle.size(), desc->nnpg, pieng.data(), ord, m->pml_cptr);
+        m->pml_order.resize(ord.size());
+        for (size_t i = 0; i < ord.size(); ++i) m->pml_order[i] = pmle[ord[i]];
+    } else {
+        m->pml_cptr.assign(1, 0);
+    }
+    return 0;
+}
+
+extern "C" int f25_mesh_create(const f25_mesh_desc *desc, f25_mesh **out) {
+    if (!out) FAIL(F25_ERR_ARG, "null output handle");
+    *out = nullptr;
+    std::unique_ptr<f25_mesh> m(new f25_mesh_s());
+    TRY(init_handle(m.get(), desc));
+    int r = build_numbering(m->d, m->tab, m->g);
+    if (r) FAIL(r, "GLL numbering failed");
+    TRY(upload_mesh(m.get()));
+    *out = m.release();
+    return 0;
+}
+
+extern "C" int f25_mesh_create_with_graph(const f25_mesh_desc *desc, int ndof, const int *lm,
+                                          const int *irptr, const int *jcptr, f25_mesh **out) {
+    if (!out || !lm || ndof <= 0) FAIL(F25_ERR_ARG, "invalid graph arguments");
+    *out = nullptr;
+    std::unique_ptr<f25_mesh> m(new f25_mesh_s());
+    TRY(init_handle(m.get(), desc));
+    const int nen = m->tab.nen;
+    m->g.ndof = ndof;
+    m->g.lm.assign(lm, lm + (size_t)desc->nelem * nen * 3);
+    for (int v : m->g.lm)
+        if (v < 0 || v > ndof) FAIL(F25_ERR_ARG, "LM entry out of range");
+    m->g.nnp = 0;
+    m->g.cnp.assign(ndof, ' ');
+    for (char dom : {'A', 'E', 'I'})
+        for (int e = 0; e < desc->nelem; ++e) {
+            if (m->cdom[e] != dom) continue;
+            for (int k = 0; k < 3 * nen; ++k) {
+                int dd = m->g.lm[(size_t)e * 3 * nen + k];
+                if (dd <= 0) continue;
+                char &c = m->g.cnp[dd - 1];
+                if (dom == 'A') c = 'A';
+                else if (dom == 'E') { if (c == ' ') c = 'E'; }
+                else if (c == ' ') c = 'I';
+                else if (c == 'E') c = 'B';
+            }
+        }
+    if (irptr) {
+        m->g.irptr.resize(ndof + 1);
+        for (int i = 0; i <= ndof; ++i) m->g.irptr[i] = (int64_t)irptr[i] - 1;
+        m->g.jcptr.assign(jcptr, jcptr + m->g.irptr[ndof]);
+        m->g.have_csr = true;
+    }
+    TRY(upload_mesh(m.get()));
+    *out = m.release();
+    return 0;
+}
+
+extern "C" int f25_mesh_destroy(f25_mesh *m) {
+    if (!m) return 0;
+    (void)hipSetDevice(m->dev);
+    if (m->st) (void)hipStreamSynchronize(m->st);
+    dfree(m->d_ieng); dfree(m->d_lm); dfree(m->d_order); dfree(m->d_pml_order); dfree(m->d_err);
+    dfree(m->d_xl); dfree(m->d_zl); dfree(m->d_xd); dfree(m->d_zd); dfree(m->d_shl); dfree(m->d_wq);
+    dfree(m->d_xq); dfree(m->d_cdom); dfree(m->d_cnp); dfree(m->d_rho); dfree(m->d_lam); dfree(m->d_mu);
+    dfree(m->d_blk); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_dblk); dfree(m->d_irptr);
+    dfree(m->d_jcptr); dfree(m->d_comp); dfree(m->d_v0); dfree(m->d_v1); dfree(m->d_v2);
+    dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_col_elem); dfree(m->d_col_corner);
+    dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_col_cp); dfree(m->d_col_cs);
+    dfree(m->d_ew);
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->st) (void)hipStreamDestroy(m->st);
+    delete m;
+    return 0;
+}
+
+static int ensure_csr(f25_mesh *m) {
+    if (!m->g.have_csr) {
+        int r = build_csr(m->d, m->tab, m->g);
+        if (r) FAIL(r, "CSR pattern build failed");
+    }
+    if (!m->d_irptr) {
+        hipStream_t s = m->st;
+        std::vector<int> jc0(m->g.jcptr.size());
+        for (size_t i = 0; i < jc0.size(); ++i) jc0[i] = m->g.jcptr[i] - 1;
+        TRY(upload(&m->d_irptr, m->g.irptr.data(), m->g.irptr.size(), s));
+        TRY(upload(&m->d_jcptr, jc0.data(), jc0.size(), s));
+        std::vector<signed char> comp(m->g.ndof, 0);
+        for (size_t i = 0; i < m->g.lm.size(); ++i)
+            if (m->g.lm[i] > 0) comp[m->g.lm[i] - 1] = (signed char)(i % 3);
+        TRY(upload(&m->d_comp, comp.data(), comp.size(), s));
+        TRY(upload(&m->d_cnp, m->g.cnp.data(), m->g.cnp.size(), s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    return 0;
+}
+
+extern "C" int f25_mesh_sizes(f25_mesh *m, int64_t sizes[6]) {
+    if (!m || !sizes) FAIL(F25_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(ensure_csr(m));
+    sizes[0] = m->g.nnp;
+    sizes[1] = m->g.ndof;
+    sizes[2] = m->g.irptr.empty() ? 0 : m->g.irptr[m->g.ndof];
+    sizes[3] = (int64_t)m->cptr.size() - 1;
+    sizes[4] = m->tab.nen;
+    sizes[5] = m->tab.nq1;
+    return 0;
+}
+
+extern "C" int f25_mesh_graph(f25_mesh *m, int *ien, int *lm, int *irptr, int *jcptr, char *cnp) {
+    if (!m) FAIL(F25_ERR_ARG, "null handle");
+    if ((irptr || jcptr)) {
+        HIPCHK(hipSetDevice(m->dev));
+        TRY(ensure_csr(m));
+    }
+    if (ien && !m->g.ien.empty()) std::copy(m->g.ien.begin(), m->g.ien.end(), ien);
+    if (lm) std::copy(m->g.lm.begin(), m->g.lm.end(), lm);
+    if (irptr)
+        for (int i = 0; i <= m->g.ndof; ++i) irptr[i] = (int)(m->g.irptr[i] + 1);
+    if (jcptr) std::copy(m->g.jcptr.begin(), m->g.jcptr.end(), jcptr);
+    if (cnp) std::copy(m->g.cnp.begin(), m->g.cnp.end(), cnp);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+template <int NL>
+static void launch_blocks(f25_mesh *m, const double *rho, const double *lam, const double *mu,
+                          double *blk, float *hi, float *lo) {
+    constexpr int NT = (NL <= 5) ? 256 : 512;
+    hipLaunchKernelGGL((k_blocks<NL, NT>), dim3(m->d.nelem), dim3(NT), 0, m->st, m->d.nelem,
+                       m->tab.nq1, m->d_shl, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, rho, lam,
+                       mu, m->d_cdom, blk, hi, lo, m->d_err);
+}
+
+static int run_blocks(f25_mesh *m, const double *rho, const double *lam, const double *mu,
+                      double *blk, float *hi, float *lo) {
+    switch (m->tab.nl) {
+        case 3: launch_blocks<3>(m, rho, lam, mu, blk, hi, lo); break;
+        case 4: launch_blocks<4>(m, rho, lam, mu, blk, hi, lo); break;
+        case 5: launch_blocks<5>(m, rho, lam, mu, blk, hi, lo); break;
+        case 6: launch_blocks<6>(m, rho, lam, mu, blk, hi, lo); break;
+        case 7: launch_blocks<7>(m, rho, lam, mu, blk, hi, lo); break;
+        default: FAIL(F25_ERR_ARG, "unsupported order");
+    }
+    HIPCHK(hipGetLastError());
+    return check_err(m);
+}
+
+extern "C" int f25_set_model(f25_mesh *m, const double *dens, const double *ecoeff) {
+    if (!m || !dens || !ecoeff) FAIL(F25_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(m->dev));
+    const int n = m->d.nnpg;
+    m->rho.assign(dens, dens + n);
+    m->lam.assign(ecoeff, ecoeff + n);
+    m->mu.assign(ecoeff + n, ecoeff + 2 * n);
+    for (int i = 0; i < n; ++i)
+        if (!(m->rho[i] > 0.0)) FAIL(F25_ERR_ARG, "density must be positive");
+    if (!m->d_rho) {
+        TRY(dalloc(&m->d_rho, n));
+        TRY(dalloc(&m->d_lam, n));
+        TRY(dalloc(&m->d_mu, n));
+    }
+    HIPCHK(hipMemcpyAsync(m->d_rho, m->rho.data(), n * 8, hipMemcpyHostToDevice, m->st));
+    HIPCHK(hipMemcpyAsync(m->d_lam, m->lam.data(), n * 8, hipMemcpyHostToDevice, m->st));
+    HIPCHK(hipMemcpyAsync(m->d_mu, m->mu.data(), n * 8, hipMemcpyHostToDevice, m->st));
+    const size_t nb = (size_t)m->d.nelem * NBLK * m->tab.nen * m->tab.nen;
+    if (!m->d_blk) {
+        TRY(dalloc(&m->d_blk, nb));
+        TRY(dalloc(&m->d_blk_hi, nb));
+        TRY(dalloc(&m->d_blk_lo, nb));
+    }
+    TRY(run_blocks(m, m->d_rho, m->d_lam, m->d_mu, m->d_blk, m->d_blk_hi, m->d_blk_lo));
+    m->have_model = true;
+    m->basis_ok = false;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+static int ensure_basis(f25_mesh *m) {
+    TRY(ensure_csr(m));
+    if (m->basis_ok) return 0;
+    const size_t nz = m->g.irptr[m->g.ndof];
+    if (!m->d_v0) {
+        TRY(dalloc(&m->d_v0, nz));
+        TRY(dalloc(&m->d_v1, nz));
+        TRY(dalloc(&m->d_v2, nz));
+    }
+    fill_async(m->d_v0, nz, 0.0, m->st);
+    fill_async(m->d_v1, nz, 0.0, m->st);
+    fill_async(m->d_v2, nz, 0.0, m->st);
+    for (size_t c = 0; c + 1 < m->cptr.size(); ++c) {
+        int n = m->cptr[c + 1] - m->cptr[c];
+        if (!n) continue;
+        hipLaunchKernelGGL(k_csr_basis, dim3(n), dim3(256), 0, m->st, m->tab.nen, n,
+                           m->d_order + m->cptr[c], m->d_blk, m->d_lm, m->d_irptr, m->d_jcptr,
+                           m->d_cdom, m->d_v0, m->d_v1, m->d_v2, m->d_err);
+    }
+    HIPCHK(hipGetLastError());
+    TRY(check_err(m));
+    m->basis_ok = true;
+    return 0;
+}
+
+template <typename CT>
+static int asmble_impl(f25_mesh *m, char cftype, double freq, double py, CT *out) {
+    const double omega = 6.2831853071795862 * freq, kv = omega * py;
+    const int ndof = m->g.ndof;
+    HIPCHK(hipEventRecord(m->ev0, m->st));
+    hipLaunchKernelGGL((k_csr_combine<CT>), dim3((ndof + 3) / 4), dim3(256), 0, m->st, ndof,
+                       m->d_irptr, m->d_jcptr, m->d_comp, m->d_v0, m->d_v1, m->d_v2, omega, kv, out);
+    if (!m->pml_order.empty()) {
+        PmlPar p;
+        p.omega = omega;
+        p.kv = kv;
+        p.xwidth = m->d.xwidth;
+        p.zwidth = m->d.zwidth;
+        const double *pp = (cftype == 'S') ? m->d.pml_srf : m->d.pml;
+        p.rcoeff = pp[0]; p.dmax = pp[1]; p.kappa = pp[2]; p.fcent = pp[3];
+        const int nen = m->tab.nen;
+        size_t lds = (8 * 7 + 3 * 8 * nen) * sizeof(double);
+        for (size_t c = 0; c + 1 < m->pml_cptr.size(); ++c) {
+            int n = m->pml_cptr[c + 1] - m->pml_cptr[c];
+            if (!n) continue;
+            hipLaunchKernelGGL((k_pml<CT>), dim3(n), dim3(256), lds, m->st, nen, m->tab.nq1, n,
+                               m->d_pml_order + m->pml_cptr[c], m->d_shl, m->d_wq, m->d_xq, m->d_ieng,
+                               m->d_xl, m->d_zl, m->d_xd, m->d_zd, m->d_rho, m->d_lam, m->d_mu, m->d_lm,
+                               m->d_irptr, m->d_jcptr, p, out, m->d_err);
+        }
+    }
+    HIPCHK(hipEventRecord(m->ev1, m->st));
+    HIPCHK(hipGetLastError());
+    TRY(check_err(m));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    record_ms("asmble", ms);
+    return 0;
+}
+
+extern "C" int f25_asmble_dev(f25_mesh *m, char cftype, double freq, double py, int precision, void *out) {
+    if (!m || !out) FAIL(F25_ERR_ARG, "null argument");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(ensure_basis(m));
+    if (precision == F25_C128) return asmble_impl(m, cftype, freq, py, (double2 *)out);
+    if (precision == F25_C64) return asmble_impl(m, cftype, freq, py, (float2 *)out);
+    FAIL(F25_ERR_ARG, "precision must be F25_C64 or F25_C128");
+}
+
+extern "C" int f25_asmble(f25_mesh *m, char cftype, double freq, double py, int precision, void *out) {
+    if (!m || !out) FAIL(F25_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(ensure_csr(m));
+    const size_t nz = m->g.irptr[m->g.ndof];
+    const size_t bytes = nz * (precision == F25_C128 ? 16 : 8);
+    void *d = nullptr;
+    HIPCHK(hipMalloc(&d, bytes));
+    int r = f25_asmble_dev(m, cftype, freq, py, precision, d);
+    if (!r) {
+        hipError_t e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    (void)hipFree(d);
+    return r;
+}
+
+extern "C" int f25_csr_apply_dev(f25_mesh *m, int precision, const void *s, const void *x, void *y) {
+    if (!m || !s || !x || !y) FAIL(F25_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(ensure_csr(m));
+    const int ndof = m->g.ndof;
+    HIPCHK(hipEventRecord(m->ev0, m->st));
+    if (precision == F25_C128)
+        hipLaunchKernelGGL((k_spmv<double2, double>), dim3((ndof + 3) / 4), dim3(256), 0, m->st, ndof,
+                           m->d_irptr, m->d_jcptr, (const double2 *)s, (const double2 *)x, (double2 *)y);
+    else if (precision == F25_C64)
+        hipLaunchKernelGGL((k_spmv<float2, float>), dim3((ndof + 3) / 4), dim3(256), 0, m->st, ndof,
+                           m->d_irptr, m->d_jcptr, (const float2 *)s, (const float2 *)x, (float2 *)y);
+    else
+        FAIL(F25_ERR_ARG, "bad precision");
+    HIPCHK(hipEventRecord(m->ev1, m->st));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(m->st));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    record_ms("spmv", ms);
+    return 0;
+}
+
+extern "C" int f25_asmb_peff(f25_mesh *m, const void *cdist, const void *ue, void *peff) {
+    if (!m || !cdist || !ue || !peff) FAIL(F25_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(ensure_csr(m));
+    const int ndof = m->g.ndof;
+    const size_t nz = m->g.irptr[ndof];
+    float2 *ds = nullptr, *du = nullptr, *dp = nullptr;
+    TRY(upload(&ds, (const float2 *)cdist, nz, m->st));
+    TRY(upload(&du, (const float2 *)ue, ndof, m->st));
+    TRY(dalloc(&dp, ndof));
+    hipLaunchKernelGGL(k_peff, dim3((ndof + 3) / 4), dim3(256), 0, m->st, ndof, m->d_irptr,
+                       m->d_jcptr, m->d_cnp, ds, du, dp);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(peff, dp, ndof * sizeof(float2), hipMemcpyDeviceToHost, m->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+    dfree(ds); dfree(du); dfree(dp);
+    if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+template <int NL, typename BT, typename XT, bool REFINE>
+static int launch_apply(f25_mesh *m, const double *om, const double *kv, int npair, const XT *x,
+                        XT *y, const BT *blk, const BT *blk_lo) {
+    typedef ApplyLds<NL, BT, REFINE> LD;
+    static bool attr_set = false;
+    auto kern = k_apply<NL, BT, XT, REFINE>;
+    if (!attr_set) {
+        HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)LD::bytes));
+        attr_set = true;
+    }
+    for (size_t c = 0; c + 1 < m->cptr.size(); ++c) {
+        int n = m->cptr[c + 1] - m->cptr[c];
+        if (!n) continue;
+        hipLaunchKernelGGL(kern, dim3(n), dim3(256), LD::bytes, m->st, m->d_order + m->cptr[c], blk,
+                           blk_lo, m->d_lm, npair, om, kv, x, y);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <typename BT, typename XT, bool REFINE>
+static int dispatch_apply(f25_mesh *m, const double *om, const double *kv, int npair, const XT *x,
+                          XT *y, const BT *blk, const BT *blk_lo) {
+    switch (m->tab.nl) {
+        case 3: return launch_apply<3, BT, XT, REFINE>(m, om, kv, npair, x, y, blk, blk_lo);
+        case 4: return launch_apply<4, BT, XT, REFINE>(m, om, kv, npair, x, y, blk, blk_lo);
+        case 5: return launch_apply<5, BT, XT, REFINE>(m, om, kv, npair, x, y, blk, blk_lo);
+        case 6: return launch_apply<6, BT, XT, REFINE>(m, om, kv, npair, x, y, blk, blk_lo);
+        case 7: return launch_apply<7, BT, XT, REFINE>(m, om, kv, npair, x, y, blk, blk_lo);
+    }
+    FAIL(F25_ERR_ARG, "unsupported order");
+}
+
+static int apply_core(f25_mesh *m, int npair, const double *d_om, const double *d_ky, int mode,
+                      const void *x, void *y, const double *blk64) {
+    const size_t n = (size_t)m->g.ndof * npair;
+    HIPCHK(hipEventRecord(m->ev0, m->st));
+    if (mode == F25_APPLY_F64) {
+        fill_async((double2 *)y, n, make_double2(0.0, 0.0), m->st);
+        TRY((dispatch_apply<double, double2, false>(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y,
+                                                    blk64, nullptr)));
+    } else if (mode == F25_APPLY_F32) {
+        fill_async((float2 *)y, n, make_float2(0.f, 0.f), m->st);
+        TRY((dispatch_apply<float, float2, false>(m, d_om, d_ky, npair, (const float2 *)x, (float2 *)y,
+                                                  m->d_blk_hi, nullptr)));
+    } else if (mode == F25_APPLY_F32_REFINED) {
+        fill_async((double2 *)y, n, make_double2(0.0, 0.0), m->st);
+        TRY((dispatch_apply<float, double2, true>(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y,
+                                                  m->d_blk_hi, m->d_blk_lo)));
+    } else {
+        FAIL(F25_ERR_ARG, "bad apply mode");
+    }
+    HIPCHK(hipEventRecord(m->ev1, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    record_ms("apply", ms);
+    return 0;
+}
+
+extern "C" int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
+                                   int mode, const void *x, void *y) {
+    if (!m || npair <= 0 || !omega || !ky || !x || !y) FAIL(F25_ERR_ARG, "invalid apply arguments");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    HIPCHK(hipSetDevice(m->dev));
+    double *d_om = nullptr, *d_ky = nullptr;
+    TRY(upload(&d_om, omega, npair, m->st));
+    TRY(upload(&d_ky, ky, npair, m->st));
+    int r = apply_core(m, npair, d_om, d_ky, mode, x, y, m->d_blk);
+    dfree(d_om);
+    dfree(d_ky);
+    return r;
+}
+
+static int transpose_dev(hipStream_t s, const double2 *in, double2 *out, int rows, int cols) {
+    dim3 b(16, 16), g((cols + 15) / 16, (rows + 15) / 16);
+    hipLaunchKernelGGL(k_transpose_c, g, b, 0, s, in, out, rows, cols);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int f25_apply_batch(f25_mesh *m, int npair, const double *omega, const double *ky, int mode,
+                               const void *x, void *y) {
+    if (!m || npair <= 0 || !x || !y) FAIL(F25_ERR_ARG, "invalid apply arguments");
+    if (mode == F25_APPLY_F32) FAIL(F25_ERR_ARG, "host wrapper takes double complex data; use F64 or F32_REFINED");
+    HIPCHK(hipSetDevice(m->dev));
+    const size_t n = (size_t)m->g.ndof * npair;
+    double2 *a = nullptr, *b = nullptr;
+    TRY(dalloc(&a, n));
+    TRY(dalloc(&b, n));
+    int r = 0;
+    hipError_t e = hipMemcpyAsync(a, x, n * 16, hipMemcpyHostToDevice, m->st);
+    if (e != hipSuccess) r = F25_ERR_HIP;
+    if (!r) r = transpose_dev(m->st, a, b, npair, m->g.ndof);
+    if (!r) r = f25_apply_batch_dev(m, npair, omega, ky, mode, b, a);
+    if (!r) r = transpose_dev(m->st, a, b, m->g.ndof, npair);
+    if (!r) {
+        e = hipMemcpyAsync(y, b, n * 16, hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(a);
+    dfree(b);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d_ky, const double2 *u,
+                     const double2 *l, double *g, int chain) {
+    const int nen = m->tab.nen;
+    const int nt = 256;
+    size_t lds = 6 * nen * sizeof(double2) + nt * 12 * sizeof(double);
+    HIPCHK(hipEventRecord(m->ev0, m->st));
+    hipLaunchKernelGGL(k_grad, dim3(m->d.nelem), dim3(nt), lds, m->st, nen, m->tab.nq1, m->d.nelem,
+                       m->d_shl, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip,
+                       (int64_t)m->g.ndof, d_om, d_ky, u, l, m->d_part, m->d_err);
+    HIPCHK(hipEventRecord(m->ev1, m->st));
+    hipLaunchKernelGGL(k_grad_gather, dim3((m->d.nnpg + 255) / 256), dim3(256), 0, m->st, m->d.nnpg,
+                       m->ncon_anchor, m->d_mconn, m->d_ieng, m->d_part, m->d_rho, m->d_lam, m->d_mu,
+                       chain, g);
+    HIPCHK(hipGetLastError());
+    TRY(check_err(m));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    record_ms("grad", ms);
+    return 0;
+}
+
+extern "C" int f25_gradient_dev(f25_mesh *m, int ntrip, const double *omega, const double *ky,
+                                const void *u, const void *lam, double *g) {
+    if (!m || ntrip <= 0 || !u || !lam || !g) FAIL(F25_ERR_ARG, "invalid gradient arguments");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    HIPCHK(hipSetDevice(m->dev));
+    double *d_om = nullptr, *d_ky = nullptr;
+    TRY(upload(&d_om, omega, ntrip, m->st));
+    TRY(upload(&d_ky, ky, ntrip, m->st));
+    int r = grad_core(m, ntrip, d_om, d_ky, (const double2 *)u, (const double2 *)lam, g, 1);
+    dfree(d_om);
+    dfree(d_ky);
+    return r;
+}
+
+static int grad_host(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
+                     const void *lam, double *g, int chain) {
+    if (!m || ntrip <= 0 || !u || !lam || !g) FAIL(F25_ERR_ARG, "invalid gradient arguments");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    HIPCHK(hipSetDevice(m->dev));
+    const size_t n = (size_t)m->g.ndof * ntrip;
+    double2 *du = nullptr, *dl = nullptr;
+    double *dg = nullptr, *d_om = nullptr, *d_ky = nullptr;
+    TRY(upload(&du, (const double2 *)u, n, m->st));
+    TRY(upload(&dl, (const double2 *)lam, n, m->st));
+    TRY(upload(&dg, g, 3 * (size_t)m->d.nnpg, m->st));
+    TRY(upload(&d_om, omega, ntrip, m->st));
+    TRY(upload(&d_ky, ky, ntrip, m->st));
+    int r = grad_core(m, ntrip, d_om, d_ky, du, dl, dg, chain);
+    if (!r) {
+        hipError_t e = hipMemcpyAsync(g, dg, 3 * (size_t)m->d.nnpg * 8, hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(du); dfree(dl); dfree(dg); dfree(d_om); dfree(d_ky);
+    return r;
+}
+
+extern "C" int f25_gradient(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
+                            const void *lam, double *g) {
+    return grad_host(m, ntrip, omega, ky, u, lam, g, 1);
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int f25_inv_setup(f25_mesh *m, const char *cinvtype, double vpvs, int linrec, int lkbdry,
+                             int nrec, const int *mrdof1, int sizes[6]) {
+    if (!m || !cinvtype) FAIL(F25_ERR_ARG, "null argument");
+    char c0 = (char)toupper(cinvtype[0]), c1 = (char)toupper(cinvtype[1]);
+    bool lisp = c0 == 'P' && c1 == 'P', liss = c0 == 'S' && c1 == 'S';
+    if (!lisp && !liss) FAIL(F25_ERR_INVTYPE, "CINVTYPE must be PP or SS (vforce25.f:66-85)");
+    if (!linrec && nrec > 0 && !mrdof1) FAIL(F25_ERR_ARG, "receiver DOFs missing");
+    HIPCHK(hipSetDevice(m->dev));
+    InvInfo inv;
+    int r = build_inv(m->d, m->g, 1, linrec, lkbdry, nrec, mrdof1, inv);
+    if (r) FAIL(r, "inversion setup failed (no inversion points)");
+    inv.lisp = lisp;
+    inv.vpvs = vpvs;
+    m->inv = std::move(inv);
+    m->have_inv = true;
+    if (sizes) {
+        sizes[0] = m->inv.nnpinv; sizes[1] = m->inv.ncon; sizes[2] = m->inv.nnpgl;
+        sizes[3] = m->inv.nz_fdist; sizes[4] = m->inv.mbufrhs; sizes[5] = m->inv.na35;
+    }
+    dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_ew);
+    dfree(m->d_col_elem); dfree(m->d_col_corner); dfree(m->d_col_cp); dfree(m->d_col_cs);
+    TRY(upload(&m->d_mygrad, m->inv.mygrad.data(), m->inv.mygrad.size(), m->st));
+    TRY(upload(&m->d_jcsc, m->inv.jcsc.data(), m->inv.jcsc.size(), m->st));
+    TRY(upload(&m->d_icsc, m->inv.icsc.data(), m->inv.icsc.size(), m->st));
+    TRY(upload(&m->d_ew, m->inv.elem_wts.data(), m->inv.elem_wts.size(), m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    return 0;
+}
+
+extern "C" int f25_inv_get(f25_mesh *m, int *maskg, int *mygrad, int *jcsc, int *icsc, double *ew) {
+    if (!m || !m->have_inv) FAIL(F25_ERR_STATE, "f25_inv_setup has not been called");
+    if (maskg) std::copy(m->inv.maskg.begin(), m->inv.maskg.end(), maskg);
+    if (mygrad) std::copy(m->inv.mygrad.begin(), m->inv.mygrad.end(), mygrad);
+    if (jcsc) std::copy(m->inv.jcsc.begin(), m->inv.jcsc.end(), jcsc);
+    if (icsc) std::copy(m->inv.icsc.begin(), m->inv.icsc.end(), icsc);
+    if (ew) std::copy(m->inv.elem_wts.begin(), m->inv.elem_wts.end(), ew);
+    return 0;
+}
+
+// per-column element/corner/collocation constants (model dependent)
+static int cfmat_columns(f25_mesh *m) {
+    const InvInfo &inv = m->inv;
+    const int n = inv.nnpgl, nnpg = m->d.nnpg;
+    std::vector<int> ce(n), ck(n);
+    std::vector<double> cp(n), cs(n);
+    const double v2 = inv.vpvs * inv.vpvs;
+    for (int c = 0; c < n; ++c) {
+        const int a = inv.node_of_col[c];
+        int e = 0;
+        for (int k = 0; k < inv.ncon; ++k) {
+            int v = inv.mconn[(size_t)k * nnpg + a];
+            if (v <= 0) break;
+            e = v;
+        }
+        if (e <= 0) FAIL(F25_ERR_STATE, "inversion anchor without elements");
+        --e;
+        int corner = -1;
+        for (int k = 0; k < 4; ++k)
+            if (m->ieng[4 * e + k] - 1 == a) corner = k;
+        ce[c] = e;
+        ck[c] = corner;
+        const double alpha = std::sqrt((m->lam[a] + 2.0 * m->mu[a]) / m->rho[a]);
+        const double beta = std::sqrt(m->mu[a] / m->rho[a]);
+        if (inv.lisp) { cp[c] = 2.0 * alpha; cs[c] = 2.0 * alpha / v2; }
+        else { cs[c] = 2.0 * beta; cp[c] = 2.0 * beta * v2; }
+    }
+    dfree(m->d_col_elem); dfree(m->d_col_corner); dfree(m->d_col_cp); dfree(m->d_col_cs);
+    TRY(upload(&m->d_col_elem, ce.data(), n, m->st));
+    TRY(upload(&m->d_col_corner, ck.data(), n, m->st));
+    TRY(upload(&m->d_col_cp, cp.data(), n, m->st));
+    TRY(upload(&m->d_col_cs, cs.data(), n, m->st));
+    return 0;
+}
+
+extern "C" int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *wave, void *fmat) {
+    if (!m || !wave || !fmat) FAIL(F25_ERR_ARG, "null argument");
+    if (!m->have_inv) FAIL(F25_ERR_STATE, "f25_inv_setup has not been called");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(cfmat_columns(m));
+    const double kv = 6.2831853071795862 * freq * py;
+    float2 *dw = nullptr, *df = nullptr;
+    TRY(upload(&dw, (const float2 *)wave, m->g.ndof, m->st));
+    TRY(dalloc(&df, m->inv.nz_fdist));
+    const int nen = m->tab.nen, nq = m->tab.nq;
+    size_t lds = 3 * nen * sizeof(double2) + 17 * (size_t)nq * sizeof(double);
+    hipLaunchKernelGGL(k_cfmat, dim3(m->inv.nnpgl), dim3(256), lds, m->st, nen, m->tab.nq1, m->inv.nnpgl,
+                       (const int *)nullptr, m->d_col_elem, m->d_col_corner, m->d_col_cp, m->d_col_cs,
+                       m->d_shl, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, m->d_ew,
+                       m->d_jcsc, m->d_icsc, kv, dw, df, m->d_err);
+    hipError_t e = hipGetLastError();
+    int r = 0;
+    if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    if (!r) r = check_err(m);
+    if (!r) {
+        e = hipMemcpyAsync(fmat, df, m->inv.nz_fdist * sizeof(float2), hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(dw);
+    dfree(df);
+    return r;
+}
+
+extern "C" int f25_adjflam_dist(f25_mesh *m, const void *fmat, const void *adjlam, float *gprod) {
+    if (!m || !fmat || !adjlam || !gprod) FAIL(F25_ERR_ARG, "null argument");
+    if (!m->have_inv) FAIL(F25_ERR_STATE, "f25_inv_setup has not been called");
+    HIPCHK(hipSetDevice(m->dev));
+    float2 *df = nullptr, *dl = nullptr;
+    float *dg = nullptr;
+    TRY(upload(&df, (const float2 *)fmat, m->inv.nz_fdist, m->st));
+    TRY(upload(&dl, (const float2 *)adjlam, m->g.ndof, m->st));
+    TRY(dalloc(&dg, m->inv.na35));
+    fill_async(dg, (size_t)m->inv.na35, 0.f, m->st);
+    hipLaunchKernelGGL(k_adjflam, dim3((m->inv.nnpgl + 3) / 4), dim3(256), 0, m->st, m->inv.nnpgl,
+                       m->d_mygrad, m->d_jcsc, m->d_icsc, df, dl, dg);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(gprod, dg, m->inv.na35 * sizeof(float), hipMemcpyDeviceToHost, m->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+    dfree(df); dfree(dl); dfree(dg);
+    if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Newton element products
+extern "C" int f25_gn_fv(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
+                         const double *v, void *out) {
+    if (!m || ntrip <= 0 || !u || !v || !out) FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    HIPCHK(hipSetDevice(m->dev));
+    const int n = m->d.nnpg;
+    // chain rule (Vp, Vs, rho) -> (lambda, mu, rho) perturbations
+    std::vector<double> dr(n), dl(n), dm(n);
+    for (int a = 0; a < n; ++a) {
+        double r = m->rho[a], vp = std::sqrt((m->lam[a] + 2.0 * m->mu[a]) / r), vs = std::sqrt(m->mu[a] / r);
+        double dvp = v[3 * a], dvs = v[3 * a + 1], drho = v[3 * a + 2];
+        dr[a] = drho;
+        dl[a] = drho * (vp * vp - 2.0 * vs * vs) + r * (2.0 * vp * dvp - 4.0 * vs * dvs);
+        dm[a] = drho * vs * vs + 2.0 * r * vs * dvs;
+    }
+    double *d_r = nullptr, *d_l = nullptr, *d_m = nullptr;
+    TRY(upload(&d_r, dr.data(), n, m->st));
+    TRY(upload(&d_l, dl.data(), n, m->st));
+    TRY(upload(&d_m, dm.data(), n, m->st));
+    const size_t nb = (size_t)m->d.nelem * NBLK * m->tab.nen * m->tab.nen;
+    if (!m->d_dblk) TRY(dalloc(&m->d_dblk, nb));
+    int r = run_blocks(m, d_r, d_l, d_m, m->d_dblk, nullptr, nullptr);
+    dfree(d_r); dfree(d_l); dfree(d_m);
+    if (r) return r;
+    const size_t nn = (size_t)m->g.ndof * ntrip;
+    double2 *a = nullptr, *b = nullptr;
+    double *d_om = nullptr, *d_ky = nullptr;
+    TRY(dalloc(&a, nn));
+    TRY(dalloc(&b, nn));
+    TRY(upload(&d_om, omega, ntrip, m->st));
+    TRY(upload(&d_ky, ky, ntrip, m->st));
+    hipError_t e = hipMemcpyAsync(a, u, nn * 16, hipMemcpyHostToDevice, m->st);
+    r = (e == hipSuccess) ? 0 : F25_ERR_HIP;
+    if (!r) r = transpose_dev(m->st, a, b, ntrip, m->g.ndof);
+    if (!r) r = apply_core(m, ntrip, d_om, d_ky, F25_APPLY_F64, b, a, m->d_dblk);
+    if (!r) r = transpose_dev(m->st, a, b, m->g.ndof, ntrip);
+    if (!r) {
+        e = hipMemcpyAsync(out, b, nn * 16, hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(a); dfree(b); dfree(d_om); dfree(d_ky);
+    return r;
+}
+
+extern "C" int f25_gn_fhw(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
+                          const void *w, double *g) {
+    return grad_host(m, ntrip, omega, ky, u, w, g, 1);
+}
+
+extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
+                           double *diag) {
+    (void)m; (void)ntrip; (void)omega; (void)ky; (void)u; (void)diag;
+    FAIL(F25_ERR_STATE, "pseudo-Hessian diagonal not available in this build");
+}
+
+// MFMA operand/result layout check (used by the self test entry point)
+extern "C" int f25_selftest_mfma(double *maxerr) {
+    double *dd = nullptr;
+    float *df = nullptr;
+    TRY(dalloc(&dd, 256));
+    TRY(dalloc(&df, 256));
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, dd, df);
+    std::vector<double> hd(256);
+    std::vector<float> hf(256);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(hd.data(), dd, 256 * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hf.data(), df, 256 * 4, hipMemcpyDeviceToHost);
+    dfree(dd);
+    dfree(df);
+    if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
+    // A[row][k] = 1 + row + 16k (lane row + 16k), B[k][col] = 1 + 0.5 col - 3k
+    double worst = 0.0;
+    for (int l = 0; l < 64; ++l)
+        for (int i = 0; i < 4; ++i) {
+            int row = 4 * (l >> 4) + i, col = l & 15;
+            double ref = 0.0;
+            for (int k = 0; k < 4; ++k) ref += (1.0 + row + 16.0 * k) * (1.0 + 0.5 * col - 3.0 * k);
+            worst = std::max(worst, std::fabs(hd[l * 4 + i] - ref));
+            worst = std::max(worst, std::fabs((double)hf[l * 4 + i] - ref));
+        }
+    if (maxerr) *maxerr = worst;
+    return worst == 0.0 ? 0 : F25_ERR_HIP;
+}

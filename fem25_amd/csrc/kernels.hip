@@ -1,0 +1,911 @@
+// HIP kernels for gfx950 (MI355X): element blocks, batched element-block
+// product on MFMA, CSR assembly, PML elements, SpMV, adjoint gradient, the
+// reference virtual-force columns and their CSC gather.
+//
+// Element operator structure (estiff25.f:61-150, restated as bilinear
+// forms).  For a non-PML element with interpolated lambda, mu, rho and
+// quadrature weight w = wx*wz*det J, the element matrix of KM25ISOQ is
+//   y1 = K11 x1 + K13 x3 - i ky T12 x2 + (ky^2 Mmu - w^2 Mrho) x1
+//   y2 = K22 x2 + i ky T12^T x1 + i ky T23 x3 + (ky^2 Ml2m - w^2 Mrho) x2
+//   y3 = K13^T x1 + K33 x3 - i ky T23^T x2 + (ky^2 Mmu - w^2 Mrho) x3
+// with nine real NEN x NEN blocks that do not depend on omega or ky:
+//   K11 = S[(l+2m) Nx Nx + m Nz Nz]   K22 = S[m (Nx Nx + Nz Nz)]
+//   K33 = S[(l+2m) Nz Nz + m Nx Nx]   K13 = S[l Nx_a Nz_b + m Nz_a Nx_b]
+//   T12 = S[l Nx_a N_b - m N_a Nx_b]  T23 = S[l N_a Nz_b - m Nz_a N_b]
+//   Mrho = S[rho N N], Mmu = S[m N N], Ml2m = S[(l+2m) N N]
+// (S = sum over quadrature points with weight w).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace f25k {
+
+enum { B_K11 = 0, B_K22, B_K33, B_K13, B_T12, B_T23, B_MRHO, B_MMU, B_ML2M, NBLK };
+
+struct Geo {   // geometry at one quadrature point
+    double det, dxdxi, dxdeta, dzdxi, dzdeta;
+    double sf[4];
+};
+
+__device__ inline int geo_at(double xi, double eta, const int *ie, const double *xl,
+                             const double *zl, Geo &g) {
+    const double eps = 2.2204e-16;
+    double sf[4] = {.25 * (1.0 - xi) * (1.0 - eta), .25 * (1.0 + xi) * (1.0 - eta),
+                    .25 * (1.0 + xi) * (1.0 + eta), .25 * (1.0 - xi) * (1.0 + eta)};
+    double dxi[4] = {-.25 * (1.0 - eta), .25 * (1.0 - eta), .25 * (1.0 + eta), -.25 * (1.0 + eta)};
+    double deta[4] = {-.25 * (1.0 - xi), -.25 * (1.0 + xi), .25 * (1.0 + xi), .25 * (1.0 - xi)};
+    double a = 0, b = 0, c = 0, d = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (fabs(sf[k]) < eps) sf[k] = 0.0;
+        if (fabs(dxi[k]) < eps) dxi[k] = 0.0;
+        if (fabs(deta[k]) < eps) deta[k] = 0.0;
+        int gi = ie[k] - 1;
+        a += dxi[k] * xl[gi];
+        b += deta[k] * xl[gi];
+        c += dxi[k] * zl[gi];
+        d += deta[k] * zl[gi];
+        g.sf[k] = sf[k];
+    }
+    g.dxdxi = a; g.dxdeta = b; g.dzdxi = c; g.dzdeta = d;
+    g.det = a * d - b * c;
+    return (fabs(g.det) <= 2.22e-16 || g.det < 0.0) ? 1 : 0;   // cjac.f:115-125
+}
+
+// ---------------------------------------------------------------------------
+// Element blocks.  One workgroup per element; every thread owns up to PPT
+// (a, b) node pairs and accumulates the nine blocks over quadrature points
+// staged in LDS chunk by chunk.
+template <int NL, int NT>
+__global__ __launch_bounds__(NT) void k_blocks(int nelem, int nq1, const double *__restrict__ shl,
+                                               const double *__restrict__ wq,
+                                               const double *__restrict__ xq,
+                                               const int *__restrict__ ieng,
+                                               const double *__restrict__ xl,
+                                               const double *__restrict__ zl,
+                                               const double *__restrict__ rho,
+                                               const double *__restrict__ lam,
+                                               const double *__restrict__ mu,
+                                               const char *__restrict__ cdom, double *__restrict__ blk,
+                                               float *__restrict__ blk_hi, float *__restrict__ blk_lo,
+                                               int *__restrict__ err) {
+    constexpr int NEN = NL * NL;
+    constexpr int NP = NEN * NEN;
+    constexpr int PPT = (NP + NT - 1) / NT;
+    constexpr int QC = 16;  // quadrature points per LDS chunk
+    __shared__ double sN[QC][NEN], sX[QC][NEN], sZ[QC][NEN];
+    __shared__ double sW[QC][4];   // w*rho, w*lambda, w*mu
+    const int e = blockIdx.x;
+    if (e >= nelem) return;
+    const int tid = threadIdx.x;
+    const size_t obase = (size_t)e * NBLK * NP;
+    if (cdom[e] == 'A') {   // PML elements are integrated per call
+        for (int i = tid; i < NBLK * NP; i += NT) {
+            blk[obase + i] = 0.0;
+            if (blk_hi) { blk_hi[obase + i] = 0.f; blk_lo[obase + i] = 0.f; }
+        }
+        return;
+    }
+    const int *ie = ieng + 4 * e;
+    double acc[PPT][NBLK];
+#pragma unroll
+    for (int p = 0; p < PPT; ++p)
+#pragma unroll
+        for (int k = 0; k < NBLK; ++k) acc[p][k] = 0.0;
+    const int nq = nq1 * nq1;
+    for (int q0 = 0; q0 < nq; q0 += QC) {
+        __syncthreads();
+        for (int i = tid; i < QC * NEN; i += NT) {
+            int qq = i / NEN, a = i % NEN, q = q0 + qq;
+            if (q < nq) {
+                int ix = q % nq1, iz = q / nq1;
+                Geo g;
+                int bad = geo_at(xq[ix], xq[iz], ie, xl, zl, g);
+                if (bad && a == 0) *err = 2;
+                const double *s = shl + ((size_t)q * NEN + a) * 3;
+                double rj = bad ? 1.0 : g.det;
+                sN[qq][a] = s[2];
+                sX[qq][a] = (s[0] * g.dzdeta - s[1] * g.dzdxi) / rj;
+                sZ[qq][a] = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / rj;
+                if (a == 0) {
+                    double w = wq[ix] * wq[iz] * g.det, r = 0, l = 0, m = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        int gi = ie[k] - 1;
+                        r += g.sf[k] * rho[gi];
+                        l += g.sf[k] * lam[gi];
+                        m += g.sf[k] * mu[gi];
+                    }
+                    sW[qq][0] = w * r; sW[qq][1] = w * l; sW[qq][2] = w * m;
+                }
+            } else {
+                sN[qq][a] = sX[qq][a] = sZ[qq][a] = 0.0;
+                if (a == 0) sW[qq][0] = sW[qq][1] = sW[qq][2] = 0.0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) {
+            int pr = tid + p * NT;
+            if (pr >= NP) break;
+            int a = pr / NEN, b = pr % NEN;
+            for (int qq = 0; qq < QC; ++qq) {
+                double wr = sW[qq][0], wl = sW[qq][1], wm = sW[qq][2];
+                double na = sN[qq][a], nb = sN[qq][b], xa = sX[qq][a], xb = sX[qq][b];
+                double za = sZ[qq][a], zb = sZ[qq][b];
+                double nn = na * nb, xx = xa * xb, zz = za * zb;
+                acc[p][B_K11] += (wl + 2.0 * wm) * xx + wm * zz;
+                acc[p][B_K22] += wm * (xx + zz);
+                acc[p][B_K33] += (wl + 2.0 * wm) * zz + wm * xx;
+                acc[p][B_K13] += wl * xa * zb + wm * za * xb;
+                acc[p][B_T12] += wl * xa * nb - wm * na * xb;
+                acc[p][B_T23] += wl * na * zb - wm * za * nb;
+                acc[p][B_MRHO] += wr * nn;
+                acc[p][B_MMU] += wm * nn;
+                acc[p][B_ML2M] += (wl + 2.0 * wm) * nn;
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < PPT; ++p) {
+        int pr = tid + p * NT;
+        if (pr >= NP) break;
+#pragma unroll
+        for (int k = 0; k < NBLK; ++k) {
+            double v = acc[p][k];
+            blk[obase + (size_t)k * NP + pr] = v;
+            if (blk_hi) {
+                float h = (float)v;
+                blk_hi[obase + (size_t)k * NP + pr] = h;
+                blk_lo[obase + (size_t)k * NP + pr] = (float)(v - (double)h);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched element-block product on MFMA.
+//
+// One workgroup = one element of the current colour, 4 waves.  The element's
+// blocks sit in LDS (row stride RS, odd, so both the direct and the
+// transposed operand reads are conflict-free).  Each wave owns N-tiles of 8
+// (omega, ky) pairs = 16 real columns (re, im interleaved as in memory).  The
+// element's input vectors stay in VGPRs in the B-operand layout of a
+// 16x16x4 MFMA (lane l: column l%16, k-row l/16 + 4s); per-column factors
+// (ky, ky^2, -omega^2, the i of the coupling blocks) are applied to the
+// B operand, so each output component is ONE accumulation over five
+// sub-blocks.  Outputs are added into y[dof][pair] (no other element of the
+// colour touches these DOFs).
+template <typename T>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    __device__ static inline v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct MfmaTraits<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    __device__ static inline v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+struct SubBlk { int blk; int trans; int xcomp; int kind; };
+// kind: 0 -> 1, 1 -> +i ky, 2 -> -i ky, 3 -> ky^2, 4 -> -omega^2
+__device__ constexpr SubBlk kSub[3][5] = {
+    {{B_K11, 0, 0, 0}, {B_K13, 0, 2, 0}, {B_T12, 0, 1, 2}, {B_MMU, 0, 0, 3}, {B_MRHO, 0, 0, 4}},
+    {{B_K22, 0, 1, 0}, {B_T12, 1, 0, 1}, {B_T23, 0, 2, 1}, {B_ML2M, 0, 1, 3}, {B_MRHO, 0, 1, 4}},
+    {{B_K13, 1, 0, 0}, {B_K33, 0, 2, 0}, {B_T23, 1, 1, 2}, {B_MMU, 0, 2, 3}, {B_MRHO, 0, 2, 4}}};
+
+template <int NL>
+struct ApplyShape {
+    static constexpr int NEN = NL * NL;
+    static constexpr int RS = (NEN % 2) ? NEN : NEN + 1;
+    static constexpr int MT = (NEN + 15) / 16;
+    static constexpr int KS = (NEN + 3) / 4;
+};
+
+// LDS bytes of k_apply for a shape; PH = blocks staged one output
+// component at a time (five blocks) when all nine do not fit.
+template <int NL, typename BT, bool REFINE>
+struct ApplyLds {
+    typedef ApplyShape<NL> S;
+    static constexpr size_t per_blk = (size_t)S::NEN * S::RS * sizeof(BT) * (REFINE ? 2 : 1);
+    static constexpr bool PH = 9 * per_blk > 150 * 1024;
+    static constexpr size_t bytes = (PH ? 5 : 9) * per_blk + 3 * S::NEN * sizeof(int);
+};
+
+// XT: storage type of x/y (double2 or float2); BT: block/MFMA type.
+// REFINE: blocks as hi+lo floats, x split hi+lo, three products
+// (hi*hi + hi*lo + lo*hi) per 4-wide step, each partial summed in double.
+template <int NL, typename BT, typename XT, bool REFINE>
+__global__ __launch_bounds__(256) void k_apply(const int *__restrict__ order,
+                                               const BT *__restrict__ blk, const BT *__restrict__ blk_lo,
+                                               const int *__restrict__ lm, int npair,
+                                               const double *__restrict__ omega,
+                                               const double *__restrict__ ky,
+                                               const XT *__restrict__ x, XT *__restrict__ y) {
+    typedef ApplyShape<NL> S;
+    typedef ApplyLds<NL, BT, REFINE> LD;
+    constexpr int NEN = S::NEN, RS = S::RS, MT = S::MT, KS = S::KS;
+    constexpr int NP = NEN * NEN;
+    constexpr bool PH = LD::PH;
+    constexpr int NSLOT = PH ? 5 : 9;
+    typedef MfmaTraits<BT> MT_;
+    typedef typename MT_::v4 v4;
+    typedef decltype(XT().x) XS;
+    extern __shared__ __align__(16) unsigned char smem[];
+    BT *sB = reinterpret_cast<BT *>(smem);
+    BT *sBl = sB + (REFINE ? NSLOT * NEN * RS : 0);
+    int *sLM = reinterpret_cast<int *>(sB + (REFINE ? 2 : 1) * NSLOT * NEN * RS);
+    const int e = order[blockIdx.x];
+    const int tid = threadIdx.x;
+    const size_t gbase = (size_t)e * NBLK * NP;
+    for (int i = tid; i < 3 * NEN; i += 256) sLM[i] = lm[(size_t)e * 3 * NEN + i];
+    const int lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 15, kg = lane >> 4, ri = col & 1;
+    const int ntile = (npair + 7) / 8;
+#pragma unroll 1
+    for (int ph = 0; ph < (PH ? 3 : 1); ++ph) {
+        __syncthreads();
+        for (int i = tid; i < NSLOT * NP; i += 256) {
+            int sl = i / NP, r = i % NP, a = r / NEN, b = r % NEN;
+            int kb = PH ? kSub[ph][sl].blk : sl;
+            sB[(sl * NEN + a) * RS + b] = blk[gbase + (size_t)kb * NP + r];
+            if (REFINE) sBl[(sl * NEN + a) * RS + b] = blk_lo[gbase + (size_t)kb * NP + r];
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int t = wave; t < ntile; t += 4) {
+            const int pair = t * 8 + (col >> 1);
+            const bool live = pair < npair;
+            const double om = live ? omega[pair] : 0.0, kv = live ? ky[pair] : 0.0;
+            // this lane's x values: comp d, k-step s -> element node b = 4s + kg
+            double xr[3][KS], xi[3][KS];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    const int b = 4 * s + kg;
+                    const int dof = (b < NEN) ? sLM[b * 3 + d] : -1;
+                    if (dof >= 0 && live) {
+                        XT v = x[(size_t)dof * npair + pair];
+                        xr[d][s] = v.x;
+                        xi[d][s] = v.y;
+                    } else {
+                        xr[d][s] = 0.0;
+                        xi[d][s] = 0.0;
+                    }
+                }
+            // per-column factors of the five operand kinds; kinds 1, 2 swap re/im
+            const double fac[5] = {1.0, ri ? kv : -kv, ri ? -kv : kv, kv * kv, -om * om};
+#pragma unroll
+            for (int cc = 0; cc < (PH ? 1 : 3); ++cc) {
+                const int c = PH ? ph : cc;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    const int arow = mt * 16 + col;
+                    double accd[4] = {0.0, 0.0, 0.0, 0.0};
+                    v4 acc = {0, 0, 0, 0};
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) {
+                        // compile-time table entries once c is known; with PH the
+                        // component is a runtime value, so branch on it
+                        int blkid, trans, xc, kind;
+                        if (PH) {
+                            blkid = j;
+                            trans = (c == 0) ? kSub[0][j].trans : (c == 1 ? kSub[1][j].trans : kSub[2][j].trans);
+                            xc = (c == 0) ? kSub[0][j].xcomp : (c == 1 ? kSub[1][j].xcomp : kSub[2][j].xcomp);
+                            kind = (c == 0) ? kSub[0][j].kind : (c == 1 ? kSub[1][j].kind : kSub[2][j].kind);
+                        } else {
+                            blkid = kSub[cc][j].blk;
+                            trans = kSub[cc][j].trans;
+                            xc = kSub[cc][j].xcomp;
+                            kind = kSub[cc][j].kind;
+                        }
+                        const BT *A = sB + blkid * NEN * RS;
+                        const BT *Al = sBl + blkid * NEN * RS;
+                        const double f = fac[kind];
+                        const bool sw = (kind == 1 || kind == 2);
+#pragma unroll
+                        for (int s = 0; s < KS; ++s) {
+                            const int k = 4 * s + kg;
+                            BT av = 0, avl = 0;
+                            if (arow < NEN && k < NEN) {
+                                const int idx = trans ? (k * RS + arow) : (arow * RS + k);
+                                av = A[idx];
+                                if (REFINE) avl = Al[idx];
+                            }
+                            double x0 = 0.0, x1 = 0.0;
+#pragma unroll
+                            for (int d = 0; d < 3; ++d)
+                                if (d == xc) { x0 = xr[d][s]; x1 = xi[d][s]; }
+                            const double xv = sw ? (ri ? x0 : x1) : (ri ? x1 : x0);
+                            const double bv = f * xv;
+                            if (REFINE) {
+                                const float bh = (float)bv, bl = (float)(bv - (double)bh);
+                                v4 z = {0, 0, 0, 0};
+                                z = MT_::mma(av, bh, z);
+                                z = MT_::mma(av, bl, z);
+                                z = MT_::mma(avl, bh, z);
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) accd[i] += (double)z[i];
+                            } else {
+                                acc = MT_::mma(av, (BT)bv, acc);
+                            }
+                        }
+                    }
+                    // rows 4*kg + i of this tile, column col
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int row = mt * 16 + 4 * kg + i;
+                        if (row < NEN && live) {
+                            const int dof = sLM[row * 3 + c];
+                            if (dof >= 0) {
+                                const double v = REFINE ? accd[i] : (double)acc[i];
+                                XS *py = reinterpret_cast<XS *>(y + (size_t)dof * npair + pair);
+                                py[ri] += (XS)v;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CSR basis: V0/V1/V2 per slot (see combine), scattered from element blocks.
+__device__ inline int64_t find_col(const int *__restrict__ jc, int64_t b, int64_t e, int col) {
+    while (b < e) {
+        int64_t m = (b + e) >> 1;
+        int v = jc[m];
+        if (v == col) return m;
+        if (v < col) b = m + 1; else e = m;
+    }
+    return -1;
+}
+
+__global__ void k_csr_basis(int nen, int ncol_elem, const int *__restrict__ order,
+                            const double *__restrict__ blk, const int *__restrict__ lm,
+                            const int64_t *__restrict__ irptr, const int *__restrict__ jcptr,
+                            const char *__restrict__ cdom, double *__restrict__ v0,
+                            double *__restrict__ v1, double *__restrict__ v2, int *__restrict__ err) {
+    const int e = order[blockIdx.x];
+    if (cdom[e] == 'A') return;
+    const int np = nen * nen;
+    const double *B = blk + (size_t)e * NBLK * np;
+    const int *L = lm + (size_t)e * 3 * nen;
+    for (int i = threadIdx.x; i < 9 * np; i += blockDim.x) {
+        int cp = i / np, r = i % np, a = r / nen, b = r % nen;
+        int ci = cp / 3, cj = cp % 3;
+        int rd = L[a * 3 + ci], cd = L[b * 3 + cj];
+        if (rd < 0 || cd < 0) continue;
+        int64_t s = find_col(jcptr, irptr[rd], irptr[rd + 1], cd);
+        if (s < 0) { *err = 3; continue; }
+        const int ab = a * nen + b, ba = b * nen + a;
+        if (ci == cj) {
+            int kb = ci == 0 ? B_K11 : (ci == 1 ? B_K22 : B_K33);
+            int mb = ci == 1 ? B_ML2M : B_MMU;
+            v0[s] += B[kb * np + ab];
+            v1[s] += B[mb * np + ab];
+            v2[s] += B[B_MRHO * np + ab];
+        } else if (ci == 0 && cj == 2) v0[s] += B[B_K13 * np + ab];
+        else if (ci == 2 && cj == 0) v0[s] += B[B_K13 * np + ba];
+        else if (ci == 0 && cj == 1) v0[s] += B[B_T12 * np + ab];
+        else if (ci == 1 && cj == 0) v0[s] += B[B_T12 * np + ba];
+        else if (ci == 1 && cj == 2) v0[s] += B[B_T23 * np + ab];
+        else v0[s] += B[B_T23 * np + ba];
+    }
+}
+
+// S(omega, ky) from the basis: same component: V0 + ky^2 V1 - omega^2 V2,
+// 1-3 coupling: V0, 1-2/2-3 couplings: -/+ i ky V0 (block signs above).
+template <typename CT>
+__global__ void k_csr_combine(int ndof, const int64_t *__restrict__ irptr, const int *__restrict__ jcptr,
+                              const signed char *__restrict__ comp, const double *__restrict__ v0,
+                              const double *__restrict__ v1, const double *__restrict__ v2, double om,
+                              double kv, CT *__restrict__ out) {
+    const int wpb = blockDim.x / 64;
+    const int r = blockIdx.x * wpb + threadIdx.x / 64;
+    if (r >= ndof) return;
+    const int cr = comp[r];
+    const double k2 = kv * kv, w2 = om * om;
+    for (int64_t s = irptr[r] + (threadIdx.x & 63); s < irptr[r + 1]; s += 64) {
+        const int cc = comp[jcptr[s]];
+        double re = 0.0, im = 0.0;
+        if (cr == cc) re = v0[s] + k2 * v1[s] - w2 * v2[s];
+        else if ((cr == 0 && cc == 2) || (cr == 2 && cc == 0)) re = v0[s];
+        else {
+            double sg = ((cr == 0 && cc == 1) || (cr == 2 && cc == 1)) ? -1.0 : 1.0;
+            im = sg * kv * v0[s];
+        }
+        out[s].x = re;
+        out[s].y = im;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PML elements (CJACCQ + CM25ISOQ, cjac.f:399-581, estiff25.f:168-334),
+// integrated per call and added into the assembled operator.
+struct PmlPar { double omega, kv, xwidth, zwidth, rcoeff, dmax, kappa, fcent; };
+
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cdiv1(double2 z) { double d = z.x * z.x + z.y * z.y; return make_double2(z.x / d, -z.y / d); }
+
+__device__ inline double2 zdampfn(double rn, double vp, double delta, const PmlPar &p) {
+    double dmax = (p.dmax > 0.0) ? p.dmax : -3.0 * vp * log(p.rcoeff) / (2.0 * delta);
+    double d = dmax * (rn / delta) * (rn / delta);
+    double alpha = M_PI * p.fcent * (1.0 - rn / delta);
+    // kappa + |d| / (alpha + i omega), then its reciprocal (cjac.f:849-851)
+    double2 t = cdiv1(make_double2(alpha, p.omega));
+    return cdiv1(make_double2(p.kappa + fabs(d) * t.x, fabs(d) * t.y));
+}
+
+template <typename CT>
+__global__ void k_pml(int nen, int nq1, int ncol_elem, const int *__restrict__ order,
+                      const double *__restrict__ shl, const double *__restrict__ wq,
+                      const double *__restrict__ xq, const int *__restrict__ ieng,
+                      const double *__restrict__ xl, const double *__restrict__ zl,
+                      const double *__restrict__ xd, const double *__restrict__ zd,
+                      const double *__restrict__ rho, const double *__restrict__ lam,
+                      const double *__restrict__ mu, const int *__restrict__ lm,
+                      const int64_t *__restrict__ irptr, const int *__restrict__ jcptr, PmlPar par,
+                      CT *__restrict__ out, int *__restrict__ err) {
+    extern __shared__ double sm[];
+    const int e = order[blockIdx.x];
+    const int nq = nq1 * nq1;
+    // LDS: per q: w*lam, w*mu, w*rho, ex(2), ez(2); then N, Nx, Nz per a for a chunk
+    const int QC = 8;
+    double *sq = sm;                 // QC*7
+    double *sN = sq + QC * 7;        // QC*nen
+    double *sX = sN + QC * nen;
+    double *sZ = sX + QC * nen;
+    const int *ie = ieng + 4 * e;
+    const int np = nen * nen;
+    const double k2 = par.kv * par.kv, w2 = par.omega * par.omega;
+    for (int p0 = 0; p0 < np; p0 += blockDim.x) {
+        const int pr = p0 + threadIdx.x;
+        const int a = pr / nen, b = pr % nen;
+        double2 acc[9];
+        for (int i = 0; i < 9; ++i) acc[i] = make_double2(0.0, 0.0);
+        double me[3] = {0.0, 0.0, 0.0};
+        for (int q0 = 0; q0 < nq; q0 += QC) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < QC * nen; i += blockDim.x) {
+                int qq = i / nen, aa = i % nen, q = q0 + qq;
+                if (q >= nq) { sN[i] = sX[i] = sZ[i] = 0.0; if (aa == 0) for (int j = 0; j < 7; ++j) sq[qq * 7 + j] = 0.0; continue; }
+                int ix = q % nq1, iz = q / nq1;
+                Geo g;
+                int bad = geo_at(xq[ix], xq[iz], ie, xl, zl, g);
+                if (bad && aa == 0) *err = 2;
+                double rj = bad ? 1.0 : g.det;
+                const double *s = shl + ((size_t)q * nen + aa) * 3;
+                sN[i] = s[2];
+                sX[i] = (s[0] * g.dzdeta - s[1] * g.dzdxi) / rj;
+                sZ[i] = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / rj;
+                if (aa == 0) {
+                    double w = wq[ix] * wq[iz] * g.det, r = 0, l = 0, m = 0, xdep = 0, zdep = 0, vel = 0;
+                    for (int k = 0; k < 4; ++k) {
+                        int gi = ie[k] - 1;
+                        r += g.sf[k] * rho[gi];
+                        l += g.sf[k] * lam[gi];
+                        m += g.sf[k] * mu[gi];
+                        xdep += g.sf[k] * xd[gi];
+                        zdep += g.sf[k] * zd[gi];
+                        vel += g.sf[k] * sqrt((lam[gi] + 2.0 * mu[gi]) / rho[gi]);
+                    }
+                    double2 ex = xdep > 2.22e-16 ? zdampfn(xdep, vel, par.xwidth, par) : make_double2(1.0, 0.0);
+                    double2 ez = zdep > 2.22e-16 ? zdampfn(zdep, vel, par.zwidth, par) : make_double2(1.0, 0.0);
+                    double *o = sq + qq * 7;
+                    o[0] = w * l; o[1] = w * m; o[2] = w * r;
+                    o[3] = ex.x; o[4] = ex.y; o[5] = ez.x; o[6] = ez.y;
+                }
+            }
+            __syncthreads();
+            if (pr < np) {
+                for (int qq = 0; qq < QC; ++qq) {
+                    const double *o = sq + qq * 7;
+                    double L = o[0], M = o[1], R = o[2];
+                    double2 ex = make_double2(o[3], o[4]), ez = make_double2(o[5], o[6]);
+                    double na = sN[qq * nen + a], nb = sN[qq * nen + b];
+                    double2 xa = make_double2(sX[qq * nen + a] * ex.x, sX[qq * nen + a] * ex.y);
+                    double2 xb = make_double2(sX[qq * nen + b] * ex.x, sX[qq * nen + b] * ex.y);
+                    double2 za = make_double2(sZ[qq * nen + a] * ez.x, sZ[qq * nen + a] * ez.y);
+                    double2 zb = make_double2(sZ[qq * nen + b] * ez.x, sZ[qq * nen + b] * ez.y);
+                    double2 xx = cmul(xa, xb), zz = cmul(za, zb), xz = cmul(xa, zb), zx = cmul(za, xb);
+                    double L2 = L + 2.0 * M;
+                    // real-coefficient blocks
+                    acc[0].x += L2 * xx.x + M * zz.x; acc[0].y += L2 * xx.y + M * zz.y;          // 11
+                    acc[4].x += M * (xx.x + zz.x);    acc[4].y += M * (xx.y + zz.y);             // 22
+                    acc[8].x += L2 * zz.x + M * xx.x; acc[8].y += L2 * zz.y + M * xx.y;          // 33
+                    acc[2].x += L * xz.x + M * zx.x;  acc[2].y += L * xz.y + M * zx.y;           // 13
+                    acc[6].x += L * zx.x + M * xz.x;  acc[6].y += L * zx.y + M * xz.y;           // 31
+                    // i ky couplings: CE12 = i ky (L xa nb - M na xb), KE(1,2) -= CE12, etc.
+                    double2 t12 = make_double2(L * xa.x * nb - M * na * xb.x, L * xa.y * nb - M * na * xb.y);
+                    double2 t21 = make_double2(L * na * xb.x - M * xa.x * nb, L * na * xb.y - M * xa.y * nb);
+                    double2 t23 = make_double2(L * na * zb.x - M * za.x * nb, L * na * zb.y - M * za.y * nb);
+                    double2 t32 = make_double2(L * za.x * nb - M * na * zb.x, L * za.y * nb - M * na * zb.y);
+                    // multiply by i ky: (x, y) -> (-ky y, ky x)
+                    acc[1].x += par.kv * t12.y;  acc[1].y -= par.kv * t12.x;    // -(i ky t12)
+                    acc[3].x -= par.kv * t21.y;  acc[3].y += par.kv * t21.x;    // +(i ky t21)
+                    acc[5].x -= par.kv * t23.y;  acc[5].y += par.kv * t23.x;    // +(i ky t23)
+                    acc[7].x += par.kv * t32.y;  acc[7].y -= par.kv * t32.x;    // -(i ky t32)
+                    double nn = na * nb;
+                    me[0] += (-w2 * R + k2 * M) * nn;
+                    me[1] += (-w2 * R + k2 * L2) * nn;
+                    me[2] += (-w2 * R + k2 * M) * nn;
+                }
+            }
+        }
+        if (pr < np) {
+            acc[0].x += me[0]; acc[4].x += me[1]; acc[8].x += me[2];
+            const int *L = lm + (size_t)e * 3 * nen;
+            for (int ci = 0; ci < 3; ++ci)
+                for (int cj = 0; cj < 3; ++cj) {
+                    int rd = L[a * 3 + ci], cd = L[b * 3 + cj];
+                    if (rd < 0 || cd < 0) continue;
+                    int64_t s = find_col(jcptr, irptr[rd], irptr[rd + 1], cd);
+                    if (s < 0) { *err = 3; continue; }
+                    out[s].x += acc[ci * 3 + cj].x;
+                    out[s].y += acc[ci * 3 + cj].y;
+                }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y = S x (one wave per row)
+template <typename CT, typename AT>
+__global__ void k_spmv(int ndof, const int64_t *__restrict__ irptr, const int *__restrict__ jcptr,
+                       const CT *__restrict__ s, const CT *__restrict__ x, CT *__restrict__ y) {
+    const int wpb = blockDim.x / 64;
+    const int r = blockIdx.x * wpb + threadIdx.x / 64;
+    const int l = threadIdx.x & 63;
+    if (r >= ndof) return;
+    AT ar = 0, ai = 0;
+    for (int64_t k = irptr[r] + l; k < irptr[r + 1]; k += 64) {
+        CT a = s[k], b = x[jcptr[k]];
+        ar += (AT)a.x * b.x - (AT)a.y * b.y;
+        ai += (AT)a.x * b.y + (AT)a.y * b.x;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        ar += __shfl_xor(ar, o);
+        ai += __shfl_xor(ai, o);
+    }
+    if (l == 0) { y[r].x = ar; y[r].y = ai; }
+}
+
+// Bielak effective force (asmble.f90:357-373 restricted to B/E couplings)
+__global__ void k_peff(int ndof, const int64_t *__restrict__ irptr, const int *__restrict__ jcptr,
+                       const char *__restrict__ cnp, const float2 *__restrict__ s,
+                       const float2 *__restrict__ ue, float2 *__restrict__ out) {
+    const int wpb = blockDim.x / 64;
+    const int r = blockIdx.x * wpb + threadIdx.x / 64;
+    const int l = threadIdx.x & 63;
+    if (r >= ndof) return;
+    const char c1 = cnp[r];
+    float ar = 0.f, ai = 0.f;
+    if (c1 == 'B' || c1 == 'E') {
+        for (int64_t k = irptr[r] + l; k < irptr[r + 1]; k += 64) {
+            int j = jcptr[k];
+            char c2 = cnp[j];
+            float sg = (c1 == 'B' && c2 == 'E') ? -1.f : ((c1 == 'E' && c2 == 'B') ? 1.f : 0.f);
+            if (sg != 0.f) {
+                float2 a = s[k], b = ue[j];
+                ar += sg * (a.x * b.x - a.y * b.y);
+                ai += sg * (a.x * b.y + a.y * b.x);
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        ar += __shfl_xor(ar, o);
+        ai += __shfl_xor(ai, o);
+    }
+    if (l == 0) out[r] = make_float2(ar, ai);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint gradient densities.  For fields u (forward) and l (adjoint) with
+// d/dy -> -i ky and test-side conjugation:
+//   d_lambda = Re[conj(div l) div u],  div u = u1,x + u3,z - i ky u2
+//   d_mu     = Re[conj(grad l) : (grad u + grad u^T)]
+//   d_rho    = -omega^2 Re[conj(l) . u]
+// accumulated over the wavefield pairs at each quadrature point, then
+// projected on the element's four anchor hat functions.
+__global__ void k_grad(int nen, int nq1, int nelem, const double *__restrict__ shl,
+                       const double *__restrict__ wq, const double *__restrict__ xq,
+                       const int *__restrict__ ieng, const double *__restrict__ xl,
+                       const double *__restrict__ zl, const char *__restrict__ cdom,
+                       const int *__restrict__ lm, int ntrip, int64_t ndof,
+                       const double *__restrict__ omega, const double *__restrict__ ky,
+                       const double2 *__restrict__ u, const double2 *__restrict__ l,
+                       double *__restrict__ part, int *__restrict__ err) {
+    extern __shared__ double sm[];
+    const int e = blockIdx.x;
+    const int nq = nq1 * nq1;
+    double2 *su = reinterpret_cast<double2 *>(sm);   // 3*nen
+    double2 *sl = su + 3 * nen;                        // 3*nen
+    double *sred = reinterpret_cast<double *>(sl + 3 * nen);   // blockDim*12
+    const int *ie = ieng + 4 * e;
+    const int *L = lm + (size_t)e * 3 * nen;
+    const bool skip = cdom[e] == 'A';
+    double g12[12];
+    for (int i = 0; i < 12; ++i) g12[i] = 0.0;
+    for (int q0 = 0; q0 < nq && !skip; q0 += blockDim.x) {
+        const int q = q0 + threadIdx.x;
+        Geo g;
+        int ix = 0, iz = 0;
+        bool act = q < nq;
+        if (act) {
+            ix = q % nq1; iz = q / nq1;
+            if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) { *err = 2; act = false; }
+        }
+        double Dl = 0.0, Dm = 0.0, Dr = 0.0;
+        for (int t = 0; t < ntrip; ++t) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < 3 * nen; i += blockDim.x) {
+                int d = L[i];
+                su[i] = d >= 0 ? u[(size_t)t * ndof + d] : make_double2(0.0, 0.0);
+                sl[i] = d >= 0 ? l[(size_t)t * ndof + d] : make_double2(0.0, 0.0);
+            }
+            __syncthreads();
+            if (!act) continue;
+            double2 U[3] = {}, UX[3] = {}, UZ[3] = {}, V[3] = {}, VX[3] = {}, VZ[3] = {};
+            for (int a = 0; a < nen; ++a) {
+                const double *s = shl + ((size_t)q * nen + a) * 3;
+                double n = s[2];
+                double nx = (s[0] * g.dzdeta - s[1] * g.dzdxi) / g.det;
+                double nz = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / g.det;
+                for (int c = 0; c < 3; ++c) {
+                    double2 uu = su[a * 3 + c], vv = sl[a * 3 + c];
+                    U[c].x += n * uu.x; U[c].y += n * uu.y;
+                    UX[c].x += nx * uu.x; UX[c].y += nx * uu.y;
+                    UZ[c].x += nz * uu.x; UZ[c].y += nz * uu.y;
+                    V[c].x += n * vv.x; V[c].y += n * vv.y;
+                    VX[c].x += nx * vv.x; VX[c].y += nx * vv.y;
+                    VZ[c].x += nz * vv.x; VZ[c].y += nz * vv.y;
+                }
+            }
+            const double k = ky[t], w = omega[t];
+            // gradients G[i][j], j = x, y, z; d/dy = -i ky
+            double2 GU[3][3], GV[3][3];
+            for (int c = 0; c < 3; ++c) {
+                GU[c][0] = UX[c]; GU[c][2] = UZ[c]; GU[c][1] = make_double2(k * U[c].y, -k * U[c].x);
+                GV[c][0] = VX[c]; GV[c][2] = VZ[c]; GV[c][1] = make_double2(k * V[c].y, -k * V[c].x);
+            }
+            double2 du = make_double2(GU[0][0].x + GU[1][1].x + GU[2][2].x, GU[0][0].y + GU[1][1].y + GU[2][2].y);
+            double2 dv = make_double2(GV[0][0].x + GV[1][1].x + GV[2][2].x, GV[0][0].y + GV[1][1].y + GV[2][2].y);
+            Dl += dv.x * du.x + dv.y * du.y;
+            double sm2 = 0.0;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double2 s2 = make_double2(GU[i][j].x + GU[j][i].x, GU[i][j].y + GU[j][i].y);
+                    sm2 += GV[i][j].x * s2.x + GV[i][j].y * s2.y;
+                }
+            Dm += sm2;
+            Dr += -w * w * (V[0].x * U[0].x + V[0].y * U[0].y + V[1].x * U[1].x + V[1].y * U[1].y +
+                            V[2].x * U[2].x + V[2].y * U[2].y);
+        }
+        if (act) {
+            double wt = wq[ix] * wq[iz] * g.det;
+            for (int k = 0; k < 4; ++k) {
+                g12[k * 3 + 0] += wt * g.sf[k] * Dl;
+                g12[k * 3 + 1] += wt * g.sf[k] * Dm;
+                g12[k * 3 + 2] += wt * g.sf[k] * Dr;
+            }
+        }
+    }
+    // block reduction of the 12 partial sums
+    for (int i = 0; i < 12; ++i) sred[threadIdx.x * 12 + i] = g12[i];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int i = 0; i < 12; ++i) sred[threadIdx.x * 12 + i] += sred[(threadIdx.x + st) * 12 + i];
+        __syncthreads();
+    }
+    if (threadIdx.x < 12) part[(size_t)e * 12 + threadIdx.x] = sred[threadIdx.x];
+}
+
+// per-anchor sum of element partials and chain rule to (Vp, Vs, rho)
+__global__ void k_grad_gather(int nnpg, int ncon, const int *__restrict__ mconn,
+                              const int *__restrict__ ieng, const double *__restrict__ part,
+                              const double *__restrict__ rho, const double *__restrict__ lam,
+                              const double *__restrict__ mu, int chain, double *__restrict__ g) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnpg) return;
+    double gl = 0, gm = 0, gr = 0;
+    for (int c = 0; c < ncon; ++c) {
+        int e = mconn[(size_t)c * nnpg + a];
+        if (e <= 0) break;
+        --e;
+        for (int k = 0; k < 4; ++k)
+            if (ieng[4 * e + k] - 1 == a) {
+                gl += part[(size_t)e * 12 + k * 3 + 0];
+                gm += part[(size_t)e * 12 + k * 3 + 1];
+                gr += part[(size_t)e * 12 + k * 3 + 2];
+            }
+    }
+    if (chain) {
+        double r = rho[a], vp = sqrt((lam[a] + 2.0 * mu[a]) / r), vs = sqrt(mu[a] / r);
+        g[3 * a + 0] += gl * 2.0 * r * vp;
+        g[3 * a + 1] += gl * (-4.0 * r * vs) + gm * 2.0 * r * vs;
+        g[3 * a + 2] += gl * (vp * vp - 2.0 * vs * vs) + gm * vs * vs + gr;
+    } else {
+        g[3 * a + 0] += gl;
+        g[3 * a + 1] += gm;
+        g[3 * a + 2] += gr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reference virtual-force columns (VFORCE25 through CFMAT_DIST, NVINV = 1).
+// Column of anchor a: element e = the last entry of MCONN(a,:) (VFORCE25
+// clears FORCEV inside its element loop, vforce25.f:252-256), corner k with
+// IENG(k,e) = a, collocation DVP = 2 alpha_a N_k, DVS = DVP / VPVS^2 ('PP')
+// or DVS = 2 beta_a N_k, DVP = DVS VPVS^2 ('SS') (djac.f:94-103,
+// vforce25.f:164-179).  DKM_VPVS (dstiff25.f:570-656) applied to u_e
+// matrix-free at the quadrature points, times ELEM_WTS(e).
+__global__ void k_cfmat(int nen, int nq1, int nnpgl, const int *__restrict__ col_node,
+                        const int *__restrict__ col_elem, const int *__restrict__ col_corner,
+                        const double *__restrict__ col_cp, const double *__restrict__ col_cs,
+                        const double *__restrict__ shl, const double *__restrict__ wq,
+                        const double *__restrict__ xq, const int *__restrict__ ieng,
+                        const double *__restrict__ xl, const double *__restrict__ zl,
+                        const int *__restrict__ lm, const double *__restrict__ ew,
+                        const int *__restrict__ jcsc, const int *__restrict__ icsc, double kv,
+                        const float2 *__restrict__ wave, float2 *__restrict__ fmat,
+                        int *__restrict__ err) {
+    extern __shared__ double sm[];
+    const int c = blockIdx.x;
+    if (c >= nnpgl) return;
+    const int e = col_elem[c], kc = col_corner[c];
+    const double cp = col_cp[c], cs = col_cs[c];
+    const int nq = nq1 * nq1;
+    double2 *su = reinterpret_cast<double2 *>(sm);     // 3*nen
+    double *sF = reinterpret_cast<double *>(su + 3 * nen);   // per q: 12 flux values
+    double *sG = sF + 12 * nq;                               // per q: geometry (det, dxdxi, dxdeta, dzdxi, dzdeta)
+    const int *ie = ieng + 4 * e;
+    const int *L = lm + (size_t)e * 3 * nen;
+    for (int i = threadIdx.x; i < 3 * nen; i += blockDim.x) {
+        int d = L[i];
+        float2 v = d >= 0 ? wave[d] : make_float2(0.f, 0.f);
+        su[i] = make_double2(v.x, v.y);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+        int ix = q % nq1, iz = q / nq1;
+        Geo g;
+        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) *err = 2;
+        double2 U[3] = {}, UX[3] = {}, UZ[3] = {};
+        for (int a = 0; a < nen; ++a) {
+            const double *s = shl + ((size_t)q * nen + a) * 3;
+            double n = s[2];
+            double nx = (s[0] * g.dzdeta - s[1] * g.dzdxi) / g.det;
+            double nz = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / g.det;
+            for (int cc = 0; cc < 3; ++cc) {
+                double2 uu = su[a * 3 + cc];
+                U[cc].x += n * uu.x; U[cc].y += n * uu.y;
+                UX[cc].x += nx * uu.x; UX[cc].y += nx * uu.y;
+                UZ[cc].x += nz * uu.x; UZ[cc].y += nz * uu.y;
+            }
+        }
+        double w = wq[ix] * wq[iz] * g.det * g.sf[kc];
+        double P = w * cp, Sq = w * cs, Q = P - 2.0 * Sq, k2 = kv * kv;
+        double *F = sF + 12 * q;
+        // row comp 1: Nx*(P u1x + Q u3z - ky Q u2) + Nz*Sq(u1z + u3x) + N*(ky Sq u2x + ky^2 Sq u1)
+        // row comp 2: N*(ky Q (u1x + u3z) + ky^2 P u2) + Nx*Sq(u2x - ky u1) + Nz*Sq(u2z - ky u3)
+        // row comp 3: Nz*(Q u1x + P u3z - ky Q u2) + Nx*Sq(u1z + u3x) + N*(ky Sq u2z + ky^2 Sq u3)
+        // stored as (re, im) for [comp][N, Nx, Nz]
+        double2 t;
+        t = make_double2(P * UX[0].x + Q * UZ[2].x - kv * Q * U[1].x, P * UX[0].y + Q * UZ[2].y - kv * Q * U[1].y);
+        F[0] = t.x; F[1] = t.y;                                     // comp1, Nx
+        F[2] = Sq * (UZ[0].x + UX[2].x); F[3] = Sq * (UZ[0].y + UX[2].y);   // comp1, Nz
+        F[4] = kv * Sq * UX[1].x + k2 * Sq * U[0].x; F[5] = kv * Sq * UX[1].y + k2 * Sq * U[0].y;   // comp1, N
+        // comp 2
+        F[6] = kv * Q * (UX[0].x + UZ[2].x) + k2 * P * U[1].x; F[7] = kv * Q * (UX[0].y + UZ[2].y) + k2 * P * U[1].y;  // N
+        sG[5 * q + 0] = Sq * (UX[1].x - kv * U[0].x);   // comp2 Nx re
+        sG[5 * q + 1] = Sq * (UX[1].y - kv * U[0].y);   // comp2 Nx im
+        sG[5 * q + 2] = Sq * (UZ[1].x - kv * U[2].x);   // comp2 Nz re
+        sG[5 * q + 3] = Sq * (UZ[1].y - kv * U[2].y);   // comp2 Nz im
+        // comp 3
+        F[8] = Q * UX[0].x + P * UZ[2].x - kv * Q * U[1].x; F[9] = Q * UX[0].y + P * UZ[2].y - kv * Q * U[1].y;   // Nz
+        F[10] = kv * Sq * UZ[1].x + k2 * Sq * U[2].x; F[11] = kv * Sq * UZ[1].y + k2 * Sq * U[2].y;  // N
+        // comp3 Nx term equals comp1 Nz term (Sq (u1z + u3x)); geometry kept for the test pass
+        sG[5 * q + 4] = g.det;
+    }
+    __syncthreads();
+    const int i1 = jcsc[c] - 1, i2 = jcsc[c + 1] - 1;
+    for (int r = i1 + threadIdx.x; r < i2; r += blockDim.x) fmat[r] = make_float2(0.f, 0.f);
+    __syncthreads();
+    const double we = ew[e];
+    for (int i = threadIdx.x; i < 3 * nen; i += blockDim.x) {
+        const int a = i / 3, cc = i % 3;
+        const int dof = L[i];
+        if (dof < 0) continue;
+        double re = 0.0, im = 0.0;
+        for (int q = 0; q < nq; ++q) {
+            int ix = q % nq1, iz = q / nq1;
+            Geo g;
+            geo_at(xq[ix], xq[iz], ie, xl, zl, g);
+            const double *s = shl + ((size_t)q * nen + a) * 3;
+            double n = s[2];
+            double nx = (s[0] * g.dzdeta - s[1] * g.dzdxi) / g.det;
+            double nz = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / g.det;
+            const double *F = sF + 12 * q;
+            if (cc == 0) {
+                re += nx * F[0] + nz * F[2] + n * F[4];
+                im += nx * F[1] + nz * F[3] + n * F[5];
+            } else if (cc == 1) {
+                re += n * F[6] + nx * sG[5 * q + 0] + nz * sG[5 * q + 2];
+                im += n * F[7] + nx * sG[5 * q + 1] + nz * sG[5 * q + 3];
+            } else {
+                re += nz * F[8] + nx * F[2] + n * F[10];
+                im += nz * F[9] + nx * F[3] + n * F[11];
+            }
+        }
+        // row position inside the column (ICSC sorted, 1-based DOFs)
+        int lo = i1, hi = i2 - 1, pos = -1, key = dof + 1;
+        while (lo <= hi) {
+            int m = (lo + hi) >> 1;
+            if (icsc[m] == key) { pos = m; break; }
+            if (icsc[m] < key) lo = m + 1; else hi = m - 1;
+        }
+        if (pos < 0) { *err = 3; continue; }
+        fmat[pos] = make_float2((float)(we * re), (float)(we * im));
+    }
+}
+
+// ADJFLAM_DIST: gprod(mygrad(c)) -= Re(conj(F) lambda) over column c
+__global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *__restrict__ jcsc,
+                          const int *__restrict__ icsc, const float2 *__restrict__ f,
+                          const float2 *__restrict__ lam, float *__restrict__ g) {
+    const int c = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int l = threadIdx.x & 63;
+    if (c >= nnpgl) return;
+    double acc = 0.0;
+    for (int i = jcsc[c] - 1 + l; i < jcsc[c + 1] - 1; i += 64) {
+        float2 a = f[i], b = lam[icsc[i] - 1];
+        acc += (double)(a.x * b.x + a.y * b.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (l == 0) g[mygrad[c] - 1] -= (float)acc;
+}
+
+// ---------------------------------------------------------------------------
+// small utilities
+template <typename T>
+__global__ void k_fill(T *p, size_t n, T v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// [pair][dof] <-> [dof][pair] transposes for the host convenience wrappers
+__global__ void k_transpose_c(const double2 *__restrict__ in, double2 *__restrict__ out, int rows, int cols) {
+    __shared__ double2 tile[16][17];
+    int c = blockIdx.x * 16 + threadIdx.x, r = blockIdx.y * 16 + threadIdx.y;
+    if (r < rows && c < cols) tile[threadIdx.y][threadIdx.x] = in[(size_t)r * cols + c];
+    __syncthreads();
+    c = blockIdx.y * 16 + threadIdx.x;
+    r = blockIdx.x * 16 + threadIdx.y;
+    if (r < cols && c < rows) out[(size_t)r * rows + c] = tile[threadIdx.x][threadIdx.y];
+}
+
+// MFMA lane-layout self check: D = A(16x4) B(4x16) for both data types
+__global__ void k_mfma_probe(double *outd, float *outf) {
+    const int l = threadIdx.x;
+    const int row = l & 15, k = l >> 4;
+    double a = 1.0 + row + 16.0 * k, b = 1.0 + 0.5 * (l & 15) - 3.0 * k;
+    MfmaTraits<double>::v4 cd = {0, 0, 0, 0};
+    cd = MfmaTraits<double>::mma(a, b, cd);
+    MfmaTraits<float>::v4 cf = {0, 0, 0, 0};
+    cf = MfmaTraits<float>::mma((float)a, (float)b, cf);
+    for (int i = 0; i < 4; ++i) {
+        outd[l * 4 + i] = cd[i];
+        outf[l * 4 + i] = cf[i];
+    }
+}
+
+}  // namespace f25k

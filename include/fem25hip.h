@@ -1,0 +1,172 @@
+/*
+ * fem25hip.h -- C ABI of the MI355X (gfx950) implementation of fem25's
+ * element-operator / assembly / gradient hot path.
+ *
+ * All arrays are plain host pointers unless an entry point says "_dev".
+ * Integer maps keep the reference's conventions: 1-based, column-major
+ * (Fortran) layouts, 0 marks a fixed DOF in LM.  Complex arrays are
+ * interleaved (re, im) pairs of float (COMPLEX*8) or double (COMPLEX*16).
+ * Every entry point returns 0 on success and a non-zero error code
+ * otherwise (the reference's IERR convention); none of them aborts.  Calls
+ * are synchronous with respect to the caller: results are complete when the
+ * call returns.  A mesh handle may be used by one host thread at a time.
+ */
+#ifndef FEM25HIP_H
+#define FEM25HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    F25_OK = 0,
+    F25_ERR_ARG = 1,        /* bad argument / size */
+    F25_ERR_JACOBIAN = 2,   /* det J <= 2.22e-16 or negative (cjac.f:115-125) */
+    F25_ERR_PATTERN = 3,    /* a (row, col) pair is missing from the CSR pattern (asmble.f90:296-300) */
+    F25_ERR_HIP = 4,        /* a HIP runtime call failed */
+    F25_ERR_NOMEM = 5,
+    F25_ERR_GRAPH = 6,      /* numbering failed (graph25.f fidelity checks) */
+    F25_ERR_INVTYPE = 7,    /* CINVTYPE other than PP/SS (vforce25.f:66-85) */
+    F25_ERR_STATE = 8       /* call order: model or inversion structures missing */
+};
+
+/* Precision selector for assembled operators and products. */
+enum { F25_C64 = 0, F25_C128 = 1 };
+/* Arithmetic of the batched element-block product. */
+enum { F25_APPLY_F64 = 0, F25_APPLY_F32 = 1, F25_APPLY_F32_REFINED = 2 };
+
+/*
+ * Anchor mesh, the fields of MESH_INFO (fwd_struc.h:10-141) the path reads.
+ * pml[4] holds RCOEFF, DMAX, KAPPA, FCENT for body waves (CFTYPE /= 'S');
+ * pml_srf[4] the surface-wave set (CFTYPE == 'S'), asmble.f90:78-88.
+ */
+typedef struct {
+    int nord;                /* NORD: GLL order, NLXI = NLETA = NORD+1 */
+    int nelem;               /* NELEM */
+    int nnpg;                /* NNPG anchors */
+    const int *ieng;         /* IENG(4,NELEM), 1-based, CCW */
+    const double *xlocs;     /* XLOCS(NNPG) m */
+    const double *zlocs;     /* ZLOCS(NNPG) m, z up */
+    const double *xd;        /* XD(NNPG) distance into PML, may be NULL without 'A' elements */
+    const double *zd;        /* ZD(NNPG) */
+    const char *cdomain;     /* CDOMAIN(NELEM): 'I', 'E', 'A' */
+    const char *cnnpg;       /* CNNPG(NNPG), CHARACTER(2): 2*NNPG chars */
+    double xwidth, zwidth;   /* XWIDTH, ZWIDTH */
+    double pml[4];
+    double pml_srf[4];
+} f25_mesh_desc;
+
+typedef struct f25_mesh_s f25_mesh;
+
+/* ---- mesh and model -------------------------------------------------- */
+
+/* Builds the GLL numbering with GEN_GRAPH25's semantics
+ * (graph25_driver.f90:30-86: GENIEN, GENID, GENLM, CNMESH) and uploads the
+ * mesh to the current HIP device.  The CSR pattern (GENCRS_QUICK) is built
+ * on first use by the assembled-operator entry points. */
+int f25_mesh_create(const f25_mesh_desc *desc, f25_mesh **out);
+/* Creates a handle on a numbering the caller already holds: LM(3,NEN,NELEM)
+ * and, optionally (irptr == NULL skips), the CSR pattern IRPTR(NDOF+1),
+ * JCPTR(NZ).  This is how the drop-in shims hand over MSH%LM, MSH%IRPTR_LOC
+ * and MSH%JCPTR_LOC unchanged. */
+int f25_mesh_create_with_graph(const f25_mesh_desc *desc, int ndof, const int *lm,
+                               const int *irptr, const int *jcptr, f25_mesh **out);
+int f25_mesh_destroy(f25_mesh *m);
+/* sizes[0] = NNP fine nodes, [1] = NDOF, [2] = NZ (builds the CSR pattern),
+ * [3] = number of element colours, [4] = NEN, [5] = quadrature points per
+ * direction (2*NORD+5). */
+int f25_mesh_sizes(f25_mesh *m, int64_t sizes[6]);
+/* Copies the numbering out in the reference's layouts (any pointer may be
+ * NULL): IEN(NEN,NELEM), LM(3,NEN,NELEM), IRPTR(NDOF+1), JCPTR(NZ), CNP(NDOF). */
+int f25_mesh_graph(f25_mesh *m, int *ien, int *lm, int *irptr, int *jcptr, char *cnp);
+/* DENS(NNPG) kg/m^3 and ECOEFF(NNPG,2) = (lambda, mu) Pa.  Recomputes the
+ * element blocks on the device. */
+int f25_set_model(f25_mesh *m, const double *dens, const double *ecoeff);
+
+/* ---- assembled operator (replaces ASMBLE_DRIVER, asmble.f90:1-231) ------ */
+
+/* S(omega, ky) = K - omega^2 M in the CSR order of the pattern, with
+ * omega = 2 pi freq and ky = omega py (estiff25.f:61).  precision F25_C64
+ * writes COMPLEX*8 (the reference's CDIST), F25_C128 writes COMPLEX*16.
+ * out is a host array of NZ complex values. */
+int f25_asmble(f25_mesh *m, char cftype, double freq, double py, int precision, void *out);
+/* Same into device memory (NZ complex values), left resident for f25_csr_apply_dev. */
+int f25_asmble_dev(f25_mesh *m, char cftype, double freq, double py, int precision, void *out_dev);
+/* y = S x with an assembled S (device pointers, NDOF-long complex vectors). */
+int f25_csr_apply_dev(f25_mesh *m, int precision, const void *s_dev, const void *x_dev, void *y_dev);
+
+/* Bielak effective force (ASMB_PEFF, asmble.f90:318-377, one rank):
+ * PEFF(row 'B') -= S u_E over 'E' columns, PEFF(row 'E') += S u_E over 'B'
+ * columns.  cdist, ue, peff are COMPLEX*8 host arrays. */
+int f25_asmb_peff(f25_mesh *m, const void *cdist, const void *ue, void *peff);
+
+/* ---- batched matrix-free product ---------------------------------------- */
+
+/* y_b = S(omega_b, ky_b) x_b for b = 0..npair-1, non-PML elements.  x and y
+ * are device arrays of npair*NDOF complex values laid out [dof][pair]
+ * (pair fastest), double complex for F25_APPLY_F64 and F25_APPLY_F32_REFINED,
+ * float complex for F25_APPLY_F32. */
+int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
+                        int mode, const void *x_dev, void *y_dev);
+/* Host convenience wrapper: x, y are [pair][dof] double complex (one vector
+ * per pair, NDOF contiguous, i.e. Fortran X(NDOF,NPAIR)). */
+int f25_apply_batch(f25_mesh *m, int npair, const double *omega, const double *ky, int mode,
+                    const void *x, void *y);
+
+/* ---- gradient ------------------------------------------------------------ */
+
+/* Exact adjoint-state gradient for ntrip (omega, ky) wavefield pairs:
+ *   g[3*a + p] += sum_t Re( lam_t^H dS(omega_t, ky_t)/dm_{a,p} u_t )
+ * for anchor a and p = (Vp, Vs, rho), non-PML elements, model as in
+ * f25_set_model (lambda = rho(Vp^2-2Vs^2), mu = rho Vs^2).  u, lam:
+ * [trip][dof] double complex host arrays.  With lam solving
+ * S^H lam = R^T (d - R u) this is the gradient of 1/2 |R u - d|^2. */
+int f25_gradient(f25_mesh *m, int ntrip, const double *omega, const double *ky,
+                 const void *u, const void *lam, double *g);
+int f25_gradient_dev(f25_mesh *m, int ntrip, const double *omega, const double *ky,
+                     const void *u_dev, const void *lam_dev, double *g_dev);
+
+/* ---- reference virtual-force path (CFMAT_DIST / ADJFLAM_DIST) ------------ */
+
+/* GENGMASK + ELEM_WTS + GRADPTRS for one rank (gengmask.f90:1-262,
+ * 758-841, 962-1198), no mask file.  cinvtype "PP" or "SS"; mrdof1 holds
+ * MRDOF(1,1:NREC) (used only when linrec == 0).  sizes: [0] NNPINV,
+ * [1] NCON, [2] NNPGL, [3] NZ_FDIST, [4] MBUFRHS, [5] NA35. */
+int f25_inv_setup(f25_mesh *m, const char *cinvtype, double vpvs, int linrec, int lkbdry,
+                  int nrec, const int *mrdof1, int sizes[6]);
+/* MASKG(NNPG), MYGRAD(NNPGL), JCSC_FDIST(NNPGL+1), ICSC_FDIST(NZ_FDIST),
+ * ELEM_WTS(NELEM); any pointer may be NULL. */
+int f25_inv_get(f25_mesh *m, int *maskg, int *mygrad, int *jcsc, int *icsc, double *elem_wts);
+/* CFMAT_DIST (hessloc.f90:1-133 + vforce25.f:1-369): FMAT_DIST(NZ_FDIST)
+ * COMPLEX*8 from WAVE(NDOF) COMPLEX*8, reference semantics. */
+int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *wave, void *fmat);
+/* ADJFLAM_DIST (adjgrad.f90:88-145, one rank): GPROD(NA35) REAL*4. */
+int f25_adjflam_dist(f25_mesh *m, const void *fmat, const void *adjlam, float *gprod);
+
+/* ---- Gauss-Newton element products ---------------------------------------- */
+
+/* F v = dS/dm[v] u for a model perturbation v (NNPG,3) = (dVp, dVs, drho)
+ * at anchors, ntrip (omega, ky, u) triples.  u and out: [trip][dof] double
+ * complex host arrays. */
+int f25_gn_fv(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
+              const double *v, void *out);
+/* Re(F^H w): the gradient form with lam := w; g (NNPG,3) accumulated. */
+int f25_gn_fhw(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
+               const void *w, double *g);
+/* Pseudo-Hessian diagonal: diag[a,p] += sum_t || dS_t/dm_{a,p} u_t ||^2. */
+int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
+                double *diag);
+
+/* ---- diagnostics ------------------------------------------------------------ */
+const char *f25_last_error(void);
+/* Average duration (ms) of the most recent launch of the named kernel
+ * family ("apply", "grad", "asmble"), measured with HIP events on the
+ * stream the kernel ran on; -1 if unknown. */
+double f25_last_kernel_ms(const char *family);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

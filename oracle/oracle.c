@@ -1,0 +1,1334 @@
+/*
+ * oracle.c -- CPU restatement of fem25's hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This file is the parity checker for the HIP implementation in fem25_amd/.
+ * It is loaded by tests/, by __graft_entry__.smoke() and by the cpu_baseline
+ * leg of bench.py, never by the product.  Every function follows the loop
+ * structure of the reference routine it restates (cited per function), so the
+ * floating-point work happens in the same order wherever that is practical.
+ *
+ * How each function is pinned (checks live in tests/):
+ *   orc_gll          numpy Legendre closed form (roots of P'_{n-1}, weights
+ *                    2/(n(n-1)P_{n-1}^2)); compiled reference GAUSSQ tables.
+ *   orc_dshl         cardinal property at the GLL nodes, partition of unity,
+ *                    exact derivative of polynomials of degree <= p; compiled
+ *                    reference DSHL tables (p = 4, 5, 6).
+ *   orc_csf2d        closed form + partition of unity.
+ *   orc_cjmk_tvi     affine rectangles: det J = hx*hz/4, diagonal inverse;
+ *                    compiled reference CJMK_TVI.
+ *   orc_km25isoq     rigid translations in the null space of K0, symmetry /
+ *                    Hermiticity, mass-block sum = -w^2 int rho + ky^2 int C;
+ *                    compiled reference KM25ISOQ.
+ *   orc_zdampfn,
+ *   orc_cjaccq,
+ *   orc_cm25isoq     reduces to KM25ISOQ when the stretch factor is 1;
+ *                    compiled reference CJACCQ/CM25ISOQ on PML elements.
+ *   orc_genien..     brute-force adjacency pattern (GENCRS_DUMB idea),
+ *   orc_gencrs       node-count formulas of the structured mesh; compiled
+ *                    reference GENIEN/GENID/GENLM/GENCRS_QUICK where it builds.
+ *   orc_asmble_*     element-sum identity against KM25ISOQ, quadratic-in-ky /
+ *                    affine-in-w^2 identity; compiled reference ASMBLE_DRIVER.
+ *   orc_djacab,
+ *   orc_dkm_vpvs     linearity in the collocation field; compiled reference.
+ *   orc_cfmat_dist   compiled reference CFMAT_DIST/VFORCE25.
+ *   orc_adjflam_dist hand-computed CSC gather on a 3-column example.
+ *   orc_grad_exact   central finite differences of the objective (exact for a
+ *                    quadratic) on small meshes.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+#define NDIM 3
+
+/* ------------------------------------------------------------------------ */
+/* gaussq.f:5-130 (GAUSSQ with kind=5, alpha=beta=0, kpts=2, endpoints -1, 1):
+ * the Golub-Welsch construction of the Gauss-Lobatto-Legendre rule --
+ * Jacobi recurrence coefficients (class, gaussq.f:160-262), Lobatto
+ * modification of the last row (solve, gaussq.f:132-155; gaussq.f:96-101),
+ * implicit-QL eigenvalues with first eigenvector components (gausq2,
+ * gaussq.f:266-403).  Restated step by step: the nodes' last bits decide
+ * GENID's fixed-DOF test (graph25.f:437-469) and must match. */
+static double gq_solve(double shift, int n, const double *a, const double *b)
+{
+    double alpha = a[1] - shift;
+    for (int i = 2; i <= n - 1; i++) alpha = a[i] - shift - (b[i - 1] * b[i - 1]) / alpha;
+    return 1.0 / alpha;
+}
+
+static void gq_gausq2(int n, double *d, double *e, double *z)
+{
+    const double machep = 2.220446049250313e-16;
+    if (n == 1) return;
+    e[n] = 0.0;
+    for (int l = 1; l <= n; l++) {
+        int j = 0, m;
+        for (;;) {
+            for (m = l; m <= n; m++) {
+                if (m == n) break;
+                if (fabs(e[m]) <= machep * (fabs(d[m]) + fabs(d[m + 1]))) break;
+            }
+            double p = d[l];
+            if (m == l) break;
+            if (j == 30) return;
+            j++;
+            double g = (d[l + 1] - p) / (2.0 * e[l]);
+            double r = sqrt(g * g + 1.0);
+            g = d[m] - p + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+            double s = 1.0, c = 1.0;
+            p = 0.0;
+            for (int ii = 1; ii <= m - l; ii++) {
+                int i = m - ii;
+                double f = s * e[i], b = c * e[i];
+                if (fabs(f) < fabs(g)) {
+                    s = f / g; r = sqrt(s * s + 1.0); e[i + 1] = g * r; c = 1.0 / r; s = s * c;
+                } else {
+                    c = g / f; r = sqrt(c * c + 1.0); e[i + 1] = f * r; s = 1.0 / r; c = c * s;
+                }
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+                f = z[i + 1];
+                z[i + 1] = s * z[i] + c * f;
+                z[i] = c * z[i] - s * f;
+            }
+            d[l] = d[l] - p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    for (int ii = 2; ii <= n; ii++) {
+        int i = ii - 1, k = i;
+        double p = d[i];
+        for (int jj = ii; jj <= n; jj++)
+            if (d[jj] < p) { k = jj; p = d[jj]; }
+        if (k != i) { d[k] = d[i]; d[i] = p; p = z[i]; z[i] = z[k]; z[k] = p; }
+    }
+}
+
+void orc_gll(int n, double *x, double *w)
+{
+    double t[66], b[66], z[66];
+    /* class, kind = 5 with alpha = beta = 0 */
+    double ab = 0.0, abi = 2.0 + ab, a2b2 = 0.0;
+    double muzero = 2.0;          /* 2^(ab+1) gamma(1) gamma(1) / gamma(2) */
+    t[1] = 0.0 / abi;
+    b[1] = sqrt(4.0 * (1.0 + 0.0) * (1.0 + 0.0) / ((abi + 1.0) * abi * abi));
+    for (int i = 2; i <= n - 1; i++) {
+        abi = 2.0 * i + ab;
+        t[i] = a2b2 / ((abi - 2.0) * abi);
+        b[i] = sqrt(4.0 * i * (i + 0.0) * (i + 0.0) * (i + ab) / ((abi * abi - 1.0) * abi * abi));
+    }
+    abi = 2.0 * n + ab;
+    t[n] = a2b2 / ((abi - 2.0) * abi);
+    /* Lobatto: both end points fixed (gaussq.f:96-101) */
+    double gam = gq_solve(-1.0, n, t, b);
+    double t1 = (-1.0 - 1.0) / (gq_solve(1.0, n, t, b) - gam);
+    b[n - 1] = sqrt(t1);
+    t[n] = -1.0 + gam * t1;
+    z[1] = 1.0;
+    for (int i = 2; i <= n; i++) z[i] = 0.0;
+    gq_gausq2(n, t, b, z);
+    for (int i = 1; i <= n; i++) {
+        x[i - 1] = t[i];
+        w[i - 1] = muzero * z[i] * z[i];
+    }
+}
+
+/* dshl.f:266-314 POLCOE: monomial coefficients of the interpolant of y on x */
+static void polcoe(const double *x, const double *y, int n, double *cof, double *s)
+{
+    for (int i = 0; i < n; i++) { s[i] = 0.0; cof[i] = 0.0; }
+    s[n - 1] = -x[0];
+    for (int i = 1; i < n; i++) {
+        for (int j = n - 1 - i; j <= n - 2; j++) s[j] -= x[i] * s[j + 1];
+        s[n - 1] -= x[i];
+    }
+    for (int j = 0; j < n; j++) {
+        double phi = (double)n;
+        for (int k = n - 1; k >= 1; k--) phi = k * s[k] + x[j] * phi;
+        double ff = y[j] / phi, b = 1.0;
+        for (int k = n; k >= 1; k--) {
+            cof[k - 1] += b * ff;
+            b = s[k - 1] + x[j] * b;
+        }
+    }
+}
+
+static double ipow(double x, int k)
+{
+    double r = 1.0;
+    for (int i = 0; i < k; i++) r *= x;
+    return r;
+}
+
+/* dshl.f:247-261 SHAPEF */
+static double shapef(int n, double xi, const double *cof)
+{
+    double s = 0.0;
+    for (int i = n; i >= 2; i--) s += cof[i - 1] * ipow(xi, i - 1);
+    s += cof[0];
+    if (fabs(s) < 1.11e-15) s = 0.0;
+    return s;
+}
+
+/* dshl.f:1-243 DSHL with IITYPE=1 (GLL interpolation points) */
+void orc_dshl(int nl, int nint, double *xipts, double *xigll, double *wgll, double *shl)
+{
+    const double eps = 2.11e-16;
+    double wt[64];
+    int nen = nl * nl;
+    orc_gll(nl, xipts, wt);
+    if (nl % 2) xipts[(nl - 1) / 2] = 0.0;
+    orc_gll(nint, xigll, wgll);
+    if (nint % 2) xigll[(nint - 1) / 2] = 0.0;
+    double lagr[64], cofxi[64], cofeta[64], s[64], xil[64], etal[64];
+    int ia = 0;
+    for (int ieta = 0; ieta < nl; ieta++) {
+        for (int k = 0; k < nl; k++) lagr[k] = (k == ieta) ? 1.0 : 0.0;
+        polcoe(xipts, lagr, nl, cofeta, s);
+        for (int iz = 0; iz < nint; iz++) {
+            double sumd = 0.0;
+            for (int k = nl; k >= 2; k--) sumd += (k - 1.0) * cofeta[k - 1] * ipow(xigll[iz], k - 2);
+            if (fabs(sumd) < eps) sumd = 0.0;
+            etal[iz] = sumd;
+        }
+        for (int ixi = 0; ixi < nl; ixi++, ia++) {
+            for (int k = 0; k < nl; k++) lagr[k] = (k == ixi) ? 1.0 : 0.0;
+            polcoe(xipts, lagr, nl, cofxi, s);
+            for (int ix = 0; ix < nint; ix++) {
+                double sumd = 0.0;
+                for (int k = nl; k >= 2; k--) sumd += (k - 1.0) * cofxi[k - 1] * ipow(xigll[ix], k - 2);
+                if (fabs(sumd) < eps) sumd = 0.0;
+                xil[ix] = sumd;
+            }
+            for (int iz = 0; iz < nint; iz++) {
+                for (int ix = 0; ix < nint; ix++) {
+                    double *o = shl + 3 * (ia + nen * (ix + nint * iz));
+                    double sna = shapef(nl, xigll[iz], cofeta);
+                    double snb = shapef(nl, xigll[ix], cofxi);
+                    o[2] = sna * snb;
+                    if (fabs(o[2]) < eps) o[2] = 0.0;
+                    o[0] = sna * xil[ix];
+                    o[1] = snb * etal[iz];
+                    if (fabs(o[0]) < eps) o[0] = 0.0;
+                    if (fabs(o[1]) < eps) o[1] = 0.0;
+                }
+            }
+        }
+    }
+}
+
+/* csf2d.f:1-131 CSF2D for NGNOD=4 */
+int orc_csf2d(double xi, double eta, double *sf, double *dsf)
+{
+    const double eps = 2.2204e-16;
+    sf[0] = .25 * (1.0 - xi) * (1.0 - eta);
+    sf[1] = .25 * (1.0 + xi) * (1.0 - eta);
+    sf[2] = .25 * (1.0 + xi) * (1.0 + eta);
+    sf[3] = .25 * (1.0 - xi) * (1.0 + eta);
+    dsf[0 + 2 * 0] = .25 * (-1.0) * (1.0 - eta);
+    dsf[0 + 2 * 1] = .25 * (1.0) * (1.0 - eta);
+    dsf[0 + 2 * 2] = .25 * (1.0) * (1.0 + eta);
+    dsf[0 + 2 * 3] = .25 * (-1.0) * (1.0 + eta);
+    dsf[1 + 2 * 0] = .25 * (1.0 - xi) * (-1.0);
+    dsf[1 + 2 * 1] = .25 * (1.0 + xi) * (-1.0);
+    dsf[1 + 2 * 2] = .25 * (1.0 + xi) * (1.0);
+    dsf[1 + 2 * 3] = .25 * (1.0 - xi) * (1.0);
+    double s2 = 0, s3 = 0;
+    for (int i = 0; i < 4; i++) {
+        if (fabs(sf[i]) < eps) sf[i] = 0.0;
+        if (fabs(dsf[2 * i]) < eps) dsf[2 * i] = 0.0;
+        if (fabs(dsf[2 * i + 1]) < eps) dsf[2 * i + 1] = 0.0;
+        s2 += dsf[2 * i];
+        s3 += dsf[2 * i + 1];
+    }
+    if (fabs(s2) > eps || fabs(s3) > eps) return 1;
+    return 0;
+}
+
+/* csf2d.f:135-200 CSF2DND (NGNOD=4) */
+static void csf2dnd(double xi, double eta, double *sf)
+{
+    const double eps = 2.2204e-16;
+    sf[0] = .25 * (1.0 - xi) * (1.0 - eta);
+    sf[1] = .25 * (1.0 + xi) * (1.0 - eta);
+    sf[2] = .25 * (1.0 + xi) * (1.0 + eta);
+    sf[3] = .25 * (1.0 - xi) * (1.0 + eta);
+    for (int i = 0; i < 4; i++)
+        if (fabs(sf[i]) < eps) sf[i] = 0.0;
+}
+
+/* shared geometric part of cjac.f:61-135 and djac.f:73-135 */
+static int jacobian_at(double xi, double eta, const int *ieng_e, const double *xlocs,
+                       const double *zlocs, double *sf, double *rjac, double *dxdxi,
+                       double *dxdeta, double *dzdxi, double *dzdeta)
+{
+    double dsf[8];
+    if (orc_csf2d(xi, eta, sf, dsf)) return 1;
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int k = 0; k < 4; k++) {
+        int g = ieng_e[k] - 1;
+        a += dsf[2 * k] * xlocs[g];
+        b += dsf[2 * k + 1] * xlocs[g];
+        c += dsf[2 * k] * zlocs[g];
+        d += dsf[2 * k + 1] * zlocs[g];
+    }
+    *dxdxi = a; *dxdeta = b; *dzdxi = c; *dzdeta = d;
+    *rjac = a * d - b * c;
+    if (fabs(*rjac) <= 2.22e-16 || *rjac < 0.0) return 2;
+    return 0;
+}
+
+static void fill_shg(int nen, const double *shl_q, double *shg_q, double rjac, double dxdxi,
+                     double dxdeta, double dzdxi, double dzdeta)
+{
+    for (int a = 0; a < nen; a++) {
+        double drnxi = shl_q[3 * a], drneta = shl_q[3 * a + 1];
+        shg_q[3 * a + 2] = shl_q[3 * a + 2];
+        shg_q[3 * a + 0] = (drnxi * dzdeta - drneta * dzdxi) / rjac;
+        shg_q[3 * a + 1] = -(drnxi * dxdeta - drneta * dxdxi) / rjac;
+    }
+}
+
+/* cjac.f:1-138 CJMK_TVI, isotropic (NCOEFF=2) */
+int orc_cjmk_tvi(int nen, int nint, const int *ieng_e, const double *xigll,
+                 const double *xlocs, const double *zlocs, const double *dens,
+                 const double *ecoeff, int nnpg, const double *shl,
+                 double *det, double *rho, double *dmat, double *shg)
+{
+    for (int ix = 0; ix < nint; ix++) {
+        for (int iz = 0; iz < nint; iz++) {
+            double sf[4], rjac, a, b, c, d;
+            int err = jacobian_at(xigll[ix], xigll[iz], ieng_e, xlocs, zlocs, sf, &rjac, &a, &b, &c, &d);
+            int q = ix + nint * iz;
+            double r = 0, l = 0, m = 0;
+            for (int k = 0; k < 4; k++) {
+                int g = ieng_e[k] - 1;
+                l += sf[k] * ecoeff[g];
+                m += sf[k] * ecoeff[g + nnpg];
+                r += sf[k] * dens[g];
+            }
+            dmat[q] = l; dmat[q + nint * nint] = m; rho[q] = r;
+            det[q] = rjac;
+            if (err) return 1;
+            fill_shg(nen, shl + 3 * nen * q, shg + 3 * nen * q, rjac, a, b, c, d);
+        }
+    }
+    return 0;
+}
+
+/* cjac.f:818-850 ZDAMPFN */
+double complex orc_zdampfn(double rn, double vp, double delta, double dmax_in,
+                           double rcoeff, double rkappa, double fcent, double omega)
+{
+    double dmax = (dmax_in > 0.0) ? dmax_in : -3.0 * vp * log(rcoeff) / (2.0 * delta);
+    double d = dmax * (rn / delta) * (rn / delta);
+    double alpha = M_PI * fcent * (1.0 - rn / delta);
+    double complex z = rkappa + fabs(d) / (alpha + I * omega);
+    return 1.0 / z;
+}
+
+/* cjac.f:399-581 CJACCQ, isotropic */
+int orc_cjaccq(int nen, int nint, double omega, double xwidth, double zwidth,
+               double rcoeff, double dmax_in, double rkappa, double fcent,
+               const int *ieng_e, const double *xigll, const double *xd, const double *zd,
+               const double *xlocs, const double *zlocs, const double *dens,
+               const double *ecoeff, int nnpg, const double *shl,
+               double *det, double *rho, double *dmat, double *shg,
+               double complex *dampx, double complex *dampz)
+{
+    const double eps = 2.22e-16;
+    for (int ix = 0; ix < nint; ix++) {
+        for (int iz = 0; iz < nint; iz++) {
+            double sf[4], rjac, a, b, c, d;
+            int err = jacobian_at(xigll[ix], xigll[iz], ieng_e, xlocs, zlocs, sf, &rjac, &a, &b, &c, &d);
+            int q = ix + nint * iz;
+            double r = 0, l = 0, m = 0, xdep = 0, zdep = 0, vel = 0;
+            for (int k = 0; k < 4; k++) {
+                int g = ieng_e[k] - 1;
+                l += sf[k] * ecoeff[g];
+                m += sf[k] * ecoeff[g + nnpg];
+                r += sf[k] * dens[g];
+                xdep += sf[k] * xd[g];
+                zdep += sf[k] * zd[g];
+                vel += sf[k] * sqrt((ecoeff[g] + 2.0 * ecoeff[g + nnpg]) / dens[g]);
+            }
+            dmat[q] = l; dmat[q + nint * nint] = m; rho[q] = r;
+            dampx[q] = (xdep > eps) ? orc_zdampfn(xdep, vel, xwidth, dmax_in, rcoeff, rkappa, fcent, omega) : 1.0;
+            dampz[q] = (zdep > eps) ? orc_zdampfn(zdep, vel, zwidth, dmax_in, rcoeff, rkappa, fcent, omega) : 1.0;
+            det[q] = rjac;
+            if (err) return 1;
+            fill_shg(nen, shl + 3 * nen * q, shg + 3 * nen * q, rjac, a, b, c, d);
+        }
+    }
+    return 0;
+}
+
+/* estiff25.f:1-164 KM25ISOQ */
+void orc_km25isoq(int nen, int nint, double omega, double py, const double *wts,
+                  const double *vlam, const double *vmu, const double *rho,
+                  const double *det, const double *shg, double *me, double complex *ke)
+{
+    int nee = NDIM * nen;
+    for (int i = 0; i < nee * nee; i++) { me[i] = 0.0; ke[i] = 0.0; }
+    double wpy = omega * py, py2 = py * py, w2 = omega * omega;
+    for (int ix = 0; ix < nint; ix++) {
+        for (int iz = 0; iz < nint; iz++) {
+            int q = ix + nint * iz;
+            double weight = wts[ix] * wts[iz] * det[q];
+            double lam = vlam[q] * weight, mu = vmu[q] * weight;
+            double c11 = lam + 2.0 * mu, c12 = lam, c13 = lam, c22 = lam + 2.0 * mu, c23 = lam;
+            double c33 = lam + 2.0 * mu, c44 = mu, c55 = mu, c66 = mu;
+            double rhow = rho[q] * weight;
+            double cm1 = -w2 * (rhow - py2 * c66), cm2 = -w2 * (rhow - py2 * c22), cm3 = -w2 * (rhow - py2 * c44);
+            const double *sq = shg + 3 * nen * q;
+            for (int ib = 0; ib < nen; ib++) {
+                double phib = sq[3 * ib + 2], phibx = sq[3 * ib], phibz = sq[3 * ib + 1];
+                double r1 = phib * cm1, r2 = phib * cm2, r3 = phib * cm3;
+                double k11t1 = c11 * phibx, k11t2 = c55 * phibz, k13t1 = c13 * phibz, k13t2 = c55 * phibx;
+                double k22t1 = c66 * phibx, k22t2 = c44 * phibz, k31t1 = c13 * phibx, k31t2 = c55 * phibz;
+                double k33t1 = c33 * phibz, k33t2 = c55 * phibx;
+                double k12t1 = wpy * c12 * phib, k12t2 = wpy * c66 * phibx;
+                double k21t1 = wpy * c12 * phibx, k21t2 = wpy * c66 * phib;
+                double k23t1 = wpy * c23 * phibz, k23t2 = wpy * c44 * phib;
+                double k32t1 = wpy * c23 * phib, k32t2 = wpy * c44 * phibz;
+                for (int ia = 0; ia < nen; ia++) {
+                    double phia = sq[3 * ia + 2], phiax = sq[3 * ia], phiaz = sq[3 * ia + 1];
+                    int ip = NDIM * ia, iq = NDIM * ib;
+                    me[ip + nee * iq] += r1 * phia;
+                    me[ip + 1 + nee * (iq + 1)] += r2 * phia;
+                    me[ip + 2 + nee * (iq + 2)] += r3 * phia;
+                    double ke11 = phiax * k11t1 + phiaz * k11t2;
+                    double ke13 = phiax * k13t1 + phiaz * k13t2;
+                    double ke22 = phiax * k22t1 + phiaz * k22t2;
+                    double ke31 = phiaz * k31t1 + phiax * k31t2;
+                    double ke33 = phiaz * k33t1 + phiax * k33t2;
+                    double ke12 = phiax * k12t1 - phia * k12t2;
+                    double ke21 = phia * k21t1 - phiax * k21t2;
+                    double ke23 = phia * k23t1 - phiaz * k23t2;
+                    double ke32 = phiaz * k32t1 - phia * k32t2;
+                    ke[ip + nee * iq] += ke11;
+                    ke[ip + nee * (iq + 1)] -= I * ke12;
+                    ke[ip + nee * (iq + 2)] += ke13;
+                    ke[ip + 1 + nee * iq] += I * ke21;
+                    ke[ip + 1 + nee * (iq + 1)] += ke22;
+                    ke[ip + 1 + nee * (iq + 2)] += I * ke23;
+                    ke[ip + 2 + nee * iq] += ke31;
+                    ke[ip + 2 + nee * (iq + 1)] -= I * ke32;
+                    ke[ip + 2 + nee * (iq + 2)] += ke33;
+                }
+            }
+        }
+    }
+}
+
+/* estiff25.f:168-334 CM25ISOQ */
+void orc_cm25isoq(int nen, int nint, double omega, double py, const double *wts,
+                  const double *vlam, const double *vmu, const double *rho,
+                  const double *det, const double *shg, const double complex *dampx,
+                  const double complex *dampz, double *me, double complex *ce)
+{
+    int nee = NDIM * nen;
+    for (int i = 0; i < nee * nee; i++) { me[i] = 0.0; ce[i] = 0.0; }
+    double wpy = omega * py, py2 = py * py, w2 = omega * omega;
+    double complex cwpy = I * wpy;
+    for (int ix = 0; ix < nint; ix++) {
+        for (int iz = 0; iz < nint; iz++) {
+            int q = ix + nint * iz;
+            double weight = wts[ix] * wts[iz] * det[q];
+            double complex ex = dampx[q], ez = dampz[q];
+            double lam = vlam[q] * weight, mu = vmu[q] * weight;
+            double complex clam = lam, cmu = mu;
+            double complex c11 = clam + cmu + cmu, c12 = clam, c13 = clam, c23 = clam;
+            double complex c33 = clam + cmu + cmu, c44 = cmu, c55 = cmu, c66 = cmu;
+            double rhow = rho[q] * weight;
+            double cm1 = -w2 * (rhow - py2 * mu), cm2 = -w2 * (rhow - py2 * (lam + 2.0 * mu)), cm3 = -w2 * (rhow - py2 * mu);
+            const double *sq = shg + 3 * nen * q;
+            for (int ib = 0; ib < nen; ib++) {
+                double phib = sq[3 * ib + 2];
+                double complex phibx = sq[3 * ib] * ex, phibz = sq[3 * ib + 1] * ez;
+                double r1 = cm1 * phib, r2 = cm2 * phib, r3 = cm3 * phib;
+                double complex k11t1 = c11 * phibx, k11t2 = c55 * phibz, k13t1 = c13 * phibz, k13t2 = c55 * phibx;
+                double complex k22t1 = c66 * phibx, k22t2 = c44 * phibz, k31t1 = c13 * phibx, k31t2 = c55 * phibz;
+                double complex k33t1 = c33 * phibz, k33t2 = c55 * phibx;
+                double complex k12t1 = cwpy * c12 * phib, k12t2 = cwpy * c66 * phibx;
+                double complex k21t1 = cwpy * c12 * phibx, k21t2 = cwpy * c66 * phib;
+                double complex k23t1 = cwpy * c23 * phibz, k23t2 = cwpy * c44 * phib;
+                double complex k32t1 = cwpy * c23 * phib, k32t2 = cwpy * c44 * phibz;
+                for (int ia = 0; ia < nen; ia++) {
+                    double phia = sq[3 * ia + 2];
+                    double complex phiax = sq[3 * ia] * ex, phiaz = sq[3 * ia + 1] * ez;
+                    int ip = NDIM * ia, iq = NDIM * ib;
+                    me[ip + nee * iq] += phia * r1;
+                    me[ip + 1 + nee * (iq + 1)] += phia * r2;
+                    me[ip + 2 + nee * (iq + 2)] += phia * r3;
+                    double complex ce11 = phiax * k11t1 + phiaz * k11t2;
+                    double complex ce13 = phiax * k13t1 + phiaz * k13t2;
+                    double complex ce22 = phiax * k22t1 + phiaz * k22t2;
+                    double complex ce31 = phiaz * k31t1 + phiax * k31t2;
+                    double complex ce33 = phiaz * k33t1 + phiax * k33t2;
+                    double complex ce12 = phiax * k12t1 - phia * k12t2;
+                    double complex ce21 = phia * k21t1 - phiax * k21t2;
+                    double complex ce23 = phia * k23t1 - phiaz * k23t2;
+                    double complex ce32 = phiaz * k32t1 - phia * k32t2;
+                    ce[ip + nee * iq] += ce11;
+                    ce[ip + nee * (iq + 1)] -= ce12;
+                    ce[ip + nee * (iq + 2)] += ce13;
+                    ce[ip + 1 + nee * iq] += ce21;
+                    ce[ip + 1 + nee * (iq + 1)] += ce22;
+                    ce[ip + 1 + nee * (iq + 2)] += ce23;
+                    ce[ip + 2 + nee * iq] += ce31;
+                    ce[ip + 2 + nee * (iq + 1)] -= ce32;
+                    ce[ip + 2 + nee * (iq + 2)] += ce33;
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* graph25.f:509-698 GENIEN.  Side nodes are deduplicated by a linear scan
+ * (graph25.f:566-575), heap-sorted by x, z-sorted within groups of x that
+ * agree to TOL (graph25.f:582-617), and numbered 1..NNPS in that order;
+ * interior nodes follow in element order (graph25.f:672-675). */
+typedef struct { double x, z; } xz_t;
+static int cmp_x(const void *a, const void *b)
+{
+    double x = ((const xz_t *)a)->x, y = ((const xz_t *)b)->x;
+    return (x > y) - (x < y);
+}
+static int cmp_z(const void *a, const void *b)
+{
+    double x = ((const xz_t *)a)->z, y = ((const xz_t *)b)->z;
+    return (x > y) - (x < y);
+}
+
+static void node_xy(const orc_mesh *m, int e, double xi, double eta, double *x, double *z)
+{
+    double sf[4];
+    csf2dnd(xi, eta, sf);
+    double xx = 0, zz = 0;
+    for (int k = 0; k < 4; k++) {
+        int g = m->ieng[4 * e + k] - 1;
+        xx += sf[k] * m->xlocs[g];
+        zz += sf[k] * m->zlocs[g];
+    }
+    *x = xx; *z = zz;
+}
+
+/* sort.f:1046-1104 IBSECT8 with JOB=1, 1-based vec */
+static int ibsect8(int n, double tol, double x, const double *vec1)
+{
+    const double *v = vec1 - 1;
+    int r = -1, jl, ju, jm = 0;
+    if (fabs(x - v[1]) < tol) return 1;
+    if (fabs(x - v[n]) < tol) { r = n; goto back; }
+    jl = 0; ju = n + 1;
+    while (ju - jl > 1) {
+        jm = (ju + jl) / 2;
+        if ((v[n] >= v[1]) == (x >= v[jm])) jl = jm; else ju = jm;
+    }
+    if (jl <= 0) return -1;
+    if (fabs(v[jl] - x) < tol) r = jl;
+    else if (fabs(v[jm] - x) < tol) r = jm;
+    else if (ju <= n && fabs(v[ju] - x) < tol) r = ju;
+    else r = -1;
+back:
+    for (int i = r; i >= 2; i--)
+        if (fabs(v[i] - v[i - 1]) > tol) return i;
+    return r;
+}
+
+int orc_genien(const orc_mesh *m, const double *xipts, int *ien)
+{
+    const double tol = 1e-8;
+    int nl = m->nord + 1, nen = nl * nl;
+    int mnodes = m->nelem * (2 * (nl - 1) + 2 * nl);
+    xz_t *nodes = (xz_t *)malloc(sizeof(xz_t) * (mnodes + 1));
+    int inodes = 0;
+    for (int e = 0; e < m->nelem; e++)
+        for (int ie = 0; ie < nl; ie++)
+            for (int ix = 0; ix < nl; ix++) {
+                if (!(ix == 0 || ix == nl - 1 || ie == 0 || ie == nl - 1)) continue;
+                double x, z;
+                node_xy(m, e, xipts[ix], xipts[ie], &x, &z);
+                int found = 0;
+                for (int j = 0; j < inodes; j++)
+                    if (fabs(nodes[j].x - x) < tol && fabs(nodes[j].z - z) < tol) { found = 1; break; }
+                if (!found) { nodes[inodes].x = x; nodes[inodes].z = z; inodes++; }
+            }
+    int nn = inodes;
+    qsort(nodes, nn, sizeof(xz_t), cmp_x);
+    /* graph25.f:590-617, transcribed with 1-based indices */
+    int jbeg = 1;
+    for (int i = 1; i <= nn; i++) {
+        if (i + 1 == nn) {
+            if (jbeg != nn) qsort(nodes + jbeg - 1, nn - jbeg + 1, sizeof(xz_t), cmp_z);
+            break;
+        }
+        if (i + 1 <= nn && fabs(nodes[i].x - nodes[i - 1].x) > tol) {
+            if (i - jbeg + 1 > 1) qsort(nodes + jbeg - 1, i - jbeg + 1, sizeof(xz_t), cmp_z);
+            jbeg = i + 1;
+        }
+    }
+    int nnps = nn;
+    double *xs = (double *)malloc(sizeof(double) * (nnps + 1));
+    for (int i = 0; i < nnps; i++) xs[i] = nodes[i].x;
+    xs[nnps] = HUGE_VAL;
+    int iint = nnps;
+    for (int e = 0; e < m->nelem; e++)
+        for (int ie = 0; ie < nl; ie++)
+            for (int ix = 0; ix < nl; ix++) {
+                int a = ie * nl + ix;
+                if (ix == 0 || ix == nl - 1 || ie == 0 || ie == nl - 1) {
+                    double x, z;
+                    node_xy(m, e, xipts[ix], xipts[ie], &x, &z);
+                    int j0 = ibsect8(nnps + 1, tol, x, xs);
+                    if (j0 < 0) { free(nodes); free(xs); return -3; }
+                    int num = -1;
+                    for (int j = j0; j <= nnps; j++)
+                        if (fabs(nodes[j - 1].x - x) < tol && fabs(nodes[j - 1].z - z) < tol) { num = j; break; }
+                    if (num < 0) { free(nodes); free(xs); return -4; }
+                    ien[a + nen * e] = num;
+                } else {
+                    ien[a + nen * e] = ++iint;
+                }
+            }
+    free(nodes); free(xs);
+    return iint;
+}
+
+/* graph25.f:315-505 GENID */
+int orc_genid(const orc_mesh *m, const double *xipts, const int *ien, int nnp, int *id)
+{
+    const double tol = 1e-10;
+    const int lnull = -5;
+    int nl = m->nord + 1, nen = nl * nl;
+    for (int i = 0; i < NDIM * nnp; i++) id[i] = lnull;
+#define FIXED(g) (m->cnnpg[2 * (g)] == 'F' && (m->cnnpg[2 * (g) + 1] == 'L' || m->cnnpg[2 * (g) + 1] == 'B' || m->cnnpg[2 * (g) + 1] == 'R'))
+    for (int e = 0; e < m->nelem; e++) {
+        if (m->cdomain[e] != 'A') continue;
+        int icheck = 0, iav[3] = {0, 0, 0};
+        for (int k = 0; k < 4; k++) {
+            int iloc = m->ieng[4 * e + k];
+            if (FIXED(iloc - 1)) {
+                icheck++;
+                iav[icheck - 1] = iloc;
+                for (int j = 0; j < 4; j++) {
+                    int jloc = m->ieng[4 * e + j];
+                    if (FIXED(jloc - 1) && iloc != jloc) {
+                        icheck++;
+                        iav[icheck - 1] = jloc;
+                        if (icheck == 3) break;
+                    }
+                }
+                break;
+            }
+        }
+        if (icheck == 0) continue;
+        int ia1 = iav[0], ia2;
+        double v1[3] = {0, 0, 0};
+        for (int loop = 1; loop <= 2; loop++) {
+            if (icheck > 1) {
+                ia1 = iav[loop - 1];
+                ia2 = (loop < 3) ? iav[loop] : 0;
+                if (ia2 == 0) break;
+                v1[0] = m->xlocs[ia2 - 1] - m->xlocs[ia1 - 1];
+                v1[2] = m->zlocs[ia2 - 1] - m->zlocs[ia1 - 1];
+            } else if (loop == 2) break;
+            for (int ie = 0; ie < nl; ie++)
+                for (int ix = 0; ix < nl; ix++) {
+                    if (!(ix == 0 || ix == nl - 1 || ie == 0 || ie == nl - 1)) continue;
+                    int inp = ien[ie * nl + ix + nen * e] - 1;
+                    double x, z;
+                    node_xy(m, e, xipts[ix], xipts[ie], &x, &z);
+                    if (icheck == 1) {
+                        if (fabs(x - m->xlocs[ia1 - 1]) < tol && fabs(z - m->zlocs[ia1 - 1]) < tol)
+                            for (int i = 0; i < NDIM; i++) id[i + NDIM * inp] = 0;
+                    } else {
+                        double v2x = x - m->xlocs[ia1 - 1], v2z = z - m->zlocs[ia1 - 1];
+                        double det = v1[0] * v2z - v1[2] * v2x;
+                        if (fabs(det) < tol)
+                            for (int i = 0; i < NDIM; i++) id[i + NDIM * inp] = 0;
+                    }
+                }
+        }
+    }
+#undef FIXED
+    int idof = 0;
+    for (int e = 0; e < m->nelem; e++)
+        for (int a = 0; a < nen; a++) {
+            int inp = ien[a + nen * e] - 1;
+            for (int i = 0; i < NDIM; i++)
+                if (id[i + NDIM * inp] == lnull) id[i + NDIM * inp] = ++idof;
+        }
+    for (int i = 0; i < NDIM * nnp; i++)
+        if (id[i] == lnull) return -1;
+    return idof;
+}
+
+void orc_genlm(int nen, int nelem, const int *id, const int *ien, int *lm)
+{
+    for (int e = 0; e < nelem; e++)
+        for (int a = 0; a < nen; a++)
+            for (int i = 0; i < NDIM; i++)
+                lm[i + NDIM * (a + nen * e)] = id[i + NDIM * (ien[a + nen * e] - 1)];
+}
+
+int orc_icmcon(int ngnod, int nnp, int nelem, const int *ien)
+{
+    int *cnt = (int *)calloc(nnp, sizeof(int));
+    for (int e = 0; e < nelem; e++)
+        for (int k = 0; k < ngnod; k++) cnt[ien[k + ngnod * e] - 1]++;
+    int mx = 0;
+    for (int i = 0; i < nnp; i++) if (cnt[i] > mx) mx = cnt[i];
+    free(cnt);
+    return mx;
+}
+
+void orc_connect(int ngnod, int nnp, int nelem, int ncon, const int *ien, int *mconn)
+{
+    memset(mconn, 0, sizeof(int) * (size_t)nnp * ncon);
+    for (int e = 0; e < nelem; e++)
+        for (int k = 0; k < ngnod; k++) {
+            int inp = ien[k + ngnod * e] - 1;
+            for (int c = 0; c < ncon; c++)
+                if (mconn[inp + (size_t)nnp * c] == 0) { mconn[inp + (size_t)nnp * c] = e + 1; break; }
+        }
+}
+
+static int cmp_int(const void *a, const void *b)
+{
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+/* graph25.f:136-266 GENCRS_QUICK (rows built in first-encounter order, then
+ * permuted to DOF order) */
+long orc_gencrs(int nen, int nnp, int nelem, int ndof, int ncon, const int *mconn,
+                const int *ien, const int *lm, int *irptr, int *jcptr, long cap)
+{
+    int nwork = NDIM * ncon * nen + 1;
+    int *work = (int *)malloc(sizeof(int) * (nwork + 1));
+    int *perm = (int *)calloc(ndof + 1, sizeof(int));
+    long *rw = (long *)malloc(sizeof(long) * (ndof + 2));
+    int *jw = (int *)malloc(sizeof(int) * cap);
+    long izero = 0;
+    int ifound = 0;
+    rw[1] = 1;
+    for (int e = 0; e < nelem; e++)
+        for (int a = 0; a < nen; a++) {
+            int inp = ien[a + nen * e] - 1;
+            for (int i = 0; i < NDIM; i++) {
+                int idof = lm[i + NDIM * (a + nen * e)];
+                if (idof <= 0 || perm[idof]) continue;
+                int nrr = 0;
+                for (int c = 0; c < ncon; c++) {
+                    int je = mconn[inp + (size_t)nnp * c];
+                    if (je == 0) break;
+                    for (int b = 0; b < nen; b++)
+                        for (int j = 0; j < NDIM; j++) {
+                            int jd = lm[j + NDIM * (b + nen * (je - 1))];
+                            if (jd > 0) work[nrr++] = jd;
+                        }
+                }
+                qsort(work, nrr, sizeof(int), cmp_int);
+                work[nrr] = 0;
+                for (int r = 0; r < nrr; r++)
+                    if (work[r] != work[r + 1]) {
+                        if (izero >= cap) { free(work); free(perm); free(rw); free(jw); return -2; }
+                        jw[izero++] = work[r];
+                    }
+                ifound++;
+                perm[idof] = ifound;
+                rw[ifound + 1] = izero + 1;
+            }
+        }
+    long iz = 0;
+    irptr[0] = 1;
+    for (int idof = 1; idof <= ndof; idof++) {
+        int irow = perm[idof];
+        for (long j = rw[irow]; j < rw[irow + 1]; j++) jcptr[iz++] = jw[j - 1];
+        irptr[idof] = (int)(iz + 1);
+    }
+    free(work); free(perm); free(rw); free(jw);
+    return iz;
+}
+
+/* bielak_utils.f:934-1025 CNMESH */
+int orc_cnmesh(int nen, int ndof, int nelem, const char *cdomain, const int *lm, char *cnp)
+{
+    int ierr = 0;
+    memset(cnp, ' ', ndof);
+    for (int e = 0; e < nelem; e++) if (cdomain[e] == 'A')
+        for (int a = 0; a < nen * NDIM; a++) { int d = lm[a + NDIM * nen * e]; if (d > 0) cnp[d - 1] = 'A'; }
+    for (int e = 0; e < nelem; e++) if (cdomain[e] == 'E')
+        for (int a = 0; a < nen * NDIM; a++) { int d = lm[a + NDIM * nen * e]; if (d > 0 && cnp[d - 1] == ' ') cnp[d - 1] = 'E'; }
+    for (int e = 0; e < nelem; e++) if (cdomain[e] == 'I')
+        for (int a = 0; a < nen * NDIM; a++) {
+            int d = lm[a + NDIM * nen * e];
+            if (d <= 0) continue;
+            if (cnp[d - 1] == ' ') cnp[d - 1] = 'I';
+            else if (cnp[d - 1] == 'E') cnp[d - 1] = 'B';
+            else if (cnp[d - 1] == 'A') ierr = 1;
+        }
+    for (int d = 0; d < ndof; d++) if (cnp[d] == ' ') ierr = 2;
+    return ierr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* sort.f:1006-1040 IBSECT on a sorted, 1-based column list (returns 1-based) */
+static int ibsect(int n, int ix, const int *v)
+{
+    int lo = 0, hi = n - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        if (v[mid] == ix) return mid + 1;
+        if (v[mid] < ix) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+typedef struct {
+    int nl, nen, nee, nint;
+    double *xipts, *xigll, *wgll, *shl, *shg, *det, *rho, *dmat, *me;
+    double complex *ke, *dampx, *dampz;
+} elem_ws;
+
+static void ws_init(elem_ws *w, int nord)
+{
+    w->nl = nord + 1; w->nen = w->nl * w->nl; w->nee = NDIM * w->nen;
+    w->nint = 2 * nord + 5;   /* asmble.f90:110-112 */
+    int q = w->nint * w->nint;
+    w->xipts = (double *)malloc(sizeof(double) * w->nl);
+    w->xigll = (double *)malloc(sizeof(double) * w->nint);
+    w->wgll = (double *)malloc(sizeof(double) * w->nint);
+    w->shl = (double *)malloc(sizeof(double) * 3 * w->nen * q);
+    w->shg = (double *)malloc(sizeof(double) * 3 * w->nen * q);
+    w->det = (double *)malloc(sizeof(double) * q);
+    w->rho = (double *)malloc(sizeof(double) * q);
+    w->dmat = (double *)malloc(sizeof(double) * 2 * q);
+    w->me = (double *)malloc(sizeof(double) * w->nee * w->nee);
+    w->ke = (double complex *)malloc(sizeof(double complex) * w->nee * w->nee);
+    w->dampx = (double complex *)malloc(sizeof(double complex) * q);
+    w->dampz = (double complex *)malloc(sizeof(double complex) * q);
+    orc_dshl(w->nl, w->nint, w->xipts, w->xigll, w->wgll, w->shl);
+}
+
+static void ws_free(elem_ws *w)
+{
+    free(w->xipts); free(w->xigll); free(w->wgll); free(w->shl); free(w->shg);
+    free(w->det); free(w->rho); free(w->dmat); free(w->me); free(w->ke);
+    free(w->dampx); free(w->dampz);
+}
+
+/* asmble.f90:140-212: element dispatch, 'A' -> CJACCQ+CM25ISOQ, else CJMK_TVI+KM25ISOQ */
+static int element_matrix(const orc_asm *a, elem_ws *w, int e, double omega, double py)
+{
+    const int *ie = a->ieng + 4 * e;
+    int q = w->nint * w->nint;
+    if (a->cdomain[e] != 'A') {
+        if (orc_cjmk_tvi(w->nen, w->nint, ie, w->xigll, a->xlocs, a->zlocs, a->dens, a->ecoeff,
+                         a->nnpg, w->shl, w->det, w->rho, w->dmat, w->shg)) return 1;
+        orc_km25isoq(w->nen, w->nint, omega, py, w->wgll, w->dmat, w->dmat + q, w->rho, w->det,
+                     w->shg, w->me, w->ke);
+    } else {
+        if (orc_cjaccq(w->nen, w->nint, omega, a->xwidth, a->zwidth, a->rcoeff, a->dmax, a->kappa,
+                       a->fcent, ie, w->xigll, a->xd, a->zd, a->xlocs, a->zlocs, a->dens, a->ecoeff,
+                       a->nnpg, w->shl, w->det, w->rho, w->dmat, w->shg, w->dampx, w->dampz)) return 1;
+        orc_cm25isoq(w->nen, w->nint, omega, py, w->wgll, w->dmat, w->dmat + q, w->rho, w->det,
+                     w->shg, w->dampx, w->dampz, w->me, w->ke);
+    }
+    return 0;
+}
+
+static int asmble_common(const orc_asm *a, double freq, double py, float complex *cdist,
+                         double complex *sz)
+{
+    const double twopi = 6.2831853071795862;
+    double omega = twopi * freq;
+    elem_ws w;
+    ws_init(&w, a->nord);
+    long nz = a->irptr[a->ndof] - 1;
+    if (cdist) for (long i = 0; i < nz; i++) cdist[i] = 0.0f;
+    if (sz) for (long i = 0; i < nz; i++) sz[i] = 0.0;
+    int ierr = 0;
+    for (int e = 0; e < a->nelem && !ierr; e++) {
+        if (element_matrix(a, &w, e, omega, py)) { ierr = 1; break; }
+        const int *lm = a->lm + NDIM * w.nen * e;
+        /* asmble.f90:279-312 ASMBLE_CDIST */
+        for (int ip = 0; ip < w.nee; ip++) {
+            int idof = lm[ip];
+            if (idof == 0) continue;
+            int jbeg = a->irptr[idof - 1], jend = a->irptr[idof] - 1;
+            int nvar = jend - jbeg + 1;
+            for (int iq = 0; iq < w.nee; iq++) {
+                int jdof = lm[iq];
+                if (jdof == 0) continue;
+                int indx = ibsect(nvar, jdof, a->jcptr + jbeg - 1);
+                if (indx <= 0) { ierr = 1; break; }
+                long iz = jbeg - 1 + indx - 1;
+                double r1 = w.me[ip + w.nee * iq] + creal(w.ke[ip + w.nee * iq]);
+                double q1 = cimag(w.ke[ip + w.nee * iq]);
+                if (cdist) {
+                    float cr = (float)r1, ci = (float)q1;
+                    float ar = crealf(cdist[iz]) + cr, ai = cimagf(cdist[iz]) + ci;
+                    cdist[iz] = ar + I * ai;
+                }
+                if (sz) sz[iz] += r1 + I * q1;
+            }
+        }
+    }
+    ws_free(&w);
+    return ierr;
+}
+
+int orc_asmble_cdist(const orc_asm *a, double freq, double py, float complex *cdist)
+{
+    return asmble_common(a, freq, py, cdist, NULL);
+}
+
+int orc_asmble_z(const orc_asm *a, double freq, double py, double complex *s)
+{
+    return asmble_common(a, freq, py, NULL, s);
+}
+
+void orc_spmv_z(int ndof, const int *irptr, const int *jcptr, const double complex *s,
+                const double complex *x, double complex *y)
+{
+    for (int i = 0; i < ndof; i++) {
+        double complex acc = 0.0;
+        for (int j = irptr[i] - 1; j < irptr[i + 1] - 1; j++) acc += s[j] * x[jcptr[j] - 1];
+        y[i] = acc;
+    }
+}
+
+void orc_spmv_c(int ndof, const int *irptr, const int *jcptr, const float complex *s,
+                const float complex *x, float complex *y)
+{
+    for (int i = 0; i < ndof; i++) {
+        float ar = 0.f, ai = 0.f;
+        for (int j = irptr[i] - 1; j < irptr[i + 1] - 1; j++) {
+            float sr = crealf(s[j]), si = cimagf(s[j]);
+            float xr = crealf(x[jcptr[j] - 1]), xi = cimagf(x[jcptr[j] - 1]);
+            ar += sr * xr - si * xi;
+            ai += sr * xi + si * xr;
+        }
+        y[i] = ar + I * ai;
+    }
+}
+
+/* asmble.f90:318-377 ASMB_PEFF without the reduce (one rank owns every row) */
+void orc_asmb_peff(int ndof, const char *cnp, const int *irptr, const int *jcptr,
+                   const float complex *cdist, const float complex *ue, float complex *peff)
+{
+    for (int i = 0; i < ndof; i++) peff[i] = 0.0f;
+    for (int i = 0; i < ndof; i++) {
+        char c1 = cnp[i];
+        if (c1 != 'B' && c1 != 'E') continue;
+        float br = crealf(peff[i]), bi = cimagf(peff[i]);
+        for (int j = irptr[i] - 1; j < irptr[i + 1] - 1; j++) {
+            int jd = jcptr[j] - 1;
+            char c2 = cnp[jd];
+            float sr = crealf(cdist[j]), si = cimagf(cdist[j]);
+            float ur = crealf(ue[jd]), ui = cimagf(ue[jd]);
+            float pr = sr * ur - si * ui, pi = sr * ui + si * ur;
+            if (c1 == 'B' && c2 == 'E') { br -= pr; bi -= pi; }
+            else if (c1 == 'E' && c2 == 'B') { br += pr; bi += pi; }
+        }
+        peff[i] = br + I * bi;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* djac.f:1-139 DJACAB */
+int orc_djacab(int nen, int nint, int inpg, const int *ieng_e, const double *xigll,
+               const double *xlocs, const double *zlocs, const double *dens,
+               const double *ecoeff, int nnpg, const double *shl,
+               double *det, double *dab, double *shg)
+{
+    for (int ix = 0; ix < nint; ix++)
+        for (int iz = 0; iz < nint; iz++) {
+            double sf[4], rjac, a, b, c, d;
+            int err = jacobian_at(xigll[ix], xigll[iz], ieng_e, xlocs, zlocs, sf, &rjac, &a, &b, &c, &d);
+            int q = ix + nint * iz;
+            double d1 = 0, d2 = 0;
+            for (int k = 0; k < 4; k++) {
+                int g = ieng_e[k];
+                double al = 0, be = 0;
+                if (g == inpg) {
+                    al = sqrt((ecoeff[g - 1] + 2.0 * ecoeff[g - 1 + nnpg]) / dens[g - 1]);
+                    be = sqrt(ecoeff[g - 1 + nnpg] / dens[g - 1]);
+                }
+                d1 += 2.0 * al * sf[k];
+                d2 += 2.0 * be * sf[k];
+            }
+            dab[q] = d1; dab[q + nint * nint] = d2;
+            det[q] = rjac;
+            if (err) return 1;
+            fill_shg(nen, shl + 3 * nen * q, shg + 3 * nen * q, rjac, a, b, c, d);
+        }
+    return 0;
+}
+
+/* dstiff25.f:498-662 DKM_VPVS */
+void orc_dkm_vpvs(int nen, int nint, double omega, double py, const double *wts,
+                  const double *dvp, const double *dvs, const double *det, const double *shg,
+                  double *dme, double complex *dke)
+{
+    int nee = NDIM * nen;
+    for (int i = 0; i < nee * nee; i++) { dme[i] = 0.0; dke[i] = 0.0; }
+    double wpy = omega * py, py2 = py * py, w2 = omega * omega;
+    for (int ix = 0; ix < nint; ix++)
+        for (int iz = 0; iz < nint; iz++) {
+            int q = ix + nint * iz;
+            double weight = wts[ix] * wts[iz] * det[q];
+            double drvp = dvp[q] * weight, drvs = dvs[q] * weight;
+            double rm1 = w2 * py2 * drvs, rm2 = w2 * py2 * drvp;
+            const double *sq = shg + 3 * nen * q;
+            for (int ib = 0; ib < nen; ib++) {
+                double phib = sq[3 * ib + 2], phibx = sq[3 * ib], phibz = sq[3 * ib + 1];
+                double rm11t1 = rm1 * phib, rm22t1 = rm2 * phib;
+                double rk11t1 = drvp * phibx, rk11t2 = drvs * phibz;
+                double rk13t1 = drvp * phibz, rk13t2 = drvs * phibx, rk13t3 = -2.0 * drvs * phibz;
+                double rk22t1 = drvs * phibx, rk22t2 = drvs * phibz;
+                double rk31t1 = drvp * phibx, rk31t2 = drvs * phibz, rk31t3 = -2.0 * drvs * phibx;
+                double rk33t1 = drvp * phibz, rk33t2 = drvs * phibx;
+                double rk12t1 = wpy * drvp * phib, rk12t2 = -2.0 * wpy * drvs * phib, rk12t3 = -wpy * drvs * phibx;
+                double rk21t1 = wpy * drvp * phibx, rk21t2 = -2.0 * wpy * drvs * phibx, rk21t3 = -wpy * drvs * phib;
+                double rk23t1 = wpy * drvp * phibz, rk23t2 = -2.0 * wpy * drvs * phibz, rk23t3 = -wpy * drvs * phib;
+                double rk32t1 = wpy * drvp * phib, rk32t2 = -2.0 * wpy * drvs * phib, rk32t3 = -wpy * drvs * phibz;
+                for (int ia = 0; ia < nen; ia++) {
+                    double phia = sq[3 * ia + 2], phiax = sq[3 * ia], phiaz = sq[3 * ia + 1];
+                    double rme11 = rm11t1 * phia, rme22 = rm22t1 * phia;
+                    double rke11 = rk11t1 * phiax + rk11t2 * phiaz;
+                    double rke13 = rk13t1 * phiax + rk13t2 * phiaz + rk13t3 * phiax;
+                    double rke22 = rk22t1 * phiax + rk22t2 * phiaz;
+                    double rke31 = rk31t1 * phiaz + rk31t2 * phiax + rk31t3 * phiaz;
+                    double rke33 = rk33t1 * phiaz + rk33t2 * phiax;
+                    double rke12 = rk12t1 * phiax + rk12t2 * phiax + rk12t3 * phia;
+                    double rke21 = rk21t1 * phia + rk21t2 * phia + rk21t3 * phiax;
+                    double rke23 = rk23t1 * phia + rk23t2 * phia + rk23t3 * phiaz;
+                    double rke32 = rk32t1 * phiaz + rk32t2 * phiaz + rk32t3 * phia;
+                    int ip = NDIM * ia, iq = NDIM * ib;
+                    dme[ip + nee * iq] += rme11;
+                    dme[ip + 1 + nee * (iq + 1)] += rme22;
+                    dme[ip + 2 + nee * (iq + 2)] += rme11;
+                    dke[ip + nee * iq] += rke11;
+                    dke[ip + 1 + nee * (iq + 1)] += rke22;
+                    dke[ip + 2 + nee * (iq + 2)] += rke33;
+                    dke[ip + nee * (iq + 2)] += rke13;
+                    dke[ip + 2 + nee * iq] += rke31;
+                    dke[ip + nee * (iq + 1)] -= rke12;
+                    dke[ip + 1 + nee * iq] += rke21;
+                    dke[ip + 1 + nee * (iq + 2)] += rke23;
+                    dke[ip + 2 + nee * (iq + 1)] -= rke32;
+                }
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+static int cnnpg_is(const char *cnnpg, int g, const char *s)
+{
+    return cnnpg[2 * g] == s[0] && cnnpg[2 * g + 1] == s[1];
+}
+
+/* gengmask.f90:1-262 GENGMASK (no .mask file: every weight is 1; one partition) */
+int orc_gengmask(const orc_asm *a, const char *cnnpg, int nvinv, int linrec, int lkbdry,
+                 int nrec, const int *mrdof1, orc_inv *inv)
+{
+    int nl = a->nord + 1, nen = nl * nl;
+    memset(inv, 0, sizeof(*inv));
+    inv->nvinv = nvinv;
+    inv->ncon = orc_icmcon(4, a->nnpg, a->nelem, a->ieng);
+    inv->mconn = (int *)malloc(sizeof(int) * (size_t)a->nnpg * inv->ncon);
+    orc_connect(4, a->nnpg, a->nelem, inv->ncon, a->ieng, inv->mconn);
+    inv->maskg = (int *)calloc(a->nnpg, sizeof(int));
+    int inpinv = 0, lshift = 0;
+    for (int g = 0; g < a->nnpg; g++) {
+        int keep = 1, e = inv->mconn[g];
+        if (e > 0 && a->cdomain[e - 1] != 'I') keep = 0;
+        if (keep && (cnnpg_is(cnnpg, g, "II") || cnnpg_is(cnnpg, g, "BI") || cnnpg_is(cnnpg, g, "FS")))
+            inv->maskg[g] = ++inpinv;
+    }
+    inv->nnpinv = inpinv;
+    int ia2iae[4] = {1, nl, nl * nl, nl * (nl - 1) + 1};
+    for (int e = 0; e < a->nelem; e++)
+        for (int k = 0; k < 4; k++) {
+            int g = a->ieng[4 * e + k] - 1;
+            if (inv->maskg[g] <= 0) continue;
+            int iae = ia2iae[k] - 1;
+            int drop = 0;
+            for (int i = 0; i < NDIM; i++)
+                if (a->lm[i + NDIM * (iae + nen * e)] <= 0) drop = 1;
+            if (!drop && !linrec)
+                for (int r = 0; r < nrec; r++)
+                    if (a->lm[NDIM * (iae + nen * e)] == mrdof1[r]) { drop = 1; break; }
+            if (drop) { inv->nnpinv--; inv->maskg[g] = 0; lshift = 1; }
+        }
+    if (!lkbdry)
+        for (int e = 0; e < a->nelem; e++) {
+            int bdr = 0;
+            for (int k = 0; k < 4; k++) {
+                int g = a->ieng[4 * e + k] - 1;
+                if (!cnnpg_is(cnnpg, g, "II") && !cnnpg_is(cnnpg, g, "FS")) bdr = 1;
+            }
+            if (bdr)
+                for (int k = 0; k < 4; k++) {
+                    int g = a->ieng[4 * e + k] - 1;
+                    if (inv->maskg[g] > 0) { inv->maskg[g] = 0; inv->nnpinv--; lshift = 1; }
+                }
+        }
+    if (inv->nnpinv < 1) return 1;
+    if (lshift) {
+        inpinv = 0;
+        for (int g = 0; g < a->nnpg; g++) if (inv->maskg[g] > 0) inv->maskg[g] = ++inpinv;
+    }
+    inv->na35 = inv->nnpinv * nvinv;
+    return 0;
+}
+
+/* gengmask.f90:962-1198 GRADPTRS (MYID = 0, one partition) */
+int orc_gradptrs(const orc_asm *a, orc_inv *inv)
+{
+    int nl = a->nord + 1, nen = nl * nl, nee = NDIM * nen;
+    inv->nnpgl = inv->nnpinv * inv->nvinv;
+    long nwork = (long)a->nnpg * inv->ncon * nee;
+    int *irow = (int *)malloc(sizeof(int) * (inv->ncon * nee + 1));
+    inv->jcsc = (int *)malloc(sizeof(int) * (inv->nnpgl + 1));
+    inv->icsc = (int *)malloc(sizeof(int) * nwork);
+    inv->mygrad = (int *)malloc(sizeof(int) * inv->nnpgl);
+    int inzfl = 0, inpgl = 0, combig = 0;
+    inv->jcsc[0] = 1;
+    for (int g = 0; g < a->nnpg; g++) {
+        int inpinv = inv->maskg[g];
+        if (inpinv <= 0) continue;
+        int nn = 0;
+        for (int c = 0; c < inv->ncon; c++) {
+            int e = inv->mconn[g + (size_t)a->nnpg * c];
+            if (e <= 0) break;
+            for (int ae = 0; ae < nen; ae++)
+                for (int i = 0; i < NDIM; i++) {
+                    int d = a->lm[i + NDIM * (ae + nen * (e - 1))];
+                    if (d > 0) irow[nn++] = d;
+                }
+        }
+        qsort(irow, nn, sizeof(int), cmp_int);
+        irow[nn] = 0;
+        for (int r = 0; r < nn; r++)
+            if (irow[r + 1] != irow[r]) inv->icsc[inzfl++] = irow[r];
+        inv->mygrad[inpgl] = (inpinv - 1) * inv->nvinv + 1;
+        inv->jcsc[inpgl + 1] = inzfl + 1;
+        int r1 = inv->jcsc[inpgl], r2 = inv->jcsc[inpgl + 1] - 1;
+        if (r2 - r1 + 1 > combig) combig = r2 - r1 + 1;
+        for (int iv = 2; iv <= inv->nvinv; iv++) {
+            for (int r = r1; r <= r2; r++) inv->icsc[inzfl++] = inv->icsc[r - 1];
+            inpgl++;
+            inv->mygrad[inpgl] = (inpinv - 1) * inv->nvinv + iv;
+            inv->jcsc[inpgl + 1] = inzfl + 1;
+        }
+        inpgl++;
+    }
+    inv->nz_fdist = inzfl;
+    inv->mbufrhs = combig;
+    free(irow);
+    return (inpgl == inv->nnpgl) ? 0 : 1;
+}
+
+/* gengmask.f90:758-841 ELEM_WTS */
+int orc_elem_wts(const orc_asm *a, orc_inv *inv)
+{
+    inv->elem_wts = (double *)calloc(a->nelem, sizeof(double));
+    char *init = (char *)calloc(a->nelem, 1);
+    double wtmax = 0.0;
+    for (int g = 0; g < a->nnpg; g++) {
+        if (inv->maskg[g] <= 0) continue;
+        for (int c = 0; c < inv->ncon; c++) {
+            int e = inv->mconn[g + (size_t)a->nnpg * c];
+            if (e <= 0) break;
+            e--;
+            if (init[e]) continue;
+            double rloc = 0.0;
+            for (int k = 0; k < 4; k++) {
+                int j1 = a->ieng[4 * e + k] - 1, j2 = a->ieng[4 * e + (k + 1) % 4] - 1;
+                rloc += a->xlocs[j1] * a->zlocs[j2] - a->xlocs[j2] * a->zlocs[j1];
+            }
+            if (rloc <= 0.0) rloc = -rloc;
+            rloc *= 0.5;
+            inv->elem_wts[e] = rloc;
+            init[e] = 1;
+            if (rloc > wtmax) wtmax = rloc;
+        }
+    }
+    for (int e = 0; e < a->nelem; e++)
+        if (inv->elem_wts[e] > 0.0) inv->elem_wts[e] = wtmax / inv->elem_wts[e];
+    free(init);
+    return wtmax > 0.0 ? 0 : 1;
+}
+
+void orc_inv_free(orc_inv *inv)
+{
+    free(inv->mconn); free(inv->maskg); free(inv->mygrad); free(inv->jcsc); free(inv->icsc);
+    free(inv->elem_wts);
+    memset(inv, 0, sizeof(*inv));
+}
+
+static inline float complex cmulf_ref(float complex a, float complex b)
+{
+    float ar = crealf(a), ai = cimagf(a), br = crealf(b), bi = cimagf(b);
+    return (ar * br - ai * bi) + I * (ar * bi + ai * br);
+}
+
+/* hessloc.f90:1-133 CFMAT_DIST driving vforce25.f:1-369 VFORCE25 (NVINV = 1).
+ * VFORCE25 clears FORCEV inside its loop over the node's elements
+ * (vforce25.f:252-256 sit inside DO 200), so each column holds the product of
+ * the last connected element only; that is restated as it is. */
+int orc_cfmat_dist(const orc_asm *a, const orc_inv *inv, const char *cinvtype, double vpvs,
+                   double freq, double py, const float complex *wave, float complex *fmat)
+{
+    const double twopi = 6.2831853071795862;
+    int lisp = (cinvtype[0] == 'P' || cinvtype[0] == 'p') && (cinvtype[1] == 'P' || cinvtype[1] == 'p');
+    int liss = (cinvtype[0] == 'S' || cinvtype[0] == 's') && (cinvtype[1] == 'S' || cinvtype[1] == 's');
+    if (!lisp && !liss) return 1;
+    if (inv->nvinv != 1) return 1;
+    double omega = twopi * freq, vpvs2 = vpvs * vpvs;
+    elem_ws w;
+    ws_init(&w, a->nord);
+    int q = w.nint * w.nint, nee = w.nee, nen = w.nen;
+    double *dab = (double *)malloc(sizeof(double) * 2 * q);
+    double *dvp = (double *)malloc(sizeof(double) * q), *dvs = (double *)malloc(sizeof(double) * q);
+    float complex *forcev = (float complex *)malloc(sizeof(float complex) * (inv->mbufrhs + 1));
+    int ierr = 0;
+    for (int inpgl = 1; inpgl <= inv->nnpgl && !ierr; inpgl += inv->nvinv) {
+        int inpg = -1;
+        for (int g = 0; g < a->nnpg; g++)
+            if ((inv->maskg[g] - 1) * inv->nvinv + 1 == inv->mygrad[inpgl - 1]) { inpg = g + 1; break; }
+        if (inpg < 0) { ierr = 1; break; }
+        int i1 = inv->jcsc[inpgl - 1], i2 = inv->jcsc[inpgl] - 1, nnz = i2 - i1 + 1;
+        for (int i = i1; i <= i2; i++) fmat[i - 1] = 0.0f;
+        for (int c = 0; c < inv->ncon && !ierr; c++) {
+            int e = inv->mconn[(inpg - 1) + (size_t)a->nnpg * c];
+            if (e <= 0) break;
+            e--;
+            if (orc_djacab(nen, w.nint, inpg, a->ieng + 4 * e, w.xigll, a->xlocs, a->zlocs, a->dens,
+                           a->ecoeff, a->nnpg, w.shl, w.det, dab, w.shg)) { ierr = 1; break; }
+            for (int k = 0; k < q; k++) {
+                if (lisp) { dvp[k] = dab[k]; dvs[k] = dab[k] / vpvs2; }
+                else { dvp[k] = dab[k + q] * vpvs2; dvs[k] = dab[k + q]; }
+            }
+            orc_dkm_vpvs(nen, w.nint, omega, py, w.wgll, dvp, dvs, w.det, w.shg, w.me, w.ke);
+            for (int r = 0; r < nnz; r++) forcev[r] = 0.0f;       /* vforce25.f:252-256 */
+            float complex cw = (float)inv->elem_wts[e] + I * 0.0f;
+            const int *lm = a->lm + NDIM * nen * e;
+            for (int ip = 0; ip < nee && !ierr; ip++) {
+                int idof = lm[ip];
+                if (idof <= 0) continue;
+                int inz = ibsect(nnz, idof, inv->icsc + i1 - 1);
+                if (inz <= 0) { ierr = 1; break; }
+                for (int iq = 0; iq < nee; iq++) {
+                    int jdof = lm[iq];
+                    if (jdof <= 0) continue;
+                    double complex v = w.me[ip + nee * iq] + w.ke[ip + nee * iq];
+                    float complex carg = (float)creal(v) + I * (float)cimag(v);
+                    forcev[inz - 1] += cmulf_ref(cmulf_ref(carg, wave[jdof - 1]), cw);
+                }
+            }
+        }
+        for (int r = 0; r < nnz; r++) fmat[i1 - 1 + r] = forcev[r];
+    }
+    free(dab); free(dvp); free(dvs); free(forcev);
+    ws_free(&w);
+    return ierr;
+}
+
+/* adjgrad.f90:88-145 ADJFLAM_DIST */
+void orc_adjflam_dist(int nnpgl, int na35, const int *mygrad, const int *jcsc, const int *icsc,
+                      const float complex *fmat, const float complex *adjlam, float *gprod)
+{
+    for (int i = 0; i < na35; i++) gprod[i] = 0.0f;
+    for (int c = 0; c < nnpgl; c++) {
+        int inpinv = mygrad[c];
+        for (int i = jcsc[c]; i <= jcsc[c + 1] - 1; i++) {
+            int jd = icsc[i - 1];
+            float fr = crealf(fmat[i - 1]), fi = cimagf(fmat[i - 1]);
+            float lr = crealf(adjlam[jd - 1]), li = cimagf(adjlam[jd - 1]);
+            gprod[inpinv - 1] -= fr * lr + fi * li;
+        }
+    }
+}
+
+/* Exact derivative of S(w,ky) with respect to anchor lambda/mu/rho.  The
+ * element matrix of KM25ISOQ is linear in the interpolated (lambda, mu, rho)
+ * fields, so dS_e/d(lambda_k) is KM25ISOQ evaluated with lambda(q) = N_k(q),
+ * mu = rho = 0 (and likewise for mu, rho).  PML elements are excluded. */
+int orc_grad_exact(const orc_asm *a, double freq, double py, const double complex *u,
+                   const double complex *lam, double *g_lmr, double *g_vvr)
+{
+    const double twopi = 6.2831853071795862;
+    double omega = twopi * freq;
+    elem_ws w;
+    ws_init(&w, a->nord);
+    int q = w.nint * w.nint, nee = w.nee, nen = w.nen;
+    double *fl = (double *)malloc(sizeof(double) * q), *fm = (double *)malloc(sizeof(double) * q);
+    double *fr = (double *)malloc(sizeof(double) * q);
+    double *nk = (double *)malloc(sizeof(double) * 4 * q);
+    int ierr = 0;
+    for (int e = 0; e < a->nelem && !ierr; e++) {
+        if (a->cdomain[e] == 'A') continue;
+        const int *ie = a->ieng + 4 * e;
+        if (orc_cjmk_tvi(nen, w.nint, ie, w.xigll, a->xlocs, a->zlocs, a->dens, a->ecoeff,
+                         a->nnpg, w.shl, w.det, w.rho, w.dmat, w.shg)) { ierr = 1; break; }
+        for (int ix = 0; ix < w.nint; ix++)
+            for (int iz = 0; iz < w.nint; iz++) {
+                double sf[4], dsf[8];
+                orc_csf2d(w.xigll[ix], w.xigll[iz], sf, dsf);
+                for (int k = 0; k < 4; k++) nk[k + 4 * (ix + w.nint * iz)] = sf[k];
+            }
+        const int *lm = a->lm + NDIM * nen * e;
+        for (int k = 0; k < 4; k++)
+            for (int p = 0; p < 3; p++) {
+                for (int j = 0; j < q; j++) {
+                    fl[j] = (p == 0) ? nk[k + 4 * j] : 0.0;
+                    fm[j] = (p == 1) ? nk[k + 4 * j] : 0.0;
+                    fr[j] = (p == 2) ? nk[k + 4 * j] : 0.0;
+                }
+                orc_km25isoq(nen, w.nint, omega, py, w.wgll, fl, fm, fr, w.det, w.shg, w.me, w.ke);
+                double complex acc = 0.0;
+                for (int ip = 0; ip < nee; ip++) {
+                    int id = lm[ip];
+                    if (id <= 0) continue;
+                    double complex row = 0.0;
+                    for (int iq = 0; iq < nee; iq++) {
+                        int jd = lm[iq];
+                        if (jd <= 0) continue;
+                        row += (w.me[ip + nee * iq] + w.ke[ip + nee * iq]) * u[jd - 1];
+                    }
+                    acc += conj(lam[id - 1]) * row;
+                }
+                g_lmr[3 * (ie[k] - 1) + p] += creal(acc);
+            }
+    }
+    if (g_vvr) {
+        for (int g = 0; g < a->nnpg; g++) {
+            double l = a->ecoeff[g], m = a->ecoeff[g + a->nnpg], r = a->dens[g];
+            double vp = sqrt((l + 2.0 * m) / r), vs = sqrt(m / r);
+            double gl = g_lmr[3 * g], gm = g_lmr[3 * g + 1], gr = g_lmr[3 * g + 2];
+            g_vvr[3 * g + 0] = gl * 2.0 * r * vp;
+            g_vvr[3 * g + 1] = gl * (-4.0 * r * vs) + gm * 2.0 * r * vs;
+            g_vvr[3 * g + 2] = gl * (vp * vp - 2.0 * vs * vs) + gm * vs * vs + gr;
+        }
+    }
+    free(fl); free(fm); free(fr); free(nk);
+    ws_free(&w);
+    return ierr;
+}

@@ -1,0 +1,219 @@
+"""HIP path against the oracle (CPU restatement pinned to the compiled
+reference): assembled operator, SpMV, batched product, gradient, virtual
+forces.  Tolerances: FP64 results 1e-12 normwise relative; COMPLEX*8
+results at FP32 level (the reference itself rounds every contribution to
+FP32)."""
+import numpy as np
+import pytest
+
+import oracle
+from fem25_amd import configs as cf
+from tests.helpers import TWOPI, csr, mesh_case, oracle_S, oracle_graph, pairs, relerr
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def _mi(name):
+    from fem25_amd.api import MeshInfo
+    m, mod, pml = mesh_case(name)
+    mi = MeshInfo(m, pml)
+    mi.set_model(mod)
+    return mi
+
+
+def test_mfma_layout():
+    import ctypes as C
+    from fem25_amd import _lib
+    err = C.c_double(-1)
+    assert _lib.load().f25_selftest_mfma(C.byref(err)) == 0, err.value
+    assert err.value == 0.0
+
+
+@pytest.mark.parametrize("name", ["tiny", "pml", "p6", "p2_pml"])
+def test_graph_bitexact(name):
+    mi = _mi(name)
+    g = mi.graph()
+    go = oracle_graph(name)
+    assert (g.ndof, g.nz) == (go.ndof, go.nz)
+    np.testing.assert_array_equal(g.ien, go.ien)
+    np.testing.assert_array_equal(g.lm, go.lm)
+    np.testing.assert_array_equal(g.irptr, go.irptr)
+    np.testing.assert_array_equal(g.jcptr, go.jcptr)
+    assert g.cnp == go.cnp
+
+
+@pytest.mark.parametrize("name,freq,py", [("tiny", 0.05, 0.0), ("tiny_layered", 0.08, 1.1e-4),
+                                          ("pml", 0.05, 0.7e-4), ("p6", 0.1, 0.5e-4),
+                                          ("p2_pml", 0.2, 1e-4)])
+def test_asmble_fp64(name, freq, py):
+    mi = _mi(name)
+    s = mi.asmble(b"B", freq, py, complex128=True)
+    m, mod, pml = mesh_case(name)
+    ref = oracle.asmble_z(m, mod, oracle_graph(name), freq, py, pml)
+    assert relerr(s, ref) < FP64_TOL
+
+
+@pytest.mark.parametrize("name,freq,py", [("tiny", 0.05, 0.0), ("pml", 0.05, 0.7e-4)])
+def test_asmble_cdist_c64(name, freq, py):
+    mi = _mi(name)
+    s = mi.asmble(b"B", freq, py)
+    m, mod, pml = mesh_case(name)
+    ref = oracle.asmble_cdist(m, mod, oracle_graph(name), freq, py, pml)
+    # entrywise against the row scale: both sides hold FP32-rounded sums
+    g = oracle_graph(name)
+    rowmax = np.maximum.reduceat(np.abs(ref.astype(np.complex128)), g.irptr[:-1] - 1)
+    rows = np.repeat(np.arange(g.ndof), np.diff(g.irptr))
+    assert np.all(np.abs(s.astype(np.complex128) - ref) <= 1e-6 * rowmax[rows])
+
+
+def test_asmble_config1_and_spmv(torch_dev):
+    torch = torch_dev
+    mi = _mi("c1")
+    freq, py = 0.05, 0.0
+    s = mi.asmble(b"B", freq, py, complex128=True)
+    m, mod, pml = mesh_case("c1")
+    g = oracle_graph("c1")
+    ref = oracle.asmble_z(m, mod, g, freq, py, pml)
+    assert relerr(s, ref) < FP64_TOL
+    x = cf.seeded_complex(g.ndof, seed=1)
+    ds = torch.from_numpy(s).cuda()
+    dx = torch.from_numpy(x).cuda()
+    dy = torch.zeros_like(dx)
+    mi.csr_apply_dev(ds, dx, dy)
+    assert relerr(dy.cpu().numpy(), oracle.spmv_z(g, ref, x)) < FP64_TOL
+
+
+@pytest.mark.parametrize("name", ["tiny_layered", "p6"])
+def test_apply_batch_fp64(name):
+    mi = _mi(name)
+    om, ky, f, py = pairs(2, 3)
+    g = oracle_graph(name)
+    x = cf.seeded_complex(g.ndof, batch=om.size, seed=3)
+    y = mi.apply_batch(om, ky, x)
+    for b in range(om.size):
+        S = oracle_S(name, f[b], py[b])
+        assert relerr(y[b], S @ x[b]) < FP64_TOL, b
+
+
+def test_apply_batch_config1_modes(torch_dev):
+    torch = torch_dev
+    from fem25_amd import _lib
+    mi = _mi("c1")
+    om, ky, f, py = pairs(2, 5)
+    g = oracle_graph("c1")
+    x = cf.seeded_complex(g.ndof, batch=om.size, seed=4)
+    xd = torch.from_numpy(np.ascontiguousarray(x.T)).cuda()        # [dof][pair]
+    y64 = torch.zeros_like(xd)
+    mi.apply_batch_dev(om, ky, xd, y64, _lib.F25_APPLY_F64)
+    y64 = y64.cpu().numpy().T
+    yref = np.stack([oracle_S("c1", f[b], py[b]) @ x[b] for b in range(om.size)])
+    for b in range(om.size):
+        assert relerr(y64[b], yref[b]) < FP64_TOL
+    x32 = xd.to(torch.complex64)
+    y32 = torch.zeros_like(x32)
+    mi.apply_batch_dev(om, ky, x32, y32, _lib.F25_APPLY_F32)
+    yr = torch.zeros_like(xd)
+    mi.apply_batch_dev(om, ky, xd, yr, _lib.F25_APPLY_F32_REFINED)
+    e32 = max(relerr(y32.cpu().numpy().T[b], yref[b]) for b in range(om.size))
+    er = max(relerr(yr.cpu().numpy().T[b], yref[b]) for b in range(om.size))
+    print(f"config-1 normwise error: fp32 {e32:.2e}, fp32-refined {er:.2e}")
+    assert e32 < 1e-5
+    assert er < 1e-6
+
+
+def test_apply_quadratic_in_ky():
+    """S(2k) - 3 S(k) + 3 S(0) - S(-k) = 0 (S is quadratic in ky)."""
+    mi = _mi("tiny_layered")
+    g = oracle_graph("tiny_layered")
+    w, k = TWOPI * 0.07, 1.3e-5
+    x = cf.seeded_complex(g.ndof, seed=11)
+    om = np.full(4, w)
+    ky = np.array([2 * k, k, 0.0, -k])
+    y = mi.apply_batch(om, ky, np.stack([x] * 4))
+    comb = y[0] - 3 * y[1] + 3 * y[2] - y[3]
+    assert np.linalg.norm(comb) < 1e-11 * np.linalg.norm(y[0])
+
+
+@pytest.mark.parametrize("name", ["tiny_layered", "pml"])
+def test_gradient_exact(name):
+    mi = _mi(name)
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    om, ky, f, py = pairs(1, 2)
+    u = cf.seeded_complex(g.ndof, batch=om.size, seed=21)
+    lam = cf.seeded_complex(g.ndof, batch=om.size, seed=22)
+    gd = mi.gradient(om, ky, u, lam)
+    ref = np.zeros((m.nnpg, 3))
+    for b in range(om.size):
+        ref += oracle.grad_exact(m, mod, g, f[b], py[b], u[b], lam[b], pml)[1]
+    for p in range(3):
+        assert relerr(gd[:, p], ref[:, p]) < 1e-11, p
+
+
+@pytest.mark.parametrize("name,cinv", [("tiny_layered", b"PP"), ("pml", b"PP"), ("tiny", b"SS")])
+def test_cfmat_adjflam(name, cinv):
+    from fem25_amd import api
+    mi = _mi(name)
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    inv = mi.inv_setup(cinv)
+    io = oracle.inv_setup(m, mod, g, pml)
+    np.testing.assert_array_equal(inv.maskg, io.maskg)
+    np.testing.assert_array_equal(inv.jcsc_fdist, io.jcsc)
+    np.testing.assert_array_equal(inv.icsc_fdist, io.icsc)
+    np.testing.assert_array_equal(inv.mygrad, io.mygrad)
+    np.testing.assert_allclose(inv.elem_wts, io.elem_wts, rtol=1e-15)
+    wave = cf.seeded_complex(g.ndof, seed=31).astype(np.complex64)
+    freq, py = 0.06, 0.8e-4
+    fm, ierr = api.cfmat_dist(inv.nz_fdist, g.ndof, freq, py, wave, mi, inv)
+    assert ierr == 0
+    ref = oracle.cfmat_dist(m, mod, g, io, freq, py, wave, cinvtype=cinv)
+    assert relerr(fm, ref) < 2e-6
+    lam = cf.seeded_complex(g.ndof, seed=32).astype(np.complex64)
+    gp = api.adjflam_dist(mi, inv, ref, lam)
+    gref = oracle.adjflam_dist(io, ref, lam)
+    assert relerr(gp, gref) < 1e-5
+
+
+def test_peff():
+    from fem25_amd import api
+    mi = _mi("pml")
+    m, mod, pml = mesh_case("pml")
+    g = oracle_graph("pml")
+    cd = oracle.asmble_cdist(m, mod, g, 0.05, 0.0, pml)
+    ue = cf.seeded_complex(g.ndof, seed=41).astype(np.complex64)
+    pe = api.asmb_peff(mi, cd, ue)
+    ref = oracle.asmb_peff(g, cd, ue)
+    assert relerr(pe, ref) < 1e-5
+    assert np.count_nonzero(ref) > 0
+
+
+def test_gn_fv_matches_operator_difference():
+    """F v = dS/dm[v] u; S is quadratic in (Vp, Vs) and linear in rho, so a
+    central difference of the oracle's FP64 operator is exact up to rounding."""
+    name = "tiny_layered"
+    mi = _mi(name)
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    rng = np.random.default_rng(51)
+    v = np.stack([rng.standard_normal(m.nnpg) * 50.0, rng.standard_normal(m.nnpg) * 30.0,
+                  rng.standard_normal(m.nnpg) * 20.0], axis=1)
+    f, py = 0.07, 0.6e-4
+    u = cf.seeded_complex(g.ndof, seed=52)
+    fv = mi.gn_fv(np.array([TWOPI * f]), np.array([TWOPI * f * py]), u, v)[0]
+    h = 1e-2
+    mp = cf.model_from_velocity(mod.vp + h * v[:, 0], mod.vs + h * v[:, 1], mod.dens + h * v[:, 2])
+    mm = cf.model_from_velocity(mod.vp - h * v[:, 0], mod.vs - h * v[:, 1], mod.dens - h * v[:, 2])
+    Sp = csr(g, oracle.asmble_z(m, mp, g, f, py, pml))
+    Sm = csr(g, oracle.asmble_z(m, mm, g, f, py, pml))
+    ref = (Sp @ u - Sm @ u) / (2 * h)
+    assert relerr(fv, ref) < 1e-9
