@@ -1,0 +1,223 @@
+"""Throughput of the batched K(omega, ky) u product (and the adjoint gradient)
+on one MI355X, on the 200 x 100 element, order-4 workload with 8 frequencies
+x 16 out-of-plane wavenumbers = 128 operator applications per step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu]
+
+Prints ONE JSON line on stdout; progress goes to stderr.  A step is one
+application of all 128 S(omega_b, ky_b) to their vectors (inputs resident in
+HBM, layout [dof][pair]).  The CPU baseline is the reference's own Fortran
+(ASMBLE_DRIVER, compiled from source into oracle/_ref) plus the CSR product,
+timed on one (omega, ky) pair of the same mesh on one host core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12
+PEAK_FP32_MFMA = 157.3e12
+PEAK_HBM = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
+    ap.add_argument("--skip-extras", action="store_true", help="only the timed FP64 product")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the batch of "
+              "(omega, ky) problems would be split across GPUs with one gradient all-reduce)",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from fem25_amd import _lib
+    from fem25_amd import configs as cf
+    from fem25_amd.api import MeshInfo
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    L = _lib.load()
+
+    shape = cf.CONFIGS[args.config]
+    mesh = cf.structured_mesh(shape["nx"], shape["nz"], shape["nord"])
+    model = cf.layered_model(mesh, seed=2)
+    vs_min = 6300.0 / math.sqrt(3.0)
+    omega, ky, freq, py = cf.freq_ky_batch(8, 16, 0.02, 0.1, vs_min)
+    npair = omega.size
+    t0 = time.time()
+    mi = MeshInfo(mesh)
+    mi.set_model(model)
+    ndof, nen = mi.ndof, mi.nen
+    log(f"setup: {mesh.nelem} elements, p={mesh.nord}, {ndof} DOF, {mi.nz} CSR nonzeros, "
+        f"{npair} pairs, {time.time() - t0:.1f} s")
+
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.randn(ndof, npair, dtype=torch.complex128, device="cuda", generator=g)
+    y = torch.empty_like(x)
+
+    def step():
+        mi.apply_batch_dev(omega, ky, x, y, _lib.F25_APPLY_F64)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    kern_ms = []
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        kern_ms.append(L.f25_last_kernel_ms(b"apply"))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    ms_step = 1e3 * wall / args.steps
+    gdofs = ndof * npair / (ms_step * 1e-3) / 1e9
+    k_ms = float(np.mean(kern_ms))
+    ncol = mi.ncolour
+    flops = 60.0 * nen * nen * mesh.nelem * npair          # useful block flops per step
+    bytes_alg = 2 * ndof * npair * 16 + 9 * nen * nen * 8 * mesh.nelem
+    achieved = flops / (k_ms * 1e-3)
+    log(f"FP64 product: {ms_step:.3f} ms/step, kernels {k_ms:.3f} ms, {gdofs:.2f} Gdof/s, "
+        f"{achieved / 1e12:.2f} TFLOP/s")
+
+    extra = {}
+    if not args.skip_extras:
+        # accuracy of the MFMA product against the assembled CSR operator (FP64)
+        b = npair - 1
+        s = torch.empty(mi.nz, dtype=torch.complex128, device="cuda")
+        mi.asmble_dev(b"B", float(freq[b]), float(py[b]), s)
+        xb = x[:, b].contiguous()
+        yb = torch.empty_like(xb)
+        mi.csr_apply_dev(s, xb, yb)
+        err64 = float(torch.linalg.norm(y[:, b] - yb) / torch.linalg.norm(yb))
+        # plain FP64 GPU path: assemble S(omega, ky) in CSR, then SpMV, per pair
+        nplain = 8
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for i in range(nplain):
+            mi.asmble_dev(b"B", float(freq[i]), float(py[i]), s)
+            mi.csr_apply_dev(s, x[:, i].contiguous(), yb)
+        torch.cuda.synchronize()
+        plain_gdofs = ndof * nplain / (time.perf_counter() - tp) / 1e9
+        # FP32 and FP32 + correction products
+        x32 = x.to(torch.complex64)
+        y32 = torch.empty_like(x32)
+        yr = torch.empty_like(x)
+        res = {}
+        for name, mode, xx, yy in (("fp32", _lib.F25_APPLY_F32, x32, y32),
+                                   ("fp32_refined", _lib.F25_APPLY_F32_REFINED, x, yr)):
+            mi.apply_batch_dev(omega, ky, xx, yy, mode)
+            torch.cuda.synchronize()
+            tt = time.perf_counter()
+            for _ in range(3):
+                mi.apply_batch_dev(omega, ky, xx, yy, mode)
+            torch.cuda.synchronize()
+            ms = 1e3 * (time.perf_counter() - tt) / 3
+            d = (yy.to(torch.complex128) - y)
+            err = float((torch.linalg.norm(d, dim=0) / torch.linalg.norm(y, dim=0)).max())
+            res[name] = {"gdof_per_s": round(ndof * npair / ms * 1e-6, 3),
+                         "max_rel_err_vs_fp64": err}
+        del x32, y32, yr
+        # adjoint gradient over 128 (omega, ky) wavefield pairs
+        u = torch.randn(npair, ndof, dtype=torch.complex128, device="cuda", generator=g)
+        lam = torch.randn(npair, ndof, dtype=torch.complex128, device="cuda", generator=g)
+        gg = torch.zeros(mesh.nnpg, 3, dtype=torch.float64, device="cuda")
+        mi.gradient_dev(omega, ky, u, lam, gg)
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        ng = 2
+        for _ in range(ng):
+            mi.gradient_dev(omega, ky, u, lam, gg)
+        torch.cuda.synchronize()
+        grad_s = (time.perf_counter() - tg) / ng
+        extra = {"fp64_rel_err_vs_assembled_csr": err64,
+                 "plain_fp64_csr_gdof_per_s": round(plain_gdofs, 3),
+                 "speedup_vs_plain_fp64": round(gdofs / plain_gdofs, 2),
+                 **res,
+                 "gradients_per_s": round(npair / grad_s, 2),
+                 "gradient_kernel_ms": L.f25_last_kernel_ms(b"grad")}
+        log(json.dumps(extra))
+
+    cpu = None
+    if not args.skip_cpu:
+        cpu = cpu_baseline(mesh, model, mi, float(freq[3]), float(py[3]))
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "K(omega,ky)*u Gdof/s (batched element-block product, 128 (omega,ky) pairs)",
+        "value": round(gdofs, 3),
+        "unit": "Gdof/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded layered model with 20 Gaussian anomalies; seeded N(0,1) vectors)",
+        "config": {"workload": f"{shape['nx']}x{shape['nz']} elements, GLL order {shape['nord']}, "
+                               f"8 freq x 16 ky = {npair} K(omega,ky) products per step",
+                   "nelem": mesh.nelem, "nord": mesh.nord, "ndof": ndof, "npair": npair},
+        "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": None,
+                     "kernel": "k_apply<5,double,double2,false>",
+                     "launches_per_step": ncol, "flops_per_step": flops,
+                     "algorithmic_bytes_per_step": bytes_alg,
+                     "kernel_ms_per_step": round(k_ms, 4)},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+    mi.close()
+    return 0
+
+
+def cpu_baseline(mesh, model, mi, freq, py):
+    """The reference path on one (omega, ky) pair: ASMBLE_DRIVER (compiled
+    reference Fortran) + CSR SpMV, single core, full mesh."""
+    try:
+        import oracle
+        from fem25_amd import configs as cf
+    except Exception as e:  # pragma: no cover
+        return {"error": str(e)}
+    g = mi.graph()
+    x = cf.seeded_complex(g.ndof, seed=5).astype(np.complex64)
+    og = oracle.Graph(g.nnp, g.ndof, g.nz, g.ien, g.lm, g.irptr, g.jcptr, g.cnp)
+    kind = "reference" if oracle.ref_available() else "port"
+    t = time.perf_counter()
+    if kind == "reference":
+        s = oracle.ref_asmble(mesh, model, og, freq, py, cf.PML())
+    else:
+        s = oracle.asmble_cdist(mesh, model, og, freq, py, cf.PML())
+    ta = time.perf_counter() - t
+    t = time.perf_counter()
+    oracle.spmv_c(og, s, x)
+    tm = time.perf_counter() - t
+    tot = ta + tm
+    return {"value": round(g.ndof / tot / 1e9, 6), "unit": "Gdof/s", "cores": 1, "kind": kind,
+            "sample": f"one (omega,ky) application on the full {mesh.nelem}-element mesh: "
+                      f"ASMBLE_DRIVER {ta:.2f} s + CSR SpMV {tm:.3f} s",
+            "seconds": round(tot, 3)}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -53,6 +53,9 @@ EXPORTS = {
     "f25_gn_fv": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp, C.c_void_p]),
     "f25_gn_fhw": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, dp]),
     "f25_gn_diag": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp]),
+    "f25_graph_build": (C.c_int, [C.POINTER(MeshDesc), C.c_int, C.POINTER(C.c_void_p), i64p]),
+    "f25_graph_get": (C.c_int, [C.c_void_p, ip, ip, ip, ip, C.c_char_p]),
+    "f25_graph_destroy": (C.c_int, [C.c_void_p]),
     "f25_last_error": (C.c_char_p, []),
     "f25_last_kernel_ms": (C.c_double, [C.c_char_p]),
     "f25_selftest_mfma": (C.c_int, [dp]),

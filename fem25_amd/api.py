@@ -29,6 +29,15 @@ def _p(a, t):
     return a.ctypes.data_as(t)
 
 
+def _ready(*tensors) -> None:
+    """Device inputs are produced on PyTorch's stream; wait for them."""
+    import torch
+    for t in tensors:
+        if not t.is_cuda or not t.is_contiguous():
+            raise ValueError("device arguments must be contiguous GPU tensors")
+    torch.cuda.current_stream().synchronize()
+
+
 @dataclasses.dataclass
 class Graph:
     """GLL numbering in the reference's conventions (1-based)."""
@@ -141,12 +150,14 @@ class MeshInfo:
     def asmble_dev(self, cftype: bytes, freq: float, py: float, out) -> None:
         """out: torch complex64/complex128 tensor of NZ values on the GPU."""
         import torch
+        _ready(out)
         prec = _lib.F25_C128 if out.dtype == torch.complex128 else _lib.F25_C64
         check(_lib.load().f25_asmble_dev(self.h, cftype, freq, py, prec, C.c_void_p(out.data_ptr())),
               "f25_asmble_dev")
 
     def csr_apply_dev(self, s, x, y) -> None:
         import torch
+        _ready(s, x, y)
         prec = _lib.F25_C128 if s.dtype == torch.complex128 else _lib.F25_C64
         check(_lib.load().f25_csr_apply_dev(self.h, prec, C.c_void_p(s.data_ptr()),
                                             C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())),
@@ -155,6 +166,7 @@ class MeshInfo:
     # -- batched element-block product ----------------------------------------
     def apply_batch_dev(self, omega, ky, x, y, mode: int = _lib.F25_APPLY_F64) -> None:
         """x, y: torch tensors [ndof, npair] (complex128, or complex64 for F32)."""
+        _ready(x, y)
         om = np.ascontiguousarray(omega, np.float64)
         kv = np.ascontiguousarray(ky, np.float64)
         check(_lib.load().f25_apply_batch_dev(self.h, om.size, _p(om, dp), _p(kv, dp), mode,
@@ -187,6 +199,7 @@ class MeshInfo:
 
     def gradient_dev(self, omega, ky, u, lam, g) -> None:
         """u, lam: torch complex128 [ntrip, ndof] on the GPU; g: torch float64 (nnpg, 3), accumulated."""
+        _ready(u, lam, g)
         om = np.ascontiguousarray(omega, np.float64)
         kv = np.ascontiguousarray(ky, np.float64)
         check(_lib.load().f25_gradient_dev(self.h, om.size, _p(om, dp), _p(kv, dp),

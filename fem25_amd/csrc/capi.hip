@@ -101,6 +101,9 @@ struct f25_mesh_s {
     int *d_col_elem = nullptr, *d_col_corner = nullptr, *d_mygrad = nullptr, *d_jcsc = nullptr, *d_icsc = nullptr;
     double *d_col_cp = nullptr, *d_col_cs = nullptr, *d_ew = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // (omega, ky) staging, grown on demand
+    double *d_omky = nullptr;
+    int omky_cap = 0;
 };
 
 extern "C" const char *f25_last_error(void) { return g_err.c_str(); }
@@ -193,7 +196,9 @@ static int init_handle(f25_mesh *m, const f25_mesh_desc *desc) {
     m->d.cnnpg = m->cnnpg.data();
     make_tables(desc->nord, m->tab);
     HIPCHK(hipGetDevice(&m->dev));
-    HIPCHK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+    // a blocking stream: it orders with work on the legacy default stream, which
+    // is where callers such as PyTorch produce the device inputs
+    HIPCHK(hipStreamCreate(&m->st));
     HIPCHK(hipEventCreate(&m->ev0));
     HIPCHK(hipEventCreate(&m->ev1));
     // colouring: all elements, and the PML elements on their own
@@ -211,6 +216,57 @@ static int init_handle(f25_mesh *m, const f25_mesh_desc *desc) {
     } else {
         m->pml_cptr.assign(1, 0);
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// host-only numbering
+struct f25_graph_s {
+    Tables tab;
+    Graph g;
+    int nelem = 0;
+};
+
+extern "C" int f25_graph_build(const f25_mesh_desc *desc, int with_csr, f25_graph **out,
+                               int64_t sizes[3]) {
+    if (!out || !desc || desc->nord < 2 || desc->nord > 6 || desc->nelem <= 0 || desc->nnpg <= 0 ||
+        !desc->ieng || !desc->xlocs || !desc->zlocs || !desc->cdomain || !desc->cnnpg)
+        FAIL(F25_ERR_ARG, "invalid mesh description (NORD must be 2..6)");
+    *out = nullptr;
+    for (size_t i = 0; i < 4 * (size_t)desc->nelem; ++i)
+        if (desc->ieng[i] < 1 || desc->ieng[i] > desc->nnpg) FAIL(F25_ERR_ARG, "IENG entry out of range");
+    std::unique_ptr<f25_graph> g(new f25_graph_s());
+    g->nelem = desc->nelem;
+    make_tables(desc->nord, g->tab);
+    int r = build_numbering(*desc, g->tab, g->g);
+    if (r) FAIL(r, "GLL numbering failed");
+    if (with_csr) {
+        r = build_csr(*desc, g->tab, g->g);
+        if (r) FAIL(r, "CSR pattern build failed");
+    }
+    if (sizes) {
+        sizes[0] = g->g.nnp;
+        sizes[1] = g->g.ndof;
+        sizes[2] = g->g.have_csr ? g->g.irptr[g->g.ndof] : 0;
+    }
+    *out = g.release();
+    return 0;
+}
+
+extern "C" int f25_graph_get(const f25_graph *g, int *ien, int *lm, int *irptr, int *jcptr, char *cnp) {
+    if (!g) FAIL(F25_ERR_ARG, "null graph");
+    if ((irptr || jcptr) && !g->g.have_csr) FAIL(F25_ERR_STATE, "graph built without CSR pattern");
+    if (ien) std::copy(g->g.ien.begin(), g->g.ien.end(), ien);
+    if (lm) std::copy(g->g.lm.begin(), g->g.lm.end(), lm);
+    if (irptr)
+        for (int i = 0; i <= g->g.ndof; ++i) irptr[i] = (int)(g->g.irptr[i] + 1);
+    if (jcptr) std::copy(g->g.jcptr.begin(), g->g.jcptr.end(), jcptr);
+    if (cnp) std::copy(g->g.cnp.begin(), g->g.cnp.end(), cnp);
+    return 0;
+}
+
+extern "C" int f25_graph_destroy(f25_graph *g) {
+    delete g;
     return 0;
 }
 
@@ -275,6 +331,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_col_elem); dfree(m->d_col_corner);
     dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_col_cp); dfree(m->d_col_cs);
     dfree(m->d_ew);
+    dfree(m->d_omky);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
     if (m->ev1) (void)hipEventDestroy(m->ev1);
     if (m->st) (void)hipStreamDestroy(m->st);
@@ -554,22 +611,23 @@ static int dispatch_apply(f25_mesh *m, const double *om, const double *kv, int n
 static int apply_core(f25_mesh *m, int npair, const double *d_om, const double *d_ky, int mode,
                       const void *x, void *y, const double *blk64) {
     const size_t n = (size_t)m->g.ndof * npair;
-    HIPCHK(hipEventRecord(m->ev0, m->st));
-    if (mode == F25_APPLY_F64) {
+    if (mode == F25_APPLY_F64 || mode == F25_APPLY_F32_REFINED)
         fill_async((double2 *)y, n, make_double2(0.0, 0.0), m->st);
+    else if (mode == F25_APPLY_F32)
+        fill_async((float2 *)y, n, make_float2(0.f, 0.f), m->st);
+    else
+        FAIL(F25_ERR_ARG, "bad apply mode");
+    // the recorded time covers the product kernels only (not the zero fill)
+    HIPCHK(hipEventRecord(m->ev0, m->st));
+    if (mode == F25_APPLY_F64)
         TRY((dispatch_apply<double, double2, false>(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y,
                                                     blk64, nullptr)));
-    } else if (mode == F25_APPLY_F32) {
-        fill_async((float2 *)y, n, make_float2(0.f, 0.f), m->st);
+    else if (mode == F25_APPLY_F32)
         TRY((dispatch_apply<float, float2, false>(m, d_om, d_ky, npair, (const float2 *)x, (float2 *)y,
                                                   m->d_blk_hi, nullptr)));
-    } else if (mode == F25_APPLY_F32_REFINED) {
-        fill_async((double2 *)y, n, make_double2(0.0, 0.0), m->st);
+    else
         TRY((dispatch_apply<float, double2, true>(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y,
                                                   m->d_blk_hi, m->d_blk_lo)));
-    } else {
-        FAIL(F25_ERR_ARG, "bad apply mode");
-    }
     HIPCHK(hipEventRecord(m->ev1, m->st));
     HIPCHK(hipStreamSynchronize(m->st));
     float ms = 0.f;
@@ -578,18 +636,24 @@ static int apply_core(f25_mesh *m, int npair, const double *d_om, const double *
     return 0;
 }
 
+static int stage_omky(f25_mesh *m, int n, const double *omega, const double *ky) {
+    if (n > m->omky_cap) {
+        dfree(m->d_omky);
+        TRY(dalloc(&m->d_omky, 2 * (size_t)n));
+        m->omky_cap = n;
+    }
+    HIPCHK(hipMemcpyAsync(m->d_omky, omega, n * sizeof(double), hipMemcpyHostToDevice, m->st));
+    HIPCHK(hipMemcpyAsync(m->d_omky + m->omky_cap, ky, n * sizeof(double), hipMemcpyHostToDevice, m->st));
+    return 0;
+}
+
 extern "C" int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
                                    int mode, const void *x, void *y) {
     if (!m || npair <= 0 || !omega || !ky || !x || !y) FAIL(F25_ERR_ARG, "invalid apply arguments");
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
     HIPCHK(hipSetDevice(m->dev));
-    double *d_om = nullptr, *d_ky = nullptr;
-    TRY(upload(&d_om, omega, npair, m->st));
-    TRY(upload(&d_ky, ky, npair, m->st));
-    int r = apply_core(m, npair, d_om, d_ky, mode, x, y, m->d_blk);
-    dfree(d_om);
-    dfree(d_ky);
-    return r;
+    TRY(stage_omky(m, npair, omega, ky));
+    return apply_core(m, npair, m->d_omky, m->d_omky + m->omky_cap, mode, x, y, m->d_blk);
 }
 
 static int transpose_dev(hipStream_t s, const double2 *in, double2 *out, int rows, int cols) {
@@ -651,13 +715,9 @@ extern "C" int f25_gradient_dev(f25_mesh *m, int ntrip, const double *omega, con
     if (!m || ntrip <= 0 || !u || !lam || !g) FAIL(F25_ERR_ARG, "invalid gradient arguments");
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
     HIPCHK(hipSetDevice(m->dev));
-    double *d_om = nullptr, *d_ky = nullptr;
-    TRY(upload(&d_om, omega, ntrip, m->st));
-    TRY(upload(&d_ky, ky, ntrip, m->st));
-    int r = grad_core(m, ntrip, d_om, d_ky, (const double2 *)u, (const double2 *)lam, g, 1);
-    dfree(d_om);
-    dfree(d_ky);
-    return r;
+    TRY(stage_omky(m, ntrip, omega, ky));
+    return grad_core(m, ntrip, m->d_omky, m->d_omky + m->omky_cap, (const double2 *)u,
+                     (const double2 *)lam, g, 1);
 }
 
 static int grad_host(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
@@ -888,13 +948,16 @@ extern "C" int f25_selftest_mfma(double *maxerr) {
     if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
     // A[row][k] = 1 + row + 16k (lane row + 16k), B[k][col] = 1 + 0.5 col - 3k
     double worst = 0.0;
+    auto ref = [](int row, int col) {
+        double r = 0.0;
+        for (int k = 0; k < 4; ++k) r += (1.0 + row + 16.0 * k) * (1.0 + 0.5 * col - 3.0 * k);
+        return r;
+    };
     for (int l = 0; l < 64; ++l)
         for (int i = 0; i < 4; ++i) {
-            int row = 4 * (l >> 4) + i, col = l & 15;
-            double ref = 0.0;
-            for (int k = 0; k < 4; ++k) ref += (1.0 + row + 16.0 * k) * (1.0 + 0.5 * col - 3.0 * k);
-            worst = std::max(worst, std::fabs(hd[l * 4 + i] - ref));
-            worst = std::max(worst, std::fabs((double)hf[l * 4 + i] - ref));
+            const int col = l & 15;
+            worst = std::max(worst, std::fabs(hd[l * 4 + i] - ref((l >> 4) + 4 * i, col)));
+            worst = std::max(worst, std::fabs((double)hf[l * 4 + i] - ref(4 * (l >> 4) + i, col)));
         }
     if (maxerr) *maxerr = worst;
     return worst == 0.0 ? 0 : F25_ERR_HIP;

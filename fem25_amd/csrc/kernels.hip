@@ -178,12 +178,17 @@ __global__ __launch_bounds__(NT) void k_blocks(int nelem, int nq1, const double 
 // colour touches these DOFs).
 template <typename T>
 struct MfmaTraits;
+// Operand layout of both 16x16x4 forms: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16].  Result element i (of 4) of lane l is column l%16 and row
+//   f64: l/16 + 4 i        f32: 4 (l/16) + i
+// (measured on gfx950 with a single-wave probe).
 template <>
 struct MfmaTraits<double> {
     typedef double v4 __attribute__((ext_vector_type(4)));
     __device__ static inline v4 mma(double a, double b, v4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __device__ static constexpr int row(int kg, int i) { return kg + 4 * i; }
 };
 template <>
 struct MfmaTraits<float> {
@@ -191,6 +196,7 @@ struct MfmaTraits<float> {
     __device__ static inline v4 mma(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    __device__ static constexpr int row(int kg, int i) { return 4 * kg + i; }
 };
 
 struct SubBlk { int blk; int trans; int xcomp; int kind; };
@@ -338,10 +344,10 @@ __global__ __launch_bounds__(256) void k_apply(const int *__restrict__ order,
                             }
                         }
                     }
-                    // rows 4*kg + i of this tile, column col
+                    // result rows of this lane (see MfmaTraits), column col
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const int row = mt * 16 + 4 * kg + i;
+                        const int row = mt * 16 + MT_::row(kg, i);
                         if (row < NEN && live) {
                             const int dof = sLM[row * 3 + c];
                             if (dof >= 0) {

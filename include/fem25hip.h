@@ -59,6 +59,18 @@ typedef struct {
 } f25_mesh_desc;
 
 typedef struct f25_mesh_s f25_mesh;
+typedef struct f25_graph_s f25_graph;
+
+/* ---- numbering on the host (no device needed) ------------------------- */
+
+/* GEN_GRAPH25's numbering sequence (graph25_driver.f90:30-86): GENIEN
+ * (graph25.f:509), GENID (graph25.f:315), GENLM (graph25.f:1), GENCRS_QUICK
+ * (graph25.f:136, when with_csr) and CNMESH (bielak_utils.f:934).  sizes:
+ * [0] NNP, [1] NDOF, [2] NZ (0 without CSR). */
+int f25_graph_build(const f25_mesh_desc *desc, int with_csr, f25_graph **out, int64_t sizes[3]);
+/* IEN(NEN,NELEM), LM(3,NEN,NELEM), IRPTR(NDOF+1), JCPTR(NZ), CNP(NDOF); NULL skips. */
+int f25_graph_get(const f25_graph *g, int *ien, int *lm, int *irptr, int *jcptr, char *cnp);
+int f25_graph_destroy(f25_graph *g);
 
 /* ---- mesh and model -------------------------------------------------- */
 
@@ -161,6 +173,10 @@ int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const double *ky, c
 
 /* ---- diagnostics ------------------------------------------------------------ */
 const char *f25_last_error(void);
+/* Runs one 16x16x4 MFMA in FP64 and FP32 on the current device and checks
+ * the operand/result lane layout the batched product relies on; *maxerr is
+ * the largest deviation (0 when the layout holds). */
+int f25_selftest_mfma(double *maxerr);
 /* Average duration (ms) of the most recent launch of the named kernel
  * family ("apply", "grad", "asmble"), measured with HIP events on the
  * stream the kernel ran on; -1 if unknown. */
