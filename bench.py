@@ -41,6 +41,8 @@ def main() -> int:
     ap.add_argument("--config", default="c2")
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--skip-extras", action="store_true", help="only the timed FP64 product")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one plain device copy of the input (known bytes) for counter scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the batch of "
@@ -97,6 +99,13 @@ def main() -> int:
     achieved = flops / (k_ms * 1e-3)
     log(f"FP64 product: {ms_step:.3f} ms/step, kernels {k_ms:.3f} ms, {gdofs:.2f} Gdof/s, "
         f"{achieved / 1e12:.2f} TFLOP/s")
+
+    if args.copy_calib:
+        yc = torch.empty_like(x)
+        yc.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {x.numel() * 16} bytes read, {x.numel() * 16} bytes written")
+        del yc
 
     extra = {}
     if not args.skip_extras:

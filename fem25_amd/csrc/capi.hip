@@ -82,6 +82,7 @@ struct f25_mesh_s {
     int dev = 0;
     // device
     int *d_ieng = nullptr, *d_lm = nullptr, *d_order = nullptr, *d_pml_order = nullptr, *d_err = nullptr;
+    unsigned long long *d_first = nullptr;   // first-touch node masks per element
     double *d_xl = nullptr, *d_zl = nullptr, *d_xd = nullptr, *d_zd = nullptr;
     double *d_shl = nullptr, *d_wq = nullptr, *d_xq = nullptr;
     char *d_cdom = nullptr, *d_cnp = nullptr;
@@ -150,6 +151,27 @@ static int upload_mesh(f25_mesh *m) {
     for (size_t i = 0; i < lm0.size(); ++i) lm0[i] = m->g.lm[i] - 1;   // 0-based, -1 fixed
     TRY(upload(&m->d_lm, lm0.data(), lm0.size(), s));
     TRY(upload(&m->d_order, m->order.data(), m->order.size(), s));
+    {   // first touch: the element of lowest colour holding a node writes its y
+        std::vector<int> colour(d.nelem), mincol(m->g.ndof + 1, 1 << 30);
+        for (size_t c = 0; c + 1 < m->cptr.size(); ++c)
+            for (int i = m->cptr[c]; i < m->cptr[c + 1]; ++i) colour[m->order[i]] = (int)c;
+        auto key = [&](int e, int a) {
+            const int *l = &m->g.lm[((size_t)e * nen + a) * 3];
+            return l[0] > 0 ? l[0] : (l[1] > 0 ? l[1] : l[2]);
+        };
+        for (int e = 0; e < d.nelem; ++e)
+            for (int a = 0; a < nen; ++a) {
+                int k = key(e, a);
+                if (k > 0) mincol[k] = std::min(mincol[k], colour[e]);
+            }
+        std::vector<unsigned long long> first(d.nelem, 0ull);
+        for (int e = 0; e < d.nelem; ++e)
+            for (int a = 0; a < nen; ++a) {
+                int k = key(e, a);
+                if (k > 0 && mincol[k] == colour[e]) first[e] |= 1ull << a;
+            }
+        TRY(upload(&m->d_first, first.data(), first.size(), s));
+    }
     TRY(upload(&m->d_pml_order, m->pml_order.data(), m->pml_order.size(), s));
     TRY(dalloc(&m->d_err, 1));
     HIPCHK(hipMemsetAsync(m->d_err, 0, sizeof(int), s));
@@ -323,7 +345,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     if (!m) return 0;
     (void)hipSetDevice(m->dev);
     if (m->st) (void)hipStreamSynchronize(m->st);
-    dfree(m->d_ieng); dfree(m->d_lm); dfree(m->d_order); dfree(m->d_pml_order); dfree(m->d_err);
+    dfree(m->d_ieng); dfree(m->d_lm); dfree(m->d_first); dfree(m->d_order); dfree(m->d_pml_order); dfree(m->d_err);
     dfree(m->d_xl); dfree(m->d_zl); dfree(m->d_xd); dfree(m->d_zd); dfree(m->d_shl); dfree(m->d_wq);
     dfree(m->d_xq); dfree(m->d_cdom); dfree(m->d_cnp); dfree(m->d_rho); dfree(m->d_lam); dfree(m->d_mu);
     dfree(m->d_blk); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_dblk); dfree(m->d_irptr);
@@ -574,25 +596,38 @@ extern "C" int f25_asmb_peff(f25_mesh *m, const void *cdist, const void *ue, voi
 }
 
 // ---------------------------------------------------------------------------
-template <int NL, typename BT, typename XT, bool REFINE>
-static int launch_apply(f25_mesh *m, const double *om, const double *kv, int npair, const XT *x,
-                        XT *y, const BT *blk, const BT *blk_lo) {
+template <int NL, typename BT, typename XT, bool REFINE, int CSEL>
+static int launch_apply_c(f25_mesh *m, const double *om, const double *kv, int npair, const XT *x,
+                          XT *y, const BT *blk, const BT *blk_lo) {
     typedef ApplyLds<NL, BT, REFINE> LD;
+    constexpr size_t lds = CSEL < 0 ? LD::bytes : (5 * LD::per_blk + 3 * ApplyShape<NL>::NEN * sizeof(int));
     static bool attr_set = false;
-    auto kern = k_apply<NL, BT, XT, REFINE>;
+    auto kern = k_apply<NL, BT, XT, REFINE, CSEL>;
     if (!attr_set) {
-        HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)LD::bytes));
+        HIPCHK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         attr_set = true;
     }
     for (size_t c = 0; c + 1 < m->cptr.size(); ++c) {
         int n = m->cptr[c + 1] - m->cptr[c];
         if (!n) continue;
-        hipLaunchKernelGGL(kern, dim3(n), dim3(256), LD::bytes, m->st, m->d_order + m->cptr[c], blk,
-                           blk_lo, m->d_lm, npair, om, kv, x, y);
+        hipLaunchKernelGGL(kern, dim3(n), dim3(256), lds, m->st, m->d_order + m->cptr[c], blk, blk_lo,
+                           m->d_lm, m->d_first, npair, om, kv, x, y);
     }
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+template <int NL, typename BT, typename XT, bool REFINE>
+static int launch_apply(f25_mesh *m, const double *om, const double *kv, int npair, const XT *x,
+                        XT *y, const BT *blk, const BT *blk_lo) {
+    if constexpr (!ApplyLds<NL, BT, REFINE>::PH) {
+        return launch_apply_c<NL, BT, XT, REFINE, -1>(m, om, kv, npair, x, y, blk, blk_lo);
+    } else {
+        // one output component per pass; the colour order is kept within each pass
+        TRY((launch_apply_c<NL, BT, XT, REFINE, 0>(m, om, kv, npair, x, y, blk, blk_lo)));
+        TRY((launch_apply_c<NL, BT, XT, REFINE, 1>(m, om, kv, npair, x, y, blk, blk_lo)));
+        return launch_apply_c<NL, BT, XT, REFINE, 2>(m, om, kv, npair, x, y, blk, blk_lo);
+    }
 }
 
 template <typename BT, typename XT, bool REFINE>
@@ -610,14 +645,9 @@ static int dispatch_apply(f25_mesh *m, const double *om, const double *kv, int n
 
 static int apply_core(f25_mesh *m, int npair, const double *d_om, const double *d_ky, int mode,
                       const void *x, void *y, const double *blk64) {
-    const size_t n = (size_t)m->g.ndof * npair;
-    if (mode == F25_APPLY_F64 || mode == F25_APPLY_F32_REFINED)
-        fill_async((double2 *)y, n, make_double2(0.0, 0.0), m->st);
-    else if (mode == F25_APPLY_F32)
-        fill_async((float2 *)y, n, make_float2(0.f, 0.f), m->st);
-    else
+    // every DOF is written by the first element touching it: no zero fill
+    if (mode != F25_APPLY_F64 && mode != F25_APPLY_F32 && mode != F25_APPLY_F32_REFINED)
         FAIL(F25_ERR_ARG, "bad apply mode");
-    // the recorded time covers the product kernels only (not the zero fill)
     HIPCHK(hipEventRecord(m->ev0, m->st));
     if (mode == F25_APPLY_F64)
         TRY((dispatch_apply<double, double2, false>(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y,

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace f25k {
 
@@ -214,23 +215,46 @@ struct ApplyShape {
     static constexpr int KS = (NEN + 3) / 4;
 };
 
-// LDS bytes of k_apply for a shape; PH = blocks staged one output
-// component at a time (five blocks) when all nine do not fit.
+// LDS bytes of k_apply for a shape.  PH: the nine blocks do not fit, so the
+// product runs as three launches, one output component each, holding that
+// component's five blocks.  PRE: the tile's y values are loaded at the start
+// of the tile, next to the first x operands (one memory latency per tile).
 template <int NL, typename BT, bool REFINE>
 struct ApplyLds {
     typedef ApplyShape<NL> S;
     static constexpr size_t per_blk = (size_t)S::NEN * S::RS * sizeof(BT) * (REFINE ? 2 : 1);
     static constexpr bool PH = 9 * per_blk > 150 * 1024;
     static constexpr size_t bytes = (PH ? 5 : 9) * per_blk + 3 * S::NEN * sizeof(int);
+    static constexpr bool PRE = S::MT <= 2;
 };
 
+// value held by lane l^1 (the re/im partner of a column pair), quad_perm [1,0,3,2]
+__device__ inline double partner(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline float partner(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
 // XT: storage type of x/y (double2 or float2); BT: block/MFMA type.
-// REFINE: blocks as hi+lo floats, x split hi+lo, three products
-// (hi*hi + hi*lo + lo*hi) per 4-wide step, each partial summed in double.
-template <int NL, typename BT, typename XT, bool REFINE>
+// CSEL: -1 all three output components, 0..2 one component (PH launches).
+// REFINE: blocks as hi+lo floats, the double operand split hi+lo, three
+// products (hi*hi + hi*lo + lo*hi) per 4-wide step, each summed in double.
+// first[e] bit a: element e is the first (lowest colour) to touch node a, so
+// it writes y instead of adding to it (no zero fill of y is needed).
+//
+// Loop order per tile: k-steps s (4 element nodes each) outermost, with the
+// x operands of step s+1 loaded while step s runs; inside a step every
+// (component, sub-block, row tile) accumulator gets its MFMA, so the five
+// operand kinds of one x value are formed once and reused.
+template <int NL, typename BT, typename XT, bool REFINE, int CSEL>
 __global__ __launch_bounds__(256) void k_apply(const int *__restrict__ order,
                                                const BT *__restrict__ blk, const BT *__restrict__ blk_lo,
-                                               const int *__restrict__ lm, int npair,
+                                               const int *__restrict__ lm,
+                                               const unsigned long long *__restrict__ first, int npair,
                                                const double *__restrict__ omega,
                                                const double *__restrict__ ky,
                                                const XT *__restrict__ x, XT *__restrict__ y) {
@@ -238,11 +262,13 @@ __global__ __launch_bounds__(256) void k_apply(const int *__restrict__ order,
     typedef ApplyLds<NL, BT, REFINE> LD;
     constexpr int NEN = S::NEN, RS = S::RS, MT = S::MT, KS = S::KS;
     constexpr int NP = NEN * NEN;
-    constexpr bool PH = LD::PH;
-    constexpr int NSLOT = PH ? 5 : 9;
+    constexpr bool PRE = LD::PRE;
+    constexpr int NC = (CSEL < 0) ? 3 : 1;
+    constexpr int NSLOT = (CSEL < 0) ? 9 : 5;
     typedef MfmaTraits<BT> MT_;
     typedef typename MT_::v4 v4;
-    typedef decltype(XT().x) XS;
+    typedef decltype(XT().x) XS;                                        // float or double
+    typedef typename std::conditional<REFINE, double, XS>::type OS;     // operand arithmetic
     extern __shared__ __align__(16) unsigned char smem[];
     BT *sB = reinterpret_cast<BT *>(smem);
     BT *sBl = sB + (REFINE ? NSLOT * NEN * RS : 0);
@@ -250,116 +276,135 @@ __global__ __launch_bounds__(256) void k_apply(const int *__restrict__ order,
     const int e = order[blockIdx.x];
     const int tid = threadIdx.x;
     const size_t gbase = (size_t)e * NBLK * NP;
+    const unsigned long long fmask = first[e];
     for (int i = tid; i < 3 * NEN; i += 256) sLM[i] = lm[(size_t)e * 3 * NEN + i];
+    for (int i = tid; i < NSLOT * NP; i += 256) {
+        const int sl = i / NP, r = i % NP, a = r / NEN, b = r % NEN;
+        const int kb = (CSEL < 0) ? sl : kSub[CSEL < 0 ? 0 : CSEL][sl].blk;
+        sB[(sl * NEN + a) * RS + b] = blk[gbase + (size_t)kb * NP + r];
+        if (REFINE) sBl[(sl * NEN + a) * RS + b] = blk_lo[gbase + (size_t)kb * NP + r];
+    }
+    __syncthreads();
     const int lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, kg = lane >> 4, ri = col & 1;
     const int ntile = (npair + 7) / 8;
 #pragma unroll 1
-    for (int ph = 0; ph < (PH ? 3 : 1); ++ph) {
-        __syncthreads();
-        for (int i = tid; i < NSLOT * NP; i += 256) {
-            int sl = i / NP, r = i % NP, a = r / NEN, b = r % NEN;
-            int kb = PH ? kSub[ph][sl].blk : sl;
-            sB[(sl * NEN + a) * RS + b] = blk[gbase + (size_t)kb * NP + r];
-            if (REFINE) sBl[(sl * NEN + a) * RS + b] = blk_lo[gbase + (size_t)kb * NP + r];
-        }
-        __syncthreads();
-#pragma unroll 1
-        for (int t = wave; t < ntile; t += 4) {
-            const int pair = t * 8 + (col >> 1);
-            const bool live = pair < npair;
-            const double om = live ? omega[pair] : 0.0, kv = live ? ky[pair] : 0.0;
-            // this lane's x values: comp d, k-step s -> element node b = 4s + kg
-            double xr[3][KS], xi[3][KS];
+    for (int t = wave; t < ntile; t += 4) {
+        const int pair = t * 8 + (col >> 1);
+        const bool live = pair < npair;
+        const OS om = live ? (OS)omega[pair] : (OS)0, kv = live ? (OS)ky[pair] : (OS)0;
+        const OS f1 = ri ? kv : -kv, f3 = kv * kv, f4 = -om * om;
+        auto load_x = [&](int s, OS *dst) {
+            const int b = 4 * s + kg;
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
+            for (int d = 0; d < 3; ++d) {
+                const int dof = (b < NEN && live) ? sLM[b * 3 + d] : -1;
+                dst[d] = dof >= 0 ? (OS) reinterpret_cast<const XS *>(x + (size_t)dof * npair + pair)[ri]
+                                  : (OS)0;
+            }
+        };
+        OS xn[3];
+        load_x(0, xn);
+        XS yv[PRE ? NC : 1][PRE ? MT : 1][4];
+        if (PRE) {
 #pragma unroll
-                for (int s = 0; s < KS; ++s) {
-                    const int b = 4 * s + kg;
-                    const int dof = (b < NEN) ? sLM[b * 3 + d] : -1;
-                    if (dof >= 0 && live) {
-                        XT v = x[(size_t)dof * npair + pair];
-                        xr[d][s] = v.x;
-                        xi[d][s] = v.y;
-                    } else {
-                        xr[d][s] = 0.0;
-                        xi[d][s] = 0.0;
-                    }
-                }
-            // per-column factors of the five operand kinds; kinds 1, 2 swap re/im
-            const double fac[5] = {1.0, ri ? kv : -kv, ri ? -kv : kv, kv * kv, -om * om};
+            for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
-            for (int cc = 0; cc < (PH ? 1 : 3); ++cc) {
-                const int c = PH ? ph : cc;
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) {
-                    const int arow = mt * 16 + col;
-                    double accd[4] = {0.0, 0.0, 0.0, 0.0};
-                    v4 acc = {0, 0, 0, 0};
-#pragma unroll
-                    for (int j = 0; j < 5; ++j) {
-                        // compile-time table entries once c is known; with PH the
-                        // component is a runtime value, so branch on it
-                        int blkid, trans, xc, kind;
-                        if (PH) {
-                            blkid = j;
-                            trans = (c == 0) ? kSub[0][j].trans : (c == 1 ? kSub[1][j].trans : kSub[2][j].trans);
-                            xc = (c == 0) ? kSub[0][j].xcomp : (c == 1 ? kSub[1][j].xcomp : kSub[2][j].xcomp);
-                            kind = (c == 0) ? kSub[0][j].kind : (c == 1 ? kSub[1][j].kind : kSub[2][j].kind);
-                        } else {
-                            blkid = kSub[cc][j].blk;
-                            trans = kSub[cc][j].trans;
-                            xc = kSub[cc][j].xcomp;
-                            kind = kSub[cc][j].kind;
-                        }
-                        const BT *A = sB + blkid * NEN * RS;
-                        const BT *Al = sBl + blkid * NEN * RS;
-                        const double f = fac[kind];
-                        const bool sw = (kind == 1 || kind == 2);
-#pragma unroll
-                        for (int s = 0; s < KS; ++s) {
-                            const int k = 4 * s + kg;
-                            BT av = 0, avl = 0;
-                            if (arow < NEN && k < NEN) {
-                                const int idx = trans ? (k * RS + arow) : (arow * RS + k);
-                                av = A[idx];
-                                if (REFINE) avl = Al[idx];
-                            }
-                            double x0 = 0.0, x1 = 0.0;
-#pragma unroll
-                            for (int d = 0; d < 3; ++d)
-                                if (d == xc) { x0 = xr[d][s]; x1 = xi[d][s]; }
-                            const double xv = sw ? (ri ? x0 : x1) : (ri ? x1 : x0);
-                            const double bv = f * xv;
-                            if (REFINE) {
-                                const float bh = (float)bv, bl = (float)(bv - (double)bh);
-                                v4 z = {0, 0, 0, 0};
-                                z = MT_::mma(av, bh, z);
-                                z = MT_::mma(av, bl, z);
-                                z = MT_::mma(avl, bh, z);
-#pragma unroll
-                                for (int i = 0; i < 4; ++i) accd[i] += (double)z[i];
-                            } else {
-                                acc = MT_::mma(av, (BT)bv, acc);
-                            }
-                        }
-                    }
-                    // result rows of this lane (see MfmaTraits), column col
+                for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
+                        const int c = (CSEL < 0) ? cc : CSEL;
                         const int row = mt * 16 + MT_::row(kg, i);
-                        if (row < NEN && live) {
-                            const int dof = sLM[row * 3 + c];
-                            if (dof >= 0) {
-                                const double v = REFINE ? accd[i] : (double)acc[i];
-                                XS *py = reinterpret_cast<XS *>(y + (size_t)dof * npair + pair);
-                                py[ri] += (XS)v;
-                            }
+                        const int dof = (row < NEN && live) ? sLM[row * 3 + c] : -1;
+                        const bool add = dof >= 0 && !((fmask >> row) & 1ull);
+                        yv[PRE ? cc : 0][PRE ? mt : 0][i] =
+                            add ? reinterpret_cast<const XS *>(y + (size_t)dof * npair + pair)[ri] : (XS)0;
+                    }
+        }
+        v4 acc[NC][MT];
+        double accd[REFINE ? NC : 1][REFINE ? MT : 1][4];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                acc[cc][mt] = v4{0, 0, 0, 0};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) accd[REFINE ? cc : 0][REFINE ? mt : 0][i] = 0.0;
+            }
+#pragma unroll 1
+        for (int s = 0; s < KS; ++s) {
+            const OS xv[3] = {xn[0], xn[1], xn[2]};
+            if (s + 1 < KS) load_x(s + 1, xn);
+            const int k = min(4 * s + kg, NEN - 1);
+            // operand kinds of each x component: 0 x, 1 +i ky x, 2 -i ky x, 3 ky^2 x, 4 -w^2 x
+            OS bop[3][5];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const OS p = partner(xv[d]);
+                bop[d][0] = xv[d];
+                bop[d][1] = f1 * p;
+                bop[d][2] = -f1 * p;
+                bop[d][3] = f3 * xv[d];
+                bop[d][4] = f4 * xv[d];
+            }
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                constexpr int C0 = (CSEL < 0) ? 0 : CSEL;
+#pragma unroll
+                for (int j = 0; j < 5; ++j) {
+                    const SubBlk sb = kSub[(CSEL < 0) ? cc : C0][j];
+                    const int slot = (CSEL < 0) ? sb.blk : j;
+                    const BT *A = sB + slot * NEN * RS;
+                    const BT *Al = sBl + slot * NEN * RS;
+                    const OS bv = bop[sb.xcomp][sb.kind];
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt) {
+                        // rows >= NEN give result rows that are never stored and the
+                        // operand is zero for k >= NEN: indices are clamped, not masked
+                        const int arow = min(mt * 16 + col, NEN - 1);
+                        const int idx = sb.trans ? (k * RS + arow) : (arow * RS + k);
+                        if (REFINE) {
+                            const float bh = (float)bv, bl = (float)((double)bv - (double)bh);
+                            const BT av = A[idx], avl = Al[idx];
+                            v4 z = {0, 0, 0, 0};
+                            z = MT_::mma(av, bh, z);
+                            z = MT_::mma(av, bl, z);
+                            z = MT_::mma(avl, bh, z);
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) accd[REFINE ? cc : 0][REFINE ? mt : 0][i] += (double)z[i];
+                        } else {
+                            acc[cc][mt] = MT_::mma(A[idx], (BT)bv, acc[cc][mt]);
                         }
                     }
                 }
             }
         }
+        // rows of this lane: add into y (first touch writes)
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int c = (CSEL < 0) ? cc : CSEL;
+                XS old[4];
+                int dofs[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int row = mt * 16 + MT_::row(kg, i);
+                    dofs[i] = (row < NEN && live) ? sLM[row * 3 + c] : -1;
+                    if (PRE) {
+                        old[i] = yv[PRE ? cc : 0][PRE ? mt : 0][i];
+                    } else {
+                        const bool add = dofs[i] >= 0 && !((fmask >> row) & 1ull);
+                        old[i] = add ? reinterpret_cast<const XS *>(y + (size_t)dofs[i] * npair + pair)[ri]
+                                     : (XS)0;
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const XS v = (XS)(REFINE ? accd[REFINE ? cc : 0][REFINE ? mt : 0][i] : (double)acc[cc][mt][i]);
+                    if (dofs[i] >= 0) reinterpret_cast<XS *>(y + (size_t)dofs[i] * npair + pair)[ri] = old[i] + v;
+                }
+            }
     }
 }
 
