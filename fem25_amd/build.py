@@ -23,6 +23,9 @@ ARCH = os.environ.get("FEM25_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
          # no packed FP32 instructions anywhere in device code
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # MFMA accumulators in ordinary VGPRs: without it the loop-carried
+         # accumulators of k_apply are copied VGPR<->AGPR on every k step
+         "-mllvm", "-amdgpu-mfma-vgpr-form",
          "-Wno-unused-result"]
 
 

@@ -4,7 +4,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/fem25hip.h"
+#include "fem25hip.h"
 
 namespace f25 {
 

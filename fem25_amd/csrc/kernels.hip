@@ -294,13 +294,21 @@ __global__ __launch_bounds__(256) void k_apply(const int *__restrict__ order,
         const bool live = pair < npair;
         const OS om = live ? (OS)omega[pair] : (OS)0, kv = live ? (OS)ky[pair] : (OS)0;
         const OS f1 = ri ? kv : -kv, f3 = kv * kv, f4 = -om * om;
+        // branch-free: out-of-range nodes and fixed DOFs read a valid row
+        // (row 0, column 0) and are zeroed afterwards, so the loads of a step
+        // issue back to back
+        const int pcl = live ? pair : 0;
         auto load_x = [&](int s, OS *dst) {
             const int b = 4 * s + kg;
+            const bool inb = b < NEN && live;
+            int dofs[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) dofs[d] = sLM[min(b, NEN - 1) * 3 + d];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const int dof = (b < NEN && live) ? sLM[b * 3 + d] : -1;
-                dst[d] = dof >= 0 ? (OS) reinterpret_cast<const XS *>(x + (size_t)dof * npair + pair)[ri]
-                                  : (OS)0;
+                const bool ok = inb && dofs[d] >= 0;
+                const XS v = reinterpret_cast<const XS *>(x + (size_t)(ok ? dofs[d] : 0) * npair + pcl)[ri];
+                dst[d] = ok ? (OS)v : (OS)0;
             }
         };
         OS xn[3];
@@ -315,10 +323,10 @@ __global__ __launch_bounds__(256) void k_apply(const int *__restrict__ order,
                     for (int i = 0; i < 4; ++i) {
                         const int c = (CSEL < 0) ? cc : CSEL;
                         const int row = mt * 16 + MT_::row(kg, i);
-                        const int dof = (row < NEN && live) ? sLM[row * 3 + c] : -1;
-                        const bool add = dof >= 0 && !((fmask >> row) & 1ull);
-                        yv[PRE ? cc : 0][PRE ? mt : 0][i] =
-                            add ? reinterpret_cast<const XS *>(y + (size_t)dof * npair + pair)[ri] : (XS)0;
+                        const int dof = sLM[min(row, NEN - 1) * 3 + c];
+                        const bool add = row < NEN && live && dof >= 0 && !((fmask >> row) & 1ull);
+                        const XS v = reinterpret_cast<const XS *>(y + (size_t)(add ? dof : 0) * npair + pcl)[ri];
+                        yv[PRE ? cc : 0][PRE ? mt : 0][i] = add ? v : (XS)0;
                     }
         }
         v4 acc[NC][MT];
