@@ -48,6 +48,12 @@ EXPORTS = {
     "f25_inv_setup": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_int, C.c_int, C.c_int,
                                 ip, ip]),
     "f25_inv_get": (C.c_int, [C.c_void_p, ip, ip, ip, ip, dp]),
+    "f25_inv_attach": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_int, C.c_int, ip, ip,
+                                 C.c_int, ip, ip, C.c_int, ip, C.c_int, dp]),
+    "f25_asmb_peff_csr": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_char_p, ip, ip, ip, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]),
+    "f25_adjflam_csc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, ip, C.c_void_p,
+                                  C.c_void_p, fp]),
     "f25_cfmat_dist": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
     "f25_adjflam_dist": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, fp]),
     "f25_gn_fv": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp, C.c_void_p]),

@@ -438,6 +438,11 @@ extern "C" int f25_set_model(f25_mesh *m, const double *dens, const double *ecoe
     if (!m || !dens || !ecoeff) FAIL(F25_ERR_ARG, "null argument");
     HIPCHK(hipSetDevice(m->dev));
     const int n = m->d.nnpg;
+    // unchanged model: the element blocks on the device are current
+    if (m->have_model && std::equal(dens, dens + n, m->rho.begin()) &&
+        std::equal(ecoeff, ecoeff + n, m->lam.begin()) && std::equal(ecoeff + n, ecoeff + 2 * n, m->mu.begin()))
+        return 0;
+    m->have_model = false;
     m->rho.assign(dens, dens + n);
     m->lam.assign(ecoeff, ecoeff + n);
     m->mu.assign(ecoeff + n, ecoeff + 2 * n);
@@ -585,8 +590,9 @@ extern "C" int f25_asmb_peff(f25_mesh *m, const void *cdist, const void *ue, voi
     TRY(upload(&ds, (const float2 *)cdist, nz, m->st));
     TRY(upload(&du, (const float2 *)ue, ndof, m->st));
     TRY(dalloc(&dp, ndof));
-    hipLaunchKernelGGL(k_peff, dim3((ndof + 3) / 4), dim3(256), 0, m->st, ndof, m->d_irptr,
-                       m->d_jcptr, m->d_cnp, ds, du, dp);
+    fill_async(dp, (size_t)ndof, make_float2(0.f, 0.f), m->st);
+    hipLaunchKernelGGL(k_peff, dim3((ndof + 3) / 4), dim3(256), 0, m->st, ndof, (const int *)nullptr,
+                       m->d_irptr, m->d_jcptr, m->d_cnp, ds, du, dp);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(peff, dp, ndof * sizeof(float2), hipMemcpyDeviceToHost, m->st);
     if (e == hipSuccess) e = hipStreamSynchronize(m->st);
@@ -779,6 +785,19 @@ extern "C" int f25_gradient(f25_mesh *m, int ntrip, const double *omega, const d
 }
 
 // ---------------------------------------------------------------------------
+static int inv_upload(f25_mesh *m) {
+    m->have_inv = false;
+    dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_ew);
+    dfree(m->d_col_elem); dfree(m->d_col_corner); dfree(m->d_col_cp); dfree(m->d_col_cs);
+    TRY(upload(&m->d_mygrad, m->inv.mygrad.data(), m->inv.mygrad.size(), m->st));
+    TRY(upload(&m->d_jcsc, m->inv.jcsc.data(), m->inv.jcsc.size(), m->st));
+    TRY(upload(&m->d_icsc, m->inv.icsc.data(), m->inv.icsc.size(), m->st));
+    TRY(upload(&m->d_ew, m->inv.elem_wts.data(), m->inv.elem_wts.size(), m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    m->have_inv = true;
+    return 0;
+}
+
 extern "C" int f25_inv_setup(f25_mesh *m, const char *cinvtype, double vpvs, int linrec, int lkbdry,
                              int nrec, const int *mrdof1, int sizes[6]) {
     if (!m || !cinvtype) FAIL(F25_ERR_ARG, "null argument");
@@ -793,19 +812,63 @@ extern "C" int f25_inv_setup(f25_mesh *m, const char *cinvtype, double vpvs, int
     inv.lisp = lisp;
     inv.vpvs = vpvs;
     m->inv = std::move(inv);
-    m->have_inv = true;
     if (sizes) {
         sizes[0] = m->inv.nnpinv; sizes[1] = m->inv.ncon; sizes[2] = m->inv.nnpgl;
         sizes[3] = m->inv.nz_fdist; sizes[4] = m->inv.mbufrhs; sizes[5] = m->inv.na35;
     }
-    dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_ew);
-    dfree(m->d_col_elem); dfree(m->d_col_corner); dfree(m->d_col_cp); dfree(m->d_col_cs);
-    TRY(upload(&m->d_mygrad, m->inv.mygrad.data(), m->inv.mygrad.size(), m->st));
-    TRY(upload(&m->d_jcsc, m->inv.jcsc.data(), m->inv.jcsc.size(), m->st));
-    TRY(upload(&m->d_icsc, m->inv.icsc.data(), m->inv.icsc.size(), m->st));
-    TRY(upload(&m->d_ew, m->inv.elem_wts.data(), m->inv.elem_wts.size(), m->st));
-    HIPCHK(hipStreamSynchronize(m->st));
-    return 0;
+    return inv_upload(m);
+}
+
+extern "C" int f25_inv_attach(f25_mesh *m, const char *cinvtype, double vpvs, int nvinv, int ncon,
+                              const int *mconn, const int *maskg, int nnpgl, const int *mygrad,
+                              const int *jcsc, int nz_fdist, const int *icsc, int na35,
+                              const double *elem_wts) {
+    if (!m || !cinvtype || !mconn || !maskg || !mygrad || !jcsc || !icsc || !elem_wts)
+        FAIL(F25_ERR_ARG, "null argument");
+    char c0 = (char)toupper(cinvtype[0]), c1 = (char)toupper(cinvtype[1]);
+    bool lisp = c0 == 'P' && c1 == 'P', liss = c0 == 'S' && c1 == 'S';
+    if (!lisp && !liss) FAIL(F25_ERR_INVTYPE, "CINVTYPE must be PP or SS (vforce25.f:66-85)");
+    if (nvinv != 1) FAIL(F25_ERR_INVTYPE, "NVINV must be 1 for PP/SS inversions");
+    const int nnpg = m->d.nnpg, nelem = m->d.nelem, ndof = m->g.ndof;
+    if (ncon <= 0 || nnpgl <= 0 || nz_fdist < 0 || na35 <= 0) FAIL(F25_ERR_ARG, "bad sizes");
+    if (jcsc[0] != 1 || jcsc[nnpgl] - 1 != nz_fdist) FAIL(F25_ERR_ARG, "JCSC_FDIST inconsistent with NZ_FDIST");
+    InvInfo inv;
+    inv.nvinv = 1;
+    inv.ncon = ncon;
+    inv.nnpgl = nnpgl;
+    inv.nz_fdist = nz_fdist;
+    inv.na35 = na35;
+    inv.lisp = lisp;
+    inv.vpvs = vpvs;
+    inv.mconn.assign(mconn, mconn + (size_t)ncon * nnpg);
+    inv.maskg.assign(maskg, maskg + nnpg);
+    inv.mygrad.assign(mygrad, mygrad + nnpgl);
+    inv.jcsc.assign(jcsc, jcsc + nnpgl + 1);
+    inv.icsc.assign(icsc, icsc + nz_fdist);
+    inv.elem_wts.assign(elem_wts, elem_wts + nelem);
+    // anchor of each column: MASKG(INPG) == MYGRAD(INPGL) (hessloc.f90:66-72, NVINV = 1)
+    std::vector<int> anchor_of(std::max(na35, 1) + 1, -1);
+    for (int a = 0; a < nnpg; ++a) {
+        const int v = maskg[a];
+        if (v > 0 && v <= na35 && anchor_of[v] < 0) anchor_of[v] = a;
+        if (v > 0) ++inv.nnpinv;
+    }
+    inv.node_of_col.assign(nnpgl, 0);
+    for (int c = 0; c < nnpgl; ++c) {
+        const int v = mygrad[c];
+        if (v <= 0 || v > na35 || anchor_of[v] < 0) FAIL(F25_ERR_ARG, "cannot find the anchor of a column");
+        inv.node_of_col[c] = anchor_of[v];
+        const int len = jcsc[c + 1] - jcsc[c];
+        if (len < 0) FAIL(F25_ERR_ARG, "JCSC_FDIST not ascending");
+        inv.mbufrhs = std::max(inv.mbufrhs, len);
+    }
+    for (int i = 0; i < nz_fdist; ++i)
+        if (icsc[i] < 1 || icsc[i] > ndof) FAIL(F25_ERR_ARG, "ICSC_FDIST out of range");
+    for (size_t i = 0; i < inv.mconn.size(); ++i)
+        if (inv.mconn[i] < 0 || inv.mconn[i] > nelem) FAIL(F25_ERR_ARG, "MCONN out of range");
+    HIPCHK(hipSetDevice(m->dev));
+    m->inv = std::move(inv);
+    return inv_upload(m);
 }
 
 extern "C" int f25_inv_get(f25_mesh *m, int *maskg, int *mygrad, int *jcsc, int *icsc, double *ew) {
@@ -901,6 +964,103 @@ extern "C" int f25_adjflam_dist(f25_mesh *m, const void *fmat, const void *adjla
     dfree(df); dfree(dl); dfree(dg);
     if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Handle-free forms for callers that hold only the sparse structures
+// (ASMB_PEFF and ADJFLAM_DIST receive no mesh).  They run on the current
+// device, on a stream of their own, and allocate per call.
+struct TmpStream {
+    hipStream_t s = nullptr;
+    ~TmpStream() { if (s) (void)hipStreamDestroy(s); }
+};
+
+extern "C" int f25_asmb_peff_csr(int ndof, int ndofl, int nzloc, const char *cnp, const int *mydofs,
+                                 const int *irptr_loc, const int *jcptr_loc, const void *cdist,
+                                 const void *ue, void *buff) {
+    if (ndof <= 0 || ndofl < 0 || ndofl > ndof || nzloc < 0 || !cnp || !mydofs || !irptr_loc ||
+        !jcptr_loc || !cdist || !ue || !buff)
+        FAIL(F25_ERR_ARG, "invalid arguments");
+    if (irptr_loc[0] != 1 || irptr_loc[ndofl] - 1 != nzloc) FAIL(F25_ERR_ARG, "IRPTR_LOC inconsistent with NZLOC");
+    std::vector<int64_t> ir(ndofl + 1);
+    for (int i = 0; i <= ndofl; ++i) {
+        ir[i] = (int64_t)irptr_loc[i] - 1;
+        if (i && ir[i] < ir[i - 1]) FAIL(F25_ERR_ARG, "IRPTR_LOC not ascending");
+    }
+    std::vector<int> jc(nzloc);
+    for (int i = 0; i < nzloc; ++i) {
+        if (jcptr_loc[i] < 1 || jcptr_loc[i] > ndof) FAIL(F25_ERR_ARG, "JCPTR_LOC out of range");
+        jc[i] = jcptr_loc[i] - 1;
+    }
+    for (int i = 0; i < ndofl; ++i)
+        if (mydofs[i] < 1 || mydofs[i] > ndof) FAIL(F25_ERR_ARG, "MYDOFS out of range");
+    TmpStream ts;
+    HIPCHK(hipStreamCreate(&ts.s));
+    int64_t *d_ir = nullptr;
+    int *d_jc = nullptr, *d_my = nullptr;
+    char *d_cnp = nullptr;
+    float2 *ds = nullptr, *du = nullptr, *dp = nullptr;
+    int r = 0;
+    r = upload(&d_ir, ir.data(), ir.size(), ts.s);
+    if (!r) r = upload(&d_jc, jc.data(), jc.size(), ts.s);
+    if (!r) r = upload(&d_my, mydofs, (size_t)ndofl, ts.s);
+    if (!r) r = upload(&d_cnp, cnp, (size_t)ndof, ts.s);
+    if (!r) r = upload(&ds, (const float2 *)cdist, (size_t)nzloc, ts.s);
+    if (!r) r = upload(&du, (const float2 *)ue, (size_t)ndof, ts.s);
+    if (!r) r = dalloc(&dp, (size_t)ndof);
+    if (!r) {
+        fill_async(dp, (size_t)ndof, make_float2(0.f, 0.f), ts.s);
+        if (ndofl > 0)
+            hipLaunchKernelGGL(k_peff, dim3((ndofl + 3) / 4), dim3(256), 0, ts.s, ndofl, d_my, d_ir, d_jc,
+                               d_cnp, ds, du, dp);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(buff, dp, ndof * sizeof(float2), hipMemcpyDeviceToHost, ts.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(ts.s);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(d_ir); dfree(d_jc); dfree(d_my); dfree(d_cnp); dfree(ds); dfree(du); dfree(dp);
+    return r;
+}
+
+extern "C" int f25_adjflam_csc(int nnpgl, int ndof, int nz_fdist, int na35, const int *mygrad,
+                               const int *jcsc, const int *icsc, const void *fmat, const void *adjlam,
+                               float *vwork) {
+    if (nnpgl < 0 || ndof <= 0 || nz_fdist < 0 || na35 <= 0 || !mygrad || !jcsc || !icsc || !fmat ||
+        !adjlam || !vwork)
+        FAIL(F25_ERR_ARG, "invalid arguments");
+    if (jcsc[0] != 1 || jcsc[nnpgl] - 1 != nz_fdist) FAIL(F25_ERR_ARG, "JCSC_FDIST inconsistent with NZ_FDIST");
+    std::vector<char> seen(na35, 0);
+    for (int c = 0; c < nnpgl; ++c) {
+        if (mygrad[c] < 1 || mygrad[c] > na35) FAIL(F25_ERR_ARG, "MYGRAD out of range");
+        if (seen[mygrad[c] - 1]) FAIL(F25_ERR_ARG, "MYGRAD repeats a gradient point");
+        seen[mygrad[c] - 1] = 1;
+        if (jcsc[c + 1] < jcsc[c]) FAIL(F25_ERR_ARG, "JCSC_FDIST not ascending");
+    }
+    for (int i = 0; i < nz_fdist; ++i)
+        if (icsc[i] < 1 || icsc[i] > ndof) FAIL(F25_ERR_ARG, "ICSC_FDIST out of range");
+    TmpStream ts;
+    HIPCHK(hipStreamCreate(&ts.s));
+    int *d_my = nullptr, *d_jc = nullptr, *d_ic = nullptr;
+    float2 *df = nullptr, *dl = nullptr;
+    float *dg = nullptr;
+    int r = upload(&d_my, mygrad, (size_t)nnpgl, ts.s);
+    if (!r) r = upload(&d_jc, jcsc, (size_t)nnpgl + 1, ts.s);
+    if (!r) r = upload(&d_ic, icsc, (size_t)nz_fdist, ts.s);
+    if (!r) r = upload(&df, (const float2 *)fmat, (size_t)nz_fdist, ts.s);
+    if (!r) r = upload(&dl, (const float2 *)adjlam, (size_t)ndof, ts.s);
+    if (!r) r = dalloc(&dg, (size_t)na35);
+    if (!r) {
+        fill_async(dg, (size_t)na35, 0.f, ts.s);
+        if (nnpgl > 0)
+            hipLaunchKernelGGL(k_adjflam, dim3((nnpgl + 3) / 4), dim3(256), 0, ts.s, nnpgl, d_my, d_jc, d_ic,
+                               df, dl, dg);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(vwork, dg, na35 * sizeof(float), hipMemcpyDeviceToHost, ts.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(ts.s);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(d_my); dfree(d_jc); dfree(d_ic); dfree(df); dfree(dl); dfree(dg);
+    return r;
 }
 
 // ---------------------------------------------------------------------------

@@ -294,21 +294,16 @@ __global__ __launch_bounds__(256) void k_apply(const int *__restrict__ order,
         const bool live = pair < npair;
         const OS om = live ? (OS)omega[pair] : (OS)0, kv = live ? (OS)ky[pair] : (OS)0;
         const OS f1 = ri ? kv : -kv, f3 = kv * kv, f4 = -om * om;
-        // branch-free: out-of-range nodes and fixed DOFs read a valid row
-        // (row 0, column 0) and are zeroed afterwards, so the loads of a step
-        // issue back to back
+        // the y preload is branch-free: rows outside the element, fixed DOFs
+        // and first touches read a valid address (row 0) and are zeroed
         const int pcl = live ? pair : 0;
         auto load_x = [&](int s, OS *dst) {
             const int b = 4 * s + kg;
-            const bool inb = b < NEN && live;
-            int dofs[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) dofs[d] = sLM[min(b, NEN - 1) * 3 + d];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const bool ok = inb && dofs[d] >= 0;
-                const XS v = reinterpret_cast<const XS *>(x + (size_t)(ok ? dofs[d] : 0) * npair + pcl)[ri];
-                dst[d] = ok ? (OS)v : (OS)0;
+                const int dof = (b < NEN && live) ? sLM[b * 3 + d] : -1;
+                dst[d] = dof >= 0 ? (OS) reinterpret_cast<const XS *>(x + (size_t)dof * npair + pair)[ri]
+                                  : (OS)0;
             }
         };
         OS xn[3];
@@ -638,15 +633,19 @@ __global__ void k_spmv(int ndof, const int64_t *__restrict__ irptr, const int *_
     if (l == 0) { y[r].x = ar; y[r].y = ai; }
 }
 
-// Bielak effective force (asmble.f90:357-373 restricted to B/E couplings)
-__global__ void k_peff(int ndof, const int64_t *__restrict__ irptr, const int *__restrict__ jcptr,
-                       const char *__restrict__ cnp, const float2 *__restrict__ s,
-                       const float2 *__restrict__ ue, float2 *__restrict__ out) {
+// Bielak effective force (asmble.f90:357-373 restricted to B/E couplings).
+// Local row r is global DOF mydofs[r]-1 (identity when mydofs is null);
+// out is indexed by global DOF, jcptr holds 0-based global columns.
+__global__ void k_peff(int ndofl, const int *__restrict__ mydofs, const int64_t *__restrict__ irptr,
+                       const int *__restrict__ jcptr, const char *__restrict__ cnp,
+                       const float2 *__restrict__ s, const float2 *__restrict__ ue,
+                       float2 *__restrict__ out) {
     const int wpb = blockDim.x / 64;
     const int r = blockIdx.x * wpb + threadIdx.x / 64;
     const int l = threadIdx.x & 63;
-    if (r >= ndof) return;
-    const char c1 = cnp[r];
+    if (r >= ndofl) return;
+    const int gr = mydofs ? mydofs[r] - 1 : r;
+    const char c1 = cnp[gr];
     float ar = 0.f, ai = 0.f;
     if (c1 == 'B' || c1 == 'E') {
         for (int64_t k = irptr[r] + l; k < irptr[r + 1]; k += 64) {
@@ -664,7 +663,7 @@ __global__ void k_peff(int ndof, const int64_t *__restrict__ irptr, const int *_
         ar += __shfl_xor(ar, o);
         ai += __shfl_xor(ai, o);
     }
-    if (l == 0) out[r] = make_float2(ar, ai);
+    if (l == 0) out[gr] = make_float2(ar, ai);
 }
 
 // ---------------------------------------------------------------------------

@@ -94,7 +94,8 @@ int f25_mesh_sizes(f25_mesh *m, int64_t sizes[6]);
  * NULL): IEN(NEN,NELEM), LM(3,NEN,NELEM), IRPTR(NDOF+1), JCPTR(NZ), CNP(NDOF). */
 int f25_mesh_graph(f25_mesh *m, int *ien, int *lm, int *irptr, int *jcptr, char *cnp);
 /* DENS(NNPG) kg/m^3 and ECOEFF(NNPG,2) = (lambda, mu) Pa.  Recomputes the
- * element blocks on the device. */
+ * element blocks on the device; a call with the model already set returns
+ * at once (so drop-in shims may call it before every product). */
 int f25_set_model(f25_mesh *m, const double *dens, const double *ecoeff);
 
 /* ---- assembled operator (replaces ASMBLE_DRIVER, asmble.f90:1-231) ------ */
@@ -151,11 +152,41 @@ int f25_inv_setup(f25_mesh *m, const char *cinvtype, double vpvs, int linrec, in
 /* MASKG(NNPG), MYGRAD(NNPGL), JCSC_FDIST(NNPGL+1), ICSC_FDIST(NZ_FDIST),
  * ELEM_WTS(NELEM); any pointer may be NULL. */
 int f25_inv_get(f25_mesh *m, int *maskg, int *mygrad, int *jcsc, int *icsc, double *elem_wts);
+/* Inversion structures as the caller already holds them (the INV_INFO
+ * fields fwd_struc.h:249-368 after GENGMASK/GRADPTRS/ELEM_WTS, whatever mask
+ * file built them), for the drop-in CFMAT_DIST: MCONN(NNPG,NCON),
+ * MASKG(NNPG), MYGRAD(NNPGL), JCSC_FDIST(NNPGL+1), ICSC_FDIST(NZ_FDIST),
+ * ELEM_WTS(NELEM), all 1-based.  Column INPGL inverts at the anchor INPG
+ * with MASKG(INPG) == MYGRAD(INPGL) (hessloc.f90:66-72).  NVINV must be 1
+ * (CINVTYPE PP or SS, vforce25.f:66-85). */
+int f25_inv_attach(f25_mesh *m, const char *cinvtype, double vpvs, int nvinv, int ncon,
+                   const int *mconn, const int *maskg, int nnpgl, const int *mygrad, const int *jcsc,
+                   int nz_fdist, const int *icsc, int na35, const double *elem_wts);
 /* CFMAT_DIST (hessloc.f90:1-133 + vforce25.f:1-369): FMAT_DIST(NZ_FDIST)
  * COMPLEX*8 from WAVE(NDOF) COMPLEX*8, reference semantics. */
 int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *wave, void *fmat);
 /* ADJFLAM_DIST (adjgrad.f90:88-145, one rank): GPROD(NA35) REAL*4. */
 int f25_adjflam_dist(f25_mesh *m, const void *fmat, const void *adjlam, float *gprod);
+
+/* ---- handle-free sparse forms -------------------------------------------- */
+
+/* ASMB_PEFF's local product (asmble.f90:348-374) on the caller's sparse
+ * structures, before the reduction: BUFF(NDOF) = 0, then for local row
+ * IDOFL (global DOF MYDOFS(IDOFL)) with CNP 'B'/'E', BUFF(IDOF) -= S u_E over
+ * 'E' columns of a 'B' row and += S u_E over 'B' columns of an 'E' row.
+ * CNP(NDOF) chars, MYDOFS(NDOFL), IRPTR_LOC(NDOFL+1), JCPTR_LOC(NZLOC) 1-based;
+ * cdist COMPLEX*8(NZLOC), ue and buff COMPLEX*8(NDOF) host arrays.  The caller
+ * sums BUFF over ranks (the reference's MPI_REDUCE, asmble.f90:374). */
+int f25_asmb_peff_csr(int ndof, int ndofl, int nzloc, const char *cnp, const int *mydofs,
+                      const int *irptr_loc, const int *jcptr_loc, const void *cdist, const void *ue,
+                      void *buff);
+/* ADJFLAM_DIST's local product (adjgrad.f90:127-139) before the reduction:
+ * VWORK(NA35) = 0, VWORK(MYGRAD(c)) -= sum over column c of
+ * Re(conj(FMAT_DIST(i)) ADJLAM(ICSC_FDIST(i))).  fmat COMPLEX*8(NZ_FDIST),
+ * adjlam COMPLEX*8(NDOF), vwork REAL*4(NA35) host arrays; MYGRAD entries
+ * distinct.  The caller reduces VWORK onto its master (adjgrad.f90:141). */
+int f25_adjflam_csc(int nnpgl, int ndof, int nz_fdist, int na35, const int *mygrad, const int *jcsc,
+                    const int *icsc, const void *fmat, const void *adjlam, float *vwork);
 
 /* ---- Gauss-Newton element products ---------------------------------------- */
 
