@@ -616,7 +616,8 @@ static int launch_apply_c(f25_mesh *m, const double *om, const double *kv, int n
     for (size_t c = 0; c + 1 < m->cptr.size(); ++c) {
         int n = m->cptr[c + 1] - m->cptr[c];
         if (!n) continue;
-        hipLaunchKernelGGL(kern, dim3(n), dim3(256), lds, m->st, m->d_order + m->cptr[c], blk, blk_lo,
+        hipLaunchKernelGGL(kern, dim3(n), dim3(64 * ApplyLds<NL, BT, REFINE>::NW), lds, m->st,
+                           m->d_order + m->cptr[c], blk, blk_lo,
                            m->d_lm, m->d_first, npair, om, kv, x, y);
     }
     HIPCHK(hipGetLastError());

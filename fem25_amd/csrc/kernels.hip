@@ -219,6 +219,9 @@ struct ApplyShape {
 // product runs as three launches, one output component each, holding that
 // component's five blocks.  PRE: the tile's y values are loaded at the start
 // of the tile, next to the first x operands (one memory latency per tile).
+// NW: waves per workgroup, chosen so that a CU (160 KB of LDS) holds about
+// twelve waves whatever the footprint: 4 waves when three elements fit, 12
+// when only one does.
 template <int NL, typename BT, bool REFINE>
 struct ApplyLds {
     typedef ApplyShape<NL> S;
@@ -226,6 +229,8 @@ struct ApplyLds {
     static constexpr bool PH = 9 * per_blk > 150 * 1024;
     static constexpr size_t bytes = (PH ? 5 : 9) * per_blk + 3 * S::NEN * sizeof(int);
     static constexpr bool PRE = S::MT <= 2;
+    static constexpr int per_cu = (int)((160 * 1024) / bytes);
+    static constexpr int NW = per_cu >= 3 ? 4 : (per_cu == 2 ? 6 : 12);
 };
 
 // value held by lane l^1 (the re/im partner of a column pair), quad_perm [1,0,3,2]
@@ -251,7 +256,7 @@ __device__ inline float partner(float v) {
 // (component, sub-block, row tile) accumulator gets its MFMA, so the five
 // operand kinds of one x value are formed once and reused.
 template <int NL, typename BT, typename XT, bool REFINE, int CSEL>
-__global__ __launch_bounds__(256) void k_apply(const int *__restrict__ order,
+__global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW)) void k_apply(const int *__restrict__ order,
                                                const BT *__restrict__ blk, const BT *__restrict__ blk_lo,
                                                const int *__restrict__ lm,
                                                const unsigned long long *__restrict__ first, int npair,
@@ -277,8 +282,9 @@ __global__ __launch_bounds__(256) void k_apply(const int *__restrict__ order,
     const int tid = threadIdx.x;
     const size_t gbase = (size_t)e * NBLK * NP;
     const unsigned long long fmask = first[e];
-    for (int i = tid; i < 3 * NEN; i += 256) sLM[i] = lm[(size_t)e * 3 * NEN + i];
-    for (int i = tid; i < NSLOT * NP; i += 256) {
+    constexpr int NW = LD::NW, NTH = 64 * NW;
+    for (int i = tid; i < 3 * NEN; i += NTH) sLM[i] = lm[(size_t)e * 3 * NEN + i];
+    for (int i = tid; i < NSLOT * NP; i += NTH) {
         const int sl = i / NP, r = i % NP, a = r / NEN, b = r % NEN;
         const int kb = (CSEL < 0) ? sl : kSub[CSEL < 0 ? 0 : CSEL][sl].blk;
         sB[(sl * NEN + a) * RS + b] = blk[gbase + (size_t)kb * NP + r];
@@ -289,7 +295,7 @@ __global__ __launch_bounds__(256) void k_apply(const int *__restrict__ order,
     const int col = lane & 15, kg = lane >> 4, ri = col & 1;
     const int ntile = (npair + 7) / 8;
 #pragma unroll 1
-    for (int t = wave; t < ntile; t += 4) {
+    for (int t = wave; t < ntile; t += NW) {
         const int pair = t * 8 + (col >> 1);
         const bool live = pair < npair;
         const OS om = live ? (OS)omega[pair] : (OS)0, kv = live ? (OS)ky[pair] : (OS)0;
