@@ -229,6 +229,16 @@ class MeshInfo:
                                      _p(g, dp)), "f25_gn_fhw")
         return g
 
+    def gn_diag(self, omega, ky, u) -> np.ndarray:
+        """Pseudo-Hessian diagonal sum_t |dS_t/dm_{a,p} u_t|^2, (NNPG, 3)."""
+        om = np.ascontiguousarray(np.atleast_1d(omega), np.float64)
+        kv = np.ascontiguousarray(np.atleast_1d(ky), np.float64)
+        u = np.ascontiguousarray(u, np.complex128).reshape(om.size, -1)
+        d = np.zeros((self.mesh.nnpg, 3))
+        check(_lib.load().f25_gn_diag(self.h, om.size, _p(om, dp), _p(kv, dp),
+                                      u.ctypes.data_as(C.c_void_p), _p(d, dp)), "f25_gn_diag")
+        return d
+
     # -- inversion bookkeeping -------------------------------------------------
     def inv_setup(self, cinvtype: bytes = b"PP", vpvs: float = 3 ** 0.5, linrec: bool = True,
                   lkbdry: bool = True, mrdof1=None) -> InvInfo:

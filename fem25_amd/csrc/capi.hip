@@ -97,6 +97,7 @@ struct f25_mesh_s {
     bool basis_ok = false;
     double *d_part = nullptr;
     int *d_mconn = nullptr;
+    int *d_pmap = nullptr;   // pseudo-Hessian patch positions
     int ncon_anchor = 0;
     // inversion (CFMAT/ADJFLAM)
     int *d_col_elem = nullptr, *d_col_corner = nullptr, *d_mygrad = nullptr, *d_jcsc = nullptr, *d_icsc = nullptr;
@@ -350,7 +351,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_xq); dfree(m->d_cdom); dfree(m->d_cnp); dfree(m->d_rho); dfree(m->d_lam); dfree(m->d_mu);
     dfree(m->d_blk); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_dblk); dfree(m->d_irptr);
     dfree(m->d_jcptr); dfree(m->d_comp); dfree(m->d_v0); dfree(m->d_v1); dfree(m->d_v2);
-    dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_col_elem); dfree(m->d_col_corner);
+    dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_pmap); dfree(m->d_col_elem); dfree(m->d_col_corner);
     dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_col_cp); dfree(m->d_col_cs);
     dfree(m->d_ew);
     dfree(m->d_omky);
@@ -1116,10 +1117,77 @@ extern "C" int f25_gn_fhw(f25_mesh *m, int ntrip, const double *omega, const dou
     return grad_host(m, ntrip, omega, ky, u, w, g, 1);
 }
 
+// patch positions of every anchor's element rows (built once per mesh)
+static int ensure_pmap(f25_mesh *m) {
+    if (m->d_pmap) return 0;
+    const int nnpg = m->d.nnpg, nen = m->tab.nen, nee = 3 * nen, ncon = m->ncon_anchor;
+    std::vector<int> cnt(nnpg, 0), pm((size_t)nnpg * ncon * nee, -1);
+    std::vector<std::vector<int>> elems(nnpg);
+    for (int e = 0; e < m->d.nelem; ++e)
+        for (int k = 0; k < 4; ++k) elems[m->ieng[4 * e + k] - 1].push_back(e);
+    std::vector<int> seen;   // DOF -> patch position, reset per anchor
+    seen.assign(m->g.ndof + 1, -1);
+    std::vector<int> touched;
+    for (int a = 0; a < nnpg; ++a) {
+        int npos = 0;
+        touched.clear();
+        for (size_t s = 0; s < elems[a].size(); ++s) {
+            const int e = elems[a][s];
+            for (int j = 0; j < nee; ++j) {
+                const int dof = m->g.lm[(size_t)e * nee + j];   // (nen, 3) rows of e, 1-based
+                if (dof <= 0) continue;
+                if (seen[dof] < 0) { seen[dof] = npos++; touched.push_back(dof); }
+                pm[((size_t)a * ncon + s) * nee + j] = seen[dof];
+            }
+        }
+        for (int d : touched) seen[d] = -1;
+    }
+    // element order per anchor must match the device connectivity (mconn)
+    return upload(&m->d_pmap, pm.data(), pm.size(), m->st);
+}
+
 extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
                            double *diag) {
-    (void)m; (void)ntrip; (void)omega; (void)ky; (void)u; (void)diag;
-    FAIL(F25_ERR_STATE, "pseudo-Hessian diagonal not available in this build");
+    if (!m || ntrip <= 0 || !omega || !ky || !u || !diag) FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(ensure_pmap(m));
+    const int nen = m->tab.nen, nee = 3 * nen, nq = m->tab.nq, nnpg = m->d.nnpg;
+    const size_t ndof = m->g.ndof;
+    double2 *du = nullptr, *dcols = nullptr;
+    double *dd = nullptr;
+    int r = dalloc(&du, ndof * ntrip);
+    if (!r) r = dalloc(&dcols, (size_t)m->d.nelem * 12 * nee);
+    if (!r) r = upload(&dd, diag, (size_t)nnpg * 3, m->st);
+    if (!r) {
+        hipError_t e = hipMemcpyAsync(du, u, ndof * ntrip * 16, hipMemcpyHostToDevice, m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    const int nt1 = std::max(64, ((nee + 63) / 64) * 64);
+    const size_t lds1 = nee * sizeof(double2) + (size_t)nq * 28 * sizeof(double);
+    const int nt2 = 256;
+    const size_t lds2 = (size_t)m->ncon_anchor * nee * 3 * sizeof(double2) + nt2 * 3 * sizeof(double);
+    if (!r && lds1 > 65536)
+        HIPCHK(hipFuncSetAttribute((const void *)k_fcols, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
+    if (!r && lds2 > 65536)
+        HIPCHK(hipFuncSetAttribute((const void *)k_fdiag, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+    for (int t = 0; t < ntrip && !r; ++t) {
+        hipLaunchKernelGGL(k_fcols, dim3(m->d.nelem), dim3(nt1), lds1, m->st, nen, m->tab.nq1, m->d_shl,
+                           m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho,
+                           m->d_lam, m->d_mu, omega[t], ky[t], du + ndof * t, dcols, m->d_err);
+        hipLaunchKernelGGL(k_fdiag, dim3(nnpg), dim3(nt2), lds2, m->st, nnpg, m->ncon_anchor, nee,
+                           m->d_mconn, m->d_ieng, m->d_pmap, dcols, dd);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    if (!r) r = check_err(m);
+    if (!r) {
+        hipError_t e = hipMemcpyAsync(diag, dd, (size_t)nnpg * 3 * sizeof(double), hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(du); dfree(dcols); dfree(dd);
+    return r;
 }
 
 // MFMA operand/result layout check (used by the self test entry point)

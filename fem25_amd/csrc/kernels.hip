@@ -806,6 +806,176 @@ __global__ void k_grad_gather(int nnpg, int ncon, const int *__restrict__ mconn,
 }
 
 // ---------------------------------------------------------------------------
+// Pseudo-Hessian diagonal, element part.  For corner c of element e and the
+// anchor's parameters p = (Vp, Vs, rho), the element rows of the column
+// dS/dm_{a,p} u are
+//   r_{b,i} = sum_q w_q N_c(q) T_{b,i}(q)
+// with the test contractions of the lambda, mu and rho terms
+//   T^l_{b,i}   = conj(D_{b,i}) div u         D_{b,x} = Nx_b, D_{b,y} = -i ky N_b, D_{b,z} = Nz_b
+//   T^m_{b,i}   = sum_k conj(d_k N_b) (G_ik + G_ki),  G_ik = d_k u_i, d_y = -i ky
+//   T^rho_{b,i} = -omega^2 N_b u_i
+// (the same bilinear forms as k_grad: lam^H r = the gradient density), then
+// the chain rule at the anchor:  Vp: 2 rho Vp T^l;  Vs: -4 rho Vs T^l + 2 rho Vs T^m;
+// rho: (Vp^2 - 2 Vs^2) T^l + Vs^2 T^m + T^rho.
+// out[((e*4 + c)*3 + p)*nee + 3b + i].  One workgroup per element; phase 1
+// puts the fields of u at every quadrature point in LDS, phase 2 gives one
+// thread per element row (b, i) and sums over the quadrature points.
+__global__ void k_fcols(int nen, int nq1, const double *__restrict__ shl, const double *__restrict__ wq,
+                        const double *__restrict__ xq, const int *__restrict__ ieng,
+                        const double *__restrict__ xl, const double *__restrict__ zl,
+                        const char *__restrict__ cdom, const int *__restrict__ lm,
+                        const double *__restrict__ rho, const double *__restrict__ lam,
+                        const double *__restrict__ mu, double omega, double kv,
+                        const double2 *__restrict__ u, double2 *__restrict__ out, int *__restrict__ err) {
+    extern __shared__ double sm[];
+    constexpr int QS = 28;   // doubles per quadrature point
+    const int e = blockIdx.x;
+    const int nq = nq1 * nq1, nee = 3 * nen;
+    double2 *su = reinterpret_cast<double2 *>(sm);   // nee
+    double *sq = reinterpret_cast<double *>(su + nee);   // nq * QS
+    const int *ie = ieng + 4 * e;
+    const int *L = lm + (size_t)e * nee;
+    double2 *o = out + (size_t)e * 12 * nee;
+    if (cdom[e] == 'A') {   // PML elements are outside the non-PML gradient path
+        for (int i = threadIdx.x; i < 12 * nee; i += blockDim.x) o[i] = make_double2(0.0, 0.0);
+        return;
+    }
+    for (int i = threadIdx.x; i < nee; i += blockDim.x) {
+        const int d = L[i];
+        su[i] = d >= 0 ? u[d] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+        const int ix = q % nq1, iz = q / nq1;
+        Geo g;
+        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
+        double2 U[3] = {}, UX[3] = {}, UZ[3] = {};
+        const double rx = g.dzdeta / g.det, sx = -g.dzdxi / g.det;
+        const double rz = -g.dxdeta / g.det, szz = g.dxdxi / g.det;
+        for (int a = 0; a < nen; ++a) {
+            const double *s = shl + ((size_t)q * nen + a) * 3;
+            const double n = s[2], nx = s[0] * rx + s[1] * sx, nz = s[0] * rz + s[1] * szz;
+            for (int c = 0; c < 3; ++c) {
+                const double2 v = su[a * 3 + c];
+                U[c].x += n * v.x; U[c].y += n * v.y;
+                UX[c].x += nx * v.x; UX[c].y += nx * v.y;
+                UZ[c].x += nz * v.x; UZ[c].y += nz * v.y;
+            }
+        }
+        // G[i][k] = d_k u_i, k = x, y, z; d_y u = -i ky u
+        double2 G[3][3];
+        for (int c = 0; c < 3; ++c) {
+            G[c][0] = UX[c];
+            G[c][1] = make_double2(kv * U[c].y, -kv * U[c].x);
+            G[c][2] = UZ[c];
+        }
+        double *p = sq + (size_t)q * QS;
+        const double w = wq[ix] * wq[iz] * g.det;
+        for (int k = 0; k < 4; ++k) p[k] = w * g.sf[k];
+        p[4] = G[0][0].x + G[1][1].x + G[2][2].x;   // div u
+        p[5] = G[0][0].y + G[1][1].y + G[2][2].y;
+        // symmetric S_ik = G_ik + G_ki: xx, yy, zz, xy, xz, yz
+        const int si[6] = {0, 1, 2, 0, 0, 1}, sk[6] = {0, 1, 2, 1, 2, 2};
+        for (int j = 0; j < 6; ++j) {
+            p[6 + 2 * j] = G[si[j]][sk[j]].x + G[sk[j]][si[j]].x;
+            p[7 + 2 * j] = G[si[j]][sk[j]].y + G[sk[j]][si[j]].y;
+        }
+        for (int c = 0; c < 3; ++c) { p[18 + 2 * c] = U[c].x; p[19 + 2 * c] = U[c].y; }
+        p[24] = rx; p[25] = sx; p[26] = rz; p[27] = szz;
+    }
+    __syncthreads();
+    const int it = threadIdx.x;
+    if (it >= nee) return;
+    const int b = it / 3, i = it % 3;
+    // index of S_ik in the packed symmetric storage for k = x, y, z
+    const int sidx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+    double2 acc[4][3];
+    for (int c = 0; c < 4; ++c)
+        for (int t = 0; t < 3; ++t) acc[c][t] = make_double2(0.0, 0.0);
+    const double w2 = omega * omega;
+    for (int q = 0; q < nq; ++q) {
+        const double *p = sq + (size_t)q * QS;
+        const double *s = shl + ((size_t)q * nen + b) * 3;
+        const double n = s[2], nx = s[0] * p[24] + s[1] * p[25], nz = s[0] * p[26] + s[1] * p[27];
+        // conj(D_{b,i}): x -> Nx, y -> +i ky N, z -> Nz
+        const double2 dv = make_double2(p[4], p[5]);
+        double2 tl;
+        if (i == 0) tl = make_double2(nx * dv.x, nx * dv.y);
+        else if (i == 2) tl = make_double2(nz * dv.x, nz * dv.y);
+        else tl = make_double2(-kv * n * dv.y, kv * n * dv.x);
+        // mu: Nx S_ix + (i ky N) S_iy + Nz S_iz
+        const double *sx_ = p + 6 + 2 * sidx[i][0], *sy_ = p + 6 + 2 * sidx[i][1], *sz_ = p + 6 + 2 * sidx[i][2];
+        const double2 tm = make_double2(nx * sx_[0] - kv * n * sy_[1] + nz * sz_[0],
+                                        nx * sx_[1] + kv * n * sy_[0] + nz * sz_[1]);
+        const double2 tr = make_double2(-w2 * n * p[18 + 2 * i], -w2 * n * p[19 + 2 * i]);
+        for (int c = 0; c < 4; ++c) {
+            const double wc = p[c];
+            acc[c][0].x += wc * tl.x; acc[c][0].y += wc * tl.y;
+            acc[c][1].x += wc * tm.x; acc[c][1].y += wc * tm.y;
+            acc[c][2].x += wc * tr.x; acc[c][2].y += wc * tr.y;
+        }
+    }
+    for (int c = 0; c < 4; ++c) {
+        const int a = ie[c] - 1;
+        const double r = rho[a], vp = sqrt((lam[a] + 2.0 * mu[a]) / r), vs = sqrt(mu[a] / r);
+        const double2 l = acc[c][0], m = acc[c][1], rr = acc[c][2];
+        const double cl0 = 2.0 * r * vp, cl1 = -4.0 * r * vs, cm1 = 2.0 * r * vs;
+        const double cl2 = vp * vp - 2.0 * vs * vs, cm2 = vs * vs;
+        o[((size_t)c * 3 + 0) * nee + it] = make_double2(cl0 * l.x, cl0 * l.y);
+        o[((size_t)c * 3 + 1) * nee + it] = make_double2(cl1 * l.x + cm1 * m.x, cl1 * l.y + cm1 * m.y);
+        o[((size_t)c * 3 + 2) * nee + it] = make_double2(cl2 * l.x + cm2 * m.x + rr.x, cl2 * l.y + cm2 * m.y + rr.y);
+    }
+}
+
+// Pseudo-Hessian diagonal, anchor part: the column of anchor a is the sum of
+// its elements' vectors on shared DOFs.  pmap[(a*ncon + s)*nee + j] is the
+// patch position of row j of the s-th element of a (-1 fixed DOF); the
+// element and its corner come from mconn.  diag[3a + p] += |column|^2.
+__global__ void k_fdiag(int nnpg, int ncon, int nee, const int *__restrict__ mconn,
+                        const int *__restrict__ ieng, const int *__restrict__ pmap,
+                        const double2 *__restrict__ cols, double *__restrict__ diag) {
+    extern __shared__ double sm[];
+    const int a = blockIdx.x;
+    if (a >= nnpg) return;
+    const int np = ncon * nee;
+    double2 *acc = reinterpret_cast<double2 *>(sm);   // np * 3
+    double *red = reinterpret_cast<double *>(acc + (size_t)np * 3);   // blockDim * 3
+    for (int i = threadIdx.x; i < np * 3; i += blockDim.x) acc[i] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int s = 0; s < ncon; ++s) {
+        const int e1 = mconn[(size_t)s * nnpg + a];
+        if (e1 <= 0) break;
+        const int e = e1 - 1;
+        int corner = 0;
+        for (int k = 0; k < 4; ++k)
+            if (ieng[4 * e + k] - 1 == a) corner = k;
+        const int *pm = pmap + ((size_t)a * ncon + s) * nee;
+        const double2 *src = cols + ((size_t)e * 4 + corner) * 3 * nee;
+        for (int j = threadIdx.x; j < nee; j += blockDim.x) {
+            const int pos = pm[j];
+            if (pos < 0) continue;
+            for (int p = 0; p < 3; ++p) {
+                const double2 v = src[(size_t)p * nee + j];
+                acc[pos * 3 + p].x += v.x;
+                acc[pos * 3 + p].y += v.y;
+            }
+        }
+        __syncthreads();   // rows of one element map to distinct positions
+    }
+    double s3[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < np; i += blockDim.x)
+        for (int p = 0; p < 3; ++p) s3[p] += acc[i * 3 + p].x * acc[i * 3 + p].x + acc[i * 3 + p].y * acc[i * 3 + p].y;
+    for (int p = 0; p < 3; ++p) red[threadIdx.x * 3 + p] = s3[p];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int p = 0; p < 3; ++p) red[threadIdx.x * 3 + p] += red[(threadIdx.x + st) * 3 + p];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) diag[3 * a + threadIdx.x] += red[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
 // Reference virtual-force columns (VFORCE25 through CFMAT_DIST, NVINV = 1).
 // Column of anchor a: element e = the last entry of MCONN(a,:) (VFORCE25
 // clears FORCEV inside its element loop, vforce25.f:252-256), corner k with

@@ -217,3 +217,32 @@ def test_gn_fv_matches_operator_difference():
     Sm = csr(g, oracle.asmble_z(m, mm, g, f, py, pml))
     ref = (Sp @ u - Sm @ u) / (2 * h)
     assert relerr(fv, ref) < 1e-9
+
+
+def test_gn_diag_matches_operator_columns():
+    """Pseudo-Hessian diagonal sum_t |dS_t/dm_{a,p} u_t|^2 against columns
+    formed by central differences of the oracle's FP64 operator.  S is
+    quadratic in Vp alone and in Vs alone and affine in rho, so a central
+    difference is exact up to rounding (h = 1 m/s, 1 kg/m^3)."""
+    name = "tiny_layered"
+    mi = _mi(name)
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    om, ky, f, py = pairs(1, 2)
+    u = cf.seeded_complex(g.ndof, batch=om.size, seed=71)
+    d = mi.gn_diag(om, ky, u)
+    ref = np.zeros_like(d)
+    for a in range(m.nnpg):
+        for p in range(3):
+            dv = np.zeros((3, m.nnpg))
+            dv[p, a] = 1.0
+            mp = cf.model_from_velocity(mod.vp + dv[0], mod.vs + dv[1], mod.dens + dv[2])
+            mm = cf.model_from_velocity(mod.vp - dv[0], mod.vs - dv[1], mod.dens - dv[2])
+            for b in range(om.size):
+                sp_ = csr(g, oracle.asmble_z(m, mp, g, f[b], py[b], pml))
+                sm_ = csr(g, oracle.asmble_z(m, mm, g, f[b], py[b], pml))
+                col = (sp_ @ u[b] - sm_ @ u[b]) / 2.0
+                ref[a, p] += float(np.vdot(col, col).real)
+    for p in range(3):
+        assert relerr(d[:, p], ref[:, p]) < 1e-9, p
+    assert np.all(d > 0)
