@@ -194,9 +194,9 @@ def seeded_complex(n: int, batch: int | None = None, seed: int = 1) -> np.ndarra
 
 # Named workloads (sizes from the benchmark definition).
 CONFIGS = {
-    "c1": dict(nx=40, nz=20, nord=4),
-    "c2": dict(nx=200, nz=100, nord=4),
-    "c5p4": dict(nx=400, nz=200, nord=4),
-    "c5p5": dict(nx=400, nz=200, nord=5),
-    "c5p6": dict(nx=400, nz=200, nord=6),
+    "c1": dict(nx=40, nz=20, nord=4, nfreq=1, nky=1),
+    "c2": dict(nx=200, nz=100, nord=4, nfreq=8, nky=16),
+    "c5p4": dict(nx=400, nz=200, nord=4, nfreq=6, nky=32),
+    "c5p5": dict(nx=400, nz=200, nord=5, nfreq=6, nky=32),
+    "c5p6": dict(nx=400, nz=200, nord=6, nfreq=6, nky=32),
 }
