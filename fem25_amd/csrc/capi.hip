@@ -88,7 +88,10 @@ struct f25_mesh_s {
     char *d_cdom = nullptr, *d_cnp = nullptr;
     double *d_rho = nullptr, *d_lam = nullptr, *d_mu = nullptr;
     double *d_blk = nullptr;
-    float *d_blk_hi = nullptr, *d_blk_lo = nullptr;
+    float *d_blk32 = nullptr;                          // fl32 of the blocks (FP32 mode)
+    float *d_blk_hi = nullptr, *d_blk_lo = nullptr;   // fixed-point split (refined mode)
+    unsigned long long *d_colmax = nullptr;           // per-pair max |x| (refined mode)
+    int colmax_cap = 0;
     double *d_dblk = nullptr;   // blocks of a model perturbation (GN products)
     int64_t *d_irptr = nullptr;
     int *d_jcptr = nullptr;
@@ -349,7 +352,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_ieng); dfree(m->d_lm); dfree(m->d_first); dfree(m->d_order); dfree(m->d_pml_order); dfree(m->d_err);
     dfree(m->d_xl); dfree(m->d_zl); dfree(m->d_xd); dfree(m->d_zd); dfree(m->d_shl); dfree(m->d_wq);
     dfree(m->d_xq); dfree(m->d_cdom); dfree(m->d_cnp); dfree(m->d_rho); dfree(m->d_lam); dfree(m->d_mu);
-    dfree(m->d_blk); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_dblk); dfree(m->d_irptr);
+    dfree(m->d_blk); dfree(m->d_blk32); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_colmax); dfree(m->d_dblk); dfree(m->d_irptr);
     dfree(m->d_jcptr); dfree(m->d_comp); dfree(m->d_v0); dfree(m->d_v1); dfree(m->d_v2);
     dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_pmap); dfree(m->d_col_elem); dfree(m->d_col_corner);
     dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_col_cp); dfree(m->d_col_cs);
@@ -414,21 +417,21 @@ extern "C" int f25_mesh_graph(f25_mesh *m, int *ien, int *lm, int *irptr, int *j
 // ---------------------------------------------------------------------------
 template <int NL>
 static void launch_blocks(f25_mesh *m, const double *rho, const double *lam, const double *mu,
-                          double *blk, float *hi, float *lo) {
+                          double *blk, float *b32, float *hi, float *lo) {
     constexpr int NT = (NL <= 5) ? 256 : 512;
     hipLaunchKernelGGL((k_blocks<NL, NT>), dim3(m->d.nelem), dim3(NT), 0, m->st, m->d.nelem,
                        m->tab.nq1, m->d_shl, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, rho, lam,
-                       mu, m->d_cdom, blk, hi, lo, m->d_err);
+                       mu, m->d_cdom, blk, b32, hi, lo, m->d_err);
 }
 
 static int run_blocks(f25_mesh *m, const double *rho, const double *lam, const double *mu,
-                      double *blk, float *hi, float *lo) {
+                      double *blk, float *b32, float *hi, float *lo) {
     switch (m->tab.nl) {
-        case 3: launch_blocks<3>(m, rho, lam, mu, blk, hi, lo); break;
-        case 4: launch_blocks<4>(m, rho, lam, mu, blk, hi, lo); break;
-        case 5: launch_blocks<5>(m, rho, lam, mu, blk, hi, lo); break;
-        case 6: launch_blocks<6>(m, rho, lam, mu, blk, hi, lo); break;
-        case 7: launch_blocks<7>(m, rho, lam, mu, blk, hi, lo); break;
+        case 3: launch_blocks<3>(m, rho, lam, mu, blk, b32, hi, lo); break;
+        case 4: launch_blocks<4>(m, rho, lam, mu, blk, b32, hi, lo); break;
+        case 5: launch_blocks<5>(m, rho, lam, mu, blk, b32, hi, lo); break;
+        case 6: launch_blocks<6>(m, rho, lam, mu, blk, b32, hi, lo); break;
+        case 7: launch_blocks<7>(m, rho, lam, mu, blk, b32, hi, lo); break;
         default: FAIL(F25_ERR_ARG, "unsupported order");
     }
     HIPCHK(hipGetLastError());
@@ -460,10 +463,11 @@ extern "C" int f25_set_model(f25_mesh *m, const double *dens, const double *ecoe
     const size_t nb = (size_t)m->d.nelem * NBLK * m->tab.nen * m->tab.nen;
     if (!m->d_blk) {
         TRY(dalloc(&m->d_blk, nb));
+        TRY(dalloc(&m->d_blk32, nb));
         TRY(dalloc(&m->d_blk_hi, nb));
         TRY(dalloc(&m->d_blk_lo, nb));
     }
-    TRY(run_blocks(m, m->d_rho, m->d_lam, m->d_mu, m->d_blk, m->d_blk_hi, m->d_blk_lo));
+    TRY(run_blocks(m, m->d_rho, m->d_lam, m->d_mu, m->d_blk, m->d_blk32, m->d_blk_hi, m->d_blk_lo));
     m->have_model = true;
     m->basis_ok = false;
     return 0;
@@ -606,6 +610,7 @@ extern "C" int f25_asmb_peff(f25_mesh *m, const void *cdist, const void *ue, voi
 template <int NL, typename BT, typename XT, bool REFINE, int CSEL>
 static int launch_apply_c(f25_mesh *m, const double *om, const double *kv, int npair, const XT *x,
                           XT *y, const BT *blk, const BT *blk_lo) {
+    const double *cmx = REFINE ? reinterpret_cast<const double *>(m->d_colmax) : nullptr;
     typedef ApplyLds<NL, BT, REFINE> LD;
     constexpr size_t lds = CSEL < 0 ? LD::bytes : (5 * LD::per_blk + 3 * ApplyShape<NL>::NEN * sizeof(int));
     static bool attr_set = false;
@@ -619,7 +624,7 @@ static int launch_apply_c(f25_mesh *m, const double *om, const double *kv, int n
         if (!n) continue;
         hipLaunchKernelGGL(kern, dim3(n), dim3(64 * ApplyLds<NL, BT, REFINE>::NW), lds, m->st,
                            m->d_order + m->cptr[c], blk, blk_lo,
-                           m->d_lm, m->d_first, npair, om, kv, x, y);
+                           m->d_lm, m->d_first, npair, om, kv, cmx, x, y);
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -662,10 +667,21 @@ static int apply_core(f25_mesh *m, int npair, const double *d_om, const double *
                                                     blk64, nullptr)));
     else if (mode == F25_APPLY_F32)
         TRY((dispatch_apply<float, float2, false>(m, d_om, d_ky, npair, (const float2 *)x, (float2 *)y,
-                                                  m->d_blk_hi, nullptr)));
-    else
+                                                  m->d_blk32, nullptr)));
+    else {
+        if (npair > m->colmax_cap) {
+            dfree(m->d_colmax);
+            TRY(dalloc(&m->d_colmax, (size_t)npair));
+            m->colmax_cap = npair;
+        }
+        fill_async(m->d_colmax, (size_t)npair, 0ull, m->st);
+        const int gy = (int)std::min<int64_t>(1024, std::max<int64_t>(1, m->g.ndof / 64));
+        hipLaunchKernelGGL(k_colmax, dim3((npair + 63) / 64, gy), dim3(256), 0, m->st, (int64_t)m->g.ndof,
+                           npair, (const double2 *)x, m->d_colmax);
+        HIPCHK(hipGetLastError());
         TRY((dispatch_apply<float, double2, true>(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y,
                                                   m->d_blk_hi, m->d_blk_lo)));
+    }
     HIPCHK(hipEventRecord(m->ev1, m->st));
     HIPCHK(hipStreamSynchronize(m->st));
     float ms = 0.f;
@@ -729,11 +745,16 @@ extern "C" int f25_apply_batch(f25_mesh *m, int npair, const double *omega, cons
 // ---------------------------------------------------------------------------
 static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d_ky, const double2 *u,
                      const double2 *l, double *g, int chain) {
-    const int nen = m->tab.nen;
-    const int nt = 256;
-    size_t lds = 6 * nen * sizeof(double2) + nt * 12 * sizeof(double);
+    const int nen = m->tab.nen, nq = m->tab.nq;
+    // trip groups per element: as many quadrature-point sets as fit 512 threads
+    const int tg = std::max(1, 512 / nq);
+    const int nt = ((nq * tg + 63) / 64) * 64;   // power of two not required below
+    int ntp = 64;
+    while (ntp < nt) ntp *= 2;                   // the tree reduction wants a power of two
+    constexpr int H = 2;   // pairs per thread per pass
+    size_t lds = (size_t)2 * H * tg * 3 * nen * sizeof(double2) + (size_t)ntp * 12 * sizeof(double);
     HIPCHK(hipEventRecord(m->ev0, m->st));
-    hipLaunchKernelGGL(k_grad, dim3(m->d.nelem), dim3(nt), lds, m->st, nen, m->tab.nq1, m->d.nelem,
+    hipLaunchKernelGGL(k_grad<H>, dim3(m->d.nelem), dim3(ntp), lds, m->st, nen, m->tab.nq1, m->d.nelem, tg,
                        m->d_shl, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip,
                        (int64_t)m->g.ndof, d_om, d_ky, u, l, m->d_part, m->d_err);
     HIPCHK(hipEventRecord(m->ev1, m->st));
@@ -1088,7 +1109,7 @@ extern "C" int f25_gn_fv(f25_mesh *m, int ntrip, const double *omega, const doub
     TRY(upload(&d_m, dm.data(), n, m->st));
     const size_t nb = (size_t)m->d.nelem * NBLK * m->tab.nen * m->tab.nen;
     if (!m->d_dblk) TRY(dalloc(&m->d_dblk, nb));
-    int r = run_blocks(m, d_r, d_l, d_m, m->d_dblk, nullptr, nullptr);
+    int r = run_blocks(m, d_r, d_l, d_m, m->d_dblk, nullptr, nullptr, nullptr);
     dfree(d_r); dfree(d_l); dfree(d_m);
     if (r) return r;
     const size_t nn = (size_t)m->g.ndof * ntrip;

@@ -68,8 +68,8 @@ __global__ __launch_bounds__(NT) void k_blocks(int nelem, int nq1, const double 
                                                const double *__restrict__ lam,
                                                const double *__restrict__ mu,
                                                const char *__restrict__ cdom, double *__restrict__ blk,
-                                               float *__restrict__ blk_hi, float *__restrict__ blk_lo,
-                                               int *__restrict__ err) {
+                                               float *__restrict__ blk32, float *__restrict__ blk_hi,
+                                               float *__restrict__ blk_lo, int *__restrict__ err) {
     constexpr int NEN = NL * NL;
     constexpr int NP = NEN * NEN;
     constexpr int PPT = (NP + NT - 1) / NT;
@@ -83,6 +83,7 @@ __global__ __launch_bounds__(NT) void k_blocks(int nelem, int nq1, const double 
     if (cdom[e] == 'A') {   // PML elements are integrated per call
         for (int i = tid; i < NBLK * NP; i += NT) {
             blk[obase + i] = 0.0;
+            if (blk32) blk32[obase + i] = 0.f;
             if (blk_hi) { blk_hi[obase + i] = 0.f; blk_lo[obase + i] = 0.f; }
         }
         return;
@@ -130,6 +131,7 @@ __global__ __launch_bounds__(NT) void k_blocks(int nelem, int nq1, const double 
             int pr = tid + p * NT;
             if (pr >= NP) break;
             int a = pr / NEN, b = pr % NEN;
+#pragma unroll 2
             for (int qq = 0; qq < QC; ++qq) {
                 double wr = sW[qq][0], wl = sW[qq][1], wm = sW[qq][2];
                 double na = sN[qq][a], nb = sN[qq][b], xa = sX[qq][a], xb = sX[qq][b];
@@ -147,6 +149,38 @@ __global__ __launch_bounds__(NT) void k_blocks(int nelem, int nq1, const double 
             }
         }
     }
+    // Split for the refined FP32 product: hi = v rounded to a multiple of
+    // 2^(E-11), where 2^E bounds the whole block, so |hi| / 2^(E-11) <= 2^11
+    // (12 significant bits, exact in FP32); lo = fl32(v - hi).
+    __shared__ int sExp[NBLK];
+    if (blk_hi) {
+        __shared__ double sMax[NT / 64][NBLK];
+        double mx[NBLK];
+#pragma unroll
+        for (int k = 0; k < NBLK; ++k) mx[k] = 0.0;
+#pragma unroll
+        for (int p = 0; p < PPT; ++p) {
+            int pr = tid + p * NT;
+            if (pr >= NP) break;
+#pragma unroll
+            for (int k = 0; k < NBLK; ++k) mx[k] = fmax(mx[k], fabs(acc[p][k]));
+        }
+#pragma unroll
+        for (int k = 0; k < NBLK; ++k)
+            for (int off = 32; off > 0; off >>= 1) mx[k] = fmax(mx[k], __shfl_xor(mx[k], off));
+        if ((tid & 63) == 0)
+#pragma unroll
+            for (int k = 0; k < NBLK; ++k) sMax[tid >> 6][k] = mx[k];
+        __syncthreads();
+        if (tid < NBLK) {
+            double v = 0.0;
+            for (int w = 0; w < NT / 64; ++w) v = fmax(v, sMax[w][tid]);
+            int ex = 0;
+            (void)frexp(v, &ex);   // v < 2^ex
+            sExp[tid] = ex;
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
         int pr = tid + p * NT;
@@ -155,10 +189,12 @@ __global__ __launch_bounds__(NT) void k_blocks(int nelem, int nq1, const double 
         for (int k = 0; k < NBLK; ++k) {
             double v = acc[p][k];
             blk[obase + (size_t)k * NP + pr] = v;
+            if (blk32) blk32[obase + (size_t)k * NP + pr] = (float)v;
             if (blk_hi) {
-                float h = (float)v;
-                blk_hi[obase + (size_t)k * NP + pr] = h;
-                blk_lo[obase + (size_t)k * NP + pr] = (float)(v - (double)h);
+                const int ex = sExp[k];
+                const double h = ldexp(rint(ldexp(v, 11 - ex)), ex - 11);
+                blk_hi[obase + (size_t)k * NP + pr] = (float)h;
+                blk_lo[obase + (size_t)k * NP + pr] = (float)(v - h);
             }
         }
     }
@@ -230,7 +266,8 @@ struct ApplyLds {
     static constexpr size_t bytes = (PH ? 5 : 9) * per_blk + 3 * S::NEN * sizeof(int);
     static constexpr bool PRE = S::MT <= 2;
     static constexpr int per_cu = (int)((160 * 1024) / bytes);
-    static constexpr int NW = per_cu >= 3 ? 4 : (per_cu == 2 ? 6 : 12);
+    // (the refined product needs more registers: at most 8 waves per element)
+    static constexpr int NW = per_cu >= 3 ? 4 : (per_cu == 2 ? 6 : (REFINE ? 8 : 12));
 };
 
 // value held by lane l^1 (the re/im partner of a column pair), quad_perm [1,0,3,2]
@@ -246,8 +283,10 @@ __device__ inline float partner(float v) {
 
 // XT: storage type of x/y (double2 or float2); BT: block/MFMA type.
 // CSEL: -1 all three output components, 0..2 one component (PH launches).
-// REFINE: blocks as hi+lo floats, the double operand split hi+lo, three
-// products (hi*hi + hi*lo + lo*hi) per 4-wide step, each summed in double.
+// REFINE: blocks and the double operand each split into a fixed-point hi
+// part (12 bits on a per-block / per-pair-and-kind exponent grid, so the
+// four-term hi*hi partial sums of one MFMA are exact in FP32) and an FP32
+// remainder; hi*hi, hi*lo, lo*hi and lo*lo on FP32 MFMA, summed in double.
 // first[e] bit a: element e is the first (lowest colour) to touch node a, so
 // it writes y instead of adding to it (no zero fill of y is needed).
 //
@@ -262,6 +301,7 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW)) void k_apply(c
                                                const unsigned long long *__restrict__ first, int npair,
                                                const double *__restrict__ omega,
                                                const double *__restrict__ ky,
+                                               const double *__restrict__ colmax,
                                                const XT *__restrict__ x, XT *__restrict__ y) {
     typedef ApplyShape<NL> S;
     typedef ApplyLds<NL, BT, REFINE> LD;
@@ -300,6 +340,15 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW)) void k_apply(c
         const bool live = pair < npair;
         const OS om = live ? (OS)omega[pair] : (OS)0, kv = live ? (OS)ky[pair] : (OS)0;
         const OS f1 = ri ? kv : -kv, f3 = kv * kv, f4 = -om * om;
+        // refined mode: per operand kind, 2^eB bounds |kind factor| * max|x| of
+        // this pair, so the hi part of an operand is a multiple of 2^(eB-11)
+        int eB[5] = {0, 0, 0, 0, 0};
+        if (REFINE) {
+            const double cm = live ? colmax[pair] : 0.0;
+            const double kap[5] = {1.0, fabs((double)kv), fabs((double)kv), (double)(kv * kv), (double)(om * om)};
+#pragma unroll
+            for (int j = 0; j < 5; ++j) (void)frexp(cm * kap[j], &eB[j]);
+        }
         // the y preload is branch-free: rows outside the element, fixed DOFs
         // and first touches read a valid address (row 0) and are zeroed
         const int pcl = live ? pair : 0;
@@ -356,6 +405,19 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW)) void k_apply(c
                 bop[d][3] = f3 * xv[d];
                 bop[d][4] = f4 * xv[d];
             }
+            // refined mode: fixed-point hi (12 bits, exact in FP32) + FP32 rest
+            float fh[REFINE ? 3 : 1][REFINE ? 5 : 1], fl[REFINE ? 3 : 1][REFINE ? 5 : 1];
+            if (REFINE) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+#pragma unroll
+                    for (int j = 0; j < 5; ++j) {
+                        const double v = (double)bop[d][j];
+                        const double h = ldexp(rint(ldexp(v, 11 - eB[j])), eB[j] - 11);
+                        fh[REFINE ? d : 0][REFINE ? j : 0] = (float)h;
+                        fl[REFINE ? d : 0][REFINE ? j : 0] = (float)(v - h);
+                    }
+            }
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) {
                 constexpr int C0 = (CSEL < 0) ? 0 : CSEL;
@@ -373,14 +435,19 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW)) void k_apply(c
                         const int arow = min(mt * 16 + col, NEN - 1);
                         const int idx = sb.trans ? (k * RS + arow) : (arow * RS + k);
                         if (REFINE) {
-                            const float bh = (float)bv, bl = (float)((double)bv - (double)bh);
+                            // hi*hi: products of 12-bit multiples of 2^(eA-11) and
+                            // 2^(eB-11), four of them summed: exact in FP32
+                            const float bh = fh[REFINE ? sb.xcomp : 0][REFINE ? sb.kind : 0];
+                            const float bl = fl[REFINE ? sb.xcomp : 0][REFINE ? sb.kind : 0];
                             const BT av = A[idx], avl = Al[idx];
-                            v4 z = {0, 0, 0, 0};
-                            z = MT_::mma(av, bh, z);
-                            z = MT_::mma(av, bl, z);
-                            z = MT_::mma(avl, bh, z);
+                            const v4 z0 = {0, 0, 0, 0};
+                            const v4 z1 = MT_::mma(av, bh, z0);
+                            v4 z2 = MT_::mma(av, bl, z0);
+                            z2 = MT_::mma(avl, bh, z2);
+                            z2 = MT_::mma(avl, bl, z2);
 #pragma unroll
-                            for (int i = 0; i < 4; ++i) accd[REFINE ? cc : 0][REFINE ? mt : 0][i] += (double)z[i];
+                            for (int i = 0; i < 4; ++i)
+                                accd[REFINE ? cc : 0][REFINE ? mt : 0][i] += (double)z1[i] + (double)z2[i];
                         } else {
                             acc[cc][mt] = MT_::mma(A[idx], (BT)bv, acc[cc][mt]);
                         }
@@ -680,7 +747,13 @@ __global__ void k_peff(int ndofl, const int *__restrict__ mydofs, const int64_t 
 //   d_rho    = -omega^2 Re[conj(l) . u]
 // accumulated over the wavefield pairs at each quadrature point, then
 // projected on the element's four anchor hat functions.
-__global__ void k_grad(int nen, int nq1, int nelem, const double *__restrict__ shl,
+// One workgroup per element.  Thread i handles quadrature point i % nq for
+// trip group i / nq (tg groups); the wavefield pairs are staged in LDS
+// tc = H tg at a time and every thread accumulates H of them per pass, so
+// the shape-function values and physical derivatives at (q, a) are formed
+// once per H pairs.  The geometry at q is computed once per element.
+template <int H>
+__global__ __launch_bounds__(512) void k_grad(int nen, int nq1, int nelem, int tg, const double *__restrict__ shl,
                        const double *__restrict__ wq, const double *__restrict__ xq,
                        const int *__restrict__ ieng, const double *__restrict__ xl,
                        const double *__restrict__ zl, const char *__restrict__ cdom,
@@ -690,78 +763,90 @@ __global__ void k_grad(int nen, int nq1, int nelem, const double *__restrict__ s
                        double *__restrict__ part, int *__restrict__ err) {
     extern __shared__ double sm[];
     const int e = blockIdx.x;
-    const int nq = nq1 * nq1;
-    double2 *su = reinterpret_cast<double2 *>(sm);   // 3*nen
-    double2 *sl = su + 3 * nen;                        // 3*nen
-    double *sred = reinterpret_cast<double *>(sl + 3 * nen);   // blockDim*12
+    const int nq = nq1 * nq1, nee = 3 * nen, tc = H * tg;
+    double2 *su = reinterpret_cast<double2 *>(sm);   // [tc][nee]
+    double2 *sl = su + (size_t)tc * nee;               // [tc][nee]
+    double *sred = reinterpret_cast<double *>(sl + (size_t)tc * nee);   // blockDim*12
     const int *ie = ieng + 4 * e;
-    const int *L = lm + (size_t)e * 3 * nen;
+    const int *L = lm + (size_t)e * nee;
     const bool skip = cdom[e] == 'A';
-    double g12[12];
-    for (int i = 0; i < 12; ++i) g12[i] = 0.0;
-    for (int q0 = 0; q0 < nq && !skip; q0 += blockDim.x) {
-        const int q = q0 + threadIdx.x;
-        Geo g;
-        int ix = 0, iz = 0;
-        bool act = q < nq;
-        if (act) {
-            ix = q % nq1; iz = q / nq1;
-            if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) { *err = 2; act = false; }
+    const int q = threadIdx.x % nq, grp = threadIdx.x / nq;
+    const bool act = !skip && grp < tg;
+    const int ix = q % nq1, iz = q / nq1;
+    Geo g;
+    double rx = 0, sx = 0, rz = 0, sz = 0;
+    if (act) {
+        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
+        rx = g.dzdeta / g.det; sx = -g.dzdxi / g.det;
+        rz = -g.dxdeta / g.det; sz = g.dxdxi / g.det;
+    }
+    double Dl = 0.0, Dm = 0.0, Dr = 0.0;
+    for (int t0 = 0; t0 < ntrip && !skip; t0 += tc) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < tc * nee; i += blockDim.x) {
+            const int tt = i / nee, j = i % nee, t = t0 + tt;
+            const int d = L[j];
+            const bool ok = d >= 0 && t < ntrip;
+            su[i] = ok ? u[(size_t)t * ndof + d] : make_double2(0.0, 0.0);
+            sl[i] = ok ? l[(size_t)t * ndof + d] : make_double2(0.0, 0.0);
         }
-        double Dl = 0.0, Dm = 0.0, Dr = 0.0;
-        for (int t = 0; t < ntrip; ++t) {
-            __syncthreads();
-            for (int i = threadIdx.x; i < 3 * nen; i += blockDim.x) {
-                int d = L[i];
-                su[i] = d >= 0 ? u[(size_t)t * ndof + d] : make_double2(0.0, 0.0);
-                sl[i] = d >= 0 ? l[(size_t)t * ndof + d] : make_double2(0.0, 0.0);
-            }
-            __syncthreads();
-            if (!act) continue;
-            double2 U[3] = {}, UX[3] = {}, UZ[3] = {}, V[3] = {}, VX[3] = {}, VZ[3] = {};
-            for (int a = 0; a < nen; ++a) {
-                const double *s = shl + ((size_t)q * nen + a) * 3;
-                double n = s[2];
-                double nx = (s[0] * g.dzdeta - s[1] * g.dzdxi) / g.det;
-                double nz = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / g.det;
+        __syncthreads();
+        if (!act) continue;
+        const int ta = H * grp;   // this thread's pairs in the chunk: ta .. ta + H - 1
+        double2 U[H][3] = {}, UX[H][3] = {}, UZ[H][3] = {}, V[H][3] = {}, VX[H][3] = {}, VZ[H][3] = {};
+        const double2 *u0 = su + (size_t)ta * nee, *l0 = sl + (size_t)ta * nee;
+        for (int a = 0; a < nen; ++a) {
+            const double *s = shl + ((size_t)q * nen + a) * 3;
+            const double n = s[2], nx = s[0] * rx + s[1] * sx, nz = s[0] * rz + s[1] * sz;
+#pragma unroll
+            for (int h = 0; h < H; ++h)
+#pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    double2 uu = su[a * 3 + c], vv = sl[a * 3 + c];
-                    U[c].x += n * uu.x; U[c].y += n * uu.y;
-                    UX[c].x += nx * uu.x; UX[c].y += nx * uu.y;
-                    UZ[c].x += nz * uu.x; UZ[c].y += nz * uu.y;
-                    V[c].x += n * vv.x; V[c].y += n * vv.y;
-                    VX[c].x += nx * vv.x; VX[c].y += nx * vv.y;
-                    VZ[c].x += nz * vv.x; VZ[c].y += nz * vv.y;
+                    const double2 uu = u0[h * nee + a * 3 + c], vv = l0[h * nee + a * 3 + c];
+                    U[h][c].x += n * uu.x; U[h][c].y += n * uu.y;
+                    UX[h][c].x += nx * uu.x; UX[h][c].y += nx * uu.y;
+                    UZ[h][c].x += nz * uu.x; UZ[h][c].y += nz * uu.y;
+                    V[h][c].x += n * vv.x; V[h][c].y += n * vv.y;
+                    VX[h][c].x += nx * vv.x; VX[h][c].y += nx * vv.y;
+                    VZ[h][c].x += nz * vv.x; VZ[h][c].y += nz * vv.y;
                 }
-            }
+        }
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            const int t = t0 + ta + h;
+            if (t >= ntrip) break;
             const double k = ky[t], w = omega[t];
             // gradients G[i][j], j = x, y, z; d/dy = -i ky
             double2 GU[3][3], GV[3][3];
+#pragma unroll
             for (int c = 0; c < 3; ++c) {
-                GU[c][0] = UX[c]; GU[c][2] = UZ[c]; GU[c][1] = make_double2(k * U[c].y, -k * U[c].x);
-                GV[c][0] = VX[c]; GV[c][2] = VZ[c]; GV[c][1] = make_double2(k * V[c].y, -k * V[c].x);
+                GU[c][0] = UX[h][c]; GU[c][2] = UZ[h][c]; GU[c][1] = make_double2(k * U[h][c].y, -k * U[h][c].x);
+                GV[c][0] = VX[h][c]; GV[c][2] = VZ[h][c]; GV[c][1] = make_double2(k * V[h][c].y, -k * V[h][c].x);
             }
-            double2 du = make_double2(GU[0][0].x + GU[1][1].x + GU[2][2].x, GU[0][0].y + GU[1][1].y + GU[2][2].y);
-            double2 dv = make_double2(GV[0][0].x + GV[1][1].x + GV[2][2].x, GV[0][0].y + GV[1][1].y + GV[2][2].y);
+            const double2 du = make_double2(GU[0][0].x + GU[1][1].x + GU[2][2].x, GU[0][0].y + GU[1][1].y + GU[2][2].y);
+            const double2 dv = make_double2(GV[0][0].x + GV[1][1].x + GV[2][2].x, GV[0][0].y + GV[1][1].y + GV[2][2].y);
             Dl += dv.x * du.x + dv.y * du.y;
             double sm2 = 0.0;
+#pragma unroll
             for (int i = 0; i < 3; ++i)
+#pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    double2 s2 = make_double2(GU[i][j].x + GU[j][i].x, GU[i][j].y + GU[j][i].y);
+                    const double2 s2 = make_double2(GU[i][j].x + GU[j][i].x, GU[i][j].y + GU[j][i].y);
                     sm2 += GV[i][j].x * s2.x + GV[i][j].y * s2.y;
                 }
             Dm += sm2;
-            Dr += -w * w * (V[0].x * U[0].x + V[0].y * U[0].y + V[1].x * U[1].x + V[1].y * U[1].y +
-                            V[2].x * U[2].x + V[2].y * U[2].y);
+            Dr += -w * w * (V[h][0].x * U[h][0].x + V[h][0].y * U[h][0].y + V[h][1].x * U[h][1].x +
+                            V[h][1].y * U[h][1].y + V[h][2].x * U[h][2].x + V[h][2].y * U[h][2].y);
         }
-        if (act) {
-            double wt = wq[ix] * wq[iz] * g.det;
-            for (int k = 0; k < 4; ++k) {
-                g12[k * 3 + 0] += wt * g.sf[k] * Dl;
-                g12[k * 3 + 1] += wt * g.sf[k] * Dm;
-                g12[k * 3 + 2] += wt * g.sf[k] * Dr;
-            }
-        }
+    }
+    double g12[12];
+    const double wt = act ? wq[ix] * wq[iz] * g.det : 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double f = act ? wt * g.sf[k] : 0.0;
+        g12[k * 3 + 0] = f * Dl;
+        g12[k * 3 + 1] = f * Dm;
+        g12[k * 3 + 2] = f * Dr;
     }
     // block reduction of the 12 partial sums
     for (int i = 0; i < 12; ++i) sred[threadIdx.x * 12 + i] = g12[i];
@@ -983,7 +1068,7 @@ __global__ void k_fdiag(int nnpg, int ncon, int nee, const int *__restrict__ mco
 // or DVS = 2 beta_a N_k, DVP = DVS VPVS^2 ('SS') (djac.f:94-103,
 // vforce25.f:164-179).  DKM_VPVS (dstiff25.f:570-656) applied to u_e
 // matrix-free at the quadrature points, times ELEM_WTS(e).
-__global__ void k_cfmat(int nen, int nq1, int nnpgl, const int *__restrict__ col_node,
+__global__ __launch_bounds__(256) void k_cfmat(int nen, int nq1, int nnpgl, const int *__restrict__ col_node,
                         const int *__restrict__ col_elem, const int *__restrict__ col_corner,
                         const double *__restrict__ col_cp, const double *__restrict__ col_cs,
                         const double *__restrict__ shl, const double *__restrict__ wq,
@@ -1114,6 +1199,21 @@ __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *
 template <typename T>
 __global__ void k_fill(T *p, size_t n, T v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// per-pair max over all DOFs of |Re x|, |Im x| for x[dof][pair]; non-negative
+// doubles order like their bit patterns, so an integer max reduces them
+__global__ void k_colmax(int64_t ndof, int npair, const double2 *__restrict__ x,
+                         unsigned long long *__restrict__ cm) {
+    const int pair = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (pair >= npair) return;
+    const int wpb = blockDim.x / 64;
+    double mx = 0.0;
+    for (int64_t d = (int64_t)blockIdx.y * wpb + threadIdx.x / 64; d < ndof; d += (int64_t)gridDim.y * wpb) {
+        const double2 v = x[d * npair + pair];
+        mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+    }
+    atomicMax(cm + pair, (unsigned long long)__double_as_longlong(mx));
 }
 
 // [pair][dof] <-> [dof][pair] transposes for the host convenience wrappers

@@ -127,7 +127,7 @@ def test_apply_batch_config1_modes(torch_dev):
     er = max(relerr(yr.cpu().numpy().T[b], yref[b]) for b in range(om.size))
     print(f"config-1 normwise error: fp32 {e32:.2e}, fp32-refined {er:.2e}")
     assert e32 < 1e-5
-    assert er < 1e-6
+    assert er < 1e-8   # FP32 MFMA with the fixed-point hi/lo split
 
 
 def test_apply_quadratic_in_ky():
