@@ -334,6 +334,21 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW)) void k_apply(c
     const int lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, kg = lane >> 4, ri = col & 1;
     const int ntile = (npair + 7) / 8;
+    // x operands of node 4s + kg for tile tt (zero outside the element, for
+    // fixed DOFs and for pairs beyond npair)
+    auto load_x = [&](int tt, int s, OS *dst) {
+        const int pr = tt * 8 + (col >> 1);
+        const int b = 4 * s + kg;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int dof = (b < NEN && pr < npair) ? sLM[b * 3 + d] : -1;
+            dst[d] = dof >= 0 ? (OS) reinterpret_cast<const XS *>(x + (size_t)dof * npair + pr)[ri] : (OS)0;
+        }
+    };
+    // the first step of the next tile is loaded during the last step of the
+    // current one
+    OS xn[3];
+    if (wave < ntile) load_x(wave, 0, xn);
 #pragma unroll 1
     for (int t = wave; t < ntile; t += NW) {
         const int pair = t * 8 + (col >> 1);
@@ -352,17 +367,6 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW)) void k_apply(c
         // the y preload is branch-free: rows outside the element, fixed DOFs
         // and first touches read a valid address (row 0) and are zeroed
         const int pcl = live ? pair : 0;
-        auto load_x = [&](int s, OS *dst) {
-            const int b = 4 * s + kg;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const int dof = (b < NEN && live) ? sLM[b * 3 + d] : -1;
-                dst[d] = dof >= 0 ? (OS) reinterpret_cast<const XS *>(x + (size_t)dof * npair + pair)[ri]
-                                  : (OS)0;
-            }
-        };
-        OS xn[3];
-        load_x(0, xn);
         XS yv[PRE ? NC : 1][PRE ? MT : 1][4];
         if (PRE) {
 #pragma unroll
@@ -392,7 +396,8 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW)) void k_apply(c
 #pragma unroll 1
         for (int s = 0; s < KS; ++s) {
             const OS xv[3] = {xn[0], xn[1], xn[2]};
-            if (s + 1 < KS) load_x(s + 1, xn);
+            if (s + 1 < KS) load_x(t, s + 1, xn);
+            else if (t + NW < ntile) load_x(t + NW, 0, xn);
             const int k = min(4 * s + kg, NEN - 1);
             // operand kinds of each x component: 0 x, 1 +i ky x, 2 -i ky x, 3 ky^2 x, 4 -w^2 x
             OS bop[3][5];
