@@ -239,6 +239,31 @@ class MeshInfo:
                                       u.ctypes.data_as(C.c_void_p), _p(d, dp)), "f25_gn_diag")
         return d
 
+    def pchadj_jac(self, nrhs: int, g: np.ndarray, fmat: np.ndarray, hdiag=None):
+        """PCHADJ's J = G^T F: g (nrhs, lrhs) complex64, fmat (NZ_FDIST,);
+        returns jac (nnpgl, nrhs) complex64 and the accumulated hdiag."""
+        inv = self.inv
+        g = np.ascontiguousarray(g, np.complex64)
+        fmat = np.ascontiguousarray(fmat, np.complex64)
+        jac = np.zeros((inv.nnpgl, nrhs), np.complex64)
+        hd = np.zeros(inv.na35, np.float32) if hdiag is None else np.ascontiguousarray(hdiag, np.float32)
+        check(_lib.load().f25_pchadj_jac(self.h, nrhs, g.shape[1], g.ctypes.data_as(C.c_void_p),
+                                         fmat.ctypes.data_as(C.c_void_p), jac.ctypes.data_as(C.c_void_p),
+                                         _p(hd, fp)), "f25_pchadj_jac")
+        return jac, hd
+
+    def gnmfree(self, nobs, alpha, jac, p, z=None) -> np.ndarray:
+        """GNMFREE's z += sum_j alpha_j Re(J_j^H J_j p); jac packed (j, c, obs)."""
+        inv = self.inv
+        nobs = np.ascontiguousarray(nobs, np.int32)
+        alpha = np.ascontiguousarray(alpha, np.float64)
+        jac = np.ascontiguousarray(jac, np.complex64)
+        p = np.ascontiguousarray(p, np.float64)
+        z = np.zeros(inv.na35) if z is None else np.ascontiguousarray(z, np.float64)
+        check(_lib.load().f25_gnmfree(self.h, nobs.size, _p(nobs, ip), _p(alpha, dp),
+                                      jac.ctypes.data_as(C.c_void_p), _p(p, dp), _p(z, dp)), "f25_gnmfree")
+        return z
+
     # -- inversion bookkeeping -------------------------------------------------
     def inv_setup(self, cinvtype: bytes = b"PP", vpvs: float = 3 ** 0.5, linrec: bool = True,
                   lkbdry: bool = True, mrdof1=None) -> InvInfo:

@@ -710,9 +710,11 @@ extern "C" int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, 
     return apply_core(m, npair, m->d_omky, m->d_omky + m->omky_cap, mode, x, y, m->d_blk);
 }
 
-static int transpose_dev(hipStream_t s, const double2 *in, double2 *out, int rows, int cols) {
-    dim3 b(16, 16), g((cols + 15) / 16, (rows + 15) / 16);
-    hipLaunchKernelGGL(k_transpose_c, g, b, 0, s, in, out, rows, cols);
+template <typename T>
+static int transpose_dev(hipStream_t s, const T *in, T *out, int rows, int cols) {
+    const size_t nt = (size_t)((cols + 15) / 16) * ((rows + 15) / 16);
+    if (nt > 0x7fffffff) FAIL(F25_ERR_ARG, "transpose too large");
+    hipLaunchKernelGGL(k_transpose_c<T>, dim3((unsigned)nt), dim3(16, 16), 0, s, in, out, rows, cols);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1083,6 +1085,128 @@ extern "C" int f25_adjflam_csc(int nnpgl, int ndof, int nz_fdist, int na35, cons
         if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
     }
     dfree(d_my); dfree(d_jc); dfree(d_ic); dfree(df); dfree(dl); dfree(dg);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobians (PCHADJ) and the Gauss-Newton contraction (GNMFREE)
+static int pchadj_core(f25_mesh *m, int nrhs, const float2 *gt, const float2 *f, float2 *jac, float *hd) {
+    const int n = m->inv.nnpgl;
+    HIPCHK(hipEventRecord(m->ev0, m->st));
+    hipLaunchKernelGGL(k_pchadj, dim3((n + 3) / 4), dim3(256), 0, m->st, n, nrhs, m->d_mygrad, m->d_jcsc,
+                       m->d_icsc, gt, f, jac, hd);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(m->ev1, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    record_ms("pchadj", ms);
+    return 0;
+}
+
+extern "C" int f25_pchadj_jac_dev(f25_mesh *m, int nrhs, const void *gt, const void *fmat, void *jac,
+                                  float *hdiag) {
+    if (!m || nrhs <= 0 || !gt || !fmat || !jac || !hdiag) FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!m->have_inv) FAIL(F25_ERR_STATE, "inversion structures missing (f25_inv_setup / f25_inv_attach)");
+    HIPCHK(hipSetDevice(m->dev));
+    return pchadj_core(m, nrhs, (const float2 *)gt, (const float2 *)fmat, (float2 *)jac, hdiag);
+}
+
+extern "C" int f25_pchadj_jac(f25_mesh *m, int nrhs, int lrhs, const void *grnsfns, const void *fmat,
+                              void *jac, float *hdiag) {
+    if (!m || nrhs <= 0 || !grnsfns || !fmat || !jac || !hdiag) FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!m->have_inv) FAIL(F25_ERR_STATE, "inversion structures missing (f25_inv_setup / f25_inv_attach)");
+    if (lrhs < m->g.ndof) FAIL(F25_ERR_ARG, "LRHS smaller than NDOF");
+    HIPCHK(hipSetDevice(m->dev));
+    const InvInfo &inv = m->inv;
+    float2 *dg = nullptr, *dgt = nullptr, *df = nullptr, *dj = nullptr;
+    float *dh = nullptr;
+    const size_t ng = (size_t)nrhs * lrhs;
+    int r = upload(&dg, (const float2 *)grnsfns, ng, m->st);
+    if (!r) r = dalloc(&dgt, ng);
+    if (!r) r = transpose_dev(m->st, (const float2 *)dg, dgt, nrhs, lrhs);
+    if (!r) r = upload(&df, (const float2 *)fmat, (size_t)inv.nz_fdist, m->st);
+    if (!r) r = dalloc(&dj, (size_t)nrhs * inv.nnpgl);
+    if (!r) r = upload(&dh, hdiag, (size_t)inv.na35, m->st);
+    if (!r) r = pchadj_core(m, nrhs, dgt, df, dj, dh);
+    if (!r) {
+        hipError_t e = hipMemcpyAsync(jac, dj, (size_t)nrhs * inv.nnpgl * sizeof(float2), hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(hdiag, dh, (size_t)inv.na35 * sizeof(float), hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(dg); dfree(dgt); dfree(df); dfree(dj); dfree(dh);
+    return r;
+}
+
+static int gnmfree_core(f25_mesh *m, int njac, const int *nobs, const double *alpha, const float2 *jac,
+                        const double *p, double *z) {
+    const int n = m->inv.nnpgl;
+    std::vector<int64_t> joff(njac), poff(njac);
+    int64_t jt = 0, pt = 0;
+    for (int j = 0; j < njac; ++j) {
+        if (nobs[j] < 0) FAIL(F25_ERR_ARG, "negative observation count");
+        joff[j] = jt;
+        poff[j] = pt;
+        jt += (int64_t)nobs[j] * n;
+        pt += nobs[j];
+    }
+    int *dn = nullptr;
+    int64_t *djo = nullptr, *dpo = nullptr;
+    double *da = nullptr;
+    double2 *djp = nullptr;
+    int r = upload(&dn, nobs, (size_t)njac, m->st);
+    if (!r) r = upload(&djo, joff.data(), (size_t)njac, m->st);
+    if (!r) r = upload(&dpo, poff.data(), (size_t)njac, m->st);
+    if (!r) r = upload(&da, alpha, (size_t)njac, m->st);
+    if (!r) r = dalloc(&djp, (size_t)std::max<int64_t>(pt, 1));
+    if (!r) {
+        fill_async(djp, (size_t)std::max<int64_t>(pt, 1), make_double2(0.0, 0.0), m->st);
+        const int cch = 64;
+        (void)hipEventRecord(m->ev0, m->st);
+        hipLaunchKernelGGL(k_gn_jp, dim3((n + cch - 1) / cch, njac), dim3(256), 0, m->st, n, cch, dn, djo, dpo,
+                           m->d_mygrad, jac, p, djp);
+        hipLaunchKernelGGL(k_gn_z, dim3((n + 3) / 4), dim3(256), 0, m->st, n, njac, dn, djo, dpo, da,
+                           m->d_mygrad, jac, djp, z);
+        (void)hipEventRecord(m->ev1, m->st);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+        float ms = 0.f;
+        if (!r && hipEventElapsedTime(&ms, m->ev0, m->ev1) == hipSuccess) record_ms("gnmfree", ms);
+    }
+    dfree(dn); dfree(djo); dfree(dpo); dfree(da); dfree(djp);
+    return r;
+}
+
+extern "C" int f25_gnmfree_dev(f25_mesh *m, int njac, const int *nobs, const double *alpha, const void *jac,
+                               const double *p, double *z) {
+    if (!m || njac <= 0 || njac > 65535 || !nobs || !alpha || !jac || !p || !z) FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!m->have_inv) FAIL(F25_ERR_STATE, "inversion structures missing (f25_inv_setup / f25_inv_attach)");
+    HIPCHK(hipSetDevice(m->dev));
+    return gnmfree_core(m, njac, nobs, alpha, (const float2 *)jac, p, z);
+}
+
+extern "C" int f25_gnmfree(f25_mesh *m, int njac, const int *nobs, const double *alpha, const void *jac,
+                           const double *p, double *z) {
+    if (!m || njac <= 0 || njac > 65535 || !nobs || !alpha || !jac || !p || !z) FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!m->have_inv) FAIL(F25_ERR_STATE, "inversion structures missing (f25_inv_setup / f25_inv_attach)");
+    HIPCHK(hipSetDevice(m->dev));
+    const InvInfo &inv = m->inv;
+    size_t tot = 0;
+    for (int j = 0; j < njac; ++j) tot += (size_t)std::max(nobs[j], 0) * inv.nnpgl;
+    float2 *dj = nullptr;
+    double *dp_ = nullptr, *dz = nullptr;
+    int r = upload(&dj, (const float2 *)jac, tot, m->st);
+    if (!r) r = upload(&dp_, p, (size_t)inv.na35, m->st);
+    if (!r) r = upload(&dz, z, (size_t)inv.na35, m->st);
+    if (!r) r = gnmfree_core(m, njac, nobs, alpha, dj, dp_, dz);
+    if (!r) {
+        hipError_t e = hipMemcpyAsync(z, dz, (size_t)inv.na35 * sizeof(double), hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(dj); dfree(dp_); dfree(dz);
     return r;
 }
 

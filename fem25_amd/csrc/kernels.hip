@@ -1200,6 +1200,96 @@ __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *
 }
 
 // ---------------------------------------------------------------------------
+// Jacobian of one (frequency, source): J(irhs, c) = sum over column c of F
+// of G(ICSC(i), irhs) F(i)  (PCHADJ's product, pchadj.f90:239-254), with
+// G transposed to [dof][rhs] so a wave reads one contiguous row of G per
+// nonzero of F.  One wave per column, lanes over the right-hand sides,
+// FP64 accumulation; hdiag(MYGRAD(c)) += sum_irhs |J|^2 (the 1x1 CGEMM of
+// pchadj.f90:264-281).  jac[c*nrhs + irhs].
+__global__ void k_pchadj(int nnpgl, int nrhs, const int *__restrict__ mygrad,
+                         const int *__restrict__ jcsc, const int *__restrict__ icsc,
+                         const float2 *__restrict__ gt, const float2 *__restrict__ f,
+                         float2 *__restrict__ jac, float *__restrict__ hdiag) {
+    const int c = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
+    if (c >= nnpgl) return;
+    const int i1 = jcsc[c] - 1, i2 = jcsc[c + 1] - 1;
+    double hs = 0.0;
+    for (int r0 = 0; r0 < nrhs; r0 += 64) {
+        const int r = r0 + lane;
+        double ar = 0.0, ai = 0.0;
+        if (r < nrhs) {
+            for (int i = i1; i < i2; ++i) {
+                const int row = icsc[i] - 1;
+                if (row < 0) continue;
+                const float2 fv = f[i], g = gt[(size_t)row * nrhs + r];
+                ar += (double)g.x * fv.x - (double)g.y * fv.y;
+                ai += (double)g.x * fv.y + (double)g.y * fv.x;
+            }
+            const float2 jv = make_float2((float)ar, (float)ai);
+            jac[(size_t)c * nrhs + r] = jv;
+            hs += (double)jv.x * jv.x + (double)jv.y * jv.y;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) hs += __shfl_xor(hs, o);
+    if (lane == 0) hdiag[mygrad[c] - 1] += (float)hs;
+}
+
+// GNMFREE's products (gnmfree.f90:185-212) over a batch of Jacobians with
+// uneven observation counts, J_j stored column-major (nobs_j x nnpgl) at
+// jac + joff[j]:  JP_j(obs) = sum_c J_j(obs, c) p(MYGRAD(c))  (FP64).
+// Grid (column chunks, j); each block sums its chunk of columns for every
+// observation of J_j and adds the partial sums into jp (zeroed beforehand).
+__global__ void k_gn_jp(int nnpgl, int cch, const int *__restrict__ nobs, const int64_t *__restrict__ joff,
+                        const int64_t *__restrict__ poff, const int *__restrict__ mygrad,
+                        const float2 *__restrict__ jac, const double *__restrict__ p,
+                        double2 *__restrict__ jp) {
+    const int j = blockIdx.y;
+    const int no = nobs[j];
+    const int c0 = blockIdx.x * cch, c1 = min(nnpgl, c0 + cch);
+    if (c0 >= c1) return;
+    const float2 *J = jac + joff[j];
+    double2 *out = jp + poff[j];
+    for (int o = threadIdx.x; o < no; o += blockDim.x) {
+        double ar = 0.0, ai = 0.0;
+        for (int c = c0; c < c1; ++c) {
+            const double pv = p[mygrad[c] - 1];
+            const float2 v = J[(size_t)c * no + o];
+            ar += (double)v.x * pv;
+            ai += (double)v.y * pv;
+        }
+        atomicAdd(&out[o].x, ar);
+        atomicAdd(&out[o].y, ai);
+    }
+}
+
+// z(MYGRAD(c)) += sum_j alpha_j Re( sum_obs conj(J_j(obs, c)) JP_j(obs) ),
+// one wave per column.
+__global__ void k_gn_z(int nnpgl, int njac, const int *__restrict__ nobs, const int64_t *__restrict__ joff,
+                       const int64_t *__restrict__ poff, const double *__restrict__ alpha,
+                       const int *__restrict__ mygrad, const float2 *__restrict__ jac,
+                       const double2 *__restrict__ jp, double *__restrict__ z) {
+    const int c = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
+    if (c >= nnpgl) return;
+    double s = 0.0;
+    for (int j = 0; j < njac; ++j) {
+        const int no = nobs[j];
+        const float2 *J = jac + joff[j] + (size_t)c * no;
+        const double2 *P = jp + poff[j];
+        double sj = 0.0;
+        for (int o = lane; o < no; o += 64) {
+            const float2 v = J[o];
+            const double2 a = P[o];
+            sj += (double)v.x * a.x + (double)v.y * a.y;
+        }
+        s += alpha[j] * sj;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) z[mygrad[c] - 1] += s;
+}
+
+// ---------------------------------------------------------------------------
 // small utilities
 template <typename T>
 __global__ void k_fill(T *p, size_t n, T v) {
@@ -1222,13 +1312,17 @@ __global__ void k_colmax(int64_t ndof, int npair, const double2 *__restrict__ x,
 }
 
 // [pair][dof] <-> [dof][pair] transposes for the host convenience wrappers
-__global__ void k_transpose_c(const double2 *__restrict__ in, double2 *__restrict__ out, int rows, int cols) {
-    __shared__ double2 tile[16][17];
-    int c = blockIdx.x * 16 + threadIdx.x, r = blockIdx.y * 16 + threadIdx.y;
+// tiles numbered along a 1-D grid (either dimension may be long)
+template <typename T>
+__global__ void k_transpose_c(const T *__restrict__ in, T *__restrict__ out, int rows, int cols) {
+    __shared__ T tile[16][17];
+    const int ntc = (cols + 15) / 16;
+    const int tr = blockIdx.x / ntc, tcol = blockIdx.x % ntc;
+    int c = tcol * 16 + threadIdx.x, r = tr * 16 + threadIdx.y;
     if (r < rows && c < cols) tile[threadIdx.y][threadIdx.x] = in[(size_t)r * cols + c];
     __syncthreads();
-    c = blockIdx.y * 16 + threadIdx.x;
-    r = blockIdx.x * 16 + threadIdx.y;
+    c = tr * 16 + threadIdx.x;
+    r = tcol * 16 + threadIdx.y;
     if (r < cols && c < rows) out[(size_t)r * rows + c] = tile[threadIdx.x][threadIdx.y];
 }
 

@@ -188,6 +188,34 @@ int f25_asmb_peff_csr(int ndof, int ndofl, int nzloc, const char *cnp, const int
 int f25_adjflam_csc(int nnpgl, int ndof, int nz_fdist, int na35, const int *mygrad, const int *jcsc,
                     const int *icsc, const void *fmat, const void *adjlam, float *vwork);
 
+/* ---- Jacobians and the Gauss-Newton contraction --------------------------- */
+
+/* PCHADJ's element-sparse product for one (frequency, source)
+ * (pchadj.f90:225-281, NVINV = 1) on the handle's inversion structures:
+ *   JACL(irhs, c) = sum over column c of F of
+ *                   GRNSFNS((irhs-1)*LRHS + ICSC_FDIST(i)) * FMAT_DIST(i)
+ * for irhs = 1..NRHS (receivers x components), written to jac as
+ * JACL(NRHS, NNPGL) COMPLEX*8 column-major (RCJLOC + i CCJLOC), and
+ * hdiag(MYGRAD(c)) += sum_irhs |JACL(irhs, c)|^2 (REAL*4, NA35: the 1x1
+ * CGEMM('C','N') of the gradient pre-conditioner).  grnsfns COMPLEX*8
+ * (LRHS*NRHS, LRHS >= NDOF), fmat COMPLEX*8 (NZ_FDIST); host arrays. */
+int f25_pchadj_jac(f25_mesh *m, int nrhs, int lrhs, const void *grnsfns, const void *fmat, void *jac,
+                   float *hdiag);
+/* Same on device arrays; gt_dev holds G transposed: NDOF rows of NRHS
+ * COMPLEX*8 ([dof][rhs]). */
+int f25_pchadj_jac_dev(f25_mesh *m, int nrhs, const void *gt_dev, const void *fmat_dev, void *jac_dev,
+                       float *hdiag_dev);
+/* GNMFREE's contraction (gnmfree.f90:185-212) for a batch of njac
+ * Jacobians with uneven observation counts, packed one after the other
+ * (J_j: nobs[j] x NNPGL COMPLEX*8, column-major, as PCHADJ writes them):
+ *   z(MYGRAD(c)) += sum_j alpha[j] Re( J_j(:,c)^H (J_j p(MYGRAD)) )
+ * in FP64 (alpha = 1, or |source|^2 when migrating).  z and p hold NA35
+ * values.  nobs and alpha are host arrays in both forms. */
+int f25_gnmfree(f25_mesh *m, int njac, const int *nobs, const double *alpha, const void *jac,
+                const double *p, double *z);
+int f25_gnmfree_dev(f25_mesh *m, int njac, const int *nobs, const double *alpha, const void *jac_dev,
+                    const double *p_dev, double *z_dev);
+
 /* ---- Gauss-Newton element products ---------------------------------------- */
 
 /* F v = dS/dm[v] u for a model perturbation v (NNPG,3) = (dVp, dVs, drho)

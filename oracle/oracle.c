@@ -32,6 +32,10 @@
  *   orc_dkm_vpvs     linearity in the collocation field; compiled reference.
  *   orc_cfmat_dist   compiled reference CFMAT_DIST/VFORCE25.
  *   orc_adjflam_dist hand-computed CSC gather on a 3-column example.
+ *   orc_pchadj_jac   dense J = G^T F and diag(J^H J) by numpy (pchadj.f90
+ *                    cannot be compiled here: it needs MUMPS headers).
+ *   orc_gnmfree      compiled reference GNMFREE reading Jacobian files in its
+ *                    own format; dense Re(J^H J p) by numpy.
  *   orc_grad_exact   central finite differences of the objective (exact for a
  *                    quadratic) on small meshes.
  */
@@ -1331,4 +1335,74 @@ int orc_grad_exact(const orc_asm *a, double freq, double py, const double comple
     free(fl); free(fm); free(fr); free(nk);
     ws_free(&w);
     return ierr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* pchadj.f90:225-281 with NVINV = 1: the loop over local gradient columns,
+ * receivers and components (KRHS = IRHS), the COMPLEX*8 inner product of a
+ * row of G^T with column INPGL of F (pchadj.f90:239-254), the Jacobian copy
+ * (:256-262) and the 1x1 CGEMM('C','N') into HBUFF (:264-281). */
+void orc_pchadj_jac(int nnpgl, int nrhs, int lrhs, const int *mygrad, const int *jcsc,
+                    const int *icsc, const float complex *g, const float complex *fmat,
+                    float complex *jac, float *hdiag)
+{
+    for (int c = 0; c < nnpgl; c++) {
+        int i1 = jcsc[c], i2 = jcsc[c + 1] - 1;
+        float complex hsum = 0.0f;
+        for (int irhs = 1; irhs <= nrhs; irhs++) {
+            float complex acc = 0.0f;
+            for (int inz = i1; inz <= i2; inz++) {
+                int jdof = icsc[inz - 1];
+                if (jdof == 0) continue;
+                acc += cmulf_ref(g[(size_t)(irhs - 1) * lrhs + (jdof - 1)], fmat[inz - 1]);
+            }
+            jac[(size_t)c * nrhs + (irhs - 1)] = acc;
+        }
+        for (int irhs = 0; irhs < nrhs; irhs++) {
+            float complex v = jac[(size_t)c * nrhs + irhs];
+            hsum += cmulf_ref(conjf(v), v);
+        }
+        hdiag[mygrad[c] - 1] += crealf(hsum);
+    }
+}
+
+/* gnmfree.f90:185-212: JP1(IOBS) = sum_INPGL DCMPLX(J) * DCMPLX(PW(ILOC), 0)
+ * (COMPLEX*16), RWORK(ILOC) += DREAL(conj(J) * JP2) and Z2 += ALPHAM * Z1
+ * (DAXPY), one Jacobian after the other.  The running sum starts at zero
+ * (the reference clears it with DSCAL(0) on fresh memory, gnmfree.f90:60,
+ * which is zero unless the heap held NaN there). */
+void orc_gnmfree(int njac, const int *nobs, const double *alpha, int nnpgl, const int *mygrad,
+                 int na35, const float complex *jac, const double *p, double *z)
+{
+    size_t off = 0;
+    double *rwork = calloc((size_t)na35, sizeof(double));
+    int mobs = 0;
+    for (int j = 0; j < njac; j++) mobs = nobs[j] > mobs ? nobs[j] : mobs;
+    double complex *jp = calloc((size_t)(mobs > 0 ? mobs : 1), sizeof(double complex));
+    for (int j = 0; j < njac; j++) {
+        const float complex *J = jac + off;
+        const int no = nobs[j];
+        for (int iobs = 0; iobs < no; iobs++) {
+            double complex acc = 0.0;
+            for (int c = 0; c < nnpgl; c++) {
+                float complex v = J[(size_t)c * no + iobs];
+                double pr = p[mygrad[c] - 1];
+                acc += CMPLX((double)crealf(v) * pr, (double)cimagf(v) * pr);
+            }
+            jp[iobs] = acc;
+        }
+        for (int i = 0; i < na35; i++) rwork[i] = 0.0;
+        for (int c = 0; c < nnpgl; c++) {
+            int iloc = mygrad[c] - 1;
+            for (int iobs = 0; iobs < no; iobs++) {
+                float complex v = J[(size_t)c * no + iobs];
+                double complex a = jp[iobs];
+                rwork[iloc] += (double)crealf(v) * creal(a) + (double)cimagf(v) * cimag(a);
+            }
+        }
+        for (int i = 0; i < na35; i++) z[i] += alpha[j] * rwork[i];
+        off += (size_t)no * nnpgl;
+    }
+    free(rwork);
+    free(jp);
 }

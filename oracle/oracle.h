@@ -140,6 +140,20 @@ int orc_cfmat_dist(const orc_asm *a, const orc_inv *inv, const char *cinvtype, d
 void orc_adjflam_dist(int nnpgl, int na35, const int *mygrad, const int *jcsc, const int *icsc,
                       const float complex *fmat, const float complex *adjlam, float *gprod);
 
+/* pchadj.f90:225-281 (NVINV = 1), one (frequency, source):
+ *   JACL(irhs, c) = sum over column c of F of G((irhs-1)*lrhs + ICSC(i)) FMAT(i)
+ * in COMPLEX*8 arithmetic; jac[c*nrhs + irhs] (RCJLOC/CCJLOC(KRHS, INPGL));
+ * hdiag(MYGRAD(c)) += REAL(sum_irhs conj(JACL) JACL) (the 1x1 CGEMM). */
+void orc_pchadj_jac(int nnpgl, int nrhs, int lrhs, const int *mygrad, const int *jcsc,
+                    const int *icsc, const float complex *g, const float complex *fmat,
+                    float complex *jac, float *hdiag);
+/* gnmfree.f90:95-212, one rank and one frequency group: for Jacobians j =
+ * 0..njac-1 packed one after the other (nobs[j] x nnpgl, column-major),
+ *   JP = J p(MYGRAD) (COMPLEX*16), RWORK(MYGRAD(c)) = Re(J(:,c)^H JP),
+ *   z += alpha[j] RWORK. */
+void orc_gnmfree(int njac, const int *nobs, const double *alpha, int nnpgl, const int *mygrad,
+                 int na35, const float complex *jac, const double *p, double *z);
+
 /* exact derivative of the assembled operator with respect to anchor
  * (lambda, mu, rho) and the chain rule to (Vp, Vs, rho):
  *   g[3*(anchor) + {0,1,2}] += Re( lam^H  dS/dm  u )  for interior (non-PML) elements */
