@@ -35,6 +35,7 @@ EXPORTS = {
     "f25_mesh_sizes": (C.c_int, [C.c_void_p, i64p]),
     "f25_mesh_graph": (C.c_int, [C.c_void_p, ip, ip, ip, ip, C.c_char_p]),
     "f25_set_model": (C.c_int, [C.c_void_p, dp, dp]),
+    "f25_set_attenuation": (C.c_int, [C.c_void_p, dp, dp, C.c_double]),
     "f25_asmble": (C.c_int, [C.c_void_p, C.c_char, C.c_double, C.c_double, C.c_int, C.c_void_p]),
     "f25_asmble_dev": (C.c_int, [C.c_void_p, C.c_char, C.c_double, C.c_double, C.c_int, C.c_void_p]),
     "f25_csr_apply_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),

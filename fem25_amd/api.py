@@ -138,6 +138,13 @@ class MeshInfo:
         dens = np.ascontiguousarray(model.dens)
         check(_lib.load().f25_set_model(self.h, _p(dens, dp), ec.ctypes.data_as(dp)),
               "f25_set_model")
+        if getattr(model, "qp", None) is not None:    # dispersive medium (LDISP)
+            qp = np.ascontiguousarray(model.qp, np.float64)
+            qs = np.ascontiguousarray(model.qs, np.float64)
+            check(_lib.load().f25_set_attenuation(self.h, _p(qp, dp), _p(qs, dp), float(model.freq0)),
+                  "f25_set_attenuation")
+        else:
+            check(_lib.load().f25_set_attenuation(self.h, None, None, 0.0), "f25_set_attenuation")
 
     # -- assembled operator -------------------------------------------------
     def asmble(self, cftype: bytes, freq: float, py: float, complex128: bool = False) -> np.ndarray:

@@ -56,6 +56,20 @@ class Model:
     ecoeff: np.ndarray    # (nnpg, 2): lambda, mu (Pa), Fortran ECOEFF(NNPG,2)
     vp: np.ndarray
     vs: np.ndarray
+    # dispersive medium (LDISP): quality factors at the anchors and the
+    # reference frequency FREQ0 (Hz) of VCMPLX (haskattn.f:602-641)
+    qp: np.ndarray | None = None
+    qs: np.ndarray | None = None
+    freq0: float = 0.0
+
+
+def with_attenuation(model: Model, qp: float | np.ndarray, qs: float | np.ndarray,
+                     freq0: float) -> Model:
+    """The same elastic model with quality factors QP, QS at the anchors."""
+    n = model.dens.size
+    return dataclasses.replace(model, qp=np.broadcast_to(np.asarray(qp, np.float64), (n,)).copy(),
+                               qs=np.broadcast_to(np.asarray(qs, np.float64), (n,)).copy(),
+                               freq0=float(freq0))
 
 
 @dataclasses.dataclass

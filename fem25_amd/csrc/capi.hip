@@ -78,6 +78,11 @@ struct f25_mesh_s {
     std::vector<int> order, cptr, pml_order, pml_cptr;
     std::vector<double> rho, lam, mu;
     bool have_model = false;
+    // dispersive medium (quality factors at anchors, reference frequency)
+    std::vector<double> qp, qs;
+    double freq0 = 0.0;
+    bool disp = false;
+    double *d_qp = nullptr, *d_qs = nullptr;
     hipStream_t st = nullptr;
     int dev = 0;
     // device
@@ -354,7 +359,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_xq); dfree(m->d_cdom); dfree(m->d_cnp); dfree(m->d_rho); dfree(m->d_lam); dfree(m->d_mu);
     dfree(m->d_blk); dfree(m->d_blk32); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_colmax); dfree(m->d_dblk); dfree(m->d_irptr);
     dfree(m->d_jcptr); dfree(m->d_comp); dfree(m->d_v0); dfree(m->d_v1); dfree(m->d_v2);
-    dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_pmap); dfree(m->d_col_elem); dfree(m->d_col_corner);
+    dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_pmap); dfree(m->d_qp); dfree(m->d_qs); dfree(m->d_col_elem); dfree(m->d_col_corner);
     dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_col_cp); dfree(m->d_col_cs);
     dfree(m->d_ew);
     dfree(m->d_omky);
@@ -473,6 +478,37 @@ extern "C" int f25_set_model(f25_mesh *m, const double *dens, const double *ecoe
     return 0;
 }
 
+extern "C" int f25_set_attenuation(f25_mesh *m, const double *qp, const double *qs, double freq0) {
+    if (!m) FAIL(F25_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(m->dev));
+    if (!qp || !qs) {
+        m->disp = false;
+        return 0;
+    }
+    const int n = m->d.nnpg;
+    if (m->disp && freq0 == m->freq0 && std::equal(qp, qp + n, m->qp.begin()) &&
+        std::equal(qs, qs + n, m->qs.begin()))
+        return 0;
+    m->qp.assign(qp, qp + n);
+    m->qs.assign(qs, qs + n);
+    m->freq0 = freq0;
+    if (!m->d_qp) {
+        TRY(dalloc(&m->d_qp, n));
+        TRY(dalloc(&m->d_qs, n));
+    }
+    HIPCHK(hipMemcpyAsync(m->d_qp, m->qp.data(), n * 8, hipMemcpyHostToDevice, m->st));
+    HIPCHK(hipMemcpyAsync(m->d_qs, m->qs.data(), n * 8, hipMemcpyHostToDevice, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    m->disp = true;
+    return 0;
+}
+
+// entry points built on the frequency-independent element blocks
+static int require_elastic(f25_mesh *m) {
+    if (m->disp) FAIL(F25_ERR_STATE, "dispersive media are supported by the assembled operator only");
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 static int ensure_basis(f25_mesh *m) {
     TRY(ensure_csr(m));
@@ -504,9 +540,17 @@ static int asmble_impl(f25_mesh *m, char cftype, double freq, double py, CT *out
     const double omega = 6.2831853071795862 * freq, kv = omega * py;
     const int ndof = m->g.ndof;
     HIPCHK(hipEventRecord(m->ev0, m->st));
-    hipLaunchKernelGGL((k_csr_combine<CT>), dim3((ndof + 3) / 4), dim3(256), 0, m->st, ndof,
-                       m->d_irptr, m->d_jcptr, m->d_comp, m->d_v0, m->d_v1, m->d_v2, omega, kv, out);
-    if (!m->pml_order.empty()) {
+    // dispersive media: every element is integrated per call (KMDAMP/CMDAMP);
+    // otherwise the frequency-independent basis gives the non-PML part
+    const bool disp = m->disp;
+    if (disp)
+        fill_async(out, (size_t)m->g.irptr[ndof], CT{0, 0}, m->st);
+    else
+        hipLaunchKernelGGL((k_csr_combine<CT>), dim3((ndof + 3) / 4), dim3(256), 0, m->st, ndof,
+                           m->d_irptr, m->d_jcptr, m->d_comp, m->d_v0, m->d_v1, m->d_v2, omega, kv, out);
+    const std::vector<int> &cp = disp ? m->cptr : m->pml_cptr;
+    const int *ord = disp ? m->d_order : m->d_pml_order;
+    if (disp || !m->pml_order.empty()) {
         PmlPar p;
         p.omega = omega;
         p.kv = kv;
@@ -514,14 +558,17 @@ static int asmble_impl(f25_mesh *m, char cftype, double freq, double py, CT *out
         p.zwidth = m->d.zwidth;
         const double *pp = (cftype == 'S') ? m->d.pml_srf : m->d.pml;
         p.rcoeff = pp[0]; p.dmax = pp[1]; p.kappa = pp[2]; p.fcent = pp[3];
+        p.freq = freq;
+        p.freq0 = m->freq0;
         const int nen = m->tab.nen;
-        size_t lds = (8 * 7 + 3 * 8 * nen) * sizeof(double);
-        for (size_t c = 0; c + 1 < m->pml_cptr.size(); ++c) {
-            int n = m->pml_cptr[c + 1] - m->pml_cptr[c];
+        size_t lds = (8 * 9 + 3 * 8 * nen) * sizeof(double);
+        for (size_t c = 0; c + 1 < cp.size(); ++c) {
+            int n = cp[c + 1] - cp[c];
             if (!n) continue;
             hipLaunchKernelGGL((k_pml<CT>), dim3(n), dim3(256), lds, m->st, nen, m->tab.nq1, n,
-                               m->d_pml_order + m->pml_cptr[c], m->d_shl, m->d_wq, m->d_xq, m->d_ieng,
-                               m->d_xl, m->d_zl, m->d_xd, m->d_zd, m->d_rho, m->d_lam, m->d_mu, m->d_lm,
+                               ord + cp[c], m->d_shl, m->d_wq, m->d_xq, m->d_ieng,
+                               m->d_xl, m->d_zl, m->d_xd, m->d_zd, m->d_cdom, m->d_rho, m->d_lam, m->d_mu,
+                               disp ? m->d_qp : nullptr, disp ? m->d_qs : nullptr, m->d_lm,
                                m->d_irptr, m->d_jcptr, p, out, m->d_err);
         }
     }
@@ -703,6 +750,7 @@ static int stage_omky(f25_mesh *m, int n, const double *omega, const double *ky)
 
 extern "C" int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
                                    int mode, const void *x, void *y) {
+    if (m) TRY(require_elastic(m));
     if (!m || npair <= 0 || !omega || !ky || !x || !y) FAIL(F25_ERR_ARG, "invalid apply arguments");
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
     HIPCHK(hipSetDevice(m->dev));
@@ -721,6 +769,7 @@ static int transpose_dev(hipStream_t s, const T *in, T *out, int rows, int cols)
 
 extern "C" int f25_apply_batch(f25_mesh *m, int npair, const double *omega, const double *ky, int mode,
                                const void *x, void *y) {
+    if (m) TRY(require_elastic(m));
     if (!m || npair <= 0 || !x || !y) FAIL(F25_ERR_ARG, "invalid apply arguments");
     if (mode == F25_APPLY_F32) FAIL(F25_ERR_ARG, "host wrapper takes double complex data; use F64 or F32_REFINED");
     HIPCHK(hipSetDevice(m->dev));
@@ -773,6 +822,7 @@ static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d
 
 extern "C" int f25_gradient_dev(f25_mesh *m, int ntrip, const double *omega, const double *ky,
                                 const void *u, const void *lam, double *g) {
+    if (m) TRY(require_elastic(m));
     if (!m || ntrip <= 0 || !u || !lam || !g) FAIL(F25_ERR_ARG, "invalid gradient arguments");
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
     HIPCHK(hipSetDevice(m->dev));
@@ -783,6 +833,7 @@ extern "C" int f25_gradient_dev(f25_mesh *m, int ntrip, const double *omega, con
 
 static int grad_host(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
                      const void *lam, double *g, int chain) {
+    if (m) TRY(require_elastic(m));
     if (!m || ntrip <= 0 || !u || !lam || !g) FAIL(F25_ERR_ARG, "invalid gradient arguments");
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
     HIPCHK(hipSetDevice(m->dev));
@@ -1214,6 +1265,7 @@ extern "C" int f25_gnmfree(f25_mesh *m, int njac, const int *nobs, const double 
 // Gauss-Newton element products
 extern "C" int f25_gn_fv(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
                          const double *v, void *out) {
+    if (m) TRY(require_elastic(m));
     if (!m || ntrip <= 0 || !u || !v || !out) FAIL(F25_ERR_ARG, "invalid arguments");
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
     HIPCHK(hipSetDevice(m->dev));
@@ -1293,6 +1345,7 @@ static int ensure_pmap(f25_mesh *m) {
 
 extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
                            double *diag) {
+    if (m) TRY(require_elastic(m));
     if (!m || ntrip <= 0 || !omega || !ky || !u || !diag) FAIL(F25_ERR_ARG, "invalid arguments");
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
     HIPCHK(hipSetDevice(m->dev));

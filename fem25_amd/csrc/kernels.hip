@@ -563,7 +563,16 @@ __global__ void k_csr_combine(int ndof, const int64_t *__restrict__ irptr, const
 // ---------------------------------------------------------------------------
 // PML elements (CJACCQ + CM25ISOQ, cjac.f:399-581, estiff25.f:168-334),
 // integrated per call and added into the assembled operator.
-struct PmlPar { double omega, kv, xwidth, zwidth, rcoeff, dmax, kappa, fcent; };
+struct PmlPar { double omega, kv, xwidth, zwidth, rcoeff, dmax, kappa, fcent, freq, freq0; };
+
+// haskattn.f:602-641 VCMPLX: V (1 + ln(f/f0)/(pi Q)) - i V/(2Q), real factor
+// clamped at 0.1; real V when FREQ0 = 0, Q = 9999 or Q = 0
+__device__ inline double2 vcmplx(double freq, double freq0, double v, double q) {
+    if (freq0 == 0.0 || fabs(q - 9999.0) < 1e-5 || q == 0.0) return make_double2(v, 0.0);
+    double fact = 1.0 + 1.0 / (M_PI * q) * log(freq / freq0);
+    if (fact < 0.1) fact = 0.1;
+    return make_double2(v * fact, -v / (2.0 * q));
+}
 
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline double2 cdiv1(double2 z) { double d = z.x * z.x + z.y * z.y; return make_double2(z.x / d, -z.y / d); }
@@ -577,39 +586,53 @@ __device__ inline double2 zdampfn(double rn, double vp, double delta, const PmlP
     return cdiv1(make_double2(p.kappa + fabs(d) * t.x, fabs(d) * t.y));
 }
 
+// Element matrices integrated per call and added into CSR order:
+//  * PML elements ('A', CM25ISOQ estiff25.f:168-334 with CJACCQ's stretch)
+//  * with quality factors (qp != null) every element in the density-
+//    normalised dispersive form: KMDAMP (estiff25.f:511-752) and, for 'A'
+//    elements, CMDAMP (estiff25.f:756-961), velocities from VCMPLX.
+// Both are the KM25ISOQ bilinear forms with complex coefficients at each
+// quadrature point: lambda -> L, mu -> M, rho -> R, derivatives stretched by
+// 1/e_x, 1/e_z (ZDAMPFN) inside PML elements:
+//   standard:   L = w lambda, M = w mu, R = w rho
+//   dispersive: L = w (Vp~^2 - 2 Vs~^2), M = w Vs~^2, R = w.
+// One workgroup per element of one colour.
 template <typename CT>
 __global__ void k_pml(int nen, int nq1, int ncol_elem, const int *__restrict__ order,
                       const double *__restrict__ shl, const double *__restrict__ wq,
                       const double *__restrict__ xq, const int *__restrict__ ieng,
                       const double *__restrict__ xl, const double *__restrict__ zl,
                       const double *__restrict__ xd, const double *__restrict__ zd,
+                      const char *__restrict__ cdom,
                       const double *__restrict__ rho, const double *__restrict__ lam,
-                      const double *__restrict__ mu, const int *__restrict__ lm,
+                      const double *__restrict__ mu, const double *__restrict__ qp,
+                      const double *__restrict__ qs, const int *__restrict__ lm,
                       const int64_t *__restrict__ irptr, const int *__restrict__ jcptr, PmlPar par,
                       CT *__restrict__ out, int *__restrict__ err) {
     extern __shared__ double sm[];
     const int e = order[blockIdx.x];
     const int nq = nq1 * nq1;
-    // LDS: per q: w*lam, w*mu, w*rho, ex(2), ez(2); then N, Nx, Nz per a for a chunk
-    const int QC = 8;
-    double *sq = sm;                 // QC*7
-    double *sN = sq + QC * 7;        // QC*nen
+    // LDS: per q: L(2), M(2), R, ex(2), ez(2); then N, Nx, Nz per a for a chunk
+    const int QC = 8, QS = 9;
+    double *sq = sm;                 // QC*QS
+    double *sN = sq + QC * QS;       // QC*nen
     double *sX = sN + QC * nen;
     double *sZ = sX + QC * nen;
     const int *ie = ieng + 4 * e;
+    const bool pml = cdom[e] == 'A';
     const int np = nen * nen;
     const double k2 = par.kv * par.kv, w2 = par.omega * par.omega;
     for (int p0 = 0; p0 < np; p0 += blockDim.x) {
         const int pr = p0 + threadIdx.x;
         const int a = pr / nen, b = pr % nen;
-        double2 acc[9];
+        double2 acc[9], me[3];
         for (int i = 0; i < 9; ++i) acc[i] = make_double2(0.0, 0.0);
-        double me[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < 3; ++i) me[i] = make_double2(0.0, 0.0);
         for (int q0 = 0; q0 < nq; q0 += QC) {
             __syncthreads();
             for (int i = threadIdx.x; i < QC * nen; i += blockDim.x) {
                 int qq = i / nen, aa = i % nen, q = q0 + qq;
-                if (q >= nq) { sN[i] = sX[i] = sZ[i] = 0.0; if (aa == 0) for (int j = 0; j < 7; ++j) sq[qq * 7 + j] = 0.0; continue; }
+                if (q >= nq) { sN[i] = sX[i] = sZ[i] = 0.0; if (aa == 0) for (int j = 0; j < QS; ++j) sq[qq * QS + j] = 0.0; continue; }
                 int ix = q % nq1, iz = q / nq1;
                 Geo g;
                 int bad = geo_at(xq[ix], xq[iz], ie, xl, zl, g);
@@ -621,60 +644,86 @@ __global__ void k_pml(int nen, int nq1, int ncol_elem, const int *__restrict__ o
                 sZ[i] = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / rj;
                 if (aa == 0) {
                     double w = wq[ix] * wq[iz] * g.det, r = 0, l = 0, m = 0, xdep = 0, zdep = 0, vel = 0;
+                    double vsv = 0, qpv = 0, qsv = 0;
                     for (int k = 0; k < 4; ++k) {
                         int gi = ie[k] - 1;
                         r += g.sf[k] * rho[gi];
                         l += g.sf[k] * lam[gi];
                         m += g.sf[k] * mu[gi];
-                        xdep += g.sf[k] * xd[gi];
-                        zdep += g.sf[k] * zd[gi];
+                        if (pml) {
+                            xdep += g.sf[k] * xd[gi];
+                            zdep += g.sf[k] * zd[gi];
+                        }
                         vel += g.sf[k] * sqrt((lam[gi] + 2.0 * mu[gi]) / rho[gi]);
+                        if (qp) {
+                            vsv += g.sf[k] * sqrt(mu[gi] / rho[gi]);
+                            qpv += g.sf[k] * qp[gi];
+                            qsv += g.sf[k] * qs[gi];
+                        }
                     }
                     double2 ex = xdep > 2.22e-16 ? zdampfn(xdep, vel, par.xwidth, par) : make_double2(1.0, 0.0);
                     double2 ez = zdep > 2.22e-16 ? zdampfn(zdep, vel, par.zwidth, par) : make_double2(1.0, 0.0);
-                    double *o = sq + qq * 7;
-                    o[0] = w * l; o[1] = w * m; o[2] = w * r;
-                    o[3] = ex.x; o[4] = ex.y; o[5] = ez.x; o[6] = ez.y;
+                    double *o = sq + qq * QS;
+                    if (qp) {
+                        const double2 zp = vcmplx(par.freq, par.freq0, vel, qpv);
+                        const double2 zs = vcmplx(par.freq, par.freq0, vsv, qsv);
+                        double2 zp2 = cmul(zp, zp), zs2 = cmul(zs, zs);
+                        zp2.x *= w; zp2.y *= w; zs2.x *= w; zs2.y *= w;
+                        o[0] = zp2.x - 2.0 * zs2.x; o[1] = zp2.y - 2.0 * zs2.y;
+                        o[2] = zs2.x; o[3] = zs2.y; o[4] = w;
+                    } else {
+                        o[0] = w * l; o[1] = 0.0; o[2] = w * m; o[3] = 0.0; o[4] = w * r;
+                    }
+                    o[5] = ex.x; o[6] = ex.y; o[7] = ez.x; o[8] = ez.y;
                 }
             }
             __syncthreads();
             if (pr < np) {
                 for (int qq = 0; qq < QC; ++qq) {
-                    const double *o = sq + qq * 7;
-                    double L = o[0], M = o[1], R = o[2];
-                    double2 ex = make_double2(o[3], o[4]), ez = make_double2(o[5], o[6]);
+                    const double *o = sq + qq * QS;
+                    const double2 L = make_double2(o[0], o[1]), M = make_double2(o[2], o[3]);
+                    const double R = o[4];
+                    const double2 ex = make_double2(o[5], o[6]), ez = make_double2(o[7], o[8]);
                     double na = sN[qq * nen + a], nb = sN[qq * nen + b];
                     double2 xa = make_double2(sX[qq * nen + a] * ex.x, sX[qq * nen + a] * ex.y);
                     double2 xb = make_double2(sX[qq * nen + b] * ex.x, sX[qq * nen + b] * ex.y);
                     double2 za = make_double2(sZ[qq * nen + a] * ez.x, sZ[qq * nen + a] * ez.y);
                     double2 zb = make_double2(sZ[qq * nen + b] * ez.x, sZ[qq * nen + b] * ez.y);
                     double2 xx = cmul(xa, xb), zz = cmul(za, zb), xz = cmul(xa, zb), zx = cmul(za, xb);
-                    double L2 = L + 2.0 * M;
-                    // real-coefficient blocks
-                    acc[0].x += L2 * xx.x + M * zz.x; acc[0].y += L2 * xx.y + M * zz.y;          // 11
-                    acc[4].x += M * (xx.x + zz.x);    acc[4].y += M * (xx.y + zz.y);             // 22
-                    acc[8].x += L2 * zz.x + M * xx.x; acc[8].y += L2 * zz.y + M * xx.y;          // 33
-                    acc[2].x += L * xz.x + M * zx.x;  acc[2].y += L * xz.y + M * zx.y;           // 13
-                    acc[6].x += L * zx.x + M * xz.x;  acc[6].y += L * zx.y + M * xz.y;           // 31
+                    const double2 L2 = make_double2(L.x + 2.0 * M.x, L.y + 2.0 * M.y);
+                    double2 t;
+                    t = cmul(L2, xx); acc[0].x += t.x; acc[0].y += t.y;                      // 11
+                    t = cmul(M, zz);  acc[0].x += t.x; acc[0].y += t.y;
+                    t = cmul(M, make_double2(xx.x + zz.x, xx.y + zz.y)); acc[4].x += t.x; acc[4].y += t.y;   // 22
+                    t = cmul(L2, zz); acc[8].x += t.x; acc[8].y += t.y;                      // 33
+                    t = cmul(M, xx);  acc[8].x += t.x; acc[8].y += t.y;
+                    t = cmul(L, xz);  acc[2].x += t.x; acc[2].y += t.y;                      // 13
+                    t = cmul(M, zx);  acc[2].x += t.x; acc[2].y += t.y;
+                    t = cmul(L, zx);  acc[6].x += t.x; acc[6].y += t.y;                      // 31
+                    t = cmul(M, xz);  acc[6].x += t.x; acc[6].y += t.y;
                     // i ky couplings: CE12 = i ky (L xa nb - M na xb), KE(1,2) -= CE12, etc.
-                    double2 t12 = make_double2(L * xa.x * nb - M * na * xb.x, L * xa.y * nb - M * na * xb.y);
-                    double2 t21 = make_double2(L * na * xb.x - M * xa.x * nb, L * na * xb.y - M * xa.y * nb);
-                    double2 t23 = make_double2(L * na * zb.x - M * za.x * nb, L * na * zb.y - M * za.y * nb);
-                    double2 t32 = make_double2(L * za.x * nb - M * na * zb.x, L * za.y * nb - M * na * zb.y);
+                    const double2 Lx = cmul(L, xa), Mx = cmul(M, xb), Lz = cmul(L, zb), Mz = cmul(M, za);
+                    const double2 Lxb = cmul(L, xb), Mxa = cmul(M, xa), Lza = cmul(L, za), Mzb = cmul(M, zb);
+                    double2 t12 = make_double2(Lx.x * nb - Mx.x * na, Lx.y * nb - Mx.y * na);
+                    double2 t21 = make_double2(Lxb.x * na - Mxa.x * nb, Lxb.y * na - Mxa.y * nb);
+                    double2 t23 = make_double2(Lz.x * na - Mz.x * nb, Lz.y * na - Mz.y * nb);
+                    double2 t32 = make_double2(Lza.x * nb - Mzb.x * na, Lza.y * nb - Mzb.y * na);
                     // multiply by i ky: (x, y) -> (-ky y, ky x)
                     acc[1].x += par.kv * t12.y;  acc[1].y -= par.kv * t12.x;    // -(i ky t12)
                     acc[3].x -= par.kv * t21.y;  acc[3].y += par.kv * t21.x;    // +(i ky t21)
                     acc[5].x -= par.kv * t23.y;  acc[5].y += par.kv * t23.x;    // +(i ky t23)
                     acc[7].x += par.kv * t32.y;  acc[7].y -= par.kv * t32.x;    // -(i ky t32)
-                    double nn = na * nb;
-                    me[0] += (-w2 * R + k2 * M) * nn;
-                    me[1] += (-w2 * R + k2 * L2) * nn;
-                    me[2] += (-w2 * R + k2 * M) * nn;
+                    const double nn = na * nb;
+                    me[0].x += (-w2 * R + k2 * M.x) * nn;  me[0].y += k2 * M.y * nn;
+                    me[1].x += (-w2 * R + k2 * L2.x) * nn; me[1].y += k2 * L2.y * nn;
+                    me[2].x += (-w2 * R + k2 * M.x) * nn;  me[2].y += k2 * M.y * nn;
                 }
             }
         }
         if (pr < np) {
-            acc[0].x += me[0]; acc[4].x += me[1]; acc[8].x += me[2];
+            acc[0].x += me[0].x; acc[0].y += me[0].y;
+            acc[4].x += me[1].x; acc[4].y += me[1].y;
+            acc[8].x += me[2].x; acc[8].y += me[2].y;
             const int *L = lm + (size_t)e * 3 * nen;
             for (int ci = 0; ci < 3; ++ci)
                 for (int cj = 0; cj < 3; ++cj) {

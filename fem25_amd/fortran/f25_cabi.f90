@@ -36,6 +36,12 @@
             IMPORT :: C_INT, C_PTR
             TYPE(C_PTR), VALUE :: H, DENS, ECOEFF
          END FUNCTION
+         INTEGER(C_INT) FUNCTION F25_SET_ATTENUATION(H, QP, QS, FREQ0)                          &
+                        BIND(C, NAME='f25_set_attenuation')
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: H, QP, QS
+            REAL(C_DOUBLE), VALUE :: FREQ0
+         END FUNCTION
          INTEGER(C_INT) FUNCTION F25_ASMBLE(H, CFTYPE, FREQ, PY, PREC, OUT)                    &
                         BIND(C, NAME='f25_asmble')
             IMPORT :: C_INT, C_PTR, C_CHAR, C_DOUBLE

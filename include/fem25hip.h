@@ -98,6 +98,15 @@ int f25_mesh_graph(f25_mesh *m, int *ien, int *lm, int *irptr, int *jcptr, char 
  * at once (so drop-in shims may call it before every product). */
 int f25_set_model(f25_mesh *m, const double *dens, const double *ecoeff);
 
+/* Quality factors QP(NNPG), QS(NNPG) at the anchors and the reference
+ * frequency FREQ0 (Hz): the medium becomes dispersive (MESH_INFO's LDISP,
+ * asmble.f90:159-168, 187-201) and the assembled operator uses KMDAMP /
+ * CMDAMP (estiff25.f:511-961) with VCMPLX velocities (haskattn.f:602-641).
+ * qp = NULL returns to the elastic operator.  While set, the entry points
+ * built on frequency-independent element blocks (batched product,
+ * gradient, Gauss-Newton products) return F25_ERR_STATE. */
+int f25_set_attenuation(f25_mesh *m, const double *qp, const double *qs, double freq0);
+
 /* ---- assembled operator (replaces ASMBLE_DRIVER, asmble.f90:1-231) ------ */
 
 /* S(omega, ky) = K - omega^2 M in the CSR order of the pattern, with

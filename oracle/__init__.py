@@ -61,7 +61,8 @@ class _Asm(C.Structure):
                 ("xlocs", dp), ("zlocs", dp), ("dens", dp), ("ecoeff", dp), ("xd", dp),
                 ("zd", dp), ("cdomain", C.c_char_p), ("xwidth", C.c_double),
                 ("zwidth", C.c_double), ("rcoeff", C.c_double), ("dmax", C.c_double),
-                ("kappa", C.c_double), ("fcent", C.c_double)]
+                ("kappa", C.c_double), ("fcent", C.c_double), ("ldisp", C.c_int), ("qp", dp),
+                ("qs", dp), ("freq0", C.c_double)]
 
 
 class _Mesh(C.Structure):
@@ -177,11 +178,16 @@ def _asm_struct(mesh, model, g: Graph, pml):
                 xlocs=mesh.xlocs, zlocs=mesh.zlocs, dens=model.dens,
                 ecoeff=np.asfortranarray(model.ecoeff), xd=mesh.xd, zd=mesh.zd)
     pa = pml.as_array(mesh)
+    disp = getattr(model, "qp", None) is not None
+    if disp:
+        keep["qp"] = np.ascontiguousarray(model.qp, np.float64)
+        keep["qs"] = np.ascontiguousarray(model.qs, np.float64)
     s = _Asm(mesh.nord, mesh.nelem, mesh.nnpg, g.ndof, _p(keep["ieng"], ip), _p(keep["lm"], ip),
              _p(keep["irptr"], ip), _p(keep["jcptr"], ip), _p(keep["xlocs"], dp),
              _p(keep["zlocs"], dp), _p(keep["dens"], dp), _p(keep["ecoeff"], dp),
              _p(keep["xd"], dp), _p(keep["zd"], dp), mesh.cdomain, pa[0], pa[1], pa[2],
-             pa[3], pa[4], pa[5])
+             pa[3], pa[4], pa[5], int(disp), _p(keep["qp"], dp) if disp else None,
+             _p(keep["qs"], dp) if disp else None, float(model.freq0) if disp else 0.0)
     return s, keep
 
 
@@ -211,11 +217,21 @@ def ref_asmble(mesh, model, g, freq, py, pml, cftype=b"B") -> np.ndarray:
     lm = np.ascontiguousarray(g.lm, np.int32)
     ec = np.asfortranarray(model.ecoeff)
     pa = pml.as_array(mesh)
-    ref().ref_asmble(mesh.nord, mesh.nelem, mesh.nnpg, g.ndof, g.nz, _p(ieng, ip),
-                     _p(mesh.xlocs, dp), _p(mesh.zlocs, dp), _p(model.dens, dp), _p(ec, dp),
-                     _p(mesh.xd, dp), _p(mesh.zd, dp), mesh.cdomain, _p(lm, ip),
-                     _p(g.irptr, ip), _p(g.jcptr, ip), _p(pa, dp), C.c_char(cftype),
-                     C.c_double(freq), C.c_double(py), _p(out, fp), C.byref(ierr))
+    if getattr(model, "qp", None) is not None:
+        qp = np.ascontiguousarray(model.qp, np.float64)
+        qs = np.ascontiguousarray(model.qs, np.float64)
+        ref().ref_asmble_q(mesh.nord, mesh.nelem, mesh.nnpg, g.ndof, g.nz, _p(ieng, ip),
+                           _p(mesh.xlocs, dp), _p(mesh.zlocs, dp), _p(model.dens, dp), _p(ec, dp),
+                           _p(mesh.xd, dp), _p(mesh.zd, dp), mesh.cdomain, _p(lm, ip),
+                           _p(g.irptr, ip), _p(g.jcptr, ip), _p(pa, dp), _p(qp, dp), _p(qs, dp),
+                           C.c_double(model.freq0), C.c_char(cftype), C.c_double(freq),
+                           C.c_double(py), _p(out, fp), C.byref(ierr))
+    else:
+        ref().ref_asmble(mesh.nord, mesh.nelem, mesh.nnpg, g.ndof, g.nz, _p(ieng, ip),
+                         _p(mesh.xlocs, dp), _p(mesh.zlocs, dp), _p(model.dens, dp), _p(ec, dp),
+                         _p(mesh.xd, dp), _p(mesh.zd, dp), mesh.cdomain, _p(lm, ip),
+                         _p(g.irptr, ip), _p(g.jcptr, ip), _p(pa, dp), C.c_char(cftype),
+                         C.c_double(freq), C.c_double(py), _p(out, fp), C.byref(ierr))
     if ierr.value:
         raise RuntimeError("reference ASMBLE_DRIVER failed")
     return out

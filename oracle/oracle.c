@@ -500,6 +500,147 @@ void orc_cm25isoq(int nen, int nint, double omega, double py, const double *wts,
 }
 
 /* ------------------------------------------------------------------------ */
+/* haskattn.f:602-641 VCMPLX: complex velocity of a dispersive medium (Aki &
+ * Richards 5.94): V (1 + ln(f/f0)/(pi Q)) - i V/(2Q), the real factor
+ * clamped below at 0.1; real V when FREQ0 = 0, Q = 9999 or Q = 0. */
+double complex orc_vcmplx(double freq, double freq0, double v, double q)
+{
+    const double pi = 3.1415926535897932384626434;
+    if (freq0 == 0.0 || fabs(q - 9999.0) < 1e-5) return v;
+    if (q == 0.0) return v;
+    double fac1 = log(freq / freq0);
+    double fact = 1.0 + 1.0 / (pi * q) * fac1;
+    if (fact < 0.1) fact = 0.1;
+    return v * fact + I * (-v / (2.0 * q));
+}
+
+/* cjac.f:142-250 CJKM_QUAL (and the geometry part of CJCM_QUAL,
+ * cjac.f:585-720): VP, VS, QP, QS interpolated bilinearly from the anchors
+ * (velocities from ECOEFF and DENS at each anchor), DET, SHG; with xd/zd
+ * non-NULL also DAMPX/DAMPZ from ZDAMPFN with VEL = VP. */
+int orc_cjkm_qual(int nen, int nint, const int *ieng_e, const double *xigll,
+                  const double *xlocs, const double *zlocs, const double *dens,
+                  const double *qp, const double *qs, const double *ecoeff, int nnpg,
+                  const double *shl, double omega, double xwidth, double zwidth, double rcoeff,
+                  double dmax_in, double rkappa, double fcent, const double *xd, const double *zd,
+                  double *det, double *vp, double *vs, double *qpp, double *qss, double *shg,
+                  double complex *dampx, double complex *dampz)
+{
+    const double eps = 2.22e-16;
+    for (int ix = 0; ix < nint; ix++) {
+        for (int iz = 0; iz < nint; iz++) {
+            int q = ix + nint * iz;
+            double sf[4], dsf[8];
+            if (orc_csf2d(xigll[ix], xigll[iz], sf, dsf)) return 1;
+            double v_p = 0, v_s = 0, q_p = 0, q_s = 0, xdep = 0, zdep = 0;
+            double dxdxi = 0, dxdeta = 0, dzdxi = 0, dzdeta = 0;
+            for (int ia = 0; ia < 4; ia++) {
+                int il = ieng_e[ia] - 1;
+                double rlam = ecoeff[il], rmu = ecoeff[il + nnpg];
+                double vpp = sqrt((rlam + 2.0 * rmu) / dens[il]);
+                double vss = sqrt(rmu / dens[il]);
+                v_p += sf[ia] * vpp;
+                v_s += sf[ia] * vss;
+                q_p += sf[ia] * qp[il];
+                q_s += sf[ia] * qs[il];
+                if (xd) { xdep += sf[ia] * xd[il]; zdep += sf[ia] * zd[il]; }
+                dxdxi += dsf[2 * ia] * xlocs[il];
+                dxdeta += dsf[2 * ia + 1] * xlocs[il];
+                dzdxi += dsf[2 * ia] * zlocs[il];
+                dzdeta += dsf[2 * ia + 1] * zlocs[il];
+            }
+            double rjac = dxdxi * dzdeta - dxdeta * dzdxi;
+            if (fabs(rjac) <= eps || rjac < 0.0) return 1;
+            det[q] = rjac; vp[q] = v_p; vs[q] = v_s; qpp[q] = q_p; qss[q] = q_s;
+            if (dampx) {
+                dampx[q] = xdep > eps ? orc_zdampfn(xdep, v_p, xwidth, dmax_in, rcoeff, rkappa, fcent, omega) : 1.0;
+                dampz[q] = zdep > eps ? orc_zdampfn(zdep, v_p, zwidth, dmax_in, rcoeff, rkappa, fcent, omega) : 1.0;
+            }
+            for (int a = 0; a < nen; a++) {
+                const double *s = shl + 3 * (a + nen * q);
+                double *g = shg + 3 * (a + nen * q);
+                g[2] = s[2];
+                g[0] = (s[0] * dzdeta - s[1] * dzdxi) / rjac;
+                g[1] = -(s[0] * dxdeta - s[1] * dxdxi) / rjac;
+            }
+        }
+    }
+    return 0;
+}
+
+/* estiff25.f:756-961 CMDAMP; with dampx = dampz = NULL it is KMDAMP
+ * (estiff25.f:511-752), whose real/imaginary split is the same arithmetic
+ * with unit stretching.  Density-normalised: per quadrature point
+ * ZVP2 = VCMPLX(Vp)^2 W, ZVS2 = VCMPLX(Vs)^2 W (W = weights x DET), mass
+ * coefficients -w^2 W + w^2 py^2 ZVS2 (x, z) and -w^2 W + w^2 py^2 ZVP2 (y),
+ * real part into ME and imaginary part onto the diagonal of CE. */
+void orc_kmdamp(int nen, int nint, double freq, double freq0, double py, const double *wts,
+                const double *vp, const double *vs, const double *qpp, const double *qss,
+                const double *det, const double *shg, const double complex *dampx,
+                const double complex *dampz, double *me, double complex *ce)
+{
+    const double twopi = 6.2831853071795862;
+    int nee = NDIM * nen;
+    for (int i = 0; i < nee * nee; i++) { me[i] = 0.0; ce[i] = 0.0; }
+    double omega = twopi * freq, wpy = omega * py, py2 = py * py, w2 = omega * omega;
+    double complex cwpy = I * wpy;
+    for (int ix = 0; ix < nint; ix++) {
+        for (int iz = 0; iz < nint; iz++) {
+            int q = ix + nint * iz;
+            double weight = wts[ix] * wts[iz] * det[q];
+            double complex ex = dampx ? dampx[q] : 1.0, ez = dampz ? dampz[q] : 1.0;
+            double complex zvp = orc_vcmplx(freq, freq0, vp[q], qpp[q]);
+            double complex zvs = orc_vcmplx(freq, freq0, vs[q], qss[q]);
+            double complex zvp2 = zvp * zvp * weight, zvs2 = zvs * zvs * weight;
+            double rm1 = -w2 * weight + w2 * py2 * creal(zvs2), qm1 = w2 * py2 * cimag(zvs2);
+            double rm2 = -w2 * weight + w2 * py2 * creal(zvp2), qm2 = w2 * py2 * cimag(zvp2);
+            const double *sq = shg + 3 * nen * q;
+            for (int ib = 0; ib < nen; ib++) {
+                double phib = sq[3 * ib + 2];
+                double complex phibx = sq[3 * ib] * ex, phibz = sq[3 * ib + 1] * ez;
+                double complex e11t1 = zvp2 * phibx, e11t2 = zvs2 * phibz;
+                double complex e13t1 = zvp2 * phibz, e13t2 = zvs2 * phibx, e13t3 = -2.0 * zvs2 * phibz;
+                double complex e22t1 = zvs2 * phibx, e22t2 = zvs2 * phibz;
+                double complex e31t1 = zvp2 * phibx, e31t2 = zvs2 * phibz, e31t3 = -2.0 * zvs2 * phibx;
+                double complex e33t1 = zvp2 * phibz, e33t2 = zvs2 * phibx;
+                double complex e12t1 = cwpy * zvp2 * phib, e12t2 = -2.0 * cwpy * zvs2 * phib, e12t3 = -cwpy * zvs2 * phibx;
+                double complex e21t1 = cwpy * zvp2 * phibx, e21t2 = -2.0 * cwpy * zvs2 * phibx, e21t3 = -cwpy * zvs2 * phib;
+                double complex e23t1 = cwpy * zvp2 * phibz, e23t2 = -2.0 * cwpy * zvs2 * phibz, e23t3 = -cwpy * zvs2 * phib;
+                double complex e32t1 = cwpy * zvp2 * phib, e32t2 = -2.0 * cwpy * zvs2 * phib, e32t3 = -cwpy * zvs2 * phibz;
+                for (int ia = 0; ia < nen; ia++) {
+                    double phia = sq[3 * ia + 2];
+                    double complex phiax = sq[3 * ia] * ex, phiaz = sq[3 * ia + 1] * ez;
+                    int ip = NDIM * ia, iq = NDIM * ib;
+                    double rme11 = rm1 * phib * phia, qme11 = qm1 * phib * phia;
+                    double rme22 = rm2 * phib * phia, qme22 = qm2 * phib * phia;
+                    me[ip + nee * iq] += rme11;
+                    me[ip + 1 + nee * (iq + 1)] += rme22;
+                    me[ip + 2 + nee * (iq + 2)] += rme11;
+                    double complex ce11 = e11t1 * phiax + e11t2 * phiaz;
+                    double complex ce13 = e13t1 * phiax + e13t2 * phiaz + e13t3 * phiax;
+                    double complex ce22 = e22t1 * phiax + e22t2 * phiaz;
+                    double complex ce31 = e31t1 * phiaz + e31t2 * phiax + e31t3 * phiaz;
+                    double complex ce33 = e33t1 * phiaz + e33t2 * phiax;
+                    double complex ce12 = e12t1 * phiax + e12t2 * phiax + e12t3 * phia;
+                    double complex ce21 = e21t1 * phia + e21t2 * phia + e21t3 * phiax;
+                    double complex ce23 = e23t1 * phia + e23t2 * phia + e23t3 * phiaz;
+                    double complex ce32 = e32t1 * phiaz + e32t2 * phiaz + e32t3 * phia;
+                    ce[ip + nee * iq] += ce11 + I * qme11;
+                    ce[ip + nee * (iq + 1)] -= ce12;
+                    ce[ip + nee * (iq + 2)] += ce13;
+                    ce[ip + 1 + nee * iq] += ce21;
+                    ce[ip + 1 + nee * (iq + 1)] += ce22 + I * qme22;
+                    ce[ip + 1 + nee * (iq + 2)] += ce23;
+                    ce[ip + 2 + nee * iq] += ce31;
+                    ce[ip + 2 + nee * (iq + 1)] -= ce32;
+                    ce[ip + 2 + nee * (iq + 2)] += ce33 + I * qme11;
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* graph25.f:509-698 GENIEN.  Side nodes are deduplicated by a linear scan
  * (graph25.f:566-575), heap-sorted by x, z-sorted within groups of x that
  * agree to TOL (graph25.f:582-617), and numbered 1..NNPS in that order;
@@ -804,7 +945,7 @@ static int ibsect(int n, int ix, const int *v)
 
 typedef struct {
     int nl, nen, nee, nint;
-    double *xipts, *xigll, *wgll, *shl, *shg, *det, *rho, *dmat, *me;
+    double *xipts, *xigll, *wgll, *shl, *shg, *det, *rho, *dmat, *me, *vq;
     double complex *ke, *dampx, *dampz;
 } elem_ws;
 
@@ -821,6 +962,7 @@ static void ws_init(elem_ws *w, int nord)
     w->det = (double *)malloc(sizeof(double) * q);
     w->rho = (double *)malloc(sizeof(double) * q);
     w->dmat = (double *)malloc(sizeof(double) * 2 * q);
+    w->vq = (double *)malloc(sizeof(double) * q);
     w->me = (double *)malloc(sizeof(double) * w->nee * w->nee);
     w->ke = (double complex *)malloc(sizeof(double complex) * w->nee * w->nee);
     w->dampx = (double complex *)malloc(sizeof(double complex) * q);
@@ -831,7 +973,7 @@ static void ws_init(elem_ws *w, int nord)
 static void ws_free(elem_ws *w)
 {
     free(w->xipts); free(w->xigll); free(w->wgll); free(w->shl); free(w->shg);
-    free(w->det); free(w->rho); free(w->dmat); free(w->me); free(w->ke);
+    free(w->det); free(w->rho); free(w->dmat); free(w->vq); free(w->me); free(w->ke);
     free(w->dampx); free(w->dampz);
 }
 
@@ -840,6 +982,18 @@ static int element_matrix(const orc_asm *a, elem_ws *w, int e, double omega, dou
 {
     const int *ie = a->ieng + 4 * e;
     int q = w->nint * w->nint;
+    if (a->ldisp) {   /* asmble.f90:159-168 and 187-201 */
+        const double freq = omega / 6.2831853071795862;
+        const int pml = a->cdomain[e] == 'A';
+        if (orc_cjkm_qual(w->nen, w->nint, ie, w->xigll, a->xlocs, a->zlocs, a->dens, a->qp, a->qs,
+                          a->ecoeff, a->nnpg, w->shl, omega, a->xwidth, a->zwidth, a->rcoeff, a->dmax,
+                          a->kappa, a->fcent, pml ? a->xd : NULL, pml ? a->zd : NULL, w->det,
+                          w->dmat, w->dmat + q, w->rho, w->vq, w->shg, pml ? w->dampx : NULL,
+                          pml ? w->dampz : NULL)) return 1;
+        orc_kmdamp(w->nen, w->nint, freq, a->freq0, py, w->wgll, w->dmat, w->dmat + q, w->rho, w->vq,
+                   w->det, w->shg, pml ? w->dampx : NULL, pml ? w->dampz : NULL, w->me, w->ke);
+        return 0;
+    }
     if (a->cdomain[e] != 'A') {
         if (orc_cjmk_tvi(w->nen, w->nint, ie, w->xigll, a->xlocs, a->zlocs, a->dens, a->ecoeff,
                          a->nnpg, w->shl, w->det, w->rho, w->dmat, w->shg)) return 1;

@@ -91,8 +91,28 @@ typedef struct {
     const char *cdomain;
     double xwidth, zwidth;
     double rcoeff, dmax, kappa, fcent;   /* already selected by CFTYPE */
+    /* dispersive medium (LDISP): QP, QS at anchors and FREQ0 (mesh_utils) */
+    int ldisp;
+    const double *qp, *qs;
+    double freq0;
 } orc_asm;
 
+/* haskattn.f:602-641 VCMPLX */
+double complex orc_vcmplx(double freq, double freq0, double v, double q);
+/* cjac.f:142-250 CJKM_QUAL; with xd, zd (and dampx, dampz) non-NULL the
+ * CJCM_QUAL variant (cjac.f:585-720) for PML elements */
+int orc_cjkm_qual(int nen, int nint, const int *ieng_e, const double *xigll,
+                  const double *xlocs, const double *zlocs, const double *dens,
+                  const double *qp, const double *qs, const double *ecoeff, int nnpg,
+                  const double *shl, double omega, double xwidth, double zwidth, double rcoeff,
+                  double dmax_in, double rkappa, double fcent, const double *xd, const double *zd,
+                  double *det, double *vp, double *vs, double *qpp, double *qss, double *shg,
+                  double complex *dampx, double complex *dampz);
+/* estiff25.f:511-752 KMDAMP (dampx = dampz = NULL) / 756-961 CMDAMP */
+void orc_kmdamp(int nen, int nint, double freq, double freq0, double py, const double *wts,
+                const double *vp, const double *vs, const double *qpp, const double *qss,
+                const double *det, const double *shg, const double complex *dampx,
+                const double complex *dampz, double *me, double complex *ce);
 /* asmble.f90:1-231 + asmble.f90:235-314: COMPLEX*8 CDIST in CSR order */
 int orc_asmble_cdist(const orc_asm *a, double freq, double py, float complex *cdist);
 /* same element loop, FP64 accumulation (disp25.f90:912-989 ASM_DISP style, ME+KE) */

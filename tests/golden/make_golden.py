@@ -12,6 +12,8 @@ seeds) and the reference's outputs:
   asm_<case>.npz   ASMBLE_DRIVER COMPLEX*8 CDIST at one (freq, py)
   inv_<case>.npz   GENGMASK/ELEM_WTS/GRADPTRS structures and CFMAT_DIST
                    columns for a seeded COMPLEX*8 wavefield, PP and SS
+  asmq_<case>.npz  ASMBLE_DRIVER CDIST on a dispersive medium (LDISP, seeded
+                   QP/QS, FREQ0 = 1 Hz) at two (freq, py)
   c1_graph_sha.npz SHA-256 of the config-1 (40 x 20, p = 4) numbering
 """
 from __future__ import annotations
@@ -80,8 +82,24 @@ def main() -> None:
                    icsc=inv.icsc, elem_wts=inv.elem_wts, freq=0.06, py=0.8e-4, wave_seed=31)
         np.savez_compressed(os.path.join(HERE, f"inv_{case}.npz"), **out)
         print(case, g.ndof, g.nz, "written")
+    for case in ("tiny_layered", "pml"):
+        m, mod, pml = mesh_case(case)
+        g = oracle.ref_graph(m)
+        md = q_model(mod, m.nnpg)
+        out = dict(qp=md.qp, qs=md.qs, fref=md.freq0)
+        for k, (freq, py) in enumerate(((0.05, 0.7e-4), (0.2, 0.0))):
+            out[f"cdist{k}"] = oracle.ref_asmble(m, md, g, freq, py, pml)
+            out[f"freq{k}"], out[f"py{k}"] = freq, py
+        np.savez_compressed(os.path.join(HERE, f"asmq_{case}.npz"), **out)
+        print(case, "dispersive written")
     m, mod, pml = mesh_case("c1")
     np.savez(os.path.join(HERE, "c1_graph_sha.npz"), sha=graph_sha(oracle.ref_graph(m)))
+
+
+def q_model(mod, nnpg):
+    """Seeded quality factors (QP 60-100, QS 30-50) with FREQ0 = 1 Hz."""
+    rng = np.random.default_rng(5)
+    return cf.with_attenuation(mod, 60 + 40 * rng.random(nnpg), 30 + 20 * rng.random(nnpg), 1.0)
 
 
 if __name__ == "__main__":

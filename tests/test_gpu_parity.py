@@ -290,3 +290,34 @@ def test_gnmfree_batched_uneven():
     z = mi.gnmfree(nobs, alpha, jac, p)
     zref = oracle.gnmfree(nobs, alpha, inv.nnpgl, inv.mygrad, inv.na35, jac, p)
     assert relerr(z, zref) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["tiny_layered", "pml"])
+def test_asmble_dispersive(name):
+    """Dispersive medium (KMDAMP/CMDAMP, VCMPLX): FP64 operator against the
+    oracle at 1e-12, COMPLEX*8 CDIST against the compiled reference's
+    fixture at FP32 level; the block-based product refuses the medium."""
+    import os
+    from fem25_amd import _lib
+    from fem25_amd.api import MeshInfo
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", f"asmq_{name}.npz"))
+    md = cf.with_attenuation(mod, f["qp"], f["qs"], float(f["fref"]))
+    mi = MeshInfo(m, pml)
+    mi.set_model(md)
+    for k in range(2):
+        freq, py = float(f[f"freq{k}"]), float(f[f"py{k}"])
+        s = mi.asmble(b"B", freq, py, complex128=True)
+        assert relerr(s, oracle.asmble_z(m, md, g, freq, py, pml)) < FP64_TOL
+        c = mi.asmble(b"B", freq, py)
+        ref = f[f"cdist{k}"]
+        rowmax = np.maximum.reduceat(np.abs(ref.astype(np.complex128)), g.irptr[:-1] - 1)
+        rows = np.repeat(np.arange(g.ndof), np.diff(g.irptr))
+        assert np.all(np.abs(c.astype(np.complex128) - ref) <= 1e-6 * rowmax[rows])
+    om, ky, _, _ = pairs(1, 1)
+    with pytest.raises(_lib.Fem25Error):
+        mi.apply_batch(om, ky, cf.seeded_complex(g.ndof, batch=1, seed=1))
+    mi.set_model(mod)        # back to the elastic operator
+    s = mi.asmble(b"B", 0.05, 0.7e-4, complex128=True)
+    assert relerr(s, oracle.asmble_z(m, mod, g, 0.05, 0.7e-4, pml)) < FP64_TOL

@@ -335,3 +335,31 @@ def test_pchadj_vs_dense():
     hdd = np.zeros(inv.na35)
     hdd[inv.mygrad - 1] = np.sum(np.abs(Jd) ** 2, axis=0)
     assert np.linalg.norm(hd - hdd) <= 1e-6 * np.linalg.norm(hdd)
+
+
+@pytest.mark.parametrize("case", ["tiny_layered", "pml"])
+def test_dispersive_cdist_vs_reference(case):
+    """KMDAMP/CMDAMP + VCMPLX through ASMBLE_DRIVER with LDISP: the oracle's
+    COMPLEX*8 CDIST equals the compiled reference's (fixture) bit for bit."""
+    m, mod, pml = mesh_case(case)
+    g = oracle_graph(case)
+    f = np.load(os.path.join(GOLD, f"asmq_{case}.npz"))
+    md = cf.with_attenuation(mod, f["qp"], f["qs"], float(f["fref"]))
+    for k in range(2):
+        cd = oracle.asmble_cdist(m, md, g, float(f[f"freq{k}"]), float(f[f"py{k}"]), pml)
+        np.testing.assert_array_equal(cd, f[f"cdist{k}"])
+
+
+@pytest.mark.parametrize("case", ["tiny", "pml"])
+def test_dispersive_without_attenuation_is_density_normalised(case):
+    """With Q = 9999 VCMPLX returns the real velocity, and on a homogeneous
+    model (velocities interpolated in V equal those interpolated in lambda,
+    mu) the dispersive operator is the elastic one divided by rho."""
+    m, _, pml = mesh_case(case)
+    mod = cf.homogeneous_model(m)
+    g = oracle_graph(case)
+    md = cf.with_attenuation(mod, 9999.0, 9999.0, 1.0)
+    for freq, py in ((0.05, 0.7e-4), (0.13, 0.0)):
+        s_el = oracle.asmble_z(m, mod, g, freq, py, pml)
+        s_q = oracle.asmble_z(m, md, g, freq, py, pml)
+        assert relerr(s_q * mod.dens[0], s_el) < 1e-13
