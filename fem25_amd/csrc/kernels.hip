@@ -203,16 +203,15 @@ __global__ __launch_bounds__(NT) void k_blocks(int nelem, int nq1, const double 
 // ---------------------------------------------------------------------------
 // Batched element-block product on MFMA.
 //
-// One workgroup = one element of the current colour, 4 waves.  The element's
-// blocks sit in LDS (row stride RS, odd, so both the direct and the
-// transposed operand reads are conflict-free).  Each wave owns N-tiles of 8
-// (omega, ky) pairs = 16 real columns (re, im interleaved as in memory).  The
-// element's input vectors stay in VGPRs in the B-operand layout of a
-// 16x16x4 MFMA (lane l: column l%16, k-row l/16 + 4s); per-column factors
-// (ky, ky^2, -omega^2, the i of the coupling blocks) are applied to the
-// B operand, so each output component is ONE accumulation over five
-// sub-blocks.  Outputs are added into y[dof][pair] (no other element of the
-// colour touches these DOFs).
+// One workgroup = one element of the current colour.  The element's blocks
+// sit in LDS (row stride RS, odd, so both the direct and the transposed
+// operand reads are conflict-free).  Each wave owns N-tiles of 8 (omega, ky)
+// pairs = 16 real columns (re, im interleaved as in memory).  The input
+// vector values reach the B operand of a 16x16x4 MFMA directly (lane l:
+// column l%16, k-row l/16 + 4s); per-column factors (ky, ky^2, -omega^2, the
+// i of the coupling blocks) are applied to the B operand, so each output
+// component is ONE accumulation over five sub-blocks.  Outputs are added
+// into y[dof][pair] (no other element of the colour touches these DOFs).
 template <typename T>
 struct MfmaTraits;
 // Operand layout of both 16x16x4 forms: lane l supplies A[l%16][l/16] and
@@ -293,9 +292,12 @@ __device__ inline float partner(float v) {
 // Loop order per tile: k-steps s (4 element nodes each) outermost, with the
 // x operands of step s+1 loaded while step s runs; inside a step every
 // (component, sub-block, row tile) accumulator gets its MFMA, so the five
-// operand kinds of one x value are formed once and reused.
+// operand kinds of one x value are formed once and reused.  At least three
+// waves per SIMD (two for the refined product) are requested: the register
+// budget that leaves (168) keeps the y preload and costs a few spilled
+// values, which measured faster than either two waves or no preload.
 template <int NL, typename BT, typename XT, bool REFINE, int CSEL>
-__global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW)) void k_apply(const int *__restrict__ order,
+__global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW), (REFINE ? 2 : 3)) void k_apply(const int *__restrict__ order,
                                                const BT *__restrict__ blk, const BT *__restrict__ blk_lo,
                                                const int *__restrict__ lm,
                                                const unsigned long long *__restrict__ first, int npair,
