@@ -116,60 +116,7 @@ def main() -> int:
 
     extra = {}
     if not args.skip_extras:
-        # accuracy of the MFMA product against the assembled CSR operator (FP64)
-        b = npair - 1
-        s = torch.empty(mi.nz, dtype=torch.complex128, device="cuda")
-        mi.asmble_dev(b"B", float(freq[b]), float(py[b]), s)
-        xb = x[:, b].contiguous()
-        yb = torch.empty_like(xb)
-        mi.csr_apply_dev(s, xb, yb)
-        err64 = float(torch.linalg.norm(y[:, b] - yb) / torch.linalg.norm(yb))
-        # plain FP64 GPU path: assemble S(omega, ky) in CSR, then SpMV, per pair
-        nplain = 8
-        torch.cuda.synchronize()
-        tp = time.perf_counter()
-        for i in range(nplain):
-            mi.asmble_dev(b"B", float(freq[i]), float(py[i]), s)
-            mi.csr_apply_dev(s, x[:, i].contiguous(), yb)
-        torch.cuda.synchronize()
-        plain_gdofs = ndof * nplain / (time.perf_counter() - tp) / 1e9
-        # FP32 and FP32 + correction products
-        x32 = x.to(torch.complex64)
-        y32 = torch.empty_like(x32)
-        yr = torch.empty_like(x)
-        res = {}
-        for name, mode, xx, yy in (("fp32", _lib.F25_APPLY_F32, x32, y32),
-                                   ("fp32_refined", _lib.F25_APPLY_F32_REFINED, x, yr)):
-            mi.apply_batch_dev(omega, ky, xx, yy, mode)
-            torch.cuda.synchronize()
-            tt = time.perf_counter()
-            for _ in range(3):
-                mi.apply_batch_dev(omega, ky, xx, yy, mode)
-            torch.cuda.synchronize()
-            ms = 1e3 * (time.perf_counter() - tt) / 3
-            d = (yy.to(torch.complex128) - y)
-            err = float((torch.linalg.norm(d, dim=0) / torch.linalg.norm(y, dim=0)).max())
-            res[name] = {"gdof_per_s": round(ndof * npair / ms * 1e-6, 3),
-                         "max_rel_err_vs_fp64": err}
-        del x32, y32, yr
-        # adjoint gradient over 128 (omega, ky) wavefield pairs
-        u = torch.randn(npair, ndof, dtype=torch.complex128, device="cuda", generator=g)
-        lam = torch.randn(npair, ndof, dtype=torch.complex128, device="cuda", generator=g)
-        gg = torch.zeros(mesh.nnpg, 3, dtype=torch.float64, device="cuda")
-        mi.gradient_dev(omega, ky, u, lam, gg)
-        torch.cuda.synchronize()
-        tg = time.perf_counter()
-        ng = 2
-        for _ in range(ng):
-            mi.gradient_dev(omega, ky, u, lam, gg)
-        torch.cuda.synchronize()
-        grad_s = (time.perf_counter() - tg) / ng
-        extra = {"fp64_rel_err_vs_assembled_csr": err64,
-                 "plain_fp64_csr_gdof_per_s": round(plain_gdofs, 3),
-                 "speedup_vs_plain_fp64": round(gdofs / plain_gdofs, 2),
-                 **res,
-                 "gradients_per_s": round(npair / grad_s, 2),
-                 "gradient_kernel_ms": L.f25_last_kernel_ms(b"grad")}
+        extra = extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g)
         log(json.dumps(extra))
 
     cpu = None
@@ -211,6 +158,138 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     mi.close()
     return 0
+
+
+def roof(bound, achieved, unit):
+    peak = {"hbm": PEAK_HBM, "fp64": PEAK_FP64, "fp32_mfma": PEAK_FP32_MFMA}[bound]
+    scale = 1e9 if unit == "GB/s" else 1e12
+    return {"bound": bound, "achieved": round(achieved / scale, 3), "peak": peak / scale,
+            "unit": unit, "frac": round(achieved / peak, 4)}
+
+
+def timed(fn, L, family, reps):
+    """Runs fn reps times after one warm-up; mean kernel ms of the family
+    (HIP events inside the library) and mean wall ms per call."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ks = []
+    t = time.perf_counter()
+    for _ in range(reps):
+        fn()
+        ks.append(L.f25_last_kernel_ms(family))
+    torch.cuda.synchronize()
+    return float(np.mean(ks)), 1e3 * (time.perf_counter() - t) / reps
+
+
+def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g):
+    """Accuracy of the product, the other precision modes, and every other
+    kernel of the path with its own roofline."""
+    import ctypes as C
+    from fem25_amd import _lib
+    ndof, nen, nq = mi.ndof, mi.nen, mi.nq1 * mi.nq1
+    out = {}
+    # accuracy of the MFMA product against the assembled CSR operator (FP64)
+    b = npair - 1
+    s = torch.empty(mi.nz, dtype=torch.complex128, device="cuda")
+    mi.asmble_dev(b"B", float(freq[b]), float(py[b]), s)
+    xb = x[:, b].contiguous()
+    yb = torch.empty_like(xb)
+    mi.csr_apply_dev(s, xb, yb)
+    out["fp64_rel_err_vs_assembled_csr"] = float(torch.linalg.norm(y[:, b] - yb) / torch.linalg.norm(yb))
+    # plain FP64 GPU path: assemble S(omega, ky) in CSR, then SpMV, per pair
+    nplain = 8
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for i in range(nplain):
+        mi.asmble_dev(b"B", float(freq[i]), float(py[i]), s)
+        mi.csr_apply_dev(s, x[:, i].contiguous(), yb)
+    torch.cuda.synchronize()
+    plain = ndof * nplain / (time.perf_counter() - tp) / 1e9
+    out["plain_fp64_csr_gdof_per_s"] = round(plain, 3)
+    out["speedup_vs_plain_fp64"] = round(gdofs / plain, 2)
+    # FP32 and FP32 refined products
+    x32 = x.to(torch.complex64)
+    y32 = torch.empty_like(x32)
+    yr = torch.empty_like(x)
+    for name, mode, xx, yy in (("fp32", _lib.F25_APPLY_F32, x32, y32),
+                               ("fp32_refined", _lib.F25_APPLY_F32_REFINED, x, yr)):
+        k_ms, wall = timed(lambda: mi.apply_batch_dev(omega, ky, xx, yy, mode), L, b"apply", 3)
+        d = yy.to(torch.complex128) - y
+        err = float((torch.linalg.norm(d, dim=0) / torch.linalg.norm(y, dim=0)).max())
+        out[name] = {"gdof_per_s": round(ndof * npair / wall * 1e-6, 3), "max_rel_err_vs_fp64": err,
+                     "kernel_ms": round(k_ms, 3),
+                     "roofline": roof("fp32_mfma", 60.0 * nen * nen * mesh.nelem * npair * (4 if "ref" in name else 1)
+                                      / (k_ms * 1e-3), "TFLOP/s")}
+    del x32, y32, yr
+    kern = {}
+    # assembled FP64 operator: one pass over the CSR basis (HBM)
+    k_ms, wall = timed(lambda: mi.asmble_dev(b"B", float(freq[0]), float(py[0]), s), L, b"asmble", 5)
+    nzb = mi.nz * (3 * 8 + 16 + 4 + 1) + (ndof + 1) * 8
+    kern["asmble_fp64"] = {"kernel_ms": round(k_ms, 3), "calls_per_s": round(1e3 / wall, 1),
+                           "roofline": roof("hbm", nzb / (k_ms * 1e-3), "GB/s")}
+    del s
+    # adjoint gradient over npair (omega, ky) wavefield pairs (FP64 VALU)
+    u = torch.randn(npair, ndof, dtype=torch.complex128, device="cuda")
+    lam = torch.randn(npair, ndof, dtype=torch.complex128, device="cuda")
+    gg = torch.zeros(mesh.nnpg, 3, dtype=torch.float64, device="cuda")
+    k_ms, wall = timed(lambda: mi.gradient_dev(omega, ky, u, lam, gg), L, b"grad", 2)
+    gfl = (72.0 * nen + 150.0) * nq * mesh.nelem * npair
+    kern["gradient"] = {"kernel_ms": round(k_ms, 3), "gradients_per_s": round(npair / (wall * 1e-3), 1),
+                        "roofline": roof("fp64", gfl / (k_ms * 1e-3), "TFLOP/s")}
+    out["gradients_per_s"] = kern["gradient"]["gradients_per_s"]
+    # pseudo-Hessian diagonal (host arrays, 8 wavefields)
+    nt = 8
+    uh = u[:nt].cpu().numpy()
+    t0 = time.perf_counter()
+    mi.gn_diag(omega[:nt], ky[:nt], uh)
+    wall = time.perf_counter() - t0
+    k_ms = L.f25_last_kernel_ms(b"gn_diag")
+    dfl = (4.0 * 3 * 2 * 2 * 3 * nen + 30.0 * 3 * nen + 36.0 * 2 * nen) * nq * mesh.nelem * nt
+    kern["gn_diag"] = {"kernel_ms": round(k_ms, 3), "wavefields_per_s": round(nt / (k_ms * 1e-3), 1),
+                       "roofline": roof("fp64", dfl / (k_ms * 1e-3), "TFLOP/s")}
+    del u, lam, gg, uh
+    # inversion structures of the mesh (all anchors of interior elements)
+    inv = mi.inv_setup(b"PP")
+    wave = (np.random.default_rng(7).standard_normal(ndof) + 0j).astype(np.complex64)
+    t0 = time.perf_counter()
+    fm, _ = cfmat_host(mi, inv, float(freq[0]), float(py[0]), wave)
+    k_ms = L.f25_last_kernel_ms(b"cfmat")
+    kern["cfmat_dist"] = {"kernel_ms": round(k_ms, 3), "columns_per_s": round(inv.nnpgl / (k_ms * 1e-3), 1)}
+    # PCHADJ's J = G^T F: 60 receivers x 3 components
+    nrhs = 180
+    gt = torch.randn(ndof, nrhs, dtype=torch.complex64, device="cuda")
+    fd = torch.from_numpy(fm).cuda()
+    jac = torch.empty(inv.nnpgl, nrhs, dtype=torch.complex64, device="cuda")
+    hd = torch.zeros(inv.na35, dtype=torch.float32, device="cuda")
+    args = (mi.h, nrhs, C.c_void_p(gt.data_ptr()), C.c_void_p(fd.data_ptr()), C.c_void_p(jac.data_ptr()),
+            C.c_void_p(hd.data_ptr()))
+    k_ms, wall = timed(lambda: _lib.check(L.f25_pchadj_jac_dev(*args), "pchadj"), L, b"pchadj", 3)
+    pb = ndof * nrhs * 8 + inv.nz_fdist * 12 + inv.nnpgl * nrhs * 8
+    kern["pchadj_jac"] = {"kernel_ms": round(k_ms, 3), "nrhs": nrhs,
+                          "roofline": roof("hbm", pb / (k_ms * 1e-3), "GB/s")}
+    del gt, fd, jac, hd
+    # GNMFREE over 10 frequencies x 12 sources with 20..120 receivers each
+    rng = np.random.default_rng(8)
+    nobs = (3 * rng.integers(20, 121, size=120)).astype(np.int32)
+    alpha = np.ones(120)
+    jt = torch.randn(int(nobs.sum()) * inv.nnpgl, dtype=torch.complex64, device="cuda")
+    pv = torch.randn(inv.na35, dtype=torch.float64, device="cuda")
+    zv = torch.zeros(inv.na35, dtype=torch.float64, device="cuda")
+    args = (mi.h, 120, nobs.ctypes.data_as(_lib.ip), alpha.ctypes.data_as(_lib.dp), C.c_void_p(jt.data_ptr()),
+            C.c_void_p(pv.data_ptr()), C.c_void_p(zv.data_ptr()))
+    k_ms, wall = timed(lambda: _lib.check(L.f25_gnmfree_dev(*args), "gnmfree"), L, b"gnmfree", 3)
+    jb = 2 * jt.numel() * 8
+    kern["gnmfree"] = {"kernel_ms": round(k_ms, 3), "jacobians": 120, "jacobian_bytes": jt.numel() * 8,
+                       "roofline": roof("hbm", jb / (k_ms * 1e-3), "GB/s")}
+    del jt, pv, zv
+    out["kernels"] = kern
+    return out
+
+
+def cfmat_host(mi, inv, freq, py, wave):
+    from fem25_amd import api
+    return api.cfmat_dist(inv.nz_fdist, mi.ndof, freq, py, wave, mi, inv)
 
 
 def profiled_traffic(config: str):

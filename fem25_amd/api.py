@@ -76,20 +76,7 @@ class MeshInfo:
                  graph: Graph | None = None):
         L = _lib.load()
         self.mesh = mesh
-        pml = pml or PML()
-        pml_srf = pml_srf or pml
-        self._keep = [np.ascontiguousarray(mesh.ieng, np.int32), np.ascontiguousarray(mesh.xlocs),
-                      np.ascontiguousarray(mesh.zlocs), np.ascontiguousarray(mesh.xd),
-                      np.ascontiguousarray(mesh.zd)]
-        d = _lib.MeshDesc()
-        d.nord, d.nelem, d.nnpg = mesh.nord, mesh.nelem, mesh.nnpg
-        d.ieng = _p(self._keep[0], ip)
-        d.xlocs, d.zlocs = _p(self._keep[1], dp), _p(self._keep[2], dp)
-        d.xd, d.zd = _p(self._keep[3], dp), _p(self._keep[4], dp)
-        d.cdomain, d.cnnpg = mesh.cdomain, mesh.cnnpg
-        d.xwidth, d.zwidth = mesh.xwidth, mesh.zwidth
-        d.pml[:] = [pml.rcoeff, pml.dmax, pml.kappa, pml.fcent]
-        d.pml_srf[:] = [pml_srf.rcoeff, pml_srf.dmax, pml_srf.kappa, pml_srf.fcent]
+        d, self._keep = _mesh_desc(mesh, pml, pml_srf)
         self._desc = d
         h = C.c_void_p()
         if graph is None:
@@ -296,13 +283,45 @@ class MeshInfo:
 # ---------------------------------------------------------------------------
 # Reference-named entry points.
 
+def _mesh_desc(mesh: Mesh, pml: PML | None = None, pml_srf: PML | None = None):
+    """f25_mesh_desc of a mesh and the arrays it points into."""
+    pml = pml or PML()
+    pml_srf = pml_srf or pml
+    keep = [np.ascontiguousarray(mesh.ieng, np.int32), np.ascontiguousarray(mesh.xlocs),
+            np.ascontiguousarray(mesh.zlocs), np.ascontiguousarray(mesh.xd),
+            np.ascontiguousarray(mesh.zd)]
+    d = _lib.MeshDesc()
+    d.nord, d.nelem, d.nnpg = mesh.nord, mesh.nelem, mesh.nnpg
+    d.ieng = _p(keep[0], ip)
+    d.xlocs, d.zlocs = _p(keep[1], dp), _p(keep[2], dp)
+    d.xd, d.zd = _p(keep[3], dp), _p(keep[4], dp)
+    d.cdomain, d.cnnpg = mesh.cdomain, mesh.cnnpg
+    d.xwidth, d.zwidth = mesh.xwidth, mesh.zwidth
+    d.pml[:] = [pml.rcoeff, pml.dmax, pml.kappa, pml.fcent]
+    d.pml_srf[:] = [pml_srf.rcoeff, pml_srf.dmax, pml_srf.kappa, pml_srf.fcent]
+    return d, keep
+
+
 def gen_graph25(mesh: Mesh, pml: PML | None = None) -> Graph:
-    """GLL numbering and CSR pattern with GEN_GRAPH25's semantics."""
-    mi = MeshInfo(mesh, pml)
+    """GLL numbering and CSR pattern with GEN_GRAPH25's semantics, on the
+    host (f25_graph_build: no device needed)."""
+    L = _lib.load()
+    d, keep = _mesh_desc(mesh, pml)
+    h = C.c_void_p()
+    s = np.zeros(3, np.int64)
+    check(L.f25_graph_build(C.byref(d), 1, C.byref(h), s.ctypes.data_as(_lib.i64p)), "f25_graph_build")
     try:
-        return mi.graph()
+        nnp, ndof, nz = (int(v) for v in s)
+        ien = np.zeros((mesh.nelem, mesh.nen), np.int32)
+        lm = np.zeros((mesh.nelem, mesh.nen, 3), np.int32)
+        irptr = np.zeros(ndof + 1, np.int32)
+        jcptr = np.zeros(nz, np.int32)
+        cnp = C.create_string_buffer(ndof + 1)
+        check(L.f25_graph_get(h, _p(ien, ip), _p(lm, ip), _p(irptr, ip), _p(jcptr, ip), cnp),
+              "f25_graph_get")
     finally:
-        mi.close()
+        L.f25_graph_destroy(h)
+    return Graph(nnp, ndof, nz, ien, lm, irptr, jcptr, cnp.raw[:ndof])
 
 
 def asmble_driver(nzloc: int, cftype: bytes, freq: float, py: float, msh: MeshInfo):

@@ -1004,14 +1004,18 @@ extern "C" int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *w
     TRY(dalloc(&df, m->inv.nz_fdist));
     const int nen = m->tab.nen, nq = m->tab.nq;
     size_t lds = 3 * nen * sizeof(double2) + 17 * (size_t)nq * sizeof(double);
+    (void)hipEventRecord(m->ev0, m->st);
     hipLaunchKernelGGL(k_cfmat, dim3(m->inv.nnpgl), dim3(256), lds, m->st, nen, m->tab.nq1, m->inv.nnpgl,
                        (const int *)nullptr, m->d_col_elem, m->d_col_corner, m->d_col_cp, m->d_col_cs,
                        m->d_shl, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, m->d_ew,
                        m->d_jcsc, m->d_icsc, kv, dw, df, m->d_err);
+    (void)hipEventRecord(m->ev1, m->st);
     hipError_t e = hipGetLastError();
     int r = 0;
     if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
     if (!r) r = check_err(m);
+    float cms = 0.f;
+    if (!r && hipEventElapsedTime(&cms, m->ev0, m->ev1) == hipSuccess) record_ms("cfmat", cms);
     if (!r) {
         e = hipMemcpyAsync(fmat, df, m->inv.nz_fdist * sizeof(float2), hipMemcpyDeviceToHost, m->st);
         if (e == hipSuccess) e = hipStreamSynchronize(m->st);
@@ -1369,6 +1373,7 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
         HIPCHK(hipFuncSetAttribute((const void *)k_fcols, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
     if (!r && lds2 > 65536)
         HIPCHK(hipFuncSetAttribute((const void *)k_fdiag, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+    if (!r) (void)hipEventRecord(m->ev0, m->st);
     for (int t = 0; t < ntrip && !r; ++t) {
         hipLaunchKernelGGL(k_fcols, dim3(m->d.nelem), dim3(nt1), lds1, m->st, nen, m->tab.nq1, m->d_shl,
                            m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho,
@@ -1378,7 +1383,10 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
     }
+    if (!r) (void)hipEventRecord(m->ev1, m->st);
     if (!r) r = check_err(m);
+    float dms = 0.f;
+    if (!r && hipEventElapsedTime(&dms, m->ev0, m->ev1) == hipSuccess) record_ms("gn_diag", dms);
     if (!r) {
         hipError_t e = hipMemcpyAsync(diag, dd, (size_t)nnpg * 3 * sizeof(double), hipMemcpyDeviceToHost, m->st);
         if (e == hipSuccess) e = hipStreamSynchronize(m->st);

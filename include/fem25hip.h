@@ -245,9 +245,10 @@ const char *f25_last_error(void);
  * the operand/result lane layout the batched product relies on; *maxerr is
  * the largest deviation (0 when the layout holds). */
 int f25_selftest_mfma(double *maxerr);
-/* Average duration (ms) of the most recent launch of the named kernel
- * family ("apply", "grad", "asmble"), measured with HIP events on the
- * stream the kernel ran on; -1 if unknown. */
+/* Duration (ms) of the kernels of the most recent call of the named family
+ * ("apply", "grad", "asmble", "cfmat", "gn_diag", "pchadj", "gnmfree"),
+ * measured with HIP events on the stream the kernels ran on; -1 if
+ * unknown. */
 double f25_last_kernel_ms(const char *family);
 
 #ifdef __cplusplus
