@@ -90,6 +90,7 @@ struct f25_mesh_s {
     unsigned long long *d_first = nullptr;   // first-touch node masks per element
     double *d_xl = nullptr, *d_zl = nullptr, *d_xd = nullptr, *d_zd = nullptr;
     double *d_shl = nullptr, *d_wq = nullptr, *d_xq = nullptr;
+    double *d_shlT = nullptr;   // shape tables as [a][c][q] (q fastest)
     char *d_cdom = nullptr, *d_cnp = nullptr;
     double *d_rho = nullptr, *d_lam = nullptr, *d_mu = nullptr;
     double *d_blk = nullptr;
@@ -154,6 +155,14 @@ static int upload_mesh(f25_mesh *m) {
     TRY(upload(&m->d_zd, m->zd.data(), m->zd.size(), s));
     TRY(upload(&m->d_cdom, m->cdom.data(), m->cdom.size(), s));
     TRY(upload(&m->d_shl, m->tab.shl.data(), m->tab.shl.size(), s));
+    {
+        const int nq = m->tab.nq;
+        std::vector<double> t(m->tab.shl.size());
+        for (int q = 0; q < nq; ++q)
+            for (int a = 0; a < nen; ++a)
+                for (int cc = 0; cc < 3; ++cc) t[(size_t)(a * 3 + cc) * nq + q] = m->tab.shl[((size_t)q * nen + a) * 3 + cc];
+        TRY(upload(&m->d_shlT, t.data(), t.size(), s));
+    }
     TRY(upload(&m->d_wq, m->tab.wq.data(), m->tab.wq.size(), s));
     TRY(upload(&m->d_xq, m->tab.xq.data(), m->tab.xq.size(), s));
     std::vector<int> lm0(m->g.lm.size());
@@ -355,7 +364,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     (void)hipSetDevice(m->dev);
     if (m->st) (void)hipStreamSynchronize(m->st);
     dfree(m->d_ieng); dfree(m->d_lm); dfree(m->d_first); dfree(m->d_order); dfree(m->d_pml_order); dfree(m->d_err);
-    dfree(m->d_xl); dfree(m->d_zl); dfree(m->d_xd); dfree(m->d_zd); dfree(m->d_shl); dfree(m->d_wq);
+    dfree(m->d_xl); dfree(m->d_zl); dfree(m->d_xd); dfree(m->d_zd); dfree(m->d_shl); dfree(m->d_shlT); dfree(m->d_wq);
     dfree(m->d_xq); dfree(m->d_cdom); dfree(m->d_cnp); dfree(m->d_rho); dfree(m->d_lam); dfree(m->d_mu);
     dfree(m->d_blk); dfree(m->d_blk32); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_colmax); dfree(m->d_dblk); dfree(m->d_irptr);
     dfree(m->d_jcptr); dfree(m->d_comp); dfree(m->d_v0); dfree(m->d_v1); dfree(m->d_v2);
@@ -806,7 +815,7 @@ static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d
     size_t lds = (size_t)2 * H * tg * 3 * nen * sizeof(double2) + (size_t)ntp * 12 * sizeof(double);
     HIPCHK(hipEventRecord(m->ev0, m->st));
     hipLaunchKernelGGL(k_grad<H>, dim3(m->d.nelem), dim3(ntp), lds, m->st, nen, m->tab.nq1, m->d.nelem, tg,
-                       m->d_shl, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip,
+                       m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip,
                        (int64_t)m->g.ndof, d_om, d_ky, u, l, m->d_part, m->d_err);
     HIPCHK(hipEventRecord(m->ev1, m->st));
     hipLaunchKernelGGL(k_grad_gather, dim3((m->d.nnpg + 255) / 256), dim3(256), 0, m->st, m->d.nnpg,
@@ -1376,7 +1385,7 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
     if (!r) (void)hipEventRecord(m->ev0, m->st);
     for (int t = 0; t < ntrip && !r; ++t) {
         hipLaunchKernelGGL(k_fcols, dim3(m->d.nelem), dim3(nt1), lds1, m->st, nen, m->tab.nq1, m->d_shl,
-                           m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho,
+                           m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho,
                            m->d_lam, m->d_mu, omega[t], ky[t], du + ndof * t, dcols, m->d_err);
         hipLaunchKernelGGL(k_fdiag, dim3(nnpg), dim3(nt2), lds2, m->st, nnpg, m->ncon_anchor, nee,
                            m->d_mconn, m->d_ieng, m->d_pmap, dcols, dd);

@@ -809,7 +809,7 @@ __global__ void k_peff(int ndofl, const int *__restrict__ mydofs, const int64_t 
 // the shape-function values and physical derivatives at (q, a) are formed
 // once per H pairs.  The geometry at q is computed once per element.
 template <int H>
-__global__ __launch_bounds__(512) void k_grad(int nen, int nq1, int nelem, int tg, const double *__restrict__ shl,
+__global__ __launch_bounds__(512) void k_grad(int nen, int nq1, int nelem, int tg, const double *__restrict__ shlT,
                        const double *__restrict__ wq, const double *__restrict__ xq,
                        const int *__restrict__ ieng, const double *__restrict__ xl,
                        const double *__restrict__ zl, const char *__restrict__ cdom,
@@ -852,8 +852,10 @@ __global__ __launch_bounds__(512) void k_grad(int nen, int nq1, int nelem, int t
         double2 U[H][3] = {}, UX[H][3] = {}, UZ[H][3] = {}, V[H][3] = {}, VX[H][3] = {}, VZ[H][3] = {};
         const double2 *u0 = su + (size_t)ta * nee, *l0 = sl + (size_t)ta * nee;
         for (int a = 0; a < nen; ++a) {
-            const double *s = shl + ((size_t)q * nen + a) * 3;
-            const double n = s[2], nx = s[0] * rx + s[1] * sx, nz = s[0] * rz + s[1] * sz;
+            // shlT[(a*3 + c)*nq + q]: adjacent lanes (adjacent q) read adjacent words
+            const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
+            const double n = shlT[(size_t)(a * 3 + 2) * nq + q];
+            const double nx = s0 * rx + s1 * sx, nz = s0 * rz + s1 * sz;
 #pragma unroll
             for (int h = 0; h < H; ++h)
 #pragma unroll
@@ -961,7 +963,8 @@ __global__ void k_grad_gather(int nnpg, int ncon, const int *__restrict__ mconn,
 // out[((e*4 + c)*3 + p)*nee + 3b + i].  One workgroup per element; phase 1
 // puts the fields of u at every quadrature point in LDS, phase 2 gives one
 // thread per element row (b, i) and sums over the quadrature points.
-__global__ void k_fcols(int nen, int nq1, const double *__restrict__ shl, const double *__restrict__ wq,
+__global__ void k_fcols(int nen, int nq1, const double *__restrict__ shl, const double *__restrict__ shlT,
+                        const double *__restrict__ wq,
                         const double *__restrict__ xq, const int *__restrict__ ieng,
                         const double *__restrict__ xl, const double *__restrict__ zl,
                         const char *__restrict__ cdom, const int *__restrict__ lm,
@@ -994,8 +997,8 @@ __global__ void k_fcols(int nen, int nq1, const double *__restrict__ shl, const 
         const double rx = g.dzdeta / g.det, sx = -g.dzdxi / g.det;
         const double rz = -g.dxdeta / g.det, szz = g.dxdxi / g.det;
         for (int a = 0; a < nen; ++a) {
-            const double *s = shl + ((size_t)q * nen + a) * 3;
-            const double n = s[2], nx = s[0] * rx + s[1] * sx, nz = s[0] * rz + s[1] * szz;
+            const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
+            const double n = shlT[(size_t)(a * 3 + 2) * nq + q], nx = s0 * rx + s1 * sx, nz = s0 * rz + s1 * szz;
             for (int c = 0; c < 3; ++c) {
                 const double2 v = su[a * 3 + c];
                 U[c].x += n * v.x; U[c].y += n * v.y;
@@ -1257,28 +1260,59 @@ __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *
 // nonzero of F.  One wave per column, lanes over the right-hand sides,
 // FP64 accumulation; hdiag(MYGRAD(c)) += sum_irhs |J|^2 (the 1x1 CGEMM of
 // pchadj.f90:264-281).  jac[c*nrhs + irhs].
-__global__ void k_pchadj(int nnpgl, int nrhs, const int *__restrict__ mygrad,
-                         const int *__restrict__ jcsc, const int *__restrict__ icsc,
-                         const float2 *__restrict__ gt, const float2 *__restrict__ f,
-                         float2 *__restrict__ jac, float *__restrict__ hdiag) {
+__global__ __launch_bounds__(256) void k_pchadj(int nnpgl, int nrhs, const int *__restrict__ mygrad,
+                                                 const int *__restrict__ jcsc, const int *__restrict__ icsc,
+                                                 const float2 *__restrict__ gt, const float2 *__restrict__ f,
+                                                 float2 *__restrict__ jac, float *__restrict__ hdiag) {
+    // a lane owns right-hand sides r0 + lane + 64 j (j < 3) of a 192-wide
+    // pass; the column's nonzeros are walked two at a time so six rows of G
+    // are in flight per lane
+    constexpr int R = 3;
     const int c = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
     if (c >= nnpgl) return;
     const int i1 = jcsc[c] - 1, i2 = jcsc[c + 1] - 1;
     double hs = 0.0;
-    for (int r0 = 0; r0 < nrhs; r0 += 64) {
-        const int r = r0 + lane;
-        double ar = 0.0, ai = 0.0;
-        if (r < nrhs) {
-            for (int i = i1; i < i2; ++i) {
-                const int row = icsc[i] - 1;
-                if (row < 0) continue;
-                const float2 fv = f[i], g = gt[(size_t)row * nrhs + r];
-                ar += (double)g.x * fv.x - (double)g.y * fv.y;
-                ai += (double)g.x * fv.y + (double)g.y * fv.x;
+    for (int r0 = 0; r0 < nrhs; r0 += 64 * R) {
+        double ar[R], ai[R];
+        bool ok[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) { ar[j] = ai[j] = 0.0; ok[j] = r0 + lane + 64 * j < nrhs; }
+        int i = i1;
+        for (; i + 1 < i2; i += 2) {
+            const int ra = icsc[i] - 1, rb = icsc[i + 1] - 1;
+            const float2 fa = f[i], fb = f[i + 1];
+            float2 ga[R], gb[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const int r = r0 + lane + 64 * j;
+                ga[j] = (ok[j] && ra >= 0) ? gt[(size_t)ra * nrhs + r] : make_float2(0.f, 0.f);
+                gb[j] = (ok[j] && rb >= 0) ? gt[(size_t)rb * nrhs + r] : make_float2(0.f, 0.f);
             }
-            const float2 jv = make_float2((float)ar, (float)ai);
-            jac[(size_t)c * nrhs + r] = jv;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                ar[j] += (double)ga[j].x * fa.x - (double)ga[j].y * fa.y;
+                ai[j] += (double)ga[j].x * fa.y + (double)ga[j].y * fa.x;
+                ar[j] += (double)gb[j].x * fb.x - (double)gb[j].y * fb.y;
+                ai[j] += (double)gb[j].x * fb.y + (double)gb[j].y * fb.x;
+            }
+        }
+        if (i < i2) {
+            const int ra = icsc[i] - 1;
+            const float2 fa = f[i];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const int r = r0 + lane + 64 * j;
+                const float2 ga = (ok[j] && ra >= 0) ? gt[(size_t)ra * nrhs + r] : make_float2(0.f, 0.f);
+                ar[j] += (double)ga.x * fa.x - (double)ga.y * fa.y;
+                ai[j] += (double)ga.x * fa.y + (double)ga.y * fa.x;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if (!ok[j]) continue;
+            const float2 jv = make_float2((float)ar[j], (float)ai[j]);
+            jac[(size_t)c * nrhs + r0 + lane + 64 * j] = jv;
             hs += (double)jv.x * jv.x + (double)jv.y * jv.y;
         }
     }
