@@ -248,6 +248,11 @@ struct ApplyShape {
     static constexpr int RS = (NEN % 2) ? NEN : NEN + 1;
     static constexpr int MT = (NEN + 15) / 16;
     static constexpr int KS = (NEN + 3) / 4;
+    // a last row tile holding at most 4 element rows (p5: 4, p6: 1) is done
+    // on the vector ALUs instead of a mostly empty 16-row MFMA tile
+    static constexpr int REM = NEN - 16 * (MT - 1);
+    static constexpr int NR = (MT > 1 && REM <= 4) ? REM : 0;   // rows on VALU
+    static constexpr int MTM = NR ? MT - 1 : MT;                 // row tiles on MFMA
 };
 
 // LDS bytes of k_apply for a shape.  PH: the nine blocks do not fit, so the
@@ -307,7 +312,7 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW), (REFINE ? 2 : 
                                                const XT *__restrict__ x, XT *__restrict__ y) {
     typedef ApplyShape<NL> S;
     typedef ApplyLds<NL, BT, REFINE> LD;
-    constexpr int NEN = S::NEN, RS = S::RS, MT = S::MT, KS = S::KS;
+    constexpr int NEN = S::NEN, RS = S::RS, MT = S::MTM, KS = S::KS, NR = S::NR;
     constexpr int NP = NEN * NEN;
     constexpr bool PRE = LD::PRE;
     constexpr int NC = (CSEL < 0) ? 3 : 1;
@@ -387,14 +392,19 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW), (REFINE ? 2 : 
         }
         v4 acc[NC][MT];
         double accd[REFINE ? NC : 1][REFINE ? MT : 1][4];
+        // rows 16*MT + r (r < NR): this lane's partial sum over its k values
+        OS rem[NC][NR > 0 ? NR : 1];
 #pragma unroll
-        for (int cc = 0; cc < NC; ++cc)
+        for (int cc = 0; cc < NC; ++cc) {
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 acc[cc][mt] = v4{0, 0, 0, 0};
 #pragma unroll
                 for (int i = 0; i < 4; ++i) accd[REFINE ? cc : 0][REFINE ? mt : 0][i] = 0.0;
             }
+#pragma unroll
+            for (int r = 0; r < (NR > 0 ? NR : 1); ++r) rem[cc][r] = (OS)0;
+        }
 #pragma unroll 1
         for (int s = 0; s < KS; ++s) {
             const OS xv[3] = {xn[0], xn[1], xn[2]};
@@ -459,6 +469,13 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW), (REFINE ? 2 : 
                             acc[cc][mt] = MT_::mma(A[idx], (BT)bv, acc[cc][mt]);
                         }
                     }
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) {
+                        const int row = MT * 16 + r;
+                        const int idx = sb.trans ? (k * RS + row) : (row * RS + k);
+                        const OS av = REFINE ? (OS)((double)A[idx] + (double)Al[idx]) : (OS)A[idx];
+                        rem[cc][r] += av * bv;
+                    }
                 }
             }
         }
@@ -486,6 +503,26 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW), (REFINE ? 2 : 
                 for (int i = 0; i < 4; ++i) {
                     const XS v = (XS)(REFINE ? accd[REFINE ? cc : 0][REFINE ? mt : 0][i] : (double)acc[cc][mt][i]);
                     if (dofs[i] >= 0) reinterpret_cast<XS *>(y + (size_t)dofs[i] * npair + pair)[ri] = old[i] + v;
+                }
+            }
+        // VALU rows: sum the four k-lanes of each column (lanes col + 16 kg),
+        // then lane kg = r stores row 16*MT + r
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                OS v = rem[cc][r];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                const int c = (CSEL < 0) ? cc : CSEL;
+                const int row = MT * 16 + r;
+                if (kg == (r & 3) && live) {
+                    const int dof = sLM[row * 3 + c];
+                    if (dof >= 0) {
+                        XS *py = reinterpret_cast<XS *>(y + (size_t)dof * npair + pair) + ri;
+                        const XS o = ((fmask >> row) & 1ull) ? (XS)0 : *py;
+                        *py = o + (XS)v;
+                    }
                 }
             }
     }

@@ -26,6 +26,9 @@ def mesh_case(name: str):
     if name == "p6":            # 4 x 3, p = 6
         m = cf.structured_mesh(4, 3, 6, width=8e3, depth=6e3)
         return m, cf.layered_model(m, seed=5, n_anom=2), cf.PML()
+    if name == "p5":            # 5 x 3, p = 5
+        m = cf.structured_mesh(5, 3, 5, width=10e3, depth=6e3)
+        return m, cf.layered_model(m, seed=6, n_anom=2), cf.PML()
     if name == "p2_pml":        # 8 x 6, p = 2 with PML
         m = cf.structured_mesh(8, 6, 2, width=16e3, depth=12e3, npml=2, nbielak=1)
         return m, cf.layered_model(m, seed=9, n_anom=2), cf.PML(rcoeff=1e-3, fcent=0.1)

@@ -92,7 +92,7 @@ def test_asmble_config1_and_spmv(torch_dev):
     assert relerr(dy.cpu().numpy(), oracle.spmv_z(g, ref, x)) < FP64_TOL
 
 
-@pytest.mark.parametrize("name", ["tiny_layered", "p6"])
+@pytest.mark.parametrize("name", ["tiny_layered", "p5", "p6"])
 def test_apply_batch_fp64(name):
     mi = _mi(name)
     om, ky, f, py = pairs(2, 3)
@@ -321,3 +321,27 @@ def test_asmble_dispersive(name):
     mi.set_model(mod)        # back to the elastic operator
     s = mi.asmble(b"B", 0.05, 0.7e-4, complex128=True)
     assert relerr(s, oracle.asmble_z(m, mod, g, 0.05, 0.7e-4, pml)) < FP64_TOL
+
+
+@pytest.mark.parametrize("name", ["p5", "p6"])
+def test_apply_batch_modes_high_order(name):
+    """FP32 and refined products at orders 5 and 6 (remainder rows on the
+    vector ALUs, per-component launches at p6) against the FP64 oracle."""
+    import torch
+    from fem25_amd import _lib
+    mi = _mi(name)
+    om, ky, f, py = pairs(2, 4)
+    g = oracle_graph(name)
+    x = cf.seeded_complex(g.ndof, batch=om.size, seed=6)
+    yref = np.stack([oracle_S(name, f[b], py[b]) @ x[b] for b in range(om.size)])
+    xd = torch.from_numpy(np.ascontiguousarray(x.T)).cuda()
+    x32 = xd.to(torch.complex64)
+    y32 = torch.zeros_like(x32)
+    mi.apply_batch_dev(om, ky, x32, y32, _lib.F25_APPLY_F32)
+    yr = torch.zeros_like(xd)
+    mi.apply_batch_dev(om, ky, xd, yr, _lib.F25_APPLY_F32_REFINED)
+    e32 = max(relerr(y32.cpu().numpy().T[b], yref[b]) for b in range(om.size))
+    er = max(relerr(yr.cpu().numpy().T[b], yref[b]) for b in range(om.size))
+    print(f"{name}: fp32 {e32:.2e}, refined {er:.2e}")
+    assert e32 < 1e-5
+    assert er < 1e-8
