@@ -248,10 +248,11 @@ struct ApplyShape {
     static constexpr int RS = (NEN % 2) ? NEN : NEN + 1;
     static constexpr int MT = (NEN + 15) / 16;
     static constexpr int KS = (NEN + 3) / 4;
-    // a last row tile holding at most 4 element rows (p5: 4, p6: 1) is done
-    // on the vector ALUs instead of a mostly empty 16-row MFMA tile
+    // a last row tile holding at most 2 element rows (p6: 1) is done on the
+    // vector ALUs instead of a mostly empty 16-row MFMA tile (with 4 rows,
+    // p5, the extra LDS reads cost more than the MFMA tile saves)
     static constexpr int REM = NEN - 16 * (MT - 1);
-    static constexpr int NR = (MT > 1 && REM <= 4) ? REM : 0;   // rows on VALU
+    static constexpr int NR = (MT > 1 && REM <= 2) ? REM : 0;   // rows on VALU
     static constexpr int MTM = NR ? MT - 1 : MT;                 // row tiles on MFMA
 };
 
