@@ -62,6 +62,8 @@ EXPORTS = {
                                      C.c_void_p]),
     "f25_gnmfree": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_void_p, dp, dp]),
     "f25_gnmfree_dev": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "f25_gn_jp": (C.c_int, [C.c_int, C.c_int, ip, C.c_int, ip, C.c_void_p, dp, C.c_void_p]),
+    "f25_gn_jhz": (C.c_int, [C.c_int, C.c_int, ip, C.c_int, ip, dp, C.c_void_p, C.c_void_p, dp]),
     "f25_gn_fv": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp, C.c_void_p]),
     "f25_gn_fhw": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, dp]),
     "f25_gn_diag": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp]),

@@ -1203,18 +1203,40 @@ extern "C" int f25_pchadj_jac(f25_mesh *m, int nrhs, int lrhs, const void *grnsf
     return r;
 }
 
-static int gnmfree_core(f25_mesh *m, int njac, const int *nobs, const double *alpha, const float2 *jac,
-                        const double *p, double *z) {
-    const int n = m->inv.nnpgl;
-    std::vector<int64_t> joff(njac), poff(njac);
-    int64_t jt = 0, pt = 0;
+// packed offsets of the njac Jacobians (nobs[j] x nnpgl each) and of their
+// JP vectors
+static int gn_offsets(int njac, const int *nobs, int nnpgl, std::vector<int64_t> &joff,
+                      std::vector<int64_t> &poff, int64_t &jt, int64_t &pt) {
+    joff.resize(njac);
+    poff.resize(njac);
+    jt = pt = 0;
     for (int j = 0; j < njac; ++j) {
         if (nobs[j] < 0) FAIL(F25_ERR_ARG, "negative observation count");
         joff[j] = jt;
         poff[j] = pt;
-        jt += (int64_t)nobs[j] * n;
+        jt += (int64_t)nobs[j] * nnpgl;
         pt += nobs[j];
     }
+    return 0;
+}
+
+static int gn_check(int nnpgl, int na35, const int *mygrad, int njac) {
+    if (nnpgl <= 0 || na35 <= 0 || !mygrad || njac <= 0 || njac > 65535) FAIL(F25_ERR_ARG, "invalid arguments");
+    std::vector<char> seen(na35, 0);
+    for (int c = 0; c < nnpgl; ++c) {
+        if (mygrad[c] < 1 || mygrad[c] > na35) FAIL(F25_ERR_ARG, "MYGRAD out of range");
+        if (seen[mygrad[c] - 1]) FAIL(F25_ERR_ARG, "MYGRAD repeats a gradient point");
+        seen[mygrad[c] - 1] = 1;
+    }
+    return 0;
+}
+
+static int gnmfree_core(f25_mesh *m, int njac, const int *nobs, const double *alpha, const float2 *jac,
+                        const double *p, double *z) {
+    const int n = m->inv.nnpgl;
+    std::vector<int64_t> joff, poff;
+    int64_t jt, pt;
+    TRY(gn_offsets(njac, nobs, n, joff, poff, jt, pt));
     int *dn = nullptr;
     int64_t *djo = nullptr, *dpo = nullptr;
     double *da = nullptr;
@@ -1240,6 +1262,77 @@ static int gnmfree_core(f25_mesh *m, int njac, const int *nobs, const double *al
         if (!r && hipEventElapsedTime(&ms, m->ev0, m->ev1) == hipSuccess) record_ms("gnmfree", ms);
     }
     dfree(dn); dfree(djo); dfree(dpo); dfree(da); dfree(djp);
+    return r;
+}
+
+// GNMFREE in two halves on caller-held arrays, for a caller that reduces JP
+// across ranks in between (gnmfree.f90:195-196)
+extern "C" int f25_gn_jp(int nnpgl, int na35, const int *mygrad, int njac, const int *nobs, const void *jac,
+                         const double *p, void *jp) {
+    TRY(gn_check(nnpgl, na35, mygrad, njac));
+    if (!nobs || !jac || !p || !jp) FAIL(F25_ERR_ARG, "invalid arguments");
+    std::vector<int64_t> joff, poff;
+    int64_t jt, pt;
+    TRY(gn_offsets(njac, nobs, nnpgl, joff, poff, jt, pt));
+    TmpStream ts;
+    HIPCHK(hipStreamCreate(&ts.s));
+    int *dmy = nullptr, *dn = nullptr;
+    int64_t *djo = nullptr, *dpo = nullptr;
+    float2 *dj = nullptr;
+    double *dp_ = nullptr;
+    double2 *djp = nullptr;
+    int r = upload(&dmy, mygrad, (size_t)nnpgl, ts.s);
+    if (!r) r = upload(&dn, nobs, (size_t)njac, ts.s);
+    if (!r) r = upload(&djo, joff.data(), (size_t)njac, ts.s);
+    if (!r) r = upload(&dpo, poff.data(), (size_t)njac, ts.s);
+    if (!r) r = upload(&dj, (const float2 *)jac, (size_t)jt, ts.s);
+    if (!r) r = upload(&dp_, p, (size_t)na35, ts.s);
+    if (!r) r = dalloc(&djp, (size_t)std::max<int64_t>(pt, 1));
+    if (!r) {
+        fill_async(djp, (size_t)std::max<int64_t>(pt, 1), make_double2(0.0, 0.0), ts.s);
+        const int cch = 64;
+        hipLaunchKernelGGL(k_gn_jp, dim3((nnpgl + cch - 1) / cch, njac), dim3(256), 0, ts.s, nnpgl, cch, dn, djo,
+                           dpo, dmy, dj, dp_, djp);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess && pt > 0) e = hipMemcpyAsync(jp, djp, (size_t)pt * sizeof(double2), hipMemcpyDeviceToHost, ts.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(ts.s);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(dmy); dfree(dn); dfree(djo); dfree(dpo); dfree(dj); dfree(dp_); dfree(djp);
+    return r;
+}
+
+extern "C" int f25_gn_jhz(int nnpgl, int na35, const int *mygrad, int njac, const int *nobs, const double *alpha,
+                          const void *jac, const void *jp, double *z) {
+    TRY(gn_check(nnpgl, na35, mygrad, njac));
+    if (!nobs || !alpha || !jac || !jp || !z) FAIL(F25_ERR_ARG, "invalid arguments");
+    std::vector<int64_t> joff, poff;
+    int64_t jt, pt;
+    TRY(gn_offsets(njac, nobs, nnpgl, joff, poff, jt, pt));
+    TmpStream ts;
+    HIPCHK(hipStreamCreate(&ts.s));
+    int *dmy = nullptr, *dn = nullptr;
+    int64_t *djo = nullptr, *dpo = nullptr;
+    float2 *dj = nullptr;
+    double *da = nullptr, *dz = nullptr;
+    double2 *djp = nullptr;
+    int r = upload(&dmy, mygrad, (size_t)nnpgl, ts.s);
+    if (!r) r = upload(&dn, nobs, (size_t)njac, ts.s);
+    if (!r) r = upload(&djo, joff.data(), (size_t)njac, ts.s);
+    if (!r) r = upload(&dpo, poff.data(), (size_t)njac, ts.s);
+    if (!r) r = upload(&da, alpha, (size_t)njac, ts.s);
+    if (!r) r = upload(&dj, (const float2 *)jac, (size_t)jt, ts.s);
+    if (!r) r = upload(&djp, (const double2 *)jp, (size_t)pt, ts.s);
+    if (!r) r = upload(&dz, z, (size_t)na35, ts.s);
+    if (!r) {
+        hipLaunchKernelGGL(k_gn_z, dim3((nnpgl + 3) / 4), dim3(256), 0, ts.s, nnpgl, njac, dn, djo, dpo, da, dmy,
+                           dj, djp, dz);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(z, dz, (size_t)na35 * sizeof(double), hipMemcpyDeviceToHost, ts.s);
+        if (e == hipSuccess) e = hipStreamSynchronize(ts.s);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(dmy); dfree(dn); dfree(djo); dfree(dpo); dfree(da); dfree(dj); dfree(djp); dfree(dz);
     return r;
 }
 

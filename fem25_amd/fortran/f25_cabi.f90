@@ -79,6 +79,18 @@
             INTEGER(C_INT), VALUE :: NNPGL, NDOF, NZ_FDIST, NA35
             TYPE(C_PTR), VALUE :: MYGRAD, JCSC, ICSC, FMAT, ADJLAM, VWORK
          END FUNCTION
+         INTEGER(C_INT) FUNCTION F25_GN_JP(NNPGL, NA35, MYGRAD, NJAC, NOBS, JAC, P, JP)        &
+                        BIND(C, NAME='f25_gn_jp')
+            IMPORT :: C_INT, C_PTR
+            INTEGER(C_INT), VALUE :: NNPGL, NA35, NJAC
+            TYPE(C_PTR), VALUE :: MYGRAD, NOBS, JAC, P, JP
+         END FUNCTION
+         INTEGER(C_INT) FUNCTION F25_GN_JHZ(NNPGL, NA35, MYGRAD, NJAC, NOBS, ALPHA, JAC, JP, Z) &
+                        BIND(C, NAME='f25_gn_jhz')
+            IMPORT :: C_INT, C_PTR
+            INTEGER(C_INT), VALUE :: NNPGL, NA35, NJAC
+            TYPE(C_PTR), VALUE :: MYGRAD, NOBS, ALPHA, JAC, JP, Z
+         END FUNCTION
          TYPE(C_PTR) FUNCTION F25_LAST_ERROR() BIND(C, NAME='f25_last_error')
             IMPORT :: C_PTR
          END FUNCTION

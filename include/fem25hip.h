@@ -225,6 +225,18 @@ int f25_gnmfree(f25_mesh *m, int njac, const int *nobs, const double *alpha, con
 int f25_gnmfree_dev(f25_mesh *m, int njac, const int *nobs, const double *alpha, const void *jac_dev,
                     const double *p_dev, double *z_dev);
 
+/* GNMFREE's two products on caller-held arrays (no handle), for a caller
+ * that sums JP over ranks in between as gnmfree.f90:195-196 does:
+ *   f25_gn_jp:  JP_j(obs) = sum_c J_j(obs,c) p(MYGRAD(c)), jp COMPLEX*16,
+ *               the njac vectors packed (sum of nobs values);
+ *   f25_gn_jhz: z(MYGRAD(c)) += sum_j alpha[j] Re( J_j(:,c)^H JP_j ).
+ * MYGRAD(NNPGL) 1-based and distinct, p and z REAL*8 (NA35), jac packed as
+ * for f25_gnmfree; host arrays. */
+int f25_gn_jp(int nnpgl, int na35, const int *mygrad, int njac, const int *nobs, const void *jac,
+              const double *p, void *jp);
+int f25_gn_jhz(int nnpgl, int na35, const int *mygrad, int njac, const int *nobs, const double *alpha,
+               const void *jac, const void *jp, double *z);
+
 /* ---- Gauss-Newton element products ---------------------------------------- */
 
 /* F v = dS/dm[v] u for a model perturbation v (NNPG,3) = (dVp, dVs, drho)
