@@ -64,6 +64,10 @@ EXPORTS = {
     "f25_gnmfree_dev": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_void_p, C.c_void_p, C.c_void_p]),
     "f25_gn_jp": (C.c_int, [C.c_int, C.c_int, ip, C.c_int, ip, C.c_void_p, dp, C.c_void_p]),
     "f25_gn_jhz": (C.c_int, [C.c_int, C.c_int, ip, C.c_int, ip, dp, C.c_void_p, C.c_void_p, dp]),
+    "f25_qp_nu_alpha": (C.c_int, [C.c_void_p, C.c_double, C.c_double, fp, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, fp]),
+    "f25_met225": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                             fp, fp]),
     "f25_gn_fv": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp, C.c_void_p]),
     "f25_gn_fhw": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, dp]),
     "f25_gn_diag": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp]),

@@ -258,6 +258,32 @@ class MeshInfo:
                                       jac.ctypes.data_as(C.c_void_p), _p(p, dp), _p(z, dp)), "f25_gnmfree")
         return z
 
+    def qp_nu_alpha(self, freq: float, py: float, v, adjlam, wave):
+        """CADJ_NU_ALPHA's element loops (qpmfree.f90:622-912): returns
+        (RHSBNU, RHSALP) COMPLEX*8 (NDOF) and P3 REAL*4 (NA35), before the
+        solves for nu and alpha."""
+        inv = self.inv
+        v = np.ascontiguousarray(v, np.float32)
+        adjlam = np.ascontiguousarray(adjlam, np.complex64)
+        wave = np.ascontiguousarray(wave, np.complex64)
+        bnu = np.zeros(self.ndof, np.complex64)
+        alp = np.zeros(self.ndof, np.complex64)
+        p3 = np.zeros(inv.na35, np.float32)
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)   # noqa: E731
+        check(_lib.load().f25_qp_nu_alpha(self.h, freq, py, _p(v, fp), vp(adjlam), vp(wave), vp(bnu), vp(alp),
+                                          _p(p3, fp)), "f25_qp_nu_alpha")
+        return bnu, alp, p3
+
+    def met225(self, freq: float, py: float, adjlam, nu, alpha, wave):
+        """MET225_MOD's element loops (qpmfree.f90:1000-1215): (P1, P2)."""
+        inv = self.inv
+        vecs = [np.ascontiguousarray(x, np.complex64) for x in (adjlam, nu, alpha, wave)]
+        p1 = np.zeros(inv.na35, np.float32)
+        p2 = np.zeros(inv.na35, np.float32)
+        check(_lib.load().f25_met225(self.h, freq, py, *[x.ctypes.data_as(C.c_void_p) for x in vecs],
+                                     _p(p1, fp), _p(p2, fp)), "f25_met225")
+        return p1, p2
+
     # -- inversion bookkeeping -------------------------------------------------
     def inv_setup(self, cinvtype: bytes = b"PP", vpvs: float = 3 ** 0.5, linrec: bool = True,
                   lkbdry: bool = True, mrdof1=None) -> InvInfo:

@@ -966,13 +966,15 @@ extern "C" int f25_inv_get(f25_mesh *m, int *maskg, int *mygrad, int *jcsc, int 
     return 0;
 }
 
-// per-column element/corner/collocation constants (model dependent)
-static int cfmat_columns(f25_mesh *m) {
+// the element each column's product comes from: the last of MCONN(a,:)
+// (VFORCE25 and QPMFREE clear the column per element, vforce25.f:252-256,
+// qpmfree.f90:810-813, 1155-1160), and the anchor's corner in it (-1 when
+// the element does not hold the anchor: the collocated field is then zero)
+static int last_elements(const f25_mesh *m, std::vector<int> &ce, std::vector<int> &ck) {
     const InvInfo &inv = m->inv;
     const int n = inv.nnpgl, nnpg = m->d.nnpg;
-    std::vector<int> ce(n), ck(n);
-    std::vector<double> cp(n), cs(n);
-    const double v2 = inv.vpvs * inv.vpvs;
+    ce.assign(n, 0);
+    ck.assign(n, -1);
     for (int c = 0; c < n; ++c) {
         const int a = inv.node_of_col[c];
         int e = 0;
@@ -983,15 +985,28 @@ static int cfmat_columns(f25_mesh *m) {
         }
         if (e <= 0) FAIL(F25_ERR_STATE, "inversion anchor without elements");
         --e;
-        int corner = -1;
-        for (int k = 0; k < 4; ++k)
-            if (m->ieng[4 * e + k] - 1 == a) corner = k;
         ce[c] = e;
-        ck[c] = corner;
+        for (int k = 0; k < 4; ++k)
+            if (m->ieng[4 * e + k] - 1 == a) ck[c] = k;
+    }
+    return 0;
+}
+
+// per-column element/corner/collocation constants (model dependent)
+static int cfmat_columns(f25_mesh *m) {
+    const InvInfo &inv = m->inv;
+    const int n = inv.nnpgl;
+    std::vector<int> ce, ck;
+    TRY(last_elements(m, ce, ck));
+    std::vector<double> cp(n), cs(n);
+    const double v2 = inv.vpvs * inv.vpvs;
+    for (int c = 0; c < n; ++c) {
+        const int a = inv.node_of_col[c];
         const double alpha = std::sqrt((m->lam[a] + 2.0 * m->mu[a]) / m->rho[a]);
         const double beta = std::sqrt(m->mu[a] / m->rho[a]);
         if (inv.lisp) { cp[c] = 2.0 * alpha; cs[c] = 2.0 * alpha / v2; }
         else { cs[c] = 2.0 * beta; cp[c] = 2.0 * beta * v2; }
+        if (ck[c] < 0) { ck[c] = 0; cp[c] = cs[c] = 0.0; }
     }
     dfree(m->d_col_elem); dfree(m->d_col_corner); dfree(m->d_col_cp); dfree(m->d_col_cs);
     TRY(upload(&m->d_col_elem, ce.data(), n, m->st));
@@ -1032,6 +1047,237 @@ extern "C" int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *w
     }
     dfree(dw);
     dfree(df);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// QPMFREE's element products (CADJ_NU_ALPHA, MET225_MOD) on DABMKISOB25
+// matrices.  2 Vp (PP) or 2 Vs (SS) at each column's anchor: DJACAB/DJACAB2's
+// collocation constant (djac.f:94-103, 240-250).
+static void dab_constants(const f25_mesh *m, std::vector<double> &da) {
+    const InvInfo &inv = m->inv;
+    da.resize(inv.nnpgl);
+    for (int c = 0; c < inv.nnpgl; ++c) {
+        const int a = inv.node_of_col[c];
+        da[c] = inv.lisp ? 2.0 * std::sqrt((m->lam[a] + 2.0 * m->mu[a]) / m->rho[a])
+                         : 2.0 * std::sqrt(m->mu[a] / m->rho[a]);
+    }
+}
+
+// elements holding at least one local column on a corner (every element of
+// MCONN(a,:) for every column's anchor a), coloured so that one launch per
+// colour scatters without collisions
+static int ensure_dab_part(f25_mesh *m) {
+    InvInfo &inv = m->inv;
+    if (!inv.dab_cptr.empty()) return 0;
+    const int nelem = m->d.nelem, nnpg = m->d.nnpg;
+    std::vector<int> col4((size_t)nelem * 4, -1);
+    for (int c = 0; c < inv.nnpgl; ++c) {
+        const int a = inv.node_of_col[c];
+        for (int k = 0; k < inv.ncon; ++k) {
+            const int v = inv.mconn[(size_t)k * nnpg + a];
+            if (v <= 0) break;
+            const int e = v - 1;
+            for (int kk = 0; kk < 4; ++kk)
+                if (m->ieng[4 * e + kk] - 1 == a) {
+                    if (col4[4 * e + kk] >= 0 && col4[4 * e + kk] != c)
+                        FAIL(F25_ERR_ARG, "two local columns on one anchor");
+                    col4[4 * e + kk] = c;
+                }
+        }
+    }
+    std::vector<int> list, sub;
+    for (int e = 0; e < nelem; ++e)
+        if (col4[4 * e] >= 0 || col4[4 * e + 1] >= 0 || col4[4 * e + 2] >= 0 || col4[4 * e + 3] >= 0) {
+            list.push_back(e);
+            sub.insert(sub.end(), m->ieng.begin() + 4 * e, m->ieng.begin() + 4 * e + 4);
+        }
+    std::vector<int> order, ptr;
+    if (!list.empty()) colour_elements((int)list.size(), nnpg, sub.data(), order, ptr);
+    else ptr.assign(1, 0);
+    inv.dab_elem.resize(list.size());
+    inv.dab_col4.resize(4 * list.size());
+    for (size_t i = 0; i < list.size(); ++i) {
+        const int e = list[order[i]];
+        inv.dab_elem[i] = e;
+        for (int k = 0; k < 4; ++k) inv.dab_col4[4 * i + k] = col4[4 * e + k];
+    }
+    inv.dab_cptr = ptr;
+    return 0;
+}
+
+static size_t dab_lds(const f25_mesh *m) {
+    const int nee = 3 * m->tab.nen, nq = m->tab.nq;
+    return (size_t)(2 * nee + 18 * nq) * sizeof(double2) + 5 * (size_t)nq * sizeof(double);
+}
+
+static int dab_prepare(f25_mesh *m) {
+    if (!m->have_inv) FAIL(F25_ERR_STATE, "f25_inv_setup has not been called");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    if (m->inv.nvinv != 1) FAIL(F25_ERR_ARG, "NVINV = 1 (PP or SS) only");
+    if (dab_lds(m) > 160 * 1024) FAIL(F25_ERR_ARG, "element order too high for the LDS tiling");
+    HIPCHK(hipSetDevice(m->dev));
+    if (dab_lds(m) > 64 * 1024)
+        HIPCHK(hipFuncSetAttribute((const void *)k_dab, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dab_lds(m)));
+    return 0;
+}
+
+// per-column items (mode 1): the last element, coefficient on its corner
+static int dab_column_items(const f25_mesh *m, const std::vector<double> &da, const float *v,
+                            std::vector<int> &elem, std::vector<double> &coef) {
+    const InvInfo &inv = m->inv;
+    std::vector<int> ce, ck;
+    TRY(last_elements(m, ce, ck));
+    elem = ce;
+    coef.assign(4 * (size_t)inv.nnpgl, 0.0);
+    for (int c = 0; c < inv.nnpgl; ++c) {
+        if (ck[c] < 0) continue;
+        float w = (float)inv.elem_wts[ce[c]];
+        if (v) w *= v[inv.mygrad[c] - 1];   // CMPLX(SNGL(ELEM_WTS) * V(IA35)), qpmfree.f90:866
+        coef[4 * c + ck[c]] = (double)w * da[c];
+    }
+    return 0;
+}
+
+static int dab_launch_columns(f25_mesh *m, double kv, int nvec, const std::vector<int> &elem,
+                              const std::vector<double> &coef, const float2 *x0, const float2 *x1,
+                              const float2 *z0, const float2 *z1, float *p0, float *p1) {
+    const InvInfo &inv = m->inv;
+    int *de = nullptr;
+    double *dc = nullptr;
+    int r = upload(&de, elem.data(), elem.size(), m->st);
+    if (!r) r = upload(&dc, coef.data(), coef.size(), m->st);
+    if (!r) {
+        hipLaunchKernelGGL(k_dab, dim3(inv.nnpgl), dim3(256), dab_lds(m), m->st, 1, m->tab.nen, m->tab.nq1,
+                           inv.nnpgl, nvec, inv.lisp ? 0 : 1, kv, de, dc, m->d_shl, m->d_wq, m->d_xq, m->d_ieng,
+                           m->d_xl, m->d_zl, m->d_lm, x0, x1, (double2 *)nullptr, (double2 *)nullptr, z0, z1,
+                           m->d_mygrad, m->d_jcsc, m->d_icsc, p0, p1, m->d_err);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(de); dfree(dc);
+    return r;
+}
+
+static int dab_download(f25_mesh *m, float *dst, const float *src, size_t n) {
+    hipError_t e = hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToHost, m->st);
+    if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float *v, const void *adjlam,
+                               const void *wave, void *rhsbnu, void *rhsalp, float *p3) {
+    if (!m || !v || !adjlam || !wave || !rhsbnu || !rhsalp || !p3) FAIL(F25_ERR_ARG, "null argument");
+    TRY(dab_prepare(m));
+    TRY(ensure_dab_part(m));
+    const InvInfo &inv = m->inv;
+    const int ndof = m->g.ndof;
+    const double kv = 6.2831853071795862 * freq * py;
+    std::vector<double> da;
+    dab_constants(m, da);
+    // element items: coefficient of every corner that holds a local column
+    const size_t ne = inv.dab_elem.size();
+    std::vector<double> ecoef(4 * std::max<size_t>(ne, 1), 0.0);
+    for (size_t i = 0; i < ne; ++i)
+        for (int k = 0; k < 4; ++k) {
+            const int c = inv.dab_col4[4 * i + k];
+            if (c < 0) continue;
+            const float w = (float)inv.elem_wts[inv.dab_elem[i]] * v[inv.mygrad[c] - 1];
+            ecoef[4 * i + k] = (double)w * da[c];
+        }
+    std::vector<int> celem;
+    std::vector<double> ccoef;
+    TRY(dab_column_items(m, da, v, celem, ccoef));
+    float2 *du = nullptr, *dl = nullptr;
+    double2 *ya = nullptr, *yb = nullptr;
+    int *de = nullptr;
+    double *dc = nullptr;
+    float *dp3 = nullptr;
+    int r = upload(&du, (const float2 *)wave, ndof, m->st);
+    if (!r) r = upload(&dl, (const float2 *)adjlam, ndof, m->st);
+    if (!r) r = dalloc(&ya, ndof);
+    if (!r) r = dalloc(&yb, ndof);
+    if (!r) r = upload(&de, inv.dab_elem.data(), ne, m->st);
+    if (!r) r = upload(&dc, ecoef.data(), 4 * ne, m->st);
+    if (!r) r = dalloc(&dp3, inv.na35);
+    if (!r) {
+        fill_async(ya, (size_t)ndof, make_double2(0.0, 0.0), m->st);
+        fill_async(yb, (size_t)ndof, make_double2(0.0, 0.0), m->st);
+        fill_async(dp3, (size_t)inv.na35, 0.f, m->st);
+        (void)hipEventRecord(m->ev0, m->st);
+        for (size_t col = 0; col + 1 < inv.dab_cptr.size(); ++col) {
+            const int o = inv.dab_cptr[col], n = inv.dab_cptr[col + 1] - o;
+            if (n <= 0) continue;
+            hipLaunchKernelGGL(k_dab, dim3(n), dim3(256), dab_lds(m), m->st, 0, m->tab.nen, m->tab.nq1, n, 2,
+                               inv.lisp ? 0 : 1, kv, de + o, dc + 4 * (size_t)o, m->d_shl, m->d_wq, m->d_xq,
+                               m->d_ieng, m->d_xl, m->d_zl, m->d_lm, du, dl, ya, yb, (const float2 *)nullptr,
+                               (const float2 *)nullptr, m->d_mygrad, m->d_jcsc, m->d_icsc, (float *)nullptr,
+                               (float *)nullptr, m->d_err);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+        if (!r) r = dab_launch_columns(m, kv, 1, celem, ccoef, du, nullptr, dl, nullptr, dp3, nullptr);
+        (void)hipEventRecord(m->ev1, m->st);
+    }
+    if (!r) r = check_err(m);
+    float ms = 0.f;
+    if (!r && hipEventElapsedTime(&ms, m->ev0, m->ev1) == hipSuccess) record_ms("qp", ms);
+    if (!r) {
+        std::vector<double2> ha(ndof), hb(ndof);
+        hipError_t e = hipMemcpyAsync(ha.data(), ya, ndof * sizeof(double2), hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(hb.data(), yb, ndof * sizeof(double2), hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) r = dab_download(m, p3, dp3, inv.na35);
+        if (e == hipSuccess && !r) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+        if (!r) {
+            float2 *oa = (float2 *)rhsalp, *ob = (float2 *)rhsbnu;
+            for (int i = 0; i < ndof; ++i) {
+                oa[i] = make_float2((float)ha[i].x, (float)ha[i].y);
+                ob[i] = make_float2((float)hb[i].x, (float)hb[i].y);
+            }
+        }
+    }
+    dfree(du); dfree(dl); dfree(ya); dfree(yb); dfree(de); dfree(dc); dfree(dp3);
+    return r;
+}
+
+extern "C" int f25_met225(f25_mesh *m, double freq, double py, const void *adjlam, const void *nu,
+                          const void *alpha, const void *wave, float *p1, float *p2) {
+    if (!m || !adjlam || !nu || !alpha || !wave || !p1 || !p2) FAIL(F25_ERR_ARG, "null argument");
+    TRY(dab_prepare(m));
+    const InvInfo &inv = m->inv;
+    const int ndof = m->g.ndof;
+    const double kv = 6.2831853071795862 * freq * py;
+    std::vector<double> da;
+    dab_constants(m, da);
+    std::vector<int> celem;
+    std::vector<double> ccoef;
+    TRY(dab_column_items(m, da, nullptr, celem, ccoef));
+    float2 *dl = nullptr, *dn = nullptr, *da_ = nullptr, *du = nullptr;
+    float *d1 = nullptr, *d2 = nullptr;
+    int r = upload(&dl, (const float2 *)adjlam, ndof, m->st);
+    if (!r) r = upload(&dn, (const float2 *)nu, ndof, m->st);
+    if (!r) r = upload(&da_, (const float2 *)alpha, ndof, m->st);
+    if (!r) r = upload(&du, (const float2 *)wave, ndof, m->st);
+    if (!r) r = dalloc(&d1, inv.na35);
+    if (!r) r = dalloc(&d2, inv.na35);
+    if (!r) {
+        fill_async(d1, (size_t)inv.na35, 0.f, m->st);
+        fill_async(d2, (size_t)inv.na35, 0.f, m->st);
+        (void)hipEventRecord(m->ev0, m->st);
+        r = dab_launch_columns(m, kv, 2, celem, ccoef, du, da_, dn, dl, d1, d2);
+        (void)hipEventRecord(m->ev1, m->st);
+    }
+    if (!r) r = check_err(m);
+    float ms = 0.f;
+    if (!r && hipEventElapsedTime(&ms, m->ev0, m->ev1) == hipSuccess) record_ms("met225", ms);
+    if (!r) r = dab_download(m, p1, d1, inv.na35);
+    if (!r) r = dab_download(m, p2, d2, inv.na35);
+    if (!r) {
+        hipError_t e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(dl); dfree(dn); dfree(da_); dfree(du); dfree(d1); dfree(d2);
     return r;
 }
 

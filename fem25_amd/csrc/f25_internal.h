@@ -46,6 +46,10 @@ struct InvInfo {
     std::vector<int> icsc;       // nz_fdist, 1-based DOFs
     std::vector<double> elem_wts;// nelem
     std::vector<int> node_of_col;// nnpgl: anchor (0-based) of each column
+    // second-order products (QPMFREE): elements with at least one local
+    // column on a corner, in colour order, and per element the column on
+    // each corner (-1 none); built on first use
+    std::vector<int> dab_elem, dab_cptr, dab_col4;
 };
 int build_inv(const f25_mesh_desc &d, const Graph &g, int nvinv, int linrec, int lkbdry,
               int nrec, const int *mrdof1, InvInfo &inv);

@@ -1292,6 +1292,156 @@ __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *
 }
 
 // ---------------------------------------------------------------------------
+// Second-order element products of QPMFREE (qpmfree.f90:622-1215) on the
+// DABMKISOB25 element matrices (dstiff25.f:1-203).  The derivative matrix of
+// element e collocated on its corner k is the elastic element operator with
+// the ky coupling in the imaginary part, no -w^2 rho mass term, and
+// density-normalised fields (lambda, mu) = (D, 0) for Vp (PP) or (-2D, D)
+// for Vs (SS), D(q) = 2 v_k N_k(q), v_k = Vp or Vs at the anchor.  By
+// linearity a workgroup forms one field D(q) = sum_k c_k N_k(q) from its
+// item's four corner coefficients and applies the operator M to nvec <= 2
+// vectors through fluxes at the quadrature points:
+//   mode 0 (item = element; one launch per colour, so rows never collide):
+//          y_v += M x_v (M is Hermitian: M x is also M^H x);
+//   mode 1 (item = local column c, its element the last connected one):
+//          p_v(MYGRAD(c)) = -sum over the element's rows Re(conj(M x_v) z_v),
+//          every row checked against column c's CSC pattern.
+// Flux layout per (vector, q): [comp][N, Nx, Nz] complex.
+__global__ __launch_bounds__(256) void k_dab(int mode, int nen, int nq1, int nitems, int nvec, int liss,
+                                             double kv, const int *__restrict__ item_elem,
+                                             const double *__restrict__ item_coef,
+                                             const double *__restrict__ shl, const double *__restrict__ wq,
+                                             const double *__restrict__ xq, const int *__restrict__ ieng,
+                                             const double *__restrict__ xl, const double *__restrict__ zl,
+                                             const int *__restrict__ lm, const float2 *__restrict__ x0,
+                                             const float2 *__restrict__ x1, double2 *__restrict__ y0,
+                                             double2 *__restrict__ y1, const float2 *__restrict__ z0,
+                                             const float2 *__restrict__ z1, const int *__restrict__ mygrad,
+                                             const int *__restrict__ jcsc, const int *__restrict__ icsc,
+                                             float *__restrict__ p0, float *__restrict__ p1,
+                                             int *__restrict__ err) {
+    extern __shared__ double sm[];
+    __shared__ double red[2][4];
+    const int it = blockIdx.x;
+    if (it >= nitems) return;
+    const int e = item_elem[it];
+    const double c0 = item_coef[4 * it], c1 = item_coef[4 * it + 1], c2 = item_coef[4 * it + 2],
+                 c3 = item_coef[4 * it + 3];
+    const int nq = nq1 * nq1, nee = 3 * nen;
+    double2 *su = reinterpret_cast<double2 *>(sm);           // [v][3 nen]
+    double2 *sF = su + 2 * nee;                               // [v][q][9]
+    double *sG = reinterpret_cast<double *>(sF + 2 * 9 * nq); // [q][5]
+    const int *ie = ieng + 4 * e;
+    const int *L = lm + (size_t)e * nee;
+    for (int i = threadIdx.x; i < nee; i += blockDim.x) {
+        const int d = L[i];
+        float2 a = d >= 0 ? x0[d] : make_float2(0.f, 0.f);
+        su[i] = make_double2(a.x, a.y);
+        if (nvec > 1) {
+            float2 b = d >= 0 ? x1[d] : make_float2(0.f, 0.f);
+            su[nee + i] = make_double2(b.x, b.y);
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+        const int ix = q % nq1, iz = q / nq1;
+        Geo g;
+        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) *err = 2;
+        const double D = wq[ix] * wq[iz] * g.det * (c0 * g.sf[0] + c1 * g.sf[1] + c2 * g.sf[2] + c3 * g.sf[3]);
+        const double P = liss ? 0.0 : D, Q = liss ? -2.0 * D : D, S = liss ? D : 0.0, k2 = kv * kv;
+        double *G = sG + 5 * q;
+        G[0] = g.dzdeta; G[1] = g.dzdxi; G[2] = g.dxdeta; G[3] = g.dxdxi; G[4] = g.det;
+        for (int v = 0; v < nvec; ++v) {
+            double2 U[3] = {}, UX[3] = {}, UZ[3] = {};
+            const double2 *x = su + v * nee;
+            for (int a = 0; a < nen; ++a) {
+                const double *s = shl + ((size_t)q * nen + a) * 3;
+                const double n = s[2];
+                const double nx = (s[0] * g.dzdeta - s[1] * g.dzdxi) / g.det;
+                const double nz = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / g.det;
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    const double2 w = x[3 * a + cc];
+                    U[cc].x += n * w.x; U[cc].y += n * w.y;
+                    UX[cc].x += nx * w.x; UX[cc].y += nx * w.y;
+                    UZ[cc].x += nz * w.x; UZ[cc].y += nz * w.y;
+                }
+            }
+            double2 *F = sF + ((size_t)v * nq + q) * 9;
+            // i kv X = (-kv X.y, kv X.x)
+            // comp 1: N: i kv S UX2 + kv^2 S U1; Nx: P UX1 + Q UZ3 - i kv Q U2; Nz: S (UZ1 + UX3)
+            F[0] = make_double2(-kv * S * UX[1].y + k2 * S * U[0].x, kv * S * UX[1].x + k2 * S * U[0].y);
+            F[1] = make_double2(P * UX[0].x + Q * UZ[2].x + kv * Q * U[1].y,
+                                P * UX[0].y + Q * UZ[2].y - kv * Q * U[1].x);
+            F[2] = make_double2(S * (UZ[0].x + UX[2].x), S * (UZ[0].y + UX[2].y));
+            // comp 2: N: i kv Q (UX1 + UZ3) + kv^2 P U2; Nx: S (UX2 - i kv U1); Nz: S (UZ2 - i kv U3)
+            F[3] = make_double2(-kv * Q * (UX[0].y + UZ[2].y) + k2 * P * U[1].x,
+                                kv * Q * (UX[0].x + UZ[2].x) + k2 * P * U[1].y);
+            F[4] = make_double2(S * (UX[1].x + kv * U[0].y), S * (UX[1].y - kv * U[0].x));
+            F[5] = make_double2(S * (UZ[1].x + kv * U[2].y), S * (UZ[1].y - kv * U[2].x));
+            // comp 3: N: i kv S UZ2 + kv^2 S U3; Nx: S (UZ1 + UX3); Nz: Q UX1 + P UZ3 - i kv Q U2
+            F[6] = make_double2(-kv * S * UZ[1].y + k2 * S * U[2].x, kv * S * UZ[1].x + k2 * S * U[2].y);
+            F[7] = F[2];
+            F[8] = make_double2(Q * UX[0].x + P * UZ[2].x + kv * Q * U[1].y,
+                                Q * UX[0].y + P * UZ[2].y - kv * Q * U[1].x);
+        }
+    }
+    __syncthreads();
+    int i1 = 0, i2 = 0;
+    if (mode == 1) { i1 = jcsc[it] - 1; i2 = jcsc[it + 1] - 1; }
+    double acc[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < nee; i += blockDim.x) {
+        const int a = i / 3, cc = i % 3;
+        const int dof = L[i];
+        if (dof < 0) continue;
+        double2 val[2] = {};
+        for (int q = 0; q < nq; ++q) {
+            const double *s = shl + ((size_t)q * nen + a) * 3;
+            const double *G = sG + 5 * q;
+            const double n = s[2];
+            const double nx = (s[0] * G[0] - s[1] * G[1]) / G[4];
+            const double nz = -(s[0] * G[2] - s[1] * G[3]) / G[4];
+            for (int v = 0; v < nvec; ++v) {
+                const double2 *F = sF + ((size_t)v * nq + q) * 9 + 3 * cc;
+                val[v].x += n * F[0].x + nx * F[1].x + nz * F[2].x;
+                val[v].y += n * F[0].y + nx * F[1].y + nz * F[2].y;
+            }
+        }
+        if (mode == 0) {
+            y0[dof].x += val[0].x; y0[dof].y += val[0].y;
+            if (nvec > 1) { y1[dof].x += val[1].x; y1[dof].y += val[1].y; }
+        } else {
+            int lo = i1, hi = i2 - 1, pos = -1, key = dof + 1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                if (icsc[mid] == key) { pos = mid; break; }
+                if (icsc[mid] < key) lo = mid + 1; else hi = mid - 1;
+            }
+            if (pos < 0) { *err = 3; continue; }
+            const float2 za = z0[dof];
+            acc[0] -= val[0].x * za.x + val[0].y * za.y;
+            if (nvec > 1) {
+                const float2 zb = z1[dof];
+                acc[1] -= val[1].x * zb.x + val[1].y * zb.y;
+            }
+        }
+    }
+    if (mode == 0) return;
+    for (int v = 0; v < 2; ++v)
+        for (int o = 32; o > 0; o >>= 1) acc[v] += __shfl_xor(acc[v], o);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][wv] = acc[0]; red[1][wv] = acc[1]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        double s0 = 0.0, s1 = 0.0;
+        for (int w = 0; w < nw; ++w) { s0 += red[0][w]; s1 += red[1][w]; }
+        p0[mygrad[it] - 1] = (float)s0;
+        if (nvec > 1) p1[mygrad[it] - 1] = (float)s1;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Jacobian of one (frequency, source): J(irhs, c) = sum over column c of F
 // of G(ICSC(i), irhs) F(i)  (PCHADJ's product, pchadj.f90:239-254), with
 // G transposed to [dof][rhs] so a wave reads one contiguous row of G per

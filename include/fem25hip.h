@@ -237,6 +237,29 @@ int f25_gn_jp(int nnpgl, int na35, const int *mygrad, int njac, const int *nobs,
 int f25_gn_jhz(int nnpgl, int na35, const int *mygrad, int njac, const int *nobs, const double *alpha,
                const void *jac, const void *jp, double *z);
 
+/* ---- second-order products of QPMFREE (truncated-Newton Hessian) ----------- */
+
+/* CADJ_NU_ALPHA's element loops (qpmfree.f90:622-912) on the attached
+ * inversion structures (f25_inv_setup / f25_inv_attach; NVINV = 1, PP or SS)
+ * and the model: with M_{c,e} = SNGL(ELEM_WTS(e)) V(MYGRAD(c)) dS_e/dm_c, the
+ * DABMKISOB25 matrix (dstiff25.f:1-203) of element e collocated on column
+ * c's anchor,
+ *   rhsbnu = sum_c sum_{e in MCONN} M_{c,e}^H adjlam     (RHSBNU, :868-869)
+ *   rhsalp = sum_c sum_{e in MCONN} M_{c,e} wave         (RHSALP, :870-871)
+ *   p3(MYGRAD(c)) = -Re sum_rows conj(M_{c,last} wave) adjlam   (P3BUF)
+ * before the MUMPS solves and MPI reductions the caller keeps (:914-980).
+ * v REAL*4 (NA35); adjlam, wave, rhsbnu, rhsalp COMPLEX*8 (NDOF); p3 REAL*4
+ * (NA35, zero outside the local columns).  Host arrays; FP64 inside. */
+int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float *v, const void *adjlam,
+                    const void *wave, void *rhsbnu, void *rhsalp, float *p3);
+/* MET225_MOD's element loops (qpmfree.f90:1000-1215): with T1, T2 the last
+ * connected element's SNGL(ELEM_WTS(e)) dS_e/dm_c applied to wave and alpha,
+ *   p1(MYGRAD(c)) = -Re sum_rows conj(T1) nu,  p2 = -Re sum_rows conj(T2) adjlam
+ * (PBUF1, PBUF2 before the reductions).  COMPLEX*8 vectors (NDOF), REAL*4
+ * outputs (NA35). */
+int f25_met225(f25_mesh *m, double freq, double py, const void *adjlam, const void *nu,
+               const void *alpha, const void *wave, float *p1, float *p2);
+
 /* ---- Gauss-Newton element products ---------------------------------------- */
 
 /* F v = dS/dm[v] u for a model perturbation v (NNPG,3) = (dVp, dVs, drho)

@@ -342,6 +342,132 @@ def ref_cfmat(freq, py, wave, nz_fdist):
     return out
 
 
+def qp_nu_alpha(mesh, model, g, inv: Inv, freq, py, v, adjlam, wave, cinvtype=b"PP", pml=None):
+    """CADJ_NU_ALPHA's element loops: (RHSBNU, RHSALP, P3) before the solves."""
+    from fem25_amd.configs import PML
+    s, keep = _asm_struct(mesh, model, g, pml or PML())
+    si, keep2 = _inv_struct(inv)
+    v = np.ascontiguousarray(v, np.float32)
+    adjlam = np.ascontiguousarray(adjlam, np.complex64)
+    wave = np.ascontiguousarray(wave, np.complex64)
+    bnu = np.zeros(g.ndof, np.complex64)
+    alp = np.zeros(g.ndof, np.complex64)
+    p3 = np.zeros(inv.na35, np.float32)
+    if lib().orc_qp_nu_alpha(C.byref(s), C.byref(si), cinvtype, C.c_double(freq), C.c_double(py),
+                             _p(v, fp), _p(adjlam, fp), _p(wave, fp), _p(bnu, fp), _p(alp, fp),
+                             _p(p3, fp)):
+        raise RuntimeError("orc_qp_nu_alpha failed")
+    return bnu, alp, p3
+
+
+def met225(mesh, model, g, inv: Inv, freq, py, adjlam, nu, alpha, wave, cinvtype=b"PP", pml=None):
+    """MET225_MOD's element loops: (P1, P2) before the reductions."""
+    from fem25_amd.configs import PML
+    s, keep = _asm_struct(mesh, model, g, pml or PML())
+    si, keep2 = _inv_struct(inv)
+    vecs = [np.ascontiguousarray(x, np.complex64) for x in (adjlam, nu, alpha, wave)]
+    p1 = np.zeros(inv.na35, np.float32)
+    p2 = np.zeros(inv.na35, np.float32)
+    if lib().orc_met225(C.byref(s), C.byref(si), cinvtype, C.c_double(freq), C.c_double(py),
+                        *[_p(x, fp) for x in vecs], _p(p1, fp), _p(p2, fp)):
+        raise RuntimeError("orc_met225 failed")
+    return p1, p2
+
+
+def _elem_inputs(nord, xlocs, zlocs, dens, ecoeff):
+    xlocs = np.ascontiguousarray(xlocs, np.float64)
+    zlocs = np.ascontiguousarray(zlocs, np.float64)
+    dens = np.ascontiguousarray(dens, np.float64)
+    ecoeff = np.asfortranarray(np.asarray(ecoeff, np.float64).reshape(-1, 2))
+    return xlocs, zlocs, dens, ecoeff
+
+
+def djacab2(nord, inpg, xlocs, zlocs, dens, ecoeff, tables=None):
+    """DJACAB2 on the element with anchors 1..4 (1-based inpg); returns
+    det (nq,), dab (2, nq), dab2 (2, nq), shg (nq, nen, 3), q = ix + nint*iz.
+    tables = (abscissae, shl) replaces the oracle's own DSHL tables."""
+    nint, nen = 2 * nord + 5, (nord + 1) ** 2
+    if tables is None:
+        _, xq, _, shl = dshl(nord)
+    else:
+        xq, shl = (np.ascontiguousarray(t, np.float64) for t in tables)
+    xlocs, zlocs, dens, ecoeff = _elem_inputs(nord, xlocs, zlocs, dens, ecoeff)
+    ieng = np.arange(1, 5, dtype=np.int32)
+    nq = nint * nint
+    det, dab, dab2 = np.zeros(nq), np.zeros(2 * nq), np.zeros(2 * nq)
+    shg = np.zeros(3 * nen * nq)
+    shl = np.ascontiguousarray(shl.ravel())
+    if lib().orc_djacab2(nen, nint, inpg, _p(ieng, ip), _p(xq, dp), _p(xlocs, dp), _p(zlocs, dp),
+                         _p(dens, dp), _p(ecoeff.ravel(order="F"), dp), len(dens), _p(shl, dp),
+                         _p(det, dp), _p(dab, dp), _p(dab2, dp), _p(shg, dp)):
+        raise RuntimeError("orc_djacab2 failed")
+    return det, dab.reshape(2, nq), dab2.reshape(2, nq), shg.reshape(nq, nen, 3)
+
+
+def ref_djacab2(nord, inpg, xlocs, zlocs, dens, ecoeff):
+    """The reference's DJACAB2 (compiled), same layout as djacab2()."""
+    nint, nen = 2 * nord + 5, (nord + 1) ** 2
+    xg, _, shl = ref_dshl(nord)
+    xlocs, zlocs, dens, ecoeff = _elem_inputs(nord, xlocs, zlocs, dens, ecoeff)
+    ieng = np.arange(1, 5, dtype=np.int32)
+    nq = nint * nint
+    det, dab, dab2 = np.zeros(nq), np.zeros(2 * nq), np.zeros(2 * nq)
+    shg = np.zeros(3 * nen * nq)
+    shl = np.ascontiguousarray(shl.ravel())
+    ierr = C.c_int(0)
+    i = lambda v: C.byref(C.c_int(v))   # noqa: E731
+    ec = np.ascontiguousarray(ecoeff.ravel(order="F"))
+    xg = np.ascontiguousarray(xg)
+    ref().djacab2_(i(nen), i(nint), i(nint), i(len(dens)), i(1), i(nint), i(nint), i(nen), i(4),
+                   i(len(dens)), i(inpg), _p(ieng, ip), _p(xg, dp), _p(xg, dp), _p(xlocs, dp),
+                   _p(zlocs, dp), _p(dens, dp), _p(ec, dp), _p(shl, dp), _p(det, dp), _p(dab, dp),
+                   _p(dab2, dp), _p(shg, dp), C.byref(ierr))
+    if ierr.value:
+        raise RuntimeError("reference DJACAB2 failed")
+    return det, dab.reshape(2, nq), dab2.reshape(2, nq), shg.reshape(nq, nen, 3)
+
+
+def _dab_out(nee):
+    return [np.zeros(nee * nee), np.zeros(nee * nee), np.zeros(nee * nee, np.complex128),
+            np.zeros(nee * nee, np.complex128)]
+
+
+def dabmkisob25(nord, lisp, liss, omega, py, det, dab, shg, reference=False, dab2=None, wq=None):
+    """DABMKISOB25 (or, with dab2, DABM2KISOB25) from DJACAB(2) outputs; the
+    oracle's restatement or, with reference=True, the compiled routine.
+    Returns the NEE x NEE matrices (Fortran order) DMDA, DMDB, DKDA, DKDB
+    (+ the four second-derivative ones with dab2)."""
+    nint, nen = 2 * nord + 5, (nord + 1) ** 2
+    nee = 3 * nen
+    if wq is None:
+        wq = ref_dshl(nord)[1] if reference else dshl(nord)[2]
+    wq = np.ascontiguousarray(wq, np.float64)
+    det = np.ascontiguousarray(det, np.float64)
+    dab = np.ascontiguousarray(dab, np.float64).ravel()
+    shg = np.ascontiguousarray(shg, np.float64).ravel()
+    out = _dab_out(nee) + (_dab_out(nee) if dab2 is not None else [])
+    ptrs = [_p(o, dp) for o in out]
+    if reference:
+        i = lambda v: C.byref(C.c_int(v))   # noqa: E731
+        head = [i(nen), i(nee), i(nint), i(nint), i(nen), i(nint), i(nint), i(int(lisp)), i(int(liss)),
+                C.byref(C.c_double(omega)), C.byref(C.c_double(py)), _p(wq, dp), _p(wq, dp), _p(det, dp),
+                _p(dab, dp)]
+        if dab2 is None:
+            ref().dabmkisob25_(*head, _p(shg, dp), *ptrs)
+        else:
+            d2 = np.ascontiguousarray(dab2, np.float64).ravel()
+            ref().dabm2kisob25_(*head, _p(d2, dp), _p(shg, dp), *ptrs)
+    else:
+        args = [nen, nint, int(lisp), int(liss), C.c_double(omega), C.c_double(py), _p(wq, dp),
+                _p(det, dp), _p(dab, dp)]
+        if dab2 is None:
+            lib().orc_dabmkisob25(*args, _p(shg, dp), *ptrs)
+        else:
+            d2 = np.ascontiguousarray(dab2, np.float64).ravel()
+            lib().orc_dabm2kisob25(*args, _p(d2, dp), _p(shg, dp), *ptrs)
+    return [o.reshape(nee, nee, order="F") for o in out]
+
+
 def adjflam_dist(inv: Inv, fmat, adjlam) -> np.ndarray:
     out = np.zeros(inv.na35, np.float32)
     fmat = np.ascontiguousarray(fmat, np.complex64)

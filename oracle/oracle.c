@@ -38,6 +38,18 @@
  *                    own format; dense Re(J^H J p) by numpy.
  *   orc_grad_exact   central finite differences of the objective (exact for a
  *                    quadratic) on small meshes.
+ *   orc_djacab2,
+ *   orc_dabmkisob25,
+ *   orc_dabm2kisob25 compiled reference DJACAB2/DABMKISOB25/DABM2KISOB25.
+ *   orc_qp_nu_alpha  RHSALP and RHSBNU: summed over every column and element
+ *                    they are S[lambda = l_L, mu = l_M, rho = 0] u and its
+ *                    conjugate transpose on lambda, S assembled by
+ *                    orc_asmble_z (pinned above) from the anchor field
+ *                    l = 2 Vp V (PP) or (-4 Vs V, 2 Vs V) (SS); P3 on anchors
+ *                    with one element, where the same identity fixes the
+ *                    column; elsewhere the column is the last element's, the
+ *                    element choice CFMAT_DIST makes (compiled reference).
+ *   orc_met225       as P3 (qpmfree.f90 needs MUMPS headers: not compiled).
  */
 #include "oracle.h"
 #include <math.h>
@@ -1195,6 +1207,141 @@ void orc_dkm_vpvs(int nen, int nint, double omega, double py, const double *wts,
         }
 }
 
+/* djac.f:143-288 DJACAB2: DJACAB's Jacobian, SHG and DAB plus the collocated
+ * second-derivative field DAB2(q,1) = DAB2(q,2) = 2 N_INPG(q). */
+int orc_djacab2(int nen, int nint, int inpg, const int *ieng_e, const double *xigll,
+                const double *xlocs, const double *zlocs, const double *dens,
+                const double *ecoeff, int nnpg, const double *shl,
+                double *det, double *dab, double *dab2, double *shg)
+{
+    int nq = nint * nint;
+    for (int ix = 0; ix < nint; ix++)
+        for (int iz = 0; iz < nint; iz++) {
+            double sf[4], rjac, a, b, c, d;
+            int err = jacobian_at(xigll[ix], xigll[iz], ieng_e, xlocs, zlocs, sf, &rjac, &a, &b, &c, &d);
+            int q = ix + nint * iz;
+            double d1 = 0, d2 = 0, e1 = 0, e2 = 0;
+            for (int k = 0; k < 4; k++) {
+                int g = ieng_e[k];
+                double al = 0, be = 0, da2 = 0, db2 = 0;
+                if (g == inpg) {
+                    da2 = 2.0; db2 = 2.0;
+                    al = sqrt((ecoeff[g - 1] + 2.0 * ecoeff[g - 1 + nnpg]) / dens[g - 1]);
+                    be = sqrt(ecoeff[g - 1 + nnpg] / dens[g - 1]);
+                }
+                d1 += 2.0 * al * sf[k];
+                d2 += 2.0 * be * sf[k];
+                e1 += da2 * sf[k];
+                e2 += db2 * sf[k];
+            }
+            dab[q] = d1; dab[q + nq] = d2;
+            dab2[q] = e1; dab2[q + nq] = e2;
+            det[q] = rjac;
+            if (err) return 1;
+            fill_shg(nen, shl + 3 * nen * q, shg + 3 * nen * q, rjac, a, b, c, d);
+        }
+    return 0;
+}
+
+/* dstiff25.f:1-203 DABMKISOB25: derivatives of the density-normalised element
+ * matrices with respect to alpha (Vp, lisp) and beta (Vs, liss) collocated on
+ * one anchor through DAB; the ky coupling terms are imaginary.  Outputs are
+ * NEE x NEE column-major; the explicit zero stores of the reference
+ * (DMDA x/z diagonal, DMDB y diagonal, DMDB z = DMDB x) are kept. */
+void orc_dabmkisob25(int nen, int nint, int lisp, int liss, double omega, double py,
+                     const double *wts, const double *det, const double *dab, const double *shg,
+                     double *dmda, double *dmdb, double complex *dkda, double complex *dkdb)
+{
+    int nee = NDIM * nen, nq = nint * nint;
+    for (int i = 0; i < nee * nee; i++) { dmda[i] = 0.0; dmdb[i] = 0.0; dkda[i] = 0.0; dkdb[i] = 0.0; }
+    double wpy = omega * py, wpy2 = (omega * py) * (omega * py);
+    for (int ix = 0; ix < nint; ix++)
+        for (int iz = 0; iz < nint; iz++) {
+            int q = ix + nint * iz;
+            double weight = wts[ix] * wts[iz] * det[q];
+            double dalpha = dab[q] * weight, dbeta = dab[q + nq] * weight;
+            double dmassda = wpy2 * dalpha, dmassdb = wpy2 * dbeta;
+            const double *sq = shg + 3 * nen * q;
+            for (int ib = 0; ib < nen; ib++) {
+                double phib = sq[3 * ib + 2], phibx = sq[3 * ib], phibz = sq[3 * ib + 1];
+                double dmdat1 = 0, dk11a = 0, dk13a = 0, dk31a = 0, dk33a = 0;
+                double dk12a = 0, dk21a = 0, dk23a = 0, dk32a = 0;
+                double dmdbt1 = 0, b11 = 0, b13a = 0, b13b = 0, b22a = 0, b22b = 0, b31a = 0, b31b = 0, b33 = 0;
+                double b12a = 0, b12b = 0, b21a = 0, b21b = 0, b23a = 0, b23b = 0, b32a = 0, b32b = 0;
+                if (lisp) {
+                    dmdat1 = dmassda * phib;
+                    dk11a = dalpha * phibx; dk13a = dalpha * phibz;
+                    dk31a = dalpha * phibx; dk33a = dalpha * phibz;
+                    dk12a = wpy * dalpha * phib; dk21a = wpy * dalpha * phibx;
+                    dk23a = wpy * dalpha * phibz; dk32a = wpy * dalpha * phib;
+                }
+                if (liss) {
+                    dmdbt1 = dmassdb * phib;
+                    b11 = dbeta * phibz;
+                    b13a = dbeta * phibx; b13b = -2.0 * dbeta * phibz;
+                    b22a = dbeta * phibx; b22b = dbeta * phibz;
+                    b31a = dbeta * phibz; b31b = -2.0 * dbeta * phibx;
+                    b33 = dbeta * phibx;
+                    b12a = 2.0 * wpy * dbeta * phib; b12b = wpy * dbeta * phibx;
+                    b21a = 2.0 * wpy * dbeta * phibx; b21b = wpy * dbeta * phib;
+                    b23a = 2.0 * wpy * dbeta * phibz; b23b = wpy * dbeta * phib;
+                    b32a = 2.0 * wpy * dbeta * phib; b32b = wpy * dbeta * phibz;
+                }
+                for (int ia = 0; ia < nen; ia++) {
+                    double phia = sq[3 * ia + 2], phiax = sq[3 * ia], phiaz = sq[3 * ia + 1];
+                    int ip = NDIM * ia, iq = NDIM * ib;
+#define AT(i, j) ((ip + (i)) + nee * (iq + (j)))
+                    if (lisp) {
+                        dmda[AT(0, 0)] = 0.0;
+                        dmda[AT(1, 1)] = dmda[AT(1, 1)] + dmdat1 * phia;
+                        dmda[AT(2, 2)] = 0.0;
+                    }
+                    if (liss) {
+                        dmdb[AT(0, 0)] = dmdb[AT(0, 0)] + dmdbt1 * phia;
+                        dmdb[AT(1, 1)] = 0.0;
+                        dmdb[AT(2, 2)] = dmdb[AT(0, 0)];
+                    }
+                    if (lisp) {
+                        dkda[AT(0, 0)] += phiax * dk11a;
+                        dkda[AT(0, 1)] -= I * (phiax * dk12a);
+                        dkda[AT(0, 2)] += phiax * dk13a;
+                        dkda[AT(1, 0)] += I * (phia * dk21a);
+                        dkda[AT(1, 2)] += I * (phia * dk23a);
+                        dkda[AT(2, 0)] += phiaz * dk31a;
+                        dkda[AT(2, 1)] -= I * (phiaz * dk32a);
+                        dkda[AT(2, 2)] += phiaz * dk33a;
+                    }
+                    if (liss) {
+                        dkdb[AT(0, 0)] += phiaz * b11;
+                        dkdb[AT(0, 1)] += I * (phiax * b12a + phia * b12b);
+                        dkdb[AT(0, 2)] += phiaz * b13a + phiax * b13b;
+                        dkdb[AT(1, 0)] -= I * (phia * b21a + phiax * b21b);
+                        dkdb[AT(1, 1)] += phiax * b22a + phiaz * b22b;
+                        dkdb[AT(1, 2)] -= I * (phia * b23a + phiaz * b23b);
+                        dkdb[AT(2, 0)] += phiax * b31a + phiaz * b31b;
+                        dkdb[AT(2, 1)] += I * (phiaz * b32a + phia * b32b);
+                        dkdb[AT(2, 2)] += phiax * b33;
+                    }
+#undef AT
+                }
+            }
+        }
+}
+
+/* dstiff25.f:208-494 DABM2KISOB25: the first-derivative matrices of
+ * DABMKISOB25 and the same expressions on DAB2 (second derivatives).  Each
+ * output is computed by the statements DABMKISOB25 uses for it, so this is
+ * DABMKISOB25 applied to DAB and to DAB2. */
+void orc_dabm2kisob25(int nen, int nint, int lisp, int liss, double omega, double py,
+                      const double *wts, const double *det, const double *dab, const double *dab2,
+                      const double *shg, double *dmda, double *dmdb, double complex *dkda,
+                      double complex *dkdb, double *dmda2, double *dmdb2, double complex *dkda2,
+                      double complex *dkdb2)
+{
+    orc_dabmkisob25(nen, nint, lisp, liss, omega, py, wts, det, dab, shg, dmda, dmdb, dkda, dkdb);
+    orc_dabmkisob25(nen, nint, lisp, liss, omega, py, wts, det, dab2, shg, dmda2, dmdb2, dkda2, dkdb2);
+}
+
 /* ------------------------------------------------------------------------ */
 static int cnnpg_is(const char *cnnpg, int g, const char *s)
 {
@@ -1423,6 +1570,175 @@ void orc_adjflam_dist(int nnpgl, int na35, const int *mygrad, const int *jcsc, c
             gprod[inpinv - 1] -= fr * lr + fi * li;
         }
     }
+}
+
+/* Anchor and inversion-variable index of local column inpgl (1-based), as
+ * qpmfree.f90:781-789 and :1101-1109 find them (NVINV = 1). */
+static int qp_anchor(const orc_asm *a, const orc_inv *inv, int inpgl, int *inpinv)
+{
+    for (int g = 0; g < a->nnpg; g++)
+        if ((inv->maskg[g] - 1) * inv->nvinv + 1 == inv->mygrad[inpgl - 1]) {
+            *inpinv = inv->maskg[g];
+            return g + 1;
+        }
+    return -1;
+}
+
+static int qp_kind(const char *cinvtype, int *lisp, int *liss)
+{
+    char a0 = cinvtype[0] & ~0x20, a1 = cinvtype[1] & ~0x20;
+    *lisp = a0 == 'P' && a1 == 'P';
+    *liss = a0 == 'S' && a1 == 'S';
+    return (*lisp || *liss) ? 0 : 1;
+}
+
+/* qpmfree.f90:622-912 CADJ_NU_ALPHA, its element loops: for every local
+ * column (anchor INPG, variable IA35) and every element of MCONN(INPG,:),
+ * with CWGHT = SNGL(ELEM_WTS(e)) * V(IA35) and CARG = CMPLX(DM + DK) from
+ * DJACAB2 + DABM2KISOB25 (first derivatives; the second-derivative matrices
+ * the reference also forms are not used by any output):
+ *   RHSBNU(JDOF) += ADJLAM(IDOF) * CONJG(CWGHT*CARG)
+ *   RHSALP(IDOF) += CWGHT*CARG*WAVE(JDOF)
+ *   TEMP(row)    += CWGHT*CARG*WAVE(JDOF)   (cleared per element, :810-813)
+ * and after the element loop P3(IA35) -= REAL(CONJG(TEMP(row)) ADJLAM(row))
+ * over the column's rows.  The MUMPS solves and MPI reductions that follow
+ * (:914-980) are the caller's.  COMPLEX*8 arithmetic in the reference's
+ * order; NVINV = 1 (PP or SS). */
+int orc_qp_nu_alpha(const orc_asm *a, const orc_inv *inv, const char *cinvtype, double freq,
+                    double py, const float *v, const float complex *adjlam,
+                    const float complex *wave, float complex *rhsbnu, float complex *rhsalp,
+                    float *p3)
+{
+    const double twopi = 6.2831853071795862;
+    int lisp, liss;
+    if (qp_kind(cinvtype, &lisp, &liss) || inv->nvinv != 1) return 1;
+    double omega = twopi * freq;
+    elem_ws w;
+    ws_init(&w, a->nord);
+    int q = w.nint * w.nint, nee = w.nee, nen = w.nen;
+    double *dab = (double *)malloc(sizeof(double) * 2 * q), *dab2 = (double *)malloc(sizeof(double) * 2 * q);
+    double *dmdb = (double *)malloc(sizeof(double) * nee * nee);
+    double complex *dkdb = (double complex *)malloc(sizeof(double complex) * nee * nee);
+    float complex *temp = (float complex *)malloc(sizeof(float complex) * (inv->mbufrhs + 1));
+    for (int i = 0; i < a->ndof; i++) { rhsbnu[i] = 0.0f; rhsalp[i] = 0.0f; }
+    for (int i = 0; i < inv->na35; i++) p3[i] = 0.0f;
+    int ierr = 0;
+    for (int inpgl = 1; inpgl <= inv->nnpgl && !ierr; inpgl += inv->nvinv) {
+        int inpinv, inpg = qp_anchor(a, inv, inpgl, &inpinv);
+        if (inpg < 0) { ierr = 1; break; }
+        int ia35 = (inpinv - 1) * inv->nvinv + 1;
+        int i1 = inv->jcsc[inpgl - 1], i2 = inv->jcsc[inpgl] - 1, nnz = i2 - i1 + 1;
+        for (int r = 0; r < nnz; r++) temp[r] = 0.0f;
+        for (int ic = 0; ic < inv->ncon && !ierr; ic++) {
+            int e = inv->mconn[(inpg - 1) + (size_t)a->nnpg * ic];
+            if (e <= 0) break;
+            e--;
+            if (orc_djacab2(nen, w.nint, inpg, a->ieng + 4 * e, w.xigll, a->xlocs, a->zlocs, a->dens,
+                            a->ecoeff, a->nnpg, w.shl, w.det, dab, dab2, w.shg)) { ierr = 1; break; }
+            orc_dabmkisob25(nen, w.nint, lisp, liss, omega, py, w.wgll, w.det, dab, w.shg,
+                            w.me, dmdb, w.ke, dkdb);
+            const double *dm = lisp ? w.me : dmdb;
+            const double complex *dk = lisp ? w.ke : dkdb;
+            for (int r = 0; r < nnz; r++) temp[r] = 0.0f;
+            float wv = (float)inv->elem_wts[e] * v[ia35 - 1];
+            float complex cw = wv + I * 0.0f;
+            const int *lm = a->lm + NDIM * nen * e;
+            for (int ip = 0; ip < nee && !ierr; ip++) {
+                int idof = lm[ip];
+                if (idof <= 0) continue;
+                int inz = ibsect(nnz, idof, inv->icsc + i1 - 1);
+                if (inz <= 0) { ierr = 1; break; }
+                for (int iq = 0; iq < nee; iq++) {
+                    int jdof = lm[iq];
+                    if (jdof <= 0) continue;
+                    double complex s = dm[ip + nee * iq] + dk[ip + nee * iq];
+                    float complex carg = (float)creal(s) + I * (float)cimag(s);
+                    float complex wc = cmulf_ref(cw, carg);
+                    rhsbnu[jdof - 1] += cmulf_ref(adjlam[idof - 1], conjf(wc));
+                    rhsalp[idof - 1] += cmulf_ref(wc, wave[jdof - 1]);
+                    temp[inz - 1] += cmulf_ref(wc, wave[jdof - 1]);
+                }
+            }
+        }
+        for (int r = 0; r < nnz && !ierr; r++) {
+            int idof = inv->icsc[i1 - 1 + r];
+            p3[ia35 - 1] -= crealf(cmulf_ref(conjf(temp[r]), adjlam[idof - 1]));
+        }
+    }
+    free(dab); free(dab2); free(dmdb); free(dkdb); free(temp);
+    ws_free(&w);
+    return ierr;
+}
+
+/* qpmfree.f90:1000-1215 MET225_MOD, its element loops: per local column the
+ * elements of MCONN(INPG,:) with CWGHT = SNGL(ELEM_WTS(e)) and CARG from
+ * DJACAB + DABMKISOB25,
+ *   T1(row) += CWGHT*CARG*WAVE(JDOF),  T2(row) += CWGHT*CARG*ALPHA(JDOF)
+ * (both cleared per element, :1155-1160, so the last element remains), then
+ * P1(IA35) -= REAL(CONJG(T1) NU), P2(IA35) -= REAL(CONJG(T2) ADJLAM) over
+ * the column's rows; before the MPI reductions of :1222-1226. */
+int orc_met225(const orc_asm *a, const orc_inv *inv, const char *cinvtype, double freq, double py,
+               const float complex *adjlam, const float complex *nu, const float complex *alpha,
+               const float complex *wave, float *p1, float *p2)
+{
+    const double twopi = 6.2831853071795862;
+    int lisp, liss;
+    if (qp_kind(cinvtype, &lisp, &liss) || inv->nvinv != 1) return 1;
+    double omega = twopi * freq;
+    elem_ws w;
+    ws_init(&w, a->nord);
+    int q = w.nint * w.nint, nee = w.nee, nen = w.nen;
+    double *dab = (double *)malloc(sizeof(double) * 2 * q);
+    double *dmdb = (double *)malloc(sizeof(double) * nee * nee);
+    double complex *dkdb = (double complex *)malloc(sizeof(double complex) * nee * nee);
+    float complex *t1 = (float complex *)malloc(sizeof(float complex) * (inv->mbufrhs + 1));
+    float complex *t2 = (float complex *)malloc(sizeof(float complex) * (inv->mbufrhs + 1));
+    for (int i = 0; i < inv->na35; i++) { p1[i] = 0.0f; p2[i] = 0.0f; }
+    int ierr = 0;
+    for (int inpgl = 1; inpgl <= inv->nnpgl && !ierr; inpgl += inv->nvinv) {
+        int inpinv, inpg = qp_anchor(a, inv, inpgl, &inpinv);
+        if (inpg < 0) { ierr = 1; break; }
+        int ia35 = (inpinv - 1) * inv->nvinv + 1;
+        int i1 = inv->jcsc[inpgl - 1], i2 = inv->jcsc[inpgl] - 1, nnz = i2 - i1 + 1;
+        for (int r = 0; r < nnz; r++) { t1[r] = 0.0f; t2[r] = 0.0f; }
+        for (int ic = 0; ic < inv->ncon && !ierr; ic++) {
+            int e = inv->mconn[(inpg - 1) + (size_t)a->nnpg * ic];
+            if (e <= 0) break;
+            e--;
+            float complex cw = (float)inv->elem_wts[e] + I * 0.0f;
+            if (orc_djacab(nen, w.nint, inpg, a->ieng + 4 * e, w.xigll, a->xlocs, a->zlocs, a->dens,
+                           a->ecoeff, a->nnpg, w.shl, w.det, dab, w.shg)) { ierr = 1; break; }
+            orc_dabmkisob25(nen, w.nint, lisp, liss, omega, py, w.wgll, w.det, dab, w.shg,
+                            w.me, dmdb, w.ke, dkdb);
+            const double *dm = lisp ? w.me : dmdb;
+            const double complex *dk = lisp ? w.ke : dkdb;
+            for (int r = 0; r < nnz; r++) { t1[r] = 0.0f; t2[r] = 0.0f; }
+            const int *lm = a->lm + NDIM * nen * e;
+            for (int ip = 0; ip < nee && !ierr; ip++) {
+                int idof = lm[ip];
+                if (idof <= 0) continue;
+                int inz = ibsect(nnz, idof, inv->icsc + i1 - 1);
+                if (inz <= 0) { ierr = 1; break; }
+                for (int iq = 0; iq < nee; iq++) {
+                    int jdof = lm[iq];
+                    if (jdof <= 0) continue;
+                    double complex s = dm[ip + nee * iq] + dk[ip + nee * iq];
+                    float complex carg = (float)creal(s) + I * (float)cimag(s);
+                    float complex wc = cmulf_ref(cw, carg);
+                    t1[inz - 1] += cmulf_ref(wc, wave[jdof - 1]);
+                    t2[inz - 1] += cmulf_ref(wc, alpha[jdof - 1]);
+                }
+            }
+        }
+        for (int r = 0; r < nnz && !ierr; r++) {
+            int idof = inv->icsc[i1 - 1 + r];
+            p1[ia35 - 1] -= crealf(cmulf_ref(conjf(t1[r]), nu[idof - 1]));
+            p2[ia35 - 1] -= crealf(cmulf_ref(conjf(t2[r]), adjlam[idof - 1]));
+        }
+    }
+    free(dab); free(dmdb); free(dkdb); free(t1); free(t2);
+    ws_free(&w);
+    return ierr;
 }
 
 /* Exact derivative of S(w,ky) with respect to anchor lambda/mu/rho.  The

@@ -159,6 +159,25 @@ int orc_cfmat_dist(const orc_asm *a, const orc_inv *inv, const char *cinvtype, d
 /* adjgrad.f90:88-145 ADJFLAM_DIST (single rank) */
 void orc_adjflam_dist(int nnpgl, int na35, const int *mygrad, const int *jcsc, const int *icsc,
                       const float complex *fmat, const float complex *adjlam, float *gprod);
+int orc_djacab2(int nen, int nint, int inpg, const int *ieng_e, const double *xigll,
+                const double *xlocs, const double *zlocs, const double *dens,
+                const double *ecoeff, int nnpg, const double *shl,
+                double *det, double *dab, double *dab2, double *shg);
+void orc_dabmkisob25(int nen, int nint, int lisp, int liss, double omega, double py,
+                     const double *wts, const double *det, const double *dab, const double *shg,
+                     double *dmda, double *dmdb, double complex *dkda, double complex *dkdb);
+void orc_dabm2kisob25(int nen, int nint, int lisp, int liss, double omega, double py,
+                      const double *wts, const double *det, const double *dab, const double *dab2,
+                      const double *shg, double *dmda, double *dmdb, double complex *dkda,
+                      double complex *dkdb, double *dmda2, double *dmdb2, double complex *dkda2,
+                      double complex *dkdb2);
+int orc_qp_nu_alpha(const orc_asm *a, const orc_inv *inv, const char *cinvtype, double freq,
+                    double py, const float *v, const float complex *adjlam,
+                    const float complex *wave, float complex *rhsbnu, float complex *rhsalp,
+                    float *p3);
+int orc_met225(const orc_asm *a, const orc_inv *inv, const char *cinvtype, double freq, double py,
+               const float complex *adjlam, const float complex *nu, const float complex *alpha,
+               const float complex *wave, float *p1, float *p2);
 
 /* pchadj.f90:225-281 (NVINV = 1), one (frequency, source):
  *   JACL(irhs, c) = sum over column c of F of G((irhs-1)*lrhs + ICSC(i)) FMAT(i)

@@ -273,6 +273,36 @@ def test_pchadj_jacobian():
     assert relerr(jac.T, Jd) < 1e-6      # FP64 sums rounded once to COMPLEX*8
 
 
+@pytest.mark.parametrize("name", ["tiny_layered", "pml", "p5"])
+@pytest.mark.parametrize("cinv", [b"PP", b"SS"])
+def test_qpmfree_element_products(name, cinv):
+    """CADJ_NU_ALPHA's right-hand sides and P3, and MET225_MOD's P1 and P2,
+    on the GPU (FP64, rounded once) against the oracle's COMPLEX*8
+    restatement of qpmfree.f90 (norm-wise 2e-5: the reference's single-
+    precision sums are the noise floor)."""
+    mi = _mi(name)
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    mi.inv_setup(cinv)
+    io = oracle.inv_setup(m, mod, g, pml)
+    rng = np.random.default_rng(96)
+    freq, py = 0.07, 1.1e-4
+    cvec = lambda: (rng.standard_normal(g.ndof) + 1j * rng.standard_normal(g.ndof)).astype(np.complex64)  # noqa: E731
+    u, lam, nu, alpha = cvec(), cvec(), cvec(), cvec()
+    v = rng.standard_normal(io.na35).astype(np.float32)
+    print(f"qpmfree {name} {cinv}: ndof={g.ndof} nnpgl={io.nnpgl}", flush=True)
+    bnu, alp, p3 = mi.qp_nu_alpha(freq, py, v, lam, u)
+    rb, ra, r3 = oracle.qp_nu_alpha(m, mod, g, io, freq, py, v, lam, u, cinvtype=cinv, pml=pml)
+    assert relerr(alp, ra) < 2e-5
+    assert relerr(bnu, rb) < 2e-5
+    assert relerr(p3, r3) < 2e-5
+    p1, p2 = mi.met225(freq, py, lam, nu, alpha, u)
+    r1, r2 = oracle.met225(m, mod, g, io, freq, py, lam, nu, alpha, u, cinvtype=cinv, pml=pml)
+    assert relerr(p1, r1) < 2e-5
+    assert relerr(p2, r2) < 2e-5
+    assert np.count_nonzero(p1) == np.count_nonzero(r1) > 0
+
+
 def test_gnmfree_batched_uneven():
     """GNMFREE's sum_j alpha_j Re(J_j^H J_j p) over 12 Jacobians with 20..120
     observations each, against the oracle restatement (FP64 both)."""

@@ -363,3 +363,109 @@ def test_dispersive_without_attenuation_is_density_normalised(case):
         s_el = oracle.asmble_z(m, mod, g, freq, py, pml)
         s_q = oracle.asmble_z(m, md, g, freq, py, pml)
         assert relerr(s_q * mod.dens[0], s_el) < 1e-13
+
+
+# ---- second-order (QPMFREE) element routines ---------------------------------
+_ELEM = dict(xlocs=[0.0, 1000.0, 1130.0, -60.0], zlocs=[0.0, 40.0, 910.0, 1000.0],
+             dens=[2600.0, 2750.0, 2900.0, 2680.0],
+             ecoeff=np.array([[3.1e10, 2.9e10], [3.4e10, 3.0e10], [3.9e10, 3.3e10], [3.3e10, 3.1e10]]))
+
+
+@needs_ref
+@pytest.mark.parametrize("nord", [2, 4])
+def test_djacab2_dabm_vs_reference(nord):
+    """DJACAB2, DABMKISOB25 and DABM2KISOB25 restated: bit-exact against the
+    compiled reference on a distorted element, collocated on each corner,
+    for P, S and joint (PS) derivatives.  Both sides use the compiled DSHL
+    tables (the oracle's own agree with them to 1e-13, pinned above)."""
+    omega, py = 2 * np.pi * 0.07, 1.1e-4
+    xg, wg, shl = oracle.ref_dshl(nord)
+    for inpg in range(1, 5):
+        o = oracle.djacab2(nord, inpg, tables=(xg, shl), **_ELEM)
+        r = oracle.ref_djacab2(nord, inpg, **_ELEM)
+        for a, b in zip(o, r):
+            np.testing.assert_array_equal(a, b)
+        det, dab, dab2, shg = r
+        assert np.count_nonzero(dab[0]) and np.count_nonzero(dab2[1])
+        for lisp, liss in ((1, 0), (0, 1), (1, 1)):
+            mo = oracle.dabmkisob25(nord, lisp, liss, omega, py, det, dab, shg, wq=wg)
+            mr = oracle.dabmkisob25(nord, lisp, liss, omega, py, det, dab, shg, reference=True)
+            for a, b in zip(mo, mr):
+                np.testing.assert_array_equal(a, b)
+            assert np.count_nonzero(mr[0 if lisp else 1]) and np.count_nonzero(mr[2 if lisp else 3])
+            m2o = oracle.dabmkisob25(nord, lisp, liss, omega, py, det, dab, shg, dab2=dab2, wq=wg)
+            m2r = oracle.dabmkisob25(nord, lisp, liss, omega, py, det, dab, shg, reference=True, dab2=dab2)
+            for a, b in zip(m2o, m2r):
+                np.testing.assert_array_equal(a, b)
+
+
+def _perturbation_operator(m, mod, g, pml, fl, fm, freq, py):
+    """The assembled operator of the model (lambda, mu, rho) = (fl, fm, 0) at
+    the anchors: S is linear in the three fields, so this is the sum over
+    elements of the element matrices whose DAB fields are fl, fm."""
+    import dataclasses
+    pert = dataclasses.replace(mod, dens=np.zeros_like(mod.dens),
+                               ecoeff=np.asfortranarray(np.stack([fl, fm], axis=1)))
+    return csr(g, oracle.asmble_z(m, pert, g, freq, py, pml))
+
+
+def _dab_fields(mod, inv, v, kind):
+    """Anchor fields (lambda, mu) of sum_c V_c dS/dm_c in DABMKISOB25's
+    density-normalised form: PP lambda = 2 Vp V; SS lambda = -4 Vs V, mu = 2 Vs V."""
+    lam, mu, rho = mod.ecoeff[:, 0], mod.ecoeff[:, 1], mod.dens
+    vp, vs = np.sqrt((lam + 2 * mu) / rho), np.sqrt(mu / rho)
+    w = np.zeros(mod.dens.size)
+    sel = inv.maskg > 0
+    w[sel] = np.asarray(v, np.float64)[inv.maskg[sel] - 1]
+    if kind == b"PP":
+        return 2 * vp * w, np.zeros_like(w)
+    return -4 * vs * w, 2 * vs * w
+
+
+@pytest.mark.parametrize("kind", [b"PP", b"SS"])
+def test_qp_nu_alpha_against_assembled_operator(kind):
+    """CADJ_NU_ALPHA's right-hand sides restated, against an independent
+    route: summed over every column and connected element, RHSALP is
+    S[lambda = f_L, mu = f_M, rho = 0] u and RHSBNU is that matrix's conjugate
+    transpose on lambda, with S assembled by orc_asmble_z (pinned against the
+    compiled ASMBLE_DRIVER) on a uniform interior mesh (ELEM_WTS = 1).  P3 on
+    anchors that touch one element is -Re((S_a u)^H lambda) with S_a the
+    operator of that anchor's field alone; MET225_MOD's P1, P2 likewise, and
+    P3 = V P1 wherever nu = lambda.  Tolerance: the reference's COMPLEX*8
+    accumulation."""
+    name = "tiny_layered"
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    inv = oracle.inv_setup(m, mod, g, pml)
+    assert np.all(inv.elem_wts == 1.0)
+    rng = np.random.default_rng(95)
+    freq, py = 0.07, 1.1e-4
+    v = rng.standard_normal(inv.na35).astype(np.float32)
+    u = (rng.standard_normal(g.ndof) + 1j * rng.standard_normal(g.ndof)).astype(np.complex64)
+    lam = (rng.standard_normal(g.ndof) + 1j * rng.standard_normal(g.ndof)).astype(np.complex64)
+    bnu, alp, p3 = oracle.qp_nu_alpha(m, mod, g, inv, freq, py, v, lam, u, cinvtype=kind, pml=pml)
+    fl, fm = _dab_fields(mod, inv, v, kind)
+    S = _perturbation_operator(m, mod, g, pml, fl, fm, freq, py)
+    assert relerr(alp, S @ u.astype(np.complex128)) < 1e-5
+    assert relerr(bnu, S.conj().T @ lam.astype(np.complex128)) < 1e-5
+    # P3 / P1 / P2 on anchors with a single element
+    cnt = np.bincount(m.ieng.ravel() - 1, minlength=m.nnpg)
+    corners = np.nonzero((cnt == 1) & (inv.maskg > 0))[0]
+    assert corners.size >= 2
+    nu = (rng.standard_normal(g.ndof) + 1j * rng.standard_normal(g.ndof)).astype(np.complex64)
+    alpha = (rng.standard_normal(g.ndof) + 1j * rng.standard_normal(g.ndof)).astype(np.complex64)
+    p1, p2 = oracle.met225(m, mod, g, inv, freq, py, lam, nu, alpha, u, cinvtype=kind, pml=pml)
+    for a in corners:
+        one = np.zeros(inv.na35, np.float32)
+        one[inv.maskg[a] - 1] = 1.0
+        f1l, f1m = _dab_fields(mod, inv, one, kind)
+        Sa = _perturbation_operator(m, mod, g, pml, f1l, f1m, freq, py)
+        col = inv.maskg[a] - 1
+        for got, x, y, wgt in ((p3[col], u, lam, v[col]), (p1[col], u, nu, 1.0), (p2[col], alpha, lam, 1.0)):
+            t = Sa @ x.astype(np.complex128)
+            terms = np.conj(t) * y
+            want = -wgt * np.real(terms.sum())
+            assert abs(got - want) <= 1e-5 * abs(wgt) * np.abs(terms).sum(), (a, got, want)
+    _, _, p3n = oracle.qp_nu_alpha(m, mod, g, inv, freq, py, v, nu, u, cinvtype=kind, pml=pml)
+    p1n, _ = oracle.met225(m, mod, g, inv, freq, py, lam, nu, alpha, u, cinvtype=kind, pml=pml)
+    assert np.abs(p3n - v * p1n).max() <= 1e-5 * np.abs(p3n).max()
