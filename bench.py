@@ -283,6 +283,23 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     kern["gnmfree"] = {"kernel_ms": round(k_ms, 3), "jacobians": 120, "jacobian_bytes": jt.numel() * 8,
                        "roofline": roof("hbm", jb / (k_ms * 1e-3), "GB/s")}
     del jt, pv, zv
+    # QPMFREE's second-order element products: CADJ_NU_ALPHA (every element
+    # holding local columns, two vectors, + one column per anchor) and
+    # MET225_MOD (one column per anchor, two vectors); FP64 VALU, per
+    # (item, vector) (72 NEN + 40) flops per quadrature point
+    cv = lambda s: (np.random.default_rng(s).standard_normal(ndof)   # noqa: E731
+                    + 1j * np.random.default_rng(s + 1).standard_normal(ndof)).astype(np.complex64)
+    lamv, nuv, alv = cv(11), cv(13), cv(15)
+    vq = rng.standard_normal(inv.na35).astype(np.float32)
+    per_item = (72.0 * nen + 40.0) * nq
+    mi.qp_nu_alpha(float(freq[0]), float(py[0]), vq, lamv, wave)
+    k_ms = L.f25_last_kernel_ms(b"qp")
+    qfl = per_item * (2 * mesh.nelem + inv.nnpgl)
+    kern["qp_nu_alpha"] = {"kernel_ms": round(k_ms, 3), "roofline": roof("fp64", qfl / (k_ms * 1e-3), "TFLOP/s")}
+    mi.met225(float(freq[0]), float(py[0]), lamv, nuv, alv, wave)
+    k_ms = L.f25_last_kernel_ms(b"met225")
+    mfl = per_item * 2 * inv.nnpgl
+    kern["met225"] = {"kernel_ms": round(k_ms, 3), "roofline": roof("fp64", mfl / (k_ms * 1e-3), "TFLOP/s")}
     out["kernels"] = kern
     return out
 

@@ -91,6 +91,20 @@
             INTEGER(C_INT), VALUE :: NNPGL, NA35, NJAC
             TYPE(C_PTR), VALUE :: MYGRAD, NOBS, ALPHA, JAC, JP, Z
          END FUNCTION
+         INTEGER(C_INT) FUNCTION F25_QP_NU_ALPHA(H, FREQ, PY, V, ADJLAM, WAVE, RHSBNU, RHSALP, P3) &
+                        BIND(C, NAME='f25_qp_nu_alpha')
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: H
+            REAL(C_DOUBLE), VALUE :: FREQ, PY
+            TYPE(C_PTR), VALUE :: V, ADJLAM, WAVE, RHSBNU, RHSALP, P3
+         END FUNCTION
+         INTEGER(C_INT) FUNCTION F25_MET225(H, FREQ, PY, ADJLAM, NU, ALPHA, WAVE, P1, P2)          &
+                        BIND(C, NAME='f25_met225')
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: H
+            REAL(C_DOUBLE), VALUE :: FREQ, PY
+            TYPE(C_PTR), VALUE :: ADJLAM, NU, ALPHA, WAVE, P1, P2
+         END FUNCTION
          TYPE(C_PTR) FUNCTION F25_LAST_ERROR() BIND(C, NAME='f25_last_error')
             IMPORT :: C_PTR
          END FUNCTION
@@ -107,6 +121,8 @@
 !     cached inversion structures
       INTEGER(C_INTPTR_T), SAVE :: F25_KEY_JCSC = 0, F25_KEY_ICSC = 0, F25_KEY_MASKG = 0
       INTEGER, SAVE :: F25_KEY_INV(4) = -1
+      CHARACTER(2), SAVE :: F25_KEY_CINV = '  '
+      REAL(C_DOUBLE), SAVE :: F25_KEY_VPVS = -1.D0
 
       CONTAINS
 
@@ -180,6 +196,7 @@
       F25_HMESH = C_NULL_PTR
       F25_KEY_SIZES = -1
       F25_KEY_INV = -1
+      F25_KEY_CINV = '  '
       END SUBROUTINE F25_DROPIN_RESET
 
       END MODULE F25_CABI

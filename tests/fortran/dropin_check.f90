@@ -8,8 +8,9 @@
 !     reference's MESH_INFO and INV_INFO the way its drivers do, and calls
 !     ASMBLE_DRIVER (twice: the second call reuses the cached device mesh),
 !     CFMAT_DIST, ADJFLAM_DIST and ASMB_PEFF, then ASMBLE_DRIVER once more
-!     with a wrong NZLOC, which must fail with IERR /= 0.  Writes the five
-!     status words and the outputs CDIST, FMAT_DIST, GPROD and PEFF.
+!     with a wrong NZLOC, which must fail with IERR /= 0, then MET225_MOD and
+!     F25_CADJ_NU_ALPHA_RHS.  Writes the seven status words and the outputs
+!     CDIST, FMAT_DIST, GPROD, PEFF, P1, P2, RHSBNU, RHSALP and P3.
       PROGRAM DROPIN_CHECK
       IMPLICIT NONE
       INCLUDE 'fwd_struc.h'
@@ -18,11 +19,12 @@
       TYPE (INV_INFO) INV
       CHARACTER(512) :: FIN, FOUT
       INTEGER*4 :: HDR(9), NEN, NDOF, NZ, NZF, NA35, NNPG, NELEM, I, MPIERR
-      INTEGER*4 :: ISTAT(5)
+      INTEGER*4 :: ISTAT(7)
       REAL*8 :: SC(12), VPVS, FREQ, PY
       CHARACTER(1) :: CH(4), CFTYPE
       COMPLEX*8, ALLOCATABLE :: CDIST(:), CDIST2(:), FMAT(:), WAVE(:), ADJLAM(:), UE(:), PEFF(:)
-      REAL*4, ALLOCATABLE :: GPROD(:)
+      COMPLEX*8, ALLOCATABLE :: NU(:), ALPHA(:), RHSBNU(:), RHSALP(:)
+      REAL*4, ALLOCATABLE :: GPROD(:), V(:), P1(:), P2(:), P3(:)
 
       CALL MPI_INIT(MPIERR)
       CALL GET_COMMAND_ARGUMENT(1, FIN)
@@ -110,6 +112,10 @@
       READ(10) WAVE
       READ(10) ADJLAM
       READ(10) UE
+      ALLOCATE(NU(NDOF), ALPHA(NDOF), V(NA35))
+      READ(10) NU
+      READ(10) ALPHA
+      READ(10) V
       CLOSE(10)
 
       ALLOCATE(CDIST(NZ), CDIST2(NZ), FMAT(NZF), GPROD(NA35), PEFF(NDOF))
@@ -123,6 +129,11 @@
       CALL ASMB_PEFF(0, MPI_COMM_WORLD, NDOF, NDOF, NZ, msh%CNP, msh%MYDOFS, msh%IRPTR_LOC,    &
                      msh%JCPTR_LOC, CDIST, UE, PEFF)
       CALL ASMBLE_DRIVER(NZ - 1, CFTYPE, FREQ, PY, MSH, CDIST2, ISTAT(4))
+      ALLOCATE(P1(NA35), P2(NA35), P3(NA35), RHSBNU(NDOF), RHSALP(NDOF))
+      CALL MET225_MOD(0, 0, MPI_COMM_WORLD, NDOF, FREQ, PY, ADJLAM, NU, ALPHA, WAVE, MSH, INV,   &
+                      P1, P2, ISTAT(6))
+      CALL F25_CADJ_NU_ALPHA_RHS(NDOF, NA35, FREQ, PY, V, ADJLAM, WAVE, MSH, INV, RHSBNU, RHSALP, &
+                                 P3, ISTAT(7))
 
       OPEN(11, FILE=TRIM(FOUT), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
       WRITE(11) ISTAT
@@ -130,6 +141,11 @@
       WRITE(11) FMAT
       WRITE(11) GPROD
       WRITE(11) PEFF
+      WRITE(11) P1
+      WRITE(11) P2
+      WRITE(11) RHSBNU
+      WRITE(11) RHSALP
+      WRITE(11) P3
       CLOSE(11)
       CALL MPI_FINALIZE(MPIERR)
       END PROGRAM DROPIN_CHECK

@@ -22,7 +22,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DROPIN = os.path.join(ROOT, "fem25_amd", "fortran", "_build", "libfem25dropin.so")
 CHECK = os.path.join(ROOT, "tests", "fortran", "_build", "dropin_check")
 GNCHECK = os.path.join(ROOT, "tests", "fortran", "_build", "gnmfree_check")
-SYMBOLS = ("asmble_driver_", "asmb_peff_", "cfmat_dist_", "adjflam_dist_", "gnmfree_")
+SYMBOLS = ("asmble_driver_", "asmb_peff_", "cfmat_dist_", "adjflam_dist_", "gnmfree_", "met225_mod_",
+           "f25_cadj_nu_alpha_rhs_")
 
 needs_build = pytest.mark.skipif(not (os.path.exists(DROPIN) and os.path.exists(CHECK)
                                       and os.path.exists(GNCHECK)),
@@ -39,7 +40,7 @@ def test_dropin_exports_reference_names():
         assert s in names, s
 
 
-def _write_input(path, m, mod, pml, g, io, freq, py, wave, lam, ue, cftype=b"B", cinv=b"PP",
+def _write_input(path, m, mod, pml, g, io, freq, py, wave, lam, ue, nu, alpha, v, cftype=b"B", cinv=b"PP",
                  vpvs=3 ** 0.5):
     i4 = lambda a: np.ascontiguousarray(a, dtype="<i4").tobytes()  # noqa: E731
     f8 = lambda a: np.ascontiguousarray(a, dtype="<f8").tobytes()  # noqa: E731
@@ -68,14 +69,15 @@ def _write_input(path, m, mod, pml, g, io, freq, py, wave, lam, ue, cftype=b"B",
         f.write(i4(io.jcsc))
         f.write(i4(io.icsc))
         f.write(f8(io.elem_wts))
-        for v in (wave, lam, ue):
-            f.write(c8(v))
+        for x in (wave, lam, ue, nu, alpha):
+            f.write(c8(x))
+        f.write(np.ascontiguousarray(v, "<f4").tobytes())
 
 
 @needs_build
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["pml", "tiny_layered"])
-def test_dropin_against_oracle(name, tmp_path):
+@pytest.mark.parametrize("name,cinv", [("pml", b"PP"), ("tiny_layered", b"PP"), ("tiny_layered", b"SS")])
+def test_dropin_against_oracle(name, cinv, tmp_path):
     m, mod, pml = mesh_case(name)
     g = oracle_graph(name)
     io = oracle.inv_setup(m, mod, g, pml)
@@ -83,8 +85,11 @@ def test_dropin_against_oracle(name, tmp_path):
     wave = cf.seeded_complex(g.ndof, seed=61).astype(np.complex64)
     lam = cf.seeded_complex(g.ndof, seed=62).astype(np.complex64)
     ue = cf.seeded_complex(g.ndof, seed=63).astype(np.complex64)
+    nu = cf.seeded_complex(g.ndof, seed=64).astype(np.complex64)
+    alpha = cf.seeded_complex(g.ndof, seed=65).astype(np.complex64)
+    v = np.random.default_rng(66).standard_normal(io.na35).astype(np.float32)
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    _write_input(fin, m, mod, pml, g, io, freq, py, wave, lam, ue)
+    _write_input(fin, m, mod, pml, g, io, freq, py, wave, lam, ue, nu, alpha, v, cinv=cinv)
     print(f"dropin_check {name}: ndof={g.ndof} nz={g.nz} nz_fdist={io.nz_fdist}", flush=True)
     r = subprocess.run([CHECK, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -97,20 +102,29 @@ def test_dropin_against_oracle(name, tmp_path):
         off += a.nbytes
         return a
 
-    st = take("<i4", 5)
+    st = take("<i4", 7)
     cdist = take("<c8", g.nz)
     fmat = take("<c8", io.nz_fdist)
     gprod = take("<f4", io.na35)
     peff = take("<c8", g.ndof)
+    p1 = take("<f4", io.na35)
+    p2 = take("<f4", io.na35)
+    rhsbnu = take("<c8", g.ndof)
+    rhsalp = take("<c8", g.ndof)
+    p3 = take("<f4", io.na35)
     assert off == len(raw)
-    assert st[0] == 0 and st[1] == 0 and st[2] == 0, st
+    assert st[0] == 0 and st[1] == 0 and st[2] == 0 and st[5] == 0 and st[6] == 0, st
     assert st[3] != 0, "a wrong NZLOC must be rejected"
     assert st[4] == 0, "the cached second call must reproduce the first"
+    r1, r2 = oracle.met225(m, mod, g, io, freq, py, lam, nu, alpha, wave, cinvtype=cinv, pml=pml)
+    assert relerr(p1, r1) < 2e-5 and relerr(p2, r2) < 2e-5
+    rb, ra, r3 = oracle.qp_nu_alpha(m, mod, g, io, freq, py, v, lam, wave, cinvtype=cinv, pml=pml)
+    assert relerr(rhsbnu, rb) < 2e-5 and relerr(rhsalp, ra) < 2e-5 and relerr(p3, r3) < 2e-5
     ref = oracle.asmble_cdist(m, mod, g, freq, py, pml)
     rowmax = np.maximum.reduceat(np.abs(ref.astype(np.complex128)), g.irptr[:-1] - 1)
     rows = np.repeat(np.arange(g.ndof), np.diff(g.irptr))
     assert np.all(np.abs(cdist.astype(np.complex128) - ref) <= 1e-6 * rowmax[rows])
-    fref = oracle.cfmat_dist(m, mod, g, io, freq, py, wave, cinvtype=b"PP")
+    fref = oracle.cfmat_dist(m, mod, g, io, freq, py, wave, cinvtype=cinv)
     assert relerr(fmat, fref) < 2e-6
     assert relerr(gprod, oracle.adjflam_dist(io, fmat, lam)) < 1e-5
     pref = oracle.asmb_peff(g, cdist, ue)
