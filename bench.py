@@ -300,6 +300,30 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     k_ms = L.f25_last_kernel_ms(b"met225")
     mfl = per_item * 2 * inv.nnpgl
     kern["met225"] = {"kernel_ms": round(k_ms, 3), "roofline": roof("fp64", mfl / (k_ms * 1e-3), "TFLOP/s")}
+    # adjoint right-hand sides of 10 frequencies x 12 sources, 60 receivers:
+    # one dense RHS of NDOF COMPLEX*8 per pair (HBM: the zeroed RHS writes)
+    nf, ns, nr = 10, 12, 60
+    ar = np.random.default_rng(9)
+    mr = ar.integers(1, ndof + 1, size=(nr, 3)).astype(np.int32)
+    yr = ar.uniform(-4e4, 4e4, nr)
+    ee = torch.randn(ns * nr * nf * 3, dtype=torch.complex64, device="cuda")
+    oo = torch.randn(ns * nr * nf * 3, dtype=torch.complex64, device="cuda")
+    ww = torch.rand(ns * nr * nf * 3, dtype=torch.float32, device="cuda")
+    rr = torch.empty(nf * ns * ndof, dtype=torch.complex64, device="cuda")
+    ifq = np.repeat(np.arange(1, nf + 1), ns).astype(np.int32)
+    isr = np.tile(np.arange(1, ns + 1), nf).astype(np.int32)
+    fq = 0.02 + 0.008 * ifq.astype(np.float64)
+    pyy = np.full(nf * ns, 0.5e-4)
+    iw = C.c_int(0)
+    args = (ndof, nf, nr, ns, nf * ns, ifq.ctypes.data_as(_lib.ip), isr.ctypes.data_as(_lib.ip),
+            fq.ctypes.data_as(_lib.dp), pyy.ctypes.data_as(_lib.dp), 20.0, 2, 3, 0.05, 0.0, 1, 0,
+            yr.ctypes.data_as(_lib.dp), mr.ctypes.data_as(_lib.ip), C.c_void_p(ee.data_ptr()),
+            C.c_void_p(oo.data_ptr()), C.cast(C.c_void_p(ww.data_ptr()), _lib.fp), C.c_void_p(rr.data_ptr()),
+            C.byref(iw))
+    k_ms, wall = timed(lambda: _lib.check(L.f25_cadjlam_rhs_dev(*args), "cadjlam"), L, b"cadjlam", 3)
+    kern["cadjlam_rhs"] = {"kernel_ms": round(k_ms, 3), "pairs": nf * ns,
+                           "roofline": roof("hbm", nf * ns * ndof * 8 / (k_ms * 1e-3), "GB/s")}
+    del ee, oo, ww, rr
     out["kernels"] = kern
     return out
 

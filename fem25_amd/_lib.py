@@ -68,6 +68,16 @@ EXPORTS = {
                                   C.c_void_p, fp]),
     "f25_met225": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                              fp, fp]),
+    "f25_exresp25d": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, dp, dp, C.c_void_p, ip, C.c_void_p,
+                                C.c_void_p]),
+    "f25_exresp25d_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, dp, dp, C.c_void_p, ip, C.c_void_p,
+                                    C.c_void_p]),
+    "f25_cadjlam_rhs": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, dp, dp, C.c_double, C.c_int,
+                                  C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, dp, ip, C.c_void_p, C.c_void_p,
+                                  fp, C.c_void_p, ip]),
+    "f25_cadjlam_rhs_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, dp, dp, C.c_double,
+                                      C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, dp, ip, C.c_void_p,
+                                      C.c_void_p, fp, C.c_void_p, ip]),
     "f25_gn_fv": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp, C.c_void_p]),
     "f25_gn_fhw": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, dp]),
     "f25_gn_diag": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp]),

@@ -65,6 +65,12 @@ static void fill_async(T *p, size_t n, T v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(256), 0, s, p, n, v);
 }
 
+// a stream owned by one handle-free call
+struct TmpStream {
+    hipStream_t s = nullptr;
+    ~TmpStream() { if (s) (void)hipStreamDestroy(s); }
+};
+
 struct f25_mesh_s {
     // host copies of the description
     f25_mesh_desc d{};
@@ -1051,6 +1057,154 @@ extern "C" int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *w
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint-source pipeline (handle-free: the receiver tables are the
+// caller's).  Run on the current device.
+static int check_mrdof(int nrec, int ndof, const int *mrdof, bool allow_out) {
+    if (!mrdof) FAIL(F25_ERR_ARG, "null MRDOF");
+    for (int k = 0; k < 3 * nrec; ++k)
+        if (mrdof[k] < 0 || (!allow_out && mrdof[k] > ndof)) FAIL(F25_ERR_ARG, "MRDOF out of range");
+    return 0;
+}
+
+static int exresp_core(int ndof, int nsrc, int nrec, double freq, const double *py, const double *yrec,
+                       const float2 *sol_dev, const int *mrdof, const void *recv, float2 *est_dev, hipStream_t s) {
+    if (ndof <= 0 || nsrc <= 0 || nrec <= 0 || !py || !yrec || !recv) FAIL(F25_ERR_ARG, "invalid arguments");
+    TRY(check_mrdof(nrec, ndof, mrdof, false));
+    double *dpy = nullptr, *dy = nullptr;
+    int *dm = nullptr;
+    float2 *dr = nullptr;
+    int r = upload(&dpy, py, (size_t)nsrc, s);
+    if (!r) r = upload(&dy, yrec, (size_t)nrec, s);
+    if (!r) r = upload(&dm, mrdof, 3 * (size_t)nrec, s);
+    if (!r) r = upload(&dr, (const float2 *)recv, 3 * (size_t)nrec, s);
+    if (!r) {
+        const int n = nsrc * nrec;
+        hipLaunchKernelGGL(k_exresp, dim3((n + 255) / 256), dim3(256), 0, s, ndof, nsrc, nrec,
+                           6.2831853071795862 * freq, dpy, dy, sol_dev, dm, dr, est_dev);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(dpy); dfree(dy); dfree(dm); dfree(dr);
+    return r;
+}
+
+extern "C" int f25_exresp25d_dev(int ndof, int nsrc, int nrec, double freq, const double *py, const double *yrec,
+                                 const void *sol_dev, const int *mrdof, const void *recv, void *est_dev) {
+    if (!sol_dev || !est_dev) FAIL(F25_ERR_ARG, "null argument");
+    TmpStream ts;
+    HIPCHK(hipStreamCreate(&ts.s));
+    return exresp_core(ndof, nsrc, nrec, freq, py, yrec, (const float2 *)sol_dev, mrdof, recv, (float2 *)est_dev,
+                       ts.s);
+}
+
+extern "C" int f25_exresp25d(int ndof, int nsrc, int nrec, double freq, const double *py, const double *yrec,
+                             const void *sol, const int *mrdof, const void *recv, void *est) {
+    if (!sol || !est || ndof <= 0 || nsrc <= 0 || nrec <= 0) FAIL(F25_ERR_ARG, "invalid arguments");
+    TmpStream ts;
+    HIPCHK(hipStreamCreate(&ts.s));
+    float2 *ds = nullptr, *de = nullptr;
+    const size_t ne = 3 * (size_t)nrec * nsrc;
+    int r = upload(&ds, (const float2 *)sol, (size_t)ndof * nsrc, ts.s);
+    if (!r) r = dalloc(&de, ne);
+    if (!r) r = exresp_core(ndof, nsrc, nrec, freq, py, yrec, ds, mrdof, recv, de, ts.s);
+    if (!r) {
+        hipError_t e = hipMemcpy(est, de, ne * sizeof(float2), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(ds); dfree(de);
+    return r;
+}
+
+static int cadjlam_core(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq, const int *isrc,
+                        const double *freq, const double *py, double azmod, int norm, int irestp, float epss,
+                        float dtmax, int ibphase, int lunwrap, const double *yrec, const int *mrdof,
+                        const float2 *est_dev, const float2 *obs_dev, const float *w_dev, float2 *rhs_dev,
+                        int *iwarn, hipStream_t s) {
+    if (ndof <= 0 || nfreq <= 0 || nrec <= 0 || nsrc <= 0 || npair <= 0 || npair > 65535 || !ifreq || !isrc ||
+        !freq || !py || !yrec || !iwarn)
+        FAIL(F25_ERR_ARG, "invalid arguments");
+    TRY(check_mrdof(nrec, ndof, mrdof, true));
+    for (int p = 0; p < npair; ++p)
+        if (ifreq[p] < 1 || ifreq[p] > nfreq || isrc[p] < 1 || isrc[p] > nsrc)
+            FAIL(F25_ERR_ARG, "IFREQ or ISRC out of range");
+    AdjParams prm{(float)azmod, epss, dtmax, norm, irestp, ibphase, lunwrap ? 1 : 0};
+    int *df = nullptr, *dsr = nullptr, *dm = nullptr, *dw = nullptr;
+    double *dfr = nullptr, *dpy = nullptr, *dy = nullptr;
+    float2 *dv = nullptr;
+    int r = upload(&df, ifreq, (size_t)npair, s);
+    if (!r) r = upload(&dsr, isrc, (size_t)npair, s);
+    if (!r) r = upload(&dfr, freq, (size_t)npair, s);
+    if (!r) r = upload(&dpy, py, (size_t)npair, s);
+    if (!r) r = upload(&dy, yrec, (size_t)nrec, s);
+    if (!r) r = upload(&dm, mrdof, 3 * (size_t)nrec, s);
+    if (!r) r = dalloc(&dv, 3 * (size_t)nrec * npair);
+    if (!r) r = dalloc(&dw, 1);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!r && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
+        g_err = "hipEventCreate failed";
+        r = F25_ERR_HIP;
+    }
+    if (!r) {
+        fill_async(dw, (size_t)1, 0, s);
+        (void)hipEventRecord(e0, s);
+        fill_async(rhs_dev, (size_t)ndof * npair, make_float2(0.f, 0.f), s);
+        hipLaunchKernelGGL(k_cadjlam, dim3(npair), dim3(256), 0, s, ndof, nfreq, nrec, df, dsr, dfr, dpy, prm, dy,
+                           dm, est_dev, obs_dev, w_dev, dv, rhs_dev, dw);
+        (void)hipEventRecord(e1, s);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(iwarn, dw, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+        float ms = 0.f;
+        if (!r && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) record_ms("cadjlam", ms);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(df); dfree(dsr); dfree(dfr); dfree(dpy); dfree(dy); dfree(dm); dfree(dv); dfree(dw);
+    return r;
+}
+
+extern "C" int f25_cadjlam_rhs_dev(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq,
+                                   const int *isrc, const double *freq, const double *py, double azmod, int norm,
+                                   int irestp, float epss, float dtmax, int ibphase, int lunwrap,
+                                   const double *yrec, const int *mrdof, const void *est_dev, const void *obs_dev,
+                                   const float *wghts_dev, void *rhs_dev, int *iwarn) {
+    if (!est_dev || !obs_dev || !wghts_dev || !rhs_dev) FAIL(F25_ERR_ARG, "null argument");
+    TmpStream ts;
+    HIPCHK(hipStreamCreate(&ts.s));
+    return cadjlam_core(ndof, nfreq, nrec, nsrc, npair, ifreq, isrc, freq, py, azmod, norm, irestp, epss, dtmax,
+                        ibphase, lunwrap, yrec, mrdof, (const float2 *)est_dev, (const float2 *)obs_dev, wghts_dev,
+                        (float2 *)rhs_dev, iwarn, ts.s);
+}
+
+extern "C" int f25_cadjlam_rhs(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq,
+                               const int *isrc, const double *freq, const double *py, double azmod, int norm,
+                               int irestp, float epss, float dtmax, int ibphase, int lunwrap, const double *yrec,
+                               const int *mrdof, const void *est, const void *obs, const float *wghts, void *rhs,
+                               int *iwarn) {
+    if (!est || !obs || !wghts || !rhs || ndof <= 0 || npair <= 0 || nfreq <= 0 || nrec <= 0 || nsrc <= 0)
+        FAIL(F25_ERR_ARG, "invalid arguments");
+    TmpStream ts;
+    HIPCHK(hipStreamCreate(&ts.s));
+    const size_t nd = 3 * (size_t)nfreq * nrec * nsrc;
+    float2 *de = nullptr, *dob = nullptr, *dr = nullptr;
+    float *dw = nullptr;
+    int r = upload(&de, (const float2 *)est, nd, ts.s);
+    if (!r) r = upload(&dob, (const float2 *)obs, nd, ts.s);
+    if (!r) r = upload(&dw, wghts, nd, ts.s);
+    if (!r) r = dalloc(&dr, (size_t)ndof * npair);
+    if (!r) r = cadjlam_core(ndof, nfreq, nrec, nsrc, npair, ifreq, isrc, freq, py, azmod, norm, irestp, epss,
+                             dtmax, ibphase, lunwrap, yrec, mrdof, de, dob, dw, dr, iwarn, ts.s);
+    if (!r) {
+        hipError_t e = hipMemcpy(rhs, dr, (size_t)ndof * npair * sizeof(float2), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(de); dfree(dob); dfree(dw); dfree(dr);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
 // QPMFREE's element products (CADJ_NU_ALPHA, MET225_MOD) on DABMKISOB25
 // matrices.  2 Vp (PP) or 2 Vs (SS) at each column's anchor: DJACAB/DJACAB2's
 // collocation constant (djac.f:94-103, 240-250).
@@ -1305,11 +1459,6 @@ extern "C" int f25_adjflam_dist(f25_mesh *m, const void *fmat, const void *adjla
 // Handle-free forms for callers that hold only the sparse structures
 // (ASMB_PEFF and ADJFLAM_DIST receive no mesh).  They run on the current
 // device, on a stream of their own, and allocate per call.
-struct TmpStream {
-    hipStream_t s = nullptr;
-    ~TmpStream() { if (s) (void)hipStreamDestroy(s); }
-};
-
 extern "C" int f25_asmb_peff_csr(int ndof, int ndofl, int nzloc, const char *cnp, const int *mydofs,
                                  const int *irptr_loc, const int *jcptr_loc, const void *cdist,
                                  const void *ue, void *buff) {

@@ -1563,6 +1563,177 @@ __global__ void k_gn_z(int nnpgl, int njac, const int *__restrict__ nobs, const 
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint-source pipeline (the vector work between a forward solve and the
+// adjoint solve): receiver extraction (EXRESP25D, locrec2d.f:867-925) and
+// CADJLAM's right-hand side (adjgrad.f90:229-300: rotation to (N,E,Z),
+// receiver phase shift, BPRESID4 residuals, rotation back, RESID_DRHS).
+// COMPLEX*8 arithmetic in the reference's forms: products and quotients by
+// the textbook formulas.  |z|, sin, cos and atan2 of single-precision
+// arguments are evaluated in double and rounded once, which reproduces the
+// host library's single-precision results (rounded to within half an ulp)
+// far more often than the device's own single-precision routines: the L1
+// residual dif/|dif| magnifies any difference by 1/|dif|.
+__device__ inline float2 cmulf(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cdivf(float2 a, float2 b) {
+    const float d = b.x * b.x + b.y * b.y;
+    return make_float2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ inline float cabsf_(float2 a) {
+    return (float)sqrt((double)a.x * a.x + (double)a.y * a.y);
+}
+__device__ inline float2 cexpi(float t) {   // exp(i t)
+    double s, c;
+    sincos((double)t, &s, &c);
+    return make_float2((float)c, (float)s);
+}
+__device__ inline float atan2r(float y, float x) { return (float)atan2((double)y, (double)x); }
+__device__ inline void crotate(float theta, float2 q1, float2 q2, float2 &r1, float2 &r2) {
+    const float pi180 = 0.017453292519943295f;
+    const float2 cs = cexpi(theta * pi180);
+    const float2 c = make_float2(cs.x, 0.f), s = make_float2(cs.y, 0.f);
+    const float2 a = cmulf(q1, c), b = cmulf(q2, s), e = cmulf(q1, s), d = cmulf(q2, c);
+    r1 = make_float2(a.x - b.x, a.y - b.y);
+    r2 = make_float2(e.x + d.x, e.y + d.y);
+}
+__device__ inline float2 cphm2cm(float mag, float ph) { return cmulf(make_float2(mag, 0.f), cexpi(ph)); }
+
+// objfn.f:972-1009 CRESID
+__device__ inline float2 cresid(int irestp, float2 obs, float2 est) {
+    if (obs.x == 0.f && obs.y == 0.f) return make_float2(0.f, 0.f);
+    if (irestp == 1 || irestp == 2) {
+        float ad, pd, ae, pe;
+        if (irestp == 2) {
+            ad = sqrtf(obs.x * obs.x + obs.y * obs.y); pd = 0.f;
+            ae = sqrtf(est.x * est.x + est.y * est.y); pe = 0.f;
+        } else {
+            ad = 1.f; pd = atan2r(obs.y, obs.x);
+            ae = 1.f; pe = atan2r(est.y, est.x);
+        }
+        const float2 a = cmulf(make_float2(ad, 0.f), cexpi(pd)), b = cmulf(make_float2(ae, 0.f), cexpi(pe));
+        return make_float2(a.x - b.x, a.y - b.y);
+    }
+    return make_float2(obs.x - est.x, obs.y - est.y);
+}
+
+// objfn.f:757-839 BPRESID4 for one component (EPS = 0 as in the reference)
+__device__ inline float2 bpresid(int norm, int irestp, float epss, double freq, float dtmax, float w, float2 obs,
+                                 float2 est) {
+    const float twopi = 6.283185307179586f, pi = 3.1415926535897931f, eps = 0.f, eps2 = epss * (eps * eps);
+    if (cabsf_(obs) == 0.f) return make_float2(0.f, 0.f);
+    const float2 dif = cresid(irestp, obs, est), wt = make_float2(w, 0.f), wc = make_float2(w, -0.f);
+    if (norm == 1) return cdivf(cmulf(wc, dif), make_float2(cabsf_(dif), 0.f));
+    if (norm == 3) {
+        if (cabsf_(cmulf(wt, dif)) <= eps) {
+            const float aw = cabsf_(wt);
+            return cmulf(make_float2(aw * aw / eps, 0.f), dif);
+        }
+        return cdivf(cmulf(wc, dif), make_float2(cabsf_(dif), 0.f));
+    }
+    const float aw = cabsf_(wt);
+    if (norm == 4) {
+        const float a = cabsf_(cmulf(wt, dif));
+        const float den = eps2 * sqrtf(1.f + a * a / eps2);
+        return cmulf(make_float2(aw * aw / den, 0.f), dif);
+    }
+    float2 r = cmulf(make_float2(aw * aw, 0.f), dif);
+    if (dtmax > 0.f) {
+        float dphi = atan2r(obs.y, obs.x) - atan2r(est.y, est.x);
+        if (dphi < -pi) dphi = dphi + twopi;
+        if (dphi > +pi) dphi = dphi - twopi;
+        if (fabs((double)dphi / ((double)twopi * freq)) > (double)dtmax) r = make_float2(0.f, 0.f);
+    }
+    return r;
+}
+
+// adjgrad.f90:527-566 HDIFFER (sqrt(-i w) = sqrt(w/2)(1 - i), ...)
+__device__ inline float2 hdiffer(int ib, double freq) {
+    const float om = (float)(2.0 * 3.1415926535897932384626434 * freq);
+    const float s = sqrtf(0.5f * om);
+    switch (ib) {
+        case 1: return make_float2(s, -s);
+        case 2: return make_float2(-s, s);
+        case 3: return make_float2(s, s);
+        case 4: return make_float2(-s, -s);
+        case 5: return make_float2(0.f, om);
+        case 6: return make_float2(0.f, 1.f);
+        default: return make_float2(1.f, 0.f);
+    }
+}
+
+// EST(i, irec, isrc) = RECV(i, irec) SOL(isrc, MRDOF(i, irec)) exp(+i w PY(isrc) YREC(irec)),
+// one thread per (source, receiver); MRDOF in 0..NDOF (checked by the host)
+__global__ void k_exresp(int ndof, int nsrc, int nrec, double omega, const double *__restrict__ py,
+                         const double *__restrict__ yrec, const float2 *__restrict__ sol,
+                         const int *__restrict__ mrdof, const float2 *__restrict__ recv, float2 *__restrict__ est) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsrc * nrec) return;
+    const int is = t / nrec, ir = t % nrec;
+    const float2 cf = cexpi((float)(omega * py[is] * yrec[ir]));
+    for (int i = 0; i < 3; ++i) {
+        const int d = mrdof[3 * ir + i];
+        float2 v = make_float2(0.f, 0.f);
+        if (d != 0) v = cmulf(cmulf(recv[3 * ir + i], sol[(size_t)is * ndof + d - 1]), cf);
+        est[3 * ((size_t)nrec * is + ir) + i] = v;
+    }
+}
+
+struct AdjParams {
+    float az, epss, dtmax;
+    int norm, irestp, ibphase, lunwrap;
+};
+
+// CADJLAM's right-hand side for npair (ifreq, isrc) pairs, one workgroup
+// per pair: every thread forms conj(HDIFFER * residual) of its receivers
+// (vals, [pair][rec][3]), then one lane adds them into the zeroed RHS in
+// receiver order, as RESID_DRHS does (repeated DOFs sum in that order;
+// conj(a) + conj(b) = conj(a + b) exactly); DOFs outside 1..NDOF are
+// skipped and flagged.  est/obs/wghts in the reference's (3, NFREQ, NREC,
+// NSRC) layout; ifreq/isrc checked by the host.
+__global__ __launch_bounds__(256) void k_cadjlam(int ndof, int nfreq, int nrec, const int *__restrict__ ifreq,
+                                                 const int *__restrict__ isrc, const double *__restrict__ freq,
+                                                 const double *__restrict__ py, AdjParams prm,
+                                                 const double *__restrict__ yrec, const int *__restrict__ mrdof,
+                                                 const float2 *__restrict__ est, const float2 *__restrict__ obs,
+                                                 const float *__restrict__ wghts, float2 *__restrict__ vals,
+                                                 float2 *__restrict__ rhs, int *__restrict__ warn) {
+    const int p = blockIdx.x;
+    const double f = freq[p];
+    const float2 h = hdiffer(prm.ibphase, f);
+    float2 *v = vals + (size_t)p * nrec * 3;
+    for (int ir = threadIdx.x; ir < nrec; ir += blockDim.x) {
+        const size_t base = 3 * ((size_t)(ifreq[p] - 1) + (size_t)nfreq * (ir + (size_t)nrec * (isrc[p] - 1)));
+        float2 e[3], o[3], E[3], O[3], R[3];
+        for (int i = 0; i < 3; ++i) { e[i] = est[base + i]; o[i] = obs[base + i]; }
+        if (prm.lunwrap)
+            for (int i = 0; i < 3; ++i) { e[i] = cphm2cm(e[i].x, e[i].y); o[i] = cphm2cm(o[i].x, o[i].y); }
+        crotate(prm.az, e[0], e[1], E[0], E[1]);
+        E[2] = e[2];
+        const float2 cf = cexpi(-(float)(6.2831853071795862 * f * py[p] * yrec[ir]));
+        for (int i = 0; i < 3; ++i) { O[i] = cmulf(o[i], cf); E[i] = cmulf(E[i], cf); }
+        for (int i = 0; i < 3; ++i)
+            R[i] = bpresid(prm.norm, prm.irestp, prm.epss, f, prm.dtmax, wghts[base + i], O[i], E[i]);
+        float2 q1, q2;
+        crotate(-prm.az, R[0], R[1], q1, q2);
+        R[0] = q1; R[1] = q2;
+        for (int i = 0; i < 3; ++i) {
+            const float2 t = cmulf(h, R[i]);
+            v[3 * ir + i] = make_float2(t.x, -t.y);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    float2 *out = rhs + (size_t)p * ndof;
+    for (int k = 0; k < 3 * nrec; ++k) {
+        const int d = mrdof[k];
+        if (d <= 0 || d > ndof) { atomicOr(warn, 1); continue; }
+        out[d - 1].x += v[k].x;
+        out[d - 1].y += v[k].y;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // small utilities
 template <typename T>
 __global__ void k_fill(T *p, size_t n, T v) {

@@ -237,6 +237,39 @@ int f25_gn_jp(int nnpgl, int na35, const int *mygrad, int njac, const int *nobs,
 int f25_gn_jhz(int nnpgl, int na35, const int *mygrad, int njac, const int *nobs, const double *alpha,
                const void *jac, const void *jp, double *z);
 
+/* ---- adjoint-source pipeline (between the forward and the adjoint solve) ---- */
+
+/* EXRESP25D (locrec2d.f:867-925): est(i, irec, isrc) = recv(i, irec) *
+ * sol((isrc-1)*ndof + mrdof(i, irec)) * exp(+i 2 pi freq py(isrc) yrec(irec)),
+ * zero where mrdof is 0.  sol COMPLEX*8 (ndof*nsrc), mrdof (3, nrec) in
+ * 0..ndof, recv (3, nrec) COMPLEX*8, py (nsrc), yrec (nrec), est (3, nrec,
+ * nsrc) COMPLEX*8.  Host arrays; the _dev form takes sol and est on the
+ * device (the receiver tables stay host arrays). */
+int f25_exresp25d(int ndof, int nsrc, int nrec, double freq, const double *py, const double *yrec,
+                  const void *sol, const int *mrdof, const void *recv, void *est);
+int f25_exresp25d_dev(int ndof, int nsrc, int nrec, double freq, const double *py, const double *yrec,
+                      const void *sol_dev, const int *mrdof, const void *recv, void *est_dev);
+/* CADJLAM's right-hand side (adjgrad.f90:229-300, without the data
+ * covariance option LCOVD) for npair (ifreq, isrc) pairs (1-based, with
+ * their frequency in Hz and slowness py): estimates rotated to (N,E,Z) by
+ * azmod degrees (magnitude/phase pairs turned complex first when lunwrap),
+ * data and estimates shifted by exp(-i 2 pi freq py yrec), BPRESID4
+ * residuals (objfn.f:757-839: norm, irestp, epss, dtmax), rotated back,
+ * RESID_DRHS (adjgrad.f90:448-523: HDIFFER(ibphase), scatter at mrdof,
+ * conjugate).  est, obs COMPLEX*8 and wghts REAL*4 in the reference's
+ * (3, NFREQ, NREC, NSRC) layout; rhs (ndof, npair) COMPLEX*8, the dense
+ * right-hand side the transpose solve takes.  iwarn = 1 when a receiver DOF
+ * lies outside 1..ndof (skipped, as the reference does). */
+int f25_cadjlam_rhs(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq, const int *isrc,
+                    const double *freq, const double *py, double azmod, int norm, int irestp, float epss,
+                    float dtmax, int ibphase, int lunwrap, const double *yrec, const int *mrdof, const void *est,
+                    const void *obs, const float *wghts, void *rhs, int *iwarn);
+int f25_cadjlam_rhs_dev(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq, const int *isrc,
+                        const double *freq, const double *py, double azmod, int norm, int irestp, float epss,
+                        float dtmax, int ibphase, int lunwrap, const double *yrec, const int *mrdof,
+                        const void *est_dev, const void *obs_dev, const float *wghts_dev, void *rhs_dev,
+                        int *iwarn);
+
 /* ---- second-order products of QPMFREE (truncated-Newton Hessian) ----------- */
 
 /* CADJ_NU_ALPHA's element loops (qpmfree.f90:622-912) on the attached

@@ -374,6 +374,105 @@ def met225(mesh, model, g, inv: Inv, freq, py, adjlam, nu, alpha, wave, cinvtype
     return p1, p2
 
 
+# ---- adjoint-source pipeline ----
+def _c8(a):
+    return np.ascontiguousarray(a, np.complex64)
+
+
+def exresp25d(ndof, freq, py, yrec, sol, mrdof, recv, reference=False):
+    """EXRESP25D: est (nsrc, nrec, 3) from sol (nsrc*ndof,), mrdof/recv (nrec, 3)."""
+    py = np.ascontiguousarray(py, np.float64)
+    yrec = np.ascontiguousarray(yrec, np.float64)
+    sol, recv = _c8(sol), _c8(recv)
+    mrdof = np.ascontiguousarray(mrdof, np.int32)
+    nsrc, nrec = py.size, yrec.size
+    est = np.zeros((nsrc, nrec, 3), np.complex64)
+    if reference:
+        i = lambda v: C.byref(C.c_int(v))   # noqa: E731
+        ref().exresp25d_(i(3), i(nrec), i(ndof), i(nsrc), i(nrec), i(3), C.byref(C.c_double(freq)),
+                         _p(py, dp), _p(yrec, dp), _p(sol, fp), _p(mrdof, ip), _p(recv, fp), _p(est, fp))
+    else:
+        lib().orc_exresp25d(ndof, nsrc, nrec, C.c_double(freq), _p(py, dp), _p(yrec, dp), _p(sol, fp),
+                            _p(mrdof, ip), _p(recv, fp), _p(est, fp))
+    return est
+
+
+def bpresid4(norm, irestp, epss, freq, dtmax, swght, obs, est, reference=False):
+    """BPRESID4 on (nrec, 3) arrays; returns RESID (nrec, 3) COMPLEX*8."""
+    swght = np.ascontiguousarray(swght, np.float32)
+    obs, est = _c8(obs), _c8(est)
+    nrec = obs.shape[0]
+    out = np.zeros((nrec, 3), np.complex64)
+    if reference:
+        i = lambda v: C.byref(C.c_int(v))   # noqa: E731
+        ref().bpresid4_(i(3), i(nrec), i(3), i(norm), i(irestp), C.byref(C.c_float(epss)),
+                        C.byref(C.c_double(freq)), C.byref(C.c_float(dtmax)), _p(swght, fp), _p(obs, fp),
+                        _p(est, fp), _p(out, fp))
+    else:
+        lib().orc_bpresid4(nrec, norm, irestp, C.c_float(epss), C.c_double(freq), C.c_float(dtmax),
+                           _p(swght, fp), _p(obs, fp), _p(est, fp), _p(out, fp))
+    return out
+
+
+def cresid(irestp, obs, est, reference=False):
+    obs, est = _c8(obs).ravel(), _c8(est).ravel()
+    out = np.zeros(obs.size, np.complex64)
+    f = ref().ref_cresid_n if reference else lib().orc_cresid_n
+    f(obs.size, irestp, _p(obs, fp), _p(est, fp), _p(out, fp))
+    return out
+
+
+def cphm2cm(xmag, ph, reference=False):
+    xmag = np.ascontiguousarray(xmag, np.float32).ravel()
+    ph = np.ascontiguousarray(ph, np.float32).ravel()
+    out = np.zeros(xmag.size, np.complex64)
+    f = ref().ref_cphm2cm_n if reference else lib().orc_cphm2cm_n
+    f(xmag.size, _p(xmag, fp), _p(ph, fp), _p(out, fp))
+    return out
+
+
+def crotate(theta, q1, q2, reference=False):
+    q1, q2 = _c8(q1).ravel(), _c8(q2).ravel()
+    r1, r2 = np.zeros_like(q1), np.zeros_like(q2)
+    f = ref().ref_crotate_n if reference else lib().orc_crotate_n
+    f(q1.size, C.c_float(theta), _p(q1, fp), _p(q2, fp), _p(r1, fp), _p(r2, fp))
+    return r1, r2
+
+
+def hdiffer(ibphase, freq):
+    out = np.zeros(2, np.float32)
+    lib().orc_hdiffer_v(ibphase, C.c_double(freq), _p(out, fp))
+    return np.complex64(complex(out[0], out[1]))
+
+
+def resid_drhs(ndof, ibphase, freq, mrdof, resid):
+    mrdof = np.ascontiguousarray(mrdof, np.int32)
+    resid = _c8(resid)
+    rhs = np.zeros(ndof, np.complex64)
+    iw = C.c_int(0)
+    lib().orc_resid_drhs(ndof, mrdof.shape[0], ibphase, C.c_double(freq), _p(mrdof, ip), _p(resid, fp),
+                         _p(rhs, fp), C.byref(iw))
+    return rhs, iw.value
+
+
+def cadjlam_rhs(ndof, ifreq, isrc, freq, py, azmod, norm, irestp, epss, dtmax, ibphase, lunwrap, yrec,
+                mrdof, est, obs, wghts):
+    """CADJLAM's right-hand side (before the solve) for (ifreq, isrc), 1-based;
+    est/obs COMPLEX*8 and wghts REAL*4 in the reference's (3, nfreq, nrec,
+    nsrc) layout, passed as numpy arrays of shape (nsrc, nrec, nfreq, 3)."""
+    est, obs = _c8(est), _c8(obs)
+    wghts = np.ascontiguousarray(wghts, np.float32)
+    nsrc, nrec, nfreq, _ = est.shape
+    yrec = np.ascontiguousarray(yrec, np.float64)
+    mrdof = np.ascontiguousarray(mrdof, np.int32)
+    rhs = np.zeros(ndof, np.complex64)
+    iw = C.c_int(0)
+    lib().orc_cadjlam_rhs(ndof, nfreq, nrec, ifreq, isrc, C.c_double(freq), C.c_double(py), C.c_double(azmod),
+                          norm, irestp, C.c_float(epss), C.c_float(dtmax), ibphase, int(lunwrap), _p(yrec, dp),
+                          _p(mrdof, ip), _p(est, fp), _p(obs, fp), _p(wghts, fp), _p(rhs, fp), C.byref(iw))
+    return rhs, iw.value
+
+
 def _elem_inputs(nord, xlocs, zlocs, dens, ecoeff):
     xlocs = np.ascontiguousarray(xlocs, np.float64)
     zlocs = np.ascontiguousarray(zlocs, np.float64)

@@ -50,6 +50,19 @@
  *                    column; elsewhere the column is the last element's, the
  *                    element choice CFMAT_DIST makes (compiled reference).
  *   orc_met225       as P3 (qpmfree.f90 needs MUMPS headers: not compiled).
+ *   orc_exresp25d,
+ *   orc_crotate,
+ *   orc_cphm2cm,
+ *   orc_cresid,
+ *   orc_bpresid4     compiled reference EXRESP25D, CROTATE, CPHM2CM, CRESID,
+ *                    BPRESID4 (every norm, residual type, phase gate).
+ *   orc_hdiffer,
+ *   orc_resid_drhs   closed forms (sqrt(-i w) = sqrt(w/2)(1 - i), ...); the
+ *                    scatter by brute force with repeated and invalid DOFs
+ *                    (adjgrad.f90 needs MUMPS headers: not compiled).
+ *   orc_cadjlam_rhs  composition of the above; with equal weights on the two
+ *                    horizontal components the L2 residual rotated back is
+ *                    W^2 (R(-az) OBS - EST) (closed form).
  */
 #include "oracle.h"
 #include <math.h>
@@ -1739,6 +1752,241 @@ int orc_met225(const orc_asm *a, const orc_inv *inv, const char *cinvtype, doubl
     free(dab); free(dmdb); free(dkdb); free(t1); free(t2);
     ws_free(&w);
     return ierr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint-source pipeline: receiver extraction, residuals, adjoint RHS.
+ * COMPLEX*8 / REAL*4 arithmetic as the reference compiles it: products by
+ * the textbook formula, quotients through the runtime's complex divide,
+ * elementary functions from libm's single-precision routines. */
+
+/* locrec2d.f:867-925 EXRESP25D: EST(i,irec,isrc) = RECV(i,irec) *
+ * SOL((isrc-1)*NDOF + MRDOF(i,irec)) * exp(+i w PY(isrc) YREC(irec)), zero
+ * where MRDOF is 0; the phase argument is rounded to REAL*4 first (CMPLX of
+ * default kind).  MDIM = 3, MREC = NREC. */
+void orc_exresp25d(int ndof, int nsrc, int nrec, double freq, const double *py, const double *yrec,
+                   const float complex *sol, const int *mrdof, const float complex *recv,
+                   float complex *est)
+{
+    const double twopi = 6.2831853071795862;
+    double omega = twopi * freq;
+    for (int is = 0; is < nsrc; is++) {
+        long loff = (long)is * ndof;
+        for (int ir = 0; ir < nrec; ir++) {
+            float complex cfact = cexpf(CMPLXF(0.0f, (float)(omega * py[is] * yrec[ir])));
+            for (int i = 0; i < NDIM; i++) {
+                float complex *o = est + NDIM * ((long)nrec * is + ir) + i;
+                int d = mrdof[NDIM * ir + i];
+                if (d != 0) *o = cmulf_ref(cmulf_ref(recv[NDIM * ir + i], sol[loff + d - 1]), cfact);
+                else *o = 0.0f;
+            }
+        }
+    }
+}
+
+/* wrapph.f:178-191 CROTATE (angle in degrees, REAL*4) */
+void orc_crotate(float theta, float complex q1, float complex q2, float complex *rq1, float complex *rq2)
+{
+    const float pi180 = 0.017453292519943295f;
+    float complex cost = CMPLXF(cosf(theta * pi180), 0.0f), sint = CMPLXF(sinf(theta * pi180), 0.0f);
+    float complex a = cmulf_ref(q1, cost), b = cmulf_ref(q2, sint);
+    float complex c2 = cmulf_ref(q1, sint), d = cmulf_ref(q2, cost);
+    *rq1 = CMPLXF(crealf(a) - crealf(b), cimagf(a) - cimagf(b));
+    *rq2 = CMPLXF(crealf(c2) + crealf(d), cimagf(c2) + cimagf(d));
+}
+
+/* wrapph.f:169-174 CPHM2CM: magnitude, phase -> complex */
+float complex orc_cphm2cm(float xmag, float phase)
+{
+    return cmulf_ref(CMPLXF(xmag, 0.0f), cexpf(CMPLXF(0.0f, phase)));
+}
+
+static float sphase(float complex q) { return atan2f(cimagf(q), crealf(q)); }   /* wrapph.f:154-159 */
+
+/* COMPLEX*8 quotient as the reference's compiler forms it: the textbook
+ * formula in single precision, without scaling (matches the compiled
+ * BPRESID4 bit for bit; a scaled or promoted divide does not) */
+static inline float complex cdivf_ref(float complex a, float complex b)
+{
+    float ar = crealf(a), ai = cimagf(a), br = crealf(b), bi = cimagf(b);
+    float d = br * br + bi * bi;
+    return CMPLXF((ar * br + ai * bi) / d, (ai * br - ar * bi) / d);
+}
+
+/* objfn.f:972-1009 CRESID: 1 phase only, 2 amplitude only, else OBS - EST */
+float complex orc_cresid(int irestp, float complex obs, float complex est)
+{
+    if (crealf(obs) == 0.0f && cimagf(obs) == 0.0f) return 0.0f;
+    if (irestp == 1 || irestp == 2) {
+        float robs = crealf(obs), zobs = cimagf(obs), rest = crealf(est), zest = cimagf(est);
+        float adata, pdata, aest, pest;
+        if (irestp == 2) {
+            adata = sqrtf(robs * robs + zobs * zobs); pdata = 0.0f;
+            aest = sqrtf(rest * rest + zest * zest); pest = 0.0f;
+        } else {
+            adata = 1.0f; pdata = atan2f(zobs, robs);
+            aest = 1.0f; pest = atan2f(zest, rest);
+        }
+        float complex a = cmulf_ref(CMPLXF(adata, 0.0f), cexpf(CMPLXF(0.0f, pdata)));
+        float complex b = cmulf_ref(CMPLXF(aest, 0.0f), cexpf(CMPLXF(0.0f, pest)));
+        return CMPLXF(crealf(a) - crealf(b), cimagf(a) - cimagf(b));
+    }
+    return CMPLXF(crealf(obs) - crealf(est), cimagf(obs) - cimagf(est));
+}
+
+/* objfn.f:757-839 BPRESID4 (its EPS is 0, so the Huber and combination
+ * branches give what those formulas give with EPS = 0) */
+void orc_bpresid4(int nrec, int norm, int irestp, float epss, double freq, float dtmax,
+                  const float *swght, const float complex *obs, const float complex *est,
+                  float complex *resid)
+{
+    const float twopi = 6.283185307179586f, pi = 3.1415926535897931f;
+    float eps = 0.0f, eps2 = epss * (eps * eps);
+    for (int ir = 0; ir < nrec; ir++)
+        for (int i = 0; i < NDIM; i++) {
+            int k = NDIM * ir + i;
+            resid[k] = 0.0f;
+            if (cabsf(obs[k]) == 0.0f) continue;
+            float complex dif = orc_cresid(irestp, obs[k], est[k]);
+            float complex weight = CMPLXF(swght[k], 0.0f);
+            if (norm == 1) {
+                resid[k] = cdivf_ref(cmulf_ref(conjf(weight), dif), CMPLXF(cabsf(dif), 0.0f));
+            } else if (norm == 3) {
+                if (cabsf(cmulf_ref(weight, dif)) <= eps) {
+                    float w2 = cabsf(weight) * cabsf(weight) / eps;
+                    resid[k] = cmulf_ref(CMPLXF(w2, 0.0f), dif);
+                } else {
+                    resid[k] = cdivf_ref(cmulf_ref(conjf(weight), dif), CMPLXF(cabsf(dif), 0.0f));
+                }
+            } else if (norm == 4) {
+                float w2 = cabsf(weight) * cabsf(weight);
+                float a = cabsf(cmulf_ref(weight, dif));
+                float denon = eps2 * sqrtf(1.0f + a * a / eps2);
+                resid[k] = cmulf_ref(CMPLXF(w2 / denon, 0.0f), dif);
+            } else {
+                float w2 = cabsf(weight) * cabsf(weight);
+                resid[k] = cmulf_ref(CMPLXF(w2, 0.0f), dif);
+                if (dtmax > 0.0f) {
+                    float dphi = sphase(obs[k]) - sphase(est[k]);
+                    if (dphi < -pi) dphi = dphi + twopi;
+                    if (dphi > +pi) dphi = dphi - twopi;
+                    if (fabs((double)dphi / ((double)twopi * freq)) > (double)dtmax) resid[k] = 0.0f;
+                }
+            }
+        }
+}
+
+/* adjgrad.f:527-566 HDIFFER: spreading compensation factor */
+float complex orc_hdiffer(int ibphase, double freq)
+{
+    const double pi = 3.1415926535897932384626434;
+    float omega = (float)(2.0 * pi * freq);
+    if (ibphase == 1 || ibphase == 2) {
+        float complex h = csqrtf(CMPLXF(0.0f, -omega));
+        return ibphase == 2 ? -h : h;
+    }
+    if (ibphase == 3 || ibphase == 4) {
+        float complex h = csqrtf(CMPLXF(0.0f, omega));
+        return ibphase == 4 ? -h : h;
+    }
+    if (ibphase == 5) return CMPLXF(0.0f, omega);
+    if (ibphase == 6) return CMPLXF(0.0f, 1.0f);
+    return CMPLXF(1.0f, 0.0f);
+}
+
+/* adjgrad.f:448-523 RESID_DRHS: RHS(MRDOF(i,irec)) += HFACT*RESID(i,irec)
+ * in receiver order, invalid DOFs skipped with IWARN = 1, then RHS = CONJG */
+void orc_resid_drhs(int ndof, int nrec, int ibphase, double freq, const int *mrdof,
+                    const float complex *resid, float complex *rhs, int *iwarn)
+{
+    float complex h = orc_hdiffer(ibphase, freq);
+    *iwarn = 0;
+    for (int i = 0; i < ndof; i++) rhs[i] = 0.0f;
+    for (int ir = 0; ir < nrec; ir++)
+        for (int i = 0; i < NDIM; i++) {
+            int d = mrdof[NDIM * ir + i];
+            if (d <= 0 || d > ndof) { *iwarn = 1; continue; }
+            float complex t = cmulf_ref(h, resid[NDIM * ir + i]);
+            rhs[d - 1] = CMPLXF(crealf(rhs[d - 1]) + crealf(t), cimagf(rhs[d - 1]) + cimagf(t));
+        }
+    for (int i = 0; i < ndof; i++) rhs[i] = conjf(rhs[i]);
+}
+
+/* element-wise forms of the scalar routines above, for callers that cannot
+ * receive a complex return value */
+void orc_cresid_n(int n, int irestp, const float complex *obs, const float complex *est, float complex *out)
+{
+    for (int i = 0; i < n; i++) out[i] = orc_cresid(irestp, obs[i], est[i]);
+}
+
+void orc_cphm2cm_n(int n, const float *xmag, const float *ph, float complex *out)
+{
+    for (int i = 0; i < n; i++) out[i] = orc_cphm2cm(xmag[i], ph[i]);
+}
+
+void orc_crotate_n(int n, float theta, const float complex *q1, const float complex *q2, float complex *r1,
+                   float complex *r2)
+{
+    for (int i = 0; i < n; i++) orc_crotate(theta, q1[i], q2[i], &r1[i], &r2[i]);
+}
+
+void orc_hdiffer_v(int ibphase, double freq, float *out)
+{
+    float complex h = orc_hdiffer(ibphase, freq);
+    out[0] = crealf(h);
+    out[1] = cimagf(h);
+}
+
+/* adjgrad.f:149-300 CADJLAM up to the solve (no data covariance, LCOVD
+ * false): for receiver irec of (IFREQ, ISRC), estimates rotated to (N,E,Z)
+ * by AZMOD (magnitude/phase pairs first turned complex when LUNWRAP), data
+ * and estimates shifted by exp(-i SNGL(2 pi FREQ PY YREC)), BPRESID4,
+ * residuals rotated back by -AZMOD, RESID_DRHS.  est, obs: (3, nfreq, nrec,
+ * nsrc) COMPLEX*8; wghts REAL*4 likewise; ifreq, isrc 1-based. */
+void orc_cadjlam_rhs(int ndof, int nfreq, int nrec, int ifreq, int isrc, double freq, double py,
+                     double azmod, int norm, int irestp, float epss, float dtmax, int ibphase,
+                     int lunwrap, const double *yrec, const int *mrdof, const float complex *est,
+                     const float complex *obs, const float *wghts, float complex *rhs, int *iwarn)
+{
+    const double twopi = 6.2831853071795862;
+    size_t n3 = (size_t)NDIM * (nrec > 0 ? nrec : 1);
+    float complex *e = (float complex *)calloc(n3, sizeof(float complex));
+    float complex *o = (float complex *)calloc(n3, sizeof(float complex));
+    float complex *r = (float complex *)calloc(n3, sizeof(float complex));
+    float *w = (float *)calloc(n3, sizeof(float));
+    for (int ir = 0; ir < nrec; ir++) {
+        long base = NDIM * ((long)(ifreq - 1) + (long)nfreq * (ir + (long)nrec * (isrc - 1)));
+        const float complex *ev = est + base, *ov = obs + base;
+        for (int i = 0; i < NDIM; i++) w[NDIM * ir + i] = wghts[base + i];
+        float complex q1, q2, e3, o1, o2, o3;
+        if (lunwrap) {
+            q1 = orc_cphm2cm(crealf(ev[0]), cimagf(ev[0]));
+            q2 = orc_cphm2cm(crealf(ev[1]), cimagf(ev[1]));
+            e3 = orc_cphm2cm(crealf(ev[2]), cimagf(ev[2]));
+            o1 = orc_cphm2cm(crealf(ov[0]), cimagf(ov[0]));
+            o2 = orc_cphm2cm(crealf(ov[1]), cimagf(ov[1]));
+            o3 = orc_cphm2cm(crealf(ov[2]), cimagf(ov[2]));
+        } else {
+            q1 = ev[0]; q2 = ev[1]; e3 = ev[2];
+            o1 = ov[0]; o2 = ov[1]; o3 = ov[2];
+        }
+        float complex *E = e + NDIM * ir, *O = o + NDIM * ir;
+        orc_crotate((float)azmod, q1, q2, &E[0], &E[1]);
+        E[2] = e3;
+        O[0] = o1; O[1] = o2; O[2] = o3;
+        float complex cfact = cexpf(CMPLXF(0.0f, -(float)(twopi * freq * py * yrec[ir])));
+        for (int i = 0; i < NDIM; i++) {
+            O[i] = cmulf_ref(O[i], cfact);
+            E[i] = cmulf_ref(E[i], cfact);
+        }
+    }
+    orc_bpresid4(nrec, norm, irestp, epss, freq, dtmax, w, o, e, r);
+    for (int ir = 0; ir < nrec; ir++) {
+        float complex *R = r + NDIM * ir, qn = R[0], qe = R[1];
+        orc_crotate(-(float)azmod, qn, qe, &R[0], &R[1]);
+    }
+    orc_resid_drhs(ndof, nrec, ibphase, freq, mrdof, r, rhs, iwarn);
+    free(e); free(o); free(r); free(w);
 }
 
 /* Exact derivative of S(w,ky) with respect to anchor lambda/mu/rho.  The

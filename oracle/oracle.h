@@ -178,6 +178,27 @@ int orc_qp_nu_alpha(const orc_asm *a, const orc_inv *inv, const char *cinvtype, 
 int orc_met225(const orc_asm *a, const orc_inv *inv, const char *cinvtype, double freq, double py,
                const float complex *adjlam, const float complex *nu, const float complex *alpha,
                const float complex *wave, float *p1, float *p2);
+void orc_exresp25d(int ndof, int nsrc, int nrec, double freq, const double *py, const double *yrec,
+                   const float complex *sol, const int *mrdof, const float complex *recv,
+                   float complex *est);
+void orc_crotate(float theta, float complex q1, float complex q2, float complex *rq1, float complex *rq2);
+float complex orc_cphm2cm(float xmag, float phase);
+float complex orc_cresid(int irestp, float complex obs, float complex est);
+void orc_bpresid4(int nrec, int norm, int irestp, float epss, double freq, float dtmax,
+                  const float *swght, const float complex *obs, const float complex *est,
+                  float complex *resid);
+float complex orc_hdiffer(int ibphase, double freq);
+void orc_resid_drhs(int ndof, int nrec, int ibphase, double freq, const int *mrdof,
+                    const float complex *resid, float complex *rhs, int *iwarn);
+void orc_cresid_n(int n, int irestp, const float complex *obs, const float complex *est, float complex *out);
+void orc_cphm2cm_n(int n, const float *xmag, const float *ph, float complex *out);
+void orc_crotate_n(int n, float theta, const float complex *q1, const float complex *q2, float complex *r1,
+                   float complex *r2);
+void orc_hdiffer_v(int ibphase, double freq, float *out);
+void orc_cadjlam_rhs(int ndof, int nfreq, int nrec, int ifreq, int isrc, double freq, double py,
+                     double azmod, int norm, int irestp, float epss, float dtmax, int ibphase,
+                     int lunwrap, const double *yrec, const int *mrdof, const float complex *est,
+                     const float complex *obs, const float *wghts, float complex *rhs, int *iwarn);
 
 /* pchadj.f90:225-281 (NVINV = 1), one (frequency, source):
  *   JACL(irhs, c) = sum over column c of F of G((irhs-1)*lrhs + ICSC(i)) FMAT(i)

@@ -3,6 +3,8 @@ reference): assembled operator, SpMV, batched product, gradient, virtual
 forces.  Tolerances: FP64 results 1e-12 normwise relative; COMPLEX*8
 results at FP32 level (the reference itself rounds every contribution to
 FP32)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -375,3 +377,117 @@ def test_apply_batch_modes_high_order(name):
     print(f"{name}: fp32 {e32:.2e}, refined {er:.2e}")
     assert e32 < 1e-5
     assert er < 1e-8
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _adj_case(seed, ndof=3000, nfreq=4, nrec=60, nsrc=3):
+    rng = np.random.default_rng(seed)
+    mrdof = rng.integers(1, ndof + 1, size=(nrec, 3)).astype(np.int32)
+    mrdof[4, 1] = mrdof[9, 0]                   # two receivers on one DOF
+    yrec = rng.uniform(-3e4, 3e4, nrec)
+    cv = lambda *s: (rng.standard_normal(s) + 1j * rng.standard_normal(s)).astype(np.complex64)  # noqa: E731
+    est = cv(nsrc, nrec, nfreq, 3)
+    obs = (est + 0.4 * cv(nsrc, nrec, nfreq, 3)).astype(np.complex64)
+    obs[0, ::7, :, 2] = 0                       # missing data
+    w = rng.uniform(0.3, 2.0, (nsrc, nrec, nfreq, 3)).astype(np.float32)
+    return rng, mrdof, yrec, est, obs, w
+
+
+def test_exresp25d(torch_dev):
+    """EXRESP25D on the GPU (host and device forms) against the oracle's
+    restatement (itself bit-exact with the compiled routine): 2e-7 of the
+    largest value (single-precision sin/cos of the receiver phase)."""
+    from fem25_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(111)
+    ndof, nsrc, nrec = 4000, 3, 45
+    mrdof = rng.integers(0, ndof + 1, size=(nrec, 3)).astype(np.int32)
+    recv = (rng.standard_normal((nrec, 3)) + 1j * rng.standard_normal((nrec, 3))).astype(np.complex64)
+    yrec = rng.uniform(-3e4, 3e4, nrec)
+    py = rng.uniform(0, 1.4e-4, nsrc)
+    sol = (rng.standard_normal(nsrc * ndof) + 1j * rng.standard_normal(nsrc * ndof)).astype(np.complex64)
+    ref = oracle.exresp25d(ndof, 0.083, py, yrec, sol, mrdof, recv)
+    est = np.zeros_like(ref)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)   # noqa: E731
+    _lib.check(L.f25_exresp25d(ndof, nsrc, nrec, 0.083, _ptr(py, _lib.dp), _ptr(yrec, _lib.dp), vp(sol),
+                               _ptr(mrdof, _lib.ip), vp(recv), vp(est)), "exresp25d")
+    assert np.abs(est - ref).max() <= 2e-7 * np.abs(ref).max()
+    assert np.array_equal(est == 0, ref == 0)
+    torch = torch_dev
+    sd = torch.from_numpy(sol).cuda()
+    ed = torch.zeros(est.size, dtype=torch.complex64, device="cuda")
+    _lib.check(L.f25_exresp25d_dev(ndof, nsrc, nrec, 0.083, _ptr(py, _lib.dp), _ptr(yrec, _lib.dp),
+                                   C.c_void_p(sd.data_ptr()), _ptr(mrdof, _lib.ip), vp(recv),
+                                   C.c_void_p(ed.data_ptr())), "exresp25d_dev")
+    np.testing.assert_array_equal(ed.cpu().numpy().reshape(est.shape), est)
+
+
+@pytest.mark.parametrize("norm,dtmax", [(2, 0.0), (2, 2.0), (1, 0.0), (3, 0.0), (4, 0.0)])
+def test_cadjlam_rhs(norm, dtmax):
+    """CADJLAM's right-hand side on the GPU for a batch of (frequency,
+    source) pairs covering every residual type, both data forms (complex and
+    magnitude/phase), every HDIFFER option and a rotated frame, against the
+    oracle (whose parts are bit-exact with the compiled BPRESID4, CRESID,
+    CROTATE, CPHM2CM): 2e-6 of the pair's largest value; NaN exactly where
+    the reference's EPS = 0 combination norm gives NaN.  The L1 and Huber
+    residuals dif/|dif| magnify an ulp of sin, cos or atan2 (which no two
+    math libraries round alike) by 1/|dif|; for them each entry may also
+    differ by 10x the largest change the oracle itself shows when its data
+    and estimates move by four parts in 2^23 (an ulp of a phase near pi)."""
+    from fem25_amd import _lib
+    L = _lib.load()
+    rng, mrdof, yrec, est, obs, w = _adj_case(112 + norm)
+    nsrc, nrec, nfreq, _ = est.shape
+    ndof = 3000
+    pairs = [(f + 1, s + 1) for f in range(nfreq) for s in range(nsrc)]
+    ifreq = np.array([p[0] for p in pairs], np.int32)
+    isrc = np.array([p[1] for p in pairs], np.int32)
+    freq = 0.02 + 0.02 * ifreq.astype(np.float64)
+    py = rng.uniform(0, 1.3e-4, len(pairs))
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)   # noqa: E731
+    for irestp in (1, 2, 3):
+        for lunwrap in (0, 1):
+            ib = (irestp + 2 * lunwrap + norm) % 7
+            az = 17.5 * irestp - 30.0
+            print(f"cadjlam norm={norm} dtmax={dtmax} irestp={irestp} lunwrap={lunwrap} ibphase={ib}", flush=True)
+            rhs = np.zeros((len(pairs), ndof), np.complex64)
+            iw = C.c_int(-1)
+            _lib.check(L.f25_cadjlam_rhs(ndof, nfreq, nrec, nsrc, len(pairs), _ptr(ifreq, _lib.ip),
+                                         _ptr(isrc, _lib.ip), _ptr(freq, _lib.dp), _ptr(py, _lib.dp), az,
+                                         norm, irestp, 0.05, dtmax, ib, lunwrap, _ptr(yrec, _lib.dp),
+                                         _ptr(mrdof, _lib.ip), vp(est), vp(obs), _ptr(w, _lib.fp), vp(rhs),
+                                         C.byref(iw)), "cadjlam_rhs")
+            assert iw.value == 0
+            for k, (f, s) in enumerate(pairs):
+                args = (ndof, f, s, freq[k], py[k], az, norm, irestp, 0.05, dtmax, ib, bool(lunwrap), yrec, mrdof)
+                ref, iwr = oracle.cadjlam_rhs(*args, est, obs, w)
+                assert np.array_equal(np.isnan(rhs[k]), np.isnan(ref))
+                ok = ~np.isnan(ref)
+                scale = np.abs(ref[ok]).max() if ok.any() else 0.0
+                tol = np.full(ndof, 2e-6 * max(scale, 1e-30))
+                if norm in (1, 3):
+                    sens = np.zeros(ndof)
+                    for t in range(3):
+                        sg = lambda: rng.choice([-1.0, 1.0], est.shape)   # noqa: E731
+                        pe = 1 + 2.0 ** -21 * (sg() + 1j * sg())      # moves magnitude and phase
+                        po = 1 + 2.0 ** -21 * (sg() + 1j * sg())
+                        alt, _ = oracle.cadjlam_rhs(*args, (est * pe).astype(np.complex64),
+                                                    (obs * po).astype(np.complex64), w)
+                        sens = np.maximum(sens, np.abs(alt - ref))
+                    tol += 10 * sens
+                err = np.abs(rhs[k] - ref)
+                worst = int(np.argmax(np.where(ok, err / tol, 0)))
+                assert np.all(err[ok] <= tol[ok]), (k, irestp, lunwrap, worst, err[worst], tol[worst], scale)
+    # an invalid receiver DOF is skipped and reported
+    bad = mrdof.copy()
+    bad[3, 2] = ndof + 5
+    rhs = np.zeros((1, ndof), np.complex64)
+    iw = C.c_int(0)
+    _lib.check(L.f25_cadjlam_rhs(ndof, nfreq, nrec, nsrc, 1, _ptr(ifreq, _lib.ip), _ptr(isrc, _lib.ip),
+                                 _ptr(freq, _lib.dp), _ptr(py, _lib.dp), 0.0, 2, 3, 0.05, 0.0, 0, 0,
+                                 _ptr(yrec, _lib.dp), _ptr(bad, _lib.ip), vp(est), vp(obs),
+                                 _ptr(w, _lib.fp), vp(rhs), C.byref(iw)), "cadjlam_rhs")
+    assert iw.value == 1

@@ -469,3 +469,139 @@ def test_qp_nu_alpha_against_assembled_operator(kind):
     _, _, p3n = oracle.qp_nu_alpha(m, mod, g, inv, freq, py, v, nu, u, cinvtype=kind, pml=pml)
     p1n, _ = oracle.met225(m, mod, g, inv, freq, py, lam, nu, alpha, u, cinvtype=kind, pml=pml)
     assert np.abs(p3n - v * p1n).max() <= 1e-5 * np.abs(p3n).max()
+
+
+# ---- adjoint-source pipeline --------------------------------------------------
+def _receivers(seed, nrec, ndof, zero_frac=0.1):
+    rng = np.random.default_rng(seed)
+    mrdof = rng.integers(1, ndof + 1, size=(nrec, 3)).astype(np.int32)
+    mrdof[rng.random((nrec, 3)) < zero_frac] = 0
+    recv = (rng.standard_normal((nrec, 3)) + 1j * rng.standard_normal((nrec, 3))).astype(np.complex64)
+    yrec = rng.uniform(-3e4, 3e4, nrec)
+    return mrdof, recv, yrec
+
+
+def _cvals(rng, shape, scale=1.0):
+    return (scale * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))).astype(np.complex64)
+
+
+@needs_ref
+def test_exresp25d_vs_reference():
+    """EXRESP25D restated: bit-exact against the compiled routine (several
+    sources with their own slowness, receivers off the y = 0 plane, unused
+    components)."""
+    ndof, nsrc, nrec = 500, 4, 37
+    mrdof, recv, yrec = _receivers(101, nrec, ndof)
+    rng = np.random.default_rng(102)
+    sol = _cvals(rng, nsrc * ndof)
+    py = rng.uniform(0, 1.4e-4, nsrc)
+    e = oracle.exresp25d(ndof, 0.083, py, yrec, sol, mrdof, recv)
+    r = oracle.exresp25d(ndof, 0.083, py, yrec, sol, mrdof, recv, reference=True)
+    np.testing.assert_array_equal(e, r)
+    assert np.count_nonzero(e) == nsrc * np.count_nonzero(mrdof)
+
+
+@needs_ref
+def test_residual_routines_vs_reference():
+    """CRESID, CPHM2CM, CROTATE and BPRESID4 restated: bit-exact against the
+    compiled routines for every residual type, every norm (the Huber and
+    combination branches with the reference's EPS = 0 included, NaN where the
+    reference gives NaN), the phase gate of the L2 norm, and receivers
+    without data."""
+    rng = np.random.default_rng(103)
+    n = 300
+    obs = _cvals(rng, n)
+    est = obs + _cvals(rng, n, 0.3)
+    obs[::17] = 0
+    for irestp in (1, 2, 3):
+        np.testing.assert_array_equal(oracle.cresid(irestp, obs, est),
+                                      oracle.cresid(irestp, obs, est, reference=True))
+    mag = rng.uniform(0, 3, n).astype(np.float32)
+    ph = rng.uniform(-7, 7, n).astype(np.float32)
+    np.testing.assert_array_equal(oracle.cphm2cm(mag, ph), oracle.cphm2cm(mag, ph, reference=True))
+    for th in (0.0, 33.7, -121.25):
+        a, b = oracle.crotate(th, obs, est), oracle.crotate(th, obs, est, reference=True)
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+    nrec = 100
+    o3, e3 = obs.reshape(nrec, 3), est.reshape(nrec, 3)
+    w = rng.uniform(0.2, 2.0, (nrec, 3)).astype(np.float32)
+    nan_seen = False
+    for norm in (1, 2, 3, 4):
+        for irestp in (1, 2, 3):
+            for dtmax in (0.0, 1.5):
+                a = oracle.bpresid4(norm, irestp, 0.05, 0.071, dtmax, w, o3, e3)
+                b = oracle.bpresid4(norm, irestp, 0.05, 0.071, dtmax, w, o3, e3, reference=True)
+                np.testing.assert_array_equal(a, b)
+                nan_seen |= bool(np.isnan(a).any())
+    assert nan_seen    # combination norm with EPS = 0
+    gated = oracle.bpresid4(2, 3, 0.05, 0.071, 1.5, w, o3, e3)
+    assert 0 < np.count_nonzero(gated) < np.count_nonzero(oracle.bpresid4(2, 3, 0.05, 0.071, 0.0, w, o3, e3))
+
+
+def test_hdiffer_closed_form():
+    """HDIFFER against its closed forms: sqrt(-i w) = sqrt(w/2)(1 - i),
+    sqrt(i w) = sqrt(w/2)(1 + i), i w, i, 1 (one REAL*4 rounding)."""
+    f = 0.137
+    w = np.float32(2 * np.pi * f)
+    s = np.sqrt(w / 2)
+    want = {0: 1, 1: s * (1 - 1j), 2: -s * (1 - 1j), 3: s * (1 + 1j), 4: -s * (1 + 1j), 5: 1j * w, 6: 1j, 7: 1}
+    for k, v in want.items():
+        assert abs(oracle.hdiffer(k, f) - v) <= 2e-7 * abs(v), k
+
+
+def test_resid_drhs_scatter():
+    """RESID_DRHS by brute force: repeated DOFs sum in receiver order,
+    out-of-range DOFs are skipped with IWARN = 1, the result is conjugated."""
+    rng = np.random.default_rng(104)
+    ndof, nrec = 40, 25
+    mrdof = rng.integers(1, ndof + 1, size=(nrec, 3)).astype(np.int32)
+    mrdof[3, 1] = mrdof[7, 2]                   # a repeated DOF
+    resid = _cvals(rng, (nrec, 3))
+    rhs, iw = oracle.resid_drhs(ndof, 5, 0.09, mrdof, resid)
+    h = complex(oracle.hdiffer(5, 0.09))
+    want = np.zeros(ndof, np.complex128)
+    for r in range(nrec):
+        for i in range(3):
+            want[mrdof[r, i] - 1] += h * complex(resid[r, i])
+    assert iw == 0
+    assert np.abs(rhs - np.conj(want)).max() <= 1e-6 * np.abs(want).max()
+    mrdof[5, 0] = 0
+    mrdof[6, 2] = ndof + 1
+    _, iw = oracle.resid_drhs(ndof, 0, 0.09, mrdof, resid)
+    assert iw == 1
+
+
+def test_cadjlam_rhs_closed_form():
+    """CADJLAM's right-hand side: with the L2 norm, full residuals and equal
+    weights on the two horizontal components, the residual rotated back to
+    the model frame is W^2 (R(-az) OBS - EST) after the receiver phase
+    shift, placed at the receiver DOFs and conjugated; magnitude/phase input
+    (LUNWRAP) gives the same as the equivalent complex input."""
+    rng = np.random.default_rng(105)
+    ndof, nfreq, nrec, nsrc = 120, 3, 20, 2
+    mrdof = np.arange(1, 3 * nrec + 1, dtype=np.int32).reshape(nrec, 3) * 2
+    yrec = rng.uniform(-2e4, 2e4, nrec)
+    est = _cvals(rng, (nsrc, nrec, nfreq, 3))
+    obs = _cvals(rng, (nsrc, nrec, nfreq, 3))
+    wv = rng.uniform(0.5, 2.0, (nsrc, nrec, nfreq))
+    w = np.repeat(wv[..., None], 3, axis=-1).astype(np.float32)
+    ifreq, isrc, freq, py, az = 2, 2, 0.061, 0.9e-4, 28.0
+    rhs, iw = oracle.cadjlam_rhs(ndof, ifreq, isrc, freq, py, az, 2, 3, 0.0, 0.0, 0, False, yrec, mrdof, est,
+                                 obs, w)
+    assert iw == 0
+    th = np.deg2rad(az)
+    want = np.zeros(ndof, np.complex128)
+    for r in range(nrec):
+        e = est[isrc - 1, r, ifreq - 1].astype(np.complex128)
+        o = obs[isrc - 1, r, ifreq - 1].astype(np.complex128)
+        ob = np.array([np.cos(th) * o[0] + np.sin(th) * o[1], -np.sin(th) * o[0] + np.cos(th) * o[1], o[2]])
+        sh = np.exp(-1j * 2 * np.pi * freq * py * yrec[r])
+        res = float(w[isrc - 1, r, ifreq - 1, 0]) ** 2 * (ob - e) * sh
+        want[mrdof[r] - 1] += res
+    assert np.abs(rhs - np.conj(want)).max() <= 2e-6 * np.abs(want).max()
+    # magnitude/phase form of the same data
+    mp = lambda a: (np.abs(a) + 1j * np.angle(a)).astype(np.complex64)   # noqa: E731
+    rhs2, _ = oracle.cadjlam_rhs(ndof, ifreq, isrc, freq, py, az, 2, 3, 0.0, 0.0, 0, True, yrec, mrdof, mp(est),
+                                 mp(obs), w)
+    assert np.abs(rhs2 - rhs).max() <= 2e-6 * np.abs(rhs).max()
