@@ -254,6 +254,11 @@ struct ApplyShape {
     static constexpr int REM = NEN - 16 * (MT - 1);
     static constexpr int NR = (MT > 1 && REM <= 2) ? REM : 0;   // rows on VALU
     static constexpr int MTM = NR ? MT - 1 : MT;                 // row tiles on MFMA
+    // NEN = 4 KS - 3 (p2, p4, p6): the last k step holds one node.  Its five
+    // sub-block products into one accumulator are merged: the k index of the
+    // MFMA runs over sub-blocks 0..3 (A column NEN-1 of each, their operands
+    // of node NEN-1), and sub-block 4 takes a second MFMA: 2 instead of 5.
+    static constexpr bool KM = (NEN % 4 == 1);
 };
 
 // LDS bytes of k_apply for a shape.  PH: the nine blocks do not fit, so the
@@ -314,6 +319,11 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW), (REFINE ? 2 : 
     typedef ApplyShape<NL> S;
     typedef ApplyLds<NL, BT, REFINE> LD;
     constexpr int NEN = S::NEN, RS = S::RS, MT = S::MTM, KS = S::KS, NR = S::NR;
+    // merged last k step: with the three-component shapes (p2, p4) the
+    // extra selects push the 168-register budget into spills that cost more
+    // than the three MFMAs saved (p4: 3.15 -> 3.35 ms per step); the
+    // one-component launches of p6 have the room (66.5 -> 65.1 ms)
+    constexpr bool KM = S::KM && !REFINE && (CSEL >= 0);
     constexpr int NP = NEN * NEN;
     constexpr bool PRE = LD::PRE;
     constexpr int NC = (CSEL < 0) ? 3 : 1;
@@ -346,7 +356,7 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW), (REFINE ? 2 : 
     // fixed DOFs and for pairs beyond npair)
     auto load_x = [&](int tt, int s, OS *dst) {
         const int pr = tt * 8 + (col >> 1);
-        const int b = 4 * s + kg;
+        const int b = (KM && s == KS - 1) ? NEN - 1 : 4 * s + kg;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int dof = (b < NEN && pr < npair) ? sLM[b * 3 + d] : -1;
@@ -407,7 +417,7 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW), (REFINE ? 2 : 
             for (int r = 0; r < (NR > 0 ? NR : 1); ++r) rem[cc][r] = (OS)0;
         }
 #pragma unroll 1
-        for (int s = 0; s < KS; ++s) {
+        for (int s = 0; s < (KM ? KS - 1 : KS); ++s) {
             const OS xv[3] = {xn[0], xn[1], xn[2]};
             if (s + 1 < KS) load_x(t, s + 1, xn);
             else if (t + NW < ntile) load_x(t + NW, 0, xn);
@@ -477,6 +487,45 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW), (REFINE ? 2 : 
                         const OS av = REFINE ? (OS)((double)A[idx] + (double)Al[idx]) : (OS)A[idx];
                         rem[cc][r] += av * bv;
                     }
+                }
+            }
+        }
+        if (KM) {
+            // merged last step: every lane holds node NEN-1 of its column;
+            // lane group kg supplies sub-block j = kg (A column NEN-1 and that
+            // sub-block's operand: fa x + fb partner(x), exact for the five
+            // kinds), then sub-block 4 on the kg = 0 lanes
+            constexpr int kn = NEN - 1;
+            const OS xv[3] = {xn[0], xn[1], xn[2]};
+            if (t + NW < ntile) load_x(t + NW, 0, xn);
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                const int c = (CSEL < 0) ? cc : CSEL;
+                const SubBlk s0 = kSub[c][0], s1 = kSub[c][1], s2 = kSub[c][2], s3 = kSub[c][3],
+                             s4 = kSub[c][4];
+                const SubBlk sj = kg == 0 ? s0 : kg == 1 ? s1 : kg == 2 ? s2 : s3;
+                const OS xj = sj.xcomp == 0 ? xv[0] : sj.xcomp == 1 ? xv[1] : xv[2];
+                const OS pj = partner(xj);
+                const OS fa = sj.kind == 0 ? (OS)1 : sj.kind == 3 ? f3 : sj.kind == 4 ? f4 : (OS)0;
+                const OS fb = sj.kind == 1 ? f1 : sj.kind == 2 ? -f1 : (OS)0;
+                const OS bj = fa * xj + fb * pj;
+                const OS b4 = kg == 0 ? f4 * xv[s4.xcomp] : (OS)0;   // sub-block 4: -w^2 x
+                const BT *Aj = sB + ((CSEL < 0) ? sj.blk : kg) * NEN * RS;
+                const BT *A4 = sB + ((CSEL < 0) ? s4.blk : 4) * NEN * RS;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    const int arow = min(mt * 16 + col, NEN - 1);
+                    const BT aj = Aj[sj.trans ? (kn * RS + arow) : (arow * RS + kn)];
+                    const BT a4 = kg == 0 ? A4[s4.trans ? (kn * RS + arow) : (arow * RS + kn)] : (BT)0;
+                    acc[cc][mt] = MT_::mma(aj, (BT)bj, acc[cc][mt]);
+                    acc[cc][mt] = MT_::mma(a4, (BT)b4, acc[cc][mt]);
+                }
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const int row = MT * 16 + r;
+                    const OS aj = (OS)Aj[sj.trans ? (kn * RS + row) : (row * RS + kn)];
+                    const OS a4 = kg == 0 ? (OS)A4[s4.trans ? (kn * RS + row) : (row * RS + kn)] : (OS)0;
+                    rem[cc][r] += aj * bj + a4 * b4;
                 }
             }
         }
