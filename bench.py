@@ -255,7 +255,10 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     t0 = time.perf_counter()
     fm, _ = cfmat_host(mi, inv, float(freq[0]), float(py[0]), wave)
     k_ms = L.f25_last_kernel_ms(b"cfmat")
-    kern["cfmat_dist"] = {"kernel_ms": round(k_ms, 3), "columns_per_s": round(inv.nnpgl / (k_ms * 1e-3), 1)}
+    # one element's DKM_VPVS product per column: (72 NEN + 40) flops per quadrature point
+    cfl = (72.0 * nen + 40.0) * nq * inv.nnpgl
+    kern["cfmat_dist"] = {"kernel_ms": round(k_ms, 3), "columns_per_s": round(inv.nnpgl / (k_ms * 1e-3), 1),
+                          "roofline": roof("fp64", cfl / (k_ms * 1e-3), "TFLOP/s")}
     # PCHADJ's J = G^T F: 60 receivers x 3 components
     nrhs = 180
     gt = torch.randn(ndof, nrhs, dtype=torch.complex64, device="cuda")

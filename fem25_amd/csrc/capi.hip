@@ -1261,8 +1261,24 @@ static int ensure_dab_part(f25_mesh *m) {
 }
 
 static size_t dab_lds(const f25_mesh *m) {
-    const int nee = 3 * m->tab.nen, nq = m->tab.nq;
-    return (size_t)(2 * nee + 18 * nq) * sizeof(double2) + 5 * (size_t)nq * sizeof(double);
+    const int nee = 3 * m->tab.nen, nq = m->tab.nq, qg = std::max(1, 256 / nee);
+    return (size_t)(2 * nee + 18 * nq + 2 * qg * nee) * sizeof(double2) + 4 * (size_t)nq * sizeof(double);
+}
+
+// k_dab for the mesh's order (the element loops are compiled per order)
+typedef void (*DabKernel)(int, int, int, int, double, const int *, const double *, const double *, const double *,
+                          const double *, const int *, const double *, const double *, const int *, const float2 *,
+                          const float2 *, double2 *, double2 *, const float2 *, const float2 *, const int *,
+                          const int *, const int *, float *, float *, int *);
+static DabKernel dab_kernel(int nl) {
+    switch (nl) {
+        case 3: return k_dab<3>;
+        case 4: return k_dab<4>;
+        case 5: return k_dab<5>;
+        case 6: return k_dab<6>;
+        case 7: return k_dab<7>;
+        default: return nullptr;
+    }
 }
 
 static int dab_prepare(f25_mesh *m) {
@@ -1271,8 +1287,10 @@ static int dab_prepare(f25_mesh *m) {
     if (m->inv.nvinv != 1) FAIL(F25_ERR_ARG, "NVINV = 1 (PP or SS) only");
     if (dab_lds(m) > 160 * 1024) FAIL(F25_ERR_ARG, "element order too high for the LDS tiling");
     HIPCHK(hipSetDevice(m->dev));
+    if (!dab_kernel(m->tab.nl)) FAIL(F25_ERR_ARG, "unsupported order");
     if (dab_lds(m) > 64 * 1024)
-        HIPCHK(hipFuncSetAttribute((const void *)k_dab, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dab_lds(m)));
+        HIPCHK(hipFuncSetAttribute((const void *)dab_kernel(m->tab.nl), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)dab_lds(m)));
     return 0;
 }
 
@@ -1302,8 +1320,8 @@ static int dab_launch_columns(f25_mesh *m, double kv, int nvec, const std::vecto
     int r = upload(&de, elem.data(), elem.size(), m->st);
     if (!r) r = upload(&dc, coef.data(), coef.size(), m->st);
     if (!r) {
-        hipLaunchKernelGGL(k_dab, dim3(inv.nnpgl), dim3(256), dab_lds(m), m->st, 1, m->tab.nen, m->tab.nq1,
-                           inv.nnpgl, nvec, inv.lisp ? 0 : 1, kv, de, dc, m->d_shl, m->d_wq, m->d_xq, m->d_ieng,
+        hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(inv.nnpgl), dim3(256), dab_lds(m), m->st, 1,
+                           inv.nnpgl, nvec, inv.lisp ? 0 : 1, kv, de, dc, m->d_shlT, m->d_wq, m->d_xq, m->d_ieng,
                            m->d_xl, m->d_zl, m->d_lm, x0, x1, (double2 *)nullptr, (double2 *)nullptr, z0, z1,
                            m->d_mygrad, m->d_jcsc, m->d_icsc, p0, p1, m->d_err);
         hipError_t e = hipGetLastError();
@@ -1362,8 +1380,8 @@ extern "C" int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float 
         for (size_t col = 0; col + 1 < inv.dab_cptr.size(); ++col) {
             const int o = inv.dab_cptr[col], n = inv.dab_cptr[col + 1] - o;
             if (n <= 0) continue;
-            hipLaunchKernelGGL(k_dab, dim3(n), dim3(256), dab_lds(m), m->st, 0, m->tab.nen, m->tab.nq1, n, 2,
-                               inv.lisp ? 0 : 1, kv, de + o, dc + 4 * (size_t)o, m->d_shl, m->d_wq, m->d_xq,
+            hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(n), dim3(256), dab_lds(m), m->st, 0, n, 2,
+                               inv.lisp ? 0 : 1, kv, de + o, dc + 4 * (size_t)o, m->d_shlT, m->d_wq, m->d_xq,
                                m->d_ieng, m->d_xl, m->d_zl, m->d_lm, du, dl, ya, yb, (const float2 *)nullptr,
                                (const float2 *)nullptr, m->d_mygrad, m->d_jcsc, m->d_icsc, (float *)nullptr,
                                (float *)nullptr, m->d_err);

@@ -1258,7 +1258,10 @@ __global__ __launch_bounds__(256) void k_cfmat(int nen, int nq1, int nnpgl, cons
                 UZ[cc].x += nz * uu.x; UZ[cc].y += nz * uu.y;
             }
         }
-        double w = wq[ix] * wq[iz] * g.det * g.sf[kc];
+        // the corner's hat function, selected without indexing the array
+        // (a runtime index would put the geometry in scratch memory)
+        const double sfk = kc == 0 ? g.sf[0] : kc == 1 ? g.sf[1] : kc == 2 ? g.sf[2] : g.sf[3];
+        double w = wq[ix] * wq[iz] * g.det * sfk;
         double P = w * cp, Sq = w * cs, Q = P - 2.0 * Sq, k2 = kv * kv;
         double *F = sF + 12 * q;
         // row comp 1: Nx*(P u1x + Q u3z - ky Q u2) + Nz*Sq(u1z + u3x) + N*(ky Sq u2x + ky^2 Sq u1)
@@ -1356,10 +1359,11 @@ __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *
 //          p_v(MYGRAD(c)) = -sum over the element's rows Re(conj(M x_v) z_v),
 //          every row checked against column c's CSC pattern.
 // Flux layout per (vector, q): [comp][N, Nx, Nz] complex.
-__global__ __launch_bounds__(256) void k_dab(int mode, int nen, int nq1, int nitems, int nvec, int liss,
+template <int NL>
+__global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, int liss,
                                              double kv, const int *__restrict__ item_elem,
                                              const double *__restrict__ item_coef,
-                                             const double *__restrict__ shl, const double *__restrict__ wq,
+                                             const double *__restrict__ shlT, const double *__restrict__ wq,
                                              const double *__restrict__ xq, const int *__restrict__ ieng,
                                              const double *__restrict__ xl, const double *__restrict__ zl,
                                              const int *__restrict__ lm, const float2 *__restrict__ x0,
@@ -1376,10 +1380,12 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nen, int nq1, int nit
     const int e = item_elem[it];
     const double c0 = item_coef[4 * it], c1 = item_coef[4 * it + 1], c2 = item_coef[4 * it + 2],
                  c3 = item_coef[4 * it + 3];
-    const int nq = nq1 * nq1, nee = 3 * nen;
+    constexpr int nen = NL * NL, nq1 = 2 * NL + 3, nq = nq1 * nq1, nee = 3 * nen;
+    const int QG = max(1, (int)blockDim.x / nee);              // q groups of the projection
     double2 *su = reinterpret_cast<double2 *>(sm);           // [v][3 nen]
     double2 *sF = su + 2 * nee;                               // [v][q][9]
-    double *sG = reinterpret_cast<double *>(sF + 2 * 9 * nq); // [q][5]
+    double2 *sP = sF + 2 * 9 * nq;                            // [v][group][row] partial projections
+    double *sG = reinterpret_cast<double *>(sP + 2 * QG * nee);   // [q][4]: rx, sx, rz, sz
     const int *ie = ieng + 4 * e;
     const int *L = lm + (size_t)e * nee;
     for (int i = threadIdx.x; i < nee; i += blockDim.x) {
@@ -1398,16 +1404,20 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nen, int nq1, int nit
         if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) *err = 2;
         const double D = wq[ix] * wq[iz] * g.det * (c0 * g.sf[0] + c1 * g.sf[1] + c2 * g.sf[2] + c3 * g.sf[3]);
         const double P = liss ? 0.0 : D, Q = liss ? -2.0 * D : D, S = liss ? D : 0.0, k2 = kv * kv;
-        double *G = sG + 5 * q;
-        G[0] = g.dzdeta; G[1] = g.dzdxi; G[2] = g.dxdeta; G[3] = g.dxdxi; G[4] = g.det;
-        for (int v = 0; v < nvec; ++v) {
+        const double rx = g.dzdeta / g.det, sx = -g.dzdxi / g.det, rz = -g.dxdeta / g.det, sz = g.dxdxi / g.det;
+        double *G = sG + 4 * q;
+        G[0] = rx; G[1] = sx; G[2] = rz; G[3] = sz;
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            if (v >= nvec) break;
             double2 U[3] = {}, UX[3] = {}, UZ[3] = {};
             const double2 *x = su + v * nee;
+#pragma unroll 5
             for (int a = 0; a < nen; ++a) {
-                const double *s = shl + ((size_t)q * nen + a) * 3;
-                const double n = s[2];
-                const double nx = (s[0] * g.dzdeta - s[1] * g.dzdxi) / g.det;
-                const double nz = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / g.det;
+                // shlT[(a*3 + c)*nq + q]: adjacent lanes read adjacent words
+                const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
+                const double n = shlT[(size_t)(a * 3 + 2) * nq + q];
+                const double nx = s0 * rx + s1 * sx, nz = s0 * rz + s1 * sz;
 #pragma unroll
                 for (int cc = 0; cc < 3; ++cc) {
                     const double2 w = x[3 * a + cc];
@@ -1436,24 +1446,65 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nen, int nq1, int nit
         }
     }
     __syncthreads();
+    // projection on the test functions: QG groups of threads split the q
+    // range of every row, then one pass adds the groups
+    {
+        const int i = threadIdx.x % nee, grp = threadIdx.x / nee;
+        if (grp < QG) {
+            const int a = i / 3, cc = i % 3;
+            double2 val[2] = {};
+#pragma unroll 4
+            for (int q = grp; q < nq; q += QG) {
+                const double *G = sG + 4 * q;
+                const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
+                const double n = shlT[(size_t)(a * 3 + 2) * nq + q];
+                const double nx = s0 * G[0] + s1 * G[1], nz = s0 * G[2] + s1 * G[3];
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    if (v >= nvec) break;
+                    const double2 *F = sF + ((size_t)v * nq + q) * 9 + 3 * cc;
+                    val[v].x += n * F[0].x + nx * F[1].x + nz * F[2].x;
+                    val[v].y += n * F[0].y + nx * F[1].y + nz * F[2].y;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 2; ++v) if (v < nvec) sP[((size_t)v * QG + grp) * nee + i] = val[v];
+        }
+        // rows beyond the block size (nee > blockDim, one group)
+        for (int r = threadIdx.x + blockDim.x; r < nee; r += blockDim.x) {
+            const int a = r / 3, cc = r % 3;
+            double2 val[2] = {};
+            for (int q = 0; q < nq; ++q) {
+                const double *G = sG + 4 * q;
+                const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
+                const double n = shlT[(size_t)(a * 3 + 2) * nq + q];
+                const double nx = s0 * G[0] + s1 * G[1], nz = s0 * G[2] + s1 * G[3];
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    if (v >= nvec) break;
+                    const double2 *F = sF + ((size_t)v * nq + q) * 9 + 3 * cc;
+                    val[v].x += n * F[0].x + nx * F[1].x + nz * F[2].x;
+                    val[v].y += n * F[0].y + nx * F[1].y + nz * F[2].y;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 2; ++v) if (v < nvec) sP[(size_t)v * QG * nee + r] = val[v];
+        }
+    }
+    __syncthreads();
     int i1 = 0, i2 = 0;
     if (mode == 1) { i1 = jcsc[it] - 1; i2 = jcsc[it + 1] - 1; }
     double acc[2] = {0.0, 0.0};
     for (int i = threadIdx.x; i < nee; i += blockDim.x) {
-        const int a = i / 3, cc = i % 3;
         const int dof = L[i];
         if (dof < 0) continue;
         double2 val[2] = {};
-        for (int q = 0; q < nq; ++q) {
-            const double *s = shl + ((size_t)q * nen + a) * 3;
-            const double *G = sG + 5 * q;
-            const double n = s[2];
-            const double nx = (s[0] * G[0] - s[1] * G[1]) / G[4];
-            const double nz = -(s[0] * G[2] - s[1] * G[3]) / G[4];
-            for (int v = 0; v < nvec; ++v) {
-                const double2 *F = sF + ((size_t)v * nq + q) * 9 + 3 * cc;
-                val[v].x += n * F[0].x + nx * F[1].x + nz * F[2].x;
-                val[v].y += n * F[0].y + nx * F[1].y + nz * F[2].y;
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            if (v >= nvec) break;
+            for (int gi = 0; gi < (i < QG * nee ? QG : 1); ++gi) {
+                const double2 pv = sP[((size_t)v * QG + gi) * nee + i];
+                val[v].x += pv.x; val[v].y += pv.y;
             }
         }
         if (mode == 0) {
@@ -1476,6 +1527,7 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nen, int nq1, int nit
         }
     }
     if (mode == 0) return;
+#pragma unroll
     for (int v = 0; v < 2; ++v)
         for (int o = 32; o > 0; o >>= 1) acc[v] += __shfl_xor(acc[v], o);
     const int wv = threadIdx.x >> 6;
