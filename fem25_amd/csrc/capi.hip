@@ -115,8 +115,8 @@ struct f25_mesh_s {
     int *d_pmap = nullptr;   // pseudo-Hessian patch positions
     int ncon_anchor = 0;
     // inversion (CFMAT/ADJFLAM)
-    int *d_col_elem = nullptr, *d_col_corner = nullptr, *d_mygrad = nullptr, *d_jcsc = nullptr, *d_icsc = nullptr;
-    double *d_col_cp = nullptr, *d_col_cs = nullptr, *d_ew = nullptr;
+    int *d_mygrad = nullptr, *d_jcsc = nullptr, *d_icsc = nullptr;
+    double *d_ew = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // (omega, ky) staging, grown on demand
     double *d_omky = nullptr;
@@ -374,8 +374,8 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_xq); dfree(m->d_cdom); dfree(m->d_cnp); dfree(m->d_rho); dfree(m->d_lam); dfree(m->d_mu);
     dfree(m->d_blk); dfree(m->d_blk32); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_colmax); dfree(m->d_dblk); dfree(m->d_irptr);
     dfree(m->d_jcptr); dfree(m->d_comp); dfree(m->d_v0); dfree(m->d_v1); dfree(m->d_v2);
-    dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_pmap); dfree(m->d_qp); dfree(m->d_qs); dfree(m->d_col_elem); dfree(m->d_col_corner);
-    dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_col_cp); dfree(m->d_col_cs);
+    dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_pmap); dfree(m->d_qp); dfree(m->d_qs);
+    dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc);
     dfree(m->d_ew);
     dfree(m->d_omky);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -879,7 +879,6 @@ extern "C" int f25_gradient(f25_mesh *m, int ntrip, const double *omega, const d
 static int inv_upload(f25_mesh *m) {
     m->have_inv = false;
     dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_ew);
-    dfree(m->d_col_elem); dfree(m->d_col_corner); dfree(m->d_col_cp); dfree(m->d_col_cs);
     TRY(upload(&m->d_mygrad, m->inv.mygrad.data(), m->inv.mygrad.size(), m->st));
     TRY(upload(&m->d_jcsc, m->inv.jcsc.data(), m->inv.jcsc.size(), m->st));
     TRY(upload(&m->d_icsc, m->inv.icsc.data(), m->inv.icsc.size(), m->st));
@@ -996,64 +995,6 @@ static int last_elements(const f25_mesh *m, std::vector<int> &ce, std::vector<in
             if (m->ieng[4 * e + k] - 1 == a) ck[c] = k;
     }
     return 0;
-}
-
-// per-column element/corner/collocation constants (model dependent)
-static int cfmat_columns(f25_mesh *m) {
-    const InvInfo &inv = m->inv;
-    const int n = inv.nnpgl;
-    std::vector<int> ce, ck;
-    TRY(last_elements(m, ce, ck));
-    std::vector<double> cp(n), cs(n);
-    const double v2 = inv.vpvs * inv.vpvs;
-    for (int c = 0; c < n; ++c) {
-        const int a = inv.node_of_col[c];
-        const double alpha = std::sqrt((m->lam[a] + 2.0 * m->mu[a]) / m->rho[a]);
-        const double beta = std::sqrt(m->mu[a] / m->rho[a]);
-        if (inv.lisp) { cp[c] = 2.0 * alpha; cs[c] = 2.0 * alpha / v2; }
-        else { cs[c] = 2.0 * beta; cp[c] = 2.0 * beta * v2; }
-        if (ck[c] < 0) { ck[c] = 0; cp[c] = cs[c] = 0.0; }
-    }
-    dfree(m->d_col_elem); dfree(m->d_col_corner); dfree(m->d_col_cp); dfree(m->d_col_cs);
-    TRY(upload(&m->d_col_elem, ce.data(), n, m->st));
-    TRY(upload(&m->d_col_corner, ck.data(), n, m->st));
-    TRY(upload(&m->d_col_cp, cp.data(), n, m->st));
-    TRY(upload(&m->d_col_cs, cs.data(), n, m->st));
-    return 0;
-}
-
-extern "C" int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *wave, void *fmat) {
-    if (!m || !wave || !fmat) FAIL(F25_ERR_ARG, "null argument");
-    if (!m->have_inv) FAIL(F25_ERR_STATE, "f25_inv_setup has not been called");
-    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
-    HIPCHK(hipSetDevice(m->dev));
-    TRY(cfmat_columns(m));
-    const double kv = 6.2831853071795862 * freq * py;
-    float2 *dw = nullptr, *df = nullptr;
-    TRY(upload(&dw, (const float2 *)wave, m->g.ndof, m->st));
-    TRY(dalloc(&df, m->inv.nz_fdist));
-    const int nen = m->tab.nen, nq = m->tab.nq;
-    size_t lds = 3 * nen * sizeof(double2) + 17 * (size_t)nq * sizeof(double);
-    (void)hipEventRecord(m->ev0, m->st);
-    hipLaunchKernelGGL(k_cfmat, dim3(m->inv.nnpgl), dim3(256), lds, m->st, nen, m->tab.nq1, m->inv.nnpgl,
-                       (const int *)nullptr, m->d_col_elem, m->d_col_corner, m->d_col_cp, m->d_col_cs,
-                       m->d_shl, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, m->d_ew,
-                       m->d_jcsc, m->d_icsc, kv, dw, df, m->d_err);
-    (void)hipEventRecord(m->ev1, m->st);
-    hipError_t e = hipGetLastError();
-    int r = 0;
-    if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
-    if (!r) r = check_err(m);
-    float cms = 0.f;
-    if (!r && hipEventElapsedTime(&cms, m->ev0, m->ev1) == hipSuccess) record_ms("cfmat", cms);
-    if (!r) {
-        e = hipMemcpyAsync(fmat, df, m->inv.nz_fdist * sizeof(float2), hipMemcpyDeviceToHost, m->st);
-        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
-        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
-    }
-    dfree(dw);
-    dfree(df);
-    return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -1266,20 +1207,23 @@ static size_t dab_lds(const f25_mesh *m) {
 }
 
 // k_dab for the mesh's order (the element loops are compiled per order)
-typedef void (*DabKernel)(int, int, int, int, double, const int *, const double *, const double *, const double *,
-                          const double *, const int *, const double *, const double *, const int *, const float2 *,
-                          const float2 *, double2 *, double2 *, const float2 *, const float2 *, const int *,
-                          const int *, const int *, float *, float *, int *);
-static DabKernel dab_kernel(int nl) {
+typedef void (*DabKernel)(int, int, int, double, double, double, const int *, const double *, const double *,
+                          const double *, const double *, const int *, const double *, const double *, const int *,
+                          const float2 *, const float2 *, double2 *, double2 *, const float2 *, const float2 *,
+                          const int *, const int *, const int *, float *, float *, float2 *, int *);
+template <bool IMAG>
+static DabKernel dab_kernel_t(int nl) {
     switch (nl) {
-        case 3: return k_dab<3>;
-        case 4: return k_dab<4>;
-        case 5: return k_dab<5>;
-        case 6: return k_dab<6>;
-        case 7: return k_dab<7>;
+        case 3: return k_dab<3, IMAG>;
+        case 4: return k_dab<4, IMAG>;
+        case 5: return k_dab<5, IMAG>;
+        case 6: return k_dab<6, IMAG>;
+        case 7: return k_dab<7, IMAG>;
         default: return nullptr;
     }
 }
+// DABMKISOB25 form (imaginary ky coupling) or DKM_VPVS form (real)
+static DabKernel dab_kernel(int nl, bool imag = true) { return imag ? dab_kernel_t<true>(nl) : dab_kernel_t<false>(nl); }
 
 static int dab_prepare(f25_mesh *m) {
     if (!m->have_inv) FAIL(F25_ERR_STATE, "f25_inv_setup has not been called");
@@ -1289,8 +1233,9 @@ static int dab_prepare(f25_mesh *m) {
     HIPCHK(hipSetDevice(m->dev));
     if (!dab_kernel(m->tab.nl)) FAIL(F25_ERR_ARG, "unsupported order");
     if (dab_lds(m) > 64 * 1024)
-        HIPCHK(hipFuncSetAttribute((const void *)dab_kernel(m->tab.nl), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)dab_lds(m)));
+        for (int im = 0; im < 2; ++im)
+            HIPCHK(hipFuncSetAttribute((const void *)dab_kernel(m->tab.nl, im == 1),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dab_lds(m)));
     return 0;
 }
 
@@ -1321,9 +1266,9 @@ static int dab_launch_columns(f25_mesh *m, double kv, int nvec, const std::vecto
     if (!r) r = upload(&dc, coef.data(), coef.size(), m->st);
     if (!r) {
         hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(inv.nnpgl), dim3(256), dab_lds(m), m->st, 1,
-                           inv.nnpgl, nvec, inv.lisp ? 0 : 1, kv, de, dc, m->d_shlT, m->d_wq, m->d_xq, m->d_ieng,
+                           inv.nnpgl, nvec, inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de, dc, m->d_shlT, m->d_wq, m->d_xq, m->d_ieng,
                            m->d_xl, m->d_zl, m->d_lm, x0, x1, (double2 *)nullptr, (double2 *)nullptr, z0, z1,
-                           m->d_mygrad, m->d_jcsc, m->d_icsc, p0, p1, m->d_err);
+                           m->d_mygrad, m->d_jcsc, m->d_icsc, p0, p1, (float2 *)nullptr, m->d_err);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
     }
@@ -1335,6 +1280,59 @@ static int dab_download(f25_mesh *m, float *dst, const float *src, size_t n) {
     hipError_t e = hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToHost, m->st);
     if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
     return 0;
+}
+
+// CFMAT_DIST (hessloc.f90:1-133 driving vforce25.f:1-369): per local
+// column, the last connected element's DKM_VPVS product on WAVE, weighted by
+// ELEM_WTS and written in CSC order.  DKM_VPVS's moduli are those of the
+// derivative field DVP, DVS = DVP / VPVS^2 (PP) or DVP = VPVS^2 DVS (SS)
+// (vforce25.f:164-179), DVP or DVS = 2 Vp or 2 Vs N_k (DJACAB).
+extern "C" int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *wave, void *fmat) {
+    if (!m || !wave || !fmat) FAIL(F25_ERR_ARG, "null argument");
+    TRY(dab_prepare(m));
+    const InvInfo &inv = m->inv;
+    const int n = inv.nnpgl;
+    std::vector<int> ce, ck;
+    TRY(last_elements(m, ce, ck));
+    std::vector<double> coef(4 * (size_t)n, 0.0);
+    const double v2 = inv.vpvs * inv.vpvs;
+    for (int c = 0; c < n; ++c) {
+        if (ck[c] < 0) continue;   // the element does not hold the anchor: zero column
+        const int a = inv.node_of_col[c];
+        const double vel = inv.lisp ? std::sqrt((m->lam[a] + 2.0 * m->mu[a]) / m->rho[a])
+                                    : std::sqrt(m->mu[a] / m->rho[a]);
+        coef[4 * c + ck[c]] = inv.elem_wts[ce[c]] * 2.0 * vel;
+    }
+    const double kp = inv.lisp ? 1.0 : v2, ks = inv.lisp ? 1.0 / v2 : 1.0;
+    const double kv = 6.2831853071795862 * freq * py;
+    float2 *dw = nullptr, *df = nullptr;
+    int *de = nullptr;
+    double *dc = nullptr;
+    int r = upload(&dw, (const float2 *)wave, m->g.ndof, m->st);
+    if (!r) r = dalloc(&df, inv.nz_fdist);
+    if (!r) r = upload(&de, ce.data(), ce.size(), m->st);
+    if (!r) r = upload(&dc, coef.data(), coef.size(), m->st);
+    if (!r) {
+        (void)hipEventRecord(m->ev0, m->st);
+        hipLaunchKernelGGL(dab_kernel(m->tab.nl, false), dim3(n), dim3(256), dab_lds(m), m->st, 2, n, 1, kp, ks, kv,
+                           de, dc, m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, dw,
+                           (const float2 *)nullptr, (double2 *)nullptr, (double2 *)nullptr, (const float2 *)nullptr,
+                           (const float2 *)nullptr, m->d_mygrad, m->d_jcsc, m->d_icsc, (float *)nullptr,
+                           (float *)nullptr, df, m->d_err);
+        (void)hipEventRecord(m->ev1, m->st);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    if (!r) r = check_err(m);
+    float cms = 0.f;
+    if (!r && hipEventElapsedTime(&cms, m->ev0, m->ev1) == hipSuccess) record_ms("cfmat", cms);
+    if (!r) {
+        hipError_t e = hipMemcpyAsync(fmat, df, inv.nz_fdist * sizeof(float2), hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(dw); dfree(df); dfree(de); dfree(dc);
+    return r;
 }
 
 extern "C" int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float *v, const void *adjlam,
@@ -1381,10 +1379,10 @@ extern "C" int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float 
             const int o = inv.dab_cptr[col], n = inv.dab_cptr[col + 1] - o;
             if (n <= 0) continue;
             hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(n), dim3(256), dab_lds(m), m->st, 0, n, 2,
-                               inv.lisp ? 0 : 1, kv, de + o, dc + 4 * (size_t)o, m->d_shlT, m->d_wq, m->d_xq,
-                               m->d_ieng, m->d_xl, m->d_zl, m->d_lm, du, dl, ya, yb, (const float2 *)nullptr,
-                               (const float2 *)nullptr, m->d_mygrad, m->d_jcsc, m->d_icsc, (float *)nullptr,
-                               (float *)nullptr, m->d_err);
+                               inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de + o, dc + 4 * (size_t)o,
+                               m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, du, dl, ya, yb,
+                               (const float2 *)nullptr, (const float2 *)nullptr, m->d_mygrad, m->d_jcsc,
+                               m->d_icsc, (float *)nullptr, (float *)nullptr, (float2 *)nullptr, m->d_err);
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }

@@ -666,13 +666,16 @@ __device__ inline double2 vcmplx(double freq, double freq0, double v, double q) 
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline double2 cdiv1(double2 z) { double d = z.x * z.x + z.y * z.y; return make_double2(z.x / d, -z.y / d); }
 
-__device__ inline double2 zdampfn(double rn, double vp, double delta, const PmlPar &p) {
-    double dmax = (p.dmax > 0.0) ? p.dmax : -3.0 * vp * log(p.rcoeff) / (2.0 * delta);
+// (the parameters are taken by value: a reference to the kernel's argument
+// struct would copy it to scratch memory)
+__device__ inline double2 zdampfn(double rn, double vp, double delta, double pdmax, double rcoeff, double fcent,
+                                  double omega, double kappa) {
+    double dmax = (pdmax > 0.0) ? pdmax : -3.0 * vp * log(rcoeff) / (2.0 * delta);
     double d = dmax * (rn / delta) * (rn / delta);
-    double alpha = M_PI * p.fcent * (1.0 - rn / delta);
+    double alpha = M_PI * fcent * (1.0 - rn / delta);
     // kappa + |d| / (alpha + i omega), then its reciprocal (cjac.f:849-851)
-    double2 t = cdiv1(make_double2(alpha, p.omega));
-    return cdiv1(make_double2(p.kappa + fabs(d) * t.x, fabs(d) * t.y));
+    double2 t = cdiv1(make_double2(alpha, omega));
+    return cdiv1(make_double2(kappa + fabs(d) * t.x, fabs(d) * t.y));
 }
 
 // Element matrices integrated per call and added into CSR order:
@@ -715,7 +718,9 @@ __global__ void k_pml(int nen, int nq1, int ncol_elem, const int *__restrict__ o
         const int pr = p0 + threadIdx.x;
         const int a = pr / nen, b = pr % nen;
         double2 acc[9], me[3];
+#pragma unroll
         for (int i = 0; i < 9; ++i) acc[i] = make_double2(0.0, 0.0);
+#pragma unroll
         for (int i = 0; i < 3; ++i) me[i] = make_double2(0.0, 0.0);
         for (int q0 = 0; q0 < nq; q0 += QC) {
             __syncthreads();
@@ -734,6 +739,7 @@ __global__ void k_pml(int nen, int nq1, int ncol_elem, const int *__restrict__ o
                 if (aa == 0) {
                     double w = wq[ix] * wq[iz] * g.det, r = 0, l = 0, m = 0, xdep = 0, zdep = 0, vel = 0;
                     double vsv = 0, qpv = 0, qsv = 0;
+#pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         int gi = ie[k] - 1;
                         r += g.sf[k] * rho[gi];
@@ -750,8 +756,13 @@ __global__ void k_pml(int nen, int nq1, int ncol_elem, const int *__restrict__ o
                             qsv += g.sf[k] * qs[gi];
                         }
                     }
-                    double2 ex = xdep > 2.22e-16 ? zdampfn(xdep, vel, par.xwidth, par) : make_double2(1.0, 0.0);
-                    double2 ez = zdep > 2.22e-16 ? zdampfn(zdep, vel, par.zwidth, par) : make_double2(1.0, 0.0);
+                    const double2 one = make_double2(1.0, 0.0);
+                    double2 ex = xdep > 2.22e-16
+                                     ? zdampfn(xdep, vel, par.xwidth, par.dmax, par.rcoeff, par.fcent, par.omega, par.kappa)
+                                     : one;
+                    double2 ez = zdep > 2.22e-16
+                                     ? zdampfn(zdep, vel, par.zwidth, par.dmax, par.rcoeff, par.fcent, par.omega, par.kappa)
+                                     : one;
                     double *o = sq + qq * QS;
                     if (qp) {
                         const double2 zp = vcmplx(par.freq, par.freq0, vel, qpv);
@@ -814,7 +825,10 @@ __global__ void k_pml(int nen, int nq1, int ncol_elem, const int *__restrict__ o
             acc[4].x += me[1].x; acc[4].y += me[1].y;
             acc[8].x += me[2].x; acc[8].y += me[2].y;
             const int *L = lm + (size_t)e * 3 * nen;
+            // unrolled, so that acc stays in registers
+#pragma unroll
             for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
                 for (int cj = 0; cj < 3; ++cj) {
                     int rd = L[a * 3 + ci], cd = L[b * 3 + cj];
                     if (rd < 0 || cd < 0) continue;
@@ -1206,127 +1220,6 @@ __global__ void k_fdiag(int nnpg, int ncon, int nee, const int *__restrict__ mco
     if (threadIdx.x < 3) diag[3 * a + threadIdx.x] += red[threadIdx.x];
 }
 
-// ---------------------------------------------------------------------------
-// Reference virtual-force columns (VFORCE25 through CFMAT_DIST, NVINV = 1).
-// Column of anchor a: element e = the last entry of MCONN(a,:) (VFORCE25
-// clears FORCEV inside its element loop, vforce25.f:252-256), corner k with
-// IENG(k,e) = a, collocation DVP = 2 alpha_a N_k, DVS = DVP / VPVS^2 ('PP')
-// or DVS = 2 beta_a N_k, DVP = DVS VPVS^2 ('SS') (djac.f:94-103,
-// vforce25.f:164-179).  DKM_VPVS (dstiff25.f:570-656) applied to u_e
-// matrix-free at the quadrature points, times ELEM_WTS(e).
-__global__ __launch_bounds__(256) void k_cfmat(int nen, int nq1, int nnpgl, const int *__restrict__ col_node,
-                        const int *__restrict__ col_elem, const int *__restrict__ col_corner,
-                        const double *__restrict__ col_cp, const double *__restrict__ col_cs,
-                        const double *__restrict__ shl, const double *__restrict__ wq,
-                        const double *__restrict__ xq, const int *__restrict__ ieng,
-                        const double *__restrict__ xl, const double *__restrict__ zl,
-                        const int *__restrict__ lm, const double *__restrict__ ew,
-                        const int *__restrict__ jcsc, const int *__restrict__ icsc, double kv,
-                        const float2 *__restrict__ wave, float2 *__restrict__ fmat,
-                        int *__restrict__ err) {
-    extern __shared__ double sm[];
-    const int c = blockIdx.x;
-    if (c >= nnpgl) return;
-    const int e = col_elem[c], kc = col_corner[c];
-    const double cp = col_cp[c], cs = col_cs[c];
-    const int nq = nq1 * nq1;
-    double2 *su = reinterpret_cast<double2 *>(sm);     // 3*nen
-    double *sF = reinterpret_cast<double *>(su + 3 * nen);   // per q: 12 flux values
-    double *sG = sF + 12 * nq;                               // per q: geometry (det, dxdxi, dxdeta, dzdxi, dzdeta)
-    const int *ie = ieng + 4 * e;
-    const int *L = lm + (size_t)e * 3 * nen;
-    for (int i = threadIdx.x; i < 3 * nen; i += blockDim.x) {
-        int d = L[i];
-        float2 v = d >= 0 ? wave[d] : make_float2(0.f, 0.f);
-        su[i] = make_double2(v.x, v.y);
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-        int ix = q % nq1, iz = q / nq1;
-        Geo g;
-        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) *err = 2;
-        double2 U[3] = {}, UX[3] = {}, UZ[3] = {};
-        for (int a = 0; a < nen; ++a) {
-            const double *s = shl + ((size_t)q * nen + a) * 3;
-            double n = s[2];
-            double nx = (s[0] * g.dzdeta - s[1] * g.dzdxi) / g.det;
-            double nz = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / g.det;
-            for (int cc = 0; cc < 3; ++cc) {
-                double2 uu = su[a * 3 + cc];
-                U[cc].x += n * uu.x; U[cc].y += n * uu.y;
-                UX[cc].x += nx * uu.x; UX[cc].y += nx * uu.y;
-                UZ[cc].x += nz * uu.x; UZ[cc].y += nz * uu.y;
-            }
-        }
-        // the corner's hat function, selected without indexing the array
-        // (a runtime index would put the geometry in scratch memory)
-        const double sfk = kc == 0 ? g.sf[0] : kc == 1 ? g.sf[1] : kc == 2 ? g.sf[2] : g.sf[3];
-        double w = wq[ix] * wq[iz] * g.det * sfk;
-        double P = w * cp, Sq = w * cs, Q = P - 2.0 * Sq, k2 = kv * kv;
-        double *F = sF + 12 * q;
-        // row comp 1: Nx*(P u1x + Q u3z - ky Q u2) + Nz*Sq(u1z + u3x) + N*(ky Sq u2x + ky^2 Sq u1)
-        // row comp 2: N*(ky Q (u1x + u3z) + ky^2 P u2) + Nx*Sq(u2x - ky u1) + Nz*Sq(u2z - ky u3)
-        // row comp 3: Nz*(Q u1x + P u3z - ky Q u2) + Nx*Sq(u1z + u3x) + N*(ky Sq u2z + ky^2 Sq u3)
-        // stored as (re, im) for [comp][N, Nx, Nz]
-        double2 t;
-        t = make_double2(P * UX[0].x + Q * UZ[2].x - kv * Q * U[1].x, P * UX[0].y + Q * UZ[2].y - kv * Q * U[1].y);
-        F[0] = t.x; F[1] = t.y;                                     // comp1, Nx
-        F[2] = Sq * (UZ[0].x + UX[2].x); F[3] = Sq * (UZ[0].y + UX[2].y);   // comp1, Nz
-        F[4] = kv * Sq * UX[1].x + k2 * Sq * U[0].x; F[5] = kv * Sq * UX[1].y + k2 * Sq * U[0].y;   // comp1, N
-        // comp 2
-        F[6] = kv * Q * (UX[0].x + UZ[2].x) + k2 * P * U[1].x; F[7] = kv * Q * (UX[0].y + UZ[2].y) + k2 * P * U[1].y;  // N
-        sG[5 * q + 0] = Sq * (UX[1].x - kv * U[0].x);   // comp2 Nx re
-        sG[5 * q + 1] = Sq * (UX[1].y - kv * U[0].y);   // comp2 Nx im
-        sG[5 * q + 2] = Sq * (UZ[1].x - kv * U[2].x);   // comp2 Nz re
-        sG[5 * q + 3] = Sq * (UZ[1].y - kv * U[2].y);   // comp2 Nz im
-        // comp 3
-        F[8] = Q * UX[0].x + P * UZ[2].x - kv * Q * U[1].x; F[9] = Q * UX[0].y + P * UZ[2].y - kv * Q * U[1].y;   // Nz
-        F[10] = kv * Sq * UZ[1].x + k2 * Sq * U[2].x; F[11] = kv * Sq * UZ[1].y + k2 * Sq * U[2].y;  // N
-        // comp3 Nx term equals comp1 Nz term (Sq (u1z + u3x)); geometry kept for the test pass
-        sG[5 * q + 4] = g.det;
-    }
-    __syncthreads();
-    const int i1 = jcsc[c] - 1, i2 = jcsc[c + 1] - 1;
-    for (int r = i1 + threadIdx.x; r < i2; r += blockDim.x) fmat[r] = make_float2(0.f, 0.f);
-    __syncthreads();
-    const double we = ew[e];
-    for (int i = threadIdx.x; i < 3 * nen; i += blockDim.x) {
-        const int a = i / 3, cc = i % 3;
-        const int dof = L[i];
-        if (dof < 0) continue;
-        double re = 0.0, im = 0.0;
-        for (int q = 0; q < nq; ++q) {
-            int ix = q % nq1, iz = q / nq1;
-            Geo g;
-            geo_at(xq[ix], xq[iz], ie, xl, zl, g);
-            const double *s = shl + ((size_t)q * nen + a) * 3;
-            double n = s[2];
-            double nx = (s[0] * g.dzdeta - s[1] * g.dzdxi) / g.det;
-            double nz = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / g.det;
-            const double *F = sF + 12 * q;
-            if (cc == 0) {
-                re += nx * F[0] + nz * F[2] + n * F[4];
-                im += nx * F[1] + nz * F[3] + n * F[5];
-            } else if (cc == 1) {
-                re += n * F[6] + nx * sG[5 * q + 0] + nz * sG[5 * q + 2];
-                im += n * F[7] + nx * sG[5 * q + 1] + nz * sG[5 * q + 3];
-            } else {
-                re += nz * F[8] + nx * F[2] + n * F[10];
-                im += nz * F[9] + nx * F[3] + n * F[11];
-            }
-        }
-        // row position inside the column (ICSC sorted, 1-based DOFs)
-        int lo = i1, hi = i2 - 1, pos = -1, key = dof + 1;
-        while (lo <= hi) {
-            int m = (lo + hi) >> 1;
-            if (icsc[m] == key) { pos = m; break; }
-            if (icsc[m] < key) lo = m + 1; else hi = m - 1;
-        }
-        if (pos < 0) { *err = 3; continue; }
-        fmat[pos] = make_float2((float)(we * re), (float)(we * im));
-    }
-}
-
 // ADJFLAM_DIST: gprod(mygrad(c)) -= Re(conj(F) lambda) over column c
 __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *__restrict__ jcsc,
                           const int *__restrict__ icsc, const float2 *__restrict__ f,
@@ -1344,23 +1237,30 @@ __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *
 }
 
 // ---------------------------------------------------------------------------
-// Second-order element products of QPMFREE (qpmfree.f90:622-1215) on the
-// DABMKISOB25 element matrices (dstiff25.f:1-203).  The derivative matrix of
-// element e collocated on its corner k is the elastic element operator with
-// the ky coupling in the imaginary part, no -w^2 rho mass term, and
-// density-normalised fields (lambda, mu) = (D, 0) for Vp (PP) or (-2D, D)
-// for Vs (SS), D(q) = 2 v_k N_k(q), v_k = Vp or Vs at the anchor.  By
-// linearity a workgroup forms one field D(q) = sum_k c_k N_k(q) from its
-// item's four corner coefficients and applies the operator M to nvec <= 2
-// vectors through fluxes at the quadrature points:
+// Element derivative products, collocated on one anchor (per column) or
+// summed over anchors (per element):
+//  * DABMKISOB25 (dstiff25.f:1-203), the second-order products of QPMFREE
+//    (qpmfree.f90:622-1215): IMAG, ky coupling in the imaginary part;
+//  * DKM_VPVS (dstiff25.f:498-662) for CFMAT_DIST/VFORCE25 (hessloc.f90,
+//    vforce25.f): !IMAG, ky coupling written as real numbers.
+// Both are the elastic element operator without the -w^2 rho mass term,
+// with density-normalised moduli at the quadrature points
+//   lambda + 2 mu = P = kp D,  mu = S = ks D,  lambda = Q = P - 2 S,
+//   D(q) = w(q) sum_k c_k N_k(q)
+// (DABMKISOB25: kp, ks = 1, 0 for Vp, 0, 1 for Vs; DKM_VPVS: the VPVS
+// slaving of vforce25.f:164-179), c_k the item's four corner coefficients.
+// The operator M is applied to nvec <= 2 vectors through fluxes at the
+// quadrature points:
 //   mode 0 (item = element; one launch per colour, so rows never collide):
 //          y_v += M x_v (M is Hermitian: M x is also M^H x);
 //   mode 1 (item = local column c, its element the last connected one):
-//          p_v(MYGRAD(c)) = -sum over the element's rows Re(conj(M x_v) z_v),
-//          every row checked against column c's CSC pattern.
+//          p_v(MYGRAD(c)) = -sum over the element's rows Re(conj(M x_v) z_v);
+//   mode 2 (item = local column c): column c of F in CSC order, COMPLEX*8,
+//          zero outside the element's rows.
+// Every row of modes 1 and 2 is checked against column c's CSC pattern.
 // Flux layout per (vector, q): [comp][N, Nx, Nz] complex.
-template <int NL>
-__global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, int liss,
+template <int NL, bool IMAG>
+__global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, double kp, double ks,
                                              double kv, const int *__restrict__ item_elem,
                                              const double *__restrict__ item_coef,
                                              const double *__restrict__ shlT, const double *__restrict__ wq,
@@ -1372,7 +1272,7 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, int
                                              const float2 *__restrict__ z1, const int *__restrict__ mygrad,
                                              const int *__restrict__ jcsc, const int *__restrict__ icsc,
                                              float *__restrict__ p0, float *__restrict__ p1,
-                                             int *__restrict__ err) {
+                                             float2 *__restrict__ fout, int *__restrict__ err) {
     extern __shared__ double sm[];
     __shared__ double red[2][4];
     const int it = blockIdx.x;
@@ -1388,6 +1288,9 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, int
     double *sG = reinterpret_cast<double *>(sP + 2 * QG * nee);   // [q][4]: rx, sx, rz, sz
     const int *ie = ieng + 4 * e;
     const int *L = lm + (size_t)e * nee;
+    if (mode == 2)
+        for (int r = jcsc[it] - 1 + threadIdx.x; r < jcsc[it + 1] - 1; r += blockDim.x)
+            fout[r] = make_float2(0.f, 0.f);
     for (int i = threadIdx.x; i < nee; i += blockDim.x) {
         const int d = L[i];
         float2 a = d >= 0 ? x0[d] : make_float2(0.f, 0.f);
@@ -1403,7 +1306,7 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, int
         Geo g;
         if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) *err = 2;
         const double D = wq[ix] * wq[iz] * g.det * (c0 * g.sf[0] + c1 * g.sf[1] + c2 * g.sf[2] + c3 * g.sf[3]);
-        const double P = liss ? 0.0 : D, Q = liss ? -2.0 * D : D, S = liss ? D : 0.0, k2 = kv * kv;
+        const double P = kp * D, S = ks * D, Q = P - 2.0 * S, k2 = kv * kv;
         const double rx = g.dzdeta / g.det, sx = -g.dzdxi / g.det, rz = -g.dxdeta / g.det, sz = g.dxdxi / g.det;
         double *G = sG + 4 * q;
         G[0] = rx; G[1] = sx; G[2] = rz; G[3] = sz;
@@ -1427,22 +1330,26 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, int
                 }
             }
             double2 *F = sF + ((size_t)v * nq + q) * 9;
-            // i kv X = (-kv X.y, kv X.x)
-            // comp 1: N: i kv S UX2 + kv^2 S U1; Nx: P UX1 + Q UZ3 - i kv Q U2; Nz: S (UZ1 + UX3)
-            F[0] = make_double2(-kv * S * UX[1].y + k2 * S * U[0].x, kv * S * UX[1].x + k2 * S * U[0].y);
-            F[1] = make_double2(P * UX[0].x + Q * UZ[2].x + kv * Q * U[1].y,
-                                P * UX[0].y + Q * UZ[2].y - kv * Q * U[1].x);
+            // ky coupling: mk(X) = i kv X (IMAG) or kv X
+            auto mk = [&](double2 X) {
+                return IMAG ? make_double2(-kv * X.y, kv * X.x) : make_double2(kv * X.x, kv * X.y);
+            };
+            auto lin = [](double a, double2 X, double b, double2 Y) {
+                return make_double2(a * X.x + b * Y.x, a * X.y + b * Y.y);
+            };
+            // comp 1: N: S mk(UX2) + kv^2 S U1; Nx: P UX1 + Q UZ3 - Q mk(U2); Nz: S (UZ1 + UX3)
+            const double2 m2 = mk(U[1]);
+            F[0] = lin(S, mk(UX[1]), k2 * S, U[0]);
+            F[1] = make_double2(P * UX[0].x + Q * UZ[2].x - Q * m2.x, P * UX[0].y + Q * UZ[2].y - Q * m2.y);
             F[2] = make_double2(S * (UZ[0].x + UX[2].x), S * (UZ[0].y + UX[2].y));
-            // comp 2: N: i kv Q (UX1 + UZ3) + kv^2 P U2; Nx: S (UX2 - i kv U1); Nz: S (UZ2 - i kv U3)
-            F[3] = make_double2(-kv * Q * (UX[0].y + UZ[2].y) + k2 * P * U[1].x,
-                                kv * Q * (UX[0].x + UZ[2].x) + k2 * P * U[1].y);
-            F[4] = make_double2(S * (UX[1].x + kv * U[0].y), S * (UX[1].y - kv * U[0].x));
-            F[5] = make_double2(S * (UZ[1].x + kv * U[2].y), S * (UZ[1].y - kv * U[2].x));
-            // comp 3: N: i kv S UZ2 + kv^2 S U3; Nx: S (UZ1 + UX3); Nz: Q UX1 + P UZ3 - i kv Q U2
-            F[6] = make_double2(-kv * S * UZ[1].y + k2 * S * U[2].x, kv * S * UZ[1].x + k2 * S * U[2].y);
+            // comp 2: N: Q mk(UX1 + UZ3) + kv^2 P U2; Nx: S (UX2 - mk(U1)); Nz: S (UZ2 - mk(U3))
+            F[3] = lin(Q, mk(make_double2(UX[0].x + UZ[2].x, UX[0].y + UZ[2].y)), k2 * P, U[1]);
+            F[4] = lin(S, UX[1], -S, mk(U[0]));
+            F[5] = lin(S, UZ[1], -S, mk(U[2]));
+            // comp 3: N: S mk(UZ2) + kv^2 S U3; Nx: S (UZ1 + UX3); Nz: Q UX1 + P UZ3 - Q mk(U2)
+            F[6] = lin(S, mk(UZ[1]), k2 * S, U[2]);
             F[7] = F[2];
-            F[8] = make_double2(Q * UX[0].x + P * UZ[2].x + kv * Q * U[1].y,
-                                Q * UX[0].y + P * UZ[2].y - kv * Q * U[1].x);
+            F[8] = make_double2(Q * UX[0].x + P * UZ[2].x - Q * m2.x, Q * UX[0].y + P * UZ[2].y - Q * m2.y);
         }
     }
     __syncthreads();
@@ -1493,7 +1400,7 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, int
     }
     __syncthreads();
     int i1 = 0, i2 = 0;
-    if (mode == 1) { i1 = jcsc[it] - 1; i2 = jcsc[it + 1] - 1; }
+    if (mode != 0) { i1 = jcsc[it] - 1; i2 = jcsc[it + 1] - 1; }
     double acc[2] = {0.0, 0.0};
     for (int i = threadIdx.x; i < nee; i += blockDim.x) {
         const int dof = L[i];
@@ -1518,6 +1425,10 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, int
                 if (icsc[mid] < key) lo = mid + 1; else hi = mid - 1;
             }
             if (pos < 0) { *err = 3; continue; }
+            if (mode == 2) {
+                fout[pos] = make_float2((float)val[0].x, (float)val[0].y);
+                continue;
+            }
             const float2 za = z0[dof];
             acc[0] -= val[0].x * za.x + val[0].y * za.y;
             if (nvec > 1) {
@@ -1526,7 +1437,7 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, int
             }
         }
     }
-    if (mode == 0) return;
+    if (mode != 1) return;
 #pragma unroll
     for (int v = 0; v < 2; ++v)
         for (int o = 32; o > 0; o >>= 1) acc[v] += __shfl_xor(acc[v], o);
@@ -1555,7 +1466,9 @@ __global__ __launch_bounds__(256) void k_pchadj(int nnpgl, int nrhs, const int *
                                                  float2 *__restrict__ jac, float *__restrict__ hdiag) {
     // a lane owns right-hand sides r0 + lane + 64 j (j < 3) of a 192-wide
     // pass; the column's nonzeros are walked two at a time so six rows of G
-    // are in flight per lane
+    // are in flight per lane.  Entries of F that are exactly zero (a column
+    // of CFMAT_DIST holds only its last element's rows, about a third of the
+    // pattern) skip their row of G: the sums are unchanged for finite G.
     constexpr int R = 3;
     const int c = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
@@ -1569,8 +1482,9 @@ __global__ __launch_bounds__(256) void k_pchadj(int nnpgl, int nrhs, const int *
         for (int j = 0; j < R; ++j) { ar[j] = ai[j] = 0.0; ok[j] = r0 + lane + 64 * j < nrhs; }
         int i = i1;
         for (; i + 1 < i2; i += 2) {
-            const int ra = icsc[i] - 1, rb = icsc[i + 1] - 1;
             const float2 fa = f[i], fb = f[i + 1];
+            const int ra = (fa.x != 0.f || fa.y != 0.f) ? icsc[i] - 1 : -1;
+            const int rb = (fb.x != 0.f || fb.y != 0.f) ? icsc[i + 1] - 1 : -1;
             float2 ga[R], gb[R];
 #pragma unroll
             for (int j = 0; j < R; ++j) {
@@ -1587,8 +1501,8 @@ __global__ __launch_bounds__(256) void k_pchadj(int nnpgl, int nrhs, const int *
             }
         }
         if (i < i2) {
-            const int ra = icsc[i] - 1;
             const float2 fa = f[i];
+            const int ra = (fa.x != 0.f || fa.y != 0.f) ? icsc[i] - 1 : -1;
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 const int r = r0 + lane + 64 * j;
