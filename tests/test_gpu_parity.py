@@ -491,3 +491,44 @@ def test_cadjlam_rhs(norm, dtmax):
                                  _ptr(yrec, _lib.dp), _ptr(bad, _lib.ip), vp(est), vp(obs),
                                  _ptr(w, _lib.fp), vp(rhs), C.byref(iw)), "cadjlam_rhs")
     assert iw.value == 1
+
+
+def test_qp_rhs_full_size_against_operator_difference():
+    """At the benchmark size (200 x 100 elements, p = 4, 963,603 DOFs): with
+    every anchor of a uniform mesh inverted (ELEM_WTS = 1), CADJ_NU_ALPHA's
+    RHSALP is S[lambda = f, mu = 0, rho = 0] u, f = 2 Vp V at the anchors, and
+    RHSBNU the same (Hermitian) operator on lambda; S is linear in lambda, so
+    both equal S(model with lambda + f) x - S(model) x from the batched FP64
+    product -- two independent kernels (norm-wise 1e-5: the outputs are
+    COMPLEX*8)."""
+    import dataclasses
+    from fem25_amd.api import MeshInfo
+    m = cf.structured_mesh(200, 100, 4)
+    mod = cf.layered_model(m, seed=2)
+    mi = MeshInfo(m)
+    mi.set_model(mod)
+    inv = mi.inv_setup(b"PP")
+    assert np.all(inv.elem_wts == 1.0)
+    rng = np.random.default_rng(97)
+    n = mi.ndof
+    u = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    lam = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    v = rng.standard_normal(inv.na35).astype(np.float32)
+    freq, py = 0.07, 1.1e-4
+    print(f"qp full size: ndof={n} nnpgl={inv.nnpgl}", flush=True)
+    bnu, alp, _ = mi.qp_nu_alpha(freq, py, v, lam, u)
+    vp = np.sqrt((mod.ecoeff[:, 0] + 2 * mod.ecoeff[:, 1]) / mod.dens)
+    f = np.zeros(m.nnpg)
+    sel = inv.maskg > 0
+    f[sel] = 2 * vp[sel] * v[inv.maskg[sel] - 1].astype(np.float64)
+    om = np.full(2, 2 * np.pi * freq)
+    ky = om * py
+    x = np.stack([u.astype(np.complex128), lam.astype(np.complex128)])
+    y0 = mi.apply_batch(om, ky, x)
+    ec = np.asfortranarray(np.stack([mod.ecoeff[:, 0] + f, mod.ecoeff[:, 1]], axis=1))
+    mi.set_model(dataclasses.replace(mod, ecoeff=ec))
+    y1 = mi.apply_batch(om, ky, x)
+    mi.set_model(mod)
+    d = y1 - y0
+    assert relerr(alp, d[0]) < 1e-5
+    assert relerr(bnu, d[1]) < 1e-5
