@@ -1878,19 +1878,36 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
         hipError_t e = hipMemcpyAsync(du, u, ndof * ntrip * 16, hipMemcpyHostToDevice, m->st);
         if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
     }
-    const int nt1 = std::max(64, ((nee + 63) / 64) * 64);
-    const size_t lds1 = nee * sizeof(double2) + (size_t)nq * 28 * sizeof(double);
+    typedef void (*FcolsKernel)(const double *, const double *, const double *, const double *, const int *,
+                                const double *, const double *, const char *, const int *, const double *,
+                                const double *, const double *, double, double, const double2 *, double2 *, int *);
+    FcolsKernel kf = nullptr;
+    switch (m->tab.nl) {
+        case 3: kf = k_fcols<3>; break;
+        case 4: kf = k_fcols<4>; break;
+        case 5: kf = k_fcols<5>; break;
+        case 6: kf = k_fcols<6>; break;
+        case 7: kf = k_fcols<7>; break;
+        default: break;
+    }
+    if (!kf && !r) {
+        g_err = "unsupported order";
+        r = F25_ERR_ARG;
+    }
+    const int nt1 = 256, qg = std::max(1, 256 / nee);
+    const size_t lds1 = nee * sizeof(double2) + (size_t)nq * 28 * sizeof(double) +
+                        (size_t)(qg - 1) * 12 * nee * sizeof(double2);
     const int nt2 = 256;
     const size_t lds2 = (size_t)m->ncon_anchor * nee * 3 * sizeof(double2) + nt2 * 3 * sizeof(double);
     if (!r && lds1 > 65536)
-        HIPCHK(hipFuncSetAttribute((const void *)k_fcols, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
+        HIPCHK(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
     if (!r && lds2 > 65536)
         HIPCHK(hipFuncSetAttribute((const void *)k_fdiag, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
     if (!r) (void)hipEventRecord(m->ev0, m->st);
     for (int t = 0; t < ntrip && !r; ++t) {
-        hipLaunchKernelGGL(k_fcols, dim3(m->d.nelem), dim3(nt1), lds1, m->st, nen, m->tab.nq1, m->d_shl,
-                           m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho,
-                           m->d_lam, m->d_mu, omega[t], ky[t], du + ndof * t, dcols, m->d_err);
+        hipLaunchKernelGGL(kf, dim3(m->d.nelem), dim3(nt1), lds1, m->st, m->d_shl, m->d_shlT, m->d_wq, m->d_xq,
+                           m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho, m->d_lam, m->d_mu, omega[t],
+                           ky[t], du + ndof * t, dcols, m->d_err);
         hipLaunchKernelGGL(k_fdiag, dim3(nnpg), dim3(nt2), lds2, m->st, nnpg, m->ncon_anchor, nee,
                            m->d_mconn, m->d_ieng, m->d_pmap, dcols, dd);
         hipError_t e = hipGetLastError();

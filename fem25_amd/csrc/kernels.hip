@@ -1064,7 +1064,8 @@ __global__ void k_grad_gather(int nnpg, int ncon, const int *__restrict__ mconn,
 // out[((e*4 + c)*3 + p)*nee + 3b + i].  One workgroup per element; phase 1
 // puts the fields of u at every quadrature point in LDS, phase 2 gives one
 // thread per element row (b, i) and sums over the quadrature points.
-__global__ void k_fcols(int nen, int nq1, const double *__restrict__ shl, const double *__restrict__ shlT,
+template <int NL>
+__global__ __launch_bounds__(256) void k_fcols(const double *__restrict__ shl, const double *__restrict__ shlT,
                         const double *__restrict__ wq,
                         const double *__restrict__ xq, const int *__restrict__ ieng,
                         const double *__restrict__ xl, const double *__restrict__ zl,
@@ -1074,10 +1075,13 @@ __global__ void k_fcols(int nen, int nq1, const double *__restrict__ shl, const 
                         const double2 *__restrict__ u, double2 *__restrict__ out, int *__restrict__ err) {
     extern __shared__ double sm[];
     constexpr int QS = 28;   // doubles per quadrature point
+    constexpr int nen = NL * NL, nq1 = 2 * NL + 3, nq = nq1 * nq1, nee = 3 * nen;
+    // phase 2 splits each row's quadrature points over QG thread groups
+    constexpr int QG = (256 / nee) > 0 ? (256 / nee) : 1;
     const int e = blockIdx.x;
-    const int nq = nq1 * nq1, nee = 3 * nen;
     double2 *su = reinterpret_cast<double2 *>(sm);   // nee
     double *sq = reinterpret_cast<double *>(su + nee);   // nq * QS
+    double2 *spart = reinterpret_cast<double2 *>(sq + (size_t)nq * QS);   // (QG-1) * 12 * nee
     const int *ie = ieng + 4 * e;
     const int *L = lm + (size_t)e * nee;
     double2 *o = out + (size_t)e * 12 * nee;
@@ -1097,6 +1101,7 @@ __global__ void k_fcols(int nen, int nq1, const double *__restrict__ shl, const 
         double2 U[3] = {}, UX[3] = {}, UZ[3] = {};
         const double rx = g.dzdeta / g.det, sx = -g.dzdxi / g.det;
         const double rz = -g.dxdeta / g.det, szz = g.dxdxi / g.det;
+#pragma unroll 5
         for (int a = 0; a < nen; ++a) {
             const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
             const double n = shlT[(size_t)(a * 3 + 2) * nq + q], nx = s0 * rx + s1 * sx, nz = s0 * rz + s1 * szz;
@@ -1129,16 +1134,19 @@ __global__ void k_fcols(int nen, int nq1, const double *__restrict__ shl, const 
         p[24] = rx; p[25] = sx; p[26] = rz; p[27] = szz;
     }
     __syncthreads();
-    const int it = threadIdx.x;
-    if (it >= nee) return;
+    const int it = threadIdx.x % nee, grp = threadIdx.x / nee;
+    const bool rowt = grp < QG;
     const int b = it / 3, i = it % 3;
     // index of S_ik in the packed symmetric storage for k = x, y, z
-    const int sidx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+    const int i0 = i == 0 ? 0 : (i == 1 ? 3 : 4), i1 = i == 0 ? 3 : (i == 1 ? 1 : 5), i2 = i == 0 ? 4 : (i == 1 ? 5 : 2);
     double2 acc[4][3];
+#pragma unroll
     for (int c = 0; c < 4; ++c)
+#pragma unroll
         for (int t = 0; t < 3; ++t) acc[c][t] = make_double2(0.0, 0.0);
     const double w2 = omega * omega;
-    for (int q = 0; q < nq; ++q) {
+#pragma unroll 2
+    for (int q = grp; q < nq && rowt; q += QG) {
         const double *p = sq + (size_t)q * QS;
         const double *s = shl + ((size_t)q * nen + b) * 3;
         const double n = s[2], nx = s[0] * p[24] + s[1] * p[25], nz = s[0] * p[26] + s[1] * p[27];
@@ -1149,10 +1157,11 @@ __global__ void k_fcols(int nen, int nq1, const double *__restrict__ shl, const 
         else if (i == 2) tl = make_double2(nz * dv.x, nz * dv.y);
         else tl = make_double2(-kv * n * dv.y, kv * n * dv.x);
         // mu: Nx S_ix + (i ky N) S_iy + Nz S_iz
-        const double *sx_ = p + 6 + 2 * sidx[i][0], *sy_ = p + 6 + 2 * sidx[i][1], *sz_ = p + 6 + 2 * sidx[i][2];
+        const double *sx_ = p + 6 + 2 * i0, *sy_ = p + 6 + 2 * i1, *sz_ = p + 6 + 2 * i2;
         const double2 tm = make_double2(nx * sx_[0] - kv * n * sy_[1] + nz * sz_[0],
                                         nx * sx_[1] + kv * n * sy_[0] + nz * sz_[1]);
         const double2 tr = make_double2(-w2 * n * p[18 + 2 * i], -w2 * n * p[19 + 2 * i]);
+#pragma unroll
         for (int c = 0; c < 4; ++c) {
             const double wc = p[c];
             acc[c][0].x += wc * tl.x; acc[c][0].y += wc * tl.y;
@@ -1160,6 +1169,26 @@ __global__ void k_fcols(int nen, int nq1, const double *__restrict__ shl, const 
             acc[c][2].x += wc * tr.x; acc[c][2].y += wc * tr.y;
         }
     }
+    // groups 1.. hand their partial sums to group 0
+    if (QG > 1) {
+        if (rowt && grp > 0)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int t = 0; t < 3; ++t) spart[((size_t)(grp - 1) * 12 + c * 3 + t) * nee + it] = acc[c][t];
+        __syncthreads();
+        if (grp == 0)
+            for (int g2 = 1; g2 < QG; ++g2)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        const double2 v = spart[((size_t)(g2 - 1) * 12 + c * 3 + t) * nee + it];
+                        acc[c][t].x += v.x; acc[c][t].y += v.y;
+                    }
+    }
+    if (grp != 0) return;
+#pragma unroll
     for (int c = 0; c < 4; ++c) {
         const int a = ie[c] - 1;
         const double r = rho[a], vp = sqrt((lam[a] + 2.0 * mu[a]) / r), vs = sqrt(mu[a] / r);
