@@ -1007,10 +1007,14 @@ static int check_mrdof(int nrec, int ndof, const int *mrdof, bool allow_out) {
     return 0;
 }
 
+static int exresp_check(int ndof, int nsrc, int nrec, const double *py, const double *yrec, const int *mrdof,
+                        const void *recv) {
+    if (ndof <= 0 || nsrc <= 0 || nrec <= 0 || !py || !yrec || !recv) FAIL(F25_ERR_ARG, "invalid arguments");
+    return check_mrdof(nrec, ndof, mrdof, false);
+}
+
 static int exresp_core(int ndof, int nsrc, int nrec, double freq, const double *py, const double *yrec,
                        const float2 *sol_dev, const int *mrdof, const void *recv, float2 *est_dev, hipStream_t s) {
-    if (ndof <= 0 || nsrc <= 0 || nrec <= 0 || !py || !yrec || !recv) FAIL(F25_ERR_ARG, "invalid arguments");
-    TRY(check_mrdof(nrec, ndof, mrdof, false));
     double *dpy = nullptr, *dy = nullptr;
     int *dm = nullptr;
     float2 *dr = nullptr;
@@ -1033,6 +1037,7 @@ static int exresp_core(int ndof, int nsrc, int nrec, double freq, const double *
 extern "C" int f25_exresp25d_dev(int ndof, int nsrc, int nrec, double freq, const double *py, const double *yrec,
                                  const void *sol_dev, const int *mrdof, const void *recv, void *est_dev) {
     if (!sol_dev || !est_dev) FAIL(F25_ERR_ARG, "null argument");
+    TRY(exresp_check(ndof, nsrc, nrec, py, yrec, mrdof, recv));
     TmpStream ts;
     HIPCHK(hipStreamCreate(&ts.s));
     return exresp_core(ndof, nsrc, nrec, freq, py, yrec, (const float2 *)sol_dev, mrdof, recv, (float2 *)est_dev,
@@ -1041,7 +1046,8 @@ extern "C" int f25_exresp25d_dev(int ndof, int nsrc, int nrec, double freq, cons
 
 extern "C" int f25_exresp25d(int ndof, int nsrc, int nrec, double freq, const double *py, const double *yrec,
                              const void *sol, const int *mrdof, const void *recv, void *est) {
-    if (!sol || !est || ndof <= 0 || nsrc <= 0 || nrec <= 0) FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!sol || !est) FAIL(F25_ERR_ARG, "null argument");
+    TRY(exresp_check(ndof, nsrc, nrec, py, yrec, mrdof, recv));
     TmpStream ts;
     HIPCHK(hipStreamCreate(&ts.s));
     float2 *ds = nullptr, *de = nullptr;
@@ -1057,11 +1063,8 @@ extern "C" int f25_exresp25d(int ndof, int nsrc, int nrec, double freq, const do
     return r;
 }
 
-static int cadjlam_core(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq, const int *isrc,
-                        const double *freq, const double *py, double azmod, int norm, int irestp, float epss,
-                        float dtmax, int ibphase, int lunwrap, const double *yrec, const int *mrdof,
-                        const float2 *est_dev, const float2 *obs_dev, const float *w_dev, float2 *rhs_dev,
-                        int *iwarn, hipStream_t s) {
+static int cadjlam_check(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq, const int *isrc,
+                         const double *freq, const double *py, const double *yrec, const int *mrdof, int *iwarn) {
     if (ndof <= 0 || nfreq <= 0 || nrec <= 0 || nsrc <= 0 || npair <= 0 || npair > 65535 || !ifreq || !isrc ||
         !freq || !py || !yrec || !iwarn)
         FAIL(F25_ERR_ARG, "invalid arguments");
@@ -1069,6 +1072,14 @@ static int cadjlam_core(int ndof, int nfreq, int nrec, int nsrc, int npair, cons
     for (int p = 0; p < npair; ++p)
         if (ifreq[p] < 1 || ifreq[p] > nfreq || isrc[p] < 1 || isrc[p] > nsrc)
             FAIL(F25_ERR_ARG, "IFREQ or ISRC out of range");
+    return 0;
+}
+
+static int cadjlam_core(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq, const int *isrc,
+                        const double *freq, const double *py, double azmod, int norm, int irestp, float epss,
+                        float dtmax, int ibphase, int lunwrap, const double *yrec, const int *mrdof,
+                        const float2 *est_dev, const float2 *obs_dev, const float *w_dev, float2 *rhs_dev,
+                        int *iwarn, hipStream_t s) {
     AdjParams prm{(float)azmod, epss, dtmax, norm, irestp, ibphase, lunwrap ? 1 : 0};
     int *df = nullptr, *dsr = nullptr, *dm = nullptr, *dw = nullptr;
     double *dfr = nullptr, *dpy = nullptr, *dy = nullptr;
@@ -1112,6 +1123,7 @@ extern "C" int f25_cadjlam_rhs_dev(int ndof, int nfreq, int nrec, int nsrc, int 
                                    const double *yrec, const int *mrdof, const void *est_dev, const void *obs_dev,
                                    const float *wghts_dev, void *rhs_dev, int *iwarn) {
     if (!est_dev || !obs_dev || !wghts_dev || !rhs_dev) FAIL(F25_ERR_ARG, "null argument");
+    TRY(cadjlam_check(ndof, nfreq, nrec, nsrc, npair, ifreq, isrc, freq, py, yrec, mrdof, iwarn));
     TmpStream ts;
     HIPCHK(hipStreamCreate(&ts.s));
     return cadjlam_core(ndof, nfreq, nrec, nsrc, npair, ifreq, isrc, freq, py, azmod, norm, irestp, epss, dtmax,
@@ -1124,8 +1136,8 @@ extern "C" int f25_cadjlam_rhs(int ndof, int nfreq, int nrec, int nsrc, int npai
                                int irestp, float epss, float dtmax, int ibphase, int lunwrap, const double *yrec,
                                const int *mrdof, const void *est, const void *obs, const float *wghts, void *rhs,
                                int *iwarn) {
-    if (!est || !obs || !wghts || !rhs || ndof <= 0 || npair <= 0 || nfreq <= 0 || nrec <= 0 || nsrc <= 0)
-        FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!est || !obs || !wghts || !rhs) FAIL(F25_ERR_ARG, "null argument");
+    TRY(cadjlam_check(ndof, nfreq, nrec, nsrc, npair, ifreq, isrc, freq, py, yrec, mrdof, iwarn));
     TmpStream ts;
     HIPCHK(hipStreamCreate(&ts.s));
     const size_t nd = 3 * (size_t)nfreq * nrec * nsrc;
