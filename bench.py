@@ -96,11 +96,8 @@ def main() -> int:
     ms_step = 1e3 * wall / args.steps
     gdofs = ndof * npair / (ms_step * 1e-3) / 1e9
     k_ms = float(np.mean(kern_ms))
-    # launches per step: one per element colour, three per colour when the
-    # nine FP64 blocks of an element exceed the LDS budget (one output
-    # component per launch)
-    rs = nen if nen % 2 else nen + 1
-    ncol = mi.ncolour * (3 if 9 * nen * rs * 8 > 150 * 1024 else 1)
+    # launches per step: one per element colour
+    ncol = mi.ncolour
     flops = 60.0 * nen * nen * mesh.nelem * npair          # useful block flops per step
     bytes_alg = 2 * ndof * npair * 16 + 9 * nen * nen * 8 * mesh.nelem
     achieved = flops / (k_ms * 1e-3)
@@ -145,7 +142,9 @@ def main() -> int:
         "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP64 / 1e12,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": f"k_apply<{mesh.nord + 1},double,double2,false>",
+                     "kernel": f"k_apply_v<{mesh.nord + 1}>",
+                     "compute": "FP64 FMA on the vector ALUs; on gfx950 their peak equals the FP64 "
+                                "MFMA peak (78.6 TFLOP/s)",
                      "launch_avg_ms": round(k_ms / ncol, 4),
                      "algorithmic_flops_per_launch": flops / ncol,
                      "algorithmic_bytes_per_launch": bytes_alg / ncol,
@@ -337,11 +336,11 @@ def cfmat_host(mi, inv, freq, py, wave):
 
 
 def profiled_traffic(config: str):
-    """HBM bytes per k_apply launch from the committed rocprofv3 FETCH_SIZE /
+    """HBM bytes per k_apply_v launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (None when there is none)."""
     if config != "c2":
         return None, None
-    name = "k_apply_f64_p4_200x100_128pairs.md"
+    name = "k_apply_v_f64_p4_200x100_128pairs.md"
     try:
         with open(os.path.join(ROOT, "profiles", name)) as f:
             for line in f:

@@ -105,6 +105,10 @@ struct f25_mesh_s {
     unsigned long long *d_colmax = nullptr;           // per-pair max |x| (refined mode)
     int colmax_cap = 0;
     double *d_dblk = nullptr;   // blocks of a model perturbation (GN products)
+    double *d_vcf = nullptr;    // d_blk in the coefficient layout of the FP64 product (k_apply_v)
+    double *d_dvcf = nullptr;   // d_dblk in that layout
+    double *d_vpad = nullptr;   // k_apply_v: vpad_cap zero double2, then vpad_cap double2 of store sink
+    int vpad_cap = 0;
     int64_t *d_irptr = nullptr;
     int *d_jcptr = nullptr;
     signed char *d_comp = nullptr;
@@ -372,6 +376,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_ieng); dfree(m->d_lm); dfree(m->d_first); dfree(m->d_order); dfree(m->d_pml_order); dfree(m->d_err);
     dfree(m->d_xl); dfree(m->d_zl); dfree(m->d_xd); dfree(m->d_zd); dfree(m->d_shl); dfree(m->d_shlT); dfree(m->d_wq);
     dfree(m->d_xq); dfree(m->d_cdom); dfree(m->d_cnp); dfree(m->d_rho); dfree(m->d_lam); dfree(m->d_mu);
+    dfree(m->d_vcf); dfree(m->d_dvcf); dfree(m->d_vpad);
     dfree(m->d_blk); dfree(m->d_blk32); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_colmax); dfree(m->d_dblk); dfree(m->d_irptr);
     dfree(m->d_jcptr); dfree(m->d_comp); dfree(m->d_v0); dfree(m->d_v1); dfree(m->d_v2);
     dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_pmap); dfree(m->d_qp); dfree(m->d_qs);
@@ -458,6 +463,8 @@ static int run_blocks(f25_mesh *m, const double *rho, const double *lam, const d
     return check_err(m);
 }
 
+static int vcoef(f25_mesh *m, const double *blk, double **vcf);
+
 extern "C" int f25_set_model(f25_mesh *m, const double *dens, const double *ecoeff) {
     if (!m || !dens || !ecoeff) FAIL(F25_ERR_ARG, "null argument");
     HIPCHK(hipSetDevice(m->dev));
@@ -488,6 +495,7 @@ extern "C" int f25_set_model(f25_mesh *m, const double *dens, const double *ecoe
         TRY(dalloc(&m->d_blk_lo, nb));
     }
     TRY(run_blocks(m, m->d_rho, m->d_lam, m->d_mu, m->d_blk, m->d_blk32, m->d_blk_hi, m->d_blk_lo));
+    TRY(vcoef(m, m->d_blk, &m->d_vcf));
     m->have_model = true;
     m->basis_ok = false;
     return 0;
@@ -718,15 +726,78 @@ static int dispatch_apply(f25_mesh *m, const double *om, const double *kv, int n
     FAIL(F25_ERR_ARG, "unsupported order");
 }
 
+// FP64 product (k_apply_v): one launch per element colour
+template <int NL>
+static int launch_apply_v(f25_mesh *m, const double *om, const double *kv, int npair, const double2 *x,
+                          double2 *y, const double *vcf) {
+    const double2 *zpad = reinterpret_cast<const double2 *>(m->d_vpad);
+    double2 *sink = reinterpret_cast<double2 *>(m->d_vpad) + m->vpad_cap;
+    const int ngrp = (npair + 63) / 64;
+    for (size_t c = 0; c + 1 < m->cptr.size(); ++c) {
+        int n = m->cptr[c + 1] - m->cptr[c];
+        if (!n) continue;
+        const size_t nwg = (size_t)((n + 7) / 8) * 8 * ngrp;
+        if (nwg > 0x7fffffff) FAIL(F25_ERR_ARG, "too many pairs for one launch");
+        hipLaunchKernelGGL(k_apply_v<NL>, dim3((unsigned)nwg), dim3(64 * 3 * VShape<NL>::NRT), 0, m->st,
+                           m->d_order + m->cptr[c], n, vcf, m->d_lm, m->d_first, npair, om, kv, x, y, zpad,
+                           sink);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// doubles of the k_apply_v coefficient layout for the whole mesh
+static size_t vcoef_size(const f25_mesh *m, int *nrt, int *rt, int *cw) {
+    switch (m->tab.nl) {
+#define F25_VS(N) case N: *nrt = VShape<N>::NRT; *rt = VShape<N>::RT; *cw = VShape<N>::CW; break;
+        F25_VS(3) F25_VS(4) F25_VS(5) F25_VS(6) F25_VS(7)
+#undef F25_VS
+        default: return 0;
+    }
+    return (size_t)m->d.nelem * 3 * *nrt * m->tab.nen * *cw;
+}
+
+// element blocks (NBLK per element) -> k_apply_v coefficient layout; the
+// buffer *vcf is allocated on first use
+static int vcoef(f25_mesh *m, const double *blk, double **vcf) {
+    int nrt = 0, rt = 0, cw = 0;
+    const size_t n = vcoef_size(m, &nrt, &rt, &cw);
+    if (!n) FAIL(F25_ERR_ARG, "unsupported order");
+    if (!*vcf) TRY(dalloc(vcf, n));
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_vcoef, dim3(nb), dim3(256), 0, m->st, m->tab.nen, nrt, rt, cw, m->d.nelem, blk, *vcf);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int apply_f64(f25_mesh *m, const double *om, const double *kv, int npair, const double2 *x, double2 *y,
+                     const double *vcf) {
+    // zero rows and store sink of npair values each
+    if (npair > m->vpad_cap) {
+        dfree(m->d_vpad);
+        m->vpad_cap = 0;
+        TRY(dalloc(&m->d_vpad, 4 * (size_t)npair));
+        fill_async(m->d_vpad, 2 * (size_t)npair, 0.0, m->st);
+        m->vpad_cap = npair;
+    }
+    switch (m->tab.nl) {
+        case 3: return launch_apply_v<3>(m, om, kv, npair, x, y, vcf);
+        case 4: return launch_apply_v<4>(m, om, kv, npair, x, y, vcf);
+        case 5: return launch_apply_v<5>(m, om, kv, npair, x, y, vcf);
+        case 6: return launch_apply_v<6>(m, om, kv, npair, x, y, vcf);
+        case 7: return launch_apply_v<7>(m, om, kv, npair, x, y, vcf);
+    }
+    FAIL(F25_ERR_ARG, "unsupported order");
+}
+
 static int apply_core(f25_mesh *m, int npair, const double *d_om, const double *d_ky, int mode,
-                      const void *x, void *y, const double *blk64) {
+                      const void *x, void *y, const double *vcf) {
     // every DOF is written by the first element touching it: no zero fill
     if (mode != F25_APPLY_F64 && mode != F25_APPLY_F32 && mode != F25_APPLY_F32_REFINED)
         FAIL(F25_ERR_ARG, "bad apply mode");
     HIPCHK(hipEventRecord(m->ev0, m->st));
     if (mode == F25_APPLY_F64)
-        TRY((dispatch_apply<double, double2, false>(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y,
-                                                    blk64, nullptr)));
+        TRY(apply_f64(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y, vcf));
     else if (mode == F25_APPLY_F32)
         TRY((dispatch_apply<float, float2, false>(m, d_om, d_ky, npair, (const float2 *)x, (float2 *)y,
                                                   m->d_blk32, nullptr)));
@@ -770,7 +841,7 @@ extern "C" int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, 
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
     HIPCHK(hipSetDevice(m->dev));
     TRY(stage_omky(m, npair, omega, ky));
-    return apply_core(m, npair, m->d_omky, m->d_omky + m->omky_cap, mode, x, y, m->d_blk);
+    return apply_core(m, npair, m->d_omky, m->d_omky + m->omky_cap, mode, x, y, m->d_vcf);
 }
 
 template <typename T>
@@ -1827,7 +1898,8 @@ extern "C" int f25_gn_fv(f25_mesh *m, int ntrip, const double *omega, const doub
     hipError_t e = hipMemcpyAsync(a, u, nn * 16, hipMemcpyHostToDevice, m->st);
     r = (e == hipSuccess) ? 0 : F25_ERR_HIP;
     if (!r) r = transpose_dev(m->st, a, b, ntrip, m->g.ndof);
-    if (!r) r = apply_core(m, ntrip, d_om, d_ky, F25_APPLY_F64, b, a, m->d_dblk);
+    if (!r) r = vcoef(m, m->d_dblk, &m->d_dvcf);
+    if (!r) r = apply_core(m, ntrip, d_om, d_ky, F25_APPLY_F64, b, a, m->d_dvcf);
     if (!r) r = transpose_dev(m->st, a, b, m->g.ndof, ntrip);
     if (!r) {
         e = hipMemcpyAsync(out, b, nn * 16, hipMemcpyDeviceToHost, m->st);

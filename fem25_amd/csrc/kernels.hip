@@ -1,6 +1,7 @@
 // HIP kernels for gfx950 (MI355X): element blocks, batched element-block
-// product on MFMA, CSR assembly, PML elements, SpMV, adjoint gradient, the
-// reference virtual-force columns and their CSC gather.
+// product (FP64 on the vector ALUs, FP32 on MFMA), CSR assembly, PML
+// elements, SpMV, adjoint gradient, the reference virtual-force columns and
+// their CSC gather.
 //
 // Element operator structure (estiff25.f:61-150, restated as bilinear
 // forms).  For a non-PML element with interpolated lambda, mu, rho and
@@ -18,6 +19,7 @@
 
 #include <cstdint>
 #include <type_traits>
+#include <utility>
 
 namespace f25k {
 
@@ -201,7 +203,9 @@ __global__ __launch_bounds__(NT) void k_blocks(int nelem, int nq1, const double 
 }
 
 // ---------------------------------------------------------------------------
-// Batched element-block product on MFMA.
+// Batched element-block product on MFMA: the FP32 and refined FP32 modes
+// (FP32 MFMA runs at twice the rate of FP32 FMAs on the vector ALUs).  The
+// FP64 product is k_apply_v below.
 //
 // One workgroup = one element of the current colour.  The element's blocks
 // sit in LDS (row stride RS, odd, so both the direct and the transposed
@@ -578,6 +582,228 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW), (REFINE ? 2 : 
                 }
             }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched element-block product in FP64 on the vector ALUs.
+//
+// On gfx950 an FP64 FMA on the vector ALUs runs at the FP64 MFMA rate (16
+// per cycle per SIMD), so the MFMA form above gains nothing in FP64 and pays
+// for its tile padding (NEN rows on 16-row tiles, NEN k values on 4-wide
+// steps: 1.43x the useful flops at p4).  Here one wave owns RT output rows
+// of one component c of one element for 64 (omega, ky) pairs, one pair per
+// lane (real and imaginary part in the lane), and runs only the useful
+// FMAs: 2 per (sub-block, row, node).
+//
+// The coefficients are the same for every lane.  They reach the FMA as its
+// DPP operand: each row of 16 lanes holds the same 32 coefficients (lane
+// l%16 holds numbers 2(l%16) and 2(l%16)+1 of a group of 32), and
+// v_fmac_f64 with row_newbcast:j reads the value of lane j of its row, so
+// one vector load feeds 64 FMAs with no broadcast instruction.  Coefficient
+// layout (k_vcoef), per element e, component c, row tile t, node b: CW
+// values, number u*RT + r = coefficient of sub-block kSub[c][u] between
+// output row t*RT + r and input node b (zero past the last row and past
+// 5 RT).
+template <int NL>
+struct VShape {
+    static constexpr int NEN = NL * NL;
+    static constexpr int NRT = (NEN + 24) / 25;                 // row tiles (at most 25 rows)
+    static constexpr int RT = (NEN + NRT - 1) / NRT;            // rows per tile
+    static constexpr int CW = (5 * RT + 31) / 32 * 32;          // values per node
+    static constexpr int NQ = CW / 32;                          // loads per node
+};
+
+__global__ void k_vcoef(int nen, int nrt, int rt, int cw, int nelem, const double *__restrict__ blk,
+                        double *__restrict__ vcf) {
+    const int np = nen * nen;
+    const size_t n = (size_t)nelem * 3 * nrt * nen * cw;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i % cw);
+        size_t r = i / cw;
+        const int b = (int)(r % nen);
+        r /= nen;
+        const int t = (int)(r % nrt);
+        r /= nrt;
+        const int c = (int)(r % 3);
+        const size_t e = r / 3;
+        const int u = k / rt, a = t * rt + k % rt;
+        double v = 0.0;
+        if (u < 5 && a < nen) {
+            const SubBlk sb = kSub[c][u];
+            const double *B = blk + (e * NBLK + sb.blk) * np;
+            v = sb.trans ? B[b * nen + a] : B[a * nen + b];
+        }
+        vcf[i] = v;
+    }
+}
+
+// acc += (value of lane J of this row of 16 lanes in cv) * op
+template <int J>
+__device__ __forceinline__ void fmac_bcast(double &acc, double cv, double op) {
+    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(cv), "v"(op), "i"(J));
+}
+
+// all FMAs of one coefficient load: values 32 q + 2 j + h, h = 0, 1
+template <int RT, int Q, int J>
+__device__ __forceinline__ void vstep_pair(double2 *acc, const double2 &cq, const double2 *op) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        constexpr int K0 = 32 * Q + 2 * J;
+        const int k = K0 + h;
+        if (k < 5 * RT) {
+            const int u = k / RT, r = k % RT;
+            const double cv = h ? cq.y : cq.x;
+            fmac_bcast<J>(acc[r].x, cv, op[u].x);
+            fmac_bcast<J>(acc[r].y, cv, op[u].y);
+        }
+    }
+}
+
+template <int RT, int Q, int... J>
+__device__ __forceinline__ void vstep_load(double2 *acc, const double2 &cq, const double2 *op,
+                                           std::integer_sequence<int, J...>) {
+    (vstep_pair<RT, Q, J>(acc, cq, op), ...);
+}
+
+// zpad: npair zero values (read in place of the row of a fixed DOF or of a
+// first touch), sink: npair values that absorb the stores of fixed DOFs, so
+// that every load and store of a tile is a uniform row base plus the lane's
+// pair offset and none is predicated per row.
+template <int NL, int C>
+__device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, const int *__restrict__ lme, int a0,
+                                             unsigned long long fmask, int npair, int pair, bool live,
+                                             double om, double kv, const double2 *__restrict__ x,
+                                             double2 *__restrict__ y, const double2 *__restrict__ zpad,
+                                             double2 *__restrict__ sink) {
+    typedef VShape<NL> V;
+    constexpr int NEN = V::NEN, RT = V::RT, CW = V::CW, NQ = V::NQ;
+    const int l16 = (int)(threadIdx.x & 15);
+    const double f3 = kv * kv, f4 = -om * om;
+    // a lane past npair computes pair 0 and stores nothing
+    const int pcl = live ? pair : 0;
+    // rows of this tile: start from y unless this element touches the node
+    // first (then y is written, not added to)
+    double2 acc[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        const int a = a0 + r;
+        const int dof = a < NEN ? lme[a * 3 + C] : -1;
+        const bool add = dof >= 0 && !((fmask >> a) & 1ull);
+        const double2 *row = add ? y + (size_t)dof * npair : zpad;
+        acc[r] = row[pcl];
+    }
+    // the y values are complete before the node loop (its waits then only
+    // concern the loop's own loads)
+#pragma unroll
+    for (int r = 0; r < RT; ++r) asm volatile("" : "+v"(acc[r].x), "+v"(acc[r].y));
+    const double2 *cp = reinterpret_cast<const double2 *>(cf) + l16;
+    // step b runs node b from xc, cc (loaded one step earlier) and loads node
+    // b + 1 into xn, cn: the x values once the operands of node b are formed,
+    // each group of 32 coefficients once its FMAs are issued, so that the
+    // loaded values can take the registers just freed.  The loop runs two
+    // steps per trip with the two register sets swapped (a one-set pipeline
+    // needs register copies at the back edge, which wait for the loads just
+    // issued).  Steps start at b = -1 (loads only): the compiler merges a
+    // load before the loop with the one in it into a single load at the top
+    // of the loop.  Loads past the last node read node 0 (in bounds, unused).
+    auto step = [&](int b, const double2 (&cc)[NQ], double2 (&cn)[NQ], const double2 (&xc)[3],
+                    double2 (&xn)[3]) {
+        const int bn = b + 1 < NEN ? b + 1 : 0;
+        const bool run = b >= 0 && b < NEN;
+        // operands of the five sub-blocks for node b
+        double2 op[5];
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+            const SubBlk sb = kSub[C][u];
+            const double2 xd = xc[sb.xcomp];
+            if (sb.kind == 0) op[u] = xd;
+            else if (sb.kind == 1) op[u] = make_double2(-kv * xd.y, kv * xd.x);   // +i ky x
+            else if (sb.kind == 2) op[u] = make_double2(kv * xd.y, -kv * xd.x);   // -i ky x
+            else if (sb.kind == 3) op[u] = make_double2(f3 * xd.x, f3 * xd.y);
+            else op[u] = make_double2(f4 * xd.x, f4 * xd.y);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int dof = lme[bn * 3 + d];
+            const double2 *row = dof >= 0 ? x + (size_t)dof * npair : zpad;
+            xn[d] = row[pcl];
+        }
+        const double2 *cpn = cp + (size_t)bn * (CW / 2);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            if (run) {
+                if (q == 0) vstep_load<RT, 0>(acc, cc[q], op, std::make_integer_sequence<int, 16>());
+                if (q == 1) vstep_load<RT, 1>(acc, cc[q], op, std::make_integer_sequence<int, 16>());
+                if (q == 2) vstep_load<RT, 2>(acc, cc[q], op, std::make_integer_sequence<int, 16>());
+                if (q == 3) vstep_load<RT, 3>(acc, cc[q], op, std::make_integer_sequence<int, 16>());
+                if (q == 4) vstep_load<RT, 4>(acc, cc[q], op, std::make_integer_sequence<int, 16>());
+            }
+            cn[q] = cpn[q * 16];
+        }
+    };
+    double2 xa[3], xb[3], ca[NQ], cb[NQ];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xa[d] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) ca[q] = make_double2(0.0, 0.0);
+#pragma unroll 1
+    for (int b = -1; b < NEN; b += 2) {
+        step(b, ca, cb, xa, xb);
+        step(b + 1, cb, ca, xb, xa);
+    }
+    // read the row DOFs again rather than holding them through the loop
+    asm volatile("" ::: "memory");
+    if (live) {
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const int a = a0 + r;
+            const int dof = a < NEN ? lme[a * 3 + C] : -1;
+            double2 *row = dof >= 0 ? y + (size_t)dof * npair : sink;
+            row[pair] = acc[r];
+        }
+    }
+}
+
+// One workgroup = one element of the colour and one group of 64 pairs: a
+// wave per (component, row tile), 3 NRT waves, all reading the same x rows
+// at about the same time (one fetch from HBM serves them).  The pair groups
+// of one element read the same coefficients: workgroup index
+// i = (e / 8) * 8 ngrp + 8 g + e % 8 deals them to one XCD (workgroups go
+// to the 8 XCDs round robin).  168 registers: three waves per SIMD, four
+// workgroups per CU.  (Measured at 200x100 elements, p4, 128 pairs: two
+// pair groups per workgroup, 6 waves, fit only two workgroups per CU and
+// ran 0.69 instead of 0.63 ms per launch; 13-row tiles at four waves per
+// SIMD ran 0.72 ms.)
+template <int NL>
+__global__ __launch_bounds__((64 * 3 * VShape<NL>::NRT), 3) void k_apply_v(
+        const int *__restrict__ order, int nel, const double *__restrict__ vcf, const int *__restrict__ lm,
+        const unsigned long long *__restrict__ first, int npair, const double *__restrict__ omega,
+        const double *__restrict__ ky, const double2 *__restrict__ x, double2 *__restrict__ y,
+        const double2 *__restrict__ zpad, double2 *__restrict__ sink) {
+    typedef VShape<NL> V;
+    static_assert(V::NQ <= 5, "coefficient loads per node");
+    constexpr int NEN = V::NEN, NRT = V::NRT;
+    constexpr size_t TS = (size_t)NEN * V::CW;   // values per (element, component, row tile)
+    const int ngrp = (npair + 63) >> 6;
+    const int i = blockIdx.x;
+    const int ei = (i / (8 * ngrp)) * 8 + i % 8, g = (i / 8) % ngrp;
+    if (ei >= nel) return;
+    const int e = order[ei];
+    const int ct = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const unsigned long long fmask = first[e];
+    const int *lme = lm + (size_t)e * 3 * NEN;
+    const int c = ct / NRT, t = ct % NRT;
+    const int pair = g * 64 + lane;
+    const bool live = pair < npair;
+    const double om = omega[live ? pair : 0], kv = ky[live ? pair : 0];
+    const double *cf = vcf + ((size_t)e * 3 * NRT + ct) * TS;
+    const int a0 = t * V::RT;
+    if (c == 0) apply_v_tile<NL, 0>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
+    else if (c == 1) apply_v_tile<NL, 1>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
+    else apply_v_tile<NL, 2>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
 }
 
 // ---------------------------------------------------------------------------

@@ -106,6 +106,26 @@ def test_apply_batch_fp64(name):
         assert relerr(y[b], S @ x[b]) < FP64_TOL, b
 
 
+def test_apply_batch_fp64_pair_groups():
+    """70 pairs: two groups of 64 lanes, the second partly empty.  A pair
+    repeated in both groups gives the same bits; pairs at both group edges
+    match the oracle operator."""
+    mi = _mi("tiny_layered")
+    om, ky, f, py = pairs(7, 10)
+    assert om.size == 70
+    for a, b in ((67, 3), (64, 63)):
+        om[a], ky[a], f[a], py[a] = om[b], ky[b], f[b], py[b]
+    g = oracle_graph("tiny_layered")
+    x = cf.seeded_complex(g.ndof, batch=om.size, seed=21)
+    x[67] = x[3]
+    x[64] = x[63]
+    y = mi.apply_batch(om, ky, x)
+    assert np.array_equal(y[67], y[3]) and np.array_equal(y[64], y[63])
+    for b in (0, 3, 62, 63, 65, 69):
+        S = oracle_S("tiny_layered", f[b], py[b])
+        assert relerr(y[b], S @ x[b]) < FP64_TOL, b
+
+
 def test_apply_batch_config1_modes(torch_dev):
     torch = torch_dev
     from fem25_amd import _lib
