@@ -613,6 +613,13 @@ struct VShape {
     static constexpr int NQ = CW / 32;                          // loads per node
 };
 
+// Sub-blocks of output component c (indices into kSub[c]) that act on input
+// component c, in the order kind 0, 3 (ky^2), 4 (-omega^2): their
+// coefficients are combined per pair into g = A + ky^2 B - omega^2 C before
+// one complex FMA (5 instructions instead of 6); and the two others.
+__device__ constexpr int kDiagU[3][3] = {{0, 3, 4}, {0, 3, 4}, {1, 3, 4}};
+__device__ constexpr int kOffU[3][2] = {{1, 2}, {1, 2}, {0, 2}};
+
 __global__ void k_vcoef(int nen, int nrt, int rt, int cw, int nelem, const double *__restrict__ blk,
                         double *__restrict__ vcf) {
     const int np = nen * nen;
@@ -626,7 +633,15 @@ __global__ void k_vcoef(int nen, int nrt, int rt, int cw, int nelem, const doubl
         r /= nrt;
         const int c = (int)(r % 3);
         const size_t e = r / 3;
-        const int u = k / rt, a = t * rt + k % rt;
+        int u = 5, row = 0;
+        if (k < 3 * rt) {
+            u = kDiagU[c][k % 3];
+            row = k / 3;
+        } else if (k < 5 * rt) {
+            u = kOffU[c][(k - 3 * rt) / rt];
+            row = (k - 3 * rt) % rt;
+        }
+        const int a = t * rt + row;
         double v = 0.0;
         if (u < 5 && a < nen) {
             const SubBlk sb = kSub[c][u];
@@ -645,26 +660,52 @@ __device__ __forceinline__ void fmac_bcast(double &acc, double cv, double op) {
         : "v"(cv), "v"(op), "i"(J));
 }
 
-// all FMAs of one coefficient load: values 32 q + 2 j + h, h = 0, 1
+// g = value of lane J of this row of 16 lanes in cv
+template <int J>
+__device__ __forceinline__ void mov_bcast(double &g, double cv) {
+    asm("v_mov_b64_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf" : "=v"(g) : "v"(cv), "i"(J));
+}
+
+// Per node, coefficient k of a row tile: k < 3 RT: row k / 3, sub-block
+// kDiagU[.][k % 3] (g is formed over the three, then acc += g x_c);
+// 3 RT <= k < 5 RT: sub-block kOffU[.][(k - 3 RT) / RT], row (k - 3 RT) % RT.
+// All instructions of one coefficient load: values 32 q + 2 j + h, h = 0, 1.
+struct VOps {
+    double2 xd;      // x of the component's own input component
+    double2 op[2];   // operands of the two other sub-blocks
+    double f3, f4;   // ky^2, -omega^2
+};
+
 template <int RT, int Q, int J>
-__device__ __forceinline__ void vstep_pair(double2 *acc, const double2 &cq, const double2 *op) {
+__device__ __forceinline__ void vstep_pair(double2 *acc, double &g, const double2 &cq, const VOps &o) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         constexpr int K0 = 32 * Q + 2 * J;
         const int k = K0 + h;
-        if (k < 5 * RT) {
-            const int u = k / RT, r = k % RT;
-            const double cv = h ? cq.y : cq.x;
-            fmac_bcast<J>(acc[r].x, cv, op[u].x);
-            fmac_bcast<J>(acc[r].y, cv, op[u].y);
+        const double cv = h ? cq.y : cq.x;
+        if (k < 3 * RT) {
+            const int r = k / 3, s = k % 3;
+            if (s == 0) {
+                mov_bcast<J>(g, cv);
+            } else if (s == 1) {
+                fmac_bcast<J>(g, cv, o.f3);
+            } else {
+                fmac_bcast<J>(g, cv, o.f4);
+                acc[r].x = fma(g, o.xd.x, acc[r].x);
+                acc[r].y = fma(g, o.xd.y, acc[r].y);
+            }
+        } else if (k < 5 * RT) {
+            const int u = (k - 3 * RT) / RT, r = (k - 3 * RT) % RT;
+            fmac_bcast<J>(acc[r].x, cv, o.op[u].x);
+            fmac_bcast<J>(acc[r].y, cv, o.op[u].y);
         }
     }
 }
 
 template <int RT, int Q, int... J>
-__device__ __forceinline__ void vstep_load(double2 *acc, const double2 &cq, const double2 *op,
+__device__ __forceinline__ void vstep_load(double2 *acc, double &g, const double2 &cq, const VOps &o,
                                            std::integer_sequence<int, J...>) {
-    (vstep_pair<RT, Q, J>(acc, cq, op), ...);
+    (vstep_pair<RT, Q, J>(acc, g, cq, o), ...);
 }
 
 // zpad: npair zero values (read in place of the row of a fixed DOF or of a
@@ -712,18 +753,20 @@ __device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, cons
                     double2 (&xn)[3]) {
         const int bn = b + 1 < NEN ? b + 1 : 0;
         const bool run = b >= 0 && b < NEN;
-        // operands of the five sub-blocks for node b
-        double2 op[5];
+        // operands for node b
+        VOps o;
+        o.xd = xc[C];
+        o.f3 = f3;
+        o.f4 = f4;
 #pragma unroll
-        for (int u = 0; u < 5; ++u) {
-            const SubBlk sb = kSub[C][u];
+        for (int v = 0; v < 2; ++v) {
+            const SubBlk sb = kSub[C][kOffU[C][v]];
             const double2 xd = xc[sb.xcomp];
-            if (sb.kind == 0) op[u] = xd;
-            else if (sb.kind == 1) op[u] = make_double2(-kv * xd.y, kv * xd.x);   // +i ky x
-            else if (sb.kind == 2) op[u] = make_double2(kv * xd.y, -kv * xd.x);   // -i ky x
-            else if (sb.kind == 3) op[u] = make_double2(f3 * xd.x, f3 * xd.y);
-            else op[u] = make_double2(f4 * xd.x, f4 * xd.y);
+            if (sb.kind == 1) o.op[v] = make_double2(-kv * xd.y, kv * xd.x);        // +i ky x
+            else if (sb.kind == 2) o.op[v] = make_double2(kv * xd.y, -kv * xd.x);   // -i ky x
+            else o.op[v] = xd;
         }
+        double g = 0.0;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int dof = lme[bn * 3 + d];
@@ -734,11 +777,11 @@ __device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, cons
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             if (run) {
-                if (q == 0) vstep_load<RT, 0>(acc, cc[q], op, std::make_integer_sequence<int, 16>());
-                if (q == 1) vstep_load<RT, 1>(acc, cc[q], op, std::make_integer_sequence<int, 16>());
-                if (q == 2) vstep_load<RT, 2>(acc, cc[q], op, std::make_integer_sequence<int, 16>());
-                if (q == 3) vstep_load<RT, 3>(acc, cc[q], op, std::make_integer_sequence<int, 16>());
-                if (q == 4) vstep_load<RT, 4>(acc, cc[q], op, std::make_integer_sequence<int, 16>());
+                if (q == 0) vstep_load<RT, 0>(acc, g, cc[q], o, std::make_integer_sequence<int, 16>());
+                if (q == 1) vstep_load<RT, 1>(acc, g, cc[q], o, std::make_integer_sequence<int, 16>());
+                if (q == 2) vstep_load<RT, 2>(acc, g, cc[q], o, std::make_integer_sequence<int, 16>());
+                if (q == 3) vstep_load<RT, 3>(acc, g, cc[q], o, std::make_integer_sequence<int, 16>());
+                if (q == 4) vstep_load<RT, 4>(acc, g, cc[q], o, std::make_integer_sequence<int, 16>());
             }
             cn[q] = cpn[q * 16];
         }
