@@ -224,7 +224,16 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     kern = {}
     # assembled FP64 operator: one pass over the CSR basis (HBM)
     k_ms, wall = timed(lambda: mi.asmble_dev(b"B", float(freq[0]), float(py[0]), s), L, b"asmble", 5)
-    nzb = mi.nz * (3 * 8 + 16 + 4 + 1) + (ndof + 1) * 8
+    # bytes of one pass: per entry its kind (1), V0 (8) and the value (16);
+    # V1, V2 (16) for the same-component entries only; a base per 64 entries
+    gr = mi.graph()
+    comp = np.zeros(ndof, np.int8)
+    flat = gr.lm.reshape(-1)
+    comp[flat[flat > 0] - 1] = (np.nonzero(flat > 0)[0] % 3).astype(np.int8)
+    rows = np.repeat(np.arange(ndof), np.diff(gr.irptr.astype(np.int64)))
+    n1 = int(np.count_nonzero(comp[rows] == comp[gr.jcptr - 1]))
+    del rows, flat, gr
+    nzb = mi.nz * (1 + 8 + 16) + n1 * 16 + (mi.nz // 64 + 1) * 8
     kern["asmble_fp64"] = {"kernel_ms": round(k_ms, 3), "calls_per_s": round(1e3 / wall, 1),
                            "roofline": roof("hbm", nzb / (k_ms * 1e-3), "GB/s")}
     del s

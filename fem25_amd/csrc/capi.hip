@@ -112,7 +112,10 @@ struct f25_mesh_s {
     int64_t *d_irptr = nullptr;
     int *d_jcptr = nullptr;
     signed char *d_comp = nullptr;
-    double *d_v0 = nullptr, *d_v1 = nullptr, *d_v2 = nullptr;
+    double *d_v0 = nullptr, *d_v1 = nullptr, *d_v2 = nullptr;   // V1, V2 compact (kind-0 entries)
+    signed char *d_kind = nullptr;   // CSR value kind per entry (k_csr_kinds)
+    int64_t *d_cbase = nullptr;      // kind-0 entries before each chunk of 64 entries
+    int64_t n1 = 0;
     bool basis_ok = false;
     double *d_part = nullptr;
     int *d_mconn = nullptr;
@@ -379,6 +382,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_vcf); dfree(m->d_dvcf); dfree(m->d_vpad);
     dfree(m->d_blk); dfree(m->d_blk32); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_colmax); dfree(m->d_dblk); dfree(m->d_irptr);
     dfree(m->d_jcptr); dfree(m->d_comp); dfree(m->d_v0); dfree(m->d_v1); dfree(m->d_v2);
+    dfree(m->d_kind); dfree(m->d_cbase);
     dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_pmap); dfree(m->d_qp); dfree(m->d_qs);
     dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc);
     dfree(m->d_ew);
@@ -533,26 +537,74 @@ static int require_elastic(f25_mesh *m) {
 }
 
 // ---------------------------------------------------------------------------
+// value kinds of the CSR entries and the chunk bases of the compact V1, V2
+static int ensure_kinds(f25_mesh *m) {
+    if (m->d_kind) return 0;
+    const int ndof = m->g.ndof;
+    const int64_t nz = m->g.irptr[ndof];
+    const int64_t nchunk = (nz + 63) / 64;
+    if (nchunk / 4 + 1 > 0x7fffffff) FAIL(F25_ERR_ARG, "too many CSR entries");
+    int *d_cnt = nullptr;
+    TRY(dalloc(&m->d_kind, (size_t)std::max<int64_t>(nz, 1)));
+    TRY(dalloc(&d_cnt, (size_t)std::max<int64_t>(nchunk, 1)));
+    hipLaunchKernelGGL(k_csr_kinds, dim3((ndof + 3) / 4), dim3(256), 0, m->st, ndof, m->d_irptr, m->d_jcptr,
+                       m->d_comp, m->d_kind);
+    if (nchunk > 0)
+        hipLaunchKernelGGL(k_csr_chunks, dim3((unsigned)((nchunk + 3) / 4)), dim3(256), 0, m->st, nz, m->d_kind,
+                           d_cnt);
+    std::vector<int> cnt(nchunk);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && nchunk > 0)
+        e = hipMemcpyAsync(cnt.data(), d_cnt, (size_t)nchunk * sizeof(int), hipMemcpyDeviceToHost, m->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+    dfree(d_cnt);
+    if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
+    std::vector<int64_t> base(nchunk + 1, 0);
+    for (int64_t q = 0; q < nchunk; ++q) base[q + 1] = base[q] + cnt[q];
+    m->n1 = base[nchunk];
+    TRY(upload(&m->d_cbase, base.data(), base.size(), m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    return 0;
+}
+
+// grid of a chunk-stream kernel: enough workgroups of 4 waves to fill the
+// chip several times over, each wave then striding over the chunks
+static unsigned chunk_grid(int64_t nz) {
+    const int64_t nchunk = (nz + 63) / 64;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((nchunk + 3) / 4, 8192));
+}
+
 static int ensure_basis(f25_mesh *m) {
     TRY(ensure_csr(m));
+    TRY(ensure_kinds(m));
     if (m->basis_ok) return 0;
-    const size_t nz = m->g.irptr[m->g.ndof];
+    const int ndof = m->g.ndof;
+    const size_t nz = m->g.irptr[ndof];
     if (!m->d_v0) {
         TRY(dalloc(&m->d_v0, nz));
-        TRY(dalloc(&m->d_v1, nz));
-        TRY(dalloc(&m->d_v2, nz));
+        TRY(dalloc(&m->d_v1, (size_t)std::max<int64_t>(m->n1, 1)));
+        TRY(dalloc(&m->d_v2, (size_t)std::max<int64_t>(m->n1, 1)));
     }
     fill_async(m->d_v0, nz, 0.0, m->st);
-    fill_async(m->d_v1, nz, 0.0, m->st);
-    fill_async(m->d_v2, nz, 0.0, m->st);
+    fill_async(m->d_v1, (size_t)m->n1, 0.0, m->st);
+    fill_async(m->d_v2, (size_t)m->n1, 0.0, m->st);
+    // compact positions, needed only while the basis is scattered
+    int64_t *d_cidx = nullptr;
+    TRY(dalloc(&d_cidx, nz));
+    if (nz > 0)
+        hipLaunchKernelGGL(k_csr_cidx, dim3((unsigned)((((int64_t)nz + 63) / 64 + 3) / 4)), dim3(256), 0, m->st,
+                           (int64_t)nz, m->d_cbase, m->d_kind, d_cidx);
     for (size_t c = 0; c + 1 < m->cptr.size(); ++c) {
         int n = m->cptr[c + 1] - m->cptr[c];
         if (!n) continue;
         hipLaunchKernelGGL(k_csr_basis, dim3(n), dim3(256), 0, m->st, m->tab.nen, n,
-                           m->d_order + m->cptr[c], m->d_blk, m->d_lm, m->d_irptr, m->d_jcptr,
+                           m->d_order + m->cptr[c], m->d_blk, m->d_lm, m->d_irptr, m->d_jcptr, d_cidx,
                            m->d_cdom, m->d_v0, m->d_v1, m->d_v2, m->d_err);
     }
-    HIPCHK(hipGetLastError());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+    dfree(d_cidx);
+    if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
     TRY(check_err(m));
     m->basis_ok = true;
     return 0;
@@ -569,8 +621,9 @@ static int asmble_impl(f25_mesh *m, char cftype, double freq, double py, CT *out
     if (disp)
         fill_async(out, (size_t)m->g.irptr[ndof], CT{0, 0}, m->st);
     else
-        hipLaunchKernelGGL((k_csr_combine<CT>), dim3((ndof + 3) / 4), dim3(256), 0, m->st, ndof,
-                           m->d_irptr, m->d_jcptr, m->d_comp, m->d_v0, m->d_v1, m->d_v2, omega, kv, out);
+        hipLaunchKernelGGL((k_csr_combine<CT>), dim3(chunk_grid(m->g.irptr[ndof])), dim3(256), 0, m->st,
+                           (int64_t)m->g.irptr[ndof], m->d_cbase, m->d_kind, m->d_v0, m->d_v1, m->d_v2, omega, kv,
+                           out);
     const std::vector<int> &cp = disp ? m->cptr : m->pml_cptr;
     const int *ord = disp ? m->d_order : m->d_pml_order;
     if (disp || !m->pml_order.empty()) {

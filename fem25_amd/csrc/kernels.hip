@@ -861,11 +861,53 @@ __device__ inline int64_t find_col(const int *__restrict__ jc, int64_t b, int64_
     return -1;
 }
 
+// CSR value kinds, by the components of row and column: 0 same component
+// (V0 + ky^2 V1 - omega^2 V2), 1 the 1-3 coupling (V0), 2 the 1-2 and 3-2
+// couplings (-i ky V0), 3 the 2-1 and 2-3 couplings (+i ky V0).  V1 and V2
+// are stored only for kind 0, compactly in entry order: the kind-0 entries
+// before chunk q of 64 entries number cbase[q].  One wave per row.
+__global__ void k_csr_kinds(int ndof, const int64_t *__restrict__ irptr, const int *__restrict__ jcptr,
+                            const signed char *__restrict__ comp, signed char *__restrict__ kind) {
+    const int r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (r >= ndof) return;
+    const int cr = comp[r];
+    for (int64_t s = irptr[r] + (threadIdx.x & 63); s < irptr[r + 1]; s += 64) {
+        const int cc = comp[jcptr[s]];
+        int k;
+        if (cr == cc) k = 0;
+        else if ((cr == 0 && cc == 2) || (cr == 2 && cc == 0)) k = 1;
+        else k = (cr == 1) ? 3 : 2;
+        kind[s] = (signed char)k;
+    }
+}
+
+// kind-0 entries per chunk of 64 (one wave per chunk)
+__global__ void k_csr_chunks(int64_t nz, const signed char *__restrict__ kind, int *__restrict__ cnt) {
+    const int64_t q = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int64_t s = q * 64 + (threadIdx.x & 63);
+    if (q * 64 >= nz) return;
+    const unsigned long long msk = __ballot(s < nz && kind[s] == 0);
+    if ((threadIdx.x & 63) == 0) cnt[q] = __popcll(msk);
+}
+
+// compact position of every kind-0 entry (-1 for the others)
+__global__ void k_csr_cidx(int64_t nz, const int64_t *__restrict__ cbase, const signed char *__restrict__ kind,
+                           int64_t *__restrict__ cidx) {
+    const int64_t q = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
+    const int64_t s = q * 64 + lane;
+    if (q * 64 >= nz) return;
+    const bool same = s < nz && kind[s] == 0;
+    const unsigned long long msk = __ballot(same);
+    if (s < nz) cidx[s] = same ? cbase[q] + __popcll(msk & ((1ull << lane) - 1)) : -1;
+}
+
 __global__ void k_csr_basis(int nen, int ncol_elem, const int *__restrict__ order,
                             const double *__restrict__ blk, const int *__restrict__ lm,
                             const int64_t *__restrict__ irptr, const int *__restrict__ jcptr,
-                            const char *__restrict__ cdom, double *__restrict__ v0,
-                            double *__restrict__ v1, double *__restrict__ v2, int *__restrict__ err) {
+                            const int64_t *__restrict__ cidx, const char *__restrict__ cdom,
+                            double *__restrict__ v0, double *__restrict__ v1, double *__restrict__ v2,
+                            int *__restrict__ err) {
     const int e = order[blockIdx.x];
     if (cdom[e] == 'A') return;
     const int np = nen * nen;
@@ -882,9 +924,10 @@ __global__ void k_csr_basis(int nen, int ncol_elem, const int *__restrict__ orde
         if (ci == cj) {
             int kb = ci == 0 ? B_K11 : (ci == 1 ? B_K22 : B_K33);
             int mb = ci == 1 ? B_ML2M : B_MMU;
+            const int64_t j = cidx[s];
             v0[s] += B[kb * np + ab];
-            v1[s] += B[mb * np + ab];
-            v2[s] += B[B_MRHO * np + ab];
+            v1[j] += B[mb * np + ab];
+            v2[j] += B[B_MRHO * np + ab];
         } else if (ci == 0 && cj == 2) v0[s] += B[B_K13 * np + ab];
         else if (ci == 2 && cj == 0) v0[s] += B[B_K13 * np + ba];
         else if (ci == 0 && cj == 1) v0[s] += B[B_T12 * np + ab];
@@ -896,24 +939,32 @@ __global__ void k_csr_basis(int nen, int ncol_elem, const int *__restrict__ orde
 
 // S(omega, ky) from the basis: same component: V0 + ky^2 V1 - omega^2 V2,
 // 1-3 coupling: V0, 1-2/2-3 couplings: -/+ i ky V0 (block signs above).
+// A stream over the entries, one wave per chunk of 64 (grid-stride): the
+// compact V1/V2 position of a kind-0 entry is its chunk's base plus the
+// kind-0 entries before it in the chunk (ballot and popcount).  Bytes per
+// entry: kind 1 + V0 8 + the value; V1, V2 only for kind 0.
 template <typename CT>
-__global__ void k_csr_combine(int ndof, const int64_t *__restrict__ irptr, const int *__restrict__ jcptr,
-                              const signed char *__restrict__ comp, const double *__restrict__ v0,
-                              const double *__restrict__ v1, const double *__restrict__ v2, double om,
-                              double kv, CT *__restrict__ out) {
-    const int wpb = blockDim.x / 64;
-    const int r = blockIdx.x * wpb + threadIdx.x / 64;
-    if (r >= ndof) return;
-    const int cr = comp[r];
+__global__ void k_csr_combine(int64_t nz, const int64_t *__restrict__ cbase, const signed char *__restrict__ kind,
+                              const double *__restrict__ v0, const double *__restrict__ v1,
+                              const double *__restrict__ v2, double om, double kv, CT *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
     const double k2 = kv * kv, w2 = om * om;
-    for (int64_t s = irptr[r] + (threadIdx.x & 63); s < irptr[r + 1]; s += 64) {
-        const int cc = comp[jcptr[s]];
+    const int64_t nchunk = (nz + 63) / 64, step = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t q = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; q < nchunk; q += step) {
+        const int64_t s = q * 64 + lane;
+        const bool in = s < nz;
+        const int k = in ? kind[s] : -1;
+        const unsigned long long msk = __ballot(k == 0);
+        if (!in) continue;
+        const double a = v0[s];
         double re = 0.0, im = 0.0;
-        if (cr == cc) re = v0[s] + k2 * v1[s] - w2 * v2[s];
-        else if ((cr == 0 && cc == 2) || (cr == 2 && cc == 0)) re = v0[s];
-        else {
-            double sg = ((cr == 0 && cc == 1) || (cr == 2 && cc == 1)) ? -1.0 : 1.0;
-            im = sg * kv * v0[s];
+        if (k == 0) {
+            const int64_t j = cbase[q] + __popcll(msk & ((1ull << lane) - 1));
+            re = a + k2 * v1[j] - w2 * v2[j];
+        } else if (k == 1) {
+            re = a;
+        } else {
+            im = (k == 2 ? -kv : kv) * a;
         }
         out[s].x = re;
         out[s].y = im;
