@@ -2032,8 +2032,8 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
         r = F25_ERR_ARG;
     }
     const int nt1 = 256, qg = std::max(1, 256 / nee);
-    const size_t lds1 = nee * sizeof(double2) + (size_t)nq * 28 * sizeof(double) +
-                        (size_t)(qg - 1) * 12 * nee * sizeof(double2);
+    const size_t lds1 = nee * sizeof(double2) + std::max((size_t)nq * 28 * sizeof(double),
+                                                         (size_t)(qg - 1) * 12 * nee * sizeof(double2));
     const int nt2 = 256;
     const size_t lds2 = (size_t)m->ncon_anchor * nee * 3 * sizeof(double2) + nt2 * 3 * sizeof(double);
     if (!r && lds1 > 65536)

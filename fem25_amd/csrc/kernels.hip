@@ -1403,7 +1403,9 @@ __global__ __launch_bounds__(256) void k_fcols(const double *__restrict__ shl, c
     const int e = blockIdx.x;
     double2 *su = reinterpret_cast<double2 *>(sm);   // nee
     double *sq = reinterpret_cast<double *>(su + nee);   // nq * QS
-    double2 *spart = reinterpret_cast<double2 *>(sq + (size_t)nq * QS);   // (QG-1) * 12 * nee
+    // phase 2's partial sums reuse the quadrature-point storage once every
+    // group is done reading it: (QG-1) * 12 * nee
+    double2 *spart = reinterpret_cast<double2 *>(sq);
     const int *ie = ieng + 4 * e;
     const int *L = lm + (size_t)e * nee;
     double2 *o = out + (size_t)e * 12 * nee;
@@ -1493,6 +1495,7 @@ __global__ __launch_bounds__(256) void k_fcols(const double *__restrict__ shl, c
     }
     // groups 1.. hand their partial sums to group 0
     if (QG > 1) {
+        __syncthreads();
         if (rowt && grp > 0)
 #pragma unroll
             for (int c = 0; c < 4; ++c)
