@@ -1337,9 +1337,13 @@ static int ensure_dab_part(f25_mesh *m) {
     return 0;
 }
 
-static size_t dab_lds(const f25_mesh *m) {
+// LDS of k_dab for nvec vectors: the element's vectors (room for two), the
+// geometry per quadrature point, and the fluxes (8 complex per vector and
+// point) whose space the projection's partial sums reuse
+static size_t dab_lds(const f25_mesh *m, int nvec = 2) {
     const int nee = 3 * m->tab.nen, nq = m->tab.nq, qg = std::max(1, 256 / nee);
-    return (size_t)(2 * nee + 18 * nq + 2 * qg * nee) * sizeof(double2) + 4 * (size_t)nq * sizeof(double);
+    return (size_t)2 * nee * sizeof(double2) + 4 * (size_t)nq * sizeof(double) +
+           (size_t)nvec * std::max(8 * nq, qg * nee) * sizeof(double2);
 }
 
 // k_dab for the mesh's order (the element loops are compiled per order)
@@ -1401,7 +1405,7 @@ static int dab_launch_columns(f25_mesh *m, double kv, int nvec, const std::vecto
     int r = upload(&de, elem.data(), elem.size(), m->st);
     if (!r) r = upload(&dc, coef.data(), coef.size(), m->st);
     if (!r) {
-        hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(inv.nnpgl), dim3(256), dab_lds(m), m->st, 1,
+        hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(inv.nnpgl), dim3(256), dab_lds(m, nvec), m->st, 1,
                            inv.nnpgl, nvec, inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de, dc, m->d_shlT, m->d_wq, m->d_xq, m->d_ieng,
                            m->d_xl, m->d_zl, m->d_lm, x0, x1, (double2 *)nullptr, (double2 *)nullptr, z0, z1,
                            m->d_mygrad, m->d_jcsc, m->d_icsc, p0, p1, (float2 *)nullptr, m->d_err);
@@ -1450,7 +1454,7 @@ extern "C" int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *w
     if (!r) r = upload(&dc, coef.data(), coef.size(), m->st);
     if (!r) {
         (void)hipEventRecord(m->ev0, m->st);
-        hipLaunchKernelGGL(dab_kernel(m->tab.nl, false), dim3(n), dim3(256), dab_lds(m), m->st, 2, n, 1, kp, ks, kv,
+        hipLaunchKernelGGL(dab_kernel(m->tab.nl, false), dim3(n), dim3(256), dab_lds(m, 1), m->st, 2, n, 1, kp, ks, kv,
                            de, dc, m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, dw,
                            (const float2 *)nullptr, (double2 *)nullptr, (double2 *)nullptr, (const float2 *)nullptr,
                            (const float2 *)nullptr, m->d_mygrad, m->d_jcsc, m->d_icsc, (float *)nullptr,
@@ -1514,7 +1518,7 @@ extern "C" int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float 
         for (size_t col = 0; col + 1 < inv.dab_cptr.size(); ++col) {
             const int o = inv.dab_cptr[col], n = inv.dab_cptr[col + 1] - o;
             if (n <= 0) continue;
-            hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(n), dim3(256), dab_lds(m), m->st, 0, n, 2,
+            hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(n), dim3(256), dab_lds(m, 2), m->st, 0, n, 2,
                                inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de + o, dc + 4 * (size_t)o,
                                m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, du, dl, ya, yb,
                                (const float2 *)nullptr, (const float2 *)nullptr, m->d_mygrad, m->d_jcsc,

@@ -1612,7 +1612,9 @@ __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *
 //   mode 2 (item = local column c): column c of F in CSC order, COMPLEX*8,
 //          zero outside the element's rows.
 // Every row of modes 1 and 2 is checked against column c's CSC pattern.
-// Flux layout per (vector, q): [comp][N, Nx, Nz] complex.
+// Flux layout per (vector, q), 8 complex: comp 1 N, Nx, Nz; comp 2 N, Nx,
+// Nz; comp 3 N, Nz (comp 3's Nx flux equals comp 1's Nz flux).  LDS holds
+// the fluxes of nvec vectors; the projection's partial sums reuse it.
 template <int NL, bool IMAG>
 __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, double kp, double ks,
                                              double kv, const int *__restrict__ item_elem,
@@ -1637,9 +1639,9 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, dou
     constexpr int nen = NL * NL, nq1 = 2 * NL + 3, nq = nq1 * nq1, nee = 3 * nen;
     const int QG = max(1, (int)blockDim.x / nee);              // q groups of the projection
     double2 *su = reinterpret_cast<double2 *>(sm);           // [v][3 nen]
-    double2 *sF = su + 2 * nee;                               // [v][q][9]
-    double2 *sP = sF + 2 * 9 * nq;                            // [v][group][row] partial projections
-    double *sG = reinterpret_cast<double *>(sP + 2 * QG * nee);   // [q][4]: rx, sx, rz, sz
+    double *sG = reinterpret_cast<double *>(su + 2 * nee);   // [q][4]: rx, sx, rz, sz
+    double2 *sF = reinterpret_cast<double2 *>(sG + 4 * nq);  // [v][q][8]
+    double2 *sP = sF;   // [v][group][row] partial projections, after the fluxes are read
     const int *ie = ieng + 4 * e;
     const int *L = lm + (size_t)e * nee;
     if (mode == 2)
@@ -1683,7 +1685,7 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, dou
                     UZ[cc].x += nz * w.x; UZ[cc].y += nz * w.y;
                 }
             }
-            double2 *F = sF + ((size_t)v * nq + q) * 9;
+            double2 *F = sF + ((size_t)v * nq + q) * 8;
             // ky coupling: mk(X) = i kv X (IMAG) or kv X
             auto mk = [&](double2 X) {
                 return IMAG ? make_double2(-kv * X.y, kv * X.x) : make_double2(kv * X.x, kv * X.y);
@@ -1702,8 +1704,7 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, dou
             F[5] = lin(S, UZ[1], -S, mk(U[2]));
             // comp 3: N: S mk(UZ2) + kv^2 S U3; Nx: S (UZ1 + UX3); Nz: Q UX1 + P UZ3 - Q mk(U2)
             F[6] = lin(S, mk(UZ[1]), k2 * S, U[2]);
-            F[7] = F[2];
-            F[8] = make_double2(Q * UX[0].x + P * UZ[2].x - Q * m2.x, Q * UX[0].y + P * UZ[2].y - Q * m2.y);
+            F[7] = make_double2(Q * UX[0].x + P * UZ[2].x - Q * m2.x, Q * UX[0].y + P * UZ[2].y - Q * m2.y);
         }
     }
     __syncthreads();
@@ -1711,9 +1712,11 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, dou
     // range of every row, then one pass adds the groups
     {
         const int i = threadIdx.x % nee, grp = threadIdx.x / nee;
+        double2 val[2] = {};
         if (grp < QG) {
             const int a = i / 3, cc = i % 3;
-            double2 val[2] = {};
+            // flux slots of this component: N, Nx, Nz
+            const int fN = 3 * cc, fX = cc == 2 ? 2 : 3 * cc + 1, fZ = cc == 2 ? 7 : 3 * cc + 2;
 #pragma unroll 4
             for (int q = grp; q < nq; q += QG) {
                 const double *G = sG + 4 * q;
@@ -1723,18 +1726,19 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, dou
 #pragma unroll
                 for (int v = 0; v < 2; ++v) {
                     if (v >= nvec) break;
-                    const double2 *F = sF + ((size_t)v * nq + q) * 9 + 3 * cc;
-                    val[v].x += n * F[0].x + nx * F[1].x + nz * F[2].x;
-                    val[v].y += n * F[0].y + nx * F[1].y + nz * F[2].y;
+                    const double2 *F = sF + ((size_t)v * nq + q) * 8;
+                    val[v].x += n * F[fN].x + nx * F[fX].x + nz * F[fZ].x;
+                    val[v].y += n * F[fN].y + nx * F[fX].y + nz * F[fZ].y;
                 }
             }
-#pragma unroll
-            for (int v = 0; v < 2; ++v) if (v < nvec) sP[((size_t)v * QG + grp) * nee + i] = val[v];
         }
-        // rows beyond the block size (nee > blockDim, one group)
-        for (int r = threadIdx.x + blockDim.x; r < nee; r += blockDim.x) {
+        // rows beyond the block size (nee > blockDim, one group): at most
+        // one more row per thread (nee < 2 blockDim)
+        double2 valr[2] = {};
+        const int r = threadIdx.x + blockDim.x;
+        if (r < nee) {
             const int a = r / 3, cc = r % 3;
-            double2 val[2] = {};
+            const int fN = 3 * cc, fX = cc == 2 ? 2 : 3 * cc + 1, fZ = cc == 2 ? 7 : 3 * cc + 2;
             for (int q = 0; q < nq; ++q) {
                 const double *G = sG + 4 * q;
                 const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
@@ -1743,14 +1747,20 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, dou
 #pragma unroll
                 for (int v = 0; v < 2; ++v) {
                     if (v >= nvec) break;
-                    const double2 *F = sF + ((size_t)v * nq + q) * 9 + 3 * cc;
-                    val[v].x += n * F[0].x + nx * F[1].x + nz * F[2].x;
-                    val[v].y += n * F[0].y + nx * F[1].y + nz * F[2].y;
+                    const double2 *F = sF + ((size_t)v * nq + q) * 8;
+                    valr[v].x += n * F[fN].x + nx * F[fX].x + nz * F[fZ].x;
+                    valr[v].y += n * F[fN].y + nx * F[fX].y + nz * F[fZ].y;
                 }
             }
-#pragma unroll
-            for (int v = 0; v < 2; ++v) if (v < nvec) sP[(size_t)v * QG * nee + r] = val[v];
         }
+        // every thread is done with the fluxes: the partial sums take their place
+        __syncthreads();
+        if (grp < QG)
+#pragma unroll
+            for (int v = 0; v < 2; ++v) if (v < nvec) sP[((size_t)v * QG + grp) * nee + i] = val[v];
+        if (r < nee)
+#pragma unroll
+            for (int v = 0; v < 2; ++v) if (v < nvec) sP[(size_t)v * QG * nee + r] = valr[v];
     }
     __syncthreads();
     int i1 = 0, i2 = 0;
