@@ -43,6 +43,7 @@ EXPORTS = {
     "f25_apply_batch_dev": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_int, C.c_void_p,
                                       C.c_void_p]),
     "f25_apply_batch": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_int, C.c_void_p, C.c_void_p]),
+    "f25_set_apply_tiles": (C.c_int, [C.c_void_p, C.c_int]),
     "f25_gradient": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, dp]),
     "f25_gradient_dev": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p,
                                    C.c_void_p]),

@@ -167,6 +167,10 @@ class MeshInfo:
                                               C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())),
               "f25_apply_batch_dev")
 
+    def set_apply_tiles(self, nrt: int = 0) -> None:
+        """Row tiles per output component of the FP64 product (0: the order's default)."""
+        check(_lib.load().f25_set_apply_tiles(self.h, int(nrt)), "f25_set_apply_tiles")
+
     def apply_batch(self, omega, ky, x: np.ndarray, mode: int = _lib.F25_APPLY_F64) -> np.ndarray:
         """x: [npair, ndof] complex128 host array; returns S_b x_b per row."""
         om = np.ascontiguousarray(omega, np.float64)

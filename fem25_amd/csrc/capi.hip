@@ -107,6 +107,8 @@ struct f25_mesh_s {
     double *d_dblk = nullptr;   // blocks of a model perturbation (GN products)
     double *d_vcf = nullptr;    // d_blk in the coefficient layout of the FP64 product (k_apply_v)
     double *d_dvcf = nullptr;   // d_dblk in that layout
+    size_t vcf_cap = 0, dvcf_cap = 0;
+    int apply_nrt = 0;          // row tiles of k_apply_v (0: the order's default)
     double *d_vpad = nullptr;   // k_apply_v: vpad_cap zero double2, then vpad_cap double2 of store sink
     int vpad_cap = 0;
     int64_t *d_irptr = nullptr;
@@ -780,7 +782,7 @@ static int dispatch_apply(f25_mesh *m, const double *om, const double *kv, int n
 }
 
 // FP64 product (k_apply_v): one launch per element colour
-template <int NL>
+template <int NL, int NRT>
 static int launch_apply_v(f25_mesh *m, const double *om, const double *kv, int npair, const double2 *x,
                           double2 *y, const double *vcf) {
     const double2 *zpad = reinterpret_cast<const double2 *>(m->d_vpad);
@@ -791,7 +793,7 @@ static int launch_apply_v(f25_mesh *m, const double *om, const double *kv, int n
         if (!n) continue;
         const size_t nwg = (size_t)((n + 7) / 8) * 8 * ngrp;
         if (nwg > 0x7fffffff) FAIL(F25_ERR_ARG, "too many pairs for one launch");
-        hipLaunchKernelGGL(k_apply_v<NL>, dim3((unsigned)nwg), dim3(64 * 3 * VShape<NL>::NRT), 0, m->st,
+        hipLaunchKernelGGL((k_apply_v<NL, NRT>), dim3((unsigned)nwg), dim3(64 * 3 * NRT), 0, m->st,
                            m->d_order + m->cptr[c], n, vcf, m->d_lm, m->d_first, npair, om, kv, x, y, zpad,
                            sink);
     }
@@ -799,27 +801,60 @@ static int launch_apply_v(f25_mesh *m, const double *om, const double *kv, int n
     return 0;
 }
 
+// Row tiles of the FP64 product per order (index NL = p + 1): the candidates
+// compiled in, and the default, picked by fem25_amd/tune_apply.py on MI355X
+// (400x200 elements, 192 pairs).
+static bool apply_nrt_ok(int nl, int nrt) {
+    switch (nl) {
+        case 3: case 4: return nrt == 1 || nrt == 2;
+        case 5: return nrt >= 1 && nrt <= 3;
+        case 6: case 7: return nrt >= 2 && nrt <= 4;
+    }
+    return false;
+}
+static int apply_nrt_default(int nl) { return nl <= 5 ? 1 : 2; }
+static int apply_nrt(const f25_mesh *m) { return m->apply_nrt ? m->apply_nrt : apply_nrt_default(m->tab.nl); }
+
 // doubles of the k_apply_v coefficient layout for the whole mesh
 static size_t vcoef_size(const f25_mesh *m, int *nrt, int *rt, int *cw) {
-    switch (m->tab.nl) {
-#define F25_VS(N) case N: *nrt = VShape<N>::NRT; *rt = VShape<N>::RT; *cw = VShape<N>::CW; break;
-        F25_VS(3) F25_VS(4) F25_VS(5) F25_VS(6) F25_VS(7)
-#undef F25_VS
-        default: return 0;
-    }
+    const int nl = m->tab.nl, t = apply_nrt(m);
+    if (!apply_nrt_ok(nl, t)) return 0;
+    *nrt = t;
+    *rt = (m->tab.nen + t - 1) / t;
+    *cw = (5 * *rt + 31) / 32 * 32;
     return (size_t)m->d.nelem * 3 * *nrt * m->tab.nen * *cw;
 }
 
 // element blocks (NBLK per element) -> k_apply_v coefficient layout; the
-// buffer *vcf is allocated on first use
+// buffer *vcf is (re)allocated to the layout's size
 static int vcoef(f25_mesh *m, const double *blk, double **vcf) {
     int nrt = 0, rt = 0, cw = 0;
     const size_t n = vcoef_size(m, &nrt, &rt, &cw);
     if (!n) FAIL(F25_ERR_ARG, "unsupported order");
-    if (!*vcf) TRY(dalloc(vcf, n));
+    size_t *cap = (vcf == &m->d_vcf) ? &m->vcf_cap : &m->dvcf_cap;
+    if (!*vcf || *cap < n) {
+        dfree(*vcf);
+        *cap = 0;
+        TRY(dalloc(vcf, n));
+        *cap = n;
+    }
     const int nb = (int)std::min<size_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_vcoef, dim3(nb), dim3(256), 0, m->st, m->tab.nen, nrt, rt, cw, m->d.nelem, blk, *vcf);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int f25_set_apply_tiles(f25_mesh *m, int nrt) {
+    if (!m) FAIL(F25_ERR_ARG, "null handle");
+    if (nrt != 0 && !apply_nrt_ok(m->tab.nl, nrt)) FAIL(F25_ERR_ARG, "row tile count not compiled for this order");
+    HIPCHK(hipSetDevice(m->dev));
+    const int old = m->apply_nrt;
+    m->apply_nrt = nrt;
+    if (m->have_model && apply_nrt(m) != (old ? old : apply_nrt_default(m->tab.nl))) {
+        const int r = vcoef(m, m->d_blk, &m->d_vcf);
+        if (r) return r;
+        HIPCHK(hipStreamSynchronize(m->st));
+    }
     return 0;
 }
 
@@ -833,12 +868,11 @@ static int apply_f64(f25_mesh *m, const double *om, const double *kv, int npair,
         fill_async(m->d_vpad, 2 * (size_t)npair, 0.0, m->st);
         m->vpad_cap = npair;
     }
-    switch (m->tab.nl) {
-        case 3: return launch_apply_v<3>(m, om, kv, npair, x, y, vcf);
-        case 4: return launch_apply_v<4>(m, om, kv, npair, x, y, vcf);
-        case 5: return launch_apply_v<5>(m, om, kv, npair, x, y, vcf);
-        case 6: return launch_apply_v<6>(m, om, kv, npair, x, y, vcf);
-        case 7: return launch_apply_v<7>(m, om, kv, npair, x, y, vcf);
+    switch (m->tab.nl * 8 + apply_nrt(m)) {
+#define F25_AV(N, T) case N * 8 + T: return launch_apply_v<N, T>(m, om, kv, npair, x, y, vcf);
+        F25_AV(3, 1) F25_AV(3, 2) F25_AV(4, 1) F25_AV(4, 2) F25_AV(5, 1) F25_AV(5, 2) F25_AV(5, 3)
+        F25_AV(6, 2) F25_AV(6, 3) F25_AV(6, 4) F25_AV(7, 2) F25_AV(7, 3) F25_AV(7, 4)
+#undef F25_AV
     }
     FAIL(F25_ERR_ARG, "unsupported order");
 }

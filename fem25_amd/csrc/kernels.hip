@@ -603,14 +603,16 @@ __global__ __launch_bounds__((64 * ApplyLds<NL, BT, REFINE>::NW), (REFINE ? 2 : 
 // layout (k_vcoef), per element e, component c, row tile t, node b: CW
 // values, number u*RT + r = coefficient of sub-block kSub[c][u] between
 // output row t*RT + r and input node b (zero past the last row and past
-// 5 RT).
-template <int NL>
+// 5 RT).  The number of row tiles NRT is a tuning choice per order (more
+// tiles: fewer registers and more waves, but x read once per tile).
+template <int NL, int NRT_ = (NL * NL + 24) / 25>
 struct VShape {
     static constexpr int NEN = NL * NL;
-    static constexpr int NRT = (NEN + 24) / 25;                 // row tiles (at most 25 rows)
+    static constexpr int NRT = NRT_;                            // row tiles
     static constexpr int RT = (NEN + NRT - 1) / NRT;            // rows per tile
     static constexpr int CW = (5 * RT + 31) / 32 * 32;          // values per node
     static constexpr int NQ = CW / 32;                          // loads per node
+    static constexpr int MINW = RT <= 13 ? 4 : 3;               // waves per SIMD the registers allow
 };
 
 // Sub-blocks of output component c (indices into kSub[c]) that act on input
@@ -712,13 +714,13 @@ __device__ __forceinline__ void vstep_load(double2 *acc, double &g, const double
 // first touch), sink: npair values that absorb the stores of fixed DOFs, so
 // that every load and store of a tile is a uniform row base plus the lane's
 // pair offset and none is predicated per row.
-template <int NL, int C>
+template <int NL, int NRT, int C>
 __device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, const int *__restrict__ lme, int a0,
                                              unsigned long long fmask, int npair, int pair, bool live,
                                              double om, double kv, const double2 *__restrict__ x,
                                              double2 *__restrict__ y, const double2 *__restrict__ zpad,
                                              double2 *__restrict__ sink) {
-    typedef VShape<NL> V;
+    typedef VShape<NL, NRT> V;
     constexpr int NEN = V::NEN, RT = V::RT, CW = V::CW, NQ = V::NQ;
     const int l16 = (int)(threadIdx.x & 15);
     const double f3 = kv * kv, f4 = -om * om;
@@ -819,15 +821,15 @@ __device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, cons
 // pair groups per workgroup, 6 waves, fit only two workgroups per CU and
 // ran 0.69 instead of 0.63 ms per launch; 13-row tiles at four waves per
 // SIMD ran 0.72 ms.)
-template <int NL>
-__global__ __launch_bounds__((64 * 3 * VShape<NL>::NRT), 3) void k_apply_v(
+template <int NL, int NRT>
+__global__ __launch_bounds__((64 * 3 * NRT), (VShape<NL, NRT>::MINW)) void k_apply_v(
         const int *__restrict__ order, int nel, const double *__restrict__ vcf, const int *__restrict__ lm,
         const unsigned long long *__restrict__ first, int npair, const double *__restrict__ omega,
         const double *__restrict__ ky, const double2 *__restrict__ x, double2 *__restrict__ y,
         const double2 *__restrict__ zpad, double2 *__restrict__ sink) {
-    typedef VShape<NL> V;
+    typedef VShape<NL, NRT> V;
     static_assert(V::NQ <= 5, "coefficient loads per node");
-    constexpr int NEN = V::NEN, NRT = V::NRT;
+    constexpr int NEN = V::NEN;
     constexpr size_t TS = (size_t)NEN * V::CW;   // values per (element, component, row tile)
     const int ngrp = (npair + 63) >> 6;
     const int i = blockIdx.x;
@@ -844,9 +846,9 @@ __global__ __launch_bounds__((64 * 3 * VShape<NL>::NRT), 3) void k_apply_v(
     const double om = omega[live ? pair : 0], kv = ky[live ? pair : 0];
     const double *cf = vcf + ((size_t)e * 3 * NRT + ct) * TS;
     const int a0 = t * V::RT;
-    if (c == 0) apply_v_tile<NL, 0>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
-    else if (c == 1) apply_v_tile<NL, 1>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
-    else apply_v_tile<NL, 2>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
+    if (c == 0) apply_v_tile<NL, NRT, 0>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
+    else if (c == 1) apply_v_tile<NL, NRT, 1>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
+    else apply_v_tile<NL, NRT, 2>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
 }
 
 // ---------------------------------------------------------------------------

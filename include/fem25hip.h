@@ -136,6 +136,12 @@ int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, const doubl
  * per pair, NDOF contiguous, i.e. Fortran X(NDOF,NPAIR)). */
 int f25_apply_batch(f25_mesh *m, int npair, const double *omega, const double *ky, int mode,
                     const void *x, void *y);
+/* Tuning hook of the FP64 product (no counterpart in the reference, which
+ * has no batched product): number of row tiles per output component, 0 =
+ * the order's default.  Compiled candidates: p2, p3: 1, 2; p4: 1..3; p5,
+ * p6: 2..4.  Rewrites the product's coefficient layout when the model is
+ * set.  fem25_amd/tune_apply.py times the candidates. */
+int f25_set_apply_tiles(f25_mesh *m, int nrt);
 
 /* ---- gradient ------------------------------------------------------------ */
 

@@ -126,6 +126,40 @@ def test_apply_batch_fp64_pair_groups():
         assert relerr(y[b], S @ x[b]) < FP64_TOL, b
 
 
+@pytest.mark.parametrize("name", ["p5", "p6"])
+def test_apply_batch_fp64_row_tiles_pair_groups(name):
+    """p5, p6: two 25-row tiles per component (p6's second tile has a padding
+    row) and two groups of 64 pairs, the second partly empty."""
+    mi = _mi(name)
+    om, ky, f, py = pairs(7, 10)
+    g = oracle_graph(name)
+    x = cf.seeded_complex(g.ndof, batch=om.size, seed=23)
+    y = mi.apply_batch(om, ky, x)
+    for b in (0, 63, 64, 69):
+        S = oracle_S(name, f[b], py[b])
+        assert relerr(y[b], S @ x[b]) < FP64_TOL, b
+
+
+@pytest.mark.parametrize("name,tiles", [("tiny_layered", (1, 2, 3)), ("p5", (2, 3, 4)), ("p6", (2, 3, 4))])
+def test_apply_batch_fp64_tile_candidates(name, tiles):
+    """Every compiled row-tile count of the FP64 product gives the oracle's
+    product; an unknown count is refused."""
+    from fem25_amd._lib import Fem25Error
+    mi = _mi(name)
+    om, ky, f, py = pairs(2, 3)
+    g = oracle_graph(name)
+    x = cf.seeded_complex(g.ndof, batch=om.size, seed=25)
+    refs = [oracle_S(name, f[b], py[b]) @ x[b] for b in range(om.size)]
+    for t in tiles:
+        mi.set_apply_tiles(t)
+        y = mi.apply_batch(om, ky, x)
+        for b in range(om.size):
+            assert relerr(y[b], refs[b]) < FP64_TOL, (t, b)
+    with pytest.raises(Fem25Error):
+        mi.set_apply_tiles(7)
+    mi.set_apply_tiles(0)
+
+
 def test_apply_batch_config1_modes(torch_dev):
     torch = torch_dev
     from fem25_amd import _lib
