@@ -26,6 +26,7 @@ class MeshDesc(C.Structure):
 
 F25_C64, F25_C128 = 0, 1
 F25_APPLY_F64, F25_APPLY_F32, F25_APPLY_F32_REFINED = 0, 1, 2
+F25_APPLY_GRAPH = 16   # or'ed with F25_APPLY_F64: replay a captured HIP graph
 
 EXPORTS = {
     "f25_mesh_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),

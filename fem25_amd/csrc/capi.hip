@@ -108,6 +108,13 @@ struct f25_mesh_s {
     double *d_vcf = nullptr;    // d_blk in the coefficient layout of the FP64 product (k_apply_v)
     double *d_dvcf = nullptr;   // d_dblk in that layout
     size_t vcf_cap = 0, dvcf_cap = 0;
+    // captured FP64 product (F25_APPLY_GRAPH) and the arguments it was captured for
+    struct {
+        hipGraphExec_t exec = nullptr;
+        const void *x = nullptr, *vcf = nullptr, *om = nullptr, *kv = nullptr, *pad = nullptr;
+        void *y = nullptr;
+        int npair = 0;
+    } ag;
     int apply_nrt = 0;          // row tiles of k_apply_v (0: the order's default)
     double *d_vpad = nullptr;   // k_apply_v: vpad_cap zero double2, then vpad_cap double2 of store sink
     int vpad_cap = 0;
@@ -381,6 +388,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_ieng); dfree(m->d_lm); dfree(m->d_first); dfree(m->d_order); dfree(m->d_pml_order); dfree(m->d_err);
     dfree(m->d_xl); dfree(m->d_zl); dfree(m->d_xd); dfree(m->d_zd); dfree(m->d_shl); dfree(m->d_shlT); dfree(m->d_wq);
     dfree(m->d_xq); dfree(m->d_cdom); dfree(m->d_cnp); dfree(m->d_rho); dfree(m->d_lam); dfree(m->d_mu);
+    if (m->ag.exec) (void)hipGraphExecDestroy(m->ag.exec);
     dfree(m->d_vcf); dfree(m->d_dvcf); dfree(m->d_vpad);
     dfree(m->d_blk); dfree(m->d_blk32); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_colmax); dfree(m->d_dblk); dfree(m->d_irptr);
     dfree(m->d_jcptr); dfree(m->d_comp); dfree(m->d_v0); dfree(m->d_v1); dfree(m->d_v2);
@@ -848,6 +856,11 @@ extern "C" int f25_set_apply_tiles(f25_mesh *m, int nrt) {
     if (!m) FAIL(F25_ERR_ARG, "null handle");
     if (nrt != 0 && !apply_nrt_ok(m->tab.nl, nrt)) FAIL(F25_ERR_ARG, "row tile count not compiled for this order");
     HIPCHK(hipSetDevice(m->dev));
+    if (m->ag.exec) {   // a captured product holds the previous kernel
+        HIPCHK(hipStreamSynchronize(m->st));
+        (void)hipGraphExecDestroy(m->ag.exec);
+        m->ag.exec = nullptr;
+    }
     const int old = m->apply_nrt;
     m->apply_nrt = nrt;
     if (m->have_model && apply_nrt(m) != (old ? old : apply_nrt_default(m->tab.nl))) {
@@ -858,9 +871,8 @@ extern "C" int f25_set_apply_tiles(f25_mesh *m, int nrt) {
     return 0;
 }
 
-static int apply_f64(f25_mesh *m, const double *om, const double *kv, int npair, const double2 *x, double2 *y,
-                     const double *vcf) {
-    // zero rows and store sink of npair values each
+// zero rows and store sink of the FP64 product, npair values each
+static int ensure_vpad(f25_mesh *m, int npair) {
     if (npair > m->vpad_cap) {
         dfree(m->d_vpad);
         m->vpad_cap = 0;
@@ -868,6 +880,12 @@ static int apply_f64(f25_mesh *m, const double *om, const double *kv, int npair,
         fill_async(m->d_vpad, 2 * (size_t)npair, 0.0, m->st);
         m->vpad_cap = npair;
     }
+    return 0;
+}
+
+// the launches of the FP64 product (ensure_vpad first)
+static int launch_f64(f25_mesh *m, const double *om, const double *kv, int npair, const double2 *x, double2 *y,
+                      const double *vcf) {
     switch (m->tab.nl * 8 + apply_nrt(m)) {
 #define F25_AV(N, T) case N * 8 + T: return launch_apply_v<N, T>(m, om, kv, npair, x, y, vcf);
         F25_AV(3, 1) F25_AV(3, 2) F25_AV(4, 1) F25_AV(4, 2) F25_AV(5, 1) F25_AV(5, 2) F25_AV(5, 3)
@@ -877,13 +895,57 @@ static int apply_f64(f25_mesh *m, const double *om, const double *kv, int npair,
     FAIL(F25_ERR_ARG, "unsupported order");
 }
 
+static int apply_f64(f25_mesh *m, const double *om, const double *kv, int npair, const double2 *x, double2 *y,
+                     const double *vcf) {
+    TRY(ensure_vpad(m, npair));
+    return launch_f64(m, om, kv, npair, x, y, vcf);
+}
+
+// F25_APPLY_GRAPH: capture the launches once per argument set, then replay
+static int apply_f64_graph(f25_mesh *m, const double *om, const double *kv, int npair, const double2 *x,
+                           double2 *y, const double *vcf) {
+    TRY(ensure_vpad(m, npair));
+    auto &G = m->ag;
+    if (!G.exec || G.x != x || G.y != y || G.npair != npair || G.vcf != vcf || G.om != om || G.kv != kv ||
+        G.pad != m->d_vpad) {
+        if (G.exec) {
+            HIPCHK(hipStreamSynchronize(m->st));
+            (void)hipGraphExecDestroy(G.exec);
+            G.exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
+        const int r = launch_f64(m, om, kv, npair, x, y, vcf);
+        hipError_t e = hipStreamEndCapture(m->st, &g);
+        if (r) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
+        e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            G.exec = nullptr;
+            FAIL(F25_ERR_HIP, hipGetErrorString(e));
+        }
+        G.x = x; G.y = y; G.npair = npair; G.vcf = vcf; G.om = om; G.kv = kv; G.pad = m->d_vpad;
+    }
+    HIPCHK(hipGraphLaunch(G.exec, m->st));
+    return 0;
+}
+
 static int apply_core(f25_mesh *m, int npair, const double *d_om, const double *d_ky, int mode,
                       const void *x, void *y, const double *vcf) {
     // every DOF is written by the first element touching it: no zero fill
+    const bool graph = (mode & F25_APPLY_GRAPH) != 0;
+    mode &= ~F25_APPLY_GRAPH;
     if (mode != F25_APPLY_F64 && mode != F25_APPLY_F32 && mode != F25_APPLY_F32_REFINED)
         FAIL(F25_ERR_ARG, "bad apply mode");
+    if (graph && mode != F25_APPLY_F64) FAIL(F25_ERR_ARG, "graph replay is for the FP64 product");
     HIPCHK(hipEventRecord(m->ev0, m->st));
-    if (mode == F25_APPLY_F64)
+    if (mode == F25_APPLY_F64 && graph)
+        TRY(apply_f64_graph(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y, vcf));
+    else if (mode == F25_APPLY_F64)
         TRY(apply_f64(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y, vcf));
     else if (mode == F25_APPLY_F32)
         TRY((dispatch_apply<float, float2, false>(m, d_om, d_ky, npair, (const float2 *)x, (float2 *)y,

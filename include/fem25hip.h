@@ -36,6 +36,11 @@ enum {
 enum { F25_C64 = 0, F25_C128 = 1 };
 /* Arithmetic of the batched element-block product. */
 enum { F25_APPLY_F64 = 0, F25_APPLY_F32 = 1, F25_APPLY_F32_REFINED = 2 };
+/* or'ed with F25_APPLY_F64 in f25_apply_batch_dev: the product's launches
+ * (one per element colour) are captured once into a HIP graph for the given
+ * x, y and npair and the graph is replayed by later calls with the same
+ * arguments (omega and ky may change: they are staged before the replay). */
+enum { F25_APPLY_GRAPH = 16 };
 
 /*
  * Anchor mesh, the fields of MESH_INFO (fwd_struc.h:10-141) the path reads.

@@ -160,6 +160,31 @@ def test_apply_batch_fp64_tile_candidates(name, tiles):
     mi.set_apply_tiles(0)
 
 
+def test_apply_batch_graph_replays(torch_dev):
+    """F25_APPLY_GRAPH: captured once, replayed with new (omega, ky) values
+    and with eager work between the replays; every replay equals the eager
+    product bit for bit, and new buffers are captured anew."""
+    torch = torch_dev
+    from fem25_amd import _lib
+    mi = _mi("tiny_layered")
+    g = oracle_graph("tiny_layered")
+    x = torch.from_numpy(np.ascontiguousarray(cf.seeded_complex(g.ndof, batch=70, seed=27).T)).cuda()
+    yg = torch.zeros_like(x)
+    ye = torch.zeros_like(x)
+    for rep in range(4):
+        om, ky, _, _ = pairs(7, 10, fmin=0.02 + 0.01 * rep)
+        mi.apply_batch_dev(om, ky, x, yg, _lib.F25_APPLY_F64 | _lib.F25_APPLY_GRAPH)
+        mi.apply_batch_dev(om, ky, x, ye, _lib.F25_APPLY_F64)
+        torch.cuda.synchronize()
+        assert torch.equal(yg, ye), rep
+        x.mul_(1.5)                     # eager work between the replays
+    y2 = torch.zeros_like(x)            # other output buffer: a new capture
+    mi.apply_batch_dev(om, ky, x, y2, _lib.F25_APPLY_F64 | _lib.F25_APPLY_GRAPH)
+    mi.apply_batch_dev(om, ky, x, ye, _lib.F25_APPLY_F64)
+    torch.cuda.synchronize()
+    assert torch.equal(y2, ye)
+
+
 def test_apply_batch_config1_modes(torch_dev):
     torch = torch_dev
     from fem25_amd import _lib
