@@ -242,7 +242,12 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     lam = torch.randn(npair, ndof, dtype=torch.complex128, device="cuda")
     gg = torch.zeros(mesh.nnpg, 3, dtype=torch.float64, device="cuda")
     k_ms, wall = timed(lambda: mi.gradient_dev(omega, ky, u, lam, gg), L, b"grad", 2)
-    gfl = (72.0 * nen + 150.0) * nq * mesh.nelem * npair
+    # flops per (element, triple): sum factorisation for p <= 5 (two 1-D
+    # passes per field and component: 4 (12 N1^2 Q1 + 18 N1 Q1^2)), the
+    # direct contraction (72 NEN Q) for p6, plus ~150 per quadrature point
+    n1, q1 = mesh.nord + 1, 2 * mesh.nord + 5
+    per = 4.0 * (12 * n1 * n1 * q1 + 18 * n1 * q1 * q1) if q1 <= 16 else 72.0 * nen * nq
+    gfl = (per + 150.0 * nq) * mesh.nelem * npair
     kern["gradient"] = {"kernel_ms": round(k_ms, 3), "gradients_per_s": round(npair / (wall * 1e-3), 1),
                         "roofline": roof("fp64", gfl / (k_ms * 1e-3), "TFLOP/s")}
     out["gradients_per_s"] = kern["gradient"]["gradients_per_s"]

@@ -96,6 +96,7 @@ struct f25_mesh_s {
     unsigned long long *d_first = nullptr;   // first-touch node masks per element
     double *d_xl = nullptr, *d_zl = nullptr, *d_xd = nullptr, *d_zd = nullptr;
     double *d_shl = nullptr, *d_wq = nullptr, *d_xq = nullptr;
+    double *d_t1d = nullptr;    // 1-D tables: val1 (nl*nq1) then der1
     double *d_shlT = nullptr;   // shape tables as [a][c][q] (q fastest)
     char *d_cdom = nullptr, *d_cnp = nullptr;
     double *d_rho = nullptr, *d_lam = nullptr, *d_mu = nullptr;
@@ -187,6 +188,11 @@ static int upload_mesh(f25_mesh *m) {
     }
     TRY(upload(&m->d_wq, m->tab.wq.data(), m->tab.wq.size(), s));
     TRY(upload(&m->d_xq, m->tab.xq.data(), m->tab.xq.size(), s));
+    {
+        std::vector<double> t1(m->tab.val1);
+        t1.insert(t1.end(), m->tab.der1.begin(), m->tab.der1.end());
+        TRY(upload(&m->d_t1d, t1.data(), t1.size(), s));
+    }
     std::vector<int> lm0(m->g.lm.size());
     for (size_t i = 0; i < lm0.size(); ++i) lm0[i] = m->g.lm[i] - 1;   // 0-based, -1 fixed
     TRY(upload(&m->d_lm, lm0.data(), lm0.size(), s));
@@ -386,7 +392,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     (void)hipSetDevice(m->dev);
     if (m->st) (void)hipStreamSynchronize(m->st);
     dfree(m->d_ieng); dfree(m->d_lm); dfree(m->d_first); dfree(m->d_order); dfree(m->d_pml_order); dfree(m->d_err);
-    dfree(m->d_xl); dfree(m->d_zl); dfree(m->d_xd); dfree(m->d_zd); dfree(m->d_shl); dfree(m->d_shlT); dfree(m->d_wq);
+    dfree(m->d_xl); dfree(m->d_zl); dfree(m->d_xd); dfree(m->d_zd); dfree(m->d_shl); dfree(m->d_shlT); dfree(m->d_wq); dfree(m->d_t1d);
     dfree(m->d_xq); dfree(m->d_cdom); dfree(m->d_cnp); dfree(m->d_rho); dfree(m->d_lam); dfree(m->d_mu);
     if (m->ag.exec) (void)hipGraphExecDestroy(m->ag.exec);
     dfree(m->d_vcf); dfree(m->d_dvcf); dfree(m->d_vpad);
@@ -1029,6 +1035,9 @@ extern "C" int f25_apply_batch(f25_mesh *m, int npair, const double *omega, cons
 }
 
 // ---------------------------------------------------------------------------
+#ifndef F25_GRAD_T
+#define F25_GRAD_T 2
+#endif
 static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d_ky, const double2 *u,
                      const double2 *l, double *g, int chain) {
     const int nen = m->tab.nen, nq = m->tab.nq;
@@ -1040,9 +1049,34 @@ static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d
     constexpr int H = 2;   // pairs per thread per pass
     size_t lds = (size_t)2 * H * tg * 3 * nen * sizeof(double2) + (size_t)ntp * 12 * sizeof(double);
     HIPCHK(hipEventRecord(m->ev0, m->st));
-    hipLaunchKernelGGL(k_grad<H>, dim3(m->d.nelem), dim3(ntp), lds, m->st, nen, m->tab.nq1, m->d.nelem, tg,
-                       m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip,
-                       (int64_t)m->g.ndof, d_om, d_ky, u, l, m->d_part, m->d_err);
+    // orders with at most 16 quadrature points per direction (p <= 5) run
+    // the sum-factorised kernel
+    typedef void (*GradSf)(const double *, const double *, const double *, const int *, const double *,
+                           const double *, const char *, const int *, int, int64_t, const double *,
+                           const double *, const double2 *, const double2 *, double *, int *);
+    constexpr int TS = F25_GRAD_T;   // trips per pass
+    GradSf ksf = nullptr;
+    switch (m->tab.nl) {
+        case 3: ksf = k_grad_sf<3, TS>; break;
+        case 4: ksf = k_grad_sf<4, TS>; break;
+        case 5: ksf = k_grad_sf<5, TS>; break;
+        case 6: ksf = k_grad_sf<6, TS>; break;
+        default: break;
+    }
+    if (ksf) {
+        const int nl = m->tab.nl, q1 = m->tab.nq1;
+        const size_t lsf = (size_t)TS * 2 * 3 * nen * sizeof(double2) +
+                           (size_t)TS * 2 * 3 * q1 * nl * 2 * sizeof(double2) + (size_t)q1 * q1 * 8 * sizeof(double);
+        if (lsf > 64 * 1024)
+            HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
+        hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(256), lsf, m->st, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,
+                           m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip, (int64_t)m->g.ndof, d_om, d_ky, u, l,
+                           m->d_part, m->d_err);
+    } else {
+        hipLaunchKernelGGL(k_grad<H>, dim3(m->d.nelem), dim3(ntp), lds, m->st, nen, m->tab.nq1, m->d.nelem, tg,
+                           m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip,
+                           (int64_t)m->g.ndof, d_om, d_ky, u, l, m->d_part, m->d_err);
+    }
     HIPCHK(hipEventRecord(m->ev1, m->st));
     hipLaunchKernelGGL(k_grad_gather, dim3((m->d.nnpg + 255) / 256), dim3(256), 0, m->st, m->d.nnpg,
                        m->ncon_anchor, m->d_mconn, m->d_ieng, m->d_part, m->d_rho, m->d_lam, m->d_mu,

@@ -17,6 +17,9 @@ struct Tables {
     std::vector<double> xq, wq;  // nq1
     // shl[(q*nen + a)*3 + c], q = ix + nq1*iz, c: 0 dN/dxi, 1 dN/deta, 2 N
     std::vector<double> shl;
+    // the 1-D cardinal functions and derivatives at the quadrature points,
+    // val1[k*nq1 + q], der1[k*nq1 + q] (shl is their tensor product)
+    std::vector<double> val1, der1;
 };
 void make_tables(int nord, Tables &t);
 void gll_nodes(int n, double *x, double *w);

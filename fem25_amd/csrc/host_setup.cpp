@@ -163,6 +163,8 @@ void make_tables(int nord, Tables &t) {
             der[k * t.nq1 + q] = d;
         }
     }
+    t.val1 = val;
+    t.der1 = der;
     t.shl.assign((size_t)t.nq * t.nen * 3, 0.0);
     for (int iz = 0; iz < t.nq1; ++iz)
         for (int ix = 0; ix < t.nq1; ++ix) {

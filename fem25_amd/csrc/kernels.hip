@@ -1342,6 +1342,161 @@ __global__ __launch_bounds__(512) void k_grad(int nen, int nq1, int nelem, int t
     if (threadIdx.x < 12) part[(size_t)e * 12 + threadIdx.x] = sred[threadIdx.x];
 }
 
+// Adjoint gradient densities by sum factorisation, orders with Q1 <= 16
+// (p <= 5).  The shape functions are tensor products N_a(q) = v_i(xi)
+// v_j(eta), a = j N1 + i (dshl.f:149-175), so per trip, field w (u or l)
+// and component c
+//   T1[eta][i] = sum_j v_j(eta) w_(j,i,c),   T2[eta][i] = sum_j v'_j(eta) w_(j,i,c)
+//   U = sum_i v_i(xi) T1,  U_xi = sum_i v'_i(xi) T1,  U_eta = sum_i v_i(xi) T2
+// : N1 Q1 (2 N1 + 3 Q1) complex MACs instead of 3 N1^2 Q1^2 (p4: 4.6x
+// fewer).  The second stage runs on rows of 16 lanes, one row per eta
+// point and one lane per xi point; lane i of a row holds T1, T2 of node i
+// and v_fmac_f64 with row_newbcast:i hands them to the whole row.  The
+// densities, the projection on the anchor hat functions and the reduction
+// are those of k_grad; the 1-D tables are not flushed to zero as DSHL's
+// products are (a difference below 2.2e-16 per shape value).
+template <int NL, int I, int... J>
+__device__ __forceinline__ void sf_row_sums(double2 &U, double2 &UX, double2 &UE, const double2 &T1,
+                                            const double2 &T2, const double *vx, const double *dx,
+                                            std::integer_sequence<int, J...>) {
+    ((fmac_bcast<J>(U.x, T1.x, vx[J]), fmac_bcast<J>(U.y, T1.y, vx[J]), fmac_bcast<J>(UX.x, T1.x, dx[J]),
+      fmac_bcast<J>(UX.y, T1.y, dx[J]), fmac_bcast<J>(UE.x, T2.x, vx[J]), fmac_bcast<J>(UE.y, T2.y, vx[J])),
+     ...);
+}
+
+template <int NL, int T>
+__global__ __launch_bounds__(256) void k_grad_sf(const double *__restrict__ t1d, const double *__restrict__ wq,
+                                                 const double *__restrict__ xq, const int *__restrict__ ieng,
+                                                 const double *__restrict__ xl, const double *__restrict__ zl,
+                                                 const char *__restrict__ cdom, const int *__restrict__ lm,
+                                                 int ntrip, int64_t ndof, const double *__restrict__ omega,
+                                                 const double *__restrict__ ky, const double2 *__restrict__ u,
+                                                 const double2 *__restrict__ l, double *__restrict__ part,
+                                                 int *__restrict__ err) {
+    constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen, NT = 256;
+    static_assert(Q1 <= 16, "one row of 16 lanes per eta point");
+    extern __shared__ double sm[];
+    double2 *sw = reinterpret_cast<double2 *>(sm);                     // [T][2][nee]
+    double2 *sT = sw + T * 2 * nee;                                    // [T][2][3][Q1][N1][2]
+    double *sG = reinterpret_cast<double *>(sT + T * 2 * 3 * Q1 * N1 * 2);   // [Q1*Q1][8]
+    __shared__ double red[NT / 64][12];
+    const int e = blockIdx.x, tid = threadIdx.x;
+    if (cdom[e] == 'A') {   // PML elements are outside the non-PML gradient path
+        if (tid < 12) part[(size_t)e * 12 + tid] = 0.0;
+        return;
+    }
+    const int *ie = ieng + 4 * e;
+    const int *L = lm + (size_t)e * nee;
+    const double *v1 = t1d, *d1 = t1d + N1 * Q1;   // v1[k * Q1 + q]
+    for (int q = tid; q < Q1 * Q1; q += NT) {
+        const int ix = q % Q1, iz = q / Q1;
+        Geo g;
+        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
+        double *G = sG + 8 * q;
+        G[0] = g.dzdeta / g.det; G[1] = -g.dzdxi / g.det; G[2] = -g.dxdeta / g.det; G[3] = g.dxdxi / g.det;
+        const double wt = wq[ix] * wq[iz] * g.det;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) G[4 + k] = wt * g.sf[k];
+    }
+    // second stage: row = eta point, lane = xi point (clamped when idle: an
+    // idle lane computes a copy and contributes nothing)
+    const int row = tid >> 4, x16 = tid & 15;
+    const bool act = row < Q1 && x16 < Q1;
+    const int qe = min(row, Q1 - 1), qx = min(x16, Q1 - 1), ni = min(x16, N1 - 1);
+    double vx[N1], dx[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) { vx[i] = v1[i * Q1 + qx]; dx[i] = d1[i * Q1 + qx]; }
+    double Dl = 0.0, Dm = 0.0, Dr = 0.0;
+    for (int t0 = 0; t0 < ntrip; t0 += T) {
+        __syncthreads();   // the previous pass is done with sw, sT (and sG is written)
+        for (int i = tid; i < T * 2 * nee; i += NT) {
+            const int tt = i / (2 * nee), f = (i / nee) % 2, j = i % nee, t = t0 + tt;
+            const int d = L[j];
+            const double2 *w = f ? l : u;
+            sw[i] = (d >= 0 && t < ntrip) ? w[(size_t)t * ndof + d] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int it = tid; it < T * 2 * 3 * Q1 * N1; it += NT) {
+            const int i = it % N1, qy = (it / N1) % Q1, fc = it / (N1 * Q1);   // fc = (tt*2 + f)*3 + c
+            const int c = fc % 3;
+            const double2 *w = sw + (fc / 3) * nee;
+            double2 t1 = make_double2(0.0, 0.0), t2 = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int j = 0; j < N1; ++j) {
+                const double2 a = w[(j * N1 + i) * 3 + c];
+                const double vv = v1[j * Q1 + qy], dd = d1[j * Q1 + qy];
+                t1.x += vv * a.x; t1.y += vv * a.y;
+                t2.x += dd * a.x; t2.y += dd * a.y;
+            }
+            double2 *o = sT + (size_t)((fc * Q1 + qy) * N1 + i) * 2;
+            o[0] = t1;
+            o[1] = t2;
+        }
+        __syncthreads();
+        const double *G = sG + 8 * (qx + Q1 * qe);
+        const double rx = G[0], sx = G[1], rz = G[2], sz = G[3];
+        for (int tt = 0; tt < T; ++tt) {
+            const int t = t0 + tt;
+            if (t >= ntrip) break;
+            const double k = ky[t], w = omega[t];
+            double2 GU[3][3], GV[3][3], Uv[3], Vv[3];
+#pragma unroll
+            for (int f = 0; f < 2; ++f)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double2 *o = sT + (size_t)((((tt * 2 + f) * 3 + c) * Q1 + qe) * N1 + ni) * 2;
+                    const double2 T1 = o[0], T2 = o[1];
+                    double2 U = make_double2(0.0, 0.0), UXI = U, UET = U;
+                    sf_row_sums<NL, 0>(U, UXI, UET, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+                    const double2 UX = make_double2(rx * UXI.x + sx * UET.x, rx * UXI.y + sx * UET.y);
+                    const double2 UZ = make_double2(rz * UXI.x + sz * UET.x, rz * UXI.y + sz * UET.y);
+                    // G[c][j]: d_j of component c, j = x, y, z; d/dy = -i ky
+                    const double2 UY = make_double2(k * U.y, -k * U.x);
+                    if (f == 0) {
+                        GU[c][0] = UX; GU[c][1] = UY; GU[c][2] = UZ; Uv[c] = U;
+                    } else {
+                        GV[c][0] = UX; GV[c][1] = UY; GV[c][2] = UZ; Vv[c] = U;
+                    }
+                }
+            const double2 du = make_double2(GU[0][0].x + GU[1][1].x + GU[2][2].x, GU[0][0].y + GU[1][1].y + GU[2][2].y);
+            const double2 dv = make_double2(GV[0][0].x + GV[1][1].x + GV[2][2].x, GV[0][0].y + GV[1][1].y + GV[2][2].y);
+            Dl += dv.x * du.x + dv.y * du.y;
+            double sm2 = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double2 s2 = make_double2(GU[i][j].x + GU[j][i].x, GU[i][j].y + GU[j][i].y);
+                    sm2 += GV[i][j].x * s2.x + GV[i][j].y * s2.y;
+                }
+            Dm += sm2;
+            Dr += -w * w * (Vv[0].x * Uv[0].x + Vv[0].y * Uv[0].y + Vv[1].x * Uv[1].x + Vv[1].y * Uv[1].y +
+                            Vv[2].x * Uv[2].x + Vv[2].y * Uv[2].y);
+        }
+    }
+    // projection on the anchors and block reduction of the 12 partial sums
+    double g12[12];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const double fw = act ? sG[8 * (qx + Q1 * qe) + 4 + kk] : 0.0;
+        g12[kk * 3 + 0] = fw * Dl;
+        g12[kk * 3 + 1] = fw * Dm;
+        g12[kk * 3 + 2] = fw * Dr;
+    }
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+        for (int o = 32; o > 0; o >>= 1) g12[i] += __shfl_xor(g12[i], o);
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < 12; ++i) red[tid >> 6][i] = g12[i];
+    __syncthreads();
+    if (tid < 12) {
+        double s = 0.0;
+        for (int wv = 0; wv < NT / 64; ++wv) s += red[wv][tid];
+        part[(size_t)e * 12 + tid] = s;
+    }
+}
+
 // per-anchor sum of element partials and chain rule to (Vp, Vs, rho)
 __global__ void k_grad_gather(int nnpg, int ncon, const int *__restrict__ mconn,
                               const int *__restrict__ ieng, const double *__restrict__ part,

@@ -224,12 +224,15 @@ def test_apply_quadratic_in_ky():
     assert np.linalg.norm(comb) < 1e-11 * np.linalg.norm(y[0])
 
 
-@pytest.mark.parametrize("name", ["tiny_layered", "pml"])
-def test_gradient_exact(name):
+@pytest.mark.parametrize("name,nf,nk", [("tiny_layered", 1, 2), ("pml", 1, 2), ("tiny_layered", 2, 3),
+                                         ("p5", 2, 3), ("p6", 1, 2)])
+def test_gradient_exact(name, nf, nk):
+    """p4, p5: the sum-factorised kernel (with 2 and 6 triples: a partial
+    and a full pass of 4); p6: the direct kernel."""
     mi = _mi(name)
     m, mod, pml = mesh_case(name)
     g = oracle_graph(name)
-    om, ky, f, py = pairs(1, 2)
+    om, ky, f, py = pairs(nf, nk)
     u = cf.seeded_complex(g.ndof, batch=om.size, seed=21)
     lam = cf.seeded_complex(g.ndof, batch=om.size, seed=22)
     gd = mi.gradient(om, ky, u, lam)
