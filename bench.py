@@ -268,8 +268,12 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     t0 = time.perf_counter()
     fm, _ = cfmat_host(mi, inv, float(freq[0]), float(py[0]), wave)
     k_ms = L.f25_last_kernel_ms(b"cfmat")
-    # one element's DKM_VPVS product per column: (72 NEN + 40) flops per quadrature point
-    cfl = (72.0 * nen + 40.0) * nq * inv.nnpgl
+    # one element's DKM_VPVS product per column; per (item, vector): sum
+    # factorised for p <= 5, 4 (15 N1^2 Q1 + 18 N1 Q1^2) + 84 Q, else the
+    # direct (72 NEN + 40) Q
+    n1, q1 = mesh.nord + 1, 2 * mesh.nord + 5
+    per_item = (4.0 * (15 * n1 * n1 * q1 + 18 * n1 * q1 * q1) + 84.0 * nq) if q1 <= 16 else (72.0 * nen + 40.0) * nq
+    cfl = per_item * inv.nnpgl
     kern["cfmat_dist"] = {"kernel_ms": round(k_ms, 3), "columns_per_s": round(inv.nnpgl / (k_ms * 1e-3), 1),
                           "roofline": roof("fp64", cfl / (k_ms * 1e-3), "TFLOP/s")}
     # PCHADJ's J = G^T F: 60 receivers x 3 components
@@ -302,12 +306,11 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     # QPMFREE's second-order element products: CADJ_NU_ALPHA (every element
     # holding local columns, two vectors, + one column per anchor) and
     # MET225_MOD (one column per anchor, two vectors); FP64 VALU, per
-    # (item, vector) (72 NEN + 40) flops per quadrature point
+    # flops per (item, vector) as for CFMAT_DIST
     cv = lambda s: (np.random.default_rng(s).standard_normal(ndof)   # noqa: E731
                     + 1j * np.random.default_rng(s + 1).standard_normal(ndof)).astype(np.complex64)
     lamv, nuv, alv = cv(11), cv(13), cv(15)
     vq = rng.standard_normal(inv.na35).astype(np.float32)
-    per_item = (72.0 * nen + 40.0) * nq
     mi.qp_nu_alpha(float(freq[0]), float(py[0]), vq, lamv, wave)
     k_ms = L.f25_last_kernel_ms(b"qp")
     qfl = per_item * (2 * mesh.nelem + inv.nnpgl)

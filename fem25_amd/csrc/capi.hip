@@ -1470,8 +1470,17 @@ static int ensure_dab_part(f25_mesh *m) {
 // LDS of k_dab for nvec vectors: the element's vectors (room for two), the
 // geometry per quadrature point, and the fluxes (8 complex per vector and
 // point) whose space the projection's partial sums reuse
+// (orders with at most 16 quadrature points per direction run k_dab_sf,
+// whose LDS does not depend on nvec: vectors, geometry, T1/T2, the
+// projection fields A, B, C and their first-pass sums)
+static bool dab_sf(const f25_mesh *m) { return m->tab.nq1 <= 16; }
 static size_t dab_lds(const f25_mesh *m, int nvec = 2) {
     const int nee = 3 * m->tab.nen, nq = m->tab.nq, qg = std::max(1, 256 / nee);
+    if (dab_sf(m)) {
+        const int q1 = m->tab.nq1, n1 = m->tab.nl;
+        return (size_t)2 * nee * sizeof(double2) + 5 * (size_t)nq * sizeof(double) +
+               ((size_t)3 * q1 * n1 * 2 + 9 * (size_t)nq + 3 * (size_t)q1 * n1 * 3) * sizeof(double2);
+    }
     return (size_t)2 * nee * sizeof(double2) + 4 * (size_t)nq * sizeof(double) +
            (size_t)nvec * std::max(8 * nq, qg * nee) * sizeof(double2);
 }
@@ -1484,14 +1493,16 @@ typedef void (*DabKernel)(int, int, int, double, double, double, const int *, co
 template <bool IMAG>
 static DabKernel dab_kernel_t(int nl) {
     switch (nl) {
-        case 3: return k_dab<3, IMAG>;
-        case 4: return k_dab<4, IMAG>;
-        case 5: return k_dab<5, IMAG>;
-        case 6: return k_dab<6, IMAG>;
+        case 3: return k_dab_sf<3, IMAG>;
+        case 4: return k_dab_sf<4, IMAG>;
+        case 5: return k_dab_sf<5, IMAG>;
+        case 6: return k_dab_sf<6, IMAG>;
         case 7: return k_dab<7, IMAG>;
         default: return nullptr;
     }
 }
+// shape tables of the kernel: the 1-D tables (k_dab_sf) or shlT (k_dab)
+static const double *dab_shape(const f25_mesh *m) { return dab_sf(m) ? m->d_t1d : m->d_shlT; }
 // DABMKISOB25 form (imaginary ky coupling) or DKM_VPVS form (real)
 static DabKernel dab_kernel(int nl, bool imag = true) { return imag ? dab_kernel_t<true>(nl) : dab_kernel_t<false>(nl); }
 
@@ -1536,7 +1547,7 @@ static int dab_launch_columns(f25_mesh *m, double kv, int nvec, const std::vecto
     if (!r) r = upload(&dc, coef.data(), coef.size(), m->st);
     if (!r) {
         hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(inv.nnpgl), dim3(256), dab_lds(m, nvec), m->st, 1,
-                           inv.nnpgl, nvec, inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de, dc, m->d_shlT, m->d_wq, m->d_xq, m->d_ieng,
+                           inv.nnpgl, nvec, inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de, dc, dab_shape(m), m->d_wq, m->d_xq, m->d_ieng,
                            m->d_xl, m->d_zl, m->d_lm, x0, x1, (double2 *)nullptr, (double2 *)nullptr, z0, z1,
                            m->d_mygrad, m->d_jcsc, m->d_icsc, p0, p1, (float2 *)nullptr, m->d_err);
         hipError_t e = hipGetLastError();
@@ -1585,7 +1596,7 @@ extern "C" int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *w
     if (!r) {
         (void)hipEventRecord(m->ev0, m->st);
         hipLaunchKernelGGL(dab_kernel(m->tab.nl, false), dim3(n), dim3(256), dab_lds(m, 1), m->st, 2, n, 1, kp, ks, kv,
-                           de, dc, m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, dw,
+                           de, dc, dab_shape(m), m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, dw,
                            (const float2 *)nullptr, (double2 *)nullptr, (double2 *)nullptr, (const float2 *)nullptr,
                            (const float2 *)nullptr, m->d_mygrad, m->d_jcsc, m->d_icsc, (float *)nullptr,
                            (float *)nullptr, df, m->d_err);
@@ -1650,7 +1661,7 @@ extern "C" int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float 
             if (n <= 0) continue;
             hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(n), dim3(256), dab_lds(m, 2), m->st, 0, n, 2,
                                inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de + o, dc + 4 * (size_t)o,
-                               m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, du, dl, ya, yb,
+                               dab_shape(m), m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, du, dl, ya, yb,
                                (const float2 *)nullptr, (const float2 *)nullptr, m->d_mygrad, m->d_jcsc,
                                m->d_icsc, (float *)nullptr, (float *)nullptr, (float2 *)nullptr, m->d_err);
         }

@@ -1772,6 +1772,61 @@ __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *
 // Flux layout per (vector, q), 8 complex: comp 1 N, Nx, Nz; comp 2 N, Nx,
 // Nz; comp 3 N, Nz (comp 3's Nx flux equals comp 1's Nz flux).  LDS holds
 // the fluxes of nvec vectors; the projection's partial sums reuse it.
+// Row outputs of one k_dab item (rows are threads 0..nee-1, row = -1 for
+// the others): mode 0 adds into y, mode 1 contracts with z into p(MYGRAD),
+// mode 2 writes column it of F; every row of modes 1 and 2 is looked up in
+// column it's CSC pattern (err 3 if absent).
+__device__ inline void dab_epilogue(int mode, int nvec, int it, int row, const int *__restrict__ L,
+                                    const double2 *val, const int *__restrict__ jcsc, const int *__restrict__ icsc,
+                                    const int *__restrict__ mygrad, double2 *__restrict__ y0,
+                                    double2 *__restrict__ y1, const float2 *__restrict__ z0,
+                                    const float2 *__restrict__ z1, float *__restrict__ p0, float *__restrict__ p1,
+                                    float2 *__restrict__ fout, int *__restrict__ err, double (*red)[4]) {
+    int i1 = 0, i2 = 0;
+    if (mode != 0) { i1 = jcsc[it] - 1; i2 = jcsc[it + 1] - 1; }
+    double acc[2] = {0.0, 0.0};
+    const int dof = row >= 0 ? L[row] : -1;
+    if (dof >= 0) {
+        if (mode == 0) {
+            y0[dof].x += val[0].x; y0[dof].y += val[0].y;
+            if (nvec > 1) { y1[dof].x += val[1].x; y1[dof].y += val[1].y; }
+        } else {
+            int lo = i1, hi = i2 - 1, pos = -1, key = dof + 1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                if (icsc[mid] == key) { pos = mid; break; }
+                if (icsc[mid] < key) lo = mid + 1; else hi = mid - 1;
+            }
+            if (pos < 0) {
+                *err = 3;
+            } else if (mode == 2) {
+                fout[pos] = make_float2((float)val[0].x, (float)val[0].y);
+            } else {
+                const float2 za = z0[dof];
+                acc[0] -= val[0].x * za.x + val[0].y * za.y;
+                if (nvec > 1) {
+                    const float2 zb = z1[dof];
+                    acc[1] -= val[1].x * zb.x + val[1].y * zb.y;
+                }
+            }
+        }
+    }
+    if (mode != 1) return;
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+        for (int o = 32; o > 0; o >>= 1) acc[v] += __shfl_xor(acc[v], o);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][wv] = acc[0]; red[1][wv] = acc[1]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        double s0 = 0.0, s1 = 0.0;
+        for (int w = 0; w < nw; ++w) { s0 += red[0][w]; s1 += red[1][w]; }
+        p0[mygrad[it] - 1] = (float)s0;
+        if (nvec > 1) p1[mygrad[it] - 1] = (float)s1;
+    }
+}
+
 template <int NL, bool IMAG>
 __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, double kp, double ks,
                                              double kv, const int *__restrict__ item_elem,
@@ -1889,89 +1944,189 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, dou
                 }
             }
         }
-        // rows beyond the block size (nee > blockDim, one group): at most
-        // one more row per thread (nee < 2 blockDim)
-        double2 valr[2] = {};
-        const int r = threadIdx.x + blockDim.x;
-        if (r < nee) {
-            const int a = r / 3, cc = r % 3;
-            const int fN = 3 * cc, fX = cc == 2 ? 2 : 3 * cc + 1, fZ = cc == 2 ? 7 : 3 * cc + 2;
-            for (int q = 0; q < nq; ++q) {
-                const double *G = sG + 4 * q;
-                const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
-                const double n = shlT[(size_t)(a * 3 + 2) * nq + q];
-                const double nx = s0 * G[0] + s1 * G[1], nz = s0 * G[2] + s1 * G[3];
-#pragma unroll
-                for (int v = 0; v < 2; ++v) {
-                    if (v >= nvec) break;
-                    const double2 *F = sF + ((size_t)v * nq + q) * 8;
-                    valr[v].x += n * F[fN].x + nx * F[fX].x + nz * F[fZ].x;
-                    valr[v].y += n * F[fN].y + nx * F[fX].y + nz * F[fZ].y;
-                }
-            }
-        }
         // every thread is done with the fluxes: the partial sums take their place
         __syncthreads();
         if (grp < QG)
 #pragma unroll
             for (int v = 0; v < 2; ++v) if (v < nvec) sP[((size_t)v * QG + grp) * nee + i] = val[v];
-        if (r < nee)
-#pragma unroll
-            for (int v = 0; v < 2; ++v) if (v < nvec) sP[(size_t)v * QG * nee + r] = valr[v];
     }
     __syncthreads();
-    int i1 = 0, i2 = 0;
-    if (mode != 0) { i1 = jcsc[it] - 1; i2 = jcsc[it + 1] - 1; }
-    double acc[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < nee; i += blockDim.x) {
-        const int dof = L[i];
-        if (dof < 0) continue;
-        double2 val[2] = {};
+    double2 val[2] = {};
+    const int row = threadIdx.x < nee ? (int)threadIdx.x : -1;   // nee < 256 for every order
+    if (row >= 0) {
 #pragma unroll
         for (int v = 0; v < 2; ++v) {
             if (v >= nvec) break;
-            for (int gi = 0; gi < (i < QG * nee ? QG : 1); ++gi) {
-                const double2 pv = sP[((size_t)v * QG + gi) * nee + i];
+            for (int gi = 0; gi < QG; ++gi) {
+                const double2 pv = sP[((size_t)v * QG + gi) * nee + row];
                 val[v].x += pv.x; val[v].y += pv.y;
             }
         }
-        if (mode == 0) {
-            y0[dof].x += val[0].x; y0[dof].y += val[0].y;
-            if (nvec > 1) { y1[dof].x += val[1].x; y1[dof].y += val[1].y; }
-        } else {
-            int lo = i1, hi = i2 - 1, pos = -1, key = dof + 1;
-            while (lo <= hi) {
-                const int mid = (lo + hi) >> 1;
-                if (icsc[mid] == key) { pos = mid; break; }
-                if (icsc[mid] < key) lo = mid + 1; else hi = mid - 1;
-            }
-            if (pos < 0) { *err = 3; continue; }
-            if (mode == 2) {
-                fout[pos] = make_float2((float)val[0].x, (float)val[0].y);
-                continue;
-            }
-            const float2 za = z0[dof];
-            acc[0] -= val[0].x * za.x + val[0].y * za.y;
-            if (nvec > 1) {
-                const float2 zb = z1[dof];
-                acc[1] -= val[1].x * zb.x + val[1].y * zb.y;
-            }
+    }
+    dab_epilogue(mode, nvec, it, row, L, val, jcsc, icsc, mygrad, y0, y1, z0, z1, p0, p1, fout, err, red);
+}
+
+// k_dab by sum factorisation, orders with Q1 <= 16 (p <= 5); same items,
+// modes, fluxes and outputs.  Per vector, the element field and its
+// reference derivatives at the quadrature points come from two 1-D passes
+// (as in k_grad_sf: T1, T2 along eta, then rows of 16 lanes along xi with
+// the DPP broadcast), and the projection on the test functions is split the
+// same way: with A = rx F_x + rz F_z, B = sx F_x + sz F_z, C = F_N per
+// component at every point, row (j, i) of the element vector is
+//   sum_eta v_j(eta) (PA + PC)[eta][i] + v'_j(eta) PB[eta][i],
+//   PA = sum_xi v'_i(xi) A,  PB = sum_xi v_i(xi) B,  PC = sum_xi v_i(xi) C.
+// About 4x fewer multiply-adds than the direct sums over the element
+// nodes and the quadrature points.
+template <int NL, bool IMAG>
+__global__ __launch_bounds__(256) void k_dab_sf(int mode, int nitems, int nvec, double kp, double ks,
+                                                double kv, const int *__restrict__ item_elem,
+                                                const double *__restrict__ item_coef,
+                                                const double *__restrict__ t1d, const double *__restrict__ wq,
+                                                const double *__restrict__ xq, const int *__restrict__ ieng,
+                                                const double *__restrict__ xl, const double *__restrict__ zl,
+                                                const int *__restrict__ lm, const float2 *__restrict__ x0,
+                                                const float2 *__restrict__ x1, double2 *__restrict__ y0,
+                                                double2 *__restrict__ y1, const float2 *__restrict__ z0,
+                                                const float2 *__restrict__ z1, const int *__restrict__ mygrad,
+                                                const int *__restrict__ jcsc, const int *__restrict__ icsc,
+                                                float *__restrict__ p0, float *__restrict__ p1,
+                                                float2 *__restrict__ fout, int *__restrict__ err) {
+    constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen, QQ = Q1 * Q1, NT = 256;
+    static_assert(Q1 <= 16, "one row of 16 lanes per eta point");
+    extern __shared__ double sm[];
+    __shared__ double red[2][4];
+    double2 *su = reinterpret_cast<double2 *>(sm);          // [2][nee]
+    double *sG = reinterpret_cast<double *>(su + 2 * nee);  // [QQ][5]: rx, sx, rz, sz, D
+    double2 *sT = reinterpret_cast<double2 *>(sG + 5 * QQ); // [3][Q1][N1][2]: T1, T2
+    double2 *sH = sT + 3 * Q1 * N1 * 2;                     // [3 comp][3: A, B, C][QQ]
+    double2 *sP = sH + 9 * QQ;                              // [3][Q1][N1][3]: PA, PB, PC
+    const int it = blockIdx.x, tid = threadIdx.x;
+    if (it >= nitems) return;
+    const int e = item_elem[it];
+    const double c0 = item_coef[4 * it], c1 = item_coef[4 * it + 1], c2 = item_coef[4 * it + 2],
+                 c3 = item_coef[4 * it + 3];
+    const int *ie = ieng + 4 * e;
+    const int *L = lm + (size_t)e * nee;
+    const double *v1 = t1d, *d1 = t1d + N1 * Q1;   // v1[k * Q1 + q]
+    if (mode == 2)
+        for (int r = jcsc[it] - 1 + tid; r < jcsc[it + 1] - 1; r += NT) fout[r] = make_float2(0.f, 0.f);
+    for (int i = tid; i < nee; i += NT) {
+        const int d = L[i];
+        float2 a = d >= 0 ? x0[d] : make_float2(0.f, 0.f);
+        su[i] = make_double2(a.x, a.y);
+        if (nvec > 1) {
+            float2 b = d >= 0 ? x1[d] : make_float2(0.f, 0.f);
+            su[nee + i] = make_double2(b.x, b.y);
         }
     }
-    if (mode != 1) return;
-#pragma unroll
-    for (int v = 0; v < 2; ++v)
-        for (int o = 32; o > 0; o >>= 1) acc[v] += __shfl_xor(acc[v], o);
-    const int wv = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { red[0][wv] = acc[0]; red[1][wv] = acc[1]; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = blockDim.x >> 6;
-        double s0 = 0.0, s1 = 0.0;
-        for (int w = 0; w < nw; ++w) { s0 += red[0][w]; s1 += red[1][w]; }
-        p0[mygrad[it] - 1] = (float)s0;
-        if (nvec > 1) p1[mygrad[it] - 1] = (float)s1;
+    for (int q = tid; q < QQ; q += NT) {
+        const int ix = q % Q1, iz = q / Q1;
+        Geo g;
+        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) *err = 2;
+        double *G = sG + 5 * q;
+        G[0] = g.dzdeta / g.det; G[1] = -g.dzdxi / g.det; G[2] = -g.dxdeta / g.det; G[3] = g.dxdxi / g.det;
+        G[4] = wq[ix] * wq[iz] * g.det * (c0 * g.sf[0] + c1 * g.sf[1] + c2 * g.sf[2] + c3 * g.sf[3]);
     }
+    // stage-2 lanes: row = eta point, lane = xi point (clamped when idle)
+    const int row16 = tid >> 4, x16 = tid & 15;
+    const bool act = row16 < Q1 && x16 < Q1;
+    const int qe = min(row16, Q1 - 1), qx = min(x16, Q1 - 1), ni = min(x16, N1 - 1);
+    double vx[N1], dx[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) { vx[i] = v1[i * Q1 + qx]; dx[i] = d1[i * Q1 + qx]; }
+    const double k2 = kv * kv;
+    // output row of this thread: node (j, i) = (a / N1, a % N1), component cc
+    const int orow = tid < nee ? tid : -1;
+    const int oa = tid / 3, occ = tid % 3, oj = oa / N1, oi = oa % N1;
+    double2 val[2] = {};
+    for (int v = 0; v < nvec; ++v) {
+        __syncthreads();   // su, sG written; the previous vector is done with sT, sH, sP
+        for (int k = tid; k < 3 * Q1 * N1; k += NT) {
+            const int i = k % N1, qy = (k / N1) % Q1, c = k / (N1 * Q1);
+            double2 t1 = make_double2(0.0, 0.0), t2 = t1;
+#pragma unroll
+            for (int j = 0; j < N1; ++j) {
+                const double2 a = su[v * nee + (j * N1 + i) * 3 + c];
+                const double vv = v1[j * Q1 + qy], dd = d1[j * Q1 + qy];
+                t1.x += vv * a.x; t1.y += vv * a.y;
+                t2.x += dd * a.x; t2.y += dd * a.y;
+            }
+            sT[k * 2] = t1;
+            sT[k * 2 + 1] = t2;
+        }
+        __syncthreads();
+        {
+            const double *G = sG + 5 * (qx + Q1 * qe);
+            const double rx = G[0], sx = G[1], rz = G[2], sz = G[3], D = G[4];
+            const double P = kp * D, S = ks * D, Q = P - 2.0 * S;
+            double2 U[3], UX[3], UZ[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double2 *o = sT + ((c * Q1 + qe) * N1 + ni) * 2;
+                const double2 T1 = o[0], T2 = o[1];
+                double2 u0 = make_double2(0.0, 0.0), uxi = u0, uet = u0;
+                sf_row_sums<NL, 0>(u0, uxi, uet, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+                U[c] = u0;
+                UX[c] = make_double2(rx * uxi.x + sx * uet.x, rx * uxi.y + sx * uet.y);
+                UZ[c] = make_double2(rz * uxi.x + sz * uet.x, rz * uxi.y + sz * uet.y);
+            }
+            auto mk = [&](double2 X) {
+                return IMAG ? make_double2(-kv * X.y, kv * X.x) : make_double2(kv * X.x, kv * X.y);
+            };
+            auto lin = [](double a, double2 X, double b, double2 Y) {
+                return make_double2(a * X.x + b * Y.x, a * X.y + b * Y.y);
+            };
+            // the fluxes of k_dab
+            const double2 m2 = mk(U[1]);
+            double2 F[8];
+            F[0] = lin(S, mk(UX[1]), k2 * S, U[0]);
+            F[1] = make_double2(P * UX[0].x + Q * UZ[2].x - Q * m2.x, P * UX[0].y + Q * UZ[2].y - Q * m2.y);
+            F[2] = make_double2(S * (UZ[0].x + UX[2].x), S * (UZ[0].y + UX[2].y));
+            F[3] = lin(Q, mk(make_double2(UX[0].x + UZ[2].x, UX[0].y + UZ[2].y)), k2 * P, U[1]);
+            F[4] = lin(S, UX[1], -S, mk(U[0]));
+            F[5] = lin(S, UZ[1], -S, mk(U[2]));
+            F[6] = lin(S, mk(UZ[1]), k2 * S, U[2]);
+            F[7] = make_double2(Q * UX[0].x + P * UZ[2].x - Q * m2.x, Q * UX[0].y + P * UZ[2].y - Q * m2.y);
+            if (act) {
+                const int q = qx + Q1 * qe;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double2 fN = F[3 * c], fX = F[c == 2 ? 2 : 3 * c + 1], fZ = F[c == 2 ? 7 : 3 * c + 2];
+                    sH[(c * 3 + 0) * QQ + q] = lin(rx, fX, rz, fZ);
+                    sH[(c * 3 + 1) * QQ + q] = lin(sx, fX, sz, fZ);
+                    sH[(c * 3 + 2) * QQ + q] = fN;
+                }
+            }
+        }
+        __syncthreads();
+        for (int k = tid; k < 3 * Q1 * N1; k += NT) {
+            const int i = k % N1, qy = (k / N1) % Q1, c = k / (N1 * Q1);
+            const double2 *A = sH + (c * 3 + 0) * QQ + qy * Q1, *B = sH + (c * 3 + 1) * QQ + qy * Q1,
+                          *Cc = sH + (c * 3 + 2) * QQ + qy * Q1;
+            double2 pa = make_double2(0.0, 0.0), pb = pa, pc = pa;
+            for (int x = 0; x < Q1; ++x) {
+                const double vv = v1[i * Q1 + x], dd = d1[i * Q1 + x];
+                pa.x += dd * A[x].x; pa.y += dd * A[x].y;
+                pb.x += vv * B[x].x; pb.y += vv * B[x].y;
+                pc.x += vv * Cc[x].x; pc.y += vv * Cc[x].y;
+            }
+            sP[k * 3] = pa;
+            sP[k * 3 + 1] = pb;
+            sP[k * 3 + 2] = pc;
+        }
+        __syncthreads();
+        if (orow >= 0) {
+            double2 r = make_double2(0.0, 0.0);
+            for (int qy = 0; qy < Q1; ++qy) {
+                const double2 *o = sP + ((occ * Q1 + qy) * N1 + oi) * 3;
+                const double vv = v1[oj * Q1 + qy], dd = d1[oj * Q1 + qy];
+                r.x += vv * (o[0].x + o[2].x) + dd * o[1].x;
+                r.y += vv * (o[0].y + o[2].y) + dd * o[1].y;
+            }
+            if (v == 0) val[0] = r; else val[1] = r;
+        }
+    }
+    dab_epilogue(mode, nvec, it, orow, L, val, jcsc, icsc, mygrad, y0, y1, z0, z1, p0, p1, fout, err, red);
 }
 
 // ---------------------------------------------------------------------------
