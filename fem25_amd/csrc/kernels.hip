@@ -2216,7 +2216,23 @@ __global__ void k_gn_jp(int nnpgl, int cch, const int *__restrict__ nobs, const 
     double2 *out = jp + poff[j];
     for (int o = threadIdx.x; o < no; o += blockDim.x) {
         double ar = 0.0, ai = 0.0;
-        for (int c = c0; c < c1; ++c) {
+        // eight columns per step: eight loads in flight per lane
+        int c = c0;
+        for (; c + 8 <= c1; c += 8) {
+            float2 v[8];
+            double pv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                v[k] = J[(size_t)(c + k) * no + o];
+                pv[k] = p[mygrad[c + k] - 1];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                ar += (double)v[k].x * pv[k];
+                ai += (double)v[k].y * pv[k];
+            }
+        }
+        for (; c < c1; ++c) {
             const double pv = p[mygrad[c] - 1];
             const float2 v = J[(size_t)c * no + o];
             ar += (double)v.x * pv;
@@ -2237,7 +2253,30 @@ __global__ void k_gn_z(int nnpgl, int njac, const int *__restrict__ nobs, const 
     const int lane = threadIdx.x & 63;
     if (c >= nnpgl) return;
     double s = 0.0;
-    for (int j = 0; j < njac; ++j) {
+    // four Jacobians per step: their column loads are in flight together
+    int j = 0;
+    for (; j + 4 <= njac; j += 4) {
+        double sj[4] = {0.0, 0.0, 0.0, 0.0};
+        int no[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) no[k] = nobs[j + k];
+        const int nmax = max(max(no[0], no[1]), max(no[2], no[3]));
+        for (int o = lane; o < nmax; o += 64) {
+            float2 v[4];
+            double2 a[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool in = o < no[k];
+                v[k] = in ? jac[joff[j + k] + (size_t)c * no[k] + o] : make_float2(0.f, 0.f);
+                a[k] = in ? jp[poff[j + k] + o] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sj[k] += (double)v[k].x * a[k].x + (double)v[k].y * a[k].y;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s += alpha[j + k] * sj[k];
+    }
+    for (; j < njac; ++j) {
         const int no = nobs[j];
         const float2 *J = jac + joff[j] + (size_t)c * no;
         const double2 *P = jp + poff[j];
