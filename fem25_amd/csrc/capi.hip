@@ -1035,9 +1035,6 @@ extern "C" int f25_apply_batch(f25_mesh *m, int npair, const double *omega, cons
 }
 
 // ---------------------------------------------------------------------------
-#ifndef F25_GRAD_T
-#define F25_GRAD_T 2
-#endif
 static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d_ky, const double2 *u,
                      const double2 *l, double *g, int chain) {
     const int nen = m->tab.nen, nq = m->tab.nq;
@@ -1054,7 +1051,9 @@ static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d
     typedef void (*GradSf)(const double *, const double *, const double *, const int *, const double *,
                            const double *, const char *, const int *, int, int64_t, const double *,
                            const double *, const double2 *, const double2 *, double *, int *);
-    constexpr int TS = F25_GRAD_T;   // trips per pass
+    // trips per pass: two fit three workgroups per CU (200x100 p4, 128
+    // triples: 11.4 ms; four per pass: 12.3 ms)
+    constexpr int TS = 2;
     GradSf ksf = nullptr;
     switch (m->tab.nl) {
         case 3: ksf = k_grad_sf<3, TS>; break;

@@ -243,8 +243,10 @@ def test_gradient_exact(name, nf, nk):
         assert relerr(gd[:, p], ref[:, p]) < 1e-11, p
 
 
-@pytest.mark.parametrize("name,cinv", [("tiny_layered", b"PP"), ("pml", b"PP"), ("tiny", b"SS")])
+@pytest.mark.parametrize("name,cinv", [("tiny_layered", b"PP"), ("pml", b"PP"), ("tiny", b"SS"), ("p5", b"PP"),
+                                       ("p6", b"SS")])
 def test_cfmat_adjflam(name, cinv):
+    """p <= 5: the sum-factorised element products; p6: the direct ones."""
     from fem25_amd import api
     mi = _mi(name)
     m, mod, pml = mesh_case(name)
@@ -357,7 +359,7 @@ def test_pchadj_jacobian():
     assert relerr(jac.T, Jd) < 1e-6      # FP64 sums rounded once to COMPLEX*8
 
 
-@pytest.mark.parametrize("name", ["tiny_layered", "pml", "p5"])
+@pytest.mark.parametrize("name", ["tiny_layered", "pml", "p5", "p6"])
 @pytest.mark.parametrize("cinv", [b"PP", b"SS"])
 def test_qpmfree_element_products(name, cinv):
     """CADJ_NU_ALPHA's right-hand sides and P3, and MET225_MOD's P1 and P2,
