@@ -586,7 +586,7 @@ static int ensure_kinds(f25_mesh *m) {
 // grid of a chunk-stream kernel: enough workgroups of 4 waves to fill the
 // chip several times over, each wave then striding over the chunks
 static unsigned chunk_grid(int64_t nz) {
-    const int64_t nchunk = (nz + 63) / 64;
+    const int64_t nchunk = (nz + 127) / 128;   // k_csr_combine: 128 entries per wave step
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((nchunk + 3) / 4, 8192));
 }
 

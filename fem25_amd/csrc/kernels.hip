@@ -941,35 +941,58 @@ __global__ void k_csr_basis(int nen, int ncol_elem, const int *__restrict__ orde
 
 // S(omega, ky) from the basis: same component: V0 + ky^2 V1 - omega^2 V2,
 // 1-3 coupling: V0, 1-2/2-3 couplings: -/+ i ky V0 (block signs above).
-// A stream over the entries, one wave per chunk of 64 (grid-stride): the
-// compact V1/V2 position of a kind-0 entry is its chunk's base plus the
-// kind-0 entries before it in the chunk (ballot and popcount).  Bytes per
-// entry: kind 1 + V0 8 + the value; V1, V2 only for kind 0.
+// A stream over the entries, one wave per 128 entries (two adjacent ones
+// per lane, 16-byte loads and stores; grid-stride): the compact V1/V2
+// position of a kind-0 entry is the base of its 64-entry chunk (cbase) plus
+// the kind-0 entries before it in the chunk (ballot and popcount).  Bytes
+// per entry: kind 1 + V0 8 + the value; V1, V2 only for kind 0.
+template <typename CT>
+__device__ inline void csr_value(int k, double a, double k2, double w2, double kv, const double *__restrict__ v1,
+                                 const double *__restrict__ v2, int64_t j, CT &o) {
+    double re = 0.0, im = 0.0;
+    if (k == 0) re = a + k2 * v1[j] - w2 * v2[j];
+    else if (k == 1) re = a;
+    else im = (k == 2 ? -kv : kv) * a;
+    o.x = re;
+    o.y = im;
+}
+
 template <typename CT>
 __global__ void k_csr_combine(int64_t nz, const int64_t *__restrict__ cbase, const signed char *__restrict__ kind,
                               const double *__restrict__ v0, const double *__restrict__ v1,
                               const double *__restrict__ v2, double om, double kv, CT *__restrict__ out) {
     const int lane = threadIdx.x & 63;
     const double k2 = kv * kv, w2 = om * om;
-    const int64_t nchunk = (nz + 63) / 64, step = (int64_t)gridDim.x * (blockDim.x / 64);
-    for (int64_t q = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; q < nchunk; q += step) {
-        const int64_t s = q * 64 + lane;
-        const bool in = s < nz;
-        const int k = in ? kind[s] : -1;
-        const unsigned long long msk = __ballot(k == 0);
-        if (!in) continue;
-        const double a = v0[s];
-        double re = 0.0, im = 0.0;
-        if (k == 0) {
-            const int64_t j = cbase[q] + __popcll(msk & ((1ull << lane) - 1));
-            re = a + k2 * v1[j] - w2 * v2[j];
-        } else if (k == 1) {
-            re = a;
-        } else {
-            im = (k == 2 ? -kv : kv) * a;
+    const unsigned long long lt = (1ull << lane) - 1;
+    const int64_t nq = (nz + 127) / 128, step = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t q = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; q < nq; q += step) {
+        const int64_t s = q * 128 + 2 * lane;
+        const bool in0 = s < nz, in1 = s + 1 < nz;
+        int ka = -1, kb = -1;
+        if (in1) {
+            const short kk = *reinterpret_cast<const short *>(kind + s);   // s even: 2-byte aligned
+            ka = (signed char)(kk & 0xff);
+            kb = (signed char)(kk >> 8);
+        } else if (in0) {
+            ka = kind[s];
         }
-        out[s].x = re;
-        out[s].y = im;
+        // kind-0 entries of the first 64-entry chunk (lanes 0..31) count into
+        // the second one's base; every lane's prefix starts from chunk 2q's base
+        const unsigned long long ma = __ballot(ka == 0), mb = __ballot(kb == 0);
+        const int64_t j = cbase[2 * q] + __popcll(ma & lt) + __popcll(mb & lt);
+        if (!in0) continue;
+        if (in1) {
+            const double2 a = *reinterpret_cast<const double2 *>(v0 + s);
+            CT o0, o1;
+            csr_value(ka, a.x, k2, w2, kv, v1, v2, j, o0);
+            csr_value(kb, a.y, k2, w2, kv, v1, v2, j + (ka == 0), o1);
+            out[s] = o0;
+            out[s + 1] = o1;
+        } else {
+            CT o0;
+            csr_value(ka, v0[s], k2, w2, kv, v1, v2, j, o0);
+            out[s] = o0;
+        }
     }
 }
 
