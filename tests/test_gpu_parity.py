@@ -183,6 +183,15 @@ def test_apply_batch_graph_replays(torch_dev):
     mi.apply_batch_dev(om, ky, x, ye, _lib.F25_APPLY_F64)
     torch.cuda.synchronize()
     assert torch.equal(y2, ye)
+    mi.set_apply_tiles(2)               # another kernel: the capture is dropped and redone
+    mi.apply_batch_dev(om, ky, x, y2, _lib.F25_APPLY_F64 | _lib.F25_APPLY_GRAPH)
+    mi.apply_batch_dev(om, ky, x, ye, _lib.F25_APPLY_F64)
+    torch.cuda.synchronize()
+    assert torch.equal(y2, ye)
+    mi.set_apply_tiles(0)
+    from fem25_amd._lib import Fem25Error
+    with pytest.raises(Fem25Error):     # graph replay is for the FP64 product only
+        mi.apply_batch_dev(om, ky, x, y2, _lib.F25_APPLY_F32_REFINED | _lib.F25_APPLY_GRAPH)
 
 
 def test_apply_batch_config1_modes(torch_dev):
