@@ -1051,8 +1051,8 @@ static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d
     typedef void (*GradSf)(const double *, const double *, const double *, const int *, const double *,
                            const double *, const char *, const int *, int, int64_t, const double *,
                            const double *, const double2 *, const double2 *, double *, int *);
-    // trips per pass: two fit three workgroups per CU (200x100 p4, 128
-    // triples: 11.4 ms; four per pass: 12.3 ms)
+    // trips per pass: with two, four workgroups fit per CU at p4 (LDS and
+    // registers; 200x100, 128 triples: 9.4 ms)
     constexpr int TS = 2;
     GradSf ksf = nullptr;
     switch (m->tab.nl) {
@@ -1065,7 +1065,8 @@ static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d
     if (ksf) {
         const int nl = m->tab.nl, q1 = m->tab.nq1;
         const size_t lsf = (size_t)TS * 2 * 3 * nen * sizeof(double2) +
-                           (size_t)TS * 2 * 3 * q1 * nl * 2 * sizeof(double2) + (size_t)q1 * q1 * 8 * sizeof(double);
+                           (size_t)TS * 2 * 3 * q1 * nl * 2 * sizeof(double2) + (size_t)q1 * q1 * 4 * sizeof(double) +
+                           (size_t)2 * nl * q1 * sizeof(double);
         if (lsf > 64 * 1024)
             HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
         hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(256), lsf, m->st, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,

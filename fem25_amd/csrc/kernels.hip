@@ -1377,7 +1377,13 @@ __global__ __launch_bounds__(512) void k_grad(int nen, int nq1, int nelem, int t
 // and v_fmac_f64 with row_newbcast:i hands them to the whole row.  The
 // densities, the projection on the anchor hat functions and the reduction
 // are those of k_grad; the 1-D tables are not flushed to zero as DSHL's
-// products are (a difference below 2.2e-16 per shape value).
+// products are (a difference below 2.2e-16 per shape value).  The element
+// values of the next pass of triples are loaded into registers while the
+// current pass computes, the forward field's gradient is reduced to its
+// symmetrised form before the adjoint field's components are formed one at
+// a time, and the quadrature weights are recomputed for the projection
+// instead of kept in LDS: 128 registers and 37 KB of LDS per workgroup at
+// p4, four workgroups per CU.
 template <int NL, int I, int... J>
 __device__ __forceinline__ void sf_row_sums(double2 &U, double2 &UX, double2 &UE, const double2 &T1,
                                             const double2 &T2, const double *vx, const double *dx,
@@ -1388,20 +1394,21 @@ __device__ __forceinline__ void sf_row_sums(double2 &U, double2 &UX, double2 &UE
 }
 
 template <int NL, int T>
-__global__ __launch_bounds__(256) void k_grad_sf(const double *__restrict__ t1d, const double *__restrict__ wq,
-                                                 const double *__restrict__ xq, const int *__restrict__ ieng,
-                                                 const double *__restrict__ xl, const double *__restrict__ zl,
-                                                 const char *__restrict__ cdom, const int *__restrict__ lm,
-                                                 int ntrip, int64_t ndof, const double *__restrict__ omega,
-                                                 const double *__restrict__ ky, const double2 *__restrict__ u,
-                                                 const double2 *__restrict__ l, double *__restrict__ part,
-                                                 int *__restrict__ err) {
+__global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_grad_sf(
+    const double *__restrict__ t1d, const double *__restrict__ wq, const double *__restrict__ xq,
+    const int *__restrict__ ieng, const double *__restrict__ xl, const double *__restrict__ zl,
+    const char *__restrict__ cdom, const int *__restrict__ lm, int ntrip, int64_t ndof,
+    const double *__restrict__ omega, const double *__restrict__ ky, const double2 *__restrict__ u,
+    const double2 *__restrict__ l, double *__restrict__ part, int *__restrict__ err) {
     constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen, NT = 256;
+    constexpr int NW = T * 2 * nee;            // element values gathered per pass
+    constexpr int NG = (NW + NT - 1) / NT;     // of them per thread
     static_assert(Q1 <= 16, "one row of 16 lanes per eta point");
     extern __shared__ double sm[];
     double2 *sw = reinterpret_cast<double2 *>(sm);                     // [T][2][nee]
-    double2 *sT = sw + T * 2 * nee;                                    // [T][2][3][Q1][N1][2]
-    double *sG = reinterpret_cast<double *>(sT + T * 2 * 3 * Q1 * N1 * 2);   // [Q1*Q1][8]
+    double2 *sT = sw + NW;                                             // [T][2][3][Q1][N1][2]
+    double *sG = reinterpret_cast<double *>(sT + T * 2 * 3 * Q1 * N1 * 2);   // [Q1*Q1][4]
+    double *s1 = sG + 4 * Q1 * Q1;                                     // [2][N1][Q1]
     __shared__ double red[NT / 64][12];
     const int e = blockIdx.x, tid = threadIdx.x;
     if (cdom[e] == 'A') {   // PML elements are outside the non-PML gradient path
@@ -1410,17 +1417,35 @@ __global__ __launch_bounds__(256) void k_grad_sf(const double *__restrict__ t1d,
     }
     const int *ie = ieng + 4 * e;
     const int *L = lm + (size_t)e * nee;
-    const double *v1 = t1d, *d1 = t1d + N1 * Q1;   // v1[k * Q1 + q]
     for (int q = tid; q < Q1 * Q1; q += NT) {
         const int ix = q % Q1, iz = q / Q1;
         Geo g;
         if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
-        double *G = sG + 8 * q;
+        double *G = sG + 4 * q;
         G[0] = g.dzdeta / g.det; G[1] = -g.dzdxi / g.det; G[2] = -g.dxdeta / g.det; G[3] = g.dxdxi / g.det;
-        const double wt = wq[ix] * wq[iz] * g.det;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) G[4 + k] = wt * g.sf[k];
     }
+    for (int i = tid; i < 2 * N1 * Q1; i += NT) s1[i] = t1d[i];   // v_j(eta_q), then v'_j(eta_q)
+    // the gather slots of this thread are the same in every pass: slot r is
+    // value tid + r NT of [T][2][nee]; its DOF (or -1) is fixed per element
+    int sd[NG];
+#pragma unroll
+    for (int r = 0; r < NG; ++r) {
+        const int i = tid + r * NT;
+        sd[r] = (i < NW) ? L[i % nee] : -1;
+    }
+    // the values of the next pass are loaded into registers before this
+    // pass's arithmetic and written to LDS after it
+    double2 pre[NG];
+    auto gather = [&](int t0) {
+#pragma unroll
+        for (int r = 0; r < NG; ++r) {
+            const int i = tid + r * NT;
+            const int tt = i / (2 * nee), f = (i / nee) % 2, t = t0 + tt;
+            const double2 *w = f ? l : u;
+            pre[r] = (sd[r] >= 0 && t < ntrip) ? w[(size_t)t * ndof + sd[r]] : make_double2(0.0, 0.0);
+        }
+    };
+    gather(0);
     // second stage: row = eta point, lane = xi point (clamped when idle: an
     // idle lane computes a copy and contributes nothing)
     const int row = tid >> 4, x16 = tid & 15;
@@ -1428,16 +1453,13 @@ __global__ __launch_bounds__(256) void k_grad_sf(const double *__restrict__ t1d,
     const int qe = min(row, Q1 - 1), qx = min(x16, Q1 - 1), ni = min(x16, N1 - 1);
     double vx[N1], dx[N1];
 #pragma unroll
-    for (int i = 0; i < N1; ++i) { vx[i] = v1[i * Q1 + qx]; dx[i] = d1[i * Q1 + qx]; }
+    for (int i = 0; i < N1; ++i) { vx[i] = t1d[i * Q1 + qx]; dx[i] = t1d[(N1 + i) * Q1 + qx]; }
     double Dl = 0.0, Dm = 0.0, Dr = 0.0;
     for (int t0 = 0; t0 < ntrip; t0 += T) {
-        __syncthreads();   // the previous pass is done with sw, sT (and sG is written)
-        for (int i = tid; i < T * 2 * nee; i += NT) {
-            const int tt = i / (2 * nee), f = (i / nee) % 2, j = i % nee, t = t0 + tt;
-            const int d = L[j];
-            const double2 *w = f ? l : u;
-            sw[i] = (d >= 0 && t < ntrip) ? w[(size_t)t * ndof + d] : make_double2(0.0, 0.0);
-        }
+        __syncthreads();   // the previous pass is done with sw, sT (and sG, s1 are written)
+#pragma unroll
+        for (int r = 0; r < NG; ++r)
+            if (tid + r * NT < NW) sw[tid + r * NT] = pre[r];
         __syncthreads();
         for (int it = tid; it < T * 2 * 3 * Q1 * N1; it += NT) {
             const int i = it % N1, qy = (it / N1) % Q1, fc = it / (N1 * Q1);   // fc = (tt*2 + f)*3 + c
@@ -1447,7 +1469,7 @@ __global__ __launch_bounds__(256) void k_grad_sf(const double *__restrict__ t1d,
 #pragma unroll
             for (int j = 0; j < N1; ++j) {
                 const double2 a = w[(j * N1 + i) * 3 + c];
-                const double vv = v1[j * Q1 + qy], dd = d1[j * Q1 + qy];
+                const double vv = s1[j * Q1 + qy], dd = s1[(N1 + j) * Q1 + qy];
                 t1.x += vv * a.x; t1.y += vv * a.y;
                 t2.x += dd * a.x; t2.y += dd * a.y;
             }
@@ -1455,56 +1477,76 @@ __global__ __launch_bounds__(256) void k_grad_sf(const double *__restrict__ t1d,
             o[0] = t1;
             o[1] = t2;
         }
-        __syncthreads();
-        const double *G = sG + 8 * (qx + Q1 * qe);
-        const double rx = G[0], sx = G[1], rz = G[2], sz = G[3];
+        double kt[T], wt[T];
+#pragma unroll
         for (int tt = 0; tt < T; ++tt) {
-            const int t = t0 + tt;
-            if (t >= ntrip) break;
-            const double k = ky[t], w = omega[t];
-            double2 GU[3][3], GV[3][3], Uv[3], Vv[3];
+            const int t = min(t0 + tt, ntrip - 1);
+            kt[tt] = ky[t];
+            wt[tt] = omega[t];
+        }
+        __syncthreads();
+        if (t0 + T < ntrip) gather(t0 + T);
+        const double *G = sG + 4 * (qx + Q1 * qe);
+        const double rx = G[0], sx = G[1], rz = G[2], sz = G[3];
 #pragma unroll
-            for (int f = 0; f < 2; ++f)
+        for (int tt = 0; tt < T; ++tt) {
+            if (t0 + tt >= ntrip) break;
+            const double k = kt[tt], w = wt[tt];
+            // gradient of component c of field f at this point: [d/dx, d/dy, d/dz]
+            auto grad = [&](int f, int c, double2 &U, double2 *Gc) {
+                const double2 *o = sT + (size_t)((((tt * 2 + f) * 3 + c) * Q1 + qe) * N1 + ni) * 2;
+                const double2 T1 = o[0], T2 = o[1];
+                double2 UXI = make_double2(0.0, 0.0), UET = UXI;
+                U = UXI;
+                sf_row_sums<NL, 0>(U, UXI, UET, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+                Gc[0] = make_double2(rx * UXI.x + sx * UET.x, rx * UXI.y + sx * UET.y);
+                Gc[1] = make_double2(k * U.y, -k * U.x);   // d/dy = -i ky
+                Gc[2] = make_double2(rz * UXI.x + sz * UET.x, rz * UXI.y + sz * UET.y);
+            };
+            // forward field: its values, divergence and symmetrised gradient
+            // S[i][j] = G_ij + G_ji; then the adjoint field's components one
+            // at a time against them
+            double2 GU[3][3], Uv[3];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double2 *o = sT + (size_t)((((tt * 2 + f) * 3 + c) * Q1 + qe) * N1 + ni) * 2;
-                    const double2 T1 = o[0], T2 = o[1];
-                    double2 U = make_double2(0.0, 0.0), UXI = U, UET = U;
-                    sf_row_sums<NL, 0>(U, UXI, UET, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
-                    const double2 UX = make_double2(rx * UXI.x + sx * UET.x, rx * UXI.y + sx * UET.y);
-                    const double2 UZ = make_double2(rz * UXI.x + sz * UET.x, rz * UXI.y + sz * UET.y);
-                    // G[c][j]: d_j of component c, j = x, y, z; d/dy = -i ky
-                    const double2 UY = make_double2(k * U.y, -k * U.x);
-                    if (f == 0) {
-                        GU[c][0] = UX; GU[c][1] = UY; GU[c][2] = UZ; Uv[c] = U;
-                    } else {
-                        GV[c][0] = UX; GV[c][1] = UY; GV[c][2] = UZ; Vv[c] = U;
-                    }
-                }
+            for (int c = 0; c < 3; ++c) grad(0, c, Uv[c], GU[c]);
             const double2 du = make_double2(GU[0][0].x + GU[1][1].x + GU[2][2].x, GU[0][0].y + GU[1][1].y + GU[2][2].y);
-            const double2 dv = make_double2(GV[0][0].x + GV[1][1].x + GV[2][2].x, GV[0][0].y + GV[1][1].y + GV[2][2].y);
-            Dl += dv.x * du.x + dv.y * du.y;
-            double sm2 = 0.0;
+            double2 S[3][3];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double2 s2 = make_double2(GU[i][j].x + GU[j][i].x, GU[i][j].y + GU[j][i].y);
-                    sm2 += GV[i][j].x * s2.x + GV[i][j].y * s2.y;
-                }
+                for (int j = 0; j < 3; ++j) S[i][j] = make_double2(GU[i][j].x + GU[j][i].x, GU[i][j].y + GU[j][i].y);
+            double2 dv = make_double2(0.0, 0.0);
+            double sm2 = 0.0, vu = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double2 V, GV[3];
+                grad(1, c, V, GV);
+                dv = c ? make_double2(dv.x + GV[c].x, dv.y + GV[c].y) : GV[0];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) sm2 += GV[j].x * S[c][j].x + GV[j].y * S[c][j].y;
+                vu += V.x * Uv[c].x + V.y * Uv[c].y;
+            }
+            Dl += dv.x * du.x + dv.y * du.y;
             Dm += sm2;
-            Dr += -w * w * (Vv[0].x * Uv[0].x + Vv[0].y * Uv[0].y + Vv[1].x * Uv[1].x + Vv[1].y * Uv[1].y +
-                            Vv[2].x * Uv[2].x + Vv[2].y * Uv[2].y);
+            Dr += -w * w * vu;
         }
     }
-    // projection on the anchors and block reduction of the 12 partial sums
+    // projection on the anchor hat functions (weights recomputed here, as the
+    // quadrature loop above computes them) and block reduction of the 12 sums
+    double fw[4] = {0.0, 0.0, 0.0, 0.0};
+    if (act) {
+        Geo g;
+        if (geo_at(xq[qx], xq[qe], ie, xl, zl, g)) g.det = 1.0;
+        const double wq2 = wq[qx] * wq[qe] * g.det;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) fw[kk] = wq2 * g.sf[kk];
+    }
     double g12[12];
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
-        const double fw = act ? sG[8 * (qx + Q1 * qe) + 4 + kk] : 0.0;
-        g12[kk * 3 + 0] = fw * Dl;
-        g12[kk * 3 + 1] = fw * Dm;
-        g12[kk * 3 + 2] = fw * Dr;
+        g12[kk * 3 + 0] = fw[kk] * Dl;
+        g12[kk * 3 + 1] = fw[kk] * Dm;
+        g12[kk * 3 + 2] = fw[kk] * Dr;
     }
 #pragma unroll
     for (int i = 0; i < 12; ++i)
