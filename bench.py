@@ -258,7 +258,15 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     mi.gn_diag(omega[:nt], ky[:nt], uh)
     wall = time.perf_counter() - t0
     k_ms = L.f25_last_kernel_ms(b"gn_diag")
-    dfl = (4.0 * 3 * 2 * 2 * 3 * nen + 30.0 * 3 * nen + 36.0 * 2 * nen) * nq * mesh.nelem * nt
+    # flops per (element, wavefield): sum factorisation for p <= 5 (fields
+    # and gradients at the points 24 N1^2 Q1 + 36 N1 Q1^2, 17 row sums per
+    # row and output lane 136 N1 Q1^2, the eta contraction 224 N1^2 Q1, about
+    # 100 per point), the direct projection for p6
+    if q1 <= 16:
+        dper = 248.0 * n1 * n1 * q1 + 172.0 * n1 * q1 * q1 + 100.0 * nq
+    else:
+        dper = (4.0 * 3 * 2 * 2 * 3 * nen + 30.0 * 3 * nen + 36.0 * 2 * nen) * nq
+    dfl = dper * mesh.nelem * nt
     kern["gn_diag"] = {"kernel_ms": round(k_ms, 3), "wavefields_per_s": round(nt / (k_ms * 1e-3), 1),
                        "roofline": roof("fp64", dfl / (k_ms * 1e-3), "TFLOP/s")}
     del u, lam, gg, uh

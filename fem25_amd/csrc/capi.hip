@@ -2176,6 +2176,24 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
         g_err = "unsupported order";
         r = F25_ERR_ARG;
     }
+    // orders with at most 16 quadrature points per direction (p <= 5) run
+    // the sum-factorised element kernel
+    typedef void (*FcolsSf)(const double *, const double *, const double *, const int *, const double *,
+                            const double *, const char *, const int *, const double *, const double *,
+                            const double *, double, double, const double2 *, double2 *, int *);
+    FcolsSf ksf = nullptr;
+    switch (m->tab.nl) {
+        case 3: ksf = k_fcols_sf<3>; break;
+        case 4: ksf = k_fcols_sf<4>; break;
+        case 5: ksf = k_fcols_sf<5>; break;
+        case 6: ksf = k_fcols_sf<6>; break;
+        default: break;
+    }
+    const int nl = m->tab.nl, q1 = m->tab.nq1;
+    const size_t lsf = (size_t)(2 * nl * q1 + 2 * q1) * sizeof(double) +
+                       (size_t)std::max(nee + 6 * q1 * nl, 28 * q1 * nl) * sizeof(double2);
+    if (!r && ksf && lsf > 65536)
+        HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
     const int nt1 = 256, qg = std::max(1, 256 / nee);
     const size_t lds1 = nee * sizeof(double2) + std::max((size_t)nq * 28 * sizeof(double),
                                                          (size_t)(qg - 1) * 12 * nee * sizeof(double2));
@@ -2187,9 +2205,14 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
         HIPCHK(hipFuncSetAttribute((const void *)k_fdiag, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
     if (!r) (void)hipEventRecord(m->ev0, m->st);
     for (int t = 0; t < ntrip && !r; ++t) {
-        hipLaunchKernelGGL(kf, dim3(m->d.nelem), dim3(nt1), lds1, m->st, m->d_shl, m->d_shlT, m->d_wq, m->d_xq,
-                           m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho, m->d_lam, m->d_mu, omega[t],
-                           ky[t], du + ndof * t, dcols, m->d_err);
+        if (ksf)
+            hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(256), lsf, m->st, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,
+                               m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho, m->d_lam, m->d_mu, omega[t], ky[t],
+                               du + ndof * t, dcols, m->d_err);
+        else
+            hipLaunchKernelGGL(kf, dim3(m->d.nelem), dim3(nt1), lds1, m->st, m->d_shl, m->d_shlT, m->d_wq,
+                               m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho, m->d_lam,
+                               m->d_mu, omega[t], ky[t], du + ndof * t, dcols, m->d_err);
         hipLaunchKernelGGL(k_fdiag, dim3(nnpg), dim3(nt2), lds2, m->st, nnpg, m->ncon_anchor, nee,
                            m->d_mconn, m->d_ieng, m->d_pmap, dcols, dd);
         hipError_t e = hipGetLastError();

@@ -1748,6 +1748,215 @@ __global__ __launch_bounds__(256) void k_fcols(const double *__restrict__ shl, c
     }
 }
 
+// fmac_bcast kept in program order (volatile): the sums of k_fcols_sf run one
+// after another instead of being interleaved into more live registers
+template <int J>
+__device__ __forceinline__ void fmac_bcast_ordered(double &acc, double cv, double op) {
+    asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+                 : "+v"(acc)
+                 : "v"(cv), "v"(op), "i"(J));
+}
+
+// The first two FMAs of a row sum whose operand F was just written by
+// vector-ALU instructions: gfx950 needs two wait states between a VALU
+// write of a VGPR and a DPP read of it, and the compiler does not look into
+// inline assembly, so the s_nop is part of the same statement.
+template <int J>
+__device__ __forceinline__ void fmac2_bcast_first(double2 &acc, const double2 &F, double c) {
+    asm volatile(
+        "s_nop 1\n\t"
+        "v_fmac_f64_dpp %0, %2, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %1, %3, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc.x), "+v"(acc.y)
+        : "v"(F.x), "v"(F.y), "v"(c), "i"(J));
+}
+
+// Row sum over the xi points of a row of 16 lanes: acc += sum_J c[J] F(lane J).
+template <int J0, int... J>
+__device__ __forceinline__ void sf_row_dot(double2 &acc, const double2 &F, const double *c,
+                                           std::integer_sequence<int, J0, J...>) {
+    fmac2_bcast_first<J0>(acc, F, c[J0]);
+    ((fmac_bcast_ordered<J>(acc.x, F.x, c[J]), fmac_bcast_ordered<J>(acc.y, F.y, c[J])), ...);
+}
+
+// The element part of k_fcols by sum factorisation, orders with Q1 <= 16
+// (p <= 5); same output.  With N_b = v_bi(xi) v_bj(eta) and the anchor hat
+// functions sf_c = h_cx(xi) h_cz(eta) (h_0 = (1 - s)/2, h_1 = (1 + s)/2;
+// corners c = 0..3 at (cx, cz) = (0,0), (1,0), (1,1), (0,1)), every column
+// entry is
+//   sum_q w sf_c [ dN_b/dxi A + dN_b/deta B + N_b C ]
+//   = sum_eta h_cz v_bj P + h_cz v'_bj R,
+//   P[cx][eta][bi] = sum_xi h_cx (v'_bi A + v_bi C),  R = sum_xi h_cx v_bi B,
+// where (A, B, C) per quadrature point are, for the density l (i = x, y, z):
+// (rx div, sx div, 0), (0, 0, i ky div), (rz div, sz div, 0); for m:
+// (rx S_ix + rz S_iz, sx S_ix + sz S_iz, i ky S_iy); for the rho term:
+// (0, 0, -omega^2 u_i), all times the quadrature weight w (the chain-rule
+// factors of k_fcols follow).  Stages: the fields and their gradients at
+// the quadrature points (rows of 16 lanes, one per eta point, as in
+// k_grad_sf); per channel, lane (cx, bi) of each row forms P and R from the
+// row's points by DPP broadcast; then one thread per (corner, node,
+// component) contracts over eta.  14 planes of P and R (9 P, 5 R) per
+// (cx, eta, bi).
+template <int NL>
+__global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_fcols_sf(const double *__restrict__ t1d, const double *__restrict__ wq,
+                                                  const double *__restrict__ xq, const int *__restrict__ ieng,
+                                                  const double *__restrict__ xl, const double *__restrict__ zl,
+                                                  const char *__restrict__ cdom, const int *__restrict__ lm,
+                                                  const double *__restrict__ rho, const double *__restrict__ lam,
+                                                  const double *__restrict__ mu, double omega, double kv,
+                                                  const double2 *__restrict__ u, double2 *__restrict__ out,
+                                                  int *__restrict__ err) {
+    constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen, NT = 256;
+    static_assert(Q1 <= 16 && 2 * N1 <= 16, "one row of 16 lanes per eta point");
+    extern __shared__ double sm[];
+    double *s1 = sm;                                  // [2][N1][Q1] v_j(s_q), v'_j(s_q); [2][Q1] h_0, h_1
+    double *sh = s1 + 2 * N1 * Q1;
+    double2 *su = reinterpret_cast<double2 *>(sh + 2 * Q1);   // nee element values
+    double2 *sT = su + nee;                                   // [3][Q1][N1][2] T1, T2
+    double2 *sP = su;   // [14][2][Q1][N1], written once su and sT are read
+    const int e = blockIdx.x, tid = threadIdx.x;
+    double2 *o = out + (size_t)e * 12 * nee;
+    if (cdom[e] == 'A') {   // PML elements are outside the non-PML gradient path
+        for (int i = tid; i < 12 * nee; i += NT) o[i] = make_double2(0.0, 0.0);
+        return;
+    }
+    const int *ie = ieng + 4 * e;
+    const int *L = lm + (size_t)e * nee;
+    for (int i = tid; i < 2 * N1 * Q1; i += NT) s1[i] = t1d[i];
+    for (int i = tid; i < 2 * Q1; i += NT) sh[i] = 0.5 * (i < Q1 ? 1.0 - xq[i] : 1.0 + xq[i - Q1]);
+    for (int i = tid; i < nee; i += NT) {
+        const int d = L[i];
+        su[i] = d >= 0 ? u[d] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int it = tid; it < 3 * Q1 * N1; it += NT) {
+        const int i = it % N1, qy = (it / N1) % Q1, c = it / (N1 * Q1);
+        double2 t1 = make_double2(0.0, 0.0), t2 = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            const double2 a = su[(j * N1 + i) * 3 + c];
+            const double vv = s1[j * Q1 + qy], dd = s1[(N1 + j) * Q1 + qy];
+            t1.x += vv * a.x; t1.y += vv * a.y;
+            t2.x += dd * a.x; t2.y += dd * a.y;
+        }
+        sT[((c * Q1 + qy) * N1 + i) * 2] = t1;
+        sT[((c * Q1 + qy) * N1 + i) * 2 + 1] = t2;
+    }
+    __syncthreads();
+    // rows of 16 lanes: row = eta point, lane = xi point (clamped copies
+    // where idle: they are never read or stored)
+    const int row = tid >> 4, x16 = tid & 15;
+    const int qe = min(row, Q1 - 1), qx = min(x16, Q1 - 1), ni = min(x16, N1 - 1);
+    Geo g;
+    if (geo_at(xq[qx], xq[qe], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
+    const double rx = g.dzdeta / g.det, sx = -g.dzdxi / g.det, rz = -g.dxdeta / g.det, sz = g.dxdxi / g.det;
+    const double w = wq[qx] * wq[qe] * g.det;
+    double2 U[3], G[3][3];
+    {
+        double vx[N1], dx[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) { vx[i] = s1[i * Q1 + qx]; dx[i] = s1[(N1 + i) * Q1 + qx]; }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double2 T1 = sT[((c * Q1 + qe) * N1 + ni) * 2], T2 = sT[((c * Q1 + qe) * N1 + ni) * 2 + 1];
+            double2 Uc = make_double2(0.0, 0.0), UXI = Uc, UET = Uc;
+            sf_row_sums<NL, 0>(Uc, UXI, UET, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+            G[c][0] = make_double2(rx * UXI.x + sx * UET.x, rx * UXI.y + sx * UET.y);
+            G[c][1] = make_double2(kv * Uc.y, -kv * Uc.x);   // d/dy = -i ky
+            G[c][2] = make_double2(rz * UXI.x + sz * UET.x, rz * UXI.y + sz * UET.y);
+            U[c] = Uc;
+        }
+    }
+    // weighted point values: divergence, symmetrised gradient S_ik = G_ik + G_ki
+    // (xx, yy, zz, xy, xz, yz) and -omega^2 u
+    const double2 dv = make_double2(w * (G[0][0].x + G[1][1].x + G[2][2].x), w * (G[0][0].y + G[1][1].y + G[2][2].y));
+    double2 Sw[6];
+    {
+        constexpr int si[6] = {0, 1, 2, 0, 0, 1}, sk[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+            Sw[j] = make_double2(w * (G[si[j]][sk[j]].x + G[sk[j]][si[j]].x), w * (G[si[j]][sk[j]].y + G[sk[j]][si[j]].y));
+    }
+    const double w2 = -omega * omega * w;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) U[i] = make_double2(w2 * U[i].x, w2 * U[i].y);
+    // output slot of this lane in stage 1: (cx, bi), lanes 2 N1.. clamped
+    const int slot = min(x16, 2 * N1 - 1), cx = slot / N1, bi = slot % N1;
+    double cv[Q1], cd[Q1];
+#pragma unroll
+    for (int q = 0; q < Q1; ++q) {
+        const double h = sh[cx * Q1 + q];
+        cv[q] = h * s1[bi * Q1 + q];
+        cd[q] = h * s1[(N1 + bi) * Q1 + q];
+    }
+    __syncthreads();   // su and sT are read: sP may overwrite them
+    const bool wr = row < Q1 && x16 < 2 * N1;
+    const auto seq = std::make_integer_sequence<int, Q1>();
+    auto put = [&](int plane, const double2 &v) {
+        if (wr) sP[((plane * 2 + cx) * Q1 + qe) * N1 + bi] = v;
+    };
+    {   // density l: x, y, z rows
+        double2 P = make_double2(0.0, 0.0), R = P;
+        sf_row_dot(P, make_double2(rx * dv.x, rx * dv.y), cd, seq);
+        sf_row_dot(R, make_double2(sx * dv.x, sx * dv.y), cv, seq);
+        put(0, P); put(9, R);
+        P = make_double2(0.0, 0.0);
+        sf_row_dot(P, make_double2(-kv * dv.y, kv * dv.x), cv, seq);
+        put(1, P);
+        P = make_double2(0.0, 0.0); R = P;
+        sf_row_dot(P, make_double2(rz * dv.x, rz * dv.y), cd, seq);
+        sf_row_dot(R, make_double2(sz * dv.x, sz * dv.y), cv, seq);
+        put(2, P); put(10, R);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {   // density m: rows of component i
+        constexpr int ix[3] = {0, 3, 4}, iy[3] = {3, 1, 5}, iz[3] = {4, 5, 2};   // S_ix, S_iy, S_iz in Sw
+        const double2 Sx = Sw[ix[i]], Sy = Sw[iy[i]], Sz = Sw[iz[i]];
+        double2 P = make_double2(0.0, 0.0), R = P;
+        sf_row_dot(P, make_double2(rx * Sx.x + rz * Sz.x, rx * Sx.y + rz * Sz.y), cd, seq);
+        sf_row_dot(P, make_double2(-kv * Sy.y, kv * Sy.x), cv, seq);
+        sf_row_dot(R, make_double2(sx * Sx.x + sz * Sz.x, sx * Sx.y + sz * Sz.y), cv, seq);
+        put(3 + i, P); put(11 + i, R);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {   // rho term
+        double2 P = make_double2(0.0, 0.0);
+        sf_row_dot(P, U[i], cv, seq);
+        put(6 + i, P);
+    }
+    __syncthreads();
+    // contraction over eta: one thread per (corner c, node b, component i)
+    for (int it = tid; it < 12 * nen; it += NT) {
+        const int i = it % 3, b = (it / 3) % nen, c = it / (3 * nen);
+        const int pi = b % N1, pj = b / N1, px = (c == 1 || c == 2) ? 1 : 0, pz = c >= 2 ? 1 : 0;
+        const int rl = i == 0 ? 9 : (i == 2 ? 10 : -1);
+        double2 l = make_double2(0.0, 0.0), m = l, rr = l;
+#pragma unroll 3
+        for (int q = 0; q < Q1; ++q) {
+            const double h = sh[pz * Q1 + q], fv = h * s1[pj * Q1 + q], fd = h * s1[(N1 + pj) * Q1 + q];
+            const double2 *P = sP + (px * Q1 + q) * N1 + pi;   // plane k at P[k * 2 Q1 N1]
+            constexpr int PS = 2 * Q1 * N1;
+            const double2 a = P[i * PS], b2 = P[(3 + i) * PS], c2 = P[(6 + i) * PS], d2 = P[(11 + i) * PS];
+            l.x += fv * a.x; l.y += fv * a.y;
+            if (rl >= 0) {
+                const double2 f = P[rl * PS];
+                l.x += fd * f.x; l.y += fd * f.y;
+            }
+            m.x += fv * b2.x + fd * d2.x; m.y += fv * b2.y + fd * d2.y;
+            rr.x += fv * c2.x; rr.y += fv * c2.y;
+        }
+        const int a = ie[c] - 1;
+        const double r = rho[a], vp = sqrt((lam[a] + 2.0 * mu[a]) / r), vs = sqrt(mu[a] / r);
+        const double cl0 = 2.0 * r * vp, cl1 = -4.0 * r * vs, cm1 = 2.0 * r * vs;
+        const double cl2 = vp * vp - 2.0 * vs * vs, cm2 = vs * vs;
+        const int row3 = b * 3 + i;
+        o[((size_t)c * 3 + 0) * nee + row3] = make_double2(cl0 * l.x, cl0 * l.y);
+        o[((size_t)c * 3 + 1) * nee + row3] = make_double2(cl1 * l.x + cm1 * m.x, cl1 * l.y + cm1 * m.y);
+        o[((size_t)c * 3 + 2) * nee + row3] =
+            make_double2(cl2 * l.x + cm2 * m.x + rr.x, cl2 * l.y + cm2 * m.y + rr.y);
+    }
+}
+
 // Pseudo-Hessian diagonal, anchor part: the column of anchor a is the sum of
 // its elements' vectors on shared DOFs.  pmap[(a*ncon + s)*nee + j] is the
 // patch position of row j of the s-th element of a (-1 fixed DOF); the
