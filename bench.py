@@ -242,11 +242,11 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     lam = torch.randn(npair, ndof, dtype=torch.complex128, device="cuda")
     gg = torch.zeros(mesh.nnpg, 3, dtype=torch.float64, device="cuda")
     k_ms, wall = timed(lambda: mi.gradient_dev(omega, ky, u, lam, gg), L, b"grad", 2)
-    # flops per (element, triple): sum factorisation for p <= 5 (two 1-D
-    # passes per field and component: 4 (12 N1^2 Q1 + 18 N1 Q1^2)), the
-    # direct contraction (72 NEN Q) for p6, plus ~150 per quadrature point
+    # flops per (element, triple): sum factorisation (two 1-D passes per
+    # field and component: 4 (12 N1^2 Q1 + 18 N1 Q1^2)) up to p6, the direct
+    # contraction (72 NEN Q) above, plus ~150 per quadrature point
     n1, q1 = mesh.nord + 1, 2 * mesh.nord + 5
-    per = 4.0 * (12 * n1 * n1 * q1 + 18 * n1 * q1 * q1) if q1 <= 16 else 72.0 * nen * nq
+    per = 4.0 * (12 * n1 * n1 * q1 + 18 * n1 * q1 * q1) if q1 <= 17 else 72.0 * nen * nq
     gfl = (per + 150.0 * nq) * mesh.nelem * npair
     kern["gradient"] = {"kernel_ms": round(k_ms, 3), "gradients_per_s": round(npair / (wall * 1e-3), 1),
                         "roofline": roof("fp64", gfl / (k_ms * 1e-3), "TFLOP/s")}
@@ -258,11 +258,11 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     mi.gn_diag(omega[:nt], ky[:nt], uh)
     wall = time.perf_counter() - t0
     k_ms = L.f25_last_kernel_ms(b"gn_diag")
-    # flops per (element, wavefield): sum factorisation for p <= 5 (fields
+    # flops per (element, wavefield): sum factorisation up to p6 (fields
     # and gradients at the points 24 N1^2 Q1 + 36 N1 Q1^2, 17 row sums per
     # row and output lane 136 N1 Q1^2, the eta contraction 224 N1^2 Q1, about
-    # 100 per point), the direct projection for p6
-    if q1 <= 16:
+    # 100 per point), the direct projection above
+    if q1 <= 17:
         dper = 248.0 * n1 * n1 * q1 + 172.0 * n1 * q1 * q1 + 100.0 * nq
     else:
         dper = (4.0 * 3 * 2 * 2 * 3 * nen + 30.0 * 3 * nen + 36.0 * 2 * nen) * nq

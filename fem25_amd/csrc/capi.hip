@@ -1046,8 +1046,7 @@ static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d
     constexpr int H = 2;   // pairs per thread per pass
     size_t lds = (size_t)2 * H * tg * 3 * nen * sizeof(double2) + (size_t)ntp * 12 * sizeof(double);
     HIPCHK(hipEventRecord(m->ev0, m->st));
-    // orders with at most 16 quadrature points per direction (p <= 5) run
-    // the sum-factorised kernel
+    // every order up to p6 runs the sum-factorised kernel
     typedef void (*GradSf)(const double *, const double *, const double *, const int *, const double *,
                            const double *, const char *, const int *, int, int64_t, const double *,
                            const double *, const double2 *, const double2 *, double *, int *);
@@ -1060,6 +1059,7 @@ static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d
         case 4: ksf = k_grad_sf<4, TS>; break;
         case 5: ksf = k_grad_sf<5, TS>; break;
         case 6: ksf = k_grad_sf<6, TS>; break;
+        case 7: ksf = k_grad_sf<7, TS>; break;
         default: break;
     }
     if (ksf) {
@@ -1069,9 +1069,10 @@ static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d
                            (size_t)2 * nl * q1 * sizeof(double);
         if (lsf > 64 * 1024)
             HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
-        hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(256), lsf, m->st, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,
-                           m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip, (int64_t)m->g.ndof, d_om, d_ky, u, l,
-                           m->d_part, m->d_err);
+        // Q1 = 17 (p6) adds a column of 17 lanes to the 17 rows: five waves
+        hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(q1 > 16 ? 320 : 256), lsf, m->st, m->d_t1d, m->d_wq,
+                           m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip, (int64_t)m->g.ndof,
+                           d_om, d_ky, u, l, m->d_part, m->d_err);
     } else {
         hipLaunchKernelGGL(k_grad<H>, dim3(m->d.nelem), dim3(ntp), lds, m->st, nen, m->tab.nq1, m->d.nelem, tg,
                            m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip,
@@ -2176,8 +2177,7 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
         g_err = "unsupported order";
         r = F25_ERR_ARG;
     }
-    // orders with at most 16 quadrature points per direction (p <= 5) run
-    // the sum-factorised element kernel
+    // the sum-factorised element kernel (every order up to p6)
     typedef void (*FcolsSf)(const double *, const double *, const double *, const int *, const double *,
                             const double *, const char *, const int *, const double *, const double *,
                             const double *, double, double, const double2 *, double2 *, int *);
@@ -2187,11 +2187,15 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
         case 4: ksf = k_fcols_sf<4>; break;
         case 5: ksf = k_fcols_sf<5>; break;
         case 6: ksf = k_fcols_sf<6>; break;
+        case 7: ksf = k_fcols_sf<7>; break;
         default: break;
     }
     const int nl = m->tab.nl, q1 = m->tab.nq1;
+    // Q1 = 17 (p6): five waves and the 17 point fields of the xi = 16 column
+    const int ntsf = q1 > 16 ? 320 : 256;
     const size_t lsf = (size_t)(2 * nl * q1 + 2 * q1) * sizeof(double) +
-                       (size_t)std::max(nee + 6 * q1 * nl, 28 * q1 * nl) * sizeof(double2);
+                       (size_t)std::max(nee + 6 * q1 * nl, 28 * q1 * nl) * sizeof(double2) +
+                       (q1 > 16 ? (size_t)17 * q1 * sizeof(double2) : 0);
     if (!r && ksf && lsf > 65536)
         HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
     const int nt1 = 256, qg = std::max(1, 256 / nee);
@@ -2206,7 +2210,7 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
     if (!r) (void)hipEventRecord(m->ev0, m->st);
     for (int t = 0; t < ntrip && !r; ++t) {
         if (ksf)
-            hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(256), lsf, m->st, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,
+            hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(ntsf), lsf, m->st, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,
                                m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho, m->d_lam, m->d_mu, omega[t], ky[t],
                                du + ndof * t, dcols, m->d_err);
         else

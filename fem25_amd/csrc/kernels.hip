@@ -1394,16 +1394,20 @@ __device__ __forceinline__ void sf_row_sums(double2 &U, double2 &UX, double2 &UE
 }
 
 template <int NL, int T>
-__global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_grad_sf(
+__global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) void k_grad_sf(
     const double *__restrict__ t1d, const double *__restrict__ wq, const double *__restrict__ xq,
     const int *__restrict__ ieng, const double *__restrict__ xl, const double *__restrict__ zl,
     const char *__restrict__ cdom, const int *__restrict__ lm, int ntrip, int64_t ndof,
     const double *__restrict__ omega, const double *__restrict__ ky, const double2 *__restrict__ u,
     const double2 *__restrict__ l, double *__restrict__ part, int *__restrict__ err) {
-    constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen, NT = 256;
+    constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen;
+    // Q1 = 17 (p6): rows 0..16 of 16 lanes hold xi points 0..15 and the 17
+    // lanes after them hold xi point 16 of eta rows 0..16 (five waves)
+    constexpr bool OVF = Q1 > 16;
+    constexpr int NT = OVF ? 320 : 256;
     constexpr int NW = T * 2 * nee;            // element values gathered per pass
     constexpr int NG = (NW + NT - 1) / NT;     // of them per thread
-    static_assert(Q1 <= 16, "one row of 16 lanes per eta point");
+    static_assert(Q1 <= 17, "one row of 16 lanes per eta point, plus one column");
     extern __shared__ double sm[];
     double2 *sw = reinterpret_cast<double2 *>(sm);                     // [T][2][nee]
     double2 *sT = sw + NW;                                             // [T][2][3][Q1][N1][2]
@@ -1449,8 +1453,10 @@ __global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_grad_sf(
     // second stage: row = eta point, lane = xi point (clamped when idle: an
     // idle lane computes a copy and contributes nothing)
     const int row = tid >> 4, x16 = tid & 15;
-    const bool act = row < Q1 && x16 < Q1;
-    const int qe = min(row, Q1 - 1), qx = min(x16, Q1 - 1), ni = min(x16, N1 - 1);
+    const bool ovl = OVF && tid >= 16 * Q1 && tid < 17 * Q1;   // a lane of the xi = 16 column
+    const bool act = (row < Q1 && x16 < Q1) || ovl;
+    const int qe = ovl ? tid - 16 * Q1 : min(row, Q1 - 1), qx = ovl ? 16 : min(x16, Q1 - 1);
+    const int ni = min(x16, N1 - 1);
     double vx[N1], dx[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) { vx[i] = t1d[i * Q1 + qx]; dx[i] = t1d[(N1 + i) * Q1 + qx]; }
@@ -1498,7 +1504,18 @@ __global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_grad_sf(
                 const double2 T1 = o[0], T2 = o[1];
                 double2 UXI = make_double2(0.0, 0.0), UET = UXI;
                 U = UXI;
-                sf_row_sums<NL, 0>(U, UXI, UET, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+                if (!ovl) {
+                    sf_row_sums<NL, 0>(U, UXI, UET, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+                } else {   // the xi = 16 column reads the row's T1, T2 from LDS
+                    const double2 *r = o - 2 * ni;
+#pragma unroll
+                    for (int j = 0; j < N1; ++j) {
+                        const double2 a1 = r[2 * j], a2 = r[2 * j + 1];
+                        U.x += vx[j] * a1.x; U.y += vx[j] * a1.y;
+                        UXI.x += dx[j] * a1.x; UXI.y += dx[j] * a1.y;
+                        UET.x += vx[j] * a2.x; UET.y += vx[j] * a2.y;
+                    }
+                }
                 Gc[0] = make_double2(rx * UXI.x + sx * UET.x, rx * UXI.y + sx * UET.y);
                 Gc[1] = make_double2(k * U.y, -k * U.x);   // d/dy = -i ky
                 Gc[2] = make_double2(rz * UXI.x + sz * UET.x, rz * UXI.y + sz * UET.y);
@@ -1798,7 +1815,7 @@ __device__ __forceinline__ void sf_row_dot(double2 &acc, const double2 &F, const
 // component) contracts over eta.  14 planes of P and R (9 P, 5 R) per
 // (cx, eta, bi).
 template <int NL>
-__global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_fcols_sf(const double *__restrict__ t1d, const double *__restrict__ wq,
+__global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) void k_fcols_sf(const double *__restrict__ t1d, const double *__restrict__ wq,
                                                   const double *__restrict__ xq, const int *__restrict__ ieng,
                                                   const double *__restrict__ xl, const double *__restrict__ zl,
                                                   const char *__restrict__ cdom, const int *__restrict__ lm,
@@ -1806,14 +1823,21 @@ __global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_fcols_sf(const doubl
                                                   const double *__restrict__ mu, double omega, double kv,
                                                   const double2 *__restrict__ u, double2 *__restrict__ out,
                                                   int *__restrict__ err) {
-    constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen, NT = 256;
-    static_assert(Q1 <= 16 && 2 * N1 <= 16, "one row of 16 lanes per eta point");
+    constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen;
+    // Q1 = 17 (p6): rows 0..16 of 16 lanes hold xi points 0..15 and the 17
+    // lanes after them xi point 16 of eta rows 0..16 (five waves); that
+    // column's 17 point fields go through LDS to the row sums
+    constexpr bool OVF = Q1 > 16;
+    constexpr int NT = OVF ? 320 : 256, QD = OVF ? 16 : Q1;   // QD: xi points summed by DPP
+    constexpr int REG = (nee + 6 * Q1 * N1 > 28 * Q1 * N1) ? nee + 6 * Q1 * N1 : 28 * Q1 * N1;
+    static_assert(Q1 <= 17 && 2 * N1 <= 16, "one row of 16 lanes per eta point, plus one column");
     extern __shared__ double sm[];
     double *s1 = sm;                                  // [2][N1][Q1] v_j(s_q), v'_j(s_q); [2][Q1] h_0, h_1
     double *sh = s1 + 2 * N1 * Q1;
     double2 *su = reinterpret_cast<double2 *>(sh + 2 * Q1);   // nee element values
     double2 *sT = su + nee;                                   // [3][Q1][N1][2] T1, T2
     double2 *sP = su;   // [14][2][Q1][N1], written once su and sT are read
+    double2 *sF = su + REG;   // [17][Q1] point fields of the xi = 16 column (Q1 = 17)
     const int e = blockIdx.x, tid = threadIdx.x;
     double2 *o = out + (size_t)e * 12 * nee;
     if (cdom[e] == 'A') {   // PML elements are outside the non-PML gradient path
@@ -1846,7 +1870,9 @@ __global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_fcols_sf(const doubl
     // rows of 16 lanes: row = eta point, lane = xi point (clamped copies
     // where idle: they are never read or stored)
     const int row = tid >> 4, x16 = tid & 15;
-    const int qe = min(row, Q1 - 1), qx = min(x16, Q1 - 1), ni = min(x16, N1 - 1);
+    const bool ovl = OVF && tid >= 16 * Q1 && tid < 17 * Q1;   // a lane of the xi = 16 column
+    const int qe = ovl ? tid - 16 * Q1 : min(row, Q1 - 1), qx = ovl ? 16 : min(x16, Q1 - 1);
+    const int ni = min(x16, N1 - 1);
     Geo g;
     if (geo_at(xq[qx], xq[qe], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
     const double rx = g.dzdeta / g.det, sx = -g.dzdxi / g.det, rz = -g.dxdeta / g.det, sz = g.dxdxi / g.det;
@@ -1860,7 +1886,18 @@ __global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_fcols_sf(const doubl
         for (int c = 0; c < 3; ++c) {
             const double2 T1 = sT[((c * Q1 + qe) * N1 + ni) * 2], T2 = sT[((c * Q1 + qe) * N1 + ni) * 2 + 1];
             double2 Uc = make_double2(0.0, 0.0), UXI = Uc, UET = Uc;
-            sf_row_sums<NL, 0>(Uc, UXI, UET, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+            if (!ovl) {
+                sf_row_sums<NL, 0>(Uc, UXI, UET, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+            } else {   // the xi = 16 column reads the row's T1, T2 from LDS
+                const double2 *r = sT + (c * Q1 + qe) * N1 * 2;
+#pragma unroll
+                for (int j = 0; j < N1; ++j) {
+                    const double2 a1 = r[2 * j], a2 = r[2 * j + 1];
+                    Uc.x += vx[j] * a1.x; Uc.y += vx[j] * a1.y;
+                    UXI.x += dx[j] * a1.x; UXI.y += dx[j] * a1.y;
+                    UET.x += vx[j] * a2.x; UET.y += vx[j] * a2.y;
+                }
+            }
             G[c][0] = make_double2(rx * UXI.x + sx * UET.x, rx * UXI.y + sx * UET.y);
             G[c][1] = make_double2(kv * Uc.y, -kv * Uc.x);   // d/dy = -i ky
             G[c][2] = make_double2(rz * UXI.x + sz * UET.x, rz * UXI.y + sz * UET.y);
@@ -1880,6 +1917,21 @@ __global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_fcols_sf(const doubl
     const double w2 = -omega * omega * w;
 #pragma unroll
     for (int i = 0; i < 3; ++i) U[i] = make_double2(w2 * U[i].x, w2 * U[i].y);
+    if (OVF && ovl) {   // the column's point fields, in the order of the row sums below
+        const double2 F[17] = {
+            make_double2(rx * dv.x, rx * dv.y), make_double2(sx * dv.x, sx * dv.y),
+            make_double2(-kv * dv.y, kv * dv.x), make_double2(rz * dv.x, rz * dv.y),
+            make_double2(sz * dv.x, sz * dv.y),
+            make_double2(rx * Sw[0].x + rz * Sw[4].x, rx * Sw[0].y + rz * Sw[4].y), make_double2(-kv * Sw[3].y, kv * Sw[3].x),
+            make_double2(sx * Sw[0].x + sz * Sw[4].x, sx * Sw[0].y + sz * Sw[4].y),
+            make_double2(rx * Sw[3].x + rz * Sw[5].x, rx * Sw[3].y + rz * Sw[5].y), make_double2(-kv * Sw[1].y, kv * Sw[1].x),
+            make_double2(sx * Sw[3].x + sz * Sw[5].x, sx * Sw[3].y + sz * Sw[5].y),
+            make_double2(rx * Sw[4].x + rz * Sw[2].x, rx * Sw[4].y + rz * Sw[2].y), make_double2(-kv * Sw[5].y, kv * Sw[5].x),
+            make_double2(sx * Sw[4].x + sz * Sw[2].x, sx * Sw[4].y + sz * Sw[2].y),
+            U[0], U[1], U[2]};
+#pragma unroll
+        for (int f = 0; f < 17; ++f) sF[f * Q1 + qe] = F[f];
+    }
     // output slot of this lane in stage 1: (cx, bi), lanes 2 N1.. clamped
     const int slot = min(x16, 2 * N1 - 1), cx = slot / N1, bi = slot % N1;
     double cv[Q1], cd[Q1];
@@ -1891,21 +1943,30 @@ __global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_fcols_sf(const doubl
     }
     __syncthreads();   // su and sT are read: sP may overwrite them
     const bool wr = row < Q1 && x16 < 2 * N1;
-    const auto seq = std::make_integer_sequence<int, Q1>();
+    const auto seq = std::make_integer_sequence<int, QD>();
+    // row sum of point field number f over the xi points of this row
+    auto sf_row_dot = [&](double2 &acc, const double2 &F, const double *c, int f) {
+        f25k::sf_row_dot(acc, F, c, seq);
+        if constexpr (OVF) {
+            const double2 v = sF[f * Q1 + qe];
+            acc.x += c[QD] * v.x;
+            acc.y += c[QD] * v.y;
+        }
+    };
     auto put = [&](int plane, const double2 &v) {
         if (wr) sP[((plane * 2 + cx) * Q1 + qe) * N1 + bi] = v;
     };
     {   // density l: x, y, z rows
         double2 P = make_double2(0.0, 0.0), R = P;
-        sf_row_dot(P, make_double2(rx * dv.x, rx * dv.y), cd, seq);
-        sf_row_dot(R, make_double2(sx * dv.x, sx * dv.y), cv, seq);
+        sf_row_dot(P, make_double2(rx * dv.x, rx * dv.y), cd, 0);
+        sf_row_dot(R, make_double2(sx * dv.x, sx * dv.y), cv, 1);
         put(0, P); put(9, R);
         P = make_double2(0.0, 0.0);
-        sf_row_dot(P, make_double2(-kv * dv.y, kv * dv.x), cv, seq);
+        sf_row_dot(P, make_double2(-kv * dv.y, kv * dv.x), cv, 2);
         put(1, P);
         P = make_double2(0.0, 0.0); R = P;
-        sf_row_dot(P, make_double2(rz * dv.x, rz * dv.y), cd, seq);
-        sf_row_dot(R, make_double2(sz * dv.x, sz * dv.y), cv, seq);
+        sf_row_dot(P, make_double2(rz * dv.x, rz * dv.y), cd, 3);
+        sf_row_dot(R, make_double2(sz * dv.x, sz * dv.y), cv, 4);
         put(2, P); put(10, R);
     }
 #pragma unroll
@@ -1913,15 +1974,15 @@ __global__ __launch_bounds__(256, (NL <= 5 ? 4 : 3)) void k_fcols_sf(const doubl
         constexpr int ix[3] = {0, 3, 4}, iy[3] = {3, 1, 5}, iz[3] = {4, 5, 2};   // S_ix, S_iy, S_iz in Sw
         const double2 Sx = Sw[ix[i]], Sy = Sw[iy[i]], Sz = Sw[iz[i]];
         double2 P = make_double2(0.0, 0.0), R = P;
-        sf_row_dot(P, make_double2(rx * Sx.x + rz * Sz.x, rx * Sx.y + rz * Sz.y), cd, seq);
-        sf_row_dot(P, make_double2(-kv * Sy.y, kv * Sy.x), cv, seq);
-        sf_row_dot(R, make_double2(sx * Sx.x + sz * Sz.x, sx * Sx.y + sz * Sz.y), cv, seq);
+        sf_row_dot(P, make_double2(rx * Sx.x + rz * Sz.x, rx * Sx.y + rz * Sz.y), cd, 5 + 3 * i);
+        sf_row_dot(P, make_double2(-kv * Sy.y, kv * Sy.x), cv, 6 + 3 * i);
+        sf_row_dot(R, make_double2(sx * Sx.x + sz * Sz.x, sx * Sx.y + sz * Sz.y), cv, 7 + 3 * i);
         put(3 + i, P); put(11 + i, R);
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {   // rho term
         double2 P = make_double2(0.0, 0.0);
-        sf_row_dot(P, U[i], cv, seq);
+        sf_row_dot(P, U[i], cv, 14 + i);
         put(6 + i, P);
     }
     __syncthreads();

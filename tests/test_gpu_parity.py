@@ -236,8 +236,8 @@ def test_apply_quadratic_in_ky():
 @pytest.mark.parametrize("name,nf,nk", [("tiny_layered", 1, 2), ("pml", 1, 2), ("tiny_layered", 2, 3),
                                          ("p5", 2, 3), ("p6", 1, 2)])
 def test_gradient_exact(name, nf, nk):
-    """p4, p5: the sum-factorised kernel (with 2 and 6 triples: a partial
-    and a full pass of 4); p6: the direct kernel."""
+    """The sum-factorised kernel (with 2 and 6 triples: one and three
+    passes of 2); p6 adds a column of lanes for the 17th xi point."""
     mi = _mi(name)
     m, mod, pml = mesh_case(name)
     g = oracle_graph(name)
@@ -319,8 +319,9 @@ def test_gn_diag_matches_operator_columns(name, stride):
     """Pseudo-Hessian diagonal sum_t |dS_t/dm_{a,p} u_t|^2 against columns
     formed by central differences of the oracle's FP64 operator.  S is
     quadratic in Vp alone and in Vs alone and affine in rho, so a central
-    difference is exact up to rounding (h = 1 m/s, 1 kg/m^3).  p2 to p5
-    run the sum-factorised element kernel, p6 the direct one.  PML elements
+    difference is exact up to rounding (h = 1 m/s, 1 kg/m^3).  Every order
+    runs the sum-factorised element kernel (p6 with a column of lanes for the
+    17th xi point).  PML elements
     are outside the path, so on the PML meshes every stride-th anchor that
     touches no PML element is checked."""
     mi = _mi(name)
