@@ -242,11 +242,10 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     lam = torch.randn(npair, ndof, dtype=torch.complex128, device="cuda")
     gg = torch.zeros(mesh.nnpg, 3, dtype=torch.float64, device="cuda")
     k_ms, wall = timed(lambda: mi.gradient_dev(omega, ky, u, lam, gg), L, b"grad", 2)
-    # flops per (element, triple): sum factorisation (two 1-D passes per
-    # field and component: 4 (12 N1^2 Q1 + 18 N1 Q1^2)) up to p6, the direct
-    # contraction (72 NEN Q) above, plus ~150 per quadrature point
+    # flops per (element, triple), sum factorised: two 1-D passes per field
+    # and component, 4 (12 N1^2 Q1 + 18 N1 Q1^2), plus ~150 per quadrature point
     n1, q1 = mesh.nord + 1, 2 * mesh.nord + 5
-    per = 4.0 * (12 * n1 * n1 * q1 + 18 * n1 * q1 * q1) if q1 <= 17 else 72.0 * nen * nq
+    per = 4.0 * (12 * n1 * n1 * q1 + 18 * n1 * q1 * q1)
     gfl = (per + 150.0 * nq) * mesh.nelem * npair
     kern["gradient"] = {"kernel_ms": round(k_ms, 3), "gradients_per_s": round(npair / (wall * 1e-3), 1),
                         "roofline": roof("fp64", gfl / (k_ms * 1e-3), "TFLOP/s")}
@@ -258,14 +257,10 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     mi.gn_diag(omega[:nt], ky[:nt], uh)
     wall = time.perf_counter() - t0
     k_ms = L.f25_last_kernel_ms(b"gn_diag")
-    # flops per (element, wavefield): sum factorisation up to p6 (fields
-    # and gradients at the points 24 N1^2 Q1 + 36 N1 Q1^2, 17 row sums per
-    # row and output lane 136 N1 Q1^2, the eta contraction 224 N1^2 Q1, about
-    # 100 per point), the direct projection above
-    if q1 <= 17:
-        dper = 248.0 * n1 * n1 * q1 + 172.0 * n1 * q1 * q1 + 100.0 * nq
-    else:
-        dper = (4.0 * 3 * 2 * 2 * 3 * nen + 30.0 * 3 * nen + 36.0 * 2 * nen) * nq
+    # flops per (element, wavefield), sum factorised: fields and gradients
+    # at the points 24 N1^2 Q1 + 36 N1 Q1^2, 17 row sums per row and output
+    # lane 136 N1 Q1^2, the eta contraction 224 N1^2 Q1, about 100 per point
+    dper = 248.0 * n1 * n1 * q1 + 172.0 * n1 * q1 * q1 + 100.0 * nq
     dfl = dper * mesh.nelem * nt
     kern["gn_diag"] = {"kernel_ms": round(k_ms, 3), "wavefields_per_s": round(nt / (k_ms * 1e-3), 1),
                        "roofline": roof("fp64", dfl / (k_ms * 1e-3), "TFLOP/s")}
@@ -276,11 +271,10 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     t0 = time.perf_counter()
     fm, _ = cfmat_host(mi, inv, float(freq[0]), float(py[0]), wave)
     k_ms = L.f25_last_kernel_ms(b"cfmat")
-    # one element's DKM_VPVS product per column; per (item, vector): sum
-    # factorised for p <= 5, 4 (15 N1^2 Q1 + 18 N1 Q1^2) + 84 Q, else the
-    # direct (72 NEN + 40) Q
+    # one element's DKM_VPVS product per column; per (item, vector), sum
+    # factorised: 4 (15 N1^2 Q1 + 18 N1 Q1^2) + 84 Q
     n1, q1 = mesh.nord + 1, 2 * mesh.nord + 5
-    per_item = (4.0 * (15 * n1 * n1 * q1 + 18 * n1 * q1 * q1) + 84.0 * nq) if q1 <= 16 else (72.0 * nen + 40.0) * nq
+    per_item = 4.0 * (15 * n1 * n1 * q1 + 18 * n1 * q1 * q1) + 84.0 * nq
     cfl = per_item * inv.nnpgl
     kern["cfmat_dist"] = {"kernel_ms": round(k_ms, 3), "columns_per_s": round(inv.nnpgl / (k_ms * 1e-3), 1),
                           "roofline": roof("fp64", cfl / (k_ms * 1e-3), "TFLOP/s")}

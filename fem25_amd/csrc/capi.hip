@@ -97,7 +97,6 @@ struct f25_mesh_s {
     double *d_xl = nullptr, *d_zl = nullptr, *d_xd = nullptr, *d_zd = nullptr;
     double *d_shl = nullptr, *d_wq = nullptr, *d_xq = nullptr;
     double *d_t1d = nullptr;    // 1-D tables: val1 (nl*nq1) then der1
-    double *d_shlT = nullptr;   // shape tables as [a][c][q] (q fastest)
     char *d_cdom = nullptr, *d_cnp = nullptr;
     double *d_rho = nullptr, *d_lam = nullptr, *d_mu = nullptr;
     double *d_blk = nullptr;
@@ -178,14 +177,6 @@ static int upload_mesh(f25_mesh *m) {
     TRY(upload(&m->d_zd, m->zd.data(), m->zd.size(), s));
     TRY(upload(&m->d_cdom, m->cdom.data(), m->cdom.size(), s));
     TRY(upload(&m->d_shl, m->tab.shl.data(), m->tab.shl.size(), s));
-    {
-        const int nq = m->tab.nq;
-        std::vector<double> t(m->tab.shl.size());
-        for (int q = 0; q < nq; ++q)
-            for (int a = 0; a < nen; ++a)
-                for (int cc = 0; cc < 3; ++cc) t[(size_t)(a * 3 + cc) * nq + q] = m->tab.shl[((size_t)q * nen + a) * 3 + cc];
-        TRY(upload(&m->d_shlT, t.data(), t.size(), s));
-    }
     TRY(upload(&m->d_wq, m->tab.wq.data(), m->tab.wq.size(), s));
     TRY(upload(&m->d_xq, m->tab.xq.data(), m->tab.xq.size(), s));
     {
@@ -392,7 +383,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     (void)hipSetDevice(m->dev);
     if (m->st) (void)hipStreamSynchronize(m->st);
     dfree(m->d_ieng); dfree(m->d_lm); dfree(m->d_first); dfree(m->d_order); dfree(m->d_pml_order); dfree(m->d_err);
-    dfree(m->d_xl); dfree(m->d_zl); dfree(m->d_xd); dfree(m->d_zd); dfree(m->d_shl); dfree(m->d_shlT); dfree(m->d_wq); dfree(m->d_t1d);
+    dfree(m->d_xl); dfree(m->d_zl); dfree(m->d_xd); dfree(m->d_zd); dfree(m->d_shl); dfree(m->d_wq); dfree(m->d_t1d);
     dfree(m->d_xq); dfree(m->d_cdom); dfree(m->d_cnp); dfree(m->d_rho); dfree(m->d_lam); dfree(m->d_mu);
     if (m->ag.exec) (void)hipGraphExecDestroy(m->ag.exec);
     dfree(m->d_vcf); dfree(m->d_dvcf); dfree(m->d_vpad);
@@ -1037,15 +1028,7 @@ extern "C" int f25_apply_batch(f25_mesh *m, int npair, const double *omega, cons
 // ---------------------------------------------------------------------------
 static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d_ky, const double2 *u,
                      const double2 *l, double *g, int chain) {
-    const int nen = m->tab.nen, nq = m->tab.nq;
-    // trip groups per element: as many quadrature-point sets as fit 512 threads
-    const int tg = std::max(1, 512 / nq);
-    const int nt = ((nq * tg + 63) / 64) * 64;   // power of two not required below
-    int ntp = 64;
-    while (ntp < nt) ntp *= 2;                   // the tree reduction wants a power of two
-    constexpr int H = 2;   // pairs per thread per pass
-    size_t lds = (size_t)2 * H * tg * 3 * nen * sizeof(double2) + (size_t)ntp * 12 * sizeof(double);
-    HIPCHK(hipEventRecord(m->ev0, m->st));
+    const int nen = m->tab.nen;
     // every order up to p6 runs the sum-factorised kernel
     typedef void (*GradSf)(const double *, const double *, const double *, const int *, const double *,
                            const double *, const char *, const int *, int, int64_t, const double *,
@@ -1062,21 +1045,19 @@ static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d
         case 7: ksf = k_grad_sf<7, TS>; break;
         default: break;
     }
-    if (ksf) {
+    if (!ksf) FAIL(F25_ERR_ARG, "unsupported order");
+    {
         const int nl = m->tab.nl, q1 = m->tab.nq1;
         const size_t lsf = (size_t)TS * 2 * 3 * nen * sizeof(double2) +
                            (size_t)TS * 2 * 3 * q1 * nl * 2 * sizeof(double2) + (size_t)q1 * q1 * 4 * sizeof(double) +
                            (size_t)2 * nl * q1 * sizeof(double);
         if (lsf > 64 * 1024)
             HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
+        HIPCHK(hipEventRecord(m->ev0, m->st));
         // Q1 = 17 (p6) adds a column of 17 lanes to the 17 rows: five waves
         hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(q1 > 16 ? 320 : 256), lsf, m->st, m->d_t1d, m->d_wq,
                            m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip, (int64_t)m->g.ndof,
                            d_om, d_ky, u, l, m->d_part, m->d_err);
-    } else {
-        hipLaunchKernelGGL(k_grad<H>, dim3(m->d.nelem), dim3(ntp), lds, m->st, nen, m->tab.nq1, m->d.nelem, tg,
-                           m->d_shlT, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip,
-                           (int64_t)m->g.ndof, d_om, d_ky, u, l, m->d_part, m->d_err);
     }
     HIPCHK(hipEventRecord(m->ev1, m->st));
     hipLaunchKernelGGL(k_grad_gather, dim3((m->d.nnpg + 255) / 256), dim3(256), 0, m->st, m->d.nnpg,
@@ -1468,25 +1449,18 @@ static int ensure_dab_part(f25_mesh *m) {
     return 0;
 }
 
-// LDS of k_dab for nvec vectors: the element's vectors (room for two), the
-// geometry per quadrature point, and the fluxes (8 complex per vector and
-// point) whose space the projection's partial sums reuse
-// (orders with at most 16 quadrature points per direction run k_dab_sf,
-// whose LDS does not depend on nvec: vectors, geometry, T1/T2, the
-// projection fields A, B, C and their first-pass sums)
-static bool dab_sf(const f25_mesh *m) { return m->tab.nq1 <= 16; }
-static size_t dab_lds(const f25_mesh *m, int nvec = 2) {
-    const int nee = 3 * m->tab.nen, nq = m->tab.nq, qg = std::max(1, 256 / nee);
-    if (dab_sf(m)) {
-        const int q1 = m->tab.nq1, n1 = m->tab.nl;
-        return (size_t)2 * nee * sizeof(double2) + 5 * (size_t)nq * sizeof(double) +
-               ((size_t)3 * q1 * n1 * 2 + 9 * (size_t)nq + 3 * (size_t)q1 * n1 * 3) * sizeof(double2);
-    }
-    return (size_t)2 * nee * sizeof(double2) + 4 * (size_t)nq * sizeof(double) +
-           (size_t)nvec * std::max(8 * nq, qg * nee) * sizeof(double2);
+// LDS of k_dab_sf (independent of the number of vectors): the element's
+// vectors (room for two), the geometry per quadrature point, T1/T2, the
+// projection fields A, B, C and their first-pass sums
+static size_t dab_lds(const f25_mesh *m) {
+    const int nee = 3 * m->tab.nen, nq = m->tab.nq, q1 = m->tab.nq1, n1 = m->tab.nl;
+    return (size_t)2 * nee * sizeof(double2) + 5 * (size_t)nq * sizeof(double) +
+           ((size_t)3 * q1 * n1 * 2 + 9 * (size_t)nq + 3 * (size_t)q1 * n1 * 3) * sizeof(double2);
 }
+// threads of a k_dab_sf workgroup: five waves at p6 (Q1 = 17)
+static int dab_threads(const f25_mesh *m) { return m->tab.nq1 > 16 ? 320 : 256; }
 
-// k_dab for the mesh's order (the element loops are compiled per order)
+// k_dab_sf for the mesh's order (the element loops are compiled per order)
 typedef void (*DabKernel)(int, int, int, double, double, double, const int *, const double *, const double *,
                           const double *, const double *, const int *, const double *, const double *, const int *,
                           const float2 *, const float2 *, double2 *, double2 *, const float2 *, const float2 *,
@@ -1498,12 +1472,10 @@ static DabKernel dab_kernel_t(int nl) {
         case 4: return k_dab_sf<4, IMAG>;
         case 5: return k_dab_sf<5, IMAG>;
         case 6: return k_dab_sf<6, IMAG>;
-        case 7: return k_dab<7, IMAG>;
+        case 7: return k_dab_sf<7, IMAG>;
         default: return nullptr;
     }
 }
-// shape tables of the kernel: the 1-D tables (k_dab_sf) or shlT (k_dab)
-static const double *dab_shape(const f25_mesh *m) { return dab_sf(m) ? m->d_t1d : m->d_shlT; }
 // DABMKISOB25 form (imaginary ky coupling) or DKM_VPVS form (real)
 static DabKernel dab_kernel(int nl, bool imag = true) { return imag ? dab_kernel_t<true>(nl) : dab_kernel_t<false>(nl); }
 
@@ -1547,8 +1519,8 @@ static int dab_launch_columns(f25_mesh *m, double kv, int nvec, const std::vecto
     int r = upload(&de, elem.data(), elem.size(), m->st);
     if (!r) r = upload(&dc, coef.data(), coef.size(), m->st);
     if (!r) {
-        hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(inv.nnpgl), dim3(256), dab_lds(m, nvec), m->st, 1,
-                           inv.nnpgl, nvec, inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de, dc, dab_shape(m), m->d_wq, m->d_xq, m->d_ieng,
+        hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(inv.nnpgl), dim3(dab_threads(m)), dab_lds(m), m->st, 1,
+                           inv.nnpgl, nvec, inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de, dc, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,
                            m->d_xl, m->d_zl, m->d_lm, x0, x1, (double2 *)nullptr, (double2 *)nullptr, z0, z1,
                            m->d_mygrad, m->d_jcsc, m->d_icsc, p0, p1, (float2 *)nullptr, m->d_err);
         hipError_t e = hipGetLastError();
@@ -1596,8 +1568,9 @@ extern "C" int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *w
     if (!r) r = upload(&dc, coef.data(), coef.size(), m->st);
     if (!r) {
         (void)hipEventRecord(m->ev0, m->st);
-        hipLaunchKernelGGL(dab_kernel(m->tab.nl, false), dim3(n), dim3(256), dab_lds(m, 1), m->st, 2, n, 1, kp, ks, kv,
-                           de, dc, dab_shape(m), m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, dw,
+        hipLaunchKernelGGL(dab_kernel(m->tab.nl, false), dim3(n), dim3(dab_threads(m)), dab_lds(m), m->st, 2, n, 1,
+                           kp, ks, kv,
+                           de, dc, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, dw,
                            (const float2 *)nullptr, (double2 *)nullptr, (double2 *)nullptr, (const float2 *)nullptr,
                            (const float2 *)nullptr, m->d_mygrad, m->d_jcsc, m->d_icsc, (float *)nullptr,
                            (float *)nullptr, df, m->d_err);
@@ -1660,9 +1633,9 @@ extern "C" int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float 
         for (size_t col = 0; col + 1 < inv.dab_cptr.size(); ++col) {
             const int o = inv.dab_cptr[col], n = inv.dab_cptr[col + 1] - o;
             if (n <= 0) continue;
-            hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(n), dim3(256), dab_lds(m, 2), m->st, 0, n, 2,
+            hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(n), dim3(dab_threads(m)), dab_lds(m), m->st, 0, n, 2,
                                inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de + o, dc + 4 * (size_t)o,
-                               dab_shape(m), m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, du, dl, ya, yb,
+                               m->d_t1d, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, du, dl, ya, yb,
                                (const float2 *)nullptr, (const float2 *)nullptr, m->d_mygrad, m->d_jcsc,
                                m->d_icsc, (float *)nullptr, (float *)nullptr, (float2 *)nullptr, m->d_err);
         }
@@ -2161,22 +2134,6 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
         hipError_t e = hipMemcpyAsync(du, u, ndof * ntrip * 16, hipMemcpyHostToDevice, m->st);
         if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
     }
-    typedef void (*FcolsKernel)(const double *, const double *, const double *, const double *, const int *,
-                                const double *, const double *, const char *, const int *, const double *,
-                                const double *, const double *, double, double, const double2 *, double2 *, int *);
-    FcolsKernel kf = nullptr;
-    switch (m->tab.nl) {
-        case 3: kf = k_fcols<3>; break;
-        case 4: kf = k_fcols<4>; break;
-        case 5: kf = k_fcols<5>; break;
-        case 6: kf = k_fcols<6>; break;
-        case 7: kf = k_fcols<7>; break;
-        default: break;
-    }
-    if (!kf && !r) {
-        g_err = "unsupported order";
-        r = F25_ERR_ARG;
-    }
     // the sum-factorised element kernel (every order up to p6)
     typedef void (*FcolsSf)(const double *, const double *, const double *, const int *, const double *,
                             const double *, const char *, const int *, const double *, const double *,
@@ -2190,33 +2147,30 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
         case 7: ksf = k_fcols_sf<7>; break;
         default: break;
     }
+    if (!ksf && !r) {
+        g_err = "unsupported order";
+        r = F25_ERR_ARG;
+    }
     const int nl = m->tab.nl, q1 = m->tab.nq1;
     // Q1 = 17 (p6): five waves and the 17 point fields of the xi = 16 column
     const int ntsf = q1 > 16 ? 320 : 256;
     const size_t lsf = (size_t)(2 * nl * q1 + 2 * q1) * sizeof(double) +
                        (size_t)std::max(nee + 6 * q1 * nl, 28 * q1 * nl) * sizeof(double2) +
                        (q1 > 16 ? (size_t)17 * q1 * sizeof(double2) : 0);
-    if (!r && ksf && lsf > 65536)
-        HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
-    const int nt1 = 256, qg = std::max(1, 256 / nee);
-    const size_t lds1 = nee * sizeof(double2) + std::max((size_t)nq * 28 * sizeof(double),
-                                                         (size_t)(qg - 1) * 12 * nee * sizeof(double2));
     const int nt2 = 256;
     const size_t lds2 = (size_t)m->ncon_anchor * nee * 3 * sizeof(double2) + nt2 * 3 * sizeof(double);
-    if (!r && lds1 > 65536)
-        HIPCHK(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
-    if (!r && lds2 > 65536)
-        HIPCHK(hipFuncSetAttribute((const void *)k_fdiag, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+    for (int k = 0; k < 2 && !r; ++k) {
+        const size_t l = k ? lds2 : lsf;
+        if (l <= 65536) continue;
+        hipError_t e = hipFuncSetAttribute(k ? (const void *)k_fdiag : (const void *)ksf,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)l);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
     if (!r) (void)hipEventRecord(m->ev0, m->st);
     for (int t = 0; t < ntrip && !r; ++t) {
-        if (ksf)
-            hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(ntsf), lsf, m->st, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,
-                               m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho, m->d_lam, m->d_mu, omega[t], ky[t],
-                               du + ndof * t, dcols, m->d_err);
-        else
-            hipLaunchKernelGGL(kf, dim3(m->d.nelem), dim3(nt1), lds1, m->st, m->d_shl, m->d_shlT, m->d_wq,
-                               m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho, m->d_lam,
-                               m->d_mu, omega[t], ky[t], du + ndof * t, dcols, m->d_err);
+        hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(ntsf), lsf, m->st, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,
+                           m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho, m->d_lam, m->d_mu, omega[t], ky[t],
+                           du + ndof * t, dcols, m->d_err);
         hipLaunchKernelGGL(k_fdiag, dim3(nnpg), dim3(nt2), lds2, m->st, nnpg, m->ncon_anchor, nee,
                            m->d_mconn, m->d_ieng, m->d_pmap, dcols, dd);
         hipError_t e = hipGetLastError();

@@ -1251,122 +1251,9 @@ __global__ void k_peff(int ndofl, const int *__restrict__ mydofs, const int64_t 
 //   d_rho    = -omega^2 Re[conj(l) . u]
 // accumulated over the wavefield pairs at each quadrature point, then
 // projected on the element's four anchor hat functions.
-// One workgroup per element.  Thread i handles quadrature point i % nq for
-// trip group i / nq (tg groups); the wavefield pairs are staged in LDS
-// tc = H tg at a time and every thread accumulates H of them per pass, so
-// the shape-function values and physical derivatives at (q, a) are formed
-// once per H pairs.  The geometry at q is computed once per element.
-template <int H>
-__global__ __launch_bounds__(512) void k_grad(int nen, int nq1, int nelem, int tg, const double *__restrict__ shlT,
-                       const double *__restrict__ wq, const double *__restrict__ xq,
-                       const int *__restrict__ ieng, const double *__restrict__ xl,
-                       const double *__restrict__ zl, const char *__restrict__ cdom,
-                       const int *__restrict__ lm, int ntrip, int64_t ndof,
-                       const double *__restrict__ omega, const double *__restrict__ ky,
-                       const double2 *__restrict__ u, const double2 *__restrict__ l,
-                       double *__restrict__ part, int *__restrict__ err) {
-    extern __shared__ double sm[];
-    const int e = blockIdx.x;
-    const int nq = nq1 * nq1, nee = 3 * nen, tc = H * tg;
-    double2 *su = reinterpret_cast<double2 *>(sm);   // [tc][nee]
-    double2 *sl = su + (size_t)tc * nee;               // [tc][nee]
-    double *sred = reinterpret_cast<double *>(sl + (size_t)tc * nee);   // blockDim*12
-    const int *ie = ieng + 4 * e;
-    const int *L = lm + (size_t)e * nee;
-    const bool skip = cdom[e] == 'A';
-    const int q = threadIdx.x % nq, grp = threadIdx.x / nq;
-    const bool act = !skip && grp < tg;
-    const int ix = q % nq1, iz = q / nq1;
-    Geo g;
-    double rx = 0, sx = 0, rz = 0, sz = 0;
-    if (act) {
-        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
-        rx = g.dzdeta / g.det; sx = -g.dzdxi / g.det;
-        rz = -g.dxdeta / g.det; sz = g.dxdxi / g.det;
-    }
-    double Dl = 0.0, Dm = 0.0, Dr = 0.0;
-    for (int t0 = 0; t0 < ntrip && !skip; t0 += tc) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < tc * nee; i += blockDim.x) {
-            const int tt = i / nee, j = i % nee, t = t0 + tt;
-            const int d = L[j];
-            const bool ok = d >= 0 && t < ntrip;
-            su[i] = ok ? u[(size_t)t * ndof + d] : make_double2(0.0, 0.0);
-            sl[i] = ok ? l[(size_t)t * ndof + d] : make_double2(0.0, 0.0);
-        }
-        __syncthreads();
-        if (!act) continue;
-        const int ta = H * grp;   // this thread's pairs in the chunk: ta .. ta + H - 1
-        double2 U[H][3] = {}, UX[H][3] = {}, UZ[H][3] = {}, V[H][3] = {}, VX[H][3] = {}, VZ[H][3] = {};
-        const double2 *u0 = su + (size_t)ta * nee, *l0 = sl + (size_t)ta * nee;
-        for (int a = 0; a < nen; ++a) {
-            // shlT[(a*3 + c)*nq + q]: adjacent lanes (adjacent q) read adjacent words
-            const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
-            const double n = shlT[(size_t)(a * 3 + 2) * nq + q];
-            const double nx = s0 * rx + s1 * sx, nz = s0 * rz + s1 * sz;
-#pragma unroll
-            for (int h = 0; h < H; ++h)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double2 uu = u0[h * nee + a * 3 + c], vv = l0[h * nee + a * 3 + c];
-                    U[h][c].x += n * uu.x; U[h][c].y += n * uu.y;
-                    UX[h][c].x += nx * uu.x; UX[h][c].y += nx * uu.y;
-                    UZ[h][c].x += nz * uu.x; UZ[h][c].y += nz * uu.y;
-                    V[h][c].x += n * vv.x; V[h][c].y += n * vv.y;
-                    VX[h][c].x += nx * vv.x; VX[h][c].y += nx * vv.y;
-                    VZ[h][c].x += nz * vv.x; VZ[h][c].y += nz * vv.y;
-                }
-        }
-#pragma unroll
-        for (int h = 0; h < H; ++h) {
-            const int t = t0 + ta + h;
-            if (t >= ntrip) break;
-            const double k = ky[t], w = omega[t];
-            // gradients G[i][j], j = x, y, z; d/dy = -i ky
-            double2 GU[3][3], GV[3][3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                GU[c][0] = UX[h][c]; GU[c][2] = UZ[h][c]; GU[c][1] = make_double2(k * U[h][c].y, -k * U[h][c].x);
-                GV[c][0] = VX[h][c]; GV[c][2] = VZ[h][c]; GV[c][1] = make_double2(k * V[h][c].y, -k * V[h][c].x);
-            }
-            const double2 du = make_double2(GU[0][0].x + GU[1][1].x + GU[2][2].x, GU[0][0].y + GU[1][1].y + GU[2][2].y);
-            const double2 dv = make_double2(GV[0][0].x + GV[1][1].x + GV[2][2].x, GV[0][0].y + GV[1][1].y + GV[2][2].y);
-            Dl += dv.x * du.x + dv.y * du.y;
-            double sm2 = 0.0;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double2 s2 = make_double2(GU[i][j].x + GU[j][i].x, GU[i][j].y + GU[j][i].y);
-                    sm2 += GV[i][j].x * s2.x + GV[i][j].y * s2.y;
-                }
-            Dm += sm2;
-            Dr += -w * w * (V[h][0].x * U[h][0].x + V[h][0].y * U[h][0].y + V[h][1].x * U[h][1].x +
-                            V[h][1].y * U[h][1].y + V[h][2].x * U[h][2].x + V[h][2].y * U[h][2].y);
-        }
-    }
-    double g12[12];
-    const double wt = act ? wq[ix] * wq[iz] * g.det : 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const double f = act ? wt * g.sf[k] : 0.0;
-        g12[k * 3 + 0] = f * Dl;
-        g12[k * 3 + 1] = f * Dm;
-        g12[k * 3 + 2] = f * Dr;
-    }
-    // block reduction of the 12 partial sums
-    for (int i = 0; i < 12; ++i) sred[threadIdx.x * 12 + i] = g12[i];
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int i = 0; i < 12; ++i) sred[threadIdx.x * 12 + i] += sred[(threadIdx.x + st) * 12 + i];
-        __syncthreads();
-    }
-    if (threadIdx.x < 12) part[(size_t)e * 12 + threadIdx.x] = sred[threadIdx.x];
-}
 
-// Adjoint gradient densities by sum factorisation, orders with Q1 <= 16
-// (p <= 5).  The shape functions are tensor products N_a(q) = v_i(xi)
+// Adjoint gradient densities by sum factorisation (every order, p2..p6).
+// The shape functions are tensor products N_a(q) = v_i(xi)
 // v_j(eta), a = j N1 + i (dshl.f:149-175), so per trip, field w (u or l)
 // and component c
 //   T1[eta][i] = sum_j v_j(eta) w_(j,i,c),   T2[eta][i] = sum_j v'_j(eta) w_(j,i,c)
@@ -1374,9 +1261,9 @@ __global__ __launch_bounds__(512) void k_grad(int nen, int nq1, int nelem, int t
 // : N1 Q1 (2 N1 + 3 Q1) complex MACs instead of 3 N1^2 Q1^2 (p4: 4.6x
 // fewer).  The second stage runs on rows of 16 lanes, one row per eta
 // point and one lane per xi point; lane i of a row holds T1, T2 of node i
-// and v_fmac_f64 with row_newbcast:i hands them to the whole row.  The
-// densities, the projection on the anchor hat functions and the reduction
-// are those of k_grad; the 1-D tables are not flushed to zero as DSHL's
+// and v_fmac_f64 with row_newbcast:i hands them to the whole row.  One
+// workgroup per element, the geometry at q computed once per element; the
+// 1-D tables are not flushed to zero as DSHL's
 // products are (a difference below 2.2e-16 per shape value).  The element
 // values of the next pass of triples are loaded into registers while the
 // current pass computes, the forward field's gradient is reduced to its
@@ -1619,151 +1506,10 @@ __global__ void k_grad_gather(int nnpg, int ncon, const int *__restrict__ mconn,
 //   T^l_{b,i}   = conj(D_{b,i}) div u         D_{b,x} = Nx_b, D_{b,y} = -i ky N_b, D_{b,z} = Nz_b
 //   T^m_{b,i}   = sum_k conj(d_k N_b) (G_ik + G_ki),  G_ik = d_k u_i, d_y = -i ky
 //   T^rho_{b,i} = -omega^2 N_b u_i
-// (the same bilinear forms as k_grad: lam^H r = the gradient density), then
-// the chain rule at the anchor:  Vp: 2 rho Vp T^l;  Vs: -4 rho Vs T^l + 2 rho Vs T^m;
-// rho: (Vp^2 - 2 Vs^2) T^l + Vs^2 T^m + T^rho.
-// out[((e*4 + c)*3 + p)*nee + 3b + i].  One workgroup per element; phase 1
-// puts the fields of u at every quadrature point in LDS, phase 2 gives one
-// thread per element row (b, i) and sums over the quadrature points.
-template <int NL>
-__global__ __launch_bounds__(256) void k_fcols(const double *__restrict__ shl, const double *__restrict__ shlT,
-                        const double *__restrict__ wq,
-                        const double *__restrict__ xq, const int *__restrict__ ieng,
-                        const double *__restrict__ xl, const double *__restrict__ zl,
-                        const char *__restrict__ cdom, const int *__restrict__ lm,
-                        const double *__restrict__ rho, const double *__restrict__ lam,
-                        const double *__restrict__ mu, double omega, double kv,
-                        const double2 *__restrict__ u, double2 *__restrict__ out, int *__restrict__ err) {
-    extern __shared__ double sm[];
-    constexpr int QS = 28;   // doubles per quadrature point
-    constexpr int nen = NL * NL, nq1 = 2 * NL + 3, nq = nq1 * nq1, nee = 3 * nen;
-    // phase 2 splits each row's quadrature points over QG thread groups
-    constexpr int QG = (256 / nee) > 0 ? (256 / nee) : 1;
-    const int e = blockIdx.x;
-    double2 *su = reinterpret_cast<double2 *>(sm);   // nee
-    double *sq = reinterpret_cast<double *>(su + nee);   // nq * QS
-    // phase 2's partial sums reuse the quadrature-point storage once every
-    // group is done reading it: (QG-1) * 12 * nee
-    double2 *spart = reinterpret_cast<double2 *>(sq);
-    const int *ie = ieng + 4 * e;
-    const int *L = lm + (size_t)e * nee;
-    double2 *o = out + (size_t)e * 12 * nee;
-    if (cdom[e] == 'A') {   // PML elements are outside the non-PML gradient path
-        for (int i = threadIdx.x; i < 12 * nee; i += blockDim.x) o[i] = make_double2(0.0, 0.0);
-        return;
-    }
-    for (int i = threadIdx.x; i < nee; i += blockDim.x) {
-        const int d = L[i];
-        su[i] = d >= 0 ? u[d] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-        const int ix = q % nq1, iz = q / nq1;
-        Geo g;
-        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
-        double2 U[3] = {}, UX[3] = {}, UZ[3] = {};
-        const double rx = g.dzdeta / g.det, sx = -g.dzdxi / g.det;
-        const double rz = -g.dxdeta / g.det, szz = g.dxdxi / g.det;
-#pragma unroll 5
-        for (int a = 0; a < nen; ++a) {
-            const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
-            const double n = shlT[(size_t)(a * 3 + 2) * nq + q], nx = s0 * rx + s1 * sx, nz = s0 * rz + s1 * szz;
-            for (int c = 0; c < 3; ++c) {
-                const double2 v = su[a * 3 + c];
-                U[c].x += n * v.x; U[c].y += n * v.y;
-                UX[c].x += nx * v.x; UX[c].y += nx * v.y;
-                UZ[c].x += nz * v.x; UZ[c].y += nz * v.y;
-            }
-        }
-        // G[i][k] = d_k u_i, k = x, y, z; d_y u = -i ky u
-        double2 G[3][3];
-        for (int c = 0; c < 3; ++c) {
-            G[c][0] = UX[c];
-            G[c][1] = make_double2(kv * U[c].y, -kv * U[c].x);
-            G[c][2] = UZ[c];
-        }
-        double *p = sq + (size_t)q * QS;
-        const double w = wq[ix] * wq[iz] * g.det;
-        for (int k = 0; k < 4; ++k) p[k] = w * g.sf[k];
-        p[4] = G[0][0].x + G[1][1].x + G[2][2].x;   // div u
-        p[5] = G[0][0].y + G[1][1].y + G[2][2].y;
-        // symmetric S_ik = G_ik + G_ki: xx, yy, zz, xy, xz, yz
-        const int si[6] = {0, 1, 2, 0, 0, 1}, sk[6] = {0, 1, 2, 1, 2, 2};
-        for (int j = 0; j < 6; ++j) {
-            p[6 + 2 * j] = G[si[j]][sk[j]].x + G[sk[j]][si[j]].x;
-            p[7 + 2 * j] = G[si[j]][sk[j]].y + G[sk[j]][si[j]].y;
-        }
-        for (int c = 0; c < 3; ++c) { p[18 + 2 * c] = U[c].x; p[19 + 2 * c] = U[c].y; }
-        p[24] = rx; p[25] = sx; p[26] = rz; p[27] = szz;
-    }
-    __syncthreads();
-    const int it = threadIdx.x % nee, grp = threadIdx.x / nee;
-    const bool rowt = grp < QG;
-    const int b = it / 3, i = it % 3;
-    // index of S_ik in the packed symmetric storage for k = x, y, z
-    const int i0 = i == 0 ? 0 : (i == 1 ? 3 : 4), i1 = i == 0 ? 3 : (i == 1 ? 1 : 5), i2 = i == 0 ? 4 : (i == 1 ? 5 : 2);
-    double2 acc[4][3];
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int t = 0; t < 3; ++t) acc[c][t] = make_double2(0.0, 0.0);
-    const double w2 = omega * omega;
-#pragma unroll 2
-    for (int q = grp; q < nq && rowt; q += QG) {
-        const double *p = sq + (size_t)q * QS;
-        const double *s = shl + ((size_t)q * nen + b) * 3;
-        const double n = s[2], nx = s[0] * p[24] + s[1] * p[25], nz = s[0] * p[26] + s[1] * p[27];
-        // conj(D_{b,i}): x -> Nx, y -> +i ky N, z -> Nz
-        const double2 dv = make_double2(p[4], p[5]);
-        double2 tl;
-        if (i == 0) tl = make_double2(nx * dv.x, nx * dv.y);
-        else if (i == 2) tl = make_double2(nz * dv.x, nz * dv.y);
-        else tl = make_double2(-kv * n * dv.y, kv * n * dv.x);
-        // mu: Nx S_ix + (i ky N) S_iy + Nz S_iz
-        const double *sx_ = p + 6 + 2 * i0, *sy_ = p + 6 + 2 * i1, *sz_ = p + 6 + 2 * i2;
-        const double2 tm = make_double2(nx * sx_[0] - kv * n * sy_[1] + nz * sz_[0],
-                                        nx * sx_[1] + kv * n * sy_[0] + nz * sz_[1]);
-        const double2 tr = make_double2(-w2 * n * p[18 + 2 * i], -w2 * n * p[19 + 2 * i]);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const double wc = p[c];
-            acc[c][0].x += wc * tl.x; acc[c][0].y += wc * tl.y;
-            acc[c][1].x += wc * tm.x; acc[c][1].y += wc * tm.y;
-            acc[c][2].x += wc * tr.x; acc[c][2].y += wc * tr.y;
-        }
-    }
-    // groups 1.. hand their partial sums to group 0
-    if (QG > 1) {
-        __syncthreads();
-        if (rowt && grp > 0)
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-#pragma unroll
-                for (int t = 0; t < 3; ++t) spart[((size_t)(grp - 1) * 12 + c * 3 + t) * nee + it] = acc[c][t];
-        __syncthreads();
-        if (grp == 0)
-            for (int g2 = 1; g2 < QG; ++g2)
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-#pragma unroll
-                    for (int t = 0; t < 3; ++t) {
-                        const double2 v = spart[((size_t)(g2 - 1) * 12 + c * 3 + t) * nee + it];
-                        acc[c][t].x += v.x; acc[c][t].y += v.y;
-                    }
-    }
-    if (grp != 0) return;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int a = ie[c] - 1;
-        const double r = rho[a], vp = sqrt((lam[a] + 2.0 * mu[a]) / r), vs = sqrt(mu[a] / r);
-        const double2 l = acc[c][0], m = acc[c][1], rr = acc[c][2];
-        const double cl0 = 2.0 * r * vp, cl1 = -4.0 * r * vs, cm1 = 2.0 * r * vs;
-        const double cl2 = vp * vp - 2.0 * vs * vs, cm2 = vs * vs;
-        o[((size_t)c * 3 + 0) * nee + it] = make_double2(cl0 * l.x, cl0 * l.y);
-        o[((size_t)c * 3 + 1) * nee + it] = make_double2(cl1 * l.x + cm1 * m.x, cl1 * l.y + cm1 * m.y);
-        o[((size_t)c * 3 + 2) * nee + it] = make_double2(cl2 * l.x + cm2 * m.x + rr.x, cl2 * l.y + cm2 * m.y + rr.y);
-    }
-}
+// (the same bilinear forms as the gradient densities: lam^H r = the
+// gradient density), then the chain rule at the anchor:  Vp: 2 rho Vp T^l;
+// Vs: -4 rho Vs T^l + 2 rho Vs T^m;  rho: (Vp^2 - 2 Vs^2) T^l + Vs^2 T^m + T^rho.
+// out[((e*4 + c)*3 + p)*nee + 3b + i].
 
 // fmac_bcast kept in program order (volatile): the sums of k_fcols_sf run one
 // after another instead of being interleaved into more live registers
@@ -1796,8 +1542,8 @@ __device__ __forceinline__ void sf_row_dot(double2 &acc, const double2 &F, const
     ((fmac_bcast_ordered<J>(acc.x, F.x, c[J]), fmac_bcast_ordered<J>(acc.y, F.y, c[J])), ...);
 }
 
-// The element part of k_fcols by sum factorisation, orders with Q1 <= 16
-// (p <= 5); same output.  With N_b = v_bi(xi) v_bj(eta) and the anchor hat
+// The element part by sum factorisation (every order, p2..p6).  With
+// N_b = v_bi(xi) v_bj(eta) and the anchor hat
 // functions sf_c = h_cx(xi) h_cz(eta) (h_0 = (1 - s)/2, h_1 = (1 + s)/2;
 // corners c = 0..3 at (cx, cz) = (0,0), (1,0), (1,1), (0,1)), every column
 // entry is
@@ -1807,8 +1553,8 @@ __device__ __forceinline__ void sf_row_dot(double2 &acc, const double2 &F, const
 // where (A, B, C) per quadrature point are, for the density l (i = x, y, z):
 // (rx div, sx div, 0), (0, 0, i ky div), (rz div, sz div, 0); for m:
 // (rx S_ix + rz S_iz, sx S_ix + sz S_iz, i ky S_iy); for the rho term:
-// (0, 0, -omega^2 u_i), all times the quadrature weight w (the chain-rule
-// factors of k_fcols follow).  Stages: the fields and their gradients at
+// (0, 0, -omega^2 u_i), all times the quadrature weight w (then the chain
+// rule above).  One workgroup per element.  Stages: the fields and their gradients at
 // the quadrature points (rows of 16 lanes, one per eta point, as in
 // k_grad_sf); per channel, lane (cx, bi) of each row forms P and R from the
 // row's points by DPP broadcast; then one thread per (corner, node,
@@ -2104,10 +1850,9 @@ __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *
 //   mode 2 (item = local column c): column c of F in CSC order, COMPLEX*8,
 //          zero outside the element's rows.
 // Every row of modes 1 and 2 is checked against column c's CSC pattern.
-// Flux layout per (vector, q), 8 complex: comp 1 N, Nx, Nz; comp 2 N, Nx,
-// Nz; comp 3 N, Nz (comp 3's Nx flux equals comp 1's Nz flux).  LDS holds
-// the fluxes of nvec vectors; the projection's partial sums reuse it.
-// Row outputs of one k_dab item (rows are threads 0..nee-1, row = -1 for
+// Fluxes per (vector, q), 8 complex: comp 1 N, Nx, Nz; comp 2 N, Nx, Nz;
+// comp 3 N, Nz (comp 3's Nx flux equals comp 1's Nz flux).
+// Row outputs of one item (rows are threads 0..nee-1, row = -1 for
 // the others): mode 0 adds into y, mode 1 contracts with z into p(MYGRAD),
 // mode 2 writes column it of F; every row of modes 1 and 2 is looked up in
 // column it's CSC pattern (err 3 if absent).
@@ -2116,7 +1861,7 @@ __device__ inline void dab_epilogue(int mode, int nvec, int it, int row, const i
                                     const int *__restrict__ mygrad, double2 *__restrict__ y0,
                                     double2 *__restrict__ y1, const float2 *__restrict__ z0,
                                     const float2 *__restrict__ z1, float *__restrict__ p0, float *__restrict__ p1,
-                                    float2 *__restrict__ fout, int *__restrict__ err, double (*red)[4]) {
+                                    float2 *__restrict__ fout, int *__restrict__ err, double (*red)[8]) {
     int i1 = 0, i2 = 0;
     if (mode != 0) { i1 = jcsc[it] - 1; i2 = jcsc[it + 1] - 1; }
     double acc[2] = {0.0, 0.0};
@@ -2162,147 +1907,8 @@ __device__ inline void dab_epilogue(int mode, int nvec, int it, int row, const i
     }
 }
 
-template <int NL, bool IMAG>
-__global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, double kp, double ks,
-                                             double kv, const int *__restrict__ item_elem,
-                                             const double *__restrict__ item_coef,
-                                             const double *__restrict__ shlT, const double *__restrict__ wq,
-                                             const double *__restrict__ xq, const int *__restrict__ ieng,
-                                             const double *__restrict__ xl, const double *__restrict__ zl,
-                                             const int *__restrict__ lm, const float2 *__restrict__ x0,
-                                             const float2 *__restrict__ x1, double2 *__restrict__ y0,
-                                             double2 *__restrict__ y1, const float2 *__restrict__ z0,
-                                             const float2 *__restrict__ z1, const int *__restrict__ mygrad,
-                                             const int *__restrict__ jcsc, const int *__restrict__ icsc,
-                                             float *__restrict__ p0, float *__restrict__ p1,
-                                             float2 *__restrict__ fout, int *__restrict__ err) {
-    extern __shared__ double sm[];
-    __shared__ double red[2][4];
-    const int it = blockIdx.x;
-    if (it >= nitems) return;
-    const int e = item_elem[it];
-    const double c0 = item_coef[4 * it], c1 = item_coef[4 * it + 1], c2 = item_coef[4 * it + 2],
-                 c3 = item_coef[4 * it + 3];
-    constexpr int nen = NL * NL, nq1 = 2 * NL + 3, nq = nq1 * nq1, nee = 3 * nen;
-    const int QG = max(1, (int)blockDim.x / nee);              // q groups of the projection
-    double2 *su = reinterpret_cast<double2 *>(sm);           // [v][3 nen]
-    double *sG = reinterpret_cast<double *>(su + 2 * nee);   // [q][4]: rx, sx, rz, sz
-    double2 *sF = reinterpret_cast<double2 *>(sG + 4 * nq);  // [v][q][8]
-    double2 *sP = sF;   // [v][group][row] partial projections, after the fluxes are read
-    const int *ie = ieng + 4 * e;
-    const int *L = lm + (size_t)e * nee;
-    if (mode == 2)
-        for (int r = jcsc[it] - 1 + threadIdx.x; r < jcsc[it + 1] - 1; r += blockDim.x)
-            fout[r] = make_float2(0.f, 0.f);
-    for (int i = threadIdx.x; i < nee; i += blockDim.x) {
-        const int d = L[i];
-        float2 a = d >= 0 ? x0[d] : make_float2(0.f, 0.f);
-        su[i] = make_double2(a.x, a.y);
-        if (nvec > 1) {
-            float2 b = d >= 0 ? x1[d] : make_float2(0.f, 0.f);
-            su[nee + i] = make_double2(b.x, b.y);
-        }
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-        const int ix = q % nq1, iz = q / nq1;
-        Geo g;
-        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) *err = 2;
-        const double D = wq[ix] * wq[iz] * g.det * (c0 * g.sf[0] + c1 * g.sf[1] + c2 * g.sf[2] + c3 * g.sf[3]);
-        const double P = kp * D, S = ks * D, Q = P - 2.0 * S, k2 = kv * kv;
-        const double rx = g.dzdeta / g.det, sx = -g.dzdxi / g.det, rz = -g.dxdeta / g.det, sz = g.dxdxi / g.det;
-        double *G = sG + 4 * q;
-        G[0] = rx; G[1] = sx; G[2] = rz; G[3] = sz;
-#pragma unroll
-        for (int v = 0; v < 2; ++v) {
-            if (v >= nvec) break;
-            double2 U[3] = {}, UX[3] = {}, UZ[3] = {};
-            const double2 *x = su + v * nee;
-#pragma unroll 5
-            for (int a = 0; a < nen; ++a) {
-                // shlT[(a*3 + c)*nq + q]: adjacent lanes read adjacent words
-                const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
-                const double n = shlT[(size_t)(a * 3 + 2) * nq + q];
-                const double nx = s0 * rx + s1 * sx, nz = s0 * rz + s1 * sz;
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc) {
-                    const double2 w = x[3 * a + cc];
-                    U[cc].x += n * w.x; U[cc].y += n * w.y;
-                    UX[cc].x += nx * w.x; UX[cc].y += nx * w.y;
-                    UZ[cc].x += nz * w.x; UZ[cc].y += nz * w.y;
-                }
-            }
-            double2 *F = sF + ((size_t)v * nq + q) * 8;
-            // ky coupling: mk(X) = i kv X (IMAG) or kv X
-            auto mk = [&](double2 X) {
-                return IMAG ? make_double2(-kv * X.y, kv * X.x) : make_double2(kv * X.x, kv * X.y);
-            };
-            auto lin = [](double a, double2 X, double b, double2 Y) {
-                return make_double2(a * X.x + b * Y.x, a * X.y + b * Y.y);
-            };
-            // comp 1: N: S mk(UX2) + kv^2 S U1; Nx: P UX1 + Q UZ3 - Q mk(U2); Nz: S (UZ1 + UX3)
-            const double2 m2 = mk(U[1]);
-            F[0] = lin(S, mk(UX[1]), k2 * S, U[0]);
-            F[1] = make_double2(P * UX[0].x + Q * UZ[2].x - Q * m2.x, P * UX[0].y + Q * UZ[2].y - Q * m2.y);
-            F[2] = make_double2(S * (UZ[0].x + UX[2].x), S * (UZ[0].y + UX[2].y));
-            // comp 2: N: Q mk(UX1 + UZ3) + kv^2 P U2; Nx: S (UX2 - mk(U1)); Nz: S (UZ2 - mk(U3))
-            F[3] = lin(Q, mk(make_double2(UX[0].x + UZ[2].x, UX[0].y + UZ[2].y)), k2 * P, U[1]);
-            F[4] = lin(S, UX[1], -S, mk(U[0]));
-            F[5] = lin(S, UZ[1], -S, mk(U[2]));
-            // comp 3: N: S mk(UZ2) + kv^2 S U3; Nx: S (UZ1 + UX3); Nz: Q UX1 + P UZ3 - Q mk(U2)
-            F[6] = lin(S, mk(UZ[1]), k2 * S, U[2]);
-            F[7] = make_double2(Q * UX[0].x + P * UZ[2].x - Q * m2.x, Q * UX[0].y + P * UZ[2].y - Q * m2.y);
-        }
-    }
-    __syncthreads();
-    // projection on the test functions: QG groups of threads split the q
-    // range of every row, then one pass adds the groups
-    {
-        const int i = threadIdx.x % nee, grp = threadIdx.x / nee;
-        double2 val[2] = {};
-        if (grp < QG) {
-            const int a = i / 3, cc = i % 3;
-            // flux slots of this component: N, Nx, Nz
-            const int fN = 3 * cc, fX = cc == 2 ? 2 : 3 * cc + 1, fZ = cc == 2 ? 7 : 3 * cc + 2;
-#pragma unroll 4
-            for (int q = grp; q < nq; q += QG) {
-                const double *G = sG + 4 * q;
-                const double s0 = shlT[(size_t)(a * 3) * nq + q], s1 = shlT[(size_t)(a * 3 + 1) * nq + q];
-                const double n = shlT[(size_t)(a * 3 + 2) * nq + q];
-                const double nx = s0 * G[0] + s1 * G[1], nz = s0 * G[2] + s1 * G[3];
-#pragma unroll
-                for (int v = 0; v < 2; ++v) {
-                    if (v >= nvec) break;
-                    const double2 *F = sF + ((size_t)v * nq + q) * 8;
-                    val[v].x += n * F[fN].x + nx * F[fX].x + nz * F[fZ].x;
-                    val[v].y += n * F[fN].y + nx * F[fX].y + nz * F[fZ].y;
-                }
-            }
-        }
-        // every thread is done with the fluxes: the partial sums take their place
-        __syncthreads();
-        if (grp < QG)
-#pragma unroll
-            for (int v = 0; v < 2; ++v) if (v < nvec) sP[((size_t)v * QG + grp) * nee + i] = val[v];
-    }
-    __syncthreads();
-    double2 val[2] = {};
-    const int row = threadIdx.x < nee ? (int)threadIdx.x : -1;   // nee < 256 for every order
-    if (row >= 0) {
-#pragma unroll
-        for (int v = 0; v < 2; ++v) {
-            if (v >= nvec) break;
-            for (int gi = 0; gi < QG; ++gi) {
-                const double2 pv = sP[((size_t)v * QG + gi) * nee + row];
-                val[v].x += pv.x; val[v].y += pv.y;
-            }
-        }
-    }
-    dab_epilogue(mode, nvec, it, row, L, val, jcsc, icsc, mygrad, y0, y1, z0, z1, p0, p1, fout, err, red);
-}
-
-// k_dab by sum factorisation, orders with Q1 <= 16 (p <= 5); same items,
-// modes, fluxes and outputs.  Per vector, the element field and its
+// The element derivative products above by sum factorisation (every
+// order, p2..p6), one workgroup per item.  Per vector, the element field and its
 // reference derivatives at the quadrature points come from two 1-D passes
 // (as in k_grad_sf: T1, T2 along eta, then rows of 16 lanes along xi with
 // the DPP broadcast), and the projection on the test functions is split the
@@ -2313,7 +1919,7 @@ __global__ __launch_bounds__(256) void k_dab(int mode, int nitems, int nvec, dou
 // About 4x fewer multiply-adds than the direct sums over the element
 // nodes and the quadrature points.
 template <int NL, bool IMAG>
-__global__ __launch_bounds__(256) void k_dab_sf(int mode, int nitems, int nvec, double kp, double ks,
+__global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mode, int nitems, int nvec, double kp, double ks,
                                                 double kv, const int *__restrict__ item_elem,
                                                 const double *__restrict__ item_coef,
                                                 const double *__restrict__ t1d, const double *__restrict__ wq,
@@ -2326,10 +1932,14 @@ __global__ __launch_bounds__(256) void k_dab_sf(int mode, int nitems, int nvec, 
                                                 const int *__restrict__ jcsc, const int *__restrict__ icsc,
                                                 float *__restrict__ p0, float *__restrict__ p1,
                                                 float2 *__restrict__ fout, int *__restrict__ err) {
-    constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen, QQ = Q1 * Q1, NT = 256;
-    static_assert(Q1 <= 16, "one row of 16 lanes per eta point");
+    constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen, QQ = Q1 * Q1;
+    // Q1 = 17 (p6): rows 0..16 of 16 lanes hold xi points 0..15 and the 17
+    // lanes after them xi point 16 of eta rows 0..16 (five waves)
+    constexpr bool OVF = Q1 > 16;
+    constexpr int NT = OVF ? 320 : 256;
+    static_assert(Q1 <= 17, "one row of 16 lanes per eta point, plus one column");
     extern __shared__ double sm[];
-    __shared__ double red[2][4];
+    __shared__ double red[2][8];
     double2 *su = reinterpret_cast<double2 *>(sm);          // [2][nee]
     double *sG = reinterpret_cast<double *>(su + 2 * nee);  // [QQ][5]: rx, sx, rz, sz, D
     double2 *sT = reinterpret_cast<double2 *>(sG + 5 * QQ); // [3][Q1][N1][2]: T1, T2
@@ -2364,8 +1974,10 @@ __global__ __launch_bounds__(256) void k_dab_sf(int mode, int nitems, int nvec, 
     }
     // stage-2 lanes: row = eta point, lane = xi point (clamped when idle)
     const int row16 = tid >> 4, x16 = tid & 15;
-    const bool act = row16 < Q1 && x16 < Q1;
-    const int qe = min(row16, Q1 - 1), qx = min(x16, Q1 - 1), ni = min(x16, N1 - 1);
+    const bool ovl = OVF && tid >= 16 * Q1 && tid < 17 * Q1;   // a lane of the xi = 16 column
+    const bool act = (row16 < Q1 && x16 < Q1) || ovl;
+    const int qe = ovl ? tid - 16 * Q1 : min(row16, Q1 - 1), qx = ovl ? 16 : min(x16, Q1 - 1);
+    const int ni = min(x16, N1 - 1);
     double vx[N1], dx[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) { vx[i] = v1[i * Q1 + qx]; dx[i] = d1[i * Q1 + qx]; }
@@ -2400,7 +2012,18 @@ __global__ __launch_bounds__(256) void k_dab_sf(int mode, int nitems, int nvec, 
                 const double2 *o = sT + ((c * Q1 + qe) * N1 + ni) * 2;
                 const double2 T1 = o[0], T2 = o[1];
                 double2 u0 = make_double2(0.0, 0.0), uxi = u0, uet = u0;
-                sf_row_sums<NL, 0>(u0, uxi, uet, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+                if (!ovl) {
+                    sf_row_sums<NL, 0>(u0, uxi, uet, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+                } else {   // the xi = 16 column reads the row's T1, T2 from LDS
+                    const double2 *r = o - 2 * ni;
+#pragma unroll
+                    for (int j = 0; j < N1; ++j) {
+                        const double2 a1 = r[2 * j], a2 = r[2 * j + 1];
+                        u0.x += vx[j] * a1.x; u0.y += vx[j] * a1.y;
+                        uxi.x += dx[j] * a1.x; uxi.y += dx[j] * a1.y;
+                        uet.x += vx[j] * a2.x; uet.y += vx[j] * a2.y;
+                    }
+                }
                 U[c] = u0;
                 UX[c] = make_double2(rx * uxi.x + sx * uet.x, rx * uxi.y + sx * uet.y);
                 UZ[c] = make_double2(rz * uxi.x + sz * uet.x, rz * uxi.y + sz * uet.y);
@@ -2411,7 +2034,7 @@ __global__ __launch_bounds__(256) void k_dab_sf(int mode, int nitems, int nvec, 
             auto lin = [](double a, double2 X, double b, double2 Y) {
                 return make_double2(a * X.x + b * Y.x, a * X.y + b * Y.y);
             };
-            // the fluxes of k_dab
+            // the fluxes above
             const double2 m2 = mk(U[1]);
             double2 F[8];
             F[0] = lin(S, mk(UX[1]), k2 * S, U[0]);
