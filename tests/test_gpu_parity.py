@@ -234,10 +234,12 @@ def test_apply_quadratic_in_ky():
 
 
 @pytest.mark.parametrize("name,nf,nk", [("tiny_layered", 1, 2), ("pml", 1, 2), ("tiny_layered", 2, 3),
-                                         ("p5", 2, 3), ("p6", 1, 2)])
+                                         ("p5", 2, 3), ("p6", 1, 2), ("tiny_layered", 1, 3), ("p6", 1, 1),
+                                         ("p5", 1, 5)])
 def test_gradient_exact(name, nf, nk):
-    """The sum-factorised kernel (with 2 and 6 triples: one and three
-    passes of 2); p6 adds a column of lanes for the 17th xi point."""
+    """The sum-factorised kernel with 1 to 6 triples (passes of 2, the last
+    one half empty for an odd count); p6 adds a column of lanes for the 17th
+    xi point."""
     mi = _mi(name)
     m, mod, pml = mesh_case(name)
     g = oracle_graph(name)
