@@ -1511,8 +1511,9 @@ __global__ void k_grad_gather(int nnpg, int ncon, const int *__restrict__ mconn,
 // Vs: -4 rho Vs T^l + 2 rho Vs T^m;  rho: (Vp^2 - 2 Vs^2) T^l + Vs^2 T^m + T^rho.
 // out[((e*4 + c)*3 + p)*nee + 3b + i].
 
-// fmac_bcast kept in program order (volatile): the sums of k_fcols_sf run one
-// after another instead of being interleaved into more live registers
+// fmac_bcast as a volatile statement: kept in program order after the
+// first pair of its row sum (fmac2_bcast_first, which carries the wait
+// states)
 template <int J>
 __device__ __forceinline__ void fmac_bcast_ordered(double &acc, double cv, double op) {
     asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
@@ -1691,8 +1692,8 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     const bool wr = row < Q1 && x16 < 2 * N1;
     const auto seq = std::make_integer_sequence<int, QD>();
     // row sum of point field number f over the xi points of this row
-    auto sf_row_dot = [&](double2 &acc, const double2 &F, const double *c, int f) {
-        f25k::sf_row_dot(acc, F, c, seq);
+    auto row_sum = [&](double2 &acc, const double2 &F, const double *c, int f) {
+        sf_row_dot(acc, F, c, seq);
         if constexpr (OVF) {
             const double2 v = sF[f * Q1 + qe];
             acc.x += c[QD] * v.x;
@@ -1704,15 +1705,15 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     };
     {   // density l: x, y, z rows
         double2 P = make_double2(0.0, 0.0), R = P;
-        sf_row_dot(P, make_double2(rx * dv.x, rx * dv.y), cd, 0);
-        sf_row_dot(R, make_double2(sx * dv.x, sx * dv.y), cv, 1);
+        row_sum(P, make_double2(rx * dv.x, rx * dv.y), cd, 0);
+        row_sum(R, make_double2(sx * dv.x, sx * dv.y), cv, 1);
         put(0, P); put(9, R);
         P = make_double2(0.0, 0.0);
-        sf_row_dot(P, make_double2(-kv * dv.y, kv * dv.x), cv, 2);
+        row_sum(P, make_double2(-kv * dv.y, kv * dv.x), cv, 2);
         put(1, P);
         P = make_double2(0.0, 0.0); R = P;
-        sf_row_dot(P, make_double2(rz * dv.x, rz * dv.y), cd, 3);
-        sf_row_dot(R, make_double2(sz * dv.x, sz * dv.y), cv, 4);
+        row_sum(P, make_double2(rz * dv.x, rz * dv.y), cd, 3);
+        row_sum(R, make_double2(sz * dv.x, sz * dv.y), cv, 4);
         put(2, P); put(10, R);
     }
 #pragma unroll
@@ -1720,15 +1721,15 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
         constexpr int ix[3] = {0, 3, 4}, iy[3] = {3, 1, 5}, iz[3] = {4, 5, 2};   // S_ix, S_iy, S_iz in Sw
         const double2 Sx = Sw[ix[i]], Sy = Sw[iy[i]], Sz = Sw[iz[i]];
         double2 P = make_double2(0.0, 0.0), R = P;
-        sf_row_dot(P, make_double2(rx * Sx.x + rz * Sz.x, rx * Sx.y + rz * Sz.y), cd, 5 + 3 * i);
-        sf_row_dot(P, make_double2(-kv * Sy.y, kv * Sy.x), cv, 6 + 3 * i);
-        sf_row_dot(R, make_double2(sx * Sx.x + sz * Sz.x, sx * Sx.y + sz * Sz.y), cv, 7 + 3 * i);
+        row_sum(P, make_double2(rx * Sx.x + rz * Sz.x, rx * Sx.y + rz * Sz.y), cd, 5 + 3 * i);
+        row_sum(P, make_double2(-kv * Sy.y, kv * Sy.x), cv, 6 + 3 * i);
+        row_sum(R, make_double2(sx * Sx.x + sz * Sz.x, sx * Sx.y + sz * Sz.y), cv, 7 + 3 * i);
         put(3 + i, P); put(11 + i, R);
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {   // rho term
         double2 P = make_double2(0.0, 0.0);
-        sf_row_dot(P, U[i], cv, 14 + i);
+        row_sum(P, U[i], cv, 14 + i);
         put(6 + i, P);
     }
     __syncthreads();
