@@ -94,6 +94,40 @@ def test_asmble_config1_and_spmv(torch_dev):
     assert relerr(dy.cpu().numpy(), oracle.spmv_z(g, ref, x)) < FP64_TOL
 
 
+def test_apply_full_size_against_assembled_csr(torch_dev):
+    """At the benchmark size (200 x 100 elements, p = 4, 963,603 DOFs,
+    8 frequencies x 16 ky = 128 pairs, layered model with anomalies): the
+    batched FP64 product (element by element, one launch per colour) against
+    the assembled FP64 CSR operator of the same pair followed by its SpMV,
+    for the first, a middle and the last pair; and two runs of the product
+    bit for bit equal (the colour scatter has no races)."""
+    torch = torch_dev
+    from fem25_amd.api import MeshInfo
+    m = cf.structured_mesh(200, 100, 4)
+    mi = MeshInfo(m)
+    mi.set_model(cf.layered_model(m, seed=2))
+    om, ky, f, py = cf.freq_ky_batch(8, 16, 0.02, 0.1, 6300.0 / np.sqrt(3.0))
+    print(f"full-size product: ndof={mi.ndof} npair={om.size}", flush=True)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn(mi.ndof, om.size, dtype=torch.complex128, device="cuda", generator=gen)
+    y = torch.empty_like(x)
+    mi.apply_batch_dev(om, ky, x, y)
+    y2 = torch.empty_like(x)
+    mi.apply_batch_dev(om, ky, x, y2)
+    torch.cuda.synchronize()
+    assert torch.equal(y, y2)
+    del y2
+    s = torch.empty(mi.nz, dtype=torch.complex128, device="cuda")
+    for b in (0, om.size // 2 + 3, om.size - 1):
+        mi.asmble_dev(b"B", float(f[b]), float(py[b]), s)
+        xb = x[:, b].contiguous()
+        yb = torch.empty_like(xb)
+        mi.csr_apply_dev(s, xb, yb)
+        err = float(torch.linalg.norm(y[:, b] - yb) / torch.linalg.norm(yb))
+        assert err < 1e-13, (b, err)
+    mi.close()
+
+
 @pytest.mark.parametrize("name", ["tiny_layered", "p5", "p6"])
 def test_apply_batch_fp64(name):
     mi = _mi(name)
