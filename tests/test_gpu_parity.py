@@ -128,6 +128,39 @@ def test_apply_full_size_against_assembled_csr(torch_dev):
     mi.close()
 
 
+def test_gradient_full_size_additive_and_linear(torch_dev):
+    """At the benchmark size (200 x 100 elements, p = 4, 128 triples): the
+    gradient over all triples equals the sum of the gradients over four
+    quarters of them (the kernel takes two triples per pass and loads the
+    next pass during the current one), and it is linear in the adjoint
+    field: g(u, l1 + 2 l2) = g(u, l1) + 2 g(u, l2)."""
+    torch = torch_dev
+    from fem25_amd.api import MeshInfo
+    m = cf.structured_mesh(200, 100, 4)
+    mi = MeshInfo(m)
+    mi.set_model(cf.layered_model(m, seed=2))
+    om, ky, f, py = cf.freq_ky_batch(8, 16, 0.02, 0.1, 6300.0 / np.sqrt(3.0))
+    nt, n = om.size, mi.ndof
+    print(f"full-size gradient: ndof={n} ntrip={nt}", flush=True)
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    u = torch.randn(nt, n, dtype=torch.complex128, device="cuda", generator=gen)
+    l1 = torch.randn(nt, n, dtype=torch.complex128, device="cuda", generator=gen)
+    l2 = torch.randn(nt, n, dtype=torch.complex128, device="cuda", generator=gen)
+
+    def grad(lam, sl=slice(None)):
+        g = torch.zeros(m.nnpg, 3, dtype=torch.float64, device="cuda")
+        mi.gradient_dev(om[sl], ky[sl], u[sl].contiguous(), lam[sl].contiguous(), g)
+        return g
+
+    g1 = grad(l1)
+    parts = sum(grad(l1, slice(q * nt // 4, (q + 1) * nt // 4)) for q in range(4))
+    assert float(torch.linalg.norm(parts - g1) / torch.linalg.norm(g1)) < 1e-12
+    g2 = grad(l2)
+    g12 = grad(l1 + 2 * l2)
+    assert float(torch.linalg.norm(g12 - (g1 + 2 * g2)) / torch.linalg.norm(g12)) < 1e-12
+    mi.close()
+
+
 @pytest.mark.parametrize("name", ["tiny_layered", "p5", "p6"])
 def test_apply_batch_fp64(name):
     mi = _mi(name)
