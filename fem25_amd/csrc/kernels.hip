@@ -1593,6 +1593,17 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     }
     const int *ie = ieng + 4 * e;
     const int *L = lm + (size_t)e * nee;
+    // rows of 16 lanes: row = eta point, lane = xi point (clamped copies
+    // where idle: they are never read or stored); the geometry of the lane's
+    // point first, so that its loads overlap the stages before its use
+    const int row = tid >> 4, x16 = tid & 15;
+    const bool ovl = OVF && tid >= 16 * Q1 && tid < 17 * Q1;   // a lane of the xi = 16 column
+    const int qe = ovl ? tid - 16 * Q1 : min(row, Q1 - 1), qx = ovl ? 16 : min(x16, Q1 - 1);
+    const int ni = min(x16, N1 - 1);
+    Geo g;
+    if (geo_at(xq[qx], xq[qe], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
+    const double rx = g.dzdeta / g.det, sx = -g.dzdxi / g.det, rz = -g.dxdeta / g.det, sz = g.dxdxi / g.det;
+    const double w = wq[qx] * wq[qe] * g.det;
     for (int i = tid; i < 2 * N1 * Q1; i += NT) s1[i] = t1d[i];
     for (int i = tid; i < 2 * Q1; i += NT) sh[i] = 0.5 * (i < Q1 ? 1.0 - xq[i] : 1.0 + xq[i - Q1]);
     for (int i = tid; i < nee; i += NT) {
@@ -1614,16 +1625,6 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
         sT[((c * Q1 + qy) * N1 + i) * 2 + 1] = t2;
     }
     __syncthreads();
-    // rows of 16 lanes: row = eta point, lane = xi point (clamped copies
-    // where idle: they are never read or stored)
-    const int row = tid >> 4, x16 = tid & 15;
-    const bool ovl = OVF && tid >= 16 * Q1 && tid < 17 * Q1;   // a lane of the xi = 16 column
-    const int qe = ovl ? tid - 16 * Q1 : min(row, Q1 - 1), qx = ovl ? 16 : min(x16, Q1 - 1);
-    const int ni = min(x16, N1 - 1);
-    Geo g;
-    if (geo_at(xq[qx], xq[qe], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
-    const double rx = g.dzdeta / g.det, sx = -g.dzdxi / g.det, rz = -g.dxdeta / g.det, sz = g.dxdxi / g.det;
-    const double w = wq[qx] * wq[qe] * g.det;
     double2 U[3], G[3][3];
     {
         double vx[N1], dx[N1];
