@@ -1770,16 +1770,16 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
 // its elements' vectors on shared DOFs.  pmap[(a*ncon + s)*nee + j] is the
 // patch position of row j of the s-th element of a (-1 fixed DOF); the
 // element and its corner come from mconn.  diag[3a + p] += |column|^2.
-__global__ void k_fdiag(int nnpg, int ncon, int nee, const int *__restrict__ mconn,
-                        const int *__restrict__ ieng, const int *__restrict__ pmap,
-                        const double2 *__restrict__ cols, double *__restrict__ diag) {
+__global__ __launch_bounds__(256) void k_fdiag(int nnpg, int ncon, int nee, const int *__restrict__ mconn,
+                                                const int *__restrict__ ieng, const int *__restrict__ pmap,
+                                                const double2 *__restrict__ cols, double *__restrict__ diag) {
     extern __shared__ double sm[];
-    const int a = blockIdx.x;
+    __shared__ double red[3][16];
+    const int a = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     if (a >= nnpg) return;
     const int np = ncon * nee;
     double2 *acc = reinterpret_cast<double2 *>(sm);   // np * 3
-    double *red = reinterpret_cast<double *>(acc + (size_t)np * 3);   // blockDim * 3
-    for (int i = threadIdx.x; i < np * 3; i += blockDim.x) acc[i] = make_double2(0.0, 0.0);
+    for (int i = tid; i < np * 3; i += nt) acc[i] = make_double2(0.0, 0.0);
     __syncthreads();
     for (int s = 0; s < ncon; ++s) {
         const int e1 = mconn[(size_t)s * nnpg + a];
@@ -1790,28 +1790,32 @@ __global__ void k_fdiag(int nnpg, int ncon, int nee, const int *__restrict__ mco
             if (ieng[4 * e + k] - 1 == a) corner = k;
         const int *pm = pmap + ((size_t)a * ncon + s) * nee;
         const double2 *src = cols + ((size_t)e * 4 + corner) * 3 * nee;
-        for (int j = threadIdx.x; j < nee; j += blockDim.x) {
+        // one thread per (parameter p, row j): the rows of one element map
+        // to distinct positions, so no two threads add to the same value
+        for (int i = tid; i < 3 * nee; i += nt) {
+            const int p = i / nee, j = i - p * nee;
             const int pos = pm[j];
             if (pos < 0) continue;
-            for (int p = 0; p < 3; ++p) {
-                const double2 v = src[(size_t)p * nee + j];
-                acc[pos * 3 + p].x += v.x;
-                acc[pos * 3 + p].y += v.y;
-            }
+            const double2 v = src[i];
+            acc[pos * 3 + p].x += v.x;
+            acc[pos * 3 + p].y += v.y;
         }
-        __syncthreads();   // rows of one element map to distinct positions
-    }
-    double s3[3] = {0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < np; i += blockDim.x)
-        for (int p = 0; p < 3; ++p) s3[p] += acc[i * 3 + p].x * acc[i * 3 + p].x + acc[i * 3 + p].y * acc[i * 3 + p].y;
-    for (int p = 0; p < 3; ++p) red[threadIdx.x * 3 + p] = s3[p];
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int p = 0; p < 3; ++p) red[threadIdx.x * 3 + p] += red[(threadIdx.x + st) * 3 + p];
         __syncthreads();
     }
-    if (threadIdx.x < 3) diag[3 * a + threadIdx.x] += red[threadIdx.x];
+    double s3[3] = {0.0, 0.0, 0.0};
+    for (int i = tid; i < np; i += nt)
+        for (int p = 0; p < 3; ++p) s3[p] += acc[i * 3 + p].x * acc[i * 3 + p].x + acc[i * 3 + p].y * acc[i * 3 + p].y;
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+        for (int o = 32; o > 0; o >>= 1) s3[p] += __shfl_xor(s3[p], o);
+    if ((tid & 63) == 0)
+        for (int p = 0; p < 3; ++p) red[p][tid >> 6] = s3[p];
+    __syncthreads();
+    if (tid < 3) {
+        double t = 0.0;
+        for (int w = 0; w < (nt + 63) / 64; ++w) t += red[tid][w];
+        diag[3 * a + tid] += t;
+    }
 }
 
 // ADJFLAM_DIST: gprod(mygrad(c)) -= Re(conj(F) lambda) over column c
