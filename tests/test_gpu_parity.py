@@ -361,10 +361,11 @@ def test_peff():
     assert np.count_nonzero(ref) > 0
 
 
-def test_gn_fv_matches_operator_difference():
-    """F v = dS/dm[v] u against a central difference of the oracle's FP64
-    operator (S is cubic in the step through rho Vp^2, so the step is small)."""
-    name = "tiny_layered"
+@pytest.mark.parametrize("name", ["tiny_layered", "p5", "p6"])
+def test_gn_fv_matches_operator_difference(name):
+    """F v = dS/dm[v] u (the batched product on derivative blocks) against a
+    central difference of the oracle's FP64 operator (S is cubic in the step
+    through rho Vp^2, so the step is small); p4, p5 and p6."""
     mi = _mi(name)
     m, mod, pml = mesh_case(name)
     g = oracle_graph(name)
