@@ -384,6 +384,36 @@ def test_gn_fv_matches_operator_difference(name):
     assert relerr(fv, ref) < 1e-9
 
 
+@pytest.mark.parametrize("name", ["tiny_layered", "p6", "bench"])
+def test_gn_adjoint_identity(name):
+    """J v (f25_gn_fv: the batched product on derivative blocks) and J^H w
+    (f25_gn_fhw: the gradient kernel) are adjoint:
+    sum_t Re <w_t, J_t v> = v . (J^H w), over three (omega, ky) pairs; on
+    small meshes and at the benchmark size (200 x 100, p = 4)."""
+    if name == "bench":
+        from fem25_amd.api import MeshInfo
+        m = cf.structured_mesh(200, 100, 4)
+        mi = MeshInfo(m)
+        mi.set_model(cf.layered_model(m, seed=2))
+        ndof = mi.ndof
+    else:
+        mi = _mi(name)
+        m, _, _ = mesh_case(name)
+        ndof = oracle_graph(name).ndof
+    om, ky, f, py = pairs(1, 3)
+    rng = np.random.default_rng(61)
+    v = rng.standard_normal((m.nnpg, 3)) * np.array([50.0, 30.0, 20.0])
+    u = cf.seeded_complex(ndof, batch=om.size, seed=62)
+    w = cf.seeded_complex(ndof, batch=om.size, seed=63)
+    fv = mi.gn_fv(om, ky, u, v)
+    g = mi.gn_fhw(om, ky, u, w)
+    lhs = float(sum(np.vdot(w[t], fv[t]).real for t in range(om.size)))
+    rhs = float(np.sum(v * g))
+    scale = float(sum(np.linalg.norm(w[t]) * np.linalg.norm(fv[t]) for t in range(om.size)))
+    assert abs(lhs) > 1e-6 * scale          # a non-trivial inner product
+    assert abs(lhs - rhs) < 1e-11 * scale, (lhs, rhs)
+
+
 @pytest.mark.parametrize("name,stride", [("tiny_layered", 1), ("p5", 1), ("p6", 1), ("pml", 2), ("p2_pml", 1)])
 def test_gn_diag_matches_operator_columns(name, stride):
     """Pseudo-Hessian diagonal sum_t |dS_t/dm_{a,p} u_t|^2 against columns
