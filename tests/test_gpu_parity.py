@@ -414,6 +414,36 @@ def test_gn_adjoint_identity(name):
     assert abs(lhs - rhs) < 1e-11 * scale, (lhs, rhs)
 
 
+@pytest.mark.parametrize("name", ["p6", "bench"])
+def test_gn_diag_against_jv_columns(name):
+    """Each pseudo-Hessian diagonal entry (k_fcols_sf + k_fdiag) equals
+    sum_t |J_t e_{a,p}|^2 with J e from f25_gn_fv (the batched product on
+    derivative blocks): two independent kernels, on a p6 test mesh and at
+    the benchmark size (200 x 100, p = 4), for a few anchors."""
+    if name == "bench":
+        from fem25_amd.api import MeshInfo
+        m = cf.structured_mesh(200, 100, 4)
+        mi = MeshInfo(m)
+        mi.set_model(cf.layered_model(m, seed=2))
+        ndof = mi.ndof
+        picks = [(m.nnpg // 2 + 7, 0), (1234, 1), (m.nnpg - 3, 2), (0, 0)]
+    else:
+        mi = _mi(name)
+        m, _, _ = mesh_case(name)
+        ndof = oracle_graph(name).ndof
+        picks = [(a, p) for a in (0, 6, m.nnpg // 2, m.nnpg - 1) for p in range(3)]
+    om, ky, f, py = pairs(1, 2)
+    u = cf.seeded_complex(ndof, batch=om.size, seed=64)
+    d = mi.gn_diag(om, ky, u)
+    for a, p in picks:
+        v = np.zeros((m.nnpg, 3))
+        v[a, p] = 1.0
+        fv = mi.gn_fv(om, ky, u, v)
+        ref = float(sum(np.vdot(fv[t], fv[t]).real for t in range(om.size)))
+        assert ref > 0
+        assert abs(d[a, p] - ref) < 1e-10 * ref, (a, p, d[a, p], ref)
+
+
 @pytest.mark.parametrize("name,stride", [("tiny_layered", 1), ("p5", 1), ("p6", 1), ("pml", 2), ("p2_pml", 1)])
 def test_gn_diag_matches_operator_columns(name, stride):
     """Pseudo-Hessian diagonal sum_t |dS_t/dm_{a,p} u_t|^2 against columns
