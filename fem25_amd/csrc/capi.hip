@@ -1450,12 +1450,13 @@ static int ensure_dab_part(f25_mesh *m) {
 }
 
 // LDS of k_dab_sf (independent of the number of vectors): the element's
-// vectors (room for two), the geometry per quadrature point, T1/T2, the
-// projection fields A, B, C and their first-pass sums
+// vectors (room for two), the geometry per quadrature point, T1/T2 (whose
+// space the projection's first-pass sums reuse) and the projection fields
+// A, B, C
 static size_t dab_lds(const f25_mesh *m) {
     const int nee = 3 * m->tab.nen, nq = m->tab.nq, q1 = m->tab.nq1, n1 = m->tab.nl;
     return (size_t)2 * nee * sizeof(double2) + 5 * (size_t)nq * sizeof(double) +
-           ((size_t)3 * q1 * n1 * 2 + 9 * (size_t)nq + 3 * (size_t)q1 * n1 * 3) * sizeof(double2);
+           ((size_t)3 * q1 * n1 * 3 + 9 * (size_t)nq) * sizeof(double2);
 }
 // threads of a k_dab_sf workgroup: five waves at p6 (Q1 = 17)
 static int dab_threads(const f25_mesh *m) { return m->tab.nq1 > 16 ? 320 : 256; }

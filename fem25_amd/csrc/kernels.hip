@@ -1949,8 +1949,10 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
     double2 *su = reinterpret_cast<double2 *>(sm);          // [2][nee]
     double *sG = reinterpret_cast<double *>(su + 2 * nee);  // [QQ][5]: rx, sx, rz, sz, D
     double2 *sT = reinterpret_cast<double2 *>(sG + 5 * QQ); // [3][Q1][N1][2]: T1, T2
-    double2 *sH = sT + 3 * Q1 * N1 * 2;                     // [3 comp][3: A, B, C][QQ]
-    double2 *sP = sH + 9 * QQ;                              // [3][Q1][N1][3]: PA, PB, PC
+    // PA, PB, PC [3][Q1][N1][3] take T1/T2's place: T1/T2 are last read in
+    // the point stage, PA.. first written in the projection after it
+    double2 *sP = sT;
+    double2 *sH = sT + 3 * Q1 * N1 * 3;                     // [3 comp][3: A, B, C][QQ]
     const int it = blockIdx.x, tid = threadIdx.x;
     if (it >= nitems) return;
     const int e = item_elem[it];
