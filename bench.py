@@ -1,14 +1,18 @@
 """Throughput of the batched K(omega, ky) u product (and the adjoint gradient)
-on one MI355X, on the 200 x 100 element, order-4 workload with 8 frequencies
-x 16 out-of-plane wavenumbers = 128 operator applications per step.
+on one MI355X.  Default workload: the order-6 Gauss-Newton configuration,
+400 x 200 elements, GLL order 6 (8,650,803 DOFs), 6 frequencies x 32
+out-of-plane wavenumbers = 192 operator applications per step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5p6|c2|...] [--skip-cpu]
 
 Prints ONE JSON line on stdout; progress goes to stderr.  A step is one
-application of all 128 S(omega_b, ky_b) to their vectors (inputs resident in
+application of all 192 S(omega_b, ky_b) to their vectors (inputs resident in
 HBM, layout [dof][pair]).  The CPU baseline is the reference's own Fortran
 (ASMBLE_DRIVER, compiled from source into oracle/_ref) plus the CSR product,
-timed on one (omega, ky) pair of the same mesh on one host core.
+one single-threaded process per host core on disjoint (omega, ky) pairs of a
+40 x 20 element piece of the same mesh (same element size and order), the
+per-element cost extrapolated linearly to the full mesh; it runs before the
+GPU is touched.
 """
 from __future__ import annotations
 
@@ -41,7 +45,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--config", default="c5p6")
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--skip-extras", action="store_true", help="only the timed FP64 product")
     ap.add_argument("--copy-calib", action="store_true",
@@ -52,6 +56,13 @@ def main() -> int:
               "(omega, ky) problems would be split across GPUs with one gradient all-reduce)",
               file=sys.stderr)
         return 2
+
+    # the CPU leg first, while no GPU state exists in this process (its
+    # worker processes are started from here)
+    cpu = None
+    if not args.skip_cpu:
+        cpu = cpu_baseline(args.config)
+        log(f"cpu baseline: {cpu}")
 
     import torch
 
@@ -115,11 +126,6 @@ def main() -> int:
     if not args.skip_extras:
         extra = extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g)
         log(json.dumps(extra))
-
-    cpu = None
-    if not args.skip_cpu:
-        cpu = cpu_baseline(mesh, model, mi, float(freq[3]), float(py[3]))
-        log(f"cpu baseline: {cpu}")
 
     traffic, traffic_src = profiled_traffic(args.config)
     out = {
@@ -354,12 +360,19 @@ def cfmat_host(mi, inv, freq, py, wave):
     return api.cfmat_dist(inv.nz_fdist, mi.ndof, freq, py, wave, mi, inv)
 
 
+# rocprofv3 FETCH_SIZE / WRITE_SIZE summaries of the product, per workload
+TRAFFIC_PROFILES = {
+    "c2": "k_apply_v_f64_p4_200x100_128pairs.md",
+    "c5p6": "k_apply_v_f64_p6_400x200_192pairs.md",
+}
+
+
 def profiled_traffic(config: str):
     """HBM bytes per k_apply_v launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (None when there is none)."""
-    if config != "c2":
+    name = TRAFFIC_PROFILES.get(config)
+    if name is None:
         return None, None
-    name = "k_apply_v_f64_p4_200x100_128pairs.md"
     try:
         with open(os.path.join(ROOT, "profiles", name)) as f:
             for line in f:
@@ -370,32 +383,82 @@ def profiled_traffic(config: str):
     return None, None
 
 
-def cpu_baseline(mesh, model, mi, freq, py):
-    """The reference path on one (omega, ky) pair: ASMBLE_DRIVER (compiled
-    reference Fortran) + CSR SpMV, single core, full mesh."""
+# the CPU sample: a piece of the workload's mesh with the same element size
+# and order, each worker process one (omega, ky) pair after another
+CPU_SAMPLE = (40, 20)
+CPU_PAIRS_PER_WORKER = 2
+
+
+def _cpu_worker(args):
+    """One single-threaded process: ASMBLE_DRIVER (compiled reference Fortran,
+    or the C restatement where it is not built) and the CSR product for its
+    (omega, ky) pairs on the sample mesh; returns seconds and counts."""
+    nx, nz, nord, width, depth, pairs, seed = args
+    import oracle
+    from fem25_amd import configs as cf
+    mesh = cf.structured_mesh(nx, nz, nord, width=width, depth=depth)
+    model = cf.layered_model(mesh, seed=2)
+    g = oracle.graph(mesh)
+    x = cf.seeded_complex(g.ndof, seed=seed).astype(np.complex64)
+    kind = "reference" if oracle.ref_available() else "port"
+    ta = tm = 0.0
+    for freq, py in pairs:
+        t = time.perf_counter()
+        if kind == "reference":
+            s = oracle.ref_asmble(mesh, model, g, freq, py, cf.PML())
+        else:
+            s = oracle.asmble_cdist(mesh, model, g, freq, py, cf.PML())
+        ta += time.perf_counter() - t
+        t = time.perf_counter()
+        oracle.spmv_c(g, s, x)
+        tm += time.perf_counter() - t
+    return {"kind": kind, "asm_s": ta, "spmv_s": tm, "npair": len(pairs), "nelem": mesh.nelem,
+            "ndof": g.ndof}
+
+
+def cpu_baseline(config: str):
+    """The reference path (ASMBLE_DRIVER + CSR SpMV per (omega, ky) pair) on
+    the host cores, one single-threaded process per core over disjoint pairs
+    (the reference's frequency-group parallelism, bielak25.f90:360), on a
+    40 x 20 element piece of the workload's mesh; the cost of a pair is linear
+    in the element count, so the full mesh's rate is extrapolated from the
+    per-element time."""
+    import multiprocessing as mp
+    from fem25_amd import configs as cf
+    shape = cf.CONFIGS[config]
+    nx, nz = CPU_SAMPLE
+    nx, nz = min(nx, shape["nx"]), min(nz, shape["nz"])
+    width, depth = 80e3 * nx / shape["nx"], 40e3 * nz / shape["nz"]
+    vs_min = 6300.0 / math.sqrt(3.0)
+    _, _, freq, py = cf.freq_ky_batch(shape["nfreq"], shape["nky"], 0.02, 0.1, vs_min)
+    cores = max(1, min(16, os.cpu_count() or 1))
+    jobs = []
+    for w in range(cores):
+        idx = [(w * CPU_PAIRS_PER_WORKER + k) % freq.size for k in range(CPU_PAIRS_PER_WORKER)]
+        jobs.append((nx, nz, shape["nord"], width, depth, [(float(freq[i]), float(py[i])) for i in idx], 5 + w))
     try:
-        import oracle
-        from fem25_amd import configs as cf
+        t = time.perf_counter()
+        with mp.get_context("spawn").Pool(cores) as pool:
+            res = pool.map(_cpu_worker, jobs)
+        wall = time.perf_counter() - t
     except Exception as e:  # pragma: no cover
         return {"error": str(e)}
-    g = mi.graph()
-    x = cf.seeded_complex(g.ndof, seed=5).astype(np.complex64)
-    og = oracle.Graph(g.nnp, g.ndof, g.nz, g.ien, g.lm, g.irptr, g.jcptr, g.cnp)
-    kind = "reference" if oracle.ref_available() else "port"
-    t = time.perf_counter()
-    if kind == "reference":
-        s = oracle.ref_asmble(mesh, model, og, freq, py, cf.PML())
-    else:
-        s = oracle.asmble_cdist(mesh, model, og, freq, py, cf.PML())
-    ta = time.perf_counter() - t
-    t = time.perf_counter()
-    oracle.spmv_c(og, s, x)
-    tm = time.perf_counter() - t
-    tot = ta + tm
-    return {"value": round(g.ndof / tot / 1e9, 6), "unit": "Gdof/s", "cores": 1, "kind": kind,
-            "sample": f"one (omega,ky) application on the full {mesh.nelem}-element mesh: "
-                      f"ASMBLE_DRIVER {ta:.2f} s + CSR SpMV {tm:.3f} s",
-            "seconds": round(tot, 3)}
+    # per (element, pair) seconds of one core, averaged over the workers
+    per = float(np.mean([(r["asm_s"] + r["spmv_s"]) / (r["npair"] * r["nelem"]) for r in res]))
+    n1 = shape["nord"] + 1
+    ndof_full = 3 * ((shape["nx"] * shape["nord"] + 1) * (shape["nz"] * shape["nord"] + 1))
+    nelem_full = shape["nx"] * shape["nz"]
+    t_pair = per * nelem_full                       # one core, one pair, full mesh
+    value = cores * ndof_full / t_pair / 1e9
+    asm = float(np.mean([r["asm_s"] / r["npair"] for r in res]))
+    spmv = float(np.mean([r["spmv_s"] / r["npair"] for r in res]))
+    return {"value": round(value, 7), "unit": "Gdof/s", "cores": cores, "kind": res[0]["kind"],
+            "sample": f"{cores} single-threaded processes x {CPU_PAIRS_PER_WORKER} (omega,ky) pairs on a "
+                      f"{nx}x{nz}-element piece of the mesh (p={shape['nord']}, {res[0]['ndof']} DOFs): "
+                      f"ASMBLE_DRIVER {asm:.2f} s + CSR SpMV {spmv:.3f} s per pair, {per * 1e3:.3f} ms per "
+                      f"element and pair, extrapolated to {nelem_full} elements "
+                      f"({t_pair:.0f} s per pair per core)",
+            "seconds_wall": round(wall, 1), "nen": n1 * n1}
 
 
 if __name__ == "__main__":

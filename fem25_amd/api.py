@@ -38,6 +38,26 @@ def _ready(*tensors) -> None:
     torch.cuda.current_stream().synchronize()
 
 
+def _expect(t, dtype, numel: int, name: str) -> None:
+    """A device argument must have the element type and the exact number of
+    values the kernels index (a short buffer would be read or written past
+    its end on the device)."""
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {dtype}")
+    if t.numel() != numel:
+        raise ValueError(f"{name}: {t.numel()} values, expected {numel}")
+
+
+def _model_arrays(model: Model, nnpg: int):
+    """DENS(NNPG) and ECOEFF(NNPG,2) as contiguous FP64 arrays of the mesh's size."""
+    dens = np.ascontiguousarray(model.dens, np.float64)
+    ec = np.asfortranarray(np.asarray(model.ecoeff, np.float64))
+    if dens.shape != (nnpg,) or ec.shape != (nnpg, 2):
+        raise ValueError(f"model arrays must be dens ({nnpg},) and ecoeff ({nnpg}, 2); got "
+                         f"{dens.shape} and {ec.shape}")
+    return dens, ec
+
+
 @dataclasses.dataclass
 class Graph:
     """GLL numbering in the reference's conventions (1-based)."""
@@ -121,13 +141,14 @@ class MeshInfo:
     # -- model --------------------------------------------------------------
     def set_model(self, model: Model):
         self.model = model
-        ec = np.asfortranarray(model.ecoeff)
-        dens = np.ascontiguousarray(model.dens)
+        dens, ec = _model_arrays(model, self.mesh.nnpg)
         check(_lib.load().f25_set_model(self.h, _p(dens, dp), ec.ctypes.data_as(dp)),
               "f25_set_model")
         if getattr(model, "qp", None) is not None:    # dispersive medium (LDISP)
             qp = np.ascontiguousarray(model.qp, np.float64)
             qs = np.ascontiguousarray(model.qs, np.float64)
+            if qp.shape != (self.mesh.nnpg,) or qs.shape != (self.mesh.nnpg,):
+                raise ValueError("qp and qs must hold one value per anchor")
             check(_lib.load().f25_set_attenuation(self.h, _p(qp, dp), _p(qs, dp), float(model.freq0)),
                   "f25_set_attenuation")
         else:
@@ -145,6 +166,9 @@ class MeshInfo:
         """out: torch complex64/complex128 tensor of NZ values on the GPU."""
         import torch
         _ready(out)
+        if out.dtype not in (torch.complex64, torch.complex128):
+            raise ValueError(f"out: dtype {out.dtype}, expected complex64 or complex128")
+        _expect(out, out.dtype, self.nz, "out")
         prec = _lib.F25_C128 if out.dtype == torch.complex128 else _lib.F25_C64
         check(_lib.load().f25_asmble_dev(self.h, cftype, freq, py, prec, C.c_void_p(out.data_ptr())),
               "f25_asmble_dev")
@@ -152,6 +176,11 @@ class MeshInfo:
     def csr_apply_dev(self, s, x, y) -> None:
         import torch
         _ready(s, x, y)
+        if s.dtype not in (torch.complex64, torch.complex128):
+            raise ValueError(f"s: dtype {s.dtype}, expected complex64 or complex128")
+        _expect(s, s.dtype, self.nz, "s")
+        _expect(x, s.dtype, self.ndof, "x")
+        _expect(y, s.dtype, self.ndof, "y")
         prec = _lib.F25_C128 if s.dtype == torch.complex128 else _lib.F25_C64
         check(_lib.load().f25_csr_apply_dev(self.h, prec, C.c_void_p(s.data_ptr()),
                                             C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())),
@@ -160,9 +189,15 @@ class MeshInfo:
     # -- batched element-block product ----------------------------------------
     def apply_batch_dev(self, omega, ky, x, y, mode: int = _lib.F25_APPLY_F64) -> None:
         """x, y: torch tensors [ndof, npair] (complex128, or complex64 for F32)."""
+        import torch
         _ready(x, y)
         om = np.ascontiguousarray(omega, np.float64)
         kv = np.ascontiguousarray(ky, np.float64)
+        if om.shape != kv.shape or om.ndim != 1:
+            raise ValueError("omega and ky must be 1-D arrays of one length")
+        dt = torch.complex64 if (mode & ~_lib.F25_APPLY_GRAPH) == _lib.F25_APPLY_F32 else torch.complex128
+        _expect(x, dt, self.ndof * om.size, "x")
+        _expect(y, dt, self.ndof * om.size, "y")
         check(_lib.load().f25_apply_batch_dev(self.h, om.size, _p(om, dp), _p(kv, dp), mode,
                                               C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())),
               "f25_apply_batch_dev")
@@ -197,9 +232,15 @@ class MeshInfo:
 
     def gradient_dev(self, omega, ky, u, lam, g) -> None:
         """u, lam: torch complex128 [ntrip, ndof] on the GPU; g: torch float64 (nnpg, 3), accumulated."""
+        import torch
         _ready(u, lam, g)
         om = np.ascontiguousarray(omega, np.float64)
         kv = np.ascontiguousarray(ky, np.float64)
+        if om.shape != kv.shape or om.ndim != 1:
+            raise ValueError("omega and ky must be 1-D arrays of one length")
+        _expect(u, torch.complex128, self.ndof * om.size, "u")
+        _expect(lam, torch.complex128, self.ndof * om.size, "lam")
+        _expect(g, torch.float64, self.mesh.nnpg * 3, "g")
         check(_lib.load().f25_gradient_dev(self.h, om.size, _p(om, dp), _p(kv, dp),
                                            C.c_void_p(u.data_ptr()), C.c_void_p(lam.data_ptr()),
                                            C.c_void_p(g.data_ptr())), "f25_gradient_dev")

@@ -129,6 +129,10 @@ struct f25_mesh_s {
     double *d_part = nullptr;
     int *d_mconn = nullptr;
     int *d_pmap = nullptr;   // pseudo-Hessian patch positions
+    // anchors of PML elements: parameters held fixed by the gradient-type
+    // products (the reference inverts only anchors of interior elements)
+    std::vector<char> fixa;
+    char *d_fixa = nullptr;
     int ncon_anchor = 0;
     // inversion (CFMAT/ADJFLAM)
     int *d_mygrad = nullptr, *d_jcsc = nullptr, *d_icsc = nullptr;
@@ -224,6 +228,11 @@ static int upload_mesh(f25_mesh *m) {
             mc[(size_t)f[a]++ * d.nnpg + a] = e + 1;
         }
     TRY(upload(&m->d_mconn, mc.data(), mc.size(), s));
+    m->fixa.assign(d.nnpg, 0);
+    for (int e = 0; e < d.nelem; ++e)
+        if (m->cdom[e] == 'A')
+            for (int k = 0; k < 4; ++k) m->fixa[m->ieng[4 * e + k] - 1] = 1;
+    TRY(upload(&m->d_fixa, m->fixa.data(), m->fixa.size(), s));
     TRY(dalloc(&m->d_part, (size_t)d.nelem * 12));
     (void)nen;
     HIPCHK(hipStreamSynchronize(s));
@@ -341,6 +350,19 @@ extern "C" int f25_mesh_create(const f25_mesh_desc *desc, f25_mesh **out) {
     return 0;
 }
 
+// A caller's 1-based CSR pattern: IRPTR(1) = 1, ascending row pointers,
+// column numbers in 1..ncol (the device kernels index x and the output by
+// them without further checks).
+static int check_csr(int nrow, int ncol, const int *irptr, const int *jcptr) {
+    if (irptr[0] != 1) FAIL(F25_ERR_ARG, "IRPTR(1) must be 1");
+    for (int i = 0; i < nrow; ++i)
+        if (irptr[i + 1] < irptr[i]) FAIL(F25_ERR_ARG, "IRPTR not ascending");
+    const int64_t nz = (int64_t)irptr[nrow] - 1;
+    for (int64_t k = 0; k < nz; ++k)
+        if (jcptr[k] < 1 || jcptr[k] > ncol) FAIL(F25_ERR_ARG, "JCPTR entry out of range");
+    return 0;
+}
+
 extern "C" int f25_mesh_create_with_graph(const f25_mesh_desc *desc, int ndof, const int *lm,
                                           const int *irptr, const int *jcptr, f25_mesh **out) {
     if (!out || !lm || ndof <= 0) FAIL(F25_ERR_ARG, "invalid graph arguments");
@@ -368,6 +390,8 @@ extern "C" int f25_mesh_create_with_graph(const f25_mesh_desc *desc, int ndof, c
             }
         }
     if (irptr) {
+        if (!jcptr) FAIL(F25_ERR_ARG, "JCPTR missing with IRPTR");
+        TRY(check_csr(ndof, ndof, irptr, jcptr));
         m->g.irptr.resize(ndof + 1);
         for (int i = 0; i <= ndof; ++i) m->g.irptr[i] = (int64_t)irptr[i] - 1;
         m->g.jcptr.assign(jcptr, jcptr + m->g.irptr[ndof]);
@@ -390,7 +414,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_blk); dfree(m->d_blk32); dfree(m->d_blk_hi); dfree(m->d_blk_lo); dfree(m->d_colmax); dfree(m->d_dblk); dfree(m->d_irptr);
     dfree(m->d_jcptr); dfree(m->d_comp); dfree(m->d_v0); dfree(m->d_v1); dfree(m->d_v2);
     dfree(m->d_kind); dfree(m->d_cbase);
-    dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_pmap); dfree(m->d_qp); dfree(m->d_qs);
+    dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_pmap); dfree(m->d_fixa); dfree(m->d_qp); dfree(m->d_qs);
     dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc);
     dfree(m->d_ew);
     dfree(m->d_omky);
@@ -1062,7 +1086,7 @@ static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d
     HIPCHK(hipEventRecord(m->ev1, m->st));
     hipLaunchKernelGGL(k_grad_gather, dim3((m->d.nnpg + 255) / 256), dim3(256), 0, m->st, m->d.nnpg,
                        m->ncon_anchor, m->d_mconn, m->d_ieng, m->d_part, m->d_rho, m->d_lam, m->d_mu,
-                       chain, g);
+                       m->d_fixa, chain, g);
     HIPCHK(hipGetLastError());
     TRY(check_err(m));
     float ms = 0.f;
@@ -1898,6 +1922,25 @@ static int gn_check(int nnpgl, int na35, const int *mygrad, int njac) {
     return 0;
 }
 
+// J p in two deterministic passes: column chunks of J_j into partial sums
+// (at most 256 chunks: part holds nch * pt double2), then the chunks added
+// in order.
+static void gn_jp_chunks(int nnpgl, int &nch, int &cch) {
+    nch = std::min(256, (nnpgl + 63) / 64);
+    cch = (nnpgl + nch - 1) / nch;
+    nch = (nnpgl + cch - 1) / cch;
+}
+
+static void launch_gn_jp(hipStream_t st, int nnpgl, int njac, int64_t pt, const int *dn, const int64_t *djo,
+                         const int64_t *dpo, const int *dmy, const float2 *jac, const double *p, double2 *part,
+                         double2 *jp) {
+    int nch, cch;
+    gn_jp_chunks(nnpgl, nch, cch);
+    hipLaunchKernelGGL(k_gn_jp, dim3(nch, njac), dim3(256), 0, st, nnpgl, cch, pt, dn, djo, dpo, dmy, jac, p, part);
+    const int nb = (int)std::min<int64_t>((pt + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_gn_jsum, dim3(std::max(nb, 1)), dim3(256), 0, st, nch, pt, part, jp);
+}
+
 static int gnmfree_core(f25_mesh *m, int njac, const int *nobs, const double *alpha, const float2 *jac,
                         const double *p, double *z) {
     const int n = m->inv.nnpgl;
@@ -1907,18 +1950,18 @@ static int gnmfree_core(f25_mesh *m, int njac, const int *nobs, const double *al
     int *dn = nullptr;
     int64_t *djo = nullptr, *dpo = nullptr;
     double *da = nullptr;
-    double2 *djp = nullptr;
+    double2 *djp = nullptr, *dpart = nullptr;
+    int nch, cch;
+    gn_jp_chunks(n, nch, cch);
     int r = upload(&dn, nobs, (size_t)njac, m->st);
     if (!r) r = upload(&djo, joff.data(), (size_t)njac, m->st);
     if (!r) r = upload(&dpo, poff.data(), (size_t)njac, m->st);
     if (!r) r = upload(&da, alpha, (size_t)njac, m->st);
     if (!r) r = dalloc(&djp, (size_t)std::max<int64_t>(pt, 1));
+    if (!r) r = dalloc(&dpart, (size_t)nch * std::max<int64_t>(pt, 1));
     if (!r) {
-        fill_async(djp, (size_t)std::max<int64_t>(pt, 1), make_double2(0.0, 0.0), m->st);
-        const int cch = 64;
         (void)hipEventRecord(m->ev0, m->st);
-        hipLaunchKernelGGL(k_gn_jp, dim3((n + cch - 1) / cch, njac), dim3(256), 0, m->st, n, cch, dn, djo, dpo,
-                           m->d_mygrad, jac, p, djp);
+        launch_gn_jp(m->st, n, njac, pt, dn, djo, dpo, m->d_mygrad, jac, p, dpart, djp);
         hipLaunchKernelGGL(k_gn_z, dim3((n + 3) / 4), dim3(256), 0, m->st, n, njac, dn, djo, dpo, da,
                            m->d_mygrad, jac, djp, z);
         (void)hipEventRecord(m->ev1, m->st);
@@ -1928,7 +1971,7 @@ static int gnmfree_core(f25_mesh *m, int njac, const int *nobs, const double *al
         float ms = 0.f;
         if (!r && hipEventElapsedTime(&ms, m->ev0, m->ev1) == hipSuccess) record_ms("gnmfree", ms);
     }
-    dfree(dn); dfree(djo); dfree(dpo); dfree(da); dfree(djp);
+    dfree(dn); dfree(djo); dfree(dpo); dfree(da); dfree(djp); dfree(dpart);
     return r;
 }
 
@@ -1947,7 +1990,9 @@ extern "C" int f25_gn_jp(int nnpgl, int na35, const int *mygrad, int njac, const
     int64_t *djo = nullptr, *dpo = nullptr;
     float2 *dj = nullptr;
     double *dp_ = nullptr;
-    double2 *djp = nullptr;
+    double2 *djp = nullptr, *dpart = nullptr;
+    int nch, cch;
+    gn_jp_chunks(nnpgl, nch, cch);
     int r = upload(&dmy, mygrad, (size_t)nnpgl, ts.s);
     if (!r) r = upload(&dn, nobs, (size_t)njac, ts.s);
     if (!r) r = upload(&djo, joff.data(), (size_t)njac, ts.s);
@@ -1955,17 +2000,15 @@ extern "C" int f25_gn_jp(int nnpgl, int na35, const int *mygrad, int njac, const
     if (!r) r = upload(&dj, (const float2 *)jac, (size_t)jt, ts.s);
     if (!r) r = upload(&dp_, p, (size_t)na35, ts.s);
     if (!r) r = dalloc(&djp, (size_t)std::max<int64_t>(pt, 1));
+    if (!r) r = dalloc(&dpart, (size_t)nch * std::max<int64_t>(pt, 1));
     if (!r) {
-        fill_async(djp, (size_t)std::max<int64_t>(pt, 1), make_double2(0.0, 0.0), ts.s);
-        const int cch = 64;
-        hipLaunchKernelGGL(k_gn_jp, dim3((nnpgl + cch - 1) / cch, njac), dim3(256), 0, ts.s, nnpgl, cch, dn, djo,
-                           dpo, dmy, dj, dp_, djp);
+        launch_gn_jp(ts.s, nnpgl, njac, pt, dn, djo, dpo, dmy, dj, dp_, dpart, djp);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess && pt > 0) e = hipMemcpyAsync(jp, djp, (size_t)pt * sizeof(double2), hipMemcpyDeviceToHost, ts.s);
         if (e == hipSuccess) e = hipStreamSynchronize(ts.s);
         if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
     }
-    dfree(dmy); dfree(dn); dfree(djo); dfree(dpo); dfree(dj); dfree(dp_); dfree(djp);
+    dfree(dmy); dfree(dn); dfree(djo); dfree(dpo); dfree(dj); dfree(dp_); dfree(djp); dfree(dpart);
     return r;
 }
 
@@ -2047,7 +2090,9 @@ extern "C" int f25_gn_fv(f25_mesh *m, int ntrip, const double *omega, const doub
     std::vector<double> dr(n), dl(n), dm(n);
     for (int a = 0; a < n; ++a) {
         double r = m->rho[a], vp = std::sqrt((m->lam[a] + 2.0 * m->mu[a]) / r), vs = std::sqrt(m->mu[a] / r);
-        double dvp = v[3 * a], dvs = v[3 * a + 1], drho = v[3 * a + 2];
+        // anchors of PML elements hold fixed parameters: their perturbation is ignored
+        const double keep = m->fixa[a] ? 0.0 : 1.0;
+        double dvp = keep * v[3 * a], dvs = keep * v[3 * a + 1], drho = keep * v[3 * a + 2];
         dr[a] = drho;
         dl[a] = drho * (vp * vp - 2.0 * vs * vs) + r * (2.0 * vp * dvp - 4.0 * vs * dvs);
         dm[a] = drho * vs * vs + 2.0 * r * vs * dvs;
@@ -2173,7 +2218,7 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
                            m->d_xl, m->d_zl, m->d_cdom, m->d_lm, m->d_rho, m->d_lam, m->d_mu, omega[t], ky[t],
                            du + ndof * t, dcols, m->d_err);
         hipLaunchKernelGGL(k_fdiag, dim3(nnpg), dim3(nt2), lds2, m->st, nnpg, m->ncon_anchor, nee,
-                           m->d_mconn, m->d_ieng, m->d_pmap, dcols, dd);
+                           m->d_mconn, m->d_ieng, m->d_pmap, m->d_fixa, dcols, dd);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
     }

@@ -1467,12 +1467,15 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
 }
 
 // per-anchor sum of element partials and chain rule to (Vp, Vs, rho)
+// Anchors of PML elements (fixa[a] != 0) hold fixed parameters: their
+// entries receive nothing.
 __global__ void k_grad_gather(int nnpg, int ncon, const int *__restrict__ mconn,
                               const int *__restrict__ ieng, const double *__restrict__ part,
                               const double *__restrict__ rho, const double *__restrict__ lam,
-                              const double *__restrict__ mu, int chain, double *__restrict__ g) {
+                              const double *__restrict__ mu, const char *__restrict__ fixa, int chain,
+                              double *__restrict__ g) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nnpg) return;
+    if (a >= nnpg || fixa[a]) return;
     double gl = 0, gm = 0, gr = 0;
     for (int c = 0; c < ncon; ++c) {
         int e = mconn[(size_t)c * nnpg + a];
@@ -1772,11 +1775,12 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
 // element and its corner come from mconn.  diag[3a + p] += |column|^2.
 __global__ __launch_bounds__(256) void k_fdiag(int nnpg, int ncon, int nee, const int *__restrict__ mconn,
                                                 const int *__restrict__ ieng, const int *__restrict__ pmap,
-                                                const double2 *__restrict__ cols, double *__restrict__ diag) {
+                                                const char *__restrict__ fixa, const double2 *__restrict__ cols,
+                                                double *__restrict__ diag) {
     extern __shared__ double sm[];
     __shared__ double red[3][16];
     const int a = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
-    if (a >= nnpg) return;
+    if (a >= nnpg || fixa[a]) return;   // anchors of PML elements: fixed parameters
     const int np = ncon * nee;
     double2 *acc = reinterpret_cast<double2 *>(sm);   // np * 3
     for (int i = tid; i < np * 3; i += nt) acc[i] = make_double2(0.0, 0.0);
@@ -2168,18 +2172,19 @@ __global__ __launch_bounds__(256) void k_pchadj(int nnpgl, int nrhs, const int *
 // GNMFREE's products (gnmfree.f90:185-212) over a batch of Jacobians with
 // uneven observation counts, J_j stored column-major (nobs_j x nnpgl) at
 // jac + joff[j]:  JP_j(obs) = sum_c J_j(obs, c) p(MYGRAD(c))  (FP64).
-// Grid (column chunks, j); each block sums its chunk of columns for every
-// observation of J_j and adds the partial sums into jp (zeroed beforehand).
-__global__ void k_gn_jp(int nnpgl, int cch, const int *__restrict__ nobs, const int64_t *__restrict__ joff,
-                        const int64_t *__restrict__ poff, const int *__restrict__ mygrad,
-                        const float2 *__restrict__ jac, const double *__restrict__ p,
-                        double2 *__restrict__ jp) {
+// Grid (column chunks, j); block (ch, j) sums chunk ch of the columns for
+// every observation of J_j into part[ch * pt + poff[j] + obs].  k_gn_jsum
+// then adds the chunks in chunk order, so JP does not depend on the order
+// in which blocks run (no atomics: bit-identical results run to run).
+__global__ void k_gn_jp(int nnpgl, int cch, int64_t pt, const int *__restrict__ nobs,
+                        const int64_t *__restrict__ joff, const int64_t *__restrict__ poff,
+                        const int *__restrict__ mygrad, const float2 *__restrict__ jac,
+                        const double *__restrict__ p, double2 *__restrict__ part) {
     const int j = blockIdx.y;
     const int no = nobs[j];
     const int c0 = blockIdx.x * cch, c1 = min(nnpgl, c0 + cch);
-    if (c0 >= c1) return;
     const float2 *J = jac + joff[j];
-    double2 *out = jp + poff[j];
+    double2 *out = part + (size_t)blockIdx.x * pt + poff[j];
     for (int o = threadIdx.x; o < no; o += blockDim.x) {
         double ar = 0.0, ai = 0.0;
         // eight columns per step: eight loads in flight per lane
@@ -2204,8 +2209,20 @@ __global__ void k_gn_jp(int nnpgl, int cch, const int *__restrict__ nobs, const 
             ar += (double)v.x * pv;
             ai += (double)v.y * pv;
         }
-        atomicAdd(&out[o].x, ar);
-        atomicAdd(&out[o].y, ai);
+        out[o] = make_double2(ar, ai);
+    }
+}
+
+// jp[o] = sum over chunks ch = 0..nch-1 of part[ch * pt + o], in chunk order.
+__global__ void k_gn_jsum(int nch, int64_t pt, const double2 *__restrict__ part, double2 *__restrict__ jp) {
+    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < pt; o += (int64_t)gridDim.x * blockDim.x) {
+        double ar = 0.0, ai = 0.0;
+        for (int ch = 0; ch < nch; ++ch) {
+            const double2 v = part[(size_t)ch * pt + o];
+            ar += v.x;
+            ai += v.y;
+        }
+        jp[o] = make_double2(ar, ai);
     }
 }
 

@@ -259,6 +259,18 @@ def asmb_peff(g: Graph, cdist, ue) -> np.ndarray:
     return out
 
 
+def ref_asmb_peff(g: Graph, cdist, ue) -> np.ndarray:
+    """The reference's ASMB_PEFF (asmble.f90:318-377), one rank."""
+    out = np.zeros(g.ndof, np.complex64)
+    cdist = np.ascontiguousarray(cdist, np.complex64)
+    ue = np.ascontiguousarray(ue, np.complex64)
+    irptr = np.ascontiguousarray(g.irptr, np.int32)
+    jcptr = np.ascontiguousarray(g.jcptr, np.int32)
+    ref().ref_asmb_peff(g.ndof, g.nz, g.cnp, _p(irptr, ip), _p(jcptr, ip), _p(cdist, fp), _p(ue, fp),
+                        _p(out, fp))
+    return out
+
+
 def inv_setup(mesh, model, g, pml, nvinv=1, linrec=True, lkbdry=True) -> Inv:
     """GENGMASK + GRADPTRS + ELEM_WTS restated (one rank, no .mask file)."""
     s, keep = _asm_struct(mesh, model, g, pml)

@@ -2040,6 +2040,12 @@ int orc_grad_exact(const orc_asm *a, double freq, double py, const double comple
                 g_lmr[3 * (ie[k] - 1) + p] += creal(acc);
             }
     }
+    /* anchors of PML elements hold fixed parameters (the reference inverts
+     * no anchor of an absorbing element, gengmask.f90:110-121): no entries */
+    for (int e = 0; e < a->nelem; e++)
+        if (a->cdomain[e] == 'A')
+            for (int k = 0; k < 4; k++)
+                for (int p = 0; p < 3; p++) g_lmr[3 * (a->ieng[4 * e + k] - 1) + p] = 0.0;
     if (g_vvr) {
         for (int g = 0; g < a->nnpg; g++) {
             double l = a->ecoeff[g], m = a->ecoeff[g + a->nnpg], r = a->dens[g];

@@ -557,6 +557,51 @@ def test_gnmfree_batched_uneven():
     assert relerr(z, zref) < 1e-12
 
 
+def test_gnmfree_repeatable():
+    """On the 40 x 20 mesh (861 gradient columns, 14 column chunks of J p):
+    two calls give z bit for bit (the chunks are added in a fixed order, no
+    atomics), and z matches the oracle."""
+    name = "c1"
+    mi = _mi(name)
+    inv = mi.inv_setup(b"PP")
+    rng = np.random.default_rng(93)
+    nobs = (3 * rng.integers(20, 121, size=10)).astype(np.int32)
+    tot = int(nobs.sum()) * inv.nnpgl
+    jac = (rng.standard_normal(tot) + 1j * rng.standard_normal(tot)).astype(np.complex64)
+    alpha = rng.uniform(0.5, 2.0, size=nobs.size)
+    p = rng.standard_normal(inv.na35)
+    z1 = mi.gnmfree(nobs, alpha, jac, p)
+    z2 = mi.gnmfree(nobs, alpha, jac, p)
+    np.testing.assert_array_equal(z1, z2)
+    zref = oracle.gnmfree(nobs, alpha, inv.nnpgl, inv.mygrad, inv.na35, jac, p)
+    assert relerr(z1, zref) < 1e-12
+
+
+def test_pml_anchors_fixed():
+    """Anchors of PML elements hold fixed parameters in the gradient-type
+    products: their gradient and pseudo-Hessian entries stay zero and a
+    perturbation placed only there gives F v = 0."""
+    name = "pml"
+    mi = _mi(name)
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    in_pml = np.zeros(m.nnpg, dtype=bool)
+    for e in range(m.nelem):
+        if m.cdomain[e:e + 1] == b"A":
+            in_pml[m.ieng[e] - 1] = True
+    om, ky, f, py = pairs(1, 2)
+    u = cf.seeded_complex(g.ndof, batch=om.size, seed=31)
+    lam = cf.seeded_complex(g.ndof, batch=om.size, seed=32)
+    gd = mi.gradient(om, ky, u, lam)
+    assert np.all(gd[in_pml] == 0.0) and np.all(gd[~in_pml] != 0.0)
+    dg = mi.gn_diag(om, ky, u)
+    assert np.all(dg[in_pml] == 0.0) and np.all(dg[~in_pml] > 0.0)
+    v = np.zeros((m.nnpg, 3))
+    v[in_pml] = np.random.default_rng(33).standard_normal((int(in_pml.sum()), 3))
+    fv = mi.gn_fv(om, ky, u, v)
+    assert np.all(fv == 0.0)
+
+
 @pytest.mark.parametrize("name", ["tiny_layered", "pml"])
 def test_asmble_dispersive(name):
     """Dispersive medium (KMDAMP/CMDAMP, VCMPLX): FP64 operator against the

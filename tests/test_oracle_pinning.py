@@ -287,6 +287,45 @@ def _gnmfree_dense(nobs, alpha, nnpgl, mygrad, na35, jac, p):
 
 
 @needs_ref
+@pytest.mark.parametrize("case,freq,py", [("pml", 0.05, 0.0), ("p2_pml", 0.2, 1e-4), ("pml", 0.08, 0.6e-4)])
+def test_asmb_peff_vs_reference(case, freq, py):
+    """orc_asmb_peff against the compiled ASMB_PEFF (asmble.f90:318-377) on
+    the same CDIST and u_E: bit for bit (same COMPLEX*8 products and the
+    same row order)."""
+    m, mod, pml = mesh_case(case)
+    g = oracle_graph(case)
+    cd = oracle.asmble_cdist(m, mod, g, freq, py, pml)
+    ue = cf.seeded_complex(g.ndof, seed=41).astype(np.complex64)
+    mine = oracle.asmb_peff(g, cd, ue)
+    ref = oracle.ref_asmb_peff(g, cd, ue)
+    assert np.count_nonzero(ref) > 0
+    np.testing.assert_array_equal(mine, ref)
+
+
+@pytest.mark.parametrize("case,freq,py", [("pml", 0.05, 0.0), ("p2_pml", 0.2, 1e-4)])
+def test_asmb_peff_closed_form(case, freq, py):
+    """P_eff = -S_BE u_E on 'B' rows and +S_EB u_E on 'E' rows, 0 elsewhere,
+    with S the FP64 operator (orc_asmble_z, pinned above) restricted by the
+    DOF classes: the COMPLEX*8 result within FP32 rounding of it."""
+    m, mod, pml = mesh_case(case)
+    g = oracle_graph(case)
+    cd = oracle.asmble_cdist(m, mod, g, freq, py, pml)
+    S = csr(g, oracle.asmble_z(m, mod, g, freq, py, pml)).tocoo()
+    cnp = np.frombuffer(g.cnp, dtype="S1")
+    ue = cf.seeded_complex(g.ndof, seed=42)
+    rb, cb = cnp[S.row] == b"B", cnp[S.col] == b"B"
+    re_, ce = cnp[S.row] == b"E", cnp[S.col] == b"E"
+    sign = np.where(rb & ce, -1.0, 0.0) + np.where(re_ & cb, 1.0, 0.0)
+    Sr = sp.csr_matrix((S.data * sign, (S.row, S.col)), shape=S.shape)
+    want = Sr @ ue
+    got = oracle.asmb_peff(g, cd, ue.astype(np.complex64))
+    assert np.count_nonzero(sign) > 0
+    assert relerr(got, want) < 2e-6
+    other = (cnp != b"B") & (cnp != b"E")
+    assert np.all(got[other] == 0)
+
+
+@needs_ref
 def test_gnmfree_vs_reference(tmp_path):
     """GNMFREE restated: bit-exact against the compiled reference, which
     reads Jacobian files written in its own record layout; and against the
