@@ -114,7 +114,24 @@ struct f25_mesh_s {
         const void *x = nullptr, *vcf = nullptr, *om = nullptr, *kv = nullptr, *pad = nullptr;
         void *y = nullptr;
         int npair = 0;
+        // the PML launches captured with it
+        const void *s01 = nullptr, *s2 = nullptr, *grp = nullptr, *gpair = nullptr;
+        int nom = 0, ngrp = 0;
     } ag;
+    // PML elements in the batched product: per distinct omega of the batch
+    // their element matrices by power of ky (k_pml_split), applied after the
+    // element-block product by k_apply_pml over groups of pairs of one omega
+    struct {
+        std::vector<double> om;            // distinct omegas the matrices hold
+        bool valid = false;                // cleared when the model changes
+        double2 *d_s01 = nullptr, *d_s2 = nullptr;
+        size_t s01_cap = 0;
+        double *d_om = nullptr;
+        int om_cap = 0;
+        std::vector<int> grp, gpair;       // (omega index, first, count) per group; pairs by omega
+        int *d_grp = nullptr, *d_gpair = nullptr;
+        int grp_cap = 0, gpair_cap = 0, ngrp = 0;
+    } pa;
     int apply_nrt = 0;          // row tiles of k_apply_v (0: the order's default)
     double *d_vpad = nullptr;   // k_apply_v: vpad_cap zero double2, then vpad_cap double2 of store sink
     int vpad_cap = 0;
@@ -418,6 +435,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc);
     dfree(m->d_ew);
     dfree(m->d_omky);
+    dfree(m->pa.d_s01); dfree(m->pa.d_s2); dfree(m->pa.d_om); dfree(m->pa.d_grp); dfree(m->pa.d_gpair);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
     if (m->ev1) (void)hipEventDestroy(m->ev1);
     if (m->st) (void)hipStreamDestroy(m->st);
@@ -533,6 +551,7 @@ extern "C" int f25_set_model(f25_mesh *m, const double *dens, const double *ecoe
     TRY(vcoef(m, m->d_blk, &m->d_vcf));
     m->have_model = true;
     m->basis_ok = false;
+    m->pa.valid = false;
     return 0;
 }
 
@@ -922,13 +941,115 @@ static int apply_f64(f25_mesh *m, const double *om, const double *kv, int npair,
     return launch_f64(m, om, kv, npair, x, y, vcf);
 }
 
+// ---- PML elements of the batched product -------------------------------
+// Distinct omegas of the batch (first-appearance order), the pairs of each
+// in groups of at most 16, and the element matrices for those omegas
+// (rebuilt when the omegas or the model change).  No-op without PML
+// elements.  h_om: the batch's omegas on the host.
+static int pml_prepare(f25_mesh *m, int npair, const double *h_om) {
+    auto &P = m->pa;
+    P.ngrp = 0;
+    if (m->pml_order.empty()) return 0;
+    std::vector<double> uo;
+    std::vector<int> fi(npair);
+    for (int b = 0; b < npair; ++b) {
+        size_t k = 0;
+        while (k < uo.size() && uo[k] != h_om[b]) ++k;
+        if (k == uo.size()) uo.push_back(h_om[b]);
+        fi[b] = (int)k;
+    }
+    P.grp.clear();
+    P.gpair.clear();
+    for (size_t f = 0; f < uo.size(); ++f) {
+        const int first = (int)P.gpair.size();
+        for (int b = 0; b < npair; ++b)
+            if (fi[b] == (int)f) P.gpair.push_back(b);
+        for (int g0 = first; g0 < (int)P.gpair.size(); g0 += 16) {
+            P.grp.push_back((int)f);
+            P.grp.push_back(g0);
+            P.grp.push_back(std::min(16, (int)P.gpair.size() - g0));
+        }
+    }
+    P.ngrp = (int)P.grp.size() / 3;
+    if ((int)P.grp.size() > P.grp_cap) {
+        dfree(P.d_grp);
+        P.grp_cap = 0;
+        TRY(dalloc(&P.d_grp, P.grp.size()));
+        P.grp_cap = (int)P.grp.size();
+    }
+    if ((int)P.gpair.size() > P.gpair_cap) {
+        dfree(P.d_gpair);
+        P.gpair_cap = 0;
+        TRY(dalloc(&P.d_gpair, P.gpair.size()));
+        P.gpair_cap = (int)P.gpair.size();
+    }
+    HIPCHK(hipMemcpyAsync(P.d_grp, P.grp.data(), P.grp.size() * sizeof(int), hipMemcpyHostToDevice, m->st));
+    HIPCHK(hipMemcpyAsync(P.d_gpair, P.gpair.data(), P.gpair.size() * sizeof(int), hipMemcpyHostToDevice, m->st));
+    if (P.valid && uo == P.om) return 0;
+    const int nen = m->tab.nen, nee = 3 * nen, npml = (int)m->pml_order.size(), nom = (int)uo.size();
+    const size_t n01 = (size_t)npml * nom * 2 * nee * nee;
+    if (n01 > P.s01_cap) {
+        dfree(P.d_s01);
+        P.s01_cap = 0;
+        TRY(dalloc(&P.d_s01, n01));
+        P.s01_cap = n01;
+    }
+    if (!P.d_s2) TRY(dalloc(&P.d_s2, (size_t)npml * nee * nee));
+    if (nom > P.om_cap) {
+        dfree(P.d_om);
+        P.om_cap = 0;
+        TRY(dalloc(&P.d_om, (size_t)nom));
+        P.om_cap = nom;
+    }
+    HIPCHK(hipMemcpyAsync(P.d_om, uo.data(), nom * sizeof(double), hipMemcpyHostToDevice, m->st));
+    PmlPar p{};
+    p.xwidth = m->d.xwidth;
+    p.zwidth = m->d.zwidth;
+    p.rcoeff = m->d.pml[0]; p.dmax = m->d.pml[1]; p.kappa = m->d.pml[2]; p.fcent = m->d.pml[3];
+    const size_t lds = (8 * 9 + 3 * 8 * nen) * sizeof(double);
+    for (size_t c = 0; c + 1 < m->pml_cptr.size(); ++c) {
+        const int n = m->pml_cptr[c + 1] - m->pml_cptr[c];
+        if (!n) continue;
+        hipLaunchKernelGGL(k_pml_split, dim3(n, nom), dim3(256), lds, m->st, nen, m->tab.nq1, m->pml_cptr[c], nom,
+                           m->d_pml_order + m->pml_cptr[c], P.d_om, m->d_shl, m->d_wq, m->d_xq, m->d_ieng,
+                           m->d_xl, m->d_zl, m->d_xd, m->d_zd, m->d_rho, m->d_lam, m->d_mu, p, P.d_s01, P.d_s2,
+                           m->d_err);
+    }
+    HIPCHK(hipGetLastError());
+    TRY(check_err(m));
+    P.om = uo;
+    P.valid = true;
+    return 0;
+}
+
+// y += S_e x over the PML elements, colour by colour (after the product)
+template <typename XT>
+static int pml_launch(f25_mesh *m, int npair, const double *d_ky, const XT *x, XT *y) {
+    auto &P = m->pa;
+    if (!P.ngrp) return 0;
+    const int nen = m->tab.nen, nom = (int)P.om.size();
+    const size_t lds = (size_t)3 * nen * 16 * sizeof(double2);
+    for (size_t c = 0; c + 1 < m->pml_cptr.size(); ++c) {
+        const int n = m->pml_cptr[c + 1] - m->pml_cptr[c];
+        if (!n) continue;
+        hipLaunchKernelGGL(k_apply_pml<XT>, dim3(n, P.ngrp), dim3(256), lds, m->st, nen, m->pml_cptr[c], nom,
+                           m->d_pml_order + m->pml_cptr[c], m->d_lm, P.d_grp, P.d_gpair, npair, d_ky, P.d_s01,
+                           P.d_s2, x, y);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // F25_APPLY_GRAPH: capture the launches once per argument set, then replay
 static int apply_f64_graph(f25_mesh *m, const double *om, const double *kv, int npair, const double2 *x,
                            double2 *y, const double *vcf) {
     TRY(ensure_vpad(m, npair));
     auto &G = m->ag;
+    const auto &P = m->pa;
+    const int nom = (int)P.om.size();
     if (!G.exec || G.x != x || G.y != y || G.npair != npair || G.vcf != vcf || G.om != om || G.kv != kv ||
-        G.pad != m->d_vpad) {
+        G.pad != m->d_vpad || G.s01 != P.d_s01 || G.s2 != P.d_s2 || G.grp != P.d_grp || G.gpair != P.d_gpair ||
+        G.nom != nom || G.ngrp != P.ngrp) {
         if (G.exec) {
             HIPCHK(hipStreamSynchronize(m->st));
             (void)hipGraphExecDestroy(G.exec);
@@ -936,7 +1057,8 @@ static int apply_f64_graph(f25_mesh *m, const double *om, const double *kv, int 
         }
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
-        const int r = launch_f64(m, om, kv, npair, x, y, vcf);
+        int r = launch_f64(m, om, kv, npair, x, y, vcf);
+        if (!r) r = pml_launch(m, npair, kv, x, y);
         hipError_t e = hipStreamEndCapture(m->st, &g);
         if (r) {
             if (g) (void)hipGraphDestroy(g);
@@ -950,28 +1072,36 @@ static int apply_f64_graph(f25_mesh *m, const double *om, const double *kv, int 
             FAIL(F25_ERR_HIP, hipGetErrorString(e));
         }
         G.x = x; G.y = y; G.npair = npair; G.vcf = vcf; G.om = om; G.kv = kv; G.pad = m->d_vpad;
+        G.s01 = P.d_s01; G.s2 = P.d_s2; G.grp = P.d_grp; G.gpair = P.d_gpair; G.nom = nom; G.ngrp = P.ngrp;
     }
     HIPCHK(hipGraphLaunch(G.exec, m->st));
     return 0;
 }
 
+// h_om: the batch's omegas on the host when the PML elements are to be
+// included (the operator product), null for products on derivative blocks
+// (the PML anchors' parameters are held fixed there).
 static int apply_core(f25_mesh *m, int npair, const double *d_om, const double *d_ky, int mode,
-                      const void *x, void *y, const double *vcf) {
+                      const void *x, void *y, const double *vcf, const double *h_om) {
     // every DOF is written by the first element touching it: no zero fill
     const bool graph = (mode & F25_APPLY_GRAPH) != 0;
     mode &= ~F25_APPLY_GRAPH;
     if (mode != F25_APPLY_F64 && mode != F25_APPLY_F32 && mode != F25_APPLY_F32_REFINED)
         FAIL(F25_ERR_ARG, "bad apply mode");
     if (graph && mode != F25_APPLY_F64) FAIL(F25_ERR_ARG, "graph replay is for the FP64 product");
+    if (h_om) TRY(pml_prepare(m, npair, h_om));
+    else m->pa.ngrp = 0;
     HIPCHK(hipEventRecord(m->ev0, m->st));
     if (mode == F25_APPLY_F64 && graph)
         TRY(apply_f64_graph(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y, vcf));
-    else if (mode == F25_APPLY_F64)
+    else if (mode == F25_APPLY_F64) {
         TRY(apply_f64(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y, vcf));
-    else if (mode == F25_APPLY_F32)
+        TRY(pml_launch(m, npair, d_ky, (const double2 *)x, (double2 *)y));
+    } else if (mode == F25_APPLY_F32) {
         TRY((dispatch_apply<float, float2, false>(m, d_om, d_ky, npair, (const float2 *)x, (float2 *)y,
                                                   m->d_blk32, nullptr)));
-    else {
+        TRY(pml_launch(m, npair, d_ky, (const float2 *)x, (float2 *)y));
+    } else {
         if (npair > m->colmax_cap) {
             dfree(m->d_colmax);
             TRY(dalloc(&m->d_colmax, (size_t)npair));
@@ -984,6 +1114,7 @@ static int apply_core(f25_mesh *m, int npair, const double *d_om, const double *
         HIPCHK(hipGetLastError());
         TRY((dispatch_apply<float, double2, true>(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y,
                                                   m->d_blk_hi, m->d_blk_lo)));
+        TRY(pml_launch(m, npair, d_ky, (const double2 *)x, (double2 *)y));
     }
     HIPCHK(hipEventRecord(m->ev1, m->st));
     HIPCHK(hipStreamSynchronize(m->st));
@@ -1011,7 +1142,7 @@ extern "C" int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, 
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
     HIPCHK(hipSetDevice(m->dev));
     TRY(stage_omky(m, npair, omega, ky));
-    return apply_core(m, npair, m->d_omky, m->d_omky + m->omky_cap, mode, x, y, m->d_vcf);
+    return apply_core(m, npair, m->d_omky, m->d_omky + m->omky_cap, mode, x, y, m->d_vcf, omega);
 }
 
 template <typename T>
@@ -2117,7 +2248,7 @@ extern "C" int f25_gn_fv(f25_mesh *m, int ntrip, const double *omega, const doub
     r = (e == hipSuccess) ? 0 : F25_ERR_HIP;
     if (!r) r = transpose_dev(m->st, a, b, ntrip, m->g.ndof);
     if (!r) r = vcoef(m, m->d_dblk, &m->d_dvcf);
-    if (!r) r = apply_core(m, ntrip, d_om, d_ky, F25_APPLY_F64, b, a, m->d_dvcf);
+    if (!r) r = apply_core(m, ntrip, d_om, d_ky, F25_APPLY_F64, b, a, m->d_dvcf, nullptr);
     if (!r) r = transpose_dev(m->st, a, b, m->g.ndof, ntrip);
     if (!r) {
         e = hipMemcpyAsync(out, b, nn * 16, hipMemcpyDeviceToHost, m->st);

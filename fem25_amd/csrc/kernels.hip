@@ -1036,6 +1036,128 @@ __device__ inline double2 zdampfn(double rn, double vp, double delta, double pdm
 //   standard:   L = w lambda, M = w mu, R = w rho
 //   dispersive: L = w (Vp~^2 - 2 Vs~^2), M = w Vs~^2, R = w.
 // One workgroup per element of one colour.
+// Stages QC quadrature points q0 .. q0+QC-1 of element e in LDS: per point
+// sq[qq*9 + ..] = L (re, im), M (re, im), R, e_x (re, im), e_z (re, im) and
+// per node sN, sX, sZ = N, dN/dx, dN/dz (zero past the last point).
+__device__ inline void pml_stage(int nen, int nq1, int q0, int QC, int e, const double *__restrict__ shl,
+                                 const double *__restrict__ wq, const double *__restrict__ xq,
+                                 const int *__restrict__ ieng, const double *__restrict__ xl,
+                                 const double *__restrict__ zl, const double *__restrict__ xd,
+                                 const double *__restrict__ zd, bool pml, const double *__restrict__ rho,
+                                 const double *__restrict__ lam, const double *__restrict__ mu,
+                                 const double *__restrict__ qp, const double *__restrict__ qs, const PmlPar &par,
+                                 double *sq, double *sN, double *sX, double *sZ, int *__restrict__ err) {
+    const int QS = 9, nq = nq1 * nq1;
+    const int *ie = ieng + 4 * e;
+    for (int i = threadIdx.x; i < QC * nen; i += blockDim.x) {
+        int qq = i / nen, aa = i % nen, q = q0 + qq;
+        if (q >= nq) { sN[i] = sX[i] = sZ[i] = 0.0; if (aa == 0) for (int j = 0; j < QS; ++j) sq[qq * QS + j] = 0.0; continue; }
+        int ix = q % nq1, iz = q / nq1;
+        Geo g;
+        int bad = geo_at(xq[ix], xq[iz], ie, xl, zl, g);
+        if (bad && aa == 0) *err = 2;
+        double rj = bad ? 1.0 : g.det;
+        const double *sh = shl + ((size_t)q * nen + aa) * 3;
+        sN[i] = sh[2];
+        sX[i] = (sh[0] * g.dzdeta - sh[1] * g.dzdxi) / rj;
+        sZ[i] = -(sh[0] * g.dxdeta - sh[1] * g.dxdxi) / rj;
+        if (aa == 0) {
+            double w = wq[ix] * wq[iz] * g.det, r = 0, l = 0, m = 0, xdep = 0, zdep = 0, vel = 0;
+            double vsv = 0, qpv = 0, qsv = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                int gi = ie[k] - 1;
+                r += g.sf[k] * rho[gi];
+                l += g.sf[k] * lam[gi];
+                m += g.sf[k] * mu[gi];
+                if (pml) {
+                    xdep += g.sf[k] * xd[gi];
+                    zdep += g.sf[k] * zd[gi];
+                }
+                vel += g.sf[k] * sqrt((lam[gi] + 2.0 * mu[gi]) / rho[gi]);
+                if (qp) {
+                    vsv += g.sf[k] * sqrt(mu[gi] / rho[gi]);
+                    qpv += g.sf[k] * qp[gi];
+                    qsv += g.sf[k] * qs[gi];
+                }
+            }
+            const double2 one = make_double2(1.0, 0.0);
+            double2 ex = xdep > 2.22e-16
+                             ? zdampfn(xdep, vel, par.xwidth, par.dmax, par.rcoeff, par.fcent, par.omega, par.kappa)
+                             : one;
+            double2 ez = zdep > 2.22e-16
+                             ? zdampfn(zdep, vel, par.zwidth, par.dmax, par.rcoeff, par.fcent, par.omega, par.kappa)
+                             : one;
+            double *o = sq + qq * QS;
+            if (qp) {
+                const double2 zp = vcmplx(par.freq, par.freq0, vel, qpv);
+                const double2 zs = vcmplx(par.freq, par.freq0, vsv, qsv);
+                double2 zp2 = cmul(zp, zp), zs2 = cmul(zs, zs);
+                zp2.x *= w; zp2.y *= w; zs2.x *= w; zs2.y *= w;
+                o[0] = zp2.x - 2.0 * zs2.x; o[1] = zp2.y - 2.0 * zs2.y;
+                o[2] = zs2.x; o[3] = zs2.y; o[4] = w;
+            } else {
+                o[0] = w * l; o[1] = 0.0; o[2] = w * m; o[3] = 0.0; o[4] = w * r;
+            }
+            o[5] = ex.x; o[6] = ex.y; o[7] = ez.x; o[8] = ez.y;
+        }
+    }
+}
+
+// The bilinear forms of one (a, b) node pair at one staged point, split by
+// the power of ky they carry: s0 (ky^0: the stiffness, 3x3 components,
+// without the mass), s1 (ky^1: the i ky couplings 12, 21, 23, 32 with i
+// applied; the ky factor left out), s2 (ky^2: the mass-like diagonal
+// M, L + 2M, M) and nn = N_a N_b for the -omega^2 R mass.
+__device__ inline void pml_point(const double *o, double na, double nb, double xa0, double xb0, double za0,
+                                 double zb0, double2 *s0, double2 *s1, double2 *s2, double &rnn) {
+    const double2 L = make_double2(o[0], o[1]), M = make_double2(o[2], o[3]);
+    const double R = o[4];
+    const double2 ex = make_double2(o[5], o[6]), ez = make_double2(o[7], o[8]);
+    double2 xa = make_double2(xa0 * ex.x, xa0 * ex.y), xb = make_double2(xb0 * ex.x, xb0 * ex.y);
+    double2 za = make_double2(za0 * ez.x, za0 * ez.y), zb = make_double2(zb0 * ez.x, zb0 * ez.y);
+    double2 xx = cmul(xa, xb), zz = cmul(za, zb), xz = cmul(xa, zb), zx = cmul(za, xb);
+    const double2 L2 = make_double2(L.x + 2.0 * M.x, L.y + 2.0 * M.y);
+    double2 t;
+    t = cmul(L2, xx); s0[0].x += t.x; s0[0].y += t.y;                      // 11
+    t = cmul(M, zz);  s0[0].x += t.x; s0[0].y += t.y;
+    t = cmul(M, make_double2(xx.x + zz.x, xx.y + zz.y)); s0[4].x += t.x; s0[4].y += t.y;   // 22
+    t = cmul(L2, zz); s0[8].x += t.x; s0[8].y += t.y;                      // 33
+    t = cmul(M, xx);  s0[8].x += t.x; s0[8].y += t.y;
+    t = cmul(L, xz);  s0[2].x += t.x; s0[2].y += t.y;                      // 13
+    t = cmul(M, zx);  s0[2].x += t.x; s0[2].y += t.y;
+    t = cmul(L, zx);  s0[6].x += t.x; s0[6].y += t.y;                      // 31
+    t = cmul(M, xz);  s0[6].x += t.x; s0[6].y += t.y;
+    // i ky couplings: CE12 = i ky (L xa nb - M na xb), KE(1,2) -= CE12, etc.
+    const double2 Lx = cmul(L, xa), Mx = cmul(M, xb), Lz = cmul(L, zb), Mz = cmul(M, za);
+    const double2 Lxb = cmul(L, xb), Mxa = cmul(M, xa), Lza = cmul(L, za), Mzb = cmul(M, zb);
+    double2 t12 = make_double2(Lx.x * nb - Mx.x * na, Lx.y * nb - Mx.y * na);
+    double2 t21 = make_double2(Lxb.x * na - Mxa.x * nb, Lxb.y * na - Mxa.y * nb);
+    double2 t23 = make_double2(Lz.x * na - Mz.x * nb, Lz.y * na - Mz.y * nb);
+    double2 t32 = make_double2(Lza.x * nb - Mzb.x * na, Lza.y * nb - Mzb.y * na);
+    // times i: (x, y) -> (-y, x)
+    s1[0].x += t12.y;  s1[0].y -= t12.x;    // -(i t12)
+    s1[1].x -= t21.y;  s1[1].y += t21.x;    // +(i t21)
+    s1[2].x -= t23.y;  s1[2].y += t23.x;    // +(i t23)
+    s1[3].x += t32.y;  s1[3].y -= t32.x;    // -(i t32)
+    const double nn = na * nb;
+    s2[0].x += M.x * nn;  s2[0].y += M.y * nn;
+    s2[1].x += L2.x * nn; s2[1].y += L2.y * nn;
+    s2[2].x += M.x * nn;  s2[2].y += M.y * nn;
+    rnn += R * nn;
+}
+
+// Element matrices integrated per call and added into CSR order:
+//  * PML elements ('A', CM25ISOQ estiff25.f:168-334 with CJACCQ's stretch)
+//  * with quality factors (qp != null) every element in the density-
+//    normalised dispersive form: KMDAMP (estiff25.f:511-752) and, for 'A'
+//    elements, CMDAMP (estiff25.f:756-961), velocities from VCMPLX.
+// Both are the KM25ISOQ bilinear forms with complex coefficients at each
+// quadrature point: lambda -> L, mu -> M, rho -> R, derivatives stretched by
+// 1/e_x, 1/e_z (ZDAMPFN) inside PML elements:
+//   standard:   L = w lambda, M = w mu, R = w rho
+//   dispersive: L = w (Vp~^2 - 2 Vs~^2), M = w Vs~^2, R = w.
+// One workgroup per element of one colour.
 template <typename CT>
 __global__ void k_pml(int nen, int nq1, int ncol_elem, const int *__restrict__ order,
                       const double *__restrict__ shl, const double *__restrict__ wq,
@@ -1057,120 +1179,46 @@ __global__ void k_pml(int nen, int nq1, int ncol_elem, const int *__restrict__ o
     double *sN = sq + QC * QS;       // QC*nen
     double *sX = sN + QC * nen;
     double *sZ = sX + QC * nen;
-    const int *ie = ieng + 4 * e;
     const bool pml = cdom[e] == 'A';
     const int np = nen * nen;
     const double k2 = par.kv * par.kv, w2 = par.omega * par.omega;
     for (int p0 = 0; p0 < np; p0 += blockDim.x) {
         const int pr = p0 + threadIdx.x;
         const int a = pr / nen, b = pr % nen;
-        double2 acc[9], me[3];
+        double2 s0[9], s1[4], s2[3];
+        double rnn = 0.0;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) acc[i] = make_double2(0.0, 0.0);
+        for (int i = 0; i < 9; ++i) s0[i] = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) me[i] = make_double2(0.0, 0.0);
+        for (int i = 0; i < 4; ++i) s1[i] = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) s2[i] = make_double2(0.0, 0.0);
         for (int q0 = 0; q0 < nq; q0 += QC) {
             __syncthreads();
-            for (int i = threadIdx.x; i < QC * nen; i += blockDim.x) {
-                int qq = i / nen, aa = i % nen, q = q0 + qq;
-                if (q >= nq) { sN[i] = sX[i] = sZ[i] = 0.0; if (aa == 0) for (int j = 0; j < QS; ++j) sq[qq * QS + j] = 0.0; continue; }
-                int ix = q % nq1, iz = q / nq1;
-                Geo g;
-                int bad = geo_at(xq[ix], xq[iz], ie, xl, zl, g);
-                if (bad && aa == 0) *err = 2;
-                double rj = bad ? 1.0 : g.det;
-                const double *s = shl + ((size_t)q * nen + aa) * 3;
-                sN[i] = s[2];
-                sX[i] = (s[0] * g.dzdeta - s[1] * g.dzdxi) / rj;
-                sZ[i] = -(s[0] * g.dxdeta - s[1] * g.dxdxi) / rj;
-                if (aa == 0) {
-                    double w = wq[ix] * wq[iz] * g.det, r = 0, l = 0, m = 0, xdep = 0, zdep = 0, vel = 0;
-                    double vsv = 0, qpv = 0, qsv = 0;
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        int gi = ie[k] - 1;
-                        r += g.sf[k] * rho[gi];
-                        l += g.sf[k] * lam[gi];
-                        m += g.sf[k] * mu[gi];
-                        if (pml) {
-                            xdep += g.sf[k] * xd[gi];
-                            zdep += g.sf[k] * zd[gi];
-                        }
-                        vel += g.sf[k] * sqrt((lam[gi] + 2.0 * mu[gi]) / rho[gi]);
-                        if (qp) {
-                            vsv += g.sf[k] * sqrt(mu[gi] / rho[gi]);
-                            qpv += g.sf[k] * qp[gi];
-                            qsv += g.sf[k] * qs[gi];
-                        }
-                    }
-                    const double2 one = make_double2(1.0, 0.0);
-                    double2 ex = xdep > 2.22e-16
-                                     ? zdampfn(xdep, vel, par.xwidth, par.dmax, par.rcoeff, par.fcent, par.omega, par.kappa)
-                                     : one;
-                    double2 ez = zdep > 2.22e-16
-                                     ? zdampfn(zdep, vel, par.zwidth, par.dmax, par.rcoeff, par.fcent, par.omega, par.kappa)
-                                     : one;
-                    double *o = sq + qq * QS;
-                    if (qp) {
-                        const double2 zp = vcmplx(par.freq, par.freq0, vel, qpv);
-                        const double2 zs = vcmplx(par.freq, par.freq0, vsv, qsv);
-                        double2 zp2 = cmul(zp, zp), zs2 = cmul(zs, zs);
-                        zp2.x *= w; zp2.y *= w; zs2.x *= w; zs2.y *= w;
-                        o[0] = zp2.x - 2.0 * zs2.x; o[1] = zp2.y - 2.0 * zs2.y;
-                        o[2] = zs2.x; o[3] = zs2.y; o[4] = w;
-                    } else {
-                        o[0] = w * l; o[1] = 0.0; o[2] = w * m; o[3] = 0.0; o[4] = w * r;
-                    }
-                    o[5] = ex.x; o[6] = ex.y; o[7] = ez.x; o[8] = ez.y;
-                }
-            }
+            pml_stage(nen, nq1, q0, QC, e, shl, wq, xq, ieng, xl, zl, xd, zd, pml, rho, lam, mu, qp, qs, par,
+                      sq, sN, sX, sZ, err);
             __syncthreads();
-            if (pr < np) {
-                for (int qq = 0; qq < QC; ++qq) {
-                    const double *o = sq + qq * QS;
-                    const double2 L = make_double2(o[0], o[1]), M = make_double2(o[2], o[3]);
-                    const double R = o[4];
-                    const double2 ex = make_double2(o[5], o[6]), ez = make_double2(o[7], o[8]);
-                    double na = sN[qq * nen + a], nb = sN[qq * nen + b];
-                    double2 xa = make_double2(sX[qq * nen + a] * ex.x, sX[qq * nen + a] * ex.y);
-                    double2 xb = make_double2(sX[qq * nen + b] * ex.x, sX[qq * nen + b] * ex.y);
-                    double2 za = make_double2(sZ[qq * nen + a] * ez.x, sZ[qq * nen + a] * ez.y);
-                    double2 zb = make_double2(sZ[qq * nen + b] * ez.x, sZ[qq * nen + b] * ez.y);
-                    double2 xx = cmul(xa, xb), zz = cmul(za, zb), xz = cmul(xa, zb), zx = cmul(za, xb);
-                    const double2 L2 = make_double2(L.x + 2.0 * M.x, L.y + 2.0 * M.y);
-                    double2 t;
-                    t = cmul(L2, xx); acc[0].x += t.x; acc[0].y += t.y;                      // 11
-                    t = cmul(M, zz);  acc[0].x += t.x; acc[0].y += t.y;
-                    t = cmul(M, make_double2(xx.x + zz.x, xx.y + zz.y)); acc[4].x += t.x; acc[4].y += t.y;   // 22
-                    t = cmul(L2, zz); acc[8].x += t.x; acc[8].y += t.y;                      // 33
-                    t = cmul(M, xx);  acc[8].x += t.x; acc[8].y += t.y;
-                    t = cmul(L, xz);  acc[2].x += t.x; acc[2].y += t.y;                      // 13
-                    t = cmul(M, zx);  acc[2].x += t.x; acc[2].y += t.y;
-                    t = cmul(L, zx);  acc[6].x += t.x; acc[6].y += t.y;                      // 31
-                    t = cmul(M, xz);  acc[6].x += t.x; acc[6].y += t.y;
-                    // i ky couplings: CE12 = i ky (L xa nb - M na xb), KE(1,2) -= CE12, etc.
-                    const double2 Lx = cmul(L, xa), Mx = cmul(M, xb), Lz = cmul(L, zb), Mz = cmul(M, za);
-                    const double2 Lxb = cmul(L, xb), Mxa = cmul(M, xa), Lza = cmul(L, za), Mzb = cmul(M, zb);
-                    double2 t12 = make_double2(Lx.x * nb - Mx.x * na, Lx.y * nb - Mx.y * na);
-                    double2 t21 = make_double2(Lxb.x * na - Mxa.x * nb, Lxb.y * na - Mxa.y * nb);
-                    double2 t23 = make_double2(Lz.x * na - Mz.x * nb, Lz.y * na - Mz.y * nb);
-                    double2 t32 = make_double2(Lza.x * nb - Mzb.x * na, Lza.y * nb - Mzb.y * na);
-                    // multiply by i ky: (x, y) -> (-ky y, ky x)
-                    acc[1].x += par.kv * t12.y;  acc[1].y -= par.kv * t12.x;    // -(i ky t12)
-                    acc[3].x -= par.kv * t21.y;  acc[3].y += par.kv * t21.x;    // +(i ky t21)
-                    acc[5].x -= par.kv * t23.y;  acc[5].y += par.kv * t23.x;    // +(i ky t23)
-                    acc[7].x += par.kv * t32.y;  acc[7].y -= par.kv * t32.x;    // -(i ky t32)
-                    const double nn = na * nb;
-                    me[0].x += (-w2 * R + k2 * M.x) * nn;  me[0].y += k2 * M.y * nn;
-                    me[1].x += (-w2 * R + k2 * L2.x) * nn; me[1].y += k2 * L2.y * nn;
-                    me[2].x += (-w2 * R + k2 * M.x) * nn;  me[2].y += k2 * M.y * nn;
-                }
-            }
+            if (pr < np)
+                for (int qq = 0; qq < QC; ++qq)
+                    pml_point(sq + qq * QS, sN[qq * nen + a], sN[qq * nen + b], sX[qq * nen + a],
+                              sX[qq * nen + b], sZ[qq * nen + a], sZ[qq * nen + b], s0, s1, s2, rnn);
         }
         if (pr < np) {
-            acc[0].x += me[0].x; acc[0].y += me[0].y;
-            acc[4].x += me[1].x; acc[4].y += me[1].y;
-            acc[8].x += me[2].x; acc[8].y += me[2].y;
+            double2 acc[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) acc[i] = s0[i];
+            // ky terms and the mass: S = s0 + ky s1 + ky^2 s2 - omega^2 R N N
+            const int k1[4] = {1, 3, 5, 7};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                acc[k1[i]].x += par.kv * s1[i].x;
+                acc[k1[i]].y += par.kv * s1[i].y;
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                acc[4 * i].x += -w2 * rnn + k2 * s2[i].x;
+                acc[4 * i].y += k2 * s2[i].y;
+            }
             const int *L = lm + (size_t)e * 3 * nen;
             // unrolled, so that acc stays in registers
 #pragma unroll
@@ -1184,6 +1232,133 @@ __global__ void k_pml(int nen, int nq1, int ncol_elem, const int *__restrict__ o
                     out[s].x += acc[ci * 3 + cj].x;
                     out[s].y += acc[ci * 3 + cj].y;
                 }
+        }
+    }
+}
+
+// PML elements of the batched product: per element (slot s of the PML
+// colour order) and distinct omega f of the batch, the element matrices by
+// power of ky (CM25ISOQ with CJACCQ's stretch, estiff25.f:168-334):
+//   S(omega_f, ky) = S0[s][f] + ky S1[s][f] + ky^2 S2[s]
+// with rows and columns r = 3 a + i (node a, component i), NEE = 3 NEN.
+// s01: [slot][nom][2][NEE][NEE] complex (S0 holds the -omega^2 mass);
+// s2: [slot][NEE][NEE] complex (omega-independent, written with f = 0).
+// One workgroup per (element of one colour, omega).
+__global__ void k_pml_split(int nen, int nq1, int slot0, int nom, const int *__restrict__ order,
+                            const double *__restrict__ omf, const double *__restrict__ shl,
+                            const double *__restrict__ wq, const double *__restrict__ xq,
+                            const int *__restrict__ ieng, const double *__restrict__ xl,
+                            const double *__restrict__ zl, const double *__restrict__ xd,
+                            const double *__restrict__ zd, const double *__restrict__ rho,
+                            const double *__restrict__ lam, const double *__restrict__ mu, PmlPar par,
+                            double2 *__restrict__ s01, double2 *__restrict__ s2out, int *__restrict__ err) {
+    extern __shared__ double sm[];
+    const int e = order[blockIdx.x], slot = slot0 + blockIdx.x, f = blockIdx.y;
+    const int nq = nq1 * nq1, nee = 3 * nen;
+    const int QC = 8, QS = 9;
+    double *sq = sm;
+    double *sN = sq + QC * QS;
+    double *sX = sN + QC * nen;
+    double *sZ = sX + QC * nen;
+    par.omega = omf[f];
+    const double w2 = par.omega * par.omega;
+    const int np = nen * nen;
+    double2 *S0 = s01 + ((size_t)slot * nom + f) * 2 * nee * nee;
+    double2 *S1 = S0 + (size_t)nee * nee;
+    double2 *S2 = s2out + (size_t)slot * nee * nee;
+    for (int p0 = 0; p0 < np; p0 += blockDim.x) {
+        const int pr = p0 + threadIdx.x;
+        const int a = pr / nen, b = pr % nen;
+        double2 s0[9], s1[4], s2[3];
+        double rnn = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) s0[i] = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s1[i] = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) s2[i] = make_double2(0.0, 0.0);
+        for (int q0 = 0; q0 < nq; q0 += QC) {
+            __syncthreads();
+            pml_stage(nen, nq1, q0, QC, e, shl, wq, xq, ieng, xl, zl, xd, zd, true, rho, lam, mu, nullptr,
+                      nullptr, par, sq, sN, sX, sZ, err);
+            __syncthreads();
+            if (pr < np)
+                for (int qq = 0; qq < QC; ++qq)
+                    pml_point(sq + qq * QS, sN[qq * nen + a], sN[qq * nen + b], sX[qq * nen + a],
+                              sX[qq * nen + b], sZ[qq * nen + a], sZ[qq * nen + b], s0, s1, s2, rnn);
+        }
+        if (pr < np) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) s0[4 * i].x -= w2 * rnn;
+            const double2 z = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+                for (int cj = 0; cj < 3; ++cj) {
+                    const size_t o = (size_t)(3 * a + ci) * nee + 3 * b + cj;
+                    const int k = ci * 3 + cj;
+                    S0[o] = s0[k];
+                    S1[o] = (k == 1) ? s1[0] : (k == 3) ? s1[1] : (k == 5) ? s1[2] : (k == 7) ? s1[3] : z;
+                    if (f == 0) S2[o] = (ci == cj) ? s2[ci] : z;
+                }
+        }
+    }
+}
+
+// y_b += S_e(omega_b, ky_b) x_b over the PML elements of one colour, after
+// the element-block product has written y (it stores zero rows for them).
+// Workgroup (element, group of up to 16 pairs of one omega): lane t % 16 is
+// the pair, t / 16 picks rows r = t/16, t/16 + 16, ...; the element's x
+// values of the group are staged in LDS.  grp[3 g + 0..2] = (omega index,
+// first slot in gpair, count).  Arithmetic in FP64 for every vector type.
+template <typename XT>
+__global__ __launch_bounds__(256) void k_apply_pml(int nen, int slot0, int nom, const int *__restrict__ order,
+                                                   const int *__restrict__ lm, const int *__restrict__ grp,
+                                                   const int *__restrict__ gpair, int npair,
+                                                   const double *__restrict__ ky,
+                                                   const double2 *__restrict__ s01,
+                                                   const double2 *__restrict__ s2, const XT *__restrict__ x,
+                                                   XT *__restrict__ y) {
+    extern __shared__ double2 xs[];   // [NEE][16]
+    __shared__ int sl[3 * 7 * 7];
+    const int nee = 3 * nen;
+    const int e = order[blockIdx.x], slot = slot0 + blockIdx.x, g = blockIdx.y;
+    const int f = grp[3 * g], first = grp[3 * g + 1], cnt = grp[3 * g + 2];
+    const int pi = threadIdx.x & 15, rl = threadIdx.x >> 4;
+    const bool live = pi < cnt;
+    const int pair = live ? gpair[first + pi] : 0;
+    for (int i = threadIdx.x; i < nee; i += blockDim.x) sl[i] = lm[(size_t)e * nee + i];
+    __syncthreads();
+    for (int c = rl; c < nee; c += 16) {
+        const int d = sl[c];
+        double2 v = make_double2(0.0, 0.0);
+        if (live && d >= 0) {
+            const XT t = x[(size_t)d * npair + pair];
+            v = make_double2((double)t.x, (double)t.y);
+        }
+        xs[c * 16 + pi] = v;
+    }
+    __syncthreads();
+    const double k = ky[pair], k2 = k * k;
+    const double2 *S0 = s01 + ((size_t)slot * nom + f) * 2 * nee * nee;
+    const double2 *S1 = S0 + (size_t)nee * nee;
+    const double2 *S2 = s2 + (size_t)slot * nee * nee;
+    for (int r = rl; r < nee; r += 16) {
+        const int d = sl[r];
+        const double2 *a0 = S0 + (size_t)r * nee, *a1 = S1 + (size_t)r * nee, *a2 = S2 + (size_t)r * nee;
+        double ar = 0.0, ai = 0.0;
+        for (int c = 0; c < nee; ++c) {
+            const double2 m0 = a0[c], m1 = a1[c], m2 = a2[c];
+            const double mr = m0.x + k * m1.x + k2 * m2.x, mi = m0.y + k * m1.y + k2 * m2.y;
+            const double2 v = xs[c * 16 + pi];
+            ar += mr * v.x - mi * v.y;
+            ai += mr * v.y + mi * v.x;
+        }
+        if (live && d >= 0) {
+            XT &o = y[(size_t)d * npair + pair];
+            const double2 prev = make_double2((double)o.x, (double)o.y);
+            o.x = prev.x + ar;
+            o.y = prev.y + ai;
         }
     }
 }

@@ -131,10 +131,14 @@ int f25_asmb_peff(f25_mesh *m, const void *cdist, const void *ue, void *peff);
 
 /* ---- batched matrix-free product ---------------------------------------- */
 
-/* y_b = S(omega_b, ky_b) x_b for b = 0..npair-1, non-PML elements.  x and y
- * are device arrays of npair*NDOF complex values laid out [dof][pair]
- * (pair fastest), double complex for F25_APPLY_F64 and F25_APPLY_F32_REFINED,
- * float complex for F25_APPLY_F32. */
+/* y_b = S(omega_b, ky_b) x_b for b = 0..npair-1: the operator ASMBLE_DRIVER
+ * assembles (asmble.f90:140-212) with the body-wave PML parameters (CFTYPE
+ * other than 'S').  Interior elements run as frequency-independent element
+ * blocks (KM25ISOQ, estiff25.f:1-164); PML elements (CM25ISOQ,
+ * estiff25.f:168-334) as per-omega element matrices in FP64, rebuilt when the
+ * batch's set of omegas changes.  x and y are device arrays of npair*NDOF
+ * complex values laid out [dof][pair] (pair fastest), double complex for
+ * F25_APPLY_F64 and F25_APPLY_F32_REFINED, float complex for F25_APPLY_F32. */
 int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
                         int mode, const void *x_dev, void *y_dev);
 /* Host convenience wrapper: x, y are [pair][dof] double complex (one vector
@@ -152,8 +156,11 @@ int f25_set_apply_tiles(f25_mesh *m, int nrt);
 
 /* Exact adjoint-state gradient for ntrip (omega, ky) wavefield pairs:
  *   g[3*a + p] += sum_t Re( lam_t^H dS(omega_t, ky_t)/dm_{a,p} u_t )
- * for anchor a and p = (Vp, Vs, rho), non-PML elements, model as in
- * f25_set_model (lambda = rho(Vp^2-2Vs^2), mu = rho Vs^2).  u, lam:
+ * for anchor a and p = (Vp, Vs, rho), model as in f25_set_model
+ * (lambda = rho(Vp^2-2Vs^2), mu = rho Vs^2).  Anchors of PML elements hold
+ * fixed parameters (the reference inverts none of them, gengmask.f90:110-121):
+ * their entries receive nothing; the same holds for f25_gn_fv (their
+ * perturbation is ignored), f25_gn_fhw and f25_gn_diag.  u, lam:
  * [trip][dof] double complex host arrays.  With lam solving
  * S^H lam = R^T (d - R u) this is the gradient of 1/2 |R u - d|^2. */
 int f25_gradient(f25_mesh *m, int ntrip, const double *omega, const double *ky,

@@ -161,7 +161,7 @@ def test_gradient_full_size_additive_and_linear(torch_dev):
     mi.close()
 
 
-@pytest.mark.parametrize("name", ["tiny_layered", "p5", "p6"])
+@pytest.mark.parametrize("name", ["tiny_layered", "p5", "p6", "pml", "p2_pml"])
 def test_apply_batch_fp64(name):
     mi = _mi(name)
     om, ky, f, py = pairs(2, 3)
@@ -259,6 +259,48 @@ def test_apply_batch_graph_replays(torch_dev):
     from fem25_amd._lib import Fem25Error
     with pytest.raises(Fem25Error):     # graph replay is for the FP64 product only
         mi.apply_batch_dev(om, ky, x, y2, _lib.F25_APPLY_F32_REFINED | _lib.F25_APPLY_GRAPH)
+
+
+def test_apply_batch_pml_modes_and_graph(torch_dev):
+    """PML elements in the batched product (per-omega element matrices):
+    uneven pairs per omega (a group of 16 and a partial one), FP64 against the
+    oracle operator at 1e-12, the FP32 modes at their precision, a graph
+    replayed after the omegas change (the matrices are rebuilt outside the
+    graph) equal to the eager product bit for bit."""
+    torch = torch_dev
+    from fem25_amd import _lib
+    name = "pml"
+    mi = _mi(name)
+    g = oracle_graph(name)
+    om, ky, f, py = pairs(3, 7)
+    om = np.concatenate([om, np.full(12, om[0])])         # 19 pairs of the first omega
+    ky = np.concatenate([ky, np.linspace(0.0, ky[6], 12)])
+    f, py = om / TWOPI, ky / om
+    x = cf.seeded_complex(g.ndof, batch=om.size, seed=44)
+    xd = torch.from_numpy(np.ascontiguousarray(x.T)).cuda()
+    y = torch.zeros_like(xd)
+    mi.apply_batch_dev(om, ky, xd, y, _lib.F25_APPLY_F64)
+    yh = y.cpu().numpy().T
+    yref = np.stack([oracle_S(name, f[b], py[b]) @ x[b] for b in range(om.size)])
+    for b in range(om.size):
+        assert relerr(yh[b], yref[b]) < FP64_TOL, b
+    x32 = xd.to(torch.complex64)
+    y32 = torch.zeros_like(x32)
+    mi.apply_batch_dev(om, ky, x32, y32, _lib.F25_APPLY_F32)
+    yr = torch.zeros_like(xd)
+    mi.apply_batch_dev(om, ky, xd, yr, _lib.F25_APPLY_F32_REFINED)
+    e32 = max(relerr(y32.cpu().numpy().T[b], yref[b]) for b in range(om.size))
+    er = max(relerr(yr.cpu().numpy().T[b], yref[b]) for b in range(om.size))
+    print(f"pml mesh normwise error: fp32 {e32:.2e}, fp32-refined {er:.2e}")
+    assert e32 < 1e-5 and er < 1e-8
+    yg = torch.zeros_like(xd)
+    for rep in range(3):
+        omr = om * (1.0 + 0.1 * rep)
+        mi.apply_batch_dev(omr, ky, xd, yg, _lib.F25_APPLY_F64 | _lib.F25_APPLY_GRAPH)
+        mi.apply_batch_dev(omr, ky, xd, y, _lib.F25_APPLY_F64)
+        torch.cuda.synchronize()
+        assert torch.equal(yg, y), rep
+        xd.mul_(0.5)                    # eager work between the replays
 
 
 def test_apply_batch_config1_modes(torch_dev):
