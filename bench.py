@@ -221,8 +221,11 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
                                ("fp32_refined", _lib.F25_APPLY_F32_REFINED, x, yr)):
         k_ms, wall = timed(lambda: mi.apply_batch_dev(omega, ky, xx, yy, mode), L, b"apply", 3)
         d = yy.to(torch.complex128) - y
-        err = float((torch.linalg.norm(d, dim=0) / torch.linalg.norm(y, dim=0)).max())
+        per = (torch.linalg.norm(d, dim=0) / torch.linalg.norm(y, dim=0)).cpu().numpy()
+        err = float(per.max())
         out[name] = {"gdof_per_s": round(ndof * npair / wall * 1e-6, 3), "max_rel_err_vs_fp64": err,
+                     "median_rel_err_vs_fp64": float(np.median(per)),
+                     "rel_err_per_pair": [float(f"{v:.3e}") for v in per],
                      "kernel_ms": round(k_ms, 3),
                      "roofline": roof("fp32_mfma", 60.0 * nen * nen * mesh.nelem * npair * (4 if "ref" in name else 1)
                                       / (k_ms * 1e-3), "TFLOP/s")}
@@ -256,6 +259,21 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     kern["gradient"] = {"kernel_ms": round(k_ms, 3), "gradients_per_s": round(npair / (wall * 1e-3), 1),
                         "roofline": roof("fp64", gfl / (k_ms * 1e-3), "TFLOP/s")}
     out["gradients_per_s"] = kern["gradient"]["gradients_per_s"]
+    # the FP32 gradient (complex64 fields, FP32 densities, FP64 sums) and its
+    # error per field against the FP64 gradient
+    u32, l32 = u.to(torch.complex64), lam.to(torch.complex64)
+    g64 = torch.zeros_like(gg)
+    g32 = torch.zeros_like(gg)
+    mi.gradient_dev(omega, ky, u, lam, g64)
+    k32, wall32 = timed(lambda: mi.gradient32_dev(omega, ky, u32, l32, g32), L, b"grad", 2)
+    g32.zero_()
+    mi.gradient32_dev(omega, ky, u32, l32, g32)
+    ferr = {nm: float(torch.linalg.norm(g32[:, p] - g64[:, p]) / torch.linalg.norm(g64[:, p]))
+            for p, nm in enumerate(("vp", "vs", "rho"))}
+    kern["gradient_fp32"] = {"kernel_ms": round(k32, 3), "gradients_per_s": round(npair / (wall32 * 1e-3), 1),
+                             "rel_err_vs_fp64_per_field": ferr,
+                             "roofline": roof("fp64", gfl / (k32 * 1e-3), "TFLOP/s")}
+    del u32, l32, g64, g32
     # pseudo-Hessian diagonal (host arrays, 8 wavefields)
     nt = 8
     uh = u[:nt].cpu().numpy()

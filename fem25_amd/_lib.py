@@ -45,9 +45,11 @@ EXPORTS = {
                                       C.c_void_p]),
     "f25_apply_batch": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_int, C.c_void_p, C.c_void_p]),
     "f25_set_apply_tiles": (C.c_int, [C.c_void_p, C.c_int]),
+    "f25_residual_batch_dev": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p, dp]),
     "f25_gradient": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, dp]),
     "f25_gradient_dev": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p,
                                    C.c_void_p]),
+    "f25_gradient32_dev": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, C.c_void_p]),
     "f25_inv_setup": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_int, C.c_int, C.c_int,
                                 ip, ip]),
     "f25_inv_get": (C.c_int, [C.c_void_p, ip, ip, ip, ip, dp]),
@@ -64,6 +66,8 @@ EXPORTS = {
                                      C.c_void_p]),
     "f25_gnmfree": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_void_p, dp, dp]),
     "f25_gnmfree_dev": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "f25_gn_hessian_dev": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_void_p, C.c_void_p]),
+    "f25_write_jac": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_void_p, C.c_int, ip]),
     "f25_gn_jp": (C.c_int, [C.c_int, C.c_int, ip, C.c_int, ip, C.c_void_p, dp, C.c_void_p]),
     "f25_gn_jhz": (C.c_int, [C.c_int, C.c_int, ip, C.c_int, ip, dp, C.c_void_p, C.c_void_p, dp]),
     "f25_qp_nu_alpha": (C.c_int, [C.c_void_p, C.c_double, C.c_double, fp, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -202,6 +202,23 @@ class MeshInfo:
                                               C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())),
               "f25_apply_batch_dev")
 
+    def residual_batch_dev(self, omega, ky, u, f, r) -> np.ndarray:
+        """r = f - S u per pair in FP64 ([ndof, npair] complex128 device
+        tensors); returns |r_b| / |f_b| per pair."""
+        import torch
+        _ready(u, f, r)
+        om = np.ascontiguousarray(omega, np.float64)
+        kv = np.ascontiguousarray(ky, np.float64)
+        if om.shape != kv.shape or om.ndim != 1:
+            raise ValueError("omega and ky must be 1-D arrays of one length")
+        for t, nm in ((u, "u"), (f, "f"), (r, "r")):
+            _expect(t, torch.complex128, self.ndof * om.size, nm)
+        rn = np.zeros(om.size)
+        check(_lib.load().f25_residual_batch_dev(self.h, om.size, _p(om, dp), _p(kv, dp), C.c_void_p(u.data_ptr()),
+                                                 C.c_void_p(f.data_ptr()), C.c_void_p(r.data_ptr()), _p(rn, dp)),
+              "f25_residual_batch_dev")
+        return rn
+
     def set_apply_tiles(self, nrt: int = 0) -> None:
         """Row tiles per output component of the FP64 product (0: the order's default)."""
         check(_lib.load().f25_set_apply_tiles(self.h, int(nrt)), "f25_set_apply_tiles")
@@ -244,6 +261,21 @@ class MeshInfo:
         check(_lib.load().f25_gradient_dev(self.h, om.size, _p(om, dp), _p(kv, dp),
                                            C.c_void_p(u.data_ptr()), C.c_void_p(lam.data_ptr()),
                                            C.c_void_p(g.data_ptr())), "f25_gradient_dev")
+
+    def gradient32_dev(self, omega, ky, u, lam, g) -> None:
+        """FP32 gradient: u, lam torch complex64 [ntrip, ndof]; g torch float64 (nnpg, 3), accumulated."""
+        import torch
+        _ready(u, lam, g)
+        om = np.ascontiguousarray(omega, np.float64)
+        kv = np.ascontiguousarray(ky, np.float64)
+        if om.shape != kv.shape or om.ndim != 1:
+            raise ValueError("omega and ky must be 1-D arrays of one length")
+        _expect(u, torch.complex64, self.ndof * om.size, "u")
+        _expect(lam, torch.complex64, self.ndof * om.size, "lam")
+        _expect(g, torch.float64, self.mesh.nnpg * 3, "g")
+        check(_lib.load().f25_gradient32_dev(self.h, om.size, _p(om, dp), _p(kv, dp),
+                                             C.c_void_p(u.data_ptr()), C.c_void_p(lam.data_ptr()),
+                                             C.c_void_p(g.data_ptr())), "f25_gradient32_dev")
 
     # -- Gauss-Newton element products -----------------------------------------
     def gn_fv(self, omega, ky, u: np.ndarray, v: np.ndarray) -> np.ndarray:

@@ -1,5 +1,6 @@
 // C ABI (include/fem25hip.h): mesh handles, device residency and launches.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -146,6 +147,7 @@ struct f25_mesh_s {
     double *d_part = nullptr;
     int *d_mconn = nullptr;
     int *d_pmap = nullptr;   // pseudo-Hessian patch positions
+    rocblas_handle blas = nullptr;   // the Gauss-Newton Hessian's SYRK
     // anchors of PML elements: parameters held fixed by the gradient-type
     // products (the reference inverts only anchors of interior elements)
     std::vector<char> fixa;
@@ -436,6 +438,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_ew);
     dfree(m->d_omky);
     dfree(m->pa.d_s01); dfree(m->pa.d_s2); dfree(m->pa.d_om); dfree(m->pa.d_grp); dfree(m->pa.d_gpair);
+    if (m->blas) (void)rocblas_destroy_handle(m->blas);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
     if (m->ev1) (void)hipEventDestroy(m->ev1);
     if (m->st) (void)hipStreamDestroy(m->st);
@@ -1145,6 +1148,36 @@ extern "C" int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, 
     return apply_core(m, npair, m->d_omky, m->d_omky + m->omky_cap, mode, x, y, m->d_vcf, omega);
 }
 
+extern "C" int f25_residual_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
+                                      const void *u, const void *f, void *r, double *rnorm) {
+    if (m) TRY(require_elastic(m));
+    if (!m || npair <= 0 || !omega || !ky || !u || !f || !r || !rnorm || u == r || f == r)
+        FAIL(F25_ERR_ARG, "invalid residual arguments");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(stage_omky(m, npair, omega, ky));
+    TRY(apply_core(m, npair, m->d_omky, m->d_omky + m->omky_cap, F25_APPLY_F64, u, r, m->d_vcf, omega));
+    const int gy = (int)std::min<int64_t>(256, std::max<int64_t>(1, m->g.ndof / 256));
+    double *part = nullptr, *sums = nullptr;
+    int rc = dalloc(&part, (size_t)gy * npair * 2);
+    if (!rc) rc = dalloc(&sums, (size_t)npair * 2);
+    if (!rc) {
+        hipLaunchKernelGGL(k_resid, dim3((npair + 63) / 64, gy), dim3(256), 0, m->st, (int64_t)m->g.ndof, npair,
+                           (const double2 *)f, (double2 *)r, part);
+        hipLaunchKernelGGL(k_resid_sum, dim3((2 * npair + 255) / 256), dim3(256), 0, m->st, gy, npair, part, sums);
+        std::vector<double> h(2 * (size_t)npair);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, m->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); rc = F25_ERR_HIP; }
+        for (int b = 0; b < npair && !rc; ++b)
+            rnorm[b] = h[2 * b + 1] > 0.0 ? std::sqrt(h[2 * b] / h[2 * b + 1]) : std::sqrt(h[2 * b]);
+    }
+    dfree(part);
+    dfree(sums);
+    return rc;
+}
+
 template <typename T>
 static int transpose_dev(hipStream_t s, const T *in, T *out, int rows, int cols) {
     const size_t nt = (size_t)((cols + 15) / 16) * ((rows + 15) / 16);
@@ -1181,39 +1214,45 @@ extern "C" int f25_apply_batch(f25_mesh *m, int npair, const double *omega, cons
 }
 
 // ---------------------------------------------------------------------------
-static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d_ky, const double2 *u,
-                     const double2 *l, double *g, int chain) {
+template <typename R>
+static int grad_launch(f25_mesh *m, int ntrip, const double *d_om, const double *d_ky, const void *u, const void *l) {
+    typedef typename CxOf<R>::T C2;
     const int nen = m->tab.nen;
     // every order up to p6 runs the sum-factorised kernel
     typedef void (*GradSf)(const double *, const double *, const double *, const int *, const double *,
                            const double *, const char *, const int *, int, int64_t, const double *,
-                           const double *, const double2 *, const double2 *, double *, int *);
+                           const double *, const C2 *, const C2 *, double *, int *);
     // trips per pass: with two, four workgroups fit per CU at p4 (LDS and
     // registers; 200x100, 128 triples: 9.4 ms)
     constexpr int TS = 2;
     GradSf ksf = nullptr;
     switch (m->tab.nl) {
-        case 3: ksf = k_grad_sf<3, TS>; break;
-        case 4: ksf = k_grad_sf<4, TS>; break;
-        case 5: ksf = k_grad_sf<5, TS>; break;
-        case 6: ksf = k_grad_sf<6, TS>; break;
-        case 7: ksf = k_grad_sf<7, TS>; break;
+        case 3: ksf = k_grad_sf<3, TS, R>; break;
+        case 4: ksf = k_grad_sf<4, TS, R>; break;
+        case 5: ksf = k_grad_sf<5, TS, R>; break;
+        case 6: ksf = k_grad_sf<6, TS, R>; break;
+        case 7: ksf = k_grad_sf<7, TS, R>; break;
         default: break;
     }
     if (!ksf) FAIL(F25_ERR_ARG, "unsupported order");
-    {
-        const int nl = m->tab.nl, q1 = m->tab.nq1;
-        const size_t lsf = (size_t)TS * 2 * 3 * nen * sizeof(double2) +
-                           (size_t)TS * 2 * 3 * q1 * nl * 2 * sizeof(double2) + (size_t)q1 * q1 * 4 * sizeof(double) +
-                           (size_t)2 * nl * q1 * sizeof(double);
-        if (lsf > 64 * 1024)
-            HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
-        HIPCHK(hipEventRecord(m->ev0, m->st));
-        // Q1 = 17 (p6) adds a column of 17 lanes to the 17 rows: five waves
-        hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(q1 > 16 ? 320 : 256), lsf, m->st, m->d_t1d, m->d_wq,
-                           m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip, (int64_t)m->g.ndof,
-                           d_om, d_ky, u, l, m->d_part, m->d_err);
-    }
+    const int nl = m->tab.nl, q1 = m->tab.nq1;
+    const size_t lsf = (size_t)TS * 2 * 3 * nen * sizeof(C2) + (size_t)TS * 2 * 3 * q1 * nl * 2 * sizeof(C2) +
+                       (size_t)q1 * q1 * 4 * sizeof(R) + (size_t)2 * nl * q1 * sizeof(R);
+    if (lsf > 64 * 1024)
+        HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
+    HIPCHK(hipEventRecord(m->ev0, m->st));
+    // Q1 = 17 (p6) adds a column of 17 lanes to the 17 rows: five waves
+    hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(q1 > 16 ? 320 : 256), lsf, m->st, m->d_t1d, m->d_wq,
+                       m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip, (int64_t)m->g.ndof,
+                       d_om, d_ky, (const C2 *)u, (const C2 *)l, m->d_part, m->d_err);
+    return 0;
+}
+
+// fp32: complex64 fields, FP32 interpolation and densities per triple, FP64 sums
+static int grad_core(f25_mesh *m, int ntrip, const double *d_om, const double *d_ky, const void *u,
+                     const void *l, double *g, int chain, bool fp32 = false) {
+    if (fp32) TRY(grad_launch<float>(m, ntrip, d_om, d_ky, u, l));
+    else TRY(grad_launch<double>(m, ntrip, d_om, d_ky, u, l));
     HIPCHK(hipEventRecord(m->ev1, m->st));
     hipLaunchKernelGGL(k_grad_gather, dim3((m->d.nnpg + 255) / 256), dim3(256), 0, m->st, m->d.nnpg,
                        m->ncon_anchor, m->d_mconn, m->d_ieng, m->d_part, m->d_rho, m->d_lam, m->d_mu,
@@ -1233,8 +1272,17 @@ extern "C" int f25_gradient_dev(f25_mesh *m, int ntrip, const double *omega, con
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
     HIPCHK(hipSetDevice(m->dev));
     TRY(stage_omky(m, ntrip, omega, ky));
-    return grad_core(m, ntrip, m->d_omky, m->d_omky + m->omky_cap, (const double2 *)u,
-                     (const double2 *)lam, g, 1);
+    return grad_core(m, ntrip, m->d_omky, m->d_omky + m->omky_cap, u, lam, g, 1);
+}
+
+extern "C" int f25_gradient32_dev(f25_mesh *m, int ntrip, const double *omega, const double *ky,
+                                  const void *u, const void *lam, double *g) {
+    if (m) TRY(require_elastic(m));
+    if (!m || ntrip <= 0 || !u || !lam || !g) FAIL(F25_ERR_ARG, "invalid gradient arguments");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(stage_omky(m, ntrip, omega, ky));
+    return grad_core(m, ntrip, m->d_omky, m->d_omky + m->omky_cap, u, lam, g, 1, true);
 }
 
 static int grad_host(f25_mesh *m, int ntrip, const double *omega, const double *ky, const void *u,
@@ -2183,6 +2231,105 @@ extern "C" int f25_gnmfree_dev(f25_mesh *m, int njac, const int *nobs, const dou
     if (!m->have_inv) FAIL(F25_ERR_STATE, "inversion structures missing (f25_inv_setup / f25_inv_attach)");
     HIPCHK(hipSetDevice(m->dev));
     return gnmfree_core(m, njac, nobs, alpha, (const float2 *)jac, p, z);
+}
+
+// ---- Gauss-Newton Hessian over the local columns ---------------------------
+extern "C" int f25_gn_hessian_dev(f25_mesh *m, int njac, const int *nobs, const double *alpha, const void *jac,
+                                  double *h) {
+    if (!m || njac <= 0 || !nobs || !alpha || !jac || !h) FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!m->have_inv) FAIL(F25_ERR_STATE, "inversion structures missing (f25_inv_setup / f25_inv_attach)");
+    for (int j = 0; j < njac; ++j)
+        if (!(alpha[j] >= 0.0)) FAIL(F25_ERR_ARG, "alpha must be non-negative");
+    HIPCHK(hipSetDevice(m->dev));
+    const int n = m->inv.nnpgl;
+    std::vector<int64_t> joff, poff;
+    int64_t jt, pt;
+    TRY(gn_offsets(njac, nobs, n, joff, poff, jt, pt));
+    if (!m->blas) {
+        if (rocblas_create_handle(&m->blas) != rocblas_status_success) {
+            m->blas = nullptr;
+            FAIL(F25_ERR_HIP, "rocblas_create_handle failed");
+        }
+    }
+    if (rocblas_set_stream(m->blas, m->st) != rocblas_status_success) FAIL(F25_ERR_HIP, "rocblas_set_stream failed");
+    // Jacobians in groups whose stacked rows fit a 1 GiB FP64 buffer; one
+    // SYRK (H += A^T A, upper triangle) per group
+    const int64_t cap_rows = std::max<int64_t>(2 * (int64_t)*std::max_element(nobs, nobs + njac),
+                                               (int64_t(1) << 27) / std::max(n, 1));
+    double *a = nullptr;
+    TRY(dalloc(&a, (size_t)cap_rows * n));
+    const float2 *J = static_cast<const float2 *>(jac);
+    const double one = 1.0;
+    int r = 0;
+    (void)hipEventRecord(m->ev0, m->st);
+    for (int j0 = 0; j0 < njac && !r;) {
+        int64_t k = 0;
+        int j1 = j0;
+        while (j1 < njac && k + 2 * nobs[j1] <= cap_rows) k += 2 * nobs[j1++];
+        int64_t roff = 0;
+        for (int j = j0; j < j1; ++j) {
+            if (!nobs[j]) continue;
+            const int64_t cnt = (int64_t)nobs[j] * n;
+            const int nb = (int)std::min<int64_t>((cnt + 255) / 256, 16384);
+            hipLaunchKernelGGL(k_jac_rows, dim3(nb), dim3(256), 0, m->st, n, nobs[j], std::sqrt(alpha[j]),
+                               J + joff[j], a, k, roff);
+            roff += 2 * nobs[j];
+        }
+        if (k > 0 && rocblas_dsyrk(m->blas, rocblas_fill_upper, rocblas_operation_transpose, n, (rocblas_int)k, &one,
+                                   a, (rocblas_int)k, &one, h, n) != rocblas_status_success) {
+            g_err = "rocblas_dsyrk failed";
+            r = F25_ERR_HIP;
+        }
+        j0 = j1;
+    }
+    if (!r) {
+        const int nb = (int)std::min<int64_t>(((int64_t)n * n + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_sym_lower, dim3(nb), dim3(256), 0, m->st, n, h);
+        (void)hipEventRecord(m->ev1, m->st);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+        float ms = 0.f;
+        if (!r && hipEventElapsedTime(&ms, m->ev0, m->ev1) == hipSuccess) record_ms("gn_hessian", ms);
+    }
+    (void)hipStreamSynchronize(m->st);
+    dfree(a);
+    return r;
+}
+
+// ---- Jacobian files (pchadj.f90:340-397) --------------------------------------
+// One Fortran sequential unformatted record: 4-byte length, data, length.
+static bool write_record(FILE *f, const void *p, size_t bytes) {
+    if (bytes > 0x7fffffffu) return false;
+    const int32_t n = (int32_t)bytes;
+    return fwrite(&n, 4, 1, f) == 1 && (bytes == 0 || fwrite(p, 1, bytes, f) == bytes) && fwrite(&n, 4, 1, f) == 1;
+}
+
+extern "C" int f25_write_jac(const char *path, int nrhsf, int nnpgl, int ndim, int nrec, const int *lobs,
+                             const void *jac, int jac_on_device, const int *mygrad) {
+    if (!path || nrhsf < 0 || nnpgl < 0 || ndim <= 0 || nrec < 0 || !lobs || !jac || !mygrad)
+        FAIL(F25_ERR_ARG, "invalid arguments");
+    const size_t nv = (size_t)nrhsf * nnpgl;
+    std::vector<float> c(2 * nv);
+    if (jac_on_device) {
+        HIPCHK(hipMemcpy(c.data(), jac, nv * sizeof(float2), hipMemcpyDeviceToHost));
+    } else {
+        std::memcpy(c.data(), jac, nv * sizeof(float2));
+    }
+    std::vector<float> re(nv), im(nv);
+    for (size_t i = 0; i < nv; ++i) {
+        re[i] = c[2 * i];
+        im[i] = c[2 * i + 1];
+    }
+    FILE *f = std::fopen(path, "wb");
+    if (!f) FAIL(F25_ERR_ARG, "cannot open the Jacobian file");
+    const int32_t hdr[4] = {nrhsf, nnpgl, ndim, nrec};
+    bool ok = write_record(f, hdr, sizeof(hdr)) && write_record(f, lobs, sizeof(int32_t) * (size_t)ndim * nrec) &&
+              write_record(f, re.data(), sizeof(float) * nv) && write_record(f, im.data(), sizeof(float) * nv) &&
+              write_record(f, mygrad, sizeof(int32_t) * (size_t)nnpgl);
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) FAIL(F25_ERR_ARG, "writing the Jacobian file failed");
+    return 0;
 }
 
 extern "C" int f25_gnmfree(f25_mesh *m, int njac, const int *nobs, const double *alpha, const void *jac,

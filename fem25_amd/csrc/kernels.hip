@@ -851,6 +851,46 @@ __global__ __launch_bounds__((64 * 3 * NRT), (VShape<NL, NRT>::MINW)) void k_app
     else apply_v_tile<NL, NRT, 2>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
 }
 
+// Residual of the batched product: r = f - r (r holds S u on entry) for
+// [dof][pair] vectors, and per block y of dofs the partial sums of |r|^2 and
+// |f|^2 for each pair: part[(y * npair + pair) * 2 + {0, 1}].  Grid
+// (pair groups of 64, gy); a block's four waves take DOFs y*4 + w, stepping
+// by 4 gy; lane = pair.
+__global__ __launch_bounds__(256) void k_resid(int64_t ndof, int npair, const double2 *__restrict__ f,
+                                               double2 *__restrict__ r, double *__restrict__ part) {
+    __shared__ double red[4][64][2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int pair = blockIdx.x * 64 + lane;
+    double sr = 0.0, sf = 0.0;
+    if (pair < npair) {
+        for (int64_t d = (int64_t)blockIdx.y * 4 + w; d < ndof; d += (int64_t)gridDim.y * 4) {
+            const size_t i = (size_t)d * npair + pair;
+            const double2 fv = f[i], sv = r[i];
+            const double2 rv = make_double2(fv.x - sv.x, fv.y - sv.y);
+            r[i] = rv;
+            sr += rv.x * rv.x + rv.y * rv.y;
+            sf += fv.x * fv.x + fv.y * fv.y;
+        }
+    }
+    red[w][lane][0] = sr;
+    red[w][lane][1] = sf;
+    __syncthreads();
+    if (w == 0 && pair < npair) {
+        const size_t o = ((size_t)blockIdx.y * npair + pair) * 2;
+        part[o] = red[0][lane][0] + red[1][lane][0] + red[2][lane][0] + red[3][lane][0];
+        part[o + 1] = red[0][lane][1] + red[1][lane][1] + red[2][lane][1] + red[3][lane][1];
+    }
+}
+
+// out[pair * 2 + k] = sum over y = 0..gy-1 of part[(y * npair + pair) * 2 + k], in order
+__global__ void k_resid_sum(int gy, int npair, const double *__restrict__ part, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * npair) return;
+    double s = 0.0;
+    for (int y = 0; y < gy; ++y) s += part[(size_t)y * 2 * npair + i];
+    out[i] = s;
+}
+
 // ---------------------------------------------------------------------------
 // CSR basis: V0/V1/V2 per slot (see combine), scattered from element blocks.
 __device__ inline int64_t find_col(const int *__restrict__ jc, int64_t b, int64_t e, int col) {
@@ -1455,13 +1495,23 @@ __device__ __forceinline__ void sf_row_sums(double2 &U, double2 &UX, double2 &UE
      ...);
 }
 
-template <int NL, int T>
+// R = double: FP64 fields and arithmetic.  R = float: complex64 fields, the
+// interpolation and the densities of each triple in FP32, their sums over
+// triples, quadrature points and corners in FP64 (the FP32 gradient); its
+// second stage reads T1, T2 from LDS (no DPP broadcast).
+template <typename R> struct CxOf;
+template <> struct CxOf<double> { typedef double2 T; };
+template <> struct CxOf<float> { typedef float2 T; };
+
+template <int NL, int T, typename R = double>
 __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) void k_grad_sf(
     const double *__restrict__ t1d, const double *__restrict__ wq, const double *__restrict__ xq,
     const int *__restrict__ ieng, const double *__restrict__ xl, const double *__restrict__ zl,
     const char *__restrict__ cdom, const int *__restrict__ lm, int ntrip, int64_t ndof,
-    const double *__restrict__ omega, const double *__restrict__ ky, const double2 *__restrict__ u,
-    const double2 *__restrict__ l, double *__restrict__ part, int *__restrict__ err) {
+    const double *__restrict__ omega, const double *__restrict__ ky, const typename CxOf<R>::T *__restrict__ u,
+    const typename CxOf<R>::T *__restrict__ l, double *__restrict__ part, int *__restrict__ err) {
+    typedef typename CxOf<R>::T C2;
+    constexpr bool F64 = std::is_same<R, double>::value;
     constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen;
     // Q1 = 17 (p6): rows 0..16 of 16 lanes hold xi points 0..15 and the 17
     // lanes after them hold xi point 16 of eta rows 0..16 (five waves)
@@ -1471,10 +1521,10 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     constexpr int NG = (NW + NT - 1) / NT;     // of them per thread
     static_assert(Q1 <= 17, "one row of 16 lanes per eta point, plus one column");
     extern __shared__ double sm[];
-    double2 *sw = reinterpret_cast<double2 *>(sm);                     // [T][2][nee]
-    double2 *sT = sw + NW;                                             // [T][2][3][Q1][N1][2]
-    double *sG = reinterpret_cast<double *>(sT + T * 2 * 3 * Q1 * N1 * 2);   // [Q1*Q1][4]
-    double *s1 = sG + 4 * Q1 * Q1;                                     // [2][N1][Q1]
+    C2 *sw = reinterpret_cast<C2 *>(sm);                               // [T][2][nee]
+    C2 *sT = sw + NW;                                                  // [T][2][3][Q1][N1][2]
+    R *sG = reinterpret_cast<R *>(sT + T * 2 * 3 * Q1 * N1 * 2);       // [Q1*Q1][4]
+    R *s1 = sG + 4 * Q1 * Q1;                                          // [2][N1][Q1]
     __shared__ double red[NT / 64][12];
     const int e = blockIdx.x, tid = threadIdx.x;
     if (cdom[e] == 'A') {   // PML elements are outside the non-PML gradient path
@@ -1487,10 +1537,11 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
         const int ix = q % Q1, iz = q / Q1;
         Geo g;
         if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
-        double *G = sG + 4 * q;
-        G[0] = g.dzdeta / g.det; G[1] = -g.dzdxi / g.det; G[2] = -g.dxdeta / g.det; G[3] = g.dxdxi / g.det;
+        R *G = sG + 4 * q;
+        G[0] = (R)(g.dzdeta / g.det); G[1] = (R)(-g.dzdxi / g.det); G[2] = (R)(-g.dxdeta / g.det);
+        G[3] = (R)(g.dxdxi / g.det);
     }
-    for (int i = tid; i < 2 * N1 * Q1; i += NT) s1[i] = t1d[i];   // v_j(eta_q), then v'_j(eta_q)
+    for (int i = tid; i < 2 * N1 * Q1; i += NT) s1[i] = (R)t1d[i];   // v_j(eta_q), then v'_j(eta_q)
     // the gather slots of this thread are the same in every pass: slot r is
     // value tid + r NT of [T][2][nee]; its DOF (or -1) is fixed per element
     int sd[NG];
@@ -1501,14 +1552,14 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     }
     // the values of the next pass are loaded into registers before this
     // pass's arithmetic and written to LDS after it
-    double2 pre[NG];
+    C2 pre[NG];
     auto gather = [&](int t0) {
 #pragma unroll
         for (int r = 0; r < NG; ++r) {
             const int i = tid + r * NT;
             const int tt = i / (2 * nee), f = (i / nee) % 2, t = t0 + tt;
-            const double2 *w = f ? l : u;
-            pre[r] = (sd[r] >= 0 && t < ntrip) ? w[(size_t)t * ndof + sd[r]] : make_double2(0.0, 0.0);
+            const C2 *w = f ? l : u;
+            pre[r] = (sd[r] >= 0 && t < ntrip) ? w[(size_t)t * ndof + sd[r]] : C2{R(0), R(0)};
         }
     };
     gather(0);
@@ -1519,9 +1570,9 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     const bool act = (row < Q1 && x16 < Q1) || ovl;
     const int qe = ovl ? tid - 16 * Q1 : min(row, Q1 - 1), qx = ovl ? 16 : min(x16, Q1 - 1);
     const int ni = min(x16, N1 - 1);
-    double vx[N1], dx[N1];
+    R vx[N1], dx[N1];
 #pragma unroll
-    for (int i = 0; i < N1; ++i) { vx[i] = t1d[i * Q1 + qx]; dx[i] = t1d[(N1 + i) * Q1 + qx]; }
+    for (int i = 0; i < N1; ++i) { vx[i] = (R)t1d[i * Q1 + qx]; dx[i] = (R)t1d[(N1 + i) * Q1 + qx]; }
     double Dl = 0.0, Dm = 0.0, Dr = 0.0;
     for (int t0 = 0; t0 < ntrip; t0 += T) {
         __syncthreads();   // the previous pass is done with sw, sT (and sG, s1 are written)
@@ -1532,82 +1583,89 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
         for (int it = tid; it < T * 2 * 3 * Q1 * N1; it += NT) {
             const int i = it % N1, qy = (it / N1) % Q1, fc = it / (N1 * Q1);   // fc = (tt*2 + f)*3 + c
             const int c = fc % 3;
-            const double2 *w = sw + (fc / 3) * nee;
-            double2 t1 = make_double2(0.0, 0.0), t2 = make_double2(0.0, 0.0);
+            const C2 *w = sw + (fc / 3) * nee;
+            C2 t1 = {R(0), R(0)}, t2 = t1;
 #pragma unroll
             for (int j = 0; j < N1; ++j) {
-                const double2 a = w[(j * N1 + i) * 3 + c];
-                const double vv = s1[j * Q1 + qy], dd = s1[(N1 + j) * Q1 + qy];
+                const C2 a = w[(j * N1 + i) * 3 + c];
+                const R vv = s1[j * Q1 + qy], dd = s1[(N1 + j) * Q1 + qy];
                 t1.x += vv * a.x; t1.y += vv * a.y;
                 t2.x += dd * a.x; t2.y += dd * a.y;
             }
-            double2 *o = sT + (size_t)((fc * Q1 + qy) * N1 + i) * 2;
+            C2 *o = sT + (size_t)((fc * Q1 + qy) * N1 + i) * 2;
             o[0] = t1;
             o[1] = t2;
         }
-        double kt[T], wt[T];
+        R kt[T], wt[T];
 #pragma unroll
         for (int tt = 0; tt < T; ++tt) {
             const int t = min(t0 + tt, ntrip - 1);
-            kt[tt] = ky[t];
-            wt[tt] = omega[t];
+            kt[tt] = (R)ky[t];
+            wt[tt] = (R)omega[t];
         }
         __syncthreads();
         if (t0 + T < ntrip) gather(t0 + T);
-        const double *G = sG + 4 * (qx + Q1 * qe);
-        const double rx = G[0], sx = G[1], rz = G[2], sz = G[3];
+        const R *G = sG + 4 * (qx + Q1 * qe);
+        const R rx = G[0], sx = G[1], rz = G[2], sz = G[3];
 #pragma unroll
         for (int tt = 0; tt < T; ++tt) {
             if (t0 + tt >= ntrip) break;
-            const double k = kt[tt], w = wt[tt];
+            const R k = kt[tt], w = wt[tt];
             // gradient of component c of field f at this point: [d/dx, d/dy, d/dz]
-            auto grad = [&](int f, int c, double2 &U, double2 *Gc) {
-                const double2 *o = sT + (size_t)((((tt * 2 + f) * 3 + c) * Q1 + qe) * N1 + ni) * 2;
-                const double2 T1 = o[0], T2 = o[1];
-                double2 UXI = make_double2(0.0, 0.0), UET = UXI;
+            auto grad = [&](int f, int c, C2 &U, C2 *Gc) {
+                const C2 *o = sT + (size_t)((((tt * 2 + f) * 3 + c) * Q1 + qe) * N1 + ni) * 2;
+                const C2 T1 = o[0], T2 = o[1];
+                C2 UXI = {R(0), R(0)}, UET = UXI;
                 U = UXI;
-                if (!ovl) {
-                    sf_row_sums<NL, 0>(U, UXI, UET, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
-                } else {   // the xi = 16 column reads the row's T1, T2 from LDS
-                    const double2 *r = o - 2 * ni;
+                // the FP64 form broadcasts T1, T2 along the row by DPP; the
+                // FP32 form, and the xi = 16 column, read the row's T1, T2
+                // from LDS
+                bool bc = !ovl;
+                if constexpr (F64) {
+                    if (bc) sf_row_sums<NL, 0>(U, UXI, UET, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+                } else {
+                    bc = false;
+                }
+                if (!bc) {
+                    const C2 *r = o - 2 * ni;
 #pragma unroll
                     for (int j = 0; j < N1; ++j) {
-                        const double2 a1 = r[2 * j], a2 = r[2 * j + 1];
+                        const C2 a1 = r[2 * j], a2 = r[2 * j + 1];
                         U.x += vx[j] * a1.x; U.y += vx[j] * a1.y;
                         UXI.x += dx[j] * a1.x; UXI.y += dx[j] * a1.y;
                         UET.x += vx[j] * a2.x; UET.y += vx[j] * a2.y;
                     }
                 }
-                Gc[0] = make_double2(rx * UXI.x + sx * UET.x, rx * UXI.y + sx * UET.y);
-                Gc[1] = make_double2(k * U.y, -k * U.x);   // d/dy = -i ky
-                Gc[2] = make_double2(rz * UXI.x + sz * UET.x, rz * UXI.y + sz * UET.y);
+                Gc[0] = C2{rx * UXI.x + sx * UET.x, rx * UXI.y + sx * UET.y};
+                Gc[1] = C2{k * U.y, -k * U.x};   // d/dy = -i ky
+                Gc[2] = C2{rz * UXI.x + sz * UET.x, rz * UXI.y + sz * UET.y};
             };
             // forward field: its values, divergence and symmetrised gradient
             // S[i][j] = G_ij + G_ji; then the adjoint field's components one
             // at a time against them
-            double2 GU[3][3], Uv[3];
+            C2 GU[3][3], Uv[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) grad(0, c, Uv[c], GU[c]);
-            const double2 du = make_double2(GU[0][0].x + GU[1][1].x + GU[2][2].x, GU[0][0].y + GU[1][1].y + GU[2][2].y);
-            double2 S[3][3];
+            const C2 du = C2{GU[0][0].x + GU[1][1].x + GU[2][2].x, GU[0][0].y + GU[1][1].y + GU[2][2].y};
+            C2 S[3][3];
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) S[i][j] = make_double2(GU[i][j].x + GU[j][i].x, GU[i][j].y + GU[j][i].y);
-            double2 dv = make_double2(0.0, 0.0);
-            double sm2 = 0.0, vu = 0.0;
+                for (int j = 0; j < 3; ++j) S[i][j] = C2{GU[i][j].x + GU[j][i].x, GU[i][j].y + GU[j][i].y};
+            C2 dv = {R(0), R(0)};
+            R sm2 = R(0), vu = R(0);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                double2 V, GV[3];
+                C2 V, GV[3];
                 grad(1, c, V, GV);
-                dv = c ? make_double2(dv.x + GV[c].x, dv.y + GV[c].y) : GV[0];
+                dv = c ? C2{dv.x + GV[c].x, dv.y + GV[c].y} : GV[0];
 #pragma unroll
                 for (int j = 0; j < 3; ++j) sm2 += GV[j].x * S[c][j].x + GV[j].y * S[c][j].y;
                 vu += V.x * Uv[c].x + V.y * Uv[c].y;
             }
-            Dl += dv.x * du.x + dv.y * du.y;
-            Dm += sm2;
-            Dr += -w * w * vu;
+            Dl += (double)(dv.x * du.x + dv.y * du.y);
+            Dm += (double)sm2;
+            Dr += -(double)w * (double)w * (double)vu;
         }
     }
     // projection on the anchor hat functions (weights recomputed here, as the
@@ -2448,6 +2506,31 @@ __global__ void k_gn_z(int nnpgl, int njac, const int *__restrict__ nobs, const 
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (lane == 0) z[mygrad[c] - 1] += s;
+}
+
+// Rows of the Gauss-Newton Hessian's factor for Jacobian j (nobs x nnpgl
+// COMPLEX*8, column-major): the real parts in rows roff .. roff+no-1 and the
+// imaginary parts in the next no rows of the FP64 column-major matrix A
+// (leading dimension lda), scaled by sa = sqrt(alpha_j).
+__global__ void k_jac_rows(int nnpgl, int no, double sa, const float2 *__restrict__ jac, double *__restrict__ a,
+                           int64_t lda, int64_t roff) {
+    const int64_t n = (int64_t)no * nnpgl;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = i / no, o = i - c * no;
+        const float2 v = jac[i];
+        a[c * lda + roff + o] = sa * (double)v.x;
+        a[c * lda + roff + no + o] = sa * (double)v.y;
+    }
+}
+
+// H(j, i) = H(i, j) for i < j (column-major n x n): the lower triangle from
+// the upper one.
+__global__ void k_sym_lower(int n, double *__restrict__ h) {
+    const int64_t tot = (int64_t)n * n;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < tot; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = k / n, i = k - j * n;   // element (i, j)
+        if (i > j) h[k] = h[j + i * (int64_t)n];
+    }
 }
 
 // ---------------------------------------------------------------------------

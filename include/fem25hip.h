@@ -141,6 +141,16 @@ int f25_asmb_peff(f25_mesh *m, const void *cdist, const void *ue, void *peff);
  * F25_APPLY_F64 and F25_APPLY_F32_REFINED, float complex for F25_APPLY_F32. */
 int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
                         int mode, const void *x_dev, void *y_dev);
+/* Residual for iterative refinement of the forward solves (and of the
+ * adjoint solves on meshes without PML, where S is Hermitian): the
+ * reference factorises S in single precision (CMUMPS, bielak25.f90:203,
+ * adjgrad.f90:329); a solution u of that factorisation is refined by
+ * u += S32^-1 r with the FP64 residual
+ *   r_b = f_b - S(omega_b, ky_b) u_b   (the operator of f25_apply_batch_dev)
+ * for every pair, and rnorm[b] = |r_b|_2 / |f_b|_2 (host array of npair).
+ * u, f, r: distinct [dof][pair] double complex device arrays. */
+int f25_residual_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
+                           const void *u_dev, const void *f_dev, void *r_dev, double *rnorm);
 /* Host convenience wrapper: x, y are [pair][dof] double complex (one vector
  * per pair, NDOF contiguous, i.e. Fortran X(NDOF,NPAIR)). */
 int f25_apply_batch(f25_mesh *m, int npair, const double *omega, const double *ky, int mode,
@@ -167,6 +177,13 @@ int f25_gradient(f25_mesh *m, int ntrip, const double *omega, const double *ky,
                  const void *u, const void *lam, double *g);
 int f25_gradient_dev(f25_mesh *m, int ntrip, const double *omega, const double *ky,
                      const void *u_dev, const void *lam_dev, double *g_dev);
+
+/* The same gradient from complex64 fields (u, lam: [trip][dof] float complex
+ * device arrays): the interpolation to the quadrature points and the
+ * densities of each triple in FP32, their sums over triples, points,
+ * elements and the chain rule in FP64; g FP64 as above. */
+int f25_gradient32_dev(f25_mesh *m, int ntrip, const double *omega, const double *ky,
+                       const void *u_dev, const void *lam_dev, double *g_dev);
 
 /* ---- reference virtual-force path (CFMAT_DIST / ADJFLAM_DIST) ------------ */
 
@@ -243,6 +260,26 @@ int f25_gnmfree(f25_mesh *m, int njac, const int *nobs, const double *alpha, con
 int f25_gnmfree_dev(f25_mesh *m, int njac, const int *nobs, const double *alpha, const void *jac_dev,
                     const double *p_dev, double *z_dev);
 
+/* The Gauss-Newton Hessian over the local gradient columns, the block
+ * GENHLOC accumulates from the saved Jacobians (gnutils.f90:2408-2660:
+ * J^T J with DMAG2 = |STF|^2, the Jacobian's real and imaginary parts as
+ * separate rows, pchadj.f90:256-262):
+ *   H(c1, c2) += sum_j alpha[j] Re( J_j(:, c1)^H J_j(:, c2) ),  c1, c2 = 1..NNPGL
+ * H: NNPGL x NNPGL FP64 column-major device array, symmetric on entry
+ * (e.g. zero) and on return.  jac_dev packed as for f25_gnmfree (COMPLEX*8
+ * on the device); nobs, alpha (>= 0) host arrays.  FP64 products of the
+ * FP32 entries (exact) summed by FP64 MFMA (rocBLAS DSYRK). */
+int f25_gn_hessian_dev(f25_mesh *m, int njac, const int *nobs, const double *alpha, const void *jac_dev,
+                       double *h_dev);
+/* The Jacobian file of one (frequency, source) in the record layout PCHADJ
+ * writes (pchadj.f90:371-395, Fortran sequential unformatted: 4-byte record
+ * lengths): (NRHSF, NNPGL, NDIM, NREC), LOBS(NDIM, NREC) LOGICAL*4 as given,
+ * Re J and Im J (NRHSF x NNPGL REAL*4, rows fastest), MYGRAD(NNPGL).  jac:
+ * JACL(NRHSF, NNPGL) COMPLEX*8 as f25_pchadj_jac writes it, on the device
+ * when jac_on_device != 0.  path: the caller's file name (the reference uses
+ * ./scratch/jac_<rank>-<FREQ F12.5>-<ISRC>.dat). */
+int f25_write_jac(const char *path, int nrhsf, int nnpgl, int ndim, int nrec, const int *lobs, const void *jac,
+                  int jac_on_device, const int *mygrad);
 /* GNMFREE's two products on caller-held arrays (no handle), for a caller
  * that sums JP over ranks in between as gnmfree.f90:195-196 does:
  *   f25_gn_jp:  JP_j(obs) = sum_c J_j(obs,c) p(MYGRAD(c)), jp COMPLEX*16,

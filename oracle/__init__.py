@@ -601,6 +601,27 @@ def pchadj_jac(inv: Inv, nrhs, lrhs, g, fmat):
     return jac, hd
 
 
+def gn_hessian(nobs, alpha, nnpgl, jac):
+    """GENHLOC's local J^T J: (nnpgl, nnpgl) FP64, H[c2, c1] in row-major =
+    sum_j alpha_j Re(J_j(:,c1)^H J_j(:,c2)) (symmetric)."""
+    nobs = np.ascontiguousarray(nobs, np.int32)
+    alpha = np.ascontiguousarray(alpha, np.float64)
+    jac = np.ascontiguousarray(jac, np.complex64)
+    h = np.zeros((nnpgl, nnpgl))
+    lib().orc_gn_hessian(len(nobs), _p(nobs, ip), _p(alpha, dp), nnpgl, _p(jac, fp), _p(h, dp))
+    return h
+
+
+def ref_write_jac(path, nrhsf, nnpgl, ndim, nrec, lobs, jac, mygrad):
+    """The Jacobian file written by PCHADJ's WRITE statements (pchadj.f90:371-395),
+    compiled with the reference's compiler (harness REF_WRITE_JAC)."""
+    lobs = np.ascontiguousarray(lobs, np.int32)
+    jac = np.ascontiguousarray(jac, np.complex64)
+    mygrad = np.ascontiguousarray(mygrad, np.int32)
+    b = os.fsencode(path)
+    ref().ref_write_jac(C.c_char_p(b), len(b), nrhsf, nnpgl, ndim, nrec, _p(lobs, ip), _p(jac, fp), _p(mygrad, ip))
+
+
 def gnmfree(nobs, alpha, nnpgl, mygrad, na35, jac, p):
     """GNMFREE's z = sum_j alpha_j Re(J_j^H J_j p(MYGRAD)); jac packed (j, c, obs)."""
     nobs = np.ascontiguousarray(nobs, np.int32)

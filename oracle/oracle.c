@@ -2062,6 +2062,31 @@ int orc_grad_exact(const orc_asm *a, double freq, double py, const double comple
 }
 
 /* ------------------------------------------------------------------------ */
+/* gnutils.f90:2600-2630 GENHLOC: H += DMAG2 * J^T J per (frequency, source)
+ * with J's rows the real parts then the imaginary parts of the saved
+ * Jacobian (REAL*4 read into REAL*8), i.e. H(c1, c2) += alpha_j sum_o
+ * (Re J(o,c1) Re J(o,c2) + Im J(o,c1) Im J(o,c2)) in FP64. */
+void orc_gn_hessian(int njac, const int *nobs, const double *alpha, int nnpgl, const float complex *jac,
+                    double *h)
+{
+    size_t off = 0;
+    for (int j = 0; j < njac; j++) {
+        const int no = nobs[j];
+        const float complex *J = jac + off;
+        for (int c2 = 0; c2 < nnpgl; c2++)
+            for (int c1 = 0; c1 < nnpgl; c1++) {
+                double s = 0.0;
+                for (int o = 0; o < no; o++) {
+                    float complex a = J[(size_t)c1 * no + o], b = J[(size_t)c2 * no + o];
+                    s += (double)crealf(a) * (double)crealf(b) + (double)cimagf(a) * (double)cimagf(b);
+                }
+                h[(size_t)c2 * nnpgl + c1] += alpha[j] * s;
+            }
+        off += (size_t)no * nnpgl;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* pchadj.f90:225-281 with NVINV = 1: the loop over local gradient columns,
  * receivers and components (KRHS = IRHS), the COMPLEX*8 inner product of a
  * row of G^T with column INPGL of F (pchadj.f90:239-254), the Jacobian copy

@@ -211,6 +211,10 @@ void orc_pchadj_jac(int nnpgl, int nrhs, int lrhs, const int *mygrad, const int 
  * 0..njac-1 packed one after the other (nobs[j] x nnpgl, column-major),
  *   JP = J p(MYGRAD) (COMPLEX*16), RWORK(MYGRAD(c)) = Re(J(:,c)^H JP),
  *   z += alpha[j] RWORK. */
+/* GENHLOC's local J^T J (gnutils.f90:2600-2630) on packed COMPLEX*8
+ * Jacobians: h (nnpgl x nnpgl, column-major) += sum_j alpha_j Re(J_j^H J_j) */
+void orc_gn_hessian(int njac, const int *nobs, const double *alpha, int nnpgl, const float complex *jac,
+                    double *h);
 void orc_gnmfree(int njac, const int *nobs, const double *alpha, int nnpgl, const int *mygrad,
                  int na35, const float complex *jac, const double *p, double *z);
 

@@ -363,6 +363,28 @@ def test_gradient_exact(name, nf, nk):
         assert relerr(gd[:, p], ref[:, p]) < 1e-11, p
 
 
+@pytest.mark.parametrize("name", ["tiny_layered", "pml", "p5", "p6"])
+def test_gradient_fp32(torch_dev, name):
+    """The FP32 gradient (complex64 fields, FP32 interpolation and densities,
+    FP64 sums) against the FP64 gradient per field: at FP32 level (1e-5
+    normwise; the fields alone are rounded to 6e-8)."""
+    torch = torch_dev
+    mi = _mi(name)
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    om, ky, f, py = pairs(2, 3)
+    u = torch.from_numpy(cf.seeded_complex(g.ndof, batch=om.size, seed=23)).cuda()
+    lam = torch.from_numpy(cf.seeded_complex(g.ndof, batch=om.size, seed=24)).cuda()
+    g64 = torch.zeros(m.nnpg, 3, dtype=torch.float64, device="cuda")
+    g32 = torch.zeros_like(g64)
+    mi.gradient_dev(om, ky, u, lam, g64)
+    mi.gradient32_dev(om, ky, u.to(torch.complex64), lam.to(torch.complex64), g32)
+    for p in range(3):
+        e = float(torch.linalg.norm(g32[:, p] - g64[:, p]) / torch.linalg.norm(g64[:, p]))
+        print(f"{name} field {p}: fp32 gradient rel err {e:.2e}")
+        assert e < 1e-5, (p, e)
+
+
 @pytest.mark.parametrize("name,cinv", [("tiny_layered", b"PP"), ("pml", b"PP"), ("tiny", b"SS"), ("p5", b"PP"),
                                        ("p6", b"SS")])
 def test_cfmat_adjflam(name, cinv):
@@ -617,6 +639,46 @@ def test_gnmfree_repeatable():
     np.testing.assert_array_equal(z1, z2)
     zref = oracle.gnmfree(nobs, alpha, inv.nnpgl, inv.mygrad, inv.na35, jac, p)
     assert relerr(z1, zref) < 1e-12
+
+
+def test_gn_hessian(torch_dev):
+    """GENHLOC's local J^T J over the 861 columns of the 40 x 20 mesh for 10
+    Jacobians with uneven observation counts: FP64 MFMA (SYRK) against the
+    oracle, exactly symmetric, accumulated by a second call; the Jacobian
+    file written from device memory equals the one written from host
+    memory."""
+    torch = torch_dev
+    from fem25_amd import _lib
+    L = _lib.load()
+    mi = _mi("c1")
+    inv = mi.inv_setup(b"PP")
+    n = inv.nnpgl
+    rng = np.random.default_rng(94)
+    nobs = (3 * rng.integers(20, 121, size=10)).astype(np.int32)
+    tot = int(nobs.sum()) * n
+    jac = (rng.standard_normal(tot) + 1j * rng.standard_normal(tot)).astype(np.complex64)
+    alpha = rng.uniform(0.5, 2.0, size=nobs.size)
+    jd = torch.from_numpy(jac).cuda()
+    h = torch.zeros(n, n, dtype=torch.float64, device="cuda")
+    args = (mi.h, nobs.size, nobs.ctypes.data_as(_lib.ip), alpha.ctypes.data_as(_lib.dp), C.c_void_p(jd.data_ptr()),
+            C.c_void_p(h.data_ptr()))
+    _lib.check(L.f25_gn_hessian_dev(*args), "f25_gn_hessian_dev")
+    hh = h.cpu().numpy()
+    ref = oracle.gn_hessian(nobs, alpha, n, jac)
+    assert np.abs(hh - ref).max() <= 1e-12 * np.abs(ref).max()
+    np.testing.assert_array_equal(hh, hh.T)
+    _lib.check(L.f25_gn_hessian_dev(*args), "f25_gn_hessian_dev")
+    assert np.abs(h.cpu().numpy() - 2 * ref).max() <= 1e-12 * np.abs(ref).max()
+    import os
+    import tempfile
+    no = int(nobs[0])
+    lobs = np.ones((no // 3, 3), np.int32)
+    with tempfile.TemporaryDirectory() as d:
+        pa, pb = os.path.join(d, "dev.dat"), os.path.join(d, "host.dat")
+        for path, ptr, ondev in ((pa, C.c_void_p(jd.data_ptr()), 1), (pb, jac.ctypes.data_as(C.c_void_p), 0)):
+            _lib.check(L.f25_write_jac(path.encode(), no, n, 3, no // 3, lobs.ctypes.data_as(_lib.ip), ptr, ondev,
+                                       inv.mygrad.ctypes.data_as(_lib.ip)), "f25_write_jac")
+        assert open(pa, "rb").read() == open(pb, "rb").read()
 
 
 def test_pml_anchors_fixed():

@@ -325,6 +325,46 @@ def test_asmb_peff_closed_form(case, freq, py):
     assert np.all(got[other] == 0)
 
 
+def test_gn_hessian_vs_dense():
+    """orc_gn_hessian against the dense product sum_j alpha_j Re(J_j^H J_j)."""
+    rng = np.random.default_rng(71)
+    nnpgl, nobs = 9, np.array([4, 7, 1, 5], np.int32)
+    jac = (rng.standard_normal(int(nobs.sum()) * nnpgl)
+           + 1j * rng.standard_normal(int(nobs.sum()) * nnpgl)).astype(np.complex64)
+    alpha = rng.uniform(0.2, 3.0, nobs.size)
+    h = oracle.gn_hessian(nobs, alpha, nnpgl, jac)
+    want = np.zeros((nnpgl, nnpgl))
+    off = 0
+    for j, no in enumerate(nobs):
+        J = jac[off:off + no * nnpgl].astype(np.complex128).reshape(nnpgl, no).T   # (obs, column)
+        want += alpha[j] * (J.conj().T @ J).real
+        off += no * nnpgl
+    assert np.abs(h - want).max() <= 1e-13 * np.abs(want).max()
+    np.testing.assert_array_equal(h, h.T)
+
+
+@needs_ref
+def test_jacobian_writer_vs_reference_statements(tmp_path):
+    """f25_write_jac's bytes equal those of PCHADJ's WRITE statements
+    (pchadj.f90:371-395) compiled with the reference's compiler, names
+    aside; LOBS mixed true/false."""
+    import ctypes as C
+    from fem25_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(72)
+    ndim, nrec, nnpgl = 3, 5, 4
+    nrhsf = ndim * nrec
+    lobs = (rng.random((nrec, ndim)) < 0.7).astype(np.int32)
+    jac = (rng.standard_normal((nnpgl, nrhsf)) + 1j * rng.standard_normal((nnpgl, nrhsf))).astype(np.complex64)
+    mygrad = rng.permutation(np.arange(1, 12))[:nnpgl].astype(np.int32)
+    a, b = str(tmp_path / "mine.dat"), str(tmp_path / "ref.dat")
+    rc = L.f25_write_jac(a.encode(), nrhsf, nnpgl, ndim, nrec, lobs.ctypes.data_as(_lib.ip),
+                         jac.ctypes.data_as(C.c_void_p), 0, mygrad.ctypes.data_as(_lib.ip))
+    assert rc == 0
+    oracle.ref_write_jac(b, nrhsf, nnpgl, ndim, nrec, lobs, jac, mygrad)
+    assert open(a, "rb").read() == open(b, "rb").read()
+
+
 @needs_ref
 def test_gnmfree_vs_reference(tmp_path):
     """GNMFREE restated: bit-exact against the compiled reference, which
