@@ -68,6 +68,7 @@ EXPORTS = {
     "f25_gnmfree_dev": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_void_p, C.c_void_p, C.c_void_p]),
     "f25_gn_hessian_dev": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_void_p, C.c_void_p]),
     "f25_write_jac": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_void_p, C.c_int, ip]),
+    "f25_gnmfree_dev_noalloc": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_void_p, C.c_void_p, C.c_void_p]),
     "f25_gn_jp": (C.c_int, [C.c_int, C.c_int, ip, C.c_int, ip, C.c_void_p, dp, C.c_void_p]),
     "f25_gn_jhz": (C.c_int, [C.c_int, C.c_int, ip, C.c_int, ip, dp, C.c_void_p, C.c_void_p, dp]),
     "f25_qp_nu_alpha": (C.c_int, [C.c_void_p, C.c_double, C.c_double, fp, C.c_void_p, C.c_void_p, C.c_void_p,

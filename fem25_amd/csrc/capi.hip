@@ -2332,6 +2332,75 @@ extern "C" int f25_write_jac(const char *path, int nrhsf, int nnpgl, int ndim, i
     return 0;
 }
 
+// ---- GNMFREE without allocations ----------------------------------------
+// f25_gnmfree_dev_noalloc keeps everything f25_gnmfree_dev allocates per call
+// (the per-Jacobian tables, the JP vectors and the column-chunk partial sums)
+// in one file-scope device array.  Each GPU holds its own copy of it until
+// the process ends; its address on a device comes from hipGetSymbolAddress,
+// taken once per device with that device current.
+static constexpr size_t kScratchBytes = size_t(32) << 20;   // 32 MiB per GPU
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static std::map<int, unsigned char *> g_scratch_ptr;
+
+static int scratch_on(int dev, unsigned char **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) FAIL(F25_ERR_HIP, "hipGetSymbolAddress(scratch) failed");
+        it = g_scratch_ptr.emplace(dev, static_cast<unsigned char *>(ptr)).first;
+    }
+    *out = it->second;
+    return 0;
+}
+
+static size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+extern "C" int f25_gnmfree_dev_noalloc(f25_mesh *m, int njac, const int *nobs, const double *alpha,
+                                       const void *jac, const double *p, double *z) {
+    if (!m || njac <= 0 || njac > 65535 || !nobs || !alpha || !jac || !p || !z) FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!m->have_inv) FAIL(F25_ERR_STATE, "inversion structures missing (f25_inv_setup / f25_inv_attach)");
+    HIPCHK(hipSetDevice(m->dev));
+    unsigned char *base = nullptr;
+    TRY(scratch_on(m->dev, &base));
+    const int n = m->inv.nnpgl;
+    std::vector<int64_t> joff, poff;
+    int64_t jt, pt;
+    TRY(gn_offsets(njac, nobs, n, joff, poff, jt, pt));
+    int nch, cch;
+    gn_jp_chunks(n, nch, cch);
+    // scratch layout: nobs | joff | poff | alpha | JP (pt) | partials (nch * pt)
+    const size_t o_nobs = 0, o_joff = align256(o_nobs + sizeof(int) * njac),
+                 o_poff = align256(o_joff + sizeof(int64_t) * njac),
+                 o_alpha = align256(o_poff + sizeof(int64_t) * njac),
+                 o_jp = align256(o_alpha + sizeof(double) * njac), o_part = align256(o_jp + sizeof(double2) * pt),
+                 need = o_part + sizeof(double2) * (size_t)nch * pt;
+    if (need > kScratchBytes)
+        FAIL(F25_ERR_ARG, "batch too large for the 32 MiB scratch: sum(nobs) * (chunks + 1) * 16 bytes "
+                          "exceeds it; split the Jacobians over calls (z accumulates)");
+    int *dn = reinterpret_cast<int *>(base + o_nobs);
+    int64_t *djo = reinterpret_cast<int64_t *>(base + o_joff), *dpo = reinterpret_cast<int64_t *>(base + o_poff);
+    double *da = reinterpret_cast<double *>(base + o_alpha);
+    double2 *djp = reinterpret_cast<double2 *>(base + o_jp), *dpart = reinterpret_cast<double2 *>(base + o_part);
+    HIPCHK(hipMemcpyAsync(dn, nobs, sizeof(int) * njac, hipMemcpyHostToDevice, m->st));
+    HIPCHK(hipMemcpyAsync(djo, joff.data(), sizeof(int64_t) * njac, hipMemcpyHostToDevice, m->st));
+    HIPCHK(hipMemcpyAsync(dpo, poff.data(), sizeof(int64_t) * njac, hipMemcpyHostToDevice, m->st));
+    HIPCHK(hipMemcpyAsync(da, alpha, sizeof(double) * njac, hipMemcpyHostToDevice, m->st));
+    (void)hipEventRecord(m->ev0, m->st);
+    launch_gn_jp(m->st, n, njac, pt, dn, djo, dpo, m->d_mygrad, (const float2 *)jac, p, dpart, djp);
+    hipLaunchKernelGGL(k_gn_z, dim3((n + 3) / 4), dim3(256), 0, m->st, n, njac, dn, djo, dpo, da, m->d_mygrad,
+                       (const float2 *)jac, djp, z);
+    (void)hipEventRecord(m->ev1, m->st);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(m->st);
+    if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, m->ev0, m->ev1) == hipSuccess) record_ms("gnmfree", ms);
+    return 0;
+}
+
 extern "C" int f25_gnmfree(f25_mesh *m, int njac, const int *nobs, const double *alpha, const void *jac,
                            const double *p, double *z) {
     if (!m || njac <= 0 || njac > 65535 || !nobs || !alpha || !jac || !p || !z) FAIL(F25_ERR_ARG, "invalid arguments");

@@ -280,6 +280,18 @@ int f25_gn_hessian_dev(f25_mesh *m, int njac, const int *nobs, const double *alp
  * ./scratch/jac_<rank>-<FREQ F12.5>-<ISRC>.dat). */
 int f25_write_jac(const char *path, int nrhsf, int nnpgl, int ndim, int nrec, const int *lobs, const void *jac,
                   int jac_on_device, const int *mygrad);
+/* f25_gnmfree_dev without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-call tables, the J p vectors and their column-chunk partial sums live
+ * in a 32 MiB device array of the library, one per GPU, held until the
+ * process ends; the call returns F25_ERR_ARG when
+ *   sum(nobs) * (min(256, ceil(NNPGL / 64)) + 1) * 16 bytes + 32 * njac
+ * exceeds it (split the Jacobians over calls: z accumulates).  Results are
+ * bit for bit those of f25_gnmfree_dev.  Calls on one GPU share the array:
+ * calls that overlap on one GPU (other host threads, other handles) race. */
+int f25_gnmfree_dev_noalloc(f25_mesh *m, int njac, const int *nobs, const double *alpha, const void *jac_dev,
+                            const double *p_dev, double *z_dev);
 /* GNMFREE's two products on caller-held arrays (no handle), for a caller
  * that sums JP over ranks in between as gnmfree.f90:195-196 does:
  *   f25_gn_jp:  JP_j(obs) = sum_c J_j(obs,c) p(MYGRAD(c)), jp COMPLEX*16,

@@ -1,4 +1,5 @@
-"""The allocation-free GNMFREE entry point against the allocating one."""
+"""The allocation-free GNMFREE entry point against the allocating one
+(collected last: it leaves its per-GPU scratch array resident)."""
 import ctypes as C
 
 import numpy as np
@@ -9,14 +10,6 @@ from tests.helpers import mesh_case
 pytestmark = pytest.mark.gpu
 
 
-def _noalloc(L):
-    f = L.f25_gnmfree_dev_noalloc
-    f.restype = C.c_int
-    f.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_void_p, C.c_void_p,
-                  C.c_void_p]
-    return f
-
-
 def test_gnmfree_noalloc_matches_gnmfree_dev():
     """Two calls in a row (the second with fewer and shorter Jacobians) give
     bit for bit the z of f25_gnmfree_dev on the same device inputs; a batch
@@ -25,7 +18,7 @@ def test_gnmfree_noalloc_matches_gnmfree_dev():
     from fem25_amd import _lib
     from fem25_amd.api import MeshInfo
     L = _lib.load()
-    f = _noalloc(L)
+    f = L.f25_gnmfree_dev_noalloc
     m, mod, pml = mesh_case("c1")
     mi = MeshInfo(m, pml)
     mi.set_model(mod)
