@@ -48,6 +48,8 @@ def main() -> int:
     ap.add_argument("--config", default="c5p6")
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--skip-extras", action="store_true", help="only the timed FP64 product")
+    ap.add_argument("--only", choices=("grad", "grad32", "gn_diag", "cfmat", "qp", "met225"), default=None,
+                    help="time one secondary kernel family alone (for profiling); prints its own JSON line")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one plain device copy of the input (known bytes) for counter scaling")
     args = ap.parse_args()
@@ -88,6 +90,8 @@ def main() -> int:
         f"{npair} pairs, {time.time() - t0:.1f} s")
 
     g = torch.Generator(device="cuda").manual_seed(3)
+    if args.only:
+        return only_kernel(args, mi, mesh, L, torch, omega, ky, freq, py, g)
     x = torch.randn(ndof, npair, dtype=torch.complex128, device="cuda", generator=g)
     y = torch.empty_like(x)
 
@@ -371,6 +375,51 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     del ee, oo, ww, rr
     out["kernels"] = kern
     return out
+
+
+def only_kernel(args, mi, mesh, L, torch, omega, ky, freq, py, g):
+    """One kernel family alone, warm-up then --steps calls: its mean kernel
+    ms (HIP events inside the library) as one JSON line."""
+    ndof, npair = mi.ndof, omega.size
+    fam = args.only
+    if fam in ("grad", "grad32", "gn_diag"):
+        u = torch.randn(npair, ndof, dtype=torch.complex128, device="cuda", generator=g)
+        lam = torch.randn(npair, ndof, dtype=torch.complex128, device="cuda", generator=g)
+        gg = torch.zeros(mesh.nnpg, 3, dtype=torch.float64, device="cuda")
+        if fam == "grad":
+            fn, lib_fam = (lambda: mi.gradient_dev(omega, ky, u, lam, gg)), b"grad"
+        elif fam == "grad32":
+            u32, l32 = u.to(torch.complex64), lam.to(torch.complex64)
+            del u, lam
+            fn, lib_fam = (lambda: mi.gradient32_dev(omega, ky, u32, l32, gg)), b"grad"
+        else:
+            uh = u[:8].cpu().numpy()
+            del u, lam
+            fn, lib_fam = (lambda: mi.gn_diag(omega[:8], ky[:8], uh)), b"gn_diag"
+    else:
+        inv = mi.inv_setup(b"PP")
+        rng = np.random.default_rng(7)
+        cv = lambda: (rng.standard_normal(ndof) + 1j * rng.standard_normal(ndof)).astype(np.complex64)  # noqa: E731
+        wave, lamv, nuv, alv = cv(), cv(), cv(), cv()
+        if fam == "cfmat":
+            fn, lib_fam = (lambda: cfmat_host(mi, inv, float(freq[0]), float(py[0]), wave)), b"cfmat"
+        elif fam == "qp":
+            vq = rng.standard_normal(inv.na35).astype(np.float32)
+            fn, lib_fam = (lambda: mi.qp_nu_alpha(float(freq[0]), float(py[0]), vq, lamv, wave)), b"qp"
+        else:
+            fn, lib_fam = (lambda: mi.met225(float(freq[0]), float(py[0]), lamv, nuv, alv, wave)), b"met225"
+    for _ in range(max(args.warmup, 1)):
+        fn()
+    torch.cuda.synchronize()
+    ks = []
+    for _ in range(args.steps):
+        fn()
+        ks.append(L.f25_last_kernel_ms(lib_fam))
+    torch.cuda.synchronize()
+    print(json.dumps({"only": fam, "config": args.config, "kernel_ms": float(np.mean(ks)), "calls": args.steps}),
+          flush=True)
+    mi.close()
+    return 0
 
 
 def cfmat_host(mi, inv, freq, py, wave):

@@ -115,18 +115,20 @@ struct f25_mesh_s {
         const void *x = nullptr, *vcf = nullptr, *om = nullptr, *kv = nullptr, *pad = nullptr;
         void *y = nullptr;
         int npair = 0;
-        // the PML launches captured with it
-        const void *s01 = nullptr, *s2 = nullptr, *grp = nullptr, *gpair = nullptr;
+        // the per-omega launches captured with it
+        const void *mats = nullptr, *grp = nullptr, *gpair = nullptr;
         int nom = 0, ngrp = 0;
+        bool disp = false;
     } ag;
     // PML elements in the batched product: per distinct omega of the batch
     // their element matrices by power of ky (k_pml_split), applied after the
     // element-block product by k_apply_pml over groups of pairs of one omega
     struct {
         std::vector<double> om;            // distinct omegas the matrices hold
-        bool valid = false;                // cleared when the model changes
-        double2 *d_s01 = nullptr, *d_s2 = nullptr;
-        size_t s01_cap = 0;
+        bool valid = false;                // cleared when the model or the medium changes
+        bool disp = false;                 // the matrices cover every element (dispersive medium)
+        double2 *d_mats = nullptr;         // [slot][omega][3][NEE][NEE]
+        size_t mats_cap = 0;
         double *d_om = nullptr;
         int om_cap = 0;
         std::vector<int> grp, gpair;       // (omega index, first, count) per group; pairs by omega
@@ -437,7 +439,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc);
     dfree(m->d_ew);
     dfree(m->d_omky);
-    dfree(m->pa.d_s01); dfree(m->pa.d_s2); dfree(m->pa.d_om); dfree(m->pa.d_grp); dfree(m->pa.d_gpair);
+    dfree(m->pa.d_mats); dfree(m->pa.d_om); dfree(m->pa.d_grp); dfree(m->pa.d_gpair);
     if (m->blas) (void)rocblas_destroy_handle(m->blas);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
     if (m->ev1) (void)hipEventDestroy(m->ev1);
@@ -562,6 +564,7 @@ extern "C" int f25_set_attenuation(f25_mesh *m, const double *qp, const double *
     if (!m) FAIL(F25_ERR_ARG, "null argument");
     HIPCHK(hipSetDevice(m->dev));
     if (!qp || !qs) {
+        if (m->disp) m->pa.valid = false;
         m->disp = false;
         return 0;
     }
@@ -580,6 +583,7 @@ extern "C" int f25_set_attenuation(f25_mesh *m, const double *qp, const double *
     HIPCHK(hipMemcpyAsync(m->d_qs, m->qs.data(), n * 8, hipMemcpyHostToDevice, m->st));
     HIPCHK(hipStreamSynchronize(m->st));
     m->disp = true;
+    m->pa.valid = false;
     return 0;
 }
 
@@ -944,15 +948,22 @@ static int apply_f64(f25_mesh *m, const double *om, const double *kv, int npair,
     return launch_f64(m, om, kv, npair, x, y, vcf);
 }
 
-// ---- PML elements of the batched product -------------------------------
+// ---- elements of the batched product integrated per omega ---------------
+// The PML elements, or every element in a dispersive medium (its moduli
+// depend on the frequency through VCMPLX): their colour order and colour
+// offsets.
+static const std::vector<int> &omega_cptr(const f25_mesh *m) { return m->disp ? m->cptr : m->pml_cptr; }
+static const int *omega_order(const f25_mesh *m) { return m->disp ? m->d_order : m->d_pml_order; }
+static size_t omega_nelem(const f25_mesh *m) { return m->disp ? m->order.size() : m->pml_order.size(); }
+
 // Distinct omegas of the batch (first-appearance order), the pairs of each
 // in groups of at most 16, and the element matrices for those omegas
-// (rebuilt when the omegas or the model change).  No-op without PML
-// elements.  h_om: the batch's omegas on the host.
+// (rebuilt when the omegas, the model or the medium change).  No-op without
+// such elements.  h_om: the batch's omegas on the host.
 static int pml_prepare(f25_mesh *m, int npair, const double *h_om) {
     auto &P = m->pa;
     P.ngrp = 0;
-    if (m->pml_order.empty()) return 0;
+    if (!omega_nelem(m)) return 0;
     std::vector<double> uo;
     std::vector<int> fi(npair);
     for (int b = 0; b < npair; ++b) {
@@ -988,16 +999,15 @@ static int pml_prepare(f25_mesh *m, int npair, const double *h_om) {
     }
     HIPCHK(hipMemcpyAsync(P.d_grp, P.grp.data(), P.grp.size() * sizeof(int), hipMemcpyHostToDevice, m->st));
     HIPCHK(hipMemcpyAsync(P.d_gpair, P.gpair.data(), P.gpair.size() * sizeof(int), hipMemcpyHostToDevice, m->st));
-    if (P.valid && uo == P.om) return 0;
-    const int nen = m->tab.nen, nee = 3 * nen, npml = (int)m->pml_order.size(), nom = (int)uo.size();
-    const size_t n01 = (size_t)npml * nom * 2 * nee * nee;
-    if (n01 > P.s01_cap) {
-        dfree(P.d_s01);
-        P.s01_cap = 0;
-        TRY(dalloc(&P.d_s01, n01));
-        P.s01_cap = n01;
+    if (P.valid && P.disp == m->disp && uo == P.om) return 0;
+    const int nen = m->tab.nen, nee = 3 * nen, nom = (int)uo.size();
+    const size_t nm = omega_nelem(m) * (size_t)nom * 3 * nee * nee;
+    if (nm > P.mats_cap) {
+        dfree(P.d_mats);
+        P.mats_cap = 0;
+        TRY(dalloc(&P.d_mats, nm));
+        P.mats_cap = nm;
     }
-    if (!P.d_s2) TRY(dalloc(&P.d_s2, (size_t)npml * nee * nee));
     if (nom > P.om_cap) {
         dfree(P.d_om);
         P.om_cap = 0;
@@ -1009,38 +1019,50 @@ static int pml_prepare(f25_mesh *m, int npair, const double *h_om) {
     p.xwidth = m->d.xwidth;
     p.zwidth = m->d.zwidth;
     p.rcoeff = m->d.pml[0]; p.dmax = m->d.pml[1]; p.kappa = m->d.pml[2]; p.fcent = m->d.pml[3];
+    p.freq0 = m->freq0;
     const size_t lds = (8 * 9 + 3 * 8 * nen) * sizeof(double);
-    for (size_t c = 0; c + 1 < m->pml_cptr.size(); ++c) {
-        const int n = m->pml_cptr[c + 1] - m->pml_cptr[c];
+    const std::vector<int> &cp = omega_cptr(m);
+    for (size_t c = 0; c + 1 < cp.size(); ++c) {
+        const int n = cp[c + 1] - cp[c];
         if (!n) continue;
-        hipLaunchKernelGGL(k_pml_split, dim3(n, nom), dim3(256), lds, m->st, nen, m->tab.nq1, m->pml_cptr[c], nom,
-                           m->d_pml_order + m->pml_cptr[c], P.d_om, m->d_shl, m->d_wq, m->d_xq, m->d_ieng,
-                           m->d_xl, m->d_zl, m->d_xd, m->d_zd, m->d_rho, m->d_lam, m->d_mu, p, P.d_s01, P.d_s2,
-                           m->d_err);
+        hipLaunchKernelGGL(k_pml_split, dim3(n, nom), dim3(256), lds, m->st, nen, m->tab.nq1, cp[c], nom,
+                           omega_order(m) + cp[c], P.d_om, m->d_shl, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl,
+                           m->d_xd, m->d_zd, m->d_cdom, m->d_rho, m->d_lam, m->d_mu, m->disp ? m->d_qp : nullptr,
+                           m->disp ? m->d_qs : nullptr, p, P.d_mats, m->d_err);
     }
     HIPCHK(hipGetLastError());
     TRY(check_err(m));
     P.om = uo;
+    P.disp = m->disp;
     P.valid = true;
     return 0;
 }
 
-// y += S_e x over the PML elements, colour by colour (after the product)
+// y += S_e x over the elements held per omega, colour by colour (after the
+// product, or after a zero fill of y in a dispersive medium)
 template <typename XT>
 static int pml_launch(f25_mesh *m, int npair, const double *d_ky, const XT *x, XT *y) {
     auto &P = m->pa;
     if (!P.ngrp) return 0;
     const int nen = m->tab.nen, nom = (int)P.om.size();
     const size_t lds = (size_t)3 * nen * 16 * sizeof(double2);
-    for (size_t c = 0; c + 1 < m->pml_cptr.size(); ++c) {
-        const int n = m->pml_cptr[c + 1] - m->pml_cptr[c];
+    const std::vector<int> &cp = omega_cptr(m);
+    for (size_t c = 0; c + 1 < cp.size(); ++c) {
+        const int n = cp[c + 1] - cp[c];
         if (!n) continue;
-        hipLaunchKernelGGL(k_apply_pml<XT>, dim3(n, P.ngrp), dim3(256), lds, m->st, nen, m->pml_cptr[c], nom,
-                           m->d_pml_order + m->pml_cptr[c], m->d_lm, P.d_grp, P.d_gpair, npair, d_ky, P.d_s01,
-                           P.d_s2, x, y);
+        hipLaunchKernelGGL(k_apply_pml<XT>, dim3(n, P.ngrp), dim3(256), lds, m->st, nen, cp[c], nom,
+                           omega_order(m) + cp[c], m->d_lm, P.d_grp, P.d_gpair, npair, d_ky, P.d_mats, x, y);
     }
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+// The whole operator product in a dispersive medium: y = 0, then every
+// element's per-omega matrix
+template <typename XT>
+static int disp_product(f25_mesh *m, int npair, const double *d_ky, const XT *x, XT *y) {
+    fill_async(y, (size_t)m->g.ndof * npair, XT{0, 0}, m->st);
+    return pml_launch(m, npair, d_ky, x, y);
 }
 
 // F25_APPLY_GRAPH: capture the launches once per argument set, then replay
@@ -1051,8 +1073,8 @@ static int apply_f64_graph(f25_mesh *m, const double *om, const double *kv, int 
     const auto &P = m->pa;
     const int nom = (int)P.om.size();
     if (!G.exec || G.x != x || G.y != y || G.npair != npair || G.vcf != vcf || G.om != om || G.kv != kv ||
-        G.pad != m->d_vpad || G.s01 != P.d_s01 || G.s2 != P.d_s2 || G.grp != P.d_grp || G.gpair != P.d_gpair ||
-        G.nom != nom || G.ngrp != P.ngrp) {
+        G.pad != m->d_vpad || G.mats != P.d_mats || G.grp != P.d_grp || G.gpair != P.d_gpair || G.nom != nom ||
+        G.ngrp != P.ngrp || G.disp != m->disp) {
         if (G.exec) {
             HIPCHK(hipStreamSynchronize(m->st));
             (void)hipGraphExecDestroy(G.exec);
@@ -1060,8 +1082,8 @@ static int apply_f64_graph(f25_mesh *m, const double *om, const double *kv, int 
         }
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
-        int r = launch_f64(m, om, kv, npair, x, y, vcf);
-        if (!r) r = pml_launch(m, npair, kv, x, y);
+        int r = m->disp ? disp_product(m, npair, kv, x, y) : launch_f64(m, om, kv, npair, x, y, vcf);
+        if (!r && !m->disp) r = pml_launch(m, npair, kv, x, y);
         hipError_t e = hipStreamEndCapture(m->st, &g);
         if (r) {
             if (g) (void)hipGraphDestroy(g);
@@ -1075,7 +1097,7 @@ static int apply_f64_graph(f25_mesh *m, const double *om, const double *kv, int 
             FAIL(F25_ERR_HIP, hipGetErrorString(e));
         }
         G.x = x; G.y = y; G.npair = npair; G.vcf = vcf; G.om = om; G.kv = kv; G.pad = m->d_vpad;
-        G.s01 = P.d_s01; G.s2 = P.d_s2; G.grp = P.d_grp; G.gpair = P.d_gpair; G.nom = nom; G.ngrp = P.ngrp;
+        G.mats = P.d_mats; G.grp = P.d_grp; G.gpair = P.d_gpair; G.nom = nom; G.ngrp = P.ngrp; G.disp = m->disp;
     }
     HIPCHK(hipGraphLaunch(G.exec, m->st));
     return 0;
@@ -1094,8 +1116,13 @@ static int apply_core(f25_mesh *m, int npair, const double *d_om, const double *
     if (graph && mode != F25_APPLY_F64) FAIL(F25_ERR_ARG, "graph replay is for the FP64 product");
     if (h_om) TRY(pml_prepare(m, npair, h_om));
     else m->pa.ngrp = 0;
+    if (m->disp && !h_om) FAIL(F25_ERR_STATE, "products on element blocks need an elastic medium");
     HIPCHK(hipEventRecord(m->ev0, m->st));
-    if (mode == F25_APPLY_F64 && graph)
+    if (m->disp && !graph) {
+        // every element per omega: no element blocks
+        if (mode == F25_APPLY_F32) TRY(disp_product(m, npair, d_ky, (const float2 *)x, (float2 *)y));
+        else TRY(disp_product(m, npair, d_ky, (const double2 *)x, (double2 *)y));
+    } else if (mode == F25_APPLY_F64 && graph)
         TRY(apply_f64_graph(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y, vcf));
     else if (mode == F25_APPLY_F64) {
         TRY(apply_f64(m, d_om, d_ky, npair, (const double2 *)x, (double2 *)y, vcf));
@@ -1140,7 +1167,6 @@ static int stage_omky(f25_mesh *m, int n, const double *omega, const double *ky)
 
 extern "C" int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
                                    int mode, const void *x, void *y) {
-    if (m) TRY(require_elastic(m));
     if (!m || npair <= 0 || !omega || !ky || !x || !y) FAIL(F25_ERR_ARG, "invalid apply arguments");
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
     HIPCHK(hipSetDevice(m->dev));
@@ -1150,7 +1176,6 @@ extern "C" int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, 
 
 extern "C" int f25_residual_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
                                       const void *u, const void *f, void *r, double *rnorm) {
-    if (m) TRY(require_elastic(m));
     if (!m || npair <= 0 || !omega || !ky || !u || !f || !r || !rnorm || u == r || f == r)
         FAIL(F25_ERR_ARG, "invalid residual arguments");
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
@@ -1189,7 +1214,6 @@ static int transpose_dev(hipStream_t s, const T *in, T *out, int rows, int cols)
 
 extern "C" int f25_apply_batch(f25_mesh *m, int npair, const double *omega, const double *ky, int mode,
                                const void *x, void *y) {
-    if (m) TRY(require_elastic(m));
     if (!m || npair <= 0 || !x || !y) FAIL(F25_ERR_ARG, "invalid apply arguments");
     if (mode == F25_APPLY_F32) FAIL(F25_ERR_ARG, "host wrapper takes double complex data; use F64 or F32_REFINED");
     HIPCHK(hipSetDevice(m->dev));

@@ -1276,22 +1276,26 @@ __global__ void k_pml(int nen, int nq1, int ncol_elem, const int *__restrict__ o
     }
 }
 
-// PML elements of the batched product: per element (slot s of the PML
-// colour order) and distinct omega f of the batch, the element matrices by
-// power of ky (CM25ISOQ with CJACCQ's stretch, estiff25.f:168-334):
-//   S(omega_f, ky) = S0[s][f] + ky S1[s][f] + ky^2 S2[s]
+// Elements of the batched product integrated per omega: the PML elements,
+// and with quality factors (qp != null) every element in the dispersive
+// form (KMDAMP / CMDAMP with VCMPLX moduli, as k_pml).  Per element (slot s
+// of the launch's colour order) and distinct omega f of the batch, the
+// element matrix by power of ky (CM25ISOQ with CJACCQ's stretch,
+// estiff25.f:168-334):
+//   S(omega_f, ky) = S0[s][f] + ky S1[s][f] + ky^2 S2[s][f]
 // with rows and columns r = 3 a + i (node a, component i), NEE = 3 NEN.
-// s01: [slot][nom][2][NEE][NEE] complex (S0 holds the -omega^2 mass);
-// s2: [slot][NEE][NEE] complex (omega-independent, written with f = 0).
+// mats: [slot][nom][3][NEE][NEE] complex (S0 holds the -omega^2 mass).
 // One workgroup per (element of one colour, omega).
 __global__ void k_pml_split(int nen, int nq1, int slot0, int nom, const int *__restrict__ order,
                             const double *__restrict__ omf, const double *__restrict__ shl,
                             const double *__restrict__ wq, const double *__restrict__ xq,
                             const int *__restrict__ ieng, const double *__restrict__ xl,
                             const double *__restrict__ zl, const double *__restrict__ xd,
-                            const double *__restrict__ zd, const double *__restrict__ rho,
-                            const double *__restrict__ lam, const double *__restrict__ mu, PmlPar par,
-                            double2 *__restrict__ s01, double2 *__restrict__ s2out, int *__restrict__ err) {
+                            const double *__restrict__ zd, const char *__restrict__ cdom,
+                            const double *__restrict__ rho, const double *__restrict__ lam,
+                            const double *__restrict__ mu, const double *__restrict__ qp,
+                            const double *__restrict__ qs, PmlPar par, double2 *__restrict__ mats,
+                            int *__restrict__ err) {
     extern __shared__ double sm[];
     const int e = order[blockIdx.x], slot = slot0 + blockIdx.x, f = blockIdx.y;
     const int nq = nq1 * nq1, nee = 3 * nen;
@@ -1301,11 +1305,13 @@ __global__ void k_pml_split(int nen, int nq1, int slot0, int nom, const int *__r
     double *sX = sN + QC * nen;
     double *sZ = sX + QC * nen;
     par.omega = omf[f];
+    par.freq = omf[f] / (2.0 * M_PI);
+    const bool pml = cdom[e] == 'A';
     const double w2 = par.omega * par.omega;
     const int np = nen * nen;
-    double2 *S0 = s01 + ((size_t)slot * nom + f) * 2 * nee * nee;
+    double2 *S0 = mats + ((size_t)slot * nom + f) * 3 * nee * nee;
     double2 *S1 = S0 + (size_t)nee * nee;
-    double2 *S2 = s2out + (size_t)slot * nee * nee;
+    double2 *S2 = S1 + (size_t)nee * nee;
     for (int p0 = 0; p0 < np; p0 += blockDim.x) {
         const int pr = p0 + threadIdx.x;
         const int a = pr / nen, b = pr % nen;
@@ -1319,8 +1325,8 @@ __global__ void k_pml_split(int nen, int nq1, int slot0, int nom, const int *__r
         for (int i = 0; i < 3; ++i) s2[i] = make_double2(0.0, 0.0);
         for (int q0 = 0; q0 < nq; q0 += QC) {
             __syncthreads();
-            pml_stage(nen, nq1, q0, QC, e, shl, wq, xq, ieng, xl, zl, xd, zd, true, rho, lam, mu, nullptr,
-                      nullptr, par, sq, sN, sX, sZ, err);
+            pml_stage(nen, nq1, q0, QC, e, shl, wq, xq, ieng, xl, zl, xd, zd, pml, rho, lam, mu, qp, qs, par,
+                      sq, sN, sX, sZ, err);
             __syncthreads();
             if (pr < np)
                 for (int qq = 0; qq < QC; ++qq)
@@ -1339,25 +1345,26 @@ __global__ void k_pml_split(int nen, int nq1, int slot0, int nom, const int *__r
                     const int k = ci * 3 + cj;
                     S0[o] = s0[k];
                     S1[o] = (k == 1) ? s1[0] : (k == 3) ? s1[1] : (k == 5) ? s1[2] : (k == 7) ? s1[3] : z;
-                    if (f == 0) S2[o] = (ci == cj) ? s2[ci] : z;
+                    S2[o] = (ci == cj) ? s2[ci] : z;
                 }
         }
     }
 }
 
-// y_b += S_e(omega_b, ky_b) x_b over the PML elements of one colour, after
-// the element-block product has written y (it stores zero rows for them).
-// Workgroup (element, group of up to 16 pairs of one omega): lane t % 16 is
-// the pair, t / 16 picks rows r = t/16, t/16 + 16, ...; the element's x
-// values of the group are staged in LDS.  grp[3 g + 0..2] = (omega index,
-// first slot in gpair, count).  Arithmetic in FP64 for every vector type.
+// y_b += S_e(omega_b, ky_b) x_b over the elements of one colour whose
+// matrices k_pml_split holds, after the element-block product has written y
+// (it stores zero rows for PML elements; for a dispersive medium y is
+// zero-filled first).  Workgroup (element, group of up to 16 pairs of one
+// omega): lane t % 16 is the pair, t / 16 picks rows r = t/16, t/16 + 16,
+// ...; the element's x values of the group are staged in LDS.
+// grp[3 g + 0..2] = (omega index, first slot in gpair, count).  Arithmetic
+// in FP64 for every vector type.
 template <typename XT>
 __global__ __launch_bounds__(256) void k_apply_pml(int nen, int slot0, int nom, const int *__restrict__ order,
                                                    const int *__restrict__ lm, const int *__restrict__ grp,
                                                    const int *__restrict__ gpair, int npair,
                                                    const double *__restrict__ ky,
-                                                   const double2 *__restrict__ s01,
-                                                   const double2 *__restrict__ s2, const XT *__restrict__ x,
+                                                   const double2 *__restrict__ mats, const XT *__restrict__ x,
                                                    XT *__restrict__ y) {
     extern __shared__ double2 xs[];   // [NEE][16]
     __shared__ int sl[3 * 7 * 7];
@@ -1380,9 +1387,9 @@ __global__ __launch_bounds__(256) void k_apply_pml(int nen, int slot0, int nom, 
     }
     __syncthreads();
     const double k = ky[pair], k2 = k * k;
-    const double2 *S0 = s01 + ((size_t)slot * nom + f) * 2 * nee * nee;
+    const double2 *S0 = mats + ((size_t)slot * nom + f) * 3 * nee * nee;
     const double2 *S1 = S0 + (size_t)nee * nee;
-    const double2 *S2 = s2 + (size_t)slot * nee * nee;
+    const double2 *S2 = S1 + (size_t)nee * nee;
     for (int r = rl; r < nee; r += 16) {
         const int d = sl[r];
         const double2 *a0 = S0 + (size_t)r * nee, *a1 = S1 + (size_t)r * nee, *a2 = S2 + (size_t)r * nee;

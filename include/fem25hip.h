@@ -107,9 +107,10 @@ int f25_set_model(f25_mesh *m, const double *dens, const double *ecoeff);
  * frequency FREQ0 (Hz): the medium becomes dispersive (MESH_INFO's LDISP,
  * asmble.f90:159-168, 187-201) and the assembled operator uses KMDAMP /
  * CMDAMP (estiff25.f:511-961) with VCMPLX velocities (haskattn.f:602-641).
- * qp = NULL returns to the elastic operator.  While set, the entry points
- * built on frequency-independent element blocks (batched product,
- * gradient, Gauss-Newton products) return F25_ERR_STATE. */
+ * qp = NULL returns to the elastic operator.  While set, the batched
+ * product integrates every element per omega, and the entry points built
+ * on frequency-independent element blocks (gradient, Gauss-Newton
+ * products) return F25_ERR_STATE. */
 int f25_set_attenuation(f25_mesh *m, const double *qp, const double *qs, double freq0);
 
 /* ---- assembled operator (replaces ASMBLE_DRIVER, asmble.f90:1-231) ------ */
@@ -136,7 +137,9 @@ int f25_asmb_peff(f25_mesh *m, const void *cdist, const void *ue, void *peff);
  * other than 'S').  Interior elements run as frequency-independent element
  * blocks (KM25ISOQ, estiff25.f:1-164); PML elements (CM25ISOQ,
  * estiff25.f:168-334) as per-omega element matrices in FP64, rebuilt when the
- * batch's set of omegas changes.  x and y are device arrays of npair*NDOF
+ * batch's set of omegas changes; in a dispersive medium (f25_set_attenuation)
+ * every element runs as per-omega matrices (KMDAMP / CMDAMP,
+ * estiff25.f:511-961).  x and y are device arrays of npair*NDOF
  * complex values laid out [dof][pair] (pair fastest), double complex for
  * F25_APPLY_F64 and F25_APPLY_F32_REFINED, float complex for F25_APPLY_F32. */
 int f25_apply_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,

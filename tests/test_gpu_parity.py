@@ -710,7 +710,10 @@ def test_pml_anchors_fixed():
 def test_asmble_dispersive(name):
     """Dispersive medium (KMDAMP/CMDAMP, VCMPLX): FP64 operator against the
     oracle at 1e-12, COMPLEX*8 CDIST against the compiled reference's
-    fixture at FP32 level; the block-based product refuses the medium."""
+    fixture at FP32 level; the batched product (every element as per-omega
+    matrices) against the oracle's dispersive operator at 1e-12, eager and
+    replayed from a graph; the gradient (element blocks) refuses the
+    medium."""
     import os
     from fem25_amd import _lib
     from fem25_amd.api import MeshInfo
@@ -729,10 +732,25 @@ def test_asmble_dispersive(name):
         rowmax = np.maximum.reduceat(np.abs(ref.astype(np.complex128)), g.irptr[:-1] - 1)
         rows = np.repeat(np.arange(g.ndof), np.diff(g.irptr))
         assert np.all(np.abs(c.astype(np.complex128) - ref) <= 1e-6 * rowmax[rows])
-    om, ky, _, _ = pairs(1, 1)
+    om, ky, fq, py = pairs(2, 3)
+    x = cf.seeded_complex(g.ndof, batch=om.size, seed=1)
+    y = mi.apply_batch(om, ky, x)
+    for b in range(om.size):
+        S = csr(g, oracle.asmble_z(m, md, g, fq[b], py[b], pml))
+        assert relerr(y[b], S @ x[b]) < FP64_TOL, b
+    import torch
+    xd = torch.from_numpy(np.ascontiguousarray(x.T)).cuda()
+    yg, ye = torch.zeros_like(xd), torch.zeros_like(xd)
+    for rep in range(2):
+        mi.apply_batch_dev(om * (1 + 0.05 * rep), ky, xd, yg, _lib.F25_APPLY_F64 | _lib.F25_APPLY_GRAPH)
+        mi.apply_batch_dev(om * (1 + 0.05 * rep), ky, xd, ye, _lib.F25_APPLY_F64)
+        torch.cuda.synchronize()
+        assert torch.equal(yg, ye), rep
     with pytest.raises(_lib.Fem25Error):
-        mi.apply_batch(om, ky, cf.seeded_complex(g.ndof, batch=1, seed=1))
+        mi.gradient(om[:1], ky[:1], x[:1], x[:1])
     mi.set_model(mod)        # back to the elastic operator
+    y2 = mi.apply_batch(om[:1], ky[:1], x[:1])
+    assert relerr(y2[0], oracle_S(name, fq[0], py[0]) @ x[0]) < FP64_TOL
     s = mi.asmble(b"B", 0.05, 0.7e-4, complex128=True)
     assert relerr(s, oracle.asmble_z(m, mod, g, 0.05, 0.7e-4, pml)) < FP64_TOL
 
