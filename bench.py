@@ -217,6 +217,13 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     plain = ndof * nplain / (time.perf_counter() - tp) / 1e9
     out["plain_fp64_csr_gdof_per_s"] = round(plain, 3)
     out["speedup_vs_plain_fp64"] = round(gdofs / plain, 2)
+    # the same FP64 product replayed from a captured HIP graph (F25_APPLY_GRAPH)
+    yg = torch.empty_like(y)
+    k_g, wall_g = timed(lambda: mi.apply_batch_dev(omega, ky, x, yg, _lib.F25_APPLY_F64 | _lib.F25_APPLY_GRAPH),
+                        L, b"apply", 5)
+    out["fp64_graph_replay"] = {"ms_per_step": round(wall_g, 3), "gdof_per_s": round(ndof * npair / wall_g * 1e-6, 3),
+                                "bit_equal_to_eager": bool(torch.equal(yg, y))}
+    del yg
     # FP32 and FP32 refined products
     x32 = x.to(torch.complex64)
     y32 = torch.empty_like(x32)

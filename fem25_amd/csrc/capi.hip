@@ -1260,8 +1260,10 @@ static int grad_launch(f25_mesh *m, int ntrip, const double *d_om, const double 
     }
     if (!ksf) FAIL(F25_ERR_ARG, "unsupported order");
     const int nl = m->tab.nl, q1 = m->tab.nq1;
+    // (the FP32 form adds the inverse Jacobian at the points, 4 values each)
     const size_t lsf = (size_t)TS * 2 * 3 * nen * sizeof(C2) + (size_t)TS * 2 * 3 * q1 * nl * 2 * sizeof(C2) +
-                       (size_t)q1 * q1 * 4 * sizeof(R) + (size_t)2 * nl * q1 * sizeof(R);
+                       (size_t)2 * nl * q1 * sizeof(R) +
+                       (std::is_same<R, float>::value ? (size_t)4 * q1 * q1 * sizeof(R) : 0);
     if (lsf > 64 * 1024)
         HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
     HIPCHK(hipEventRecord(m->ev0, m->st));
@@ -1682,8 +1684,7 @@ static int ensure_dab_part(f25_mesh *m) {
 // A, B, C
 static size_t dab_lds(const f25_mesh *m) {
     const int nee = 3 * m->tab.nen, nq = m->tab.nq, q1 = m->tab.nq1, n1 = m->tab.nl;
-    return (size_t)2 * nee * sizeof(double2) + 5 * (size_t)nq * sizeof(double) +
-           ((size_t)3 * q1 * n1 * 3 + 9 * (size_t)nq) * sizeof(double2);
+    return (size_t)2 * nee * sizeof(double2) + ((size_t)3 * q1 * n1 * 3 + 9 * (size_t)nq) * sizeof(double2);
 }
 // threads of a k_dab_sf workgroup: five waves at p6 (Q1 = 17)
 static int dab_threads(const f25_mesh *m) { return m->tab.nq1 > 16 ? 320 : 256; }

@@ -1510,8 +1510,16 @@ template <typename R> struct CxOf;
 template <> struct CxOf<double> { typedef double2 T; };
 template <> struct CxOf<float> { typedef float2 T; };
 
+// Launch bounds: FP64 four workgroups per CU up to p4 (three waves per
+// SIMD above), FP32 five waves per SIMD (its registers allow it).
+template <int NL, typename R>
+struct GradOcc {
+    static constexpr int NT = 2 * NL + 3 > 16 ? 320 : 256;
+    static constexpr int MINB = std::is_same<R, float>::value ? 20 * 64 / NT : (NL <= 5 ? 4 : 3);
+};
+
 template <int NL, int T, typename R = double>
-__global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) void k_grad_sf(
+__global__ __launch_bounds__((GradOcc<NL, R>::NT), (GradOcc<NL, R>::MINB)) void k_grad_sf(
     const double *__restrict__ t1d, const double *__restrict__ wq, const double *__restrict__ xq,
     const int *__restrict__ ieng, const double *__restrict__ xl, const double *__restrict__ zl,
     const char *__restrict__ cdom, const int *__restrict__ lm, int ntrip, int64_t ndof,
@@ -1530,8 +1538,8 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     extern __shared__ double sm[];
     C2 *sw = reinterpret_cast<C2 *>(sm);                               // [T][2][nee]
     C2 *sT = sw + NW;                                                  // [T][2][3][Q1][N1][2]
-    R *sG = reinterpret_cast<R *>(sT + T * 2 * 3 * Q1 * N1 * 2);       // [Q1*Q1][4]
-    R *s1 = sG + 4 * Q1 * Q1;                                          // [2][N1][Q1]
+    R *s1 = reinterpret_cast<R *>(sT + T * 2 * 3 * Q1 * N1 * 2);       // [2][N1][Q1]
+    R *sG = s1 + 2 * N1 * Q1;                                          // FP32: [Q1*Q1][4]
     __shared__ double red[NT / 64][12];
     const int e = blockIdx.x, tid = threadIdx.x;
     if (cdom[e] == 'A') {   // PML elements are outside the non-PML gradient path
@@ -1540,15 +1548,19 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     }
     const int *ie = ieng + 4 * e;
     const int *L = lm + (size_t)e * nee;
-    for (int q = tid; q < Q1 * Q1; q += NT) {
-        const int ix = q % Q1, iz = q / Q1;
-        Geo g;
-        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
-        R *G = sG + 4 * q;
-        G[0] = (R)(g.dzdeta / g.det); G[1] = (R)(-g.dzdxi / g.det); G[2] = (R)(-g.dxdeta / g.det);
-        G[3] = (R)(g.dxdxi / g.det);
-    }
     for (int i = tid; i < 2 * N1 * Q1; i += NT) s1[i] = (R)t1d[i];   // v_j(eta_q), then v'_j(eta_q)
+    if constexpr (!F64) {
+        // FP32: the inverse Jacobian at every point in LDS (registers are the
+        // FP32 form's occupancy limit; the FP64 form keeps its point's
+        // values in registers instead, its LDS being the limit)
+        for (int q = tid; q < Q1 * Q1; q += NT) {
+            Geo g;
+            if (geo_at(xq[q % Q1], xq[q / Q1], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
+            R *G = sG + 4 * q;
+            G[0] = (R)(g.dzdeta / g.det); G[1] = (R)(-g.dzdxi / g.det); G[2] = (R)(-g.dxdeta / g.det);
+            G[3] = (R)(g.dxdxi / g.det);
+        }
+    }
     // the gather slots of this thread are the same in every pass: slot r is
     // value tid + r NT of [T][2][nee]; its DOF (or -1) is fixed per element
     int sd[NG];
@@ -1580,28 +1592,69 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     R vx[N1], dx[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) { vx[i] = (R)t1d[i * Q1 + qx]; dx[i] = (R)t1d[(N1 + i) * Q1 + qx]; }
+    // FP64: the inverse Jacobian at this lane's point, kept in registers
+    R rx = R(0), sx = R(0), rz = R(0), sz = R(0);
+    if constexpr (F64) {
+        Geo g;
+        if (geo_at(xq[qx], xq[qe], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
+        rx = g.dzdeta / g.det; sx = -g.dzdxi / g.det; rz = -g.dxdeta / g.det; sz = g.dxdxi / g.det;
+    }
     double Dl = 0.0, Dm = 0.0, Dr = 0.0;
     for (int t0 = 0; t0 < ntrip; t0 += T) {
-        __syncthreads();   // the previous pass is done with sw, sT (and sG, s1 are written)
+        // no barrier before these writes: every wave passed the previous
+        // pass's second barrier after its last read of sw, and the pass's
+        // first writes of sT come after the barrier below, which no wave
+        // reaches before its previous second stage is done with sT
 #pragma unroll
         for (int r = 0; r < NG; ++r)
             if (tid + r * NT < NW) sw[tid + r * NT] = pre[r];
-        __syncthreads();
-        for (int it = tid; it < T * 2 * 3 * Q1 * N1; it += NT) {
-            const int i = it % N1, qy = (it / N1) % Q1, fc = it / (N1 * Q1);   // fc = (tt*2 + f)*3 + c
-            const int c = fc % 3;
-            const C2 *w = sw + (fc / 3) * nee;
-            C2 t1 = {R(0), R(0)}, t2 = t1;
+        __syncthreads();   // sw (and s1) written
+        // first pass, T1 and T2 for every (fc, eta point, node column i),
+        // fc = (tt*2 + f)*3 + c.  FP64: a thread per (fc, i, third of the
+        // eta points) keeps the column's N1 element values in registers and
+        // sweeps its eta points (each value read from LDS once per third);
+        // FP32: a thread per (fc, eta point, i), fewer registers (five waves
+        // per SIMD).
+        if constexpr (F64) {
+            constexpr int QS3 = (Q1 + 2) / 3;
+            for (int it = tid; it < T * 2 * 3 * N1 * 3; it += NT) {
+                const int third = it % 3, i = (it / 3) % N1, fc = it / (3 * N1);
+                const int c = fc % 3;
+                const C2 *w = sw + (fc / 3) * nee;
+                C2 a[N1];
 #pragma unroll
-            for (int j = 0; j < N1; ++j) {
-                const C2 a = w[(j * N1 + i) * 3 + c];
-                const R vv = s1[j * Q1 + qy], dd = s1[(N1 + j) * Q1 + qy];
-                t1.x += vv * a.x; t1.y += vv * a.y;
-                t2.x += dd * a.x; t2.y += dd * a.y;
+                for (int j = 0; j < N1; ++j) a[j] = w[(j * N1 + i) * 3 + c];
+                const int q0 = third * QS3, q1 = min(Q1, q0 + QS3);
+                for (int qy = q0; qy < q1; ++qy) {
+                    C2 t1 = {R(0), R(0)}, t2 = t1;
+#pragma unroll
+                    for (int j = 0; j < N1; ++j) {
+                        const R vv = s1[j * Q1 + qy], dd = s1[(N1 + j) * Q1 + qy];
+                        t1.x += vv * a[j].x; t1.y += vv * a[j].y;
+                        t2.x += dd * a[j].x; t2.y += dd * a[j].y;
+                    }
+                    C2 *o = sT + (size_t)((fc * Q1 + qy) * N1 + i) * 2;
+                    o[0] = t1;
+                    o[1] = t2;
+                }
             }
-            C2 *o = sT + (size_t)((fc * Q1 + qy) * N1 + i) * 2;
-            o[0] = t1;
-            o[1] = t2;
+        } else {
+            for (int it = tid; it < T * 2 * 3 * Q1 * N1; it += NT) {
+                const int i = it % N1, qy = (it / N1) % Q1, fc = it / (N1 * Q1);
+                const int c = fc % 3;
+                const C2 *w = sw + (fc / 3) * nee;
+                C2 t1 = {R(0), R(0)}, t2 = t1;
+#pragma unroll
+                for (int j = 0; j < N1; ++j) {
+                    const C2 a = w[(j * N1 + i) * 3 + c];
+                    const R vv = s1[j * Q1 + qy], dd = s1[(N1 + j) * Q1 + qy];
+                    t1.x += vv * a.x; t1.y += vv * a.y;
+                    t2.x += dd * a.x; t2.y += dd * a.y;
+                }
+                C2 *o = sT + (size_t)((fc * Q1 + qy) * N1 + i) * 2;
+                o[0] = t1;
+                o[1] = t2;
+            }
         }
         R kt[T], wt[T];
 #pragma unroll
@@ -1612,8 +1665,10 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
         }
         __syncthreads();
         if (t0 + T < ntrip) gather(t0 + T);
-        const R *G = sG + 4 * (qx + Q1 * qe);
-        const R rx = G[0], sx = G[1], rz = G[2], sz = G[3];
+        if constexpr (!F64) {
+            const R *G = sG + 4 * (qx + Q1 * qe);
+            rx = G[0]; sx = G[1]; rz = G[2]; sz = G[3];
+        }
 #pragma unroll
         for (int tt = 0; tt < T; ++tt) {
             if (t0 + tt >= ntrip) break;
@@ -2191,8 +2246,7 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
     extern __shared__ double sm[];
     __shared__ double red[2][8];
     double2 *su = reinterpret_cast<double2 *>(sm);          // [2][nee]
-    double *sG = reinterpret_cast<double *>(su + 2 * nee);  // [QQ][5]: rx, sx, rz, sz, D
-    double2 *sT = reinterpret_cast<double2 *>(sG + 5 * QQ); // [3][Q1][N1][2]: T1, T2
+    double2 *sT = su + 2 * nee;                             // [3][Q1][N1][2]: T1, T2
     // PA, PB, PC [3][Q1][N1][3] take T1/T2's place: T1/T2 are last read in
     // the point stage, PA.. first written in the projection after it
     double2 *sP = sT;
@@ -2216,19 +2270,21 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
             su[nee + i] = make_double2(b.x, b.y);
         }
     }
-    for (int q = tid; q < QQ; q += NT) {
-        const int ix = q % Q1, iz = q / Q1;
-        Geo g;
-        if (geo_at(xq[ix], xq[iz], ie, xl, zl, g)) *err = 2;
-        double *G = sG + 5 * q;
-        G[0] = g.dzdeta / g.det; G[1] = -g.dzdxi / g.det; G[2] = -g.dxdeta / g.det; G[3] = g.dxdxi / g.det;
-        G[4] = wq[ix] * wq[iz] * g.det * (c0 * g.sf[0] + c1 * g.sf[1] + c2 * g.sf[2] + c3 * g.sf[3]);
-    }
     // stage-2 lanes: row = eta point, lane = xi point (clamped when idle)
     const int row16 = tid >> 4, x16 = tid & 15;
     const bool ovl = OVF && tid >= 16 * Q1 && tid < 17 * Q1;   // a lane of the xi = 16 column
     const bool act = (row16 < Q1 && x16 < Q1) || ovl;
     const int qe = ovl ? tid - 16 * Q1 : min(row16, Q1 - 1), qx = ovl ? 16 : min(x16, Q1 - 1);
+    // the inverse Jacobian and the collocated density D at this lane's point,
+    // in registers (no LDS table: a smaller workgroup footprint keeps more
+    // workgroups per CU)
+    double rx, sx, rz, sz, D;
+    {
+        Geo g;
+        if (geo_at(xq[qx], xq[qe], ie, xl, zl, g)) *err = 2;
+        rx = g.dzdeta / g.det; sx = -g.dzdxi / g.det; rz = -g.dxdeta / g.det; sz = g.dxdxi / g.det;
+        D = wq[qx] * wq[qe] * g.det * (c0 * g.sf[0] + c1 * g.sf[1] + c2 * g.sf[2] + c3 * g.sf[3]);
+    }
     const int ni = min(x16, N1 - 1);
     double vx[N1], dx[N1];
 #pragma unroll
@@ -2239,7 +2295,7 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
     const int oa = tid / 3, occ = tid % 3, oj = oa / N1, oi = oa % N1;
     double2 val[2] = {};
     for (int v = 0; v < nvec; ++v) {
-        __syncthreads();   // su, sG written; the previous vector is done with sT, sH, sP
+        __syncthreads();   // su written; the previous vector is done with sT, sH, sP
         for (int k = tid; k < 3 * Q1 * N1; k += NT) {
             const int i = k % N1, qy = (k / N1) % Q1, c = k / (N1 * Q1);
             double2 t1 = make_double2(0.0, 0.0), t2 = t1;
@@ -2255,8 +2311,6 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
         }
         __syncthreads();
         {
-            const double *G = sG + 5 * (qx + Q1 * qe);
-            const double rx = G[0], sx = G[1], rz = G[2], sz = G[3], D = G[4];
             const double P = kp * D, S = ks * D, Q = P - 2.0 * S;
             double2 U[3], UX[3], UZ[3];
 #pragma unroll

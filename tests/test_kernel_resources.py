@@ -7,8 +7,10 @@ this test keeps those budgets fixed:
   * the batched product (k_apply_v), the pseudo-Hessian element columns
     (k_fcols_sf) and the element derivative products (k_dab_sf) use no
     scratch at any order or tiling;
-  * the gradient (k_grad_sf) spills at most a few registers outside its
-    triple loop (the addresses of the prefetch), at most 32 bytes per lane;
+  * the FP64 gradient (k_grad_sf) spills at most a few registers (the
+    addresses of the prefetch and the point's inverse Jacobian, which it
+    keeps in registers to save LDS), at most 64 bytes per lane, and the FP32
+    gradient none;
   * the product for p4 with one row tile (the benchmark's kernel) keeps
     three waves per SIMD and at most 168 registers, and the gradient and
     pseudo-Hessian kernels for p2..p5 keep four.
@@ -81,7 +83,7 @@ def test_no_scratch_in_product_and_element_kernels(usage):
 
 def test_gradient_spills_bounded(usage):
     for name, v in kernels(usage, "k_grad_sf").items():
-        assert v["scratch"] <= 32, (name, v)
+        assert v["scratch"] <= (0 if "ELi2EfE" in name else 64), (name, v)
 
 
 def test_occupancy_budgets(usage):
