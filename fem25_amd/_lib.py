@@ -27,6 +27,7 @@ class MeshDesc(C.Structure):
 F25_C64, F25_C128 = 0, 1
 F25_APPLY_F64, F25_APPLY_F32, F25_APPLY_F32_REFINED = 0, 1, 2
 F25_APPLY_GRAPH = 16   # or'ed with F25_APPLY_F64: replay a captured HIP graph
+F25_ERR_ARG, F25_ERR_HIP, F25_ERR_STATE = 1, 4, 8    # status codes of fem25hip.h
 
 EXPORTS = {
     "f25_mesh_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
@@ -85,6 +86,13 @@ EXPORTS = {
     "f25_cadjlam_rhs_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, dp, dp, C.c_double,
                                       C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, dp, ip, C.c_void_p,
                                       C.c_void_p, fp, C.c_void_p, ip]),
+    "f25_cadjlam_rhs_covd": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, dp, dp, C.c_double,
+                                       C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, dp, dp, ip, C.c_void_p,
+                                       C.c_void_p, fp, C.c_void_p, ip]),
+    "f25_cadjlam_rhs_covd_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, dp, dp, C.c_double,
+                                           C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, dp, dp, ip, C.c_void_p,
+                                           C.c_void_p, fp, C.c_void_p, ip]),
+    "f25_rec_space1d": (C.c_int, [C.c_int, dp, dp]),
     "f25_gn_fv": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp, C.c_void_p]),
     "f25_gn_fhw": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, dp]),
     "f25_gn_diag": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp]),

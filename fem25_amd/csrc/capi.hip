@@ -1540,23 +1540,39 @@ static int cadjlam_check(int ndof, int nfreq, int nrec, int nsrc, int npair, con
     return 0;
 }
 
+// the data covariance (LCOVD): covd set, with ldwght and the nrec - 1
+// normalised receiver spacings dx (checked by the caller)
+struct CovdArgs {
+    int covd, ldwght;
+    const double *dx;
+};
+
 static int cadjlam_core(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq, const int *isrc,
                         const double *freq, const double *py, double azmod, int norm, int irestp, float epss,
                         float dtmax, int ibphase, int lunwrap, const double *yrec, const int *mrdof,
                         const float2 *est_dev, const float2 *obs_dev, const float *w_dev, float2 *rhs_dev,
-                        int *iwarn, hipStream_t s) {
+                        int *iwarn, hipStream_t s, CovdArgs cv = CovdArgs{0, 0, nullptr}) {
     AdjParams prm{(float)azmod, epss, dtmax, norm, irestp, ibphase, lunwrap ? 1 : 0};
-    int *df = nullptr, *dsr = nullptr, *dm = nullptr, *dw = nullptr;
-    double *dfr = nullptr, *dpy = nullptr, *dy = nullptr;
-    float2 *dv = nullptr;
+    int *df = nullptr, *dsr = nullptr, *dm = nullptr, *dw = nullptr, *dinv = nullptr, *dpos = nullptr;
+    double *dfr = nullptr, *dpy = nullptr, *dy = nullptr, *ddx = nullptr;
+    float2 *dv = nullptr, *ddif = nullptr, *dres = nullptr;
+    const size_t n3p = 3 * (size_t)nrec * npair;
     int r = upload(&df, ifreq, (size_t)npair, s);
     if (!r) r = upload(&dsr, isrc, (size_t)npair, s);
     if (!r) r = upload(&dfr, freq, (size_t)npair, s);
     if (!r) r = upload(&dpy, py, (size_t)npair, s);
     if (!r) r = upload(&dy, yrec, (size_t)nrec, s);
     if (!r) r = upload(&dm, mrdof, 3 * (size_t)nrec, s);
-    if (!r) r = dalloc(&dv, 3 * (size_t)nrec * npair);
+    if (!r) r = dalloc(&dv, n3p);
     if (!r) r = dalloc(&dw, 1);
+    if (!r && cv.covd) {
+        if (cv.ldwght && nrec > 1) r = upload(&ddx, cv.dx, (size_t)nrec - 1, s);
+        else r = dalloc(&ddx, 1);
+        if (!r) r = dalloc(&dinv, n3p);
+        if (!r) r = dalloc(&dpos, n3p);
+        if (!r) r = dalloc(&ddif, n3p);
+        if (!r) r = dalloc(&dres, n3p);
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!r && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
         g_err = "hipEventCreate failed";
@@ -1566,19 +1582,25 @@ static int cadjlam_core(int ndof, int nfreq, int nrec, int nsrc, int npair, cons
         fill_async(dw, (size_t)1, 0, s);
         (void)hipEventRecord(e0, s);
         fill_async(rhs_dev, (size_t)ndof * npair, make_float2(0.f, 0.f), s);
-        hipLaunchKernelGGL(k_cadjlam, dim3(npair), dim3(256), 0, s, ndof, nfreq, nrec, df, dsr, dfr, dpy, prm, dy,
-                           dm, est_dev, obs_dev, w_dev, dv, rhs_dev, dw);
+        if (cv.covd)
+            hipLaunchKernelGGL(k_cadjlam_covd, dim3(npair), dim3(256), 0, s, ndof, nfreq, nrec, df, dsr, dfr, dpy,
+                               prm, cv.ldwght, ddx, dy, dm, est_dev, obs_dev, w_dev, dinv, dpos, ddif, dres, dv,
+                               rhs_dev, dw);
+        else
+            hipLaunchKernelGGL(k_cadjlam, dim3(npair), dim3(256), 0, s, ndof, nfreq, nrec, df, dsr, dfr, dpy, prm,
+                               dy, dm, est_dev, obs_dev, w_dev, dv, rhs_dev, dw);
         (void)hipEventRecord(e1, s);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(iwarn, dw, sizeof(int), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
         float ms = 0.f;
-        if (!r && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) record_ms("cadjlam", ms);
+        if (!r && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) record_ms(cv.covd ? "cadjlam_covd" : "cadjlam", ms);
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     dfree(df); dfree(dsr); dfree(dfr); dfree(dpy); dfree(dy); dfree(dm); dfree(dv); dfree(dw);
+    dfree(ddx); dfree(dinv); dfree(dpos); dfree(ddif); dfree(dres);
     return r;
 }
 
@@ -1620,6 +1642,82 @@ extern "C" int f25_cadjlam_rhs(int ndof, int nfreq, int nrec, int nsrc, int npai
     }
     dfree(de); dfree(dob); dfree(dw); dfree(dr);
     return r;
+}
+
+// LAPLACE_1DV2 divides by the summed spacings: every spacing must be a
+// positive finite number (REC_SPACE1D's are >= 1); the reference stops
+// with an error when a zero sum is reached
+static int covd_check(int nrec, int ldwght, const double *dx) {
+    if (!ldwght || nrec < 2) return 0;
+    if (!dx) FAIL(F25_ERR_ARG, "dx is null");
+    for (int i = 0; i < nrec - 1; ++i)
+        if (!(dx[i] > 0.0) || !std::isfinite(dx[i])) FAIL(F25_ERR_ARG, "receiver spacings must be positive and finite");
+    return 0;
+}
+
+extern "C" int f25_cadjlam_rhs_covd_dev(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq,
+                                        const int *isrc, const double *freq, const double *py, double azmod,
+                                        int irestp, float dtmax, int ibphase, int lunwrap, int ldwght,
+                                        const double *dx, const double *yrec, const int *mrdof, const void *est_dev,
+                                        const void *obs_dev, const float *wghts_dev, void *rhs_dev, int *iwarn) {
+    if (!est_dev || !obs_dev || !wghts_dev || !rhs_dev) FAIL(F25_ERR_ARG, "null argument");
+    TRY(cadjlam_check(ndof, nfreq, nrec, nsrc, npair, ifreq, isrc, freq, py, yrec, mrdof, iwarn));
+    TRY(covd_check(nrec, ldwght, dx));
+    TmpStream ts;
+    HIPCHK(hipStreamCreate(&ts.s));
+    return cadjlam_core(ndof, nfreq, nrec, nsrc, npair, ifreq, isrc, freq, py, azmod, 2, irestp, 0.f, dtmax,
+                        ibphase, lunwrap, yrec, mrdof, (const float2 *)est_dev, (const float2 *)obs_dev, wghts_dev,
+                        (float2 *)rhs_dev, iwarn, ts.s, CovdArgs{1, ldwght ? 1 : 0, dx});
+}
+
+extern "C" int f25_cadjlam_rhs_covd(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq,
+                                    const int *isrc, const double *freq, const double *py, double azmod, int irestp,
+                                    float dtmax, int ibphase, int lunwrap, int ldwght, const double *dx,
+                                    const double *yrec, const int *mrdof, const void *est, const void *obs,
+                                    const float *wghts, void *rhs, int *iwarn) {
+    if (!est || !obs || !wghts || !rhs) FAIL(F25_ERR_ARG, "null argument");
+    TRY(cadjlam_check(ndof, nfreq, nrec, nsrc, npair, ifreq, isrc, freq, py, yrec, mrdof, iwarn));
+    TRY(covd_check(nrec, ldwght, dx));
+    TmpStream ts;
+    HIPCHK(hipStreamCreate(&ts.s));
+    const size_t nd = 3 * (size_t)nfreq * nrec * nsrc;
+    float2 *de = nullptr, *dob = nullptr, *dr = nullptr;
+    float *dw = nullptr;
+    int r = upload(&de, (const float2 *)est, nd, ts.s);
+    if (!r) r = upload(&dob, (const float2 *)obs, nd, ts.s);
+    if (!r) r = upload(&dw, wghts, nd, ts.s);
+    if (!r) r = dalloc(&dr, (size_t)ndof * npair);
+    if (!r) r = cadjlam_core(ndof, nfreq, nrec, nsrc, npair, ifreq, isrc, freq, py, azmod, 2, irestp, 0.f, dtmax,
+                             ibphase, lunwrap, yrec, mrdof, de, dob, dw, dr, iwarn, ts.s,
+                             CovdArgs{1, ldwght ? 1 : 0, dx});
+    if (!r) {
+        hipError_t e = hipMemcpy(rhs, dr, (size_t)ndof * npair * sizeof(float2), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
+    }
+    dfree(de); dfree(dob); dfree(dw); dfree(dr);
+    return r;
+}
+
+// REC_SPACE1D (locrec2d.f:157-223): receiver spacings along the line,
+// divided by the smallest; host arithmetic
+extern "C" int f25_rec_space1d(int nrec, const double *xrec, double *dx) {
+    if (nrec < 1 || !xrec || (nrec > 1 && !dx)) FAIL(F25_ERR_ARG, "invalid arguments");
+    for (int i = 0; i < nrec - 1; ++i) dx[i] = 0.0;
+    if (nrec == 1) FAIL(F25_ERR_ARG, "only one receiver");
+    const double xdir = xrec[1] - xrec[0];
+    double dxmin = HUGE_VAL;
+    for (int i = 0; i < nrec - 1; ++i) {
+        dx[i] = std::fabs(xrec[i + 1] - xrec[i]);
+        dxmin = std::fmin(dxmin, dx[i]);
+        const bool dup = dx[i] < 1e-10;
+        const bool turn = i > 0 && ((xrec[i] > xrec[i - 1] && xdir < 0.0) || (xrec[i] < xrec[i - 1] && xdir > 0.0));
+        if (dup || turn) {
+            for (int j = 0; j < nrec - 1; ++j) dx[j] = 0.0;
+            FAIL(F25_ERR_ARG, dup ? "duplicate receiver locations" : "receiver x not in order");
+        }
+    }
+    for (int i = 0; i < nrec - 1; ++i) dx[i] = dx[i] / dxmin;
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

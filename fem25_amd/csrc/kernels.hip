@@ -2765,6 +2765,173 @@ __global__ __launch_bounds__(256) void k_cadjlam(int ndof, int nfreq, int nrec, 
     }
 }
 
+// BPRESID_COVD4's (objfn.f:635-733) phase gate: the phase difference in
+// seconds, formed in single precision
+__device__ inline bool covd_gate(float dtmax, double freq, float2 obs, float2 est) {
+    const float twopi = 6.283185307179586f, pi = 3.1415926535897931f;
+    if (!(dtmax > 0.f)) return false;
+    float dphi = atan2r(obs.y, obs.x) - atan2r(est.y, est.x);
+    if (dphi < -pi) dphi = dphi + twopi;
+    if (dphi > +pi) dphi = dphi - twopi;
+    dphi = dphi / (twopi * (float)freq);
+    return fabsf(dphi) > dtmax;
+}
+
+// LAPLACE_1DV2's (locrec2d.f:334-509) element length between the j-th and
+// (j+1)-th observed receivers of a component, whose receiver numbers are
+// inv[j] / 3 and inv[j + 1] / 3: the normalised spacings summed in receiver
+// order (1 without ldwght)
+__device__ inline double covd_h(int ldwght, const double *__restrict__ dx, const int *__restrict__ inv, int j) {
+    if (!ldwght) return 1.0;
+    double xs = 0.0;
+    for (int r = inv[j] / 3, r1 = inv[j + 1] / 3; r < r1; ++r) xs += dx[r];
+    return xs;
+}
+
+// CADJLAM's right-hand side with the data covariance (adjgrad.f90:219-235,
+// 296-300), one workgroup of 256 threads per (ifreq, isrc) pair:
+//  1. per receiver: rotation to (N,E,Z), phase shift, CRESID of every
+//     component whose shifted observation is non-zero, gated by the phase
+//     window (dif, [rec][3]);
+//  2. PERM_OBS (locrec2d.f:511-549) of the untransformed observations by a
+//     ballot scan per component (taken 3, 2, 1): observed receivers first,
+//     then the others; pos[3 irec + i] = position, inv its inverse;
+//  3. RESIDV = W C W WORK with WORK = dif at the positions and W the
+//     receiver weights there, C the roughening matrix of LAPLACE_1DV2: a
+//     chain of two-node elements over each component's observed receivers,
+//     numbered consecutively over the components in the same order (so
+//     that C's numbering and PERM_OBS's positions coincide only where the
+//     earlier components are observed everywhere, as in the reference).
+//     C is tridiagonal; each row forms the three products in SGEMV's
+//     column order, in single precision without contraction, with C's
+//     entries summed in double as the element loop adds them;
+//  4. RESID at each receiver's positions, rotated back, HDIFFER, conjugate,
+//     and one lane scatters into the zeroed RHS in receiver order.
+// Scratch per pair: inv, pos (3 nrec ints), dif, res, vals (3 nrec float2).
+__global__ __launch_bounds__(256) void k_cadjlam_covd(
+    int ndof, int nfreq, int nrec, const int *__restrict__ ifreq, const int *__restrict__ isrc,
+    const double *__restrict__ freq, const double *__restrict__ py, AdjParams prm, int ldwght,
+    const double *__restrict__ dx, const double *__restrict__ yrec, const int *__restrict__ mrdof,
+    const float2 *__restrict__ est, const float2 *__restrict__ obs, const float *__restrict__ wghts,
+    int *__restrict__ sinv, int *__restrict__ spos, float2 *__restrict__ sdif, float2 *__restrict__ sres,
+    float2 *__restrict__ vals, float2 *__restrict__ rhs, int *__restrict__ warn) {
+    __shared__ int s_na[3], s_wc[4];
+    const int p = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int n3 = 3 * nrec;
+    const double f = freq[p];
+    const size_t gb0 = 3 * (size_t)(ifreq[p] - 1), gstride = 3 * (size_t)nfreq;   // receiver ir at gb0 + ir*gstride
+    const size_t gsrc = (size_t)nrec * gstride * (isrc[p] - 1);
+    int *inv = sinv + (size_t)p * n3, *pos = spos + (size_t)p * n3;
+    float2 *dif = sdif + (size_t)p * n3, *res = sres + (size_t)p * n3, *v = vals + (size_t)p * n3;
+    if (tid < 3) s_na[tid] = 0;
+    __syncthreads();
+    // 1. residuals at the receivers; observed-receiver counts per component
+    int cnt[3] = {0, 0, 0};
+    for (int ir = tid; ir < nrec; ir += blockDim.x) {
+        const size_t base = gsrc + gb0 + ir * gstride;
+        float2 e[3], o[3], E[3], O[3];
+        for (int i = 0; i < 3; ++i) {
+            e[i] = est[base + i];
+            o[i] = obs[base + i];
+            cnt[i] += cabsf_(o[i]) > 0.f;
+        }
+        if (prm.lunwrap)
+            for (int i = 0; i < 3; ++i) { e[i] = cphm2cm(e[i].x, e[i].y); o[i] = cphm2cm(o[i].x, o[i].y); }
+        crotate(prm.az, e[0], e[1], E[0], E[1]);
+        E[2] = e[2];
+        const float2 cf = cexpi(-(float)(6.2831853071795862 * f * py[p] * yrec[ir]));
+        for (int i = 0; i < 3; ++i) {
+            O[i] = cmulf(o[i], cf);
+            E[i] = cmulf(E[i], cf);
+            float2 d = make_float2(0.f, 0.f);
+            if (cabsf_(O[i]) > 0.f && !covd_gate(prm.dtmax, f, O[i], E[i])) d = cresid(prm.irestp, O[i], E[i]);
+            dif[3 * ir + i] = d;
+        }
+    }
+    for (int i = 0; i < 3; ++i) atomicAdd(&s_na[i], cnt[i]);
+    __syncthreads();
+    // 2. positions: component c = 2, 1, 0 fills positions b nrec .. (b + 1) nrec - 1, b = 2 - c
+    for (int b = 0; b < 3; ++b) {
+        const int c = 2 - b, na = s_na[c];
+        int run = 0;
+        for (int r0 = 0; r0 < nrec; r0 += 256) {
+            const int ir = r0 + tid;
+            const bool act = ir < nrec && cabsf_(obs[gsrc + gb0 + ir * gstride + c]) > 0.f;
+            const unsigned long long m = __ballot(act);
+            if (lane == 0) s_wc[wv] = __popcll(m);
+            __syncthreads();
+            int off = run;
+            for (int j = 0; j < wv; ++j) off += s_wc[j];
+            run += s_wc[0] + s_wc[1] + s_wc[2] + s_wc[3];
+            __syncthreads();
+            if (ir < nrec) {
+                const int aex = off + __popcll(m & ((1ull << lane) - 1));   // observed receivers before ir
+                const int l = b * nrec + (act ? aex : na + (ir - aex));
+                pos[3 * ir + c] = l;
+                inv[l] = 3 * ir + c;
+            }
+        }
+    }
+    __syncthreads();
+    // 3. RESIDV = W C W WORK on the tridiagonal C
+    const int na0 = s_na[2], na1 = s_na[1], nall = s_na[2] + s_na[1] + s_na[0];
+    for (int l = tid; l < n3; l += blockDim.x) {
+        float2 acc = make_float2(0.f, 0.f);
+        if (l < nall) {
+            const int b = l < na0 ? 0 : (l < na0 + na1 ? 1 : 2);
+            const int kk = l - (b == 0 ? 0 : (b == 1 ? na0 : na0 + na1)), na = s_na[2 - b];
+            const int *cinv = inv + b * nrec;       // observed receivers of the component, in order
+            double cm = 0.0, cd = 0.0, cp = 0.0;    // C(l, l-1), C(l, l), C(l, l+1)
+            if (kk > 0) {
+                const double t = 2.0 / covd_h(ldwght, dx, cinv, kk - 1);
+                cm = t * -0.5;
+                cd = cd + t * 0.5;
+            }
+            if (kk + 1 < na) {
+                const double t = 2.0 / covd_h(ldwght, dx, cinv, kk);
+                cd = cd + t * 0.5;
+                cp = t * -0.5;
+            }
+            const float wl = wghts[gsrc + gb0 + (inv[l] / 3) * gstride + inv[l] % 3];
+            for (int j = -1; j <= 1; ++j) {
+                const int k = l + j;
+                if (k < 0 || k >= n3) continue;
+                const double cv = j < 0 ? cm : (j == 0 ? cd : cp);
+                if (cv == 0.0) continue;
+                const float wk = wghts[gsrc + gb0 + (inv[k] / 3) * gstride + inv[k] % 3];
+                const float a = __fmul_rn(__fmul_rn(wl, (float)cv), wk);
+                const float2 x = dif[inv[k]];
+                acc.x = __fadd_rn(acc.x, __fmul_rn(x.x, a));
+                acc.y = __fadd_rn(acc.y, __fmul_rn(x.y, a));
+            }
+        }
+        res[l] = acc;
+    }
+    __syncthreads();
+    // 4. back to the receivers, rotated back, HDIFFER, conjugate
+    const float2 h = hdiffer(prm.ibphase, f);
+    for (int ir = tid; ir < nrec; ir += blockDim.x) {
+        float2 R[3];
+        for (int i = 0; i < 3; ++i) R[i] = res[pos[3 * ir + i]];
+        float2 q1, q2;
+        crotate(-prm.az, R[0], R[1], q1, q2);
+        R[0] = q1; R[1] = q2;
+        for (int i = 0; i < 3; ++i) {
+            const float2 t = cmulf(h, R[i]);
+            v[3 * ir + i] = make_float2(t.x, -t.y);
+        }
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    float2 *out = rhs + (size_t)p * ndof;
+    for (int k = 0; k < n3; ++k) {
+        const int d = mrdof[k];
+        if (d <= 0 || d > ndof) { atomicOr(warn, 1); continue; }
+        out[d - 1].x += v[k].x;
+        out[d - 1].y += v[k].y;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // small utilities
 template <typename T>

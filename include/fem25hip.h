@@ -320,8 +320,9 @@ int f25_exresp25d(int ndof, int nsrc, int nrec, double freq, const double *py, c
 int f25_exresp25d_dev(int ndof, int nsrc, int nrec, double freq, const double *py, const double *yrec,
                       const void *sol_dev, const int *mrdof, const void *recv, void *est_dev);
 /* CADJLAM's right-hand side (adjgrad.f90:229-300, without the data
- * covariance option LCOVD) for npair (ifreq, isrc) pairs (1-based, with
- * their frequency in Hz and slowness py): estimates rotated to (N,E,Z) by
+ * covariance option LCOVD, for which see f25_cadjlam_rhs_covd) for npair
+ * (ifreq, isrc) pairs (1-based, with their frequency in Hz and slowness
+ * py): estimates rotated to (N,E,Z) by
  * azmod degrees (magnitude/phase pairs turned complex first when lunwrap),
  * data and estimates shifted by exp(-i 2 pi freq py yrec), BPRESID4
  * residuals (objfn.f:757-839: norm, irestp, epss, dtmax), rotated back,
@@ -339,6 +340,32 @@ int f25_cadjlam_rhs_dev(int ndof, int nfreq, int nrec, int nsrc, int npair, cons
                         float dtmax, int ibphase, int lunwrap, const double *yrec, const int *mrdof,
                         const void *est_dev, const void *obs_dev, const float *wghts_dev, void *rhs_dev,
                         int *iwarn);
+
+/* CADJLAM's right-hand side with the data covariance (LCOVD,
+ * adjgrad.f90:219-235 and 296-298): per (ifreq, isrc) pair the roughening
+ * matrix of LAPLACE_1DV2 (locrec2d.f:334-509; with ldwght the element
+ * lengths are the summed normalised spacings dx between observed receivers,
+ * else 1) and PERM_OBS's ordering (locrec2d.f:511-549) of the untransformed
+ * observations, then BPRESID_COVD4 (objfn.f:635-733: CRESID(irestp) with
+ * the single-precision phase window dtmax, weighted on both sides by the
+ * receiver weights) in place of BPRESID4; the rotations, phase shift and
+ * RESID_DRHS as in f25_cadjlam_rhs.  dx holds the nrec - 1 spacings
+ * (f25_rec_space1d), each positive and finite when ldwght (F25_ERR_ARG
+ * otherwise; the reference stops on a zero spacing sum).  The roughening
+ * matrix is tridiagonal and is applied as such. */
+int f25_cadjlam_rhs_covd(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq, const int *isrc,
+                         const double *freq, const double *py, double azmod, int irestp, float dtmax, int ibphase,
+                         int lunwrap, int ldwght, const double *dx, const double *yrec, const int *mrdof,
+                         const void *est, const void *obs, const float *wghts, void *rhs, int *iwarn);
+int f25_cadjlam_rhs_covd_dev(int ndof, int nfreq, int nrec, int nsrc, int npair, const int *ifreq,
+                             const int *isrc, const double *freq, const double *py, double azmod, int irestp,
+                             float dtmax, int ibphase, int lunwrap, int ldwght, const double *dx,
+                             const double *yrec, const int *mrdof, const void *est_dev, const void *obs_dev,
+                             const float *wghts_dev, void *rhs_dev, int *iwarn);
+/* REC_SPACE1D (locrec2d.f:157-223): dx[i] = |xrec[i+1] - xrec[i]| divided by
+ * the smallest such spacing.  F25_ERR_ARG, dx zeroed, for one receiver, a
+ * spacing below 1e-10 or receivers out of order (the reference's checks). */
+int f25_rec_space1d(int nrec, const double *xrec, double *dx);
 
 /* ---- second-order products of QPMFREE (truncated-Newton Hessian) ----------- */
 

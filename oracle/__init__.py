@@ -8,8 +8,11 @@ from __future__ import annotations
 
 import ctypes as C
 import dataclasses
+import json
 import os
 import subprocess
+import sys
+import tempfile
 
 import numpy as np
 
@@ -53,6 +56,19 @@ def ref():
     if _REF is None:
         _REF = C.CDLL(os.path.join(_HERE, "_ref", "libfem25ref.so"))
     return _REF
+
+
+def _child_call(name, scalars, arrays):
+    """Runs oracle.<name>(**scalars, **arrays, reference=True) in a fresh
+    interpreter that has not loaded torch; returns its outputs (npz)."""
+    with tempfile.TemporaryDirectory(prefix="orcref") as d:
+        np.savez(os.path.join(d, "in.npz"), **arrays)
+        with open(os.path.join(d, "in.json"), "w") as f:
+            json.dump({"name": name, "scalars": scalars}, f)
+        env = dict(os.environ, PYTHONPATH=os.pathsep.join([os.path.dirname(_HERE), os.environ.get("PYTHONPATH", "")]))
+        subprocess.run([sys.executable, "-m", "oracle._child", d], check=True, env=env, cwd=os.path.dirname(_HERE))
+        with np.load(os.path.join(d, "out.npz"), allow_pickle=False) as z:
+            return {k: z[k] for k in z.files}
 
 
 class _Asm(C.Structure):
@@ -484,6 +500,74 @@ def cadjlam_rhs(ndof, ifreq, isrc, freq, py, azmod, norm, irestp, epss, dtmax, i
                           _p(mrdof, ip), _p(est, fp), _p(obs, fp), _p(wghts, fp), _p(rhs, fp), C.byref(iw))
     return rhs, iw.value
 
+
+def rec_space1d(xrec, reference=False):
+    """REC_SPACE1D: normalised receiver spacings (nrec - 1) and the error flag."""
+    xrec = np.ascontiguousarray(xrec, np.float64)
+    n = xrec.size
+    dx = np.zeros(max(n - 1, 1), np.float64)
+    if reference:
+        ierr = C.c_int(0)
+        ref().ref_rec_space1d(n, _p(xrec, dp), _p(dx, dp), C.byref(ierr))
+        ierr = ierr.value
+    else:
+        ierr = lib().orc_rec_space1d(n, _p(xrec, dp), _p(dx, dp))
+    return dx[:n - 1], ierr
+
+
+def covd_resid(nrec, irestp, ldwght, dtmax, freq, dx, obsraw, obs, est, swght, reference=False):
+    """LAPLACE_1DV2 + PERM_OBS on obsraw, BPRESID_COVD4 on (obs, est, swght),
+    all (nrec, 3).  Returns (resid (nrec, 3), COVD8 (6 nrec, 6 nrec),
+    iperm (nrec, 3), ierr)."""
+    obsraw, obs, est = _c8(obsraw), _c8(obs), _c8(est)
+    swght = np.ascontiguousarray(swght, np.float32)
+    dx = np.ascontiguousarray(np.resize(np.asarray(dx, np.float64), max(nrec - 1, 1)))
+    resid = np.zeros((nrec, 3), np.complex64)
+    covd = np.zeros((6 * nrec, 6 * nrec), np.float64)      # column-major: covd[col, row]
+    iperm = np.zeros((nrec, 3), np.int32)
+    if reference and "torch" in sys.modules:
+        # libtorch_cpu carries a static MKL whose exported entry points take
+        # the place of the MKL the compiled reference links (its SGEMV then
+        # returns wrong values): run the compiled routine in a fresh process
+        out = _child_call("covd_resid", dict(nrec=nrec, irestp=irestp, ldwght=int(ldwght), dtmax=float(dtmax),
+                                             freq=float(freq)),
+                          dict(dx=dx, obsraw=obsraw, obs=obs, est=est, swght=swght))
+        return out["resid"], out["covd"], out["iperm"], int(out["ierr"])
+    if reference:
+        ierr = C.c_int(0)
+        ref().ref_covd_resid(nrec, irestp, int(ldwght), C.c_float(dtmax), C.c_double(freq), _p(dx, dp),
+                             _p(obsraw, fp), _p(obs, fp), _p(est, fp), _p(swght, fp), _p(resid, fp),
+                             _p(covd, dp), _p(iperm, ip), C.byref(ierr))
+        ierr = ierr.value
+    else:
+        L = lib()
+        ierr = L.orc_laplace_1dv2(nrec, int(ldwght), _p(dx, dp), _p(obsraw, fp), _p(covd, dp))
+        if not ierr:
+            ierr = L.orc_perm_obs(nrec, _p(obsraw, fp), _p(iperm, ip))
+        if not ierr:
+            L.orc_bpresid_covd4(nrec, _p(iperm, ip), irestp, C.c_float(dtmax), C.c_double(freq), _p(covd, dp),
+                                _p(swght, fp), _p(obs, fp), _p(est, fp), _p(resid, fp))
+    return resid, covd.T.copy(), iperm, ierr
+
+
+def cadjlam_rhs_covd(ndof, ifreq, isrc, freq, py, azmod, irestp, dtmax, ibphase, lunwrap, ldwght, dx, yrec,
+                     mrdof, est, obs, wghts):
+    """CADJLAM's right-hand side with the data covariance (LCOVD); arguments
+    as cadjlam_rhs with ldwght and dx (REC_SPACE1D) in place of norm and
+    epss.  Returns (rhs, iwarn, ierr)."""
+    est, obs = _c8(est), _c8(obs)
+    wghts = np.ascontiguousarray(wghts, np.float32)
+    nsrc, nrec, nfreq, _ = est.shape
+    yrec = np.ascontiguousarray(yrec, np.float64)
+    mrdof = np.ascontiguousarray(mrdof, np.int32)
+    dx = np.ascontiguousarray(np.resize(np.asarray(dx, np.float64), max(nrec - 1, 1)))
+    rhs = np.zeros(ndof, np.complex64)
+    iw = C.c_int(0)
+    ierr = lib().orc_cadjlam_rhs_covd(ndof, nfreq, nrec, ifreq, isrc, C.c_double(freq), C.c_double(py),
+                                      C.c_double(azmod), irestp, C.c_float(dtmax), ibphase, int(lunwrap),
+                                      int(ldwght), _p(dx, dp), _p(yrec, dp), _p(mrdof, ip), _p(est, fp),
+                                      _p(obs, fp), _p(wghts, fp), _p(rhs, fp), C.byref(iw))
+    return rhs, iw.value, ierr
 
 def _elem_inputs(nord, xlocs, zlocs, dens, ecoeff):
     xlocs = np.ascontiguousarray(xlocs, np.float64)

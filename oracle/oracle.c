@@ -63,6 +63,16 @@
  *   orc_cadjlam_rhs  composition of the above; with equal weights on the two
  *                    horizontal components the L2 residual rotated back is
  *                    W^2 (R(-az) OBS - EST) (closed form).
+ *   orc_rec_space1d,
+ *   orc_laplace_1dv2,
+ *   orc_perm_obs,
+ *   orc_bpresid_covd4 compiled reference REC_SPACE1D, LAPLACE_1DV2, PERM_OBS,
+ *                    BPRESID_COVD4; with every receiver observed and unit
+ *                    weights the residual is the second difference of the
+ *                    residual along the line (closed form), rows sum to zero
+ *                    and <WORK, RESIDV> = sum (r_k+1 - r_k)^2 / h_k.
+ *   orc_cadjlam_rhs_covd composition of the above (compiled LAPLACE_1DV2,
+ *                    PERM_OBS and BPRESID_COVD4 behind the same frames).
  */
 #include "oracle.h"
 #include <math.h>
@@ -1876,7 +1886,7 @@ void orc_bpresid4(int nrec, int norm, int irestp, float epss, double freq, float
         }
 }
 
-/* adjgrad.f:527-566 HDIFFER: spreading compensation factor */
+/* adjgrad.f90:527-566 HDIFFER: spreading compensation factor */
 float complex orc_hdiffer(int ibphase, double freq)
 {
     const double pi = 3.1415926535897932384626434;
@@ -1894,7 +1904,7 @@ float complex orc_hdiffer(int ibphase, double freq)
     return CMPLXF(1.0f, 0.0f);
 }
 
-/* adjgrad.f:448-523 RESID_DRHS: RHS(MRDOF(i,irec)) += HFACT*RESID(i,irec)
+/* adjgrad.f90:448-523 RESID_DRHS: RHS(MRDOF(i,irec)) += HFACT*RESID(i,irec)
  * in receiver order, invalid DOFs skipped with IWARN = 1, then RHS = CONJG */
 void orc_resid_drhs(int ndof, int nrec, int ibphase, double freq, const int *mrdof,
                     const float complex *resid, float complex *rhs, int *iwarn)
@@ -1937,27 +1947,22 @@ void orc_hdiffer_v(int ibphase, double freq, float *out)
     out[1] = cimagf(h);
 }
 
-/* adjgrad.f:149-300 CADJLAM up to the solve (no data covariance, LCOVD
- * false): for receiver irec of (IFREQ, ISRC), estimates rotated to (N,E,Z)
- * by AZMOD (magnitude/phase pairs first turned complex when LUNWRAP), data
- * and estimates shifted by exp(-i SNGL(2 pi FREQ PY YREC)), BPRESID4,
- * residuals rotated back by -AZMOD, RESID_DRHS.  est, obs: (3, nfreq, nrec,
- * nsrc) COMPLEX*8; wghts REAL*4 likewise; ifreq, isrc 1-based. */
-void orc_cadjlam_rhs(int ndof, int nfreq, int nrec, int ifreq, int isrc, double freq, double py,
-                     double azmod, int norm, int irestp, float epss, float dtmax, int ibphase,
-                     int lunwrap, const double *yrec, const int *mrdof, const float complex *est,
-                     const float complex *obs, const float *wghts, float complex *rhs, int *iwarn)
+/* adjgrad.f90:240-293, the loop that readies receiver irec of (IFREQ, ISRC)
+ * for the residual: estimates rotated to (N,E,Z) by AZMOD (magnitude/phase
+ * pairs first turned complex when LUNWRAP), data and estimates shifted by
+ * exp(-i SNGL(2 pi FREQ PY YREC)).  Fills o, e, w (3 x nrec) and, when raw
+ * is given, the untransformed observations inv%OBS(1:3, IFREQ, :, ISRC). */
+static void cadj_frames(int nfreq, int nrec, int ifreq, int isrc, double freq, double py, double azmod,
+                        int lunwrap, const double *yrec, const float complex *est, const float complex *obs,
+                        const float *wghts, float complex *o, float complex *e, float *w, float complex *raw)
 {
     const double twopi = 6.2831853071795862;
-    size_t n3 = (size_t)NDIM * (nrec > 0 ? nrec : 1);
-    float complex *e = (float complex *)calloc(n3, sizeof(float complex));
-    float complex *o = (float complex *)calloc(n3, sizeof(float complex));
-    float complex *r = (float complex *)calloc(n3, sizeof(float complex));
-    float *w = (float *)calloc(n3, sizeof(float));
     for (int ir = 0; ir < nrec; ir++) {
         long base = NDIM * ((long)(ifreq - 1) + (long)nfreq * (ir + (long)nrec * (isrc - 1)));
         const float complex *ev = est + base, *ov = obs + base;
         for (int i = 0; i < NDIM; i++) w[NDIM * ir + i] = wghts[base + i];
+        if (raw)
+            for (int i = 0; i < NDIM; i++) raw[NDIM * ir + i] = ov[i];
         float complex q1, q2, e3, o1, o2, o3;
         if (lunwrap) {
             q1 = orc_cphm2cm(crealf(ev[0]), cimagf(ev[0]));
@@ -1980,6 +1985,23 @@ void orc_cadjlam_rhs(int ndof, int nfreq, int nrec, int ifreq, int isrc, double 
             E[i] = cmulf_ref(E[i], cfact);
         }
     }
+}
+
+/* adjgrad.f90:149-300 CADJLAM up to the solve (no data covariance, LCOVD
+ * false): cadj_frames, BPRESID4, residuals rotated back by -AZMOD,
+ * RESID_DRHS.  est, obs: (3, nfreq, nrec, nsrc) COMPLEX*8; wghts REAL*4
+ * likewise; ifreq, isrc 1-based. */
+void orc_cadjlam_rhs(int ndof, int nfreq, int nrec, int ifreq, int isrc, double freq, double py,
+                     double azmod, int norm, int irestp, float epss, float dtmax, int ibphase,
+                     int lunwrap, const double *yrec, const int *mrdof, const float complex *est,
+                     const float complex *obs, const float *wghts, float complex *rhs, int *iwarn)
+{
+    size_t n3 = (size_t)NDIM * (nrec > 0 ? nrec : 1);
+    float complex *e = (float complex *)calloc(n3, sizeof(float complex));
+    float complex *o = (float complex *)calloc(n3, sizeof(float complex));
+    float complex *r = (float complex *)calloc(n3, sizeof(float complex));
+    float *w = (float *)calloc(n3, sizeof(float));
+    cadj_frames(nfreq, nrec, ifreq, isrc, freq, py, azmod, lunwrap, yrec, est, obs, wghts, o, e, w, NULL);
     orc_bpresid4(nrec, norm, irestp, epss, freq, dtmax, w, o, e, r);
     for (int ir = 0; ir < nrec; ir++) {
         float complex *R = r + NDIM * ir, qn = R[0], qe = R[1];
@@ -1987,6 +2009,182 @@ void orc_cadjlam_rhs(int ndof, int nfreq, int nrec, int ifreq, int isrc, double 
     }
     orc_resid_drhs(ndof, nrec, ibphase, freq, mrdof, r, rhs, iwarn);
     free(e); free(o); free(r); free(w);
+}
+
+/* locrec2d.f:157-223 REC_SPACE1D: dx(i) = |x(i+1) - x(i)| / min_j |x(j+1) -
+ * x(j)|.  Returns 1 with dx zeroed for a single receiver, a spacing below
+ * 1e-10, or a reversal of direction (checked, as the reference checks it,
+ * at receivers 2..nrec-1 against the first spacing's sign). */
+int orc_rec_space1d(int nrec, const double *xrec, double *dx)
+{
+    for (int i = 0; i < nrec - 1; i++) dx[i] = 0.0;
+    if (nrec == 1) return 1;
+    double xdir = xrec[1] - xrec[0], dxmin = HUGE_VAL;
+    for (int i = 0; i < nrec - 1; i++) {          /* i = IREC - 1 */
+        dx[i] = fabs(xrec[i + 1] - xrec[i]);
+        dxmin = fmin(dxmin, dx[i]);
+        if (dx[i] < 1e-10 ||
+            (i > 0 && ((xrec[i] > xrec[i - 1] && xdir < 0.0) || (xrec[i] < xrec[i - 1] && xdir > 0.0)))) {
+            for (int j = 0; j < nrec - 1; j++) dx[j] = 0.0;
+            return 1;
+        }
+    }
+    for (int i = 0; i < nrec - 1; i++) dx[i] = dx[i] / dxmin;
+    return 0;
+}
+
+/* locrec2d.f:334-509 LAPLACE_1DV2: the (6 nrec x 6 nrec, column-major) data
+ * roughening matrix.  Components are taken in the order 3, 2, 1; the
+ * receivers with a non-zero observation of a component are the nodes of a
+ * chain of two-node elements with stiffness (2/h)(1/2)[1 -1; -1 1], h the
+ * summed normalised spacing to the next such receiver (1 without ldwght);
+ * the nodes take consecutive numbers over all components (a component with
+ * no observation takes none), and the same block is added at row/column
+ * offset 3 nrec for the imaginary parts.  A zero observation matrix (dead
+ * frequency) leaves COVD zero; returns 1 when a spacing sums to zero. */
+int orc_laplace_1dv2(int nrec, int ldwght, const double *dx, const float complex *obs, double *covd)
+{
+    const int ldc = 2 * NDIM * nrec;
+    memset(covd, 0, sizeof(double) * (size_t)ldc * ldc);
+    float amax = 0.0f;
+    for (int k = 0; k < NDIM * nrec; k++) amax = fmaxf(amax, cabsf(obs[k]));
+    if (amax == 0.0f) return 0;
+    int *act = (int *)malloc(sizeof(int) * nrec);
+    double *h = (double *)malloc(sizeof(double) * nrec);
+    int idof = 0, ierr = 0;
+    for (int c = NDIM - 1; c >= 0 && !ierr; c--) {
+        int na = 0;
+        for (int ir = 0; ir < nrec; ir++)
+            if (cabsf(obs[NDIM * ir + c]) > 0.0f) act[na++] = ir;
+        if (na == 0) continue;                         /* dead component */
+        const int nel = na - 1;
+        for (int j = 0; j < nel && !ierr; j++) {
+            if (ldwght) {
+                double xs = 0.0;
+                for (int r = act[j]; r < act[j + 1]; r++) xs += dx[r];
+                h[j] = xs;
+                if (xs == 0.0) ierr = 1;
+            } else {
+                h[j] = 1.0;
+            }
+        }
+        if (ierr) break;
+        for (int j = 0; j < nel; j++) {
+            const int d[2] = {idof + j, idof + j + 1};   /* 0-based node numbers */
+            const double kk[2][2] = {{0.5, -0.5}, {-0.5, 0.5}};
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++) {
+                    const double t = 2.0 / h[j] * kk[a][b];
+                    covd[(size_t)d[b] * ldc + d[a]] += t;
+                    covd[(size_t)(NDIM * nrec + d[b]) * ldc + NDIM * nrec + d[a]] += t;
+                }
+        }
+        idof += na;
+    }
+    free(act); free(h);
+    return ierr;
+}
+
+/* locrec2d.f:511-549 PERM_OBS: iperm[3 irec + i] (1-based positions) lists,
+ * for components 3, 2, 1 in turn, the receivers with a non-zero observation
+ * and then those without. */
+int orc_perm_obs(int nrec, const float complex *obs, int *iperm)
+{
+    int iobs = 0;
+    for (int k = 0; k < NDIM * nrec; k++) iperm[k] = 0;
+    for (int c = NDIM - 1; c >= 0; c--) {
+        for (int ir = 0; ir < nrec; ir++)
+            if (cabsf(obs[NDIM * ir + c]) > 0.0f) iperm[NDIM * ir + c] = ++iobs;
+        for (int ir = 0; ir < nrec; ir++)
+            if (cabsf(obs[NDIM * ir + c]) == 0.0f) iperm[NDIM * ir + c] = ++iobs;
+    }
+    for (int k = 0; k < NDIM * nrec; k++)
+        if (iperm[k] == 0) return 1;
+    return 0;
+}
+
+/* objfn.f:635-733 BPRESID_COVD4: WORK holds CRESID(IRESTP) of every
+ * receiver component with a non-zero observation at its permuted position
+ * (real parts, then imaginary parts 3 nrec further on), zeroed where the
+ * phase gate (DTMAX > 0, in seconds, single precision) fails; COVD = SNGL
+ * of COVD8 scaled on both sides by the receiver weights at the permuted
+ * positions; RESIDV = COVD WORK as SGEMV forms it (column by column, single
+ * precision); RESID(i, irec) from RESIDV at the receiver's positions. */
+void orc_bpresid_covd4(int nrec, const int *iperm, int irestp, float dtmax, double freq,
+                       const double *covd8, const float *swght, const float complex *obs,
+                       const float complex *est, float complex *resid)
+{
+    const float twopi = 6.283185307179586f, pi = 3.1415926535897931f;
+    const int n3 = NDIM * nrec, m = 2 * n3;
+    float *work = (float *)calloc((size_t)m, sizeof(float));
+    float *wp = (float *)calloc((size_t)m, sizeof(float));
+    float *rv = (float *)calloc((size_t)m, sizeof(float));
+    for (int k = 0; k < n3; k++) {
+        const int l = iperm[k] - 1;
+        wp[l] = swght[k];
+        wp[n3 + l] = swght[k];
+        if (cabsf(obs[k]) > 0.0f) {
+            const float complex dif = orc_cresid(irestp, obs[k], est[k]);
+            work[l] = crealf(dif);
+            work[n3 + l] = cimagf(dif);
+            if (dtmax > 0.0f) {
+                float dphi = sphase(obs[k]) - sphase(est[k]);
+                if (dphi < -pi) dphi = dphi + twopi;
+                if (dphi > +pi) dphi = dphi - twopi;
+                dphi = dphi / (twopi * (float)freq);
+                if (fabsf(dphi) > dtmax) {
+                    work[l] = 0.0f;
+                    work[n3 + l] = 0.0f;
+                }
+            }
+        }
+    }
+    for (int j = 0; j < m; j++) {                       /* SGEMV 'N', beta = 0 */
+        const float t = work[j];
+        for (int i = 0; i < m; i++) {
+            const float a = (wp[i] * (float)covd8[(size_t)j * m + i]) * wp[j];
+            rv[i] = rv[i] + t * a;
+        }
+    }
+    for (int k = 0; k < n3; k++) {
+        const int l = iperm[k] - 1;
+        resid[k] = CMPLXF(rv[l], rv[n3 + l]);
+    }
+    free(work); free(wp); free(rv);
+}
+
+/* adjgrad.f90:149-300 CADJLAM up to the solve with the data covariance
+ * (LCOVD): LAPLACE_1DV2 and PERM_OBS of the untransformed observations,
+ * cadj_frames, BPRESID_COVD4, residuals rotated back by -AZMOD,
+ * RESID_DRHS.  Arguments as orc_cadjlam_rhs, with ldwght and dx (nrec - 1
+ * normalised spacings, REC_SPACE1D) in place of norm and epss.  Returns
+ * LAPLACE_1DV2's or PERM_OBS's error flag (rhs untouched then). */
+int orc_cadjlam_rhs_covd(int ndof, int nfreq, int nrec, int ifreq, int isrc, double freq, double py,
+                         double azmod, int irestp, float dtmax, int ibphase, int lunwrap, int ldwght,
+                         const double *dx, const double *yrec, const int *mrdof, const float complex *est,
+                         const float complex *obs, const float *wghts, float complex *rhs, int *iwarn)
+{
+    const size_t n3 = (size_t)NDIM * (nrec > 0 ? nrec : 1), ldc = 2 * n3;
+    float complex *e = (float complex *)calloc(n3, sizeof(float complex));
+    float complex *o = (float complex *)calloc(n3, sizeof(float complex));
+    float complex *raw = (float complex *)calloc(n3, sizeof(float complex));
+    float complex *r = (float complex *)calloc(n3, sizeof(float complex));
+    float *w = (float *)calloc(n3, sizeof(float));
+    int *iperm = (int *)calloc(n3, sizeof(int));
+    double *covd = (double *)malloc(sizeof(double) * ldc * ldc);
+    cadj_frames(nfreq, nrec, ifreq, isrc, freq, py, azmod, lunwrap, yrec, est, obs, wghts, o, e, w, raw);
+    int ierr = orc_laplace_1dv2(nrec, ldwght, dx, raw, covd);
+    if (!ierr) ierr = orc_perm_obs(nrec, raw, iperm);
+    if (!ierr) {
+        orc_bpresid_covd4(nrec, iperm, irestp, dtmax, freq, covd, w, o, e, r);
+        for (int ir = 0; ir < nrec; ir++) {
+            float complex *R = r + NDIM * ir, qn = R[0], qe = R[1];
+            orc_crotate(-(float)azmod, qn, qe, &R[0], &R[1]);
+        }
+        orc_resid_drhs(ndof, nrec, ibphase, freq, mrdof, r, rhs, iwarn);
+    }
+    free(e); free(o); free(raw); free(r); free(w); free(iperm); free(covd);
+    return ierr;
 }
 
 /* Exact derivative of S(w,ky) with respect to anchor lambda/mu/rho.  The

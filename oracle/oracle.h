@@ -199,6 +199,16 @@ void orc_cadjlam_rhs(int ndof, int nfreq, int nrec, int ifreq, int isrc, double 
                      double azmod, int norm, int irestp, float epss, float dtmax, int ibphase,
                      int lunwrap, const double *yrec, const int *mrdof, const float complex *est,
                      const float complex *obs, const float *wghts, float complex *rhs, int *iwarn);
+int orc_rec_space1d(int nrec, const double *xrec, double *dx);
+int orc_laplace_1dv2(int nrec, int ldwght, const double *dx, const float complex *obs, double *covd);
+int orc_perm_obs(int nrec, const float complex *obs, int *iperm);
+void orc_bpresid_covd4(int nrec, const int *iperm, int irestp, float dtmax, double freq,
+                       const double *covd8, const float *swght, const float complex *obs,
+                       const float complex *est, float complex *resid);
+int orc_cadjlam_rhs_covd(int ndof, int nfreq, int nrec, int ifreq, int isrc, double freq, double py,
+                         double azmod, int irestp, float dtmax, int ibphase, int lunwrap, int ldwght,
+                         const double *dx, const double *yrec, const int *mrdof, const float complex *est,
+                         const float complex *obs, const float *wghts, float complex *rhs, int *iwarn);
 
 /* pchadj.f90:225-281 (NVINV = 1), one (frequency, source):
  *   JACL(irhs, c) = sum over column c of F of G((irhs-1)*lrhs + ICSC(i)) FMAT(i)

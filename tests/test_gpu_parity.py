@@ -893,6 +893,103 @@ def test_cadjlam_rhs(norm, dtmax):
     assert iw.value == 1
 
 
+def _covd_case(seed, nrec, nfreq=3, nsrc=3, ndof=5000):
+    rng = np.random.default_rng(seed)
+    mrdof = rng.integers(1, ndof + 1, size=(nrec, 3)).astype(np.int32)
+    mrdof[4, 1] = mrdof[9, 0]                   # two receivers on one DOF
+    yrec = rng.uniform(-3e4, 3e4, nrec)
+    x = np.cumsum(rng.uniform(0.5, 3.0, nrec))
+    dx, ierr = oracle.rec_space1d(x)
+    assert ierr == 0
+    cv = lambda *s: (rng.standard_normal(s) + 1j * rng.standard_normal(s)).astype(np.complex64)  # noqa: E731
+    est = cv(nsrc, nrec, nfreq, 3)
+    obs = (est + 0.4 * cv(nsrc, nrec, nfreq, 3)).astype(np.complex64)
+    obs[rng.random(obs.shape) < 0.2] = 0        # gaps in every component
+    obs[0, :, 1, 0] = 0                         # a dead component
+    obs[1, :, 0, :] = 0                         # a dead frequency
+    obs[2, :, 2, 1] = 0
+    obs[2, 7, 2, 1] = 1 + 1j                    # a component with one observed receiver
+    w = rng.uniform(0.3, 2.0, (nsrc, nrec, nfreq, 3)).astype(np.float32)
+    return rng, ndof, mrdof, yrec, dx, est, obs, w
+
+
+@pytest.mark.parametrize("ldwght,dtmax,nrec", [(1, 0.0, 60), (0, 0.0, 60), (1, 1.5, 60), (1, 0.0, 700)])
+def test_cadjlam_rhs_covd(ldwght, dtmax, nrec):
+    """CADJLAM's right-hand side with the data covariance (LAPLACE_1DV2,
+    PERM_OBS, BPRESID_COVD4) on the GPU for every (frequency, source) pair,
+    every residual type, both data forms and a rotated frame, against the
+    oracle (whose parts are pinned to the compiled routines): 2e-6 of the
+    pair's largest value.  The data hold gaps in every component (the
+    reference's roughening matrix is then numbered apart from PERM_OBS's
+    positions), a dead component, a dead frequency and a component with a
+    single observed receiver; 700 receivers take the position scan over
+    several blocks of 256."""
+    from fem25_amd import _lib
+    L = _lib.load()
+    rng, ndof, mrdof, yrec, dx, est, obs, w = _covd_case(120 + nrec + 2 * ldwght, nrec)
+    nsrc, _, nfreq, _ = est.shape
+    pairs = [(f + 1, s + 1) for f in range(nfreq) for s in range(nsrc)]
+    ifreq = np.array([p[0] for p in pairs], np.int32)
+    isrc = np.array([p[1] for p in pairs], np.int32)
+    freq = 0.02 + 0.02 * ifreq.astype(np.float64)
+    py = rng.uniform(0, 1.3e-4, len(pairs))
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)   # noqa: E731
+    for irestp in (1, 2, 3):
+        for lunwrap in (0, 1):
+            ib = (irestp + 2 * lunwrap) % 7
+            az = 17.5 * irestp - 30.0
+            print(f"cadjlam_covd ldwght={ldwght} dtmax={dtmax} nrec={nrec} irestp={irestp} lunwrap={lunwrap}",
+                  flush=True)
+            rhs = np.zeros((len(pairs), ndof), np.complex64)
+            iw = C.c_int(-1)
+            _lib.check(L.f25_cadjlam_rhs_covd(ndof, nfreq, nrec, nsrc, len(pairs), _ptr(ifreq, _lib.ip),
+                                              _ptr(isrc, _lib.ip), _ptr(freq, _lib.dp), _ptr(py, _lib.dp), az,
+                                              irestp, dtmax, ib, lunwrap, ldwght, _ptr(dx, _lib.dp),
+                                              _ptr(yrec, _lib.dp), _ptr(mrdof, _lib.ip), vp(est), vp(obs),
+                                              _ptr(w, _lib.fp), vp(rhs), C.byref(iw)), "cadjlam_rhs_covd")
+            assert iw.value == 0
+            for k, (f, s) in enumerate(pairs):
+                ref, iwr, ierr = oracle.cadjlam_rhs_covd(ndof, f, s, freq[k], py[k], az, irestp, dtmax, ib,
+                                                         bool(lunwrap), ldwght, dx, yrec, mrdof, est, obs, w)
+                assert ierr == 0 and iwr == 0
+                scale = np.abs(ref).max()
+                err = np.abs(rhs[k] - ref)
+                assert err.max() <= 2e-6 * max(scale, 1e-30), (k, irestp, lunwrap, err.max(), scale)
+                if not obs[s - 1, :, f - 1].any():
+                    assert not rhs[k].any()
+
+
+def test_cadjlam_rhs_covd_dev_matches_host_form(torch_dev):
+    """The device-pointer form of the data-covariance right-hand side gives
+    the host form's values bit for bit."""
+    from fem25_amd import _lib
+    torch = torch_dev
+    L = _lib.load()
+    rng, ndof, mrdof, yrec, dx, est, obs, w = _covd_case(131, 90)
+    nsrc, nrec, nfreq, _ = est.shape
+    ifreq = np.repeat(np.arange(1, nfreq + 1), nsrc).astype(np.int32)
+    isrc = np.tile(np.arange(1, nsrc + 1), nfreq).astype(np.int32)
+    npair = ifreq.size
+    freq = 0.03 * ifreq.astype(np.float64)
+    py = np.full(npair, 0.7e-4)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)   # noqa: E731
+    rhs = np.zeros((npair, ndof), np.complex64)
+    iw = C.c_int(-1)
+    _lib.check(L.f25_cadjlam_rhs_covd(ndof, nfreq, nrec, nsrc, npair, _ptr(ifreq, _lib.ip), _ptr(isrc, _lib.ip),
+                                      _ptr(freq, _lib.dp), _ptr(py, _lib.dp), 12.0, 3, 0.0, 1, 0, 1,
+                                      _ptr(dx, _lib.dp), _ptr(yrec, _lib.dp), _ptr(mrdof, _lib.ip), vp(est), vp(obs),
+                                      _ptr(w, _lib.fp), vp(rhs), C.byref(iw)), "cadjlam_rhs_covd")
+    ed, od, wd = (torch.from_numpy(a).cuda() for a in (est, obs, w))
+    rd = torch.full((npair * ndof,), 7 + 7j, dtype=torch.complex64, device="cuda")
+    _lib.check(L.f25_cadjlam_rhs_covd_dev(ndof, nfreq, nrec, nsrc, npair, _ptr(ifreq, _lib.ip), _ptr(isrc, _lib.ip),
+                                          _ptr(freq, _lib.dp), _ptr(py, _lib.dp), 12.0, 3, 0.0, 1, 0, 1,
+                                          _ptr(dx, _lib.dp), _ptr(yrec, _lib.dp), _ptr(mrdof, _lib.ip),
+                                          C.c_void_p(ed.data_ptr()), C.c_void_p(od.data_ptr()),
+                                          C.cast(C.c_void_p(wd.data_ptr()), _lib.fp), C.c_void_p(rd.data_ptr()),
+                                          C.byref(iw)), "cadjlam_rhs_covd_dev")
+    np.testing.assert_array_equal(rd.cpu().numpy().reshape(npair, ndof), rhs)
+
+
 def test_qp_rhs_full_size_against_operator_difference():
     """At the benchmark size (200 x 100 elements, p = 4, 963,603 DOFs): with
     every anchor of a uniform mesh inverted (ELEM_WTS = 1), CADJ_NU_ALPHA's

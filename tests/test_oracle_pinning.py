@@ -684,3 +684,184 @@ def test_cadjlam_rhs_closed_form():
     rhs2, _ = oracle.cadjlam_rhs(ndof, ifreq, isrc, freq, py, az, 2, 3, 0.0, 0.0, 0, True, yrec, mrdof, mp(est),
                                  mp(obs), w)
     assert np.abs(rhs2 - rhs).max() <= 2e-6 * np.abs(rhs).max()
+
+
+# -- CADJLAM's data covariance (LCOVD) ---------------------------------------------
+
+def _covd_obs(rng, nrec, pz, dead=()):
+    raw = _cvals(rng, (nrec, 3))
+    raw[rng.random((nrec, 3)) < pz] = 0
+    for c in dead:
+        raw[:, c] = 0
+    return raw
+
+
+@needs_ref
+def test_rec_space1d_vs_reference():
+    """REC_SPACE1D: ascending and descending lines, a duplicate, a reversal
+    (at receivers 2..nrec-1, the pairs the reference checks, and at the last
+    pair, which it does not), one receiver."""
+    rng = np.random.default_rng(201)
+    x = np.cumsum(rng.uniform(0.5, 3.0, 15))
+    cases = [x, x[::-1].copy(), np.r_[x[:5], x[4], x[5:]], np.r_[x[:6], x[3], x[6:]],
+             np.r_[x[:-1], x[-3]], x[:1], x[:2]]
+    for xr in cases:
+        a, ea = oracle.rec_space1d(xr)
+        b, eb = oracle.rec_space1d(xr, reference=True)
+        assert ea == eb, xr
+        np.testing.assert_array_equal(a, b)
+        if ea == 0:
+            d = np.abs(np.diff(xr))
+            np.testing.assert_allclose(a, d / d.min(), rtol=0, atol=0)
+
+
+@needs_ref
+@pytest.mark.parametrize("nrec,pz,ldwght,irestp,dtmax,dead", [
+    (12, 0.0, 0, 3, 0.0, ()), (12, 0.3, 1, 3, 0.0, ()), (40, 0.3, 1, 1, 0.0, ()),
+    (40, 0.5, 1, 2, 0.5, ()), (40, 0.2, 0, 3, 2.0, ()), (7, 0.2, 1, 3, 0.0, (2,)),
+    (7, 0.2, 1, 3, 0.0, (0, 1)), (5, 0.0, 1, 3, 0.0, (0, 1, 2)), (60, 0.9, 1, 3, 0.0, ())])
+def test_covd_resid_vs_reference(nrec, pz, ldwght, irestp, dtmax, dead):
+    """LAPLACE_1DV2, PERM_OBS and BPRESID_COVD4 against the compiled
+    routines: the roughening matrix and the permutation bit for bit; the
+    residual to 1e-6 of its largest value (the oracle's SGEMV adds column by
+    column as the reference BLAS does; the linked BLAS orders and fuses its
+    single-precision sums its own way).  Cases: every receiver observed,
+    missing observations in every component (the first component taken,
+    the vertical, included), a dead component, two dead components, a dead
+    frequency, nine of ten receivers unobserved."""
+    rng = np.random.default_rng(202 + nrec + 7 * ldwght + irestp)
+    x = np.cumsum(rng.uniform(1.0, 3.0, nrec))
+    dx, _ = oracle.rec_space1d(x)
+    raw = _covd_obs(rng, nrec, pz, dead)
+    obs = (raw * np.complex64(np.exp(0.3j))).astype(np.complex64)
+    est = _cvals(rng, (nrec, 3))
+    w = rng.uniform(0.5, 2.0, (nrec, 3)).astype(np.float32)
+    a = oracle.covd_resid(nrec, irestp, ldwght, dtmax, 0.25, dx, raw, obs, est, w)
+    b = oracle.covd_resid(nrec, irestp, ldwght, dtmax, 0.25, dx, raw, obs, est, w, reference=True)
+    assert a[3] == b[3] == 0
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[2], b[2])
+    scale = np.abs(b[0]).max()
+    assert np.abs(a[0] - b[0]).max() <= 1e-6 * max(scale, 1e-30)
+    if not raw.any():
+        assert not b[0].any() and not b[1].any()
+
+
+def test_covd_closed_form_chain_laplacian():
+    """With the vertical and second components observed at every receiver
+    (so that the roughening matrix's numbering and PERM_OBS's positions
+    agree) and unit weights, each component's residual is the 1-D finite
+    element Laplacian of the data residual over its observed receivers:
+    r_k = (d_k - d_k-1)/h_k-1 + (d_k - d_k+1)/h_k, h the distance to the
+    next observed receiver divided by the smallest spacing (1 without
+    ldwght), and zero at unobserved receivers; rows sum to zero (a constant
+    residual along a component gives none) and <d, r> = sum (d_k+1 - d_k)^2
+    / h_k."""
+    rng = np.random.default_rng(203)
+    nrec = 25
+    x = np.cumsum(rng.uniform(1.0, 4.0, nrec))
+    dx, _ = oracle.rec_space1d(x)
+    raw = _cvals(rng, (nrec, 3))
+    raw[[2, 3, 9, 17, 24], 0] = 0            # gaps in the last component taken
+    est = _cvals(rng, (nrec, 3))
+    w = np.ones((nrec, 3), np.float32)
+    for ldwght in (0, 1):
+        r, _, _, ierr = oracle.covd_resid(nrec, 3, ldwght, 0.0, 0.2, dx, raw, raw, est, w)
+        assert ierr == 0
+        d = (raw - est).astype(np.complex128)
+        for c in range(3):
+            act = np.flatnonzero(raw[:, c] != 0)
+            h = np.diff(x[act]) / np.diff(x).min() if ldwght else np.ones(act.size - 1)
+            want = np.zeros(nrec, np.complex128)
+            dd = d[act, c]
+            want[act[:-1]] += (dd[:-1] - dd[1:]) / h
+            want[act[1:]] += (dd[1:] - dd[:-1]) / h
+            assert np.abs(r[:, c] - want).max() <= 2e-6 * np.abs(want).max(), (ldwght, c)
+            energy = np.vdot(dd, r[act, c].astype(np.complex128)).real
+            assert abs(energy - np.sum(np.abs(np.diff(dd)) ** 2 / h)) <= 1e-5 * energy
+    const = np.ones((nrec, 3), np.complex64) * np.complex64(1 + 2j)
+    r, _, _, _ = oracle.covd_resid(nrec, 3, 1, 0.0, 0.2, dx, const, const, np.zeros_like(const), w)
+    assert np.abs(r).max() <= 1e-5
+
+
+@needs_ref
+def test_cadjlam_rhs_covd_composition():
+    """CADJLAM with LCOVD: with no rotation, no receiver shift and no
+    HDIFFER factor, the right-hand side is the conjugate of the compiled
+    BPRESID_COVD4 residual (on the compiled LAPLACE_1DV2 and PERM_OBS)
+    placed at the receiver DOFs; a rotation by az with equal horizontal
+    weights and no horizontal gaps commutes with the component-wise
+    roughening, so it equals rotating the data instead."""
+    rng = np.random.default_rng(204)
+    ndof, nfreq, nrec, nsrc = 200, 2, 30, 2
+    mrdof = np.arange(1, 3 * nrec + 1, dtype=np.int32).reshape(nrec, 3) * 2
+    yrec = rng.uniform(-2e4, 2e4, nrec)
+    x = np.cumsum(rng.uniform(1.0, 2.0, nrec))
+    dx, _ = oracle.rec_space1d(x)
+    est = _cvals(rng, (nsrc, nrec, nfreq, 3))
+    obs = _cvals(rng, (nsrc, nrec, nfreq, 3))
+    obs[1, ::4, 0, 2] = 0
+    w = rng.uniform(0.5, 2.0, (nsrc, nrec, nfreq, 3)).astype(np.float32)
+    ifreq, isrc = 1, 2
+    rhs, iw, ierr = oracle.cadjlam_rhs_covd(ndof, ifreq, isrc, 0.07, 0.0, 0.0, 3, 0.0, 0, False, True, dx, yrec,
+                                            mrdof, est, obs, w)
+    assert iw == 0 and ierr == 0
+    o, e, ww = obs[isrc - 1, :, ifreq - 1], est[isrc - 1, :, ifreq - 1], w[isrc - 1, :, ifreq - 1]
+    r, _, _, _ = oracle.covd_resid(nrec, 3, 1, 0.0, 0.07, dx, o, o, e, ww, reference=True)
+    want = np.zeros(ndof, np.complex64)
+    want[mrdof.ravel() - 1] = np.conj(r.ravel())
+    assert np.abs(rhs - want).max() <= 1e-6 * np.abs(want).max()
+    # rotation: equal horizontal weights, every component observed everywhere
+    # (a gap in an earlier component shifts the roughening matrix's
+    # numbering against PERM_OBS's positions, as in the reference)
+    wr = w.copy()
+    wr[..., 1] = wr[..., 0]
+    full = _cvals(rng, obs.shape)
+    az = 33.0
+    rot, _, _ = oracle.cadjlam_rhs_covd(ndof, ifreq, isrc, 0.07, 0.0, az, 3, 0.0, 0, False, True, dx, yrec, mrdof,
+                                        est, full, wr)
+    th = np.deg2rad(az)
+    o = full[isrc - 1, :, ifreq - 1]
+    o2 = o.astype(np.complex128)
+    ob = np.stack([np.cos(th) * o2[:, 0] + np.sin(th) * o2[:, 1], -np.sin(th) * o2[:, 0] + np.cos(th) * o2[:, 1],
+                   o2[:, 2]], axis=1).astype(np.complex64)
+    rr, _, _, _ = oracle.covd_resid(nrec, 3, 1, 0.0, 0.07, dx, o, ob, e, wr[isrc - 1, :, ifreq - 1])
+    # R(-az) C W (OBS - R(az) EST) = C W (R(-az) OBS - EST): no rotation left over
+    want = np.zeros(ndof, np.complex64)
+    want[mrdof.ravel() - 1] = np.conj(rr.ravel())
+    assert np.abs(rot - want).max() <= 5e-6 * np.abs(want).max()
+
+
+def test_covd_entry_point_argument_checks():
+    """The data-covariance entry point refuses non-positive or non-finite
+    receiver spacings (LAPLACE_1DV2 divides by their sums) before any device
+    work; REC_SPACE1D's host form reports its three errors."""
+    import ctypes as C
+    from fem25_amd import _lib
+    L = _lib.load()
+    nrec, ndof = 4, 30
+    ip, dp, fp = _lib.ip, _lib.dp, _lib.fp
+    one = np.ones(1, np.int32)
+    fq, py = np.full(1, 0.1), np.zeros(1)
+    yrec = np.zeros(nrec)
+    mrdof = np.arange(1, 3 * nrec + 1, dtype=np.int32)
+    est = np.zeros(3 * nrec, np.complex64)
+    w = np.ones(3 * nrec, np.float32)
+    rhs = np.zeros(ndof, np.complex64)
+    iw = C.c_int(0)
+    p = lambda a, t: a.ctypes.data_as(t)   # noqa: E731
+    for dx in (np.array([1.0, 0.0, 2.0]), np.array([1.0, -1.0, 2.0]), np.array([1.0, np.inf, 2.0])):
+        rc = L.f25_cadjlam_rhs_covd(ndof, 1, nrec, 1, 1, p(one, ip), p(one, ip), p(fq, dp), p(py, dp), 0.0, 3,
+                                    C.c_float(0.0), 0, 0, 1, p(dx, dp), p(yrec, dp), p(mrdof, ip),
+                                    est.ctypes.data_as(C.c_void_p), est.ctypes.data_as(C.c_void_p), p(w, fp),
+                                    rhs.ctypes.data_as(C.c_void_p), C.byref(iw))
+        assert rc == _lib.F25_ERR_ARG
+        assert b"spacings" in L.f25_last_error()
+    out = np.zeros(3)
+    x = np.array([0.0, 1.0, 3.0, 6.0])
+    assert L.f25_rec_space1d(4, p(x, dp), p(out, dp)) == 0
+    np.testing.assert_array_equal(out, [1.0, 2.0, 3.0])
+    for bad in (np.array([0.0, 1.0, 1.0, 2.0]), np.array([0.0, 1.0, 0.5, 2.0])):
+        assert L.f25_rec_space1d(4, p(bad, dp), p(out, dp)) == _lib.F25_ERR_ARG
+        assert not out.any()
+    assert L.f25_rec_space1d(1, p(x, dp), p(out, dp)) == _lib.F25_ERR_ARG
