@@ -1353,18 +1353,35 @@ static int inv_upload(f25_mesh *m) {
     return 0;
 }
 
+// CINVTYPE: PP (Vp), SS (Vs) with NVINV = 1, PS (both, NVINV = 2), as the
+// driver maps them (gn25.f90:170-183)
+static int inv_kind(const char *cinvtype, bool &lisp, bool &liss) {
+    const char c0 = (char)toupper(cinvtype[0]), c1 = (char)toupper(cinvtype[1]);
+    lisp = (c0 == 'P' && c1 == 'P') || (c0 == 'P' && c1 == 'S');
+    liss = (c0 == 'S' && c1 == 'S') || (c0 == 'P' && c1 == 'S');
+    if (!lisp && !liss) FAIL(F25_ERR_INVTYPE, "CINVTYPE must be PP, SS or PS (gn25.f90:170-183)");
+    return 0;
+}
+
+// the variable of local column c: Vp for PP and for the first column of
+// each PS pair (MYGRAD = 2 (INPINV - 1) + 1), Vs otherwise
+static bool col_is_p(const InvInfo &inv, int c) {
+    if (inv.nvinv == 2) return (inv.mygrad[c] - 1) % 2 == 0;
+    return inv.lisp;
+}
+
 extern "C" int f25_inv_setup(f25_mesh *m, const char *cinvtype, double vpvs, int linrec, int lkbdry,
                              int nrec, const int *mrdof1, int sizes[6]) {
     if (!m || !cinvtype) FAIL(F25_ERR_ARG, "null argument");
-    char c0 = (char)toupper(cinvtype[0]), c1 = (char)toupper(cinvtype[1]);
-    bool lisp = c0 == 'P' && c1 == 'P', liss = c0 == 'S' && c1 == 'S';
-    if (!lisp && !liss) FAIL(F25_ERR_INVTYPE, "CINVTYPE must be PP or SS (vforce25.f:66-85)");
+    bool lisp, liss;
+    TRY(inv_kind(cinvtype, lisp, liss));
     if (!linrec && nrec > 0 && !mrdof1) FAIL(F25_ERR_ARG, "receiver DOFs missing");
     HIPCHK(hipSetDevice(m->dev));
     InvInfo inv;
-    int r = build_inv(m->d, m->g, 1, linrec, lkbdry, nrec, mrdof1, inv);
+    int r = build_inv(m->d, m->g, lisp && liss ? 2 : 1, linrec, lkbdry, nrec, mrdof1, inv);
     if (r) FAIL(r, "inversion setup failed (no inversion points)");
     inv.lisp = lisp;
+    inv.liss = liss;
     inv.vpvs = vpvs;
     m->inv = std::move(inv);
     if (sizes) {
@@ -1380,20 +1397,20 @@ extern "C" int f25_inv_attach(f25_mesh *m, const char *cinvtype, double vpvs, in
                               const double *elem_wts) {
     if (!m || !cinvtype || !mconn || !maskg || !mygrad || !jcsc || !icsc || !elem_wts)
         FAIL(F25_ERR_ARG, "null argument");
-    char c0 = (char)toupper(cinvtype[0]), c1 = (char)toupper(cinvtype[1]);
-    bool lisp = c0 == 'P' && c1 == 'P', liss = c0 == 'S' && c1 == 'S';
-    if (!lisp && !liss) FAIL(F25_ERR_INVTYPE, "CINVTYPE must be PP or SS (vforce25.f:66-85)");
-    if (nvinv != 1) FAIL(F25_ERR_INVTYPE, "NVINV must be 1 for PP/SS inversions");
+    bool lisp, liss;
+    TRY(inv_kind(cinvtype, lisp, liss));
+    if (nvinv != (lisp && liss ? 2 : 1)) FAIL(F25_ERR_INVTYPE, "NVINV must be 1 for PP/SS and 2 for PS");
     const int nnpg = m->d.nnpg, nelem = m->d.nelem, ndof = m->g.ndof;
     if (ncon <= 0 || nnpgl <= 0 || nz_fdist < 0 || na35 <= 0) FAIL(F25_ERR_ARG, "bad sizes");
     if (jcsc[0] != 1 || jcsc[nnpgl] - 1 != nz_fdist) FAIL(F25_ERR_ARG, "JCSC_FDIST inconsistent with NZ_FDIST");
     InvInfo inv;
-    inv.nvinv = 1;
+    inv.nvinv = nvinv;
     inv.ncon = ncon;
     inv.nnpgl = nnpgl;
     inv.nz_fdist = nz_fdist;
     inv.na35 = na35;
     inv.lisp = lisp;
+    inv.liss = liss;
     inv.vpvs = vpvs;
     inv.mconn.assign(mconn, mconn + (size_t)ncon * nnpg);
     inv.maskg.assign(maskg, maskg + nnpg);
@@ -1401,18 +1418,21 @@ extern "C" int f25_inv_attach(f25_mesh *m, const char *cinvtype, double vpvs, in
     inv.jcsc.assign(jcsc, jcsc + nnpgl + 1);
     inv.icsc.assign(icsc, icsc + nz_fdist);
     inv.elem_wts.assign(elem_wts, elem_wts + nelem);
-    // anchor of each column: MASKG(INPG) == MYGRAD(INPGL) (hessloc.f90:66-72, NVINV = 1)
-    std::vector<int> anchor_of(std::max(na35, 1) + 1, -1);
+    // anchor of each column: (MASKG(INPG) - 1) NVINV + IVINV == MYGRAD(INPGL)
+    // (hessloc.f90:66-72)
+    const int ninv = na35 / nvinv;
+    std::vector<int> anchor_of(std::max(ninv, 1) + 1, -1);
     for (int a = 0; a < nnpg; ++a) {
         const int v = maskg[a];
-        if (v > 0 && v <= na35 && anchor_of[v] < 0) anchor_of[v] = a;
+        if (v > 0 && v <= ninv && anchor_of[v] < 0) anchor_of[v] = a;
         if (v > 0) ++inv.nnpinv;
     }
     inv.node_of_col.assign(nnpgl, 0);
     for (int c = 0; c < nnpgl; ++c) {
         const int v = mygrad[c];
-        if (v <= 0 || v > na35 || anchor_of[v] < 0) FAIL(F25_ERR_ARG, "cannot find the anchor of a column");
-        inv.node_of_col[c] = anchor_of[v];
+        if (v <= 0 || v > na35 || anchor_of[(v - 1) / nvinv + 1] < 0)
+            FAIL(F25_ERR_ARG, "cannot find the anchor of a column");
+        inv.node_of_col[c] = anchor_of[(v - 1) / nvinv + 1];
         const int len = jcsc[c + 1] - jcsc[c];
         if (len < 0) FAIL(F25_ERR_ARG, "JCSC_FDIST not ascending");
         inv.mbufrhs = std::max(inv.mbufrhs, len);
@@ -1729,8 +1749,8 @@ static void dab_constants(const f25_mesh *m, std::vector<double> &da) {
     da.resize(inv.nnpgl);
     for (int c = 0; c < inv.nnpgl; ++c) {
         const int a = inv.node_of_col[c];
-        da[c] = inv.lisp ? 2.0 * std::sqrt((m->lam[a] + 2.0 * m->mu[a]) / m->rho[a])
-                         : 2.0 * std::sqrt(m->mu[a] / m->rho[a]);
+        da[c] = col_is_p(inv, c) ? 2.0 * std::sqrt((m->lam[a] + 2.0 * m->mu[a]) / m->rho[a])
+                                 : 2.0 * std::sqrt(m->mu[a] / m->rho[a]);
     }
 }
 
@@ -1744,6 +1764,11 @@ static int ensure_dab_part(f25_mesh *m) {
     std::vector<int> col4((size_t)nelem * 4, -1);
     for (int c = 0; c < inv.nnpgl; ++c) {
         const int a = inv.node_of_col[c];
+        if (inv.nvinv == 2) {   // PS: the P column stands for its pair (P, S)
+            if ((inv.mygrad[c] - 1) % 2) continue;
+            if (c + 1 >= inv.nnpgl || inv.node_of_col[c + 1] != a || inv.mygrad[c + 1] != inv.mygrad[c] + 1)
+                FAIL(F25_ERR_ARG, "PS columns must come in (P, S) pairs (gengmask.f90:1078-1098)");
+        }
         for (int k = 0; k < inv.ncon; ++k) {
             const int v = inv.mconn[(size_t)k * nnpg + a];
             if (v <= 0) break;
@@ -1809,7 +1834,7 @@ static DabKernel dab_kernel(int nl, bool imag = true) { return imag ? dab_kernel
 static int dab_prepare(f25_mesh *m) {
     if (!m->have_inv) FAIL(F25_ERR_STATE, "f25_inv_setup has not been called");
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
-    if (m->inv.nvinv != 1) FAIL(F25_ERR_ARG, "NVINV = 1 (PP or SS) only");
+    if (m->inv.nvinv != (m->inv.lisp && m->inv.liss ? 2 : 1)) FAIL(F25_ERR_ARG, "NVINV inconsistent with CINVTYPE");
     if (dab_lds(m) > 160 * 1024) FAIL(F25_ERR_ARG, "element order too high for the LDS tiling");
     HIPCHK(hipSetDevice(m->dev));
     if (!dab_kernel(m->tab.nl)) FAIL(F25_ERR_ARG, "unsupported order");
@@ -1827,14 +1852,23 @@ static int dab_column_items(const f25_mesh *m, const std::vector<double> &da, co
     std::vector<int> ce, ck;
     TRY(last_elements(m, ce, ck));
     elem = ce;
-    coef.assign(4 * (size_t)inv.nnpgl, 0.0);
+    coef.assign(8 * (size_t)inv.nnpgl, 0.0);
     for (int c = 0; c < inv.nnpgl; ++c) {
         if (ck[c] < 0) continue;
         float w = (float)inv.elem_wts[ce[c]];
         if (v) w *= v[inv.mygrad[c] - 1];   // CMPLX(SNGL(ELEM_WTS) * V(IA35)), qpmfree.f90:866
-        coef[4 * c + ck[c]] = (double)w * da[c];
+        const double cv = (double)w * da[c];
+        // one field (PP, SS: the same coefficients for both); PS: the column's own
+        if (inv.nvinv == 1 || col_is_p(inv, c)) coef[8 * c + ck[c]] = cv;
+        if (inv.nvinv == 1 || !col_is_p(inv, c)) coef[8 * c + 4 + ck[c]] = cv;
     }
     return 0;
+}
+
+// kp, ks of k_dab_sf for DABMKISOB25's forms: P alone, S alone, or both
+static void dab_kpks(const InvInfo &inv, double &kp, double &ks) {
+    kp = inv.lisp ? 1.0 : 0.0;
+    ks = inv.liss ? 1.0 : 0.0;
 }
 
 static int dab_launch_columns(f25_mesh *m, double kv, int nvec, const std::vector<int> &elem,
@@ -1846,8 +1880,10 @@ static int dab_launch_columns(f25_mesh *m, double kv, int nvec, const std::vecto
     int r = upload(&de, elem.data(), elem.size(), m->st);
     if (!r) r = upload(&dc, coef.data(), coef.size(), m->st);
     if (!r) {
+        double kp, ks;
+        dab_kpks(inv, kp, ks);
         hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(inv.nnpgl), dim3(dab_threads(m)), dab_lds(m), m->st, 1,
-                           inv.nnpgl, nvec, inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de, dc, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,
+                           inv.nnpgl, nvec, kp, ks, kv, de, dc, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,
                            m->d_xl, m->d_zl, m->d_lm, x0, x1, (double2 *)nullptr, (double2 *)nullptr, z0, z1,
                            m->d_mygrad, m->d_jcsc, m->d_icsc, p0, p1, (float2 *)nullptr, m->d_err);
         hipError_t e = hipGetLastError();
@@ -1870,19 +1906,21 @@ static int dab_download(f25_mesh *m, float *dst, const float *src, size_t n) {
 // (vforce25.f:164-179), DVP or DVS = 2 Vp or 2 Vs N_k (DJACAB).
 extern "C" int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *wave, void *fmat) {
     if (!m || !wave || !fmat) FAIL(F25_ERR_ARG, "null argument");
+    if (m->have_inv && m->inv.nvinv == 2)
+        FAIL(F25_ERR_INVTYPE, "VFORCE25 has no joint P-S derivatives (vforce25.f:81-85)");
     TRY(dab_prepare(m));
     const InvInfo &inv = m->inv;
     const int n = inv.nnpgl;
     std::vector<int> ce, ck;
     TRY(last_elements(m, ce, ck));
-    std::vector<double> coef(4 * (size_t)n, 0.0);
+    std::vector<double> coef(8 * (size_t)n, 0.0);
     const double v2 = inv.vpvs * inv.vpvs;
     for (int c = 0; c < n; ++c) {
         if (ck[c] < 0) continue;   // the element does not hold the anchor: zero column
         const int a = inv.node_of_col[c];
         const double vel = inv.lisp ? std::sqrt((m->lam[a] + 2.0 * m->mu[a]) / m->rho[a])
                                     : std::sqrt(m->mu[a] / m->rho[a]);
-        coef[4 * c + ck[c]] = inv.elem_wts[ce[c]] * 2.0 * vel;
+        coef[8 * c + ck[c]] = coef[8 * c + 4 + ck[c]] = inv.elem_wts[ce[c]] * 2.0 * vel;
     }
     const double kp = inv.lisp ? 1.0 : v2, ks = inv.lisp ? 1.0 / v2 : 1.0;
     const double kv = 6.2831853071795862 * freq * py;
@@ -1928,14 +1966,21 @@ extern "C" int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float 
     std::vector<double> da;
     dab_constants(m, da);
     // element items: coefficient of every corner that holds a local column
+    // (PS: the corner's P column c and S column c + 1, each with its own V)
     const size_t ne = inv.dab_elem.size();
-    std::vector<double> ecoef(4 * std::max<size_t>(ne, 1), 0.0);
+    std::vector<double> ecoef(8 * std::max<size_t>(ne, 1), 0.0);
     for (size_t i = 0; i < ne; ++i)
         for (int k = 0; k < 4; ++k) {
             const int c = inv.dab_col4[4 * i + k];
             if (c < 0) continue;
-            const float w = (float)inv.elem_wts[inv.dab_elem[i]] * v[inv.mygrad[c] - 1];
-            ecoef[4 * i + k] = (double)w * da[c];
+            const float ew = (float)inv.elem_wts[inv.dab_elem[i]];
+            const double cp = (double)(ew * v[inv.mygrad[c] - 1]) * da[c];
+            if (inv.nvinv == 2) {
+                ecoef[8 * i + k] = cp;
+                ecoef[8 * i + 4 + k] = (double)(ew * v[inv.mygrad[c + 1] - 1]) * da[c + 1];
+            } else {
+                ecoef[8 * i + k] = ecoef[8 * i + 4 + k] = cp;
+            }
         }
     std::vector<int> celem;
     std::vector<double> ccoef;
@@ -1950,18 +1995,20 @@ extern "C" int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float 
     if (!r) r = dalloc(&ya, ndof);
     if (!r) r = dalloc(&yb, ndof);
     if (!r) r = upload(&de, inv.dab_elem.data(), ne, m->st);
-    if (!r) r = upload(&dc, ecoef.data(), 4 * ne, m->st);
+    if (!r) r = upload(&dc, ecoef.data(), 8 * ne, m->st);
     if (!r) r = dalloc(&dp3, inv.na35);
     if (!r) {
         fill_async(ya, (size_t)ndof, make_double2(0.0, 0.0), m->st);
         fill_async(yb, (size_t)ndof, make_double2(0.0, 0.0), m->st);
         fill_async(dp3, (size_t)inv.na35, 0.f, m->st);
         (void)hipEventRecord(m->ev0, m->st);
+        double kp, ks;
+        dab_kpks(inv, kp, ks);
         for (size_t col = 0; col + 1 < inv.dab_cptr.size(); ++col) {
             const int o = inv.dab_cptr[col], n = inv.dab_cptr[col + 1] - o;
             if (n <= 0) continue;
             hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(n), dim3(dab_threads(m)), dab_lds(m), m->st, 0, n, 2,
-                               inv.lisp ? 1.0 : 0.0, inv.lisp ? 0.0 : 1.0, kv, de + o, dc + 4 * (size_t)o,
+                               kp, ks, kv, de + o, dc + 8 * (size_t)o,
                                m->d_t1d, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, du, dl, ya, yb,
                                (const float2 *)nullptr, (const float2 *)nullptr, m->d_mygrad, m->d_jcsc,
                                m->d_icsc, (float *)nullptr, (float *)nullptr, (float2 *)nullptr, m->d_err);

@@ -40,7 +40,7 @@ int build_csr(const f25_mesh_desc &d, const Tables &t, Graph &g);         // IRP
 // Inversion bookkeeping (GENGMASK, GRADPTRS, ELEM_WTS, one rank).
 struct InvInfo {
     int nvinv = 1, nnpinv = 0, na35 = 0, ncon = 0, nnpgl = 0, nz_fdist = 0, mbufrhs = 0;
-    bool lisp = true;
+    bool lisp = true, liss = false;   // PP: P; SS: S; PS: both (NVINV = 2)
     double vpvs = 1.7320508075688772;
     std::vector<int> mconn;      // (ncon, nnpg): mconn[c*nnpg + g], 1-based elements, 0 none
     std::vector<int> maskg;      // nnpg

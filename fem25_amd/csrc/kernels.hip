@@ -715,7 +715,8 @@ __device__ __forceinline__ void vstep_load(double2 *acc, double &g, const double
 // that every load and store of a tile is a uniform row base plus the lane's
 // pair offset and none is predicated per row.
 template <int NL, int NRT, int C>
-__device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, const int *__restrict__ lme, int a0,
+__device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, const int *__restrict__ lme,
+                                             const int *__restrict__ lmo, int a0,
                                              unsigned long long fmask, int npair, int pair, bool live,
                                              double om, double kv, const double2 *__restrict__ x,
                                              double2 *__restrict__ y, const double2 *__restrict__ zpad,
@@ -732,7 +733,7 @@ __device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, cons
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
         const int a = a0 + r;
-        const int dof = a < NEN ? lme[a * 3 + C] : -1;
+        const int dof = a < NEN ? lmo[a * 3 + C] : -1;
         const bool add = dof >= 0 && !((fmask >> a) & 1ull);
         const double2 *row = add ? y + (size_t)dof * npair : zpad;
         acc[r] = row[pcl];
@@ -804,7 +805,7 @@ __device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, cons
 #pragma unroll
         for (int r = 0; r < RT; ++r) {
             const int a = a0 + r;
-            const int dof = a < NEN ? lme[a * 3 + C] : -1;
+            const int dof = a < NEN ? lmo[a * 3 + C] : -1;
             double2 *row = dof >= 0 ? y + (size_t)dof * npair : sink;
             row[pair] = acc[r];
         }
@@ -824,7 +825,8 @@ __device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, cons
 template <int NL, int NRT>
 __global__ __launch_bounds__((64 * 3 * NRT), (VShape<NL, NRT>::MINW)) void k_apply_v(
         const int *__restrict__ order, int nel, const double *__restrict__ vcf, const int *__restrict__ lm,
-        const unsigned long long *__restrict__ first, int npair, const double *__restrict__ omega,
+        const int *__restrict__ lm_out, const unsigned long long *__restrict__ first, int npair,
+        const double *__restrict__ omega,
         const double *__restrict__ ky, const double2 *__restrict__ x, double2 *__restrict__ y,
         const double2 *__restrict__ zpad, double2 *__restrict__ sink) {
     typedef VShape<NL, NRT> V;
@@ -839,16 +841,16 @@ __global__ __launch_bounds__((64 * 3 * NRT), (VShape<NL, NRT>::MINW)) void k_app
     const int ct = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     const unsigned long long fmask = first[e];
-    const int *lme = lm + (size_t)e * 3 * NEN;
+    const int *lme = lm + (size_t)e * 3 * NEN, *lmo = lm_out + (size_t)e * 3 * NEN;
     const int c = ct / NRT, t = ct % NRT;
     const int pair = g * 64 + lane;
     const bool live = pair < npair;
     const double om = omega[live ? pair : 0], kv = ky[live ? pair : 0];
     const double *cf = vcf + ((size_t)e * 3 * NRT + ct) * TS;
     const int a0 = t * V::RT;
-    if (c == 0) apply_v_tile<NL, NRT, 0>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
-    else if (c == 1) apply_v_tile<NL, NRT, 1>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
-    else apply_v_tile<NL, NRT, 2>(cf, lme, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
+    if (c == 0) apply_v_tile<NL, NRT, 0>(cf, lme, lmo, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
+    else if (c == 1) apply_v_tile<NL, NRT, 1>(cf, lme, lmo, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
+    else apply_v_tile<NL, NRT, 2>(cf, lme, lmo, a0, fmask, npair, pair, live, om, kv, x, y, zpad, sink);
 }
 
 // Residual of the batched product: r = f - r (r holds S u on entry) for
@@ -2142,10 +2144,12 @@ __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *
 //    vforce25.f): !IMAG, ky coupling written as real numbers.
 // Both are the elastic element operator without the -w^2 rho mass term,
 // with density-normalised moduli at the quadrature points
-//   lambda + 2 mu = P = kp D,  mu = S = ks D,  lambda = Q = P - 2 S,
-//   D(q) = w(q) sum_k c_k N_k(q)
-// (DABMKISOB25: kp, ks = 1, 0 for Vp, 0, 1 for Vs; DKM_VPVS: the VPVS
-// slaving of vforce25.f:164-179), c_k the item's four corner coefficients.
+//   lambda + 2 mu = P = kp D,  mu = S = ks Ds,  lambda = Q = P - 2 S,
+//   D(q) = w(q) sum_k c_k N_k(q),  Ds(q) = w(q) sum_k s_k N_k(q)
+// (DABMKISOB25: kp, ks = 1, 0 for Vp, 0, 1 for Vs, with s_k = c_k, and
+// 1, 1 for the joint P-S inversion, whose P and S variables carry their own
+// corner coefficients c_k and s_k; DKM_VPVS: the VPVS slaving of
+// vforce25.f:164-179, s_k = c_k), eight coefficients per item.
 // The operator M is applied to nvec <= 2 vectors through fluxes at the
 // quadrature points:
 //   mode 0 (item = element; one launch per colour, so rows never collide):
@@ -2254,8 +2258,9 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
     const int it = blockIdx.x, tid = threadIdx.x;
     if (it >= nitems) return;
     const int e = item_elem[it];
-    const double c0 = item_coef[4 * it], c1 = item_coef[4 * it + 1], c2 = item_coef[4 * it + 2],
-                 c3 = item_coef[4 * it + 3];
+    // corner coefficients of the P field (0..3) and of the S field (4..7)
+    const double *ic = item_coef + 8 * (size_t)it;
+    const double c0 = ic[0], c1 = ic[1], c2 = ic[2], c3 = ic[3], s0 = ic[4], s1 = ic[5], s2 = ic[6], s3 = ic[7];
     const int *ie = ieng + 4 * e;
     const int *L = lm + (size_t)e * nee;
     const double *v1 = t1d, *d1 = t1d + N1 * Q1;   // v1[k * Q1 + q]
@@ -2278,12 +2283,13 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
     // the inverse Jacobian and the collocated density D at this lane's point,
     // in registers (no LDS table: a smaller workgroup footprint keeps more
     // workgroups per CU)
-    double rx, sx, rz, sz, D;
+    double rx, sx, rz, sz, D, Ds;
     {
         Geo g;
         if (geo_at(xq[qx], xq[qe], ie, xl, zl, g)) *err = 2;
         rx = g.dzdeta / g.det; sx = -g.dzdxi / g.det; rz = -g.dxdeta / g.det; sz = g.dxdxi / g.det;
         D = wq[qx] * wq[qe] * g.det * (c0 * g.sf[0] + c1 * g.sf[1] + c2 * g.sf[2] + c3 * g.sf[3]);
+        Ds = wq[qx] * wq[qe] * g.det * (s0 * g.sf[0] + s1 * g.sf[1] + s2 * g.sf[2] + s3 * g.sf[3]);
     }
     const int ni = min(x16, N1 - 1);
     double vx[N1], dx[N1];
@@ -2311,7 +2317,7 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
         }
         __syncthreads();
         {
-            const double P = kp * D, S = ks * D, Q = P - 2.0 * S;
+            const double P = kp * D, S = ks * Ds, Q = P - 2.0 * S;
             double2 U[3], UX[3], UZ[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {

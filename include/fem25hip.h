@@ -191,8 +191,13 @@ int f25_gradient32_dev(f25_mesh *m, int ntrip, const double *omega, const double
 /* ---- reference virtual-force path (CFMAT_DIST / ADJFLAM_DIST) ------------ */
 
 /* GENGMASK + ELEM_WTS + GRADPTRS for one rank (gengmask.f90:1-262,
- * 758-841, 962-1198), no mask file.  cinvtype "PP" or "SS"; mrdof1 holds
- * MRDOF(1,1:NREC) (used only when linrec == 0).  sizes: [0] NNPINV,
+ * 758-841, 962-1198), no mask file.  cinvtype "PP" or "SS" (NVINV = 1) or
+ * "PS" (NVINV = 2: two columns per anchor, P then S, with the same rows and
+ * MYGRAD = 2 (INPINV - 1) + IVINV, gn25.f90:170-183); mrdof1 holds
+ * MRDOF(1,1:NREC) (used only when linrec == 0).  ICSC_FDIST is sized
+ * exactly: the reference's GRADPTRS bounds it by NNPG*NCON*NEE and stops
+ * ('ICSC_FDIST too small', gengmask.f90:1069-1073) when a PS inversion
+ * covers more than about 60% of the anchors.  sizes: [0] NNPINV,
  * [1] NCON, [2] NNPGL, [3] NZ_FDIST, [4] MBUFRHS, [5] NA35. */
 int f25_inv_setup(f25_mesh *m, const char *cinvtype, double vpvs, int linrec, int lkbdry,
                   int nrec, const int *mrdof1, int sizes[6]);
@@ -204,13 +209,15 @@ int f25_inv_get(f25_mesh *m, int *maskg, int *mygrad, int *jcsc, int *icsc, doub
  * file built them), for the drop-in CFMAT_DIST: MCONN(NNPG,NCON),
  * MASKG(NNPG), MYGRAD(NNPGL), JCSC_FDIST(NNPGL+1), ICSC_FDIST(NZ_FDIST),
  * ELEM_WTS(NELEM), all 1-based.  Column INPGL inverts at the anchor INPG
- * with MASKG(INPG) == MYGRAD(INPGL) (hessloc.f90:66-72).  NVINV must be 1
- * (CINVTYPE PP or SS, vforce25.f:66-85). */
+ * with (MASKG(INPG) - 1) NVINV + IVINV == MYGRAD(INPGL) (hessloc.f90:66-72).
+ * NVINV is 1 for CINVTYPE PP or SS and 2 for PS (F25_ERR_INVTYPE else). */
 int f25_inv_attach(f25_mesh *m, const char *cinvtype, double vpvs, int nvinv, int ncon,
                    const int *mconn, const int *maskg, int nnpgl, const int *mygrad, const int *jcsc,
                    int nz_fdist, const int *icsc, int na35, const double *elem_wts);
 /* CFMAT_DIST (hessloc.f90:1-133 + vforce25.f:1-369): FMAT_DIST(NZ_FDIST)
- * COMPLEX*8 from WAVE(NDOF) COMPLEX*8, reference semantics. */
+ * COMPLEX*8 from WAVE(NDOF) COMPLEX*8, reference semantics.  PS returns
+ * F25_ERR_INVTYPE, as VFORCE25 does (vforce25.f:81-85: no joint P and S
+ * derivatives). */
 int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *wave, void *fmat);
 /* ADJFLAM_DIST (adjgrad.f90:88-145, one rank): GPROD(NA35) REAL*4. */
 int f25_adjflam_dist(f25_mesh *m, const void *fmat, const void *adjlam, float *gprod);
@@ -370,10 +377,11 @@ int f25_rec_space1d(int nrec, const double *xrec, double *dx);
 /* ---- second-order products of QPMFREE (truncated-Newton Hessian) ----------- */
 
 /* CADJ_NU_ALPHA's element loops (qpmfree.f90:622-912) on the attached
- * inversion structures (f25_inv_setup / f25_inv_attach; NVINV = 1, PP or SS)
- * and the model: with M_{c,e} = SNGL(ELEM_WTS(e)) V(MYGRAD(c)) dS_e/dm_c, the
- * DABMKISOB25 matrix (dstiff25.f:1-203) of element e collocated on column
- * c's anchor,
+ * inversion structures (f25_inv_setup / f25_inv_attach; PP, SS or the joint
+ * PS with both of an anchor's variables, qpmfree.f90:677-693, 836-851) and
+ * the model: with M_{c,e} = SNGL(ELEM_WTS(e)) V(MYGRAD(c)) dS_e/dm_c, the
+ * DABMKISOB25 matrix (dstiff25.f:1-203, P or S form by column c's
+ * variable) of element e collocated on column c's anchor,
  *   rhsbnu = sum_c sum_{e in MCONN} M_{c,e}^H adjlam     (RHSBNU, :868-869)
  *   rhsalp = sum_c sum_{e in MCONN} M_{c,e} wave         (RHSALP, :870-871)
  *   p3(MYGRAD(c)) = -Re sum_rows conj(M_{c,last} wave) adjlam   (P3BUF)
@@ -382,7 +390,8 @@ int f25_rec_space1d(int nrec, const double *xrec, double *dx);
  * (NA35, zero outside the local columns).  Host arrays; FP64 inside. */
 int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float *v, const void *adjlam,
                     const void *wave, void *rhsbnu, void *rhsalp, float *p3);
-/* MET225_MOD's element loops (qpmfree.f90:1000-1215): with T1, T2 the last
+/* MET225_MOD's element loops (qpmfree.f90:1000-1215; PP, SS or PS as
+ * f25_qp_nu_alpha): with T1, T2 the last
  * connected element's SNGL(ELEM_WTS(e)) dS_e/dm_c applied to wave and alpha,
  *   p1(MYGRAD(c)) = -Re sum_rows conj(T1) nu,  p2 = -Re sum_rows conj(T2) adjlam
  * (PBUF1, PBUF2 before the reductions).  COMPLEX*8 vectors (NDOF), REAL*4

@@ -71,6 +71,8 @@
  *                    weights the residual is the second difference of the
  *                    residual along the line (closed form), rows sum to zero
  *                    and <WORK, RESIDV> = sum (r_k+1 - r_k)^2 / h_k.
+ *   orc_fill_uwave   compiled reference FILL_UWAVE_QMESH; closed form of the
+ *                    plane wave at the GLL nodes.
  *   orc_cadjlam_rhs_covd composition of the above (compiled LAPLACE_1DV2,
  *                    PERM_OBS and BPRESID_COVD4 behind the same frames).
  */
@@ -1454,13 +1456,19 @@ int orc_gradptrs(const orc_asm *a, orc_inv *inv)
         qsort(irow, nn, sizeof(int), cmp_int);
         irow[nn] = 0;
         for (int r = 0; r < nn; r++)
-            if (irow[r + 1] != irow[r]) inv->icsc[inzfl++] = irow[r];
+            if (irow[r + 1] != irow[r]) {
+                if (inzfl >= nwork) { free(irow); return 1; }   /* 'ICSC_FDIST too small' */
+                inv->icsc[inzfl++] = irow[r];
+            }
         inv->mygrad[inpgl] = (inpinv - 1) * inv->nvinv + 1;
         inv->jcsc[inpgl + 1] = inzfl + 1;
         int r1 = inv->jcsc[inpgl], r2 = inv->jcsc[inpgl + 1] - 1;
         if (r2 - r1 + 1 > combig) combig = r2 - r1 + 1;
         for (int iv = 2; iv <= inv->nvinv; iv++) {
-            for (int r = r1; r <= r2; r++) inv->icsc[inzfl++] = inv->icsc[r - 1];
+            for (int r = r1; r <= r2; r++) {
+                if (inzfl >= nwork) { free(irow); return 1; }
+                inv->icsc[inzfl++] = inv->icsc[r - 1];
+            }
             inpgl++;
             inv->mygrad[inpgl] = (inpinv - 1) * inv->nvinv + iv;
             inv->jcsc[inpgl + 1] = inzfl + 1;
@@ -1607,12 +1615,29 @@ static int qp_anchor(const orc_asm *a, const orc_inv *inv, int inpgl, int *inpin
     return -1;
 }
 
+/* qpmfree.f90:674-693, 1037-1055: PP -> P, SS -> S, PS -> both (NVINV = 2) */
 static int qp_kind(const char *cinvtype, int *lisp, int *liss)
 {
     char a0 = cinvtype[0] & ~0x20, a1 = cinvtype[1] & ~0x20;
-    *lisp = a0 == 'P' && a1 == 'P';
-    *liss = a0 == 'S' && a1 == 'S';
+    *lisp = (a0 == 'P' && a1 == 'P') || (a0 == 'P' && a1 == 'S');
+    *liss = (a0 == 'S' && a1 == 'S') || (a0 == 'P' && a1 == 'S');
     return (*lisp || *liss) ? 0 : 1;
+}
+
+/* CARG(IVINV) for the element matrices of one element (qpmfree.f90:836-851,
+ * 1163-1178): the P matrices first when LISP, then the S matrices */
+static void qp_carg(int lisp, int liss, int nvinv, int at, const double *dmda, const double *dmdb,
+                    const double complex *dkda, const double complex *dkdb, float complex *carg)
+{
+    int iv = 0;
+    if (lisp) {
+        double complex s = dmda[at] + dkda[at];
+        carg[iv++] = (float)creal(s) + I * (float)cimag(s);
+    }
+    if (liss && iv < nvinv) {
+        double complex s = dmdb[at] + dkdb[at];
+        carg[iv++] = (float)creal(s) + I * (float)cimag(s);
+    }
 }
 
 /* qpmfree.f90:622-912 CADJ_NU_ALPHA, its element loops: for every local
@@ -1626,7 +1651,8 @@ static int qp_kind(const char *cinvtype, int *lisp, int *liss)
  * and after the element loop P3(IA35) -= REAL(CONJG(TEMP(row)) ADJLAM(row))
  * over the column's rows.  The MUMPS solves and MPI reductions that follow
  * (:914-980) are the caller's.  COMPLEX*8 arithmetic in the reference's
- * order; NVINV = 1 (PP or SS). */
+ * order; PP or SS (NVINV = 1) or PS (NVINV = 2: per element entry the P
+ * term, then the S term, each with the weight of its own variable). */
 int orc_qp_nu_alpha(const orc_asm *a, const orc_inv *inv, const char *cinvtype, double freq,
                     double py, const float *v, const float complex *adjlam,
                     const float complex *wave, float complex *rhsbnu, float complex *rhsalp,
@@ -1634,7 +1660,8 @@ int orc_qp_nu_alpha(const orc_asm *a, const orc_inv *inv, const char *cinvtype, 
 {
     const double twopi = 6.2831853071795862;
     int lisp, liss;
-    if (qp_kind(cinvtype, &lisp, &liss) || inv->nvinv != 1) return 1;
+    if (qp_kind(cinvtype, &lisp, &liss) || inv->nvinv != lisp + liss) return 1;
+    const int nv = inv->nvinv;
     double omega = twopi * freq;
     elem_ws w;
     ws_init(&w, a->nord);
@@ -1642,16 +1669,16 @@ int orc_qp_nu_alpha(const orc_asm *a, const orc_inv *inv, const char *cinvtype, 
     double *dab = (double *)malloc(sizeof(double) * 2 * q), *dab2 = (double *)malloc(sizeof(double) * 2 * q);
     double *dmdb = (double *)malloc(sizeof(double) * nee * nee);
     double complex *dkdb = (double complex *)malloc(sizeof(double complex) * nee * nee);
-    float complex *temp = (float complex *)malloc(sizeof(float complex) * (inv->mbufrhs + 1));
+    float complex *temp = (float complex *)malloc(sizeof(float complex) * 2 * (inv->mbufrhs + 1));
     for (int i = 0; i < a->ndof; i++) { rhsbnu[i] = 0.0f; rhsalp[i] = 0.0f; }
     for (int i = 0; i < inv->na35; i++) p3[i] = 0.0f;
     int ierr = 0;
-    for (int inpgl = 1; inpgl <= inv->nnpgl && !ierr; inpgl += inv->nvinv) {
+    for (int inpgl = 1; inpgl <= inv->nnpgl && !ierr; inpgl += nv) {
         int inpinv, inpg = qp_anchor(a, inv, inpgl, &inpinv);
         if (inpg < 0) { ierr = 1; break; }
-        int ia35 = (inpinv - 1) * inv->nvinv + 1;
         int i1 = inv->jcsc[inpgl - 1], i2 = inv->jcsc[inpgl] - 1, nnz = i2 - i1 + 1;
-        for (int r = 0; r < nnz; r++) temp[r] = 0.0f;
+        const int ld = inv->mbufrhs + 1;
+        for (int r = 0; r < nv * ld; r++) temp[r] = 0.0f;
         for (int ic = 0; ic < inv->ncon && !ierr; ic++) {
             int e = inv->mconn[(inpg - 1) + (size_t)a->nnpg * ic];
             if (e <= 0) break;
@@ -1660,11 +1687,7 @@ int orc_qp_nu_alpha(const orc_asm *a, const orc_inv *inv, const char *cinvtype, 
                             a->ecoeff, a->nnpg, w.shl, w.det, dab, dab2, w.shg)) { ierr = 1; break; }
             orc_dabmkisob25(nen, w.nint, lisp, liss, omega, py, w.wgll, w.det, dab, w.shg,
                             w.me, dmdb, w.ke, dkdb);
-            const double *dm = lisp ? w.me : dmdb;
-            const double complex *dk = lisp ? w.ke : dkdb;
-            for (int r = 0; r < nnz; r++) temp[r] = 0.0f;
-            float wv = (float)inv->elem_wts[e] * v[ia35 - 1];
-            float complex cw = wv + I * 0.0f;
+            for (int r = 0; r < nv * ld; r++) temp[r] = 0.0f;
             const int *lm = a->lm + NDIM * nen * e;
             for (int ip = 0; ip < nee && !ierr; ip++) {
                 int idof = lm[ip];
@@ -1674,18 +1697,26 @@ int orc_qp_nu_alpha(const orc_asm *a, const orc_inv *inv, const char *cinvtype, 
                 for (int iq = 0; iq < nee; iq++) {
                     int jdof = lm[iq];
                     if (jdof <= 0) continue;
-                    double complex s = dm[ip + nee * iq] + dk[ip + nee * iq];
-                    float complex carg = (float)creal(s) + I * (float)cimag(s);
-                    float complex wc = cmulf_ref(cw, carg);
-                    rhsbnu[jdof - 1] += cmulf_ref(adjlam[idof - 1], conjf(wc));
-                    rhsalp[idof - 1] += cmulf_ref(wc, wave[jdof - 1]);
-                    temp[inz - 1] += cmulf_ref(wc, wave[jdof - 1]);
+                    float complex carg[2];
+                    qp_carg(lisp, liss, nv, ip + nee * iq, w.me, dmdb, w.ke, dkdb, carg);
+                    for (int iv = 0; iv < nv; iv++) {
+                        int ia35 = (inpinv - 1) * nv + iv + 1;
+                        float complex cw = (float)inv->elem_wts[e] * v[ia35 - 1] + I * 0.0f;
+                        float complex wc = cmulf_ref(cw, carg[iv]);
+                        rhsbnu[jdof - 1] += cmulf_ref(adjlam[idof - 1], conjf(wc));
+                        rhsalp[idof - 1] += cmulf_ref(wc, wave[jdof - 1]);
+                        temp[iv * ld + inz - 1] += cmulf_ref(wc, wave[jdof - 1]);
+                    }
                 }
             }
         }
-        for (int r = 0; r < nnz && !ierr; r++) {
-            int idof = inv->icsc[i1 - 1 + r];
-            p3[ia35 - 1] -= crealf(cmulf_ref(conjf(temp[r]), adjlam[idof - 1]));
+        for (int iv = 0; iv < nv && !ierr; iv++) {
+            int ia35 = (inpinv - 1) * nv + iv + 1;
+            int k1 = inv->jcsc[inpgl + iv - 1], k2 = inv->jcsc[inpgl + iv] - 1;
+            for (int r = 0; r <= k2 - k1; r++) {
+                int idof = inv->icsc[k1 - 1 + r];
+                p3[ia35 - 1] -= crealf(cmulf_ref(conjf(temp[iv * ld + r]), adjlam[idof - 1]));
+            }
         }
     }
     free(dab); free(dab2); free(dmdb); free(dkdb); free(temp);
@@ -1699,14 +1730,16 @@ int orc_qp_nu_alpha(const orc_asm *a, const orc_inv *inv, const char *cinvtype, 
  *   T1(row) += CWGHT*CARG*WAVE(JDOF),  T2(row) += CWGHT*CARG*ALPHA(JDOF)
  * (both cleared per element, :1155-1160, so the last element remains), then
  * P1(IA35) -= REAL(CONJG(T1) NU), P2(IA35) -= REAL(CONJG(T2) ADJLAM) over
- * the column's rows; before the MPI reductions of :1222-1226. */
+ * the column's rows; before the MPI reductions of :1222-1226.  PS (NVINV =
+ * 2): T1, T2 per variable from the P and the S matrices of the element. */
 int orc_met225(const orc_asm *a, const orc_inv *inv, const char *cinvtype, double freq, double py,
                const float complex *adjlam, const float complex *nu, const float complex *alpha,
                const float complex *wave, float *p1, float *p2)
 {
     const double twopi = 6.2831853071795862;
     int lisp, liss;
-    if (qp_kind(cinvtype, &lisp, &liss) || inv->nvinv != 1) return 1;
+    if (qp_kind(cinvtype, &lisp, &liss) || inv->nvinv != lisp + liss) return 1;
+    const int nv = inv->nvinv;
     double omega = twopi * freq;
     elem_ws w;
     ws_init(&w, a->nord);
@@ -1714,16 +1747,16 @@ int orc_met225(const orc_asm *a, const orc_inv *inv, const char *cinvtype, doubl
     double *dab = (double *)malloc(sizeof(double) * 2 * q);
     double *dmdb = (double *)malloc(sizeof(double) * nee * nee);
     double complex *dkdb = (double complex *)malloc(sizeof(double complex) * nee * nee);
-    float complex *t1 = (float complex *)malloc(sizeof(float complex) * (inv->mbufrhs + 1));
-    float complex *t2 = (float complex *)malloc(sizeof(float complex) * (inv->mbufrhs + 1));
+    const int ld = inv->mbufrhs + 1;
+    float complex *t1 = (float complex *)malloc(sizeof(float complex) * 2 * ld);
+    float complex *t2 = (float complex *)malloc(sizeof(float complex) * 2 * ld);
     for (int i = 0; i < inv->na35; i++) { p1[i] = 0.0f; p2[i] = 0.0f; }
     int ierr = 0;
-    for (int inpgl = 1; inpgl <= inv->nnpgl && !ierr; inpgl += inv->nvinv) {
+    for (int inpgl = 1; inpgl <= inv->nnpgl && !ierr; inpgl += nv) {
         int inpinv, inpg = qp_anchor(a, inv, inpgl, &inpinv);
         if (inpg < 0) { ierr = 1; break; }
-        int ia35 = (inpinv - 1) * inv->nvinv + 1;
         int i1 = inv->jcsc[inpgl - 1], i2 = inv->jcsc[inpgl] - 1, nnz = i2 - i1 + 1;
-        for (int r = 0; r < nnz; r++) { t1[r] = 0.0f; t2[r] = 0.0f; }
+        for (int r = 0; r < nv * ld; r++) { t1[r] = 0.0f; t2[r] = 0.0f; }
         for (int ic = 0; ic < inv->ncon && !ierr; ic++) {
             int e = inv->mconn[(inpg - 1) + (size_t)a->nnpg * ic];
             if (e <= 0) break;
@@ -1733,9 +1766,7 @@ int orc_met225(const orc_asm *a, const orc_inv *inv, const char *cinvtype, doubl
                            a->ecoeff, a->nnpg, w.shl, w.det, dab, w.shg)) { ierr = 1; break; }
             orc_dabmkisob25(nen, w.nint, lisp, liss, omega, py, w.wgll, w.det, dab, w.shg,
                             w.me, dmdb, w.ke, dkdb);
-            const double *dm = lisp ? w.me : dmdb;
-            const double complex *dk = lisp ? w.ke : dkdb;
-            for (int r = 0; r < nnz; r++) { t1[r] = 0.0f; t2[r] = 0.0f; }
+            for (int r = 0; r < nv * ld; r++) { t1[r] = 0.0f; t2[r] = 0.0f; }
             const int *lm = a->lm + NDIM * nen * e;
             for (int ip = 0; ip < nee && !ierr; ip++) {
                 int idof = lm[ip];
@@ -1745,22 +1776,77 @@ int orc_met225(const orc_asm *a, const orc_inv *inv, const char *cinvtype, doubl
                 for (int iq = 0; iq < nee; iq++) {
                     int jdof = lm[iq];
                     if (jdof <= 0) continue;
-                    double complex s = dm[ip + nee * iq] + dk[ip + nee * iq];
-                    float complex carg = (float)creal(s) + I * (float)cimag(s);
-                    float complex wc = cmulf_ref(cw, carg);
-                    t1[inz - 1] += cmulf_ref(wc, wave[jdof - 1]);
-                    t2[inz - 1] += cmulf_ref(wc, alpha[jdof - 1]);
+                    float complex carg[2];
+                    qp_carg(lisp, liss, nv, ip + nee * iq, w.me, dmdb, w.ke, dkdb, carg);
+                    for (int iv = 0; iv < nv; iv++) {
+                        float complex wc = cmulf_ref(cw, carg[iv]);
+                        t1[iv * ld + inz - 1] += cmulf_ref(wc, wave[jdof - 1]);
+                        t2[iv * ld + inz - 1] += cmulf_ref(wc, alpha[jdof - 1]);
+                    }
                 }
             }
         }
-        for (int r = 0; r < nnz && !ierr; r++) {
-            int idof = inv->icsc[i1 - 1 + r];
-            p1[ia35 - 1] -= crealf(cmulf_ref(conjf(t1[r]), nu[idof - 1]));
-            p2[ia35 - 1] -= crealf(cmulf_ref(conjf(t2[r]), adjlam[idof - 1]));
+        for (int iv = 0; iv < nv && !ierr; iv++) {
+            int ia35 = (inpinv - 1) * nv + iv + 1;
+            int k1 = inv->jcsc[inpgl + iv - 1], k2 = inv->jcsc[inpgl + iv] - 1;
+            for (int r = 0; r <= k2 - k1; r++) {
+                int idof = inv->icsc[k1 - 1 + r];
+                p1[ia35 - 1] -= crealf(cmulf_ref(conjf(t1[iv * ld + r]), nu[idof - 1]));
+                p2[ia35 - 1] -= crealf(cmulf_ref(conjf(t2[iv * ld + r]), adjlam[idof - 1]));
+            }
         }
     }
     free(dab); free(dmdb); free(dkdb); free(t1); free(t2);
     ws_free(&w);
+    return ierr;
+}
+
+/* disp25.f90:710-908 FILL_UWAVE_QMESH (LCHECK false): the plane wave
+ * u = (cos BAZ sin AOI, sin BAZ sin AOI, cos AOI) exp(-i (KAPPAX x + KAPPAZ z))
+ * at every DOF, from the first element (in element, eta, xi order) holding
+ * it; x, z by CSF2DND at the GLL point.  Returns 1 when a DOF is never
+ * reached. */
+int orc_fill_uwave(const orc_asm *a, double kappax, double kappaz, double baz, double aoi,
+                   double complex *ue)
+{
+    const double pi180 = 0.017453292519943295;
+    const int nl = a->nord + 1, nen = nl * nl;
+    double *xi = (double *)malloc(sizeof(double) * nl), *wt = (double *)malloc(sizeof(double) * nl);
+    char *init = (char *)calloc(a->ndof > 0 ? a->ndof : 1, 1);
+    orc_gll(nl, xi, wt);
+    const double cosaz = cos(baz * pi180), sinaz = sin(baz * pi180);
+    const double cosi = cos(aoi * pi180), sini = sin(aoi * pi180);
+    const double pu = cosaz * sini, pv = sinaz * sini, pw = cosi;
+    for (int i = 0; i < a->ndof; i++) ue[i] = 0.0;
+    for (int e = 0; e < a->nelem; e++)
+        for (int ie = 0; ie < nl; ie++)
+            for (int ix = 0; ix < nl; ix++) {
+                const int iae = ie * nl + ix;
+                const int *lm = a->lm + NDIM * (iae + nen * e);
+                int seen = 0;
+                for (int i = 0; i < NDIM; i++)
+                    if (lm[i] > 0 && init[lm[i] - 1]) seen = 1;
+                if (seen) continue;
+                double sf[4], x = 0.0, z = 0.0;
+                csf2dnd(xi[ix], xi[ie], sf);
+                for (int k = 0; k < 4; k++) {
+                    int g = a->ieng[4 * e + k] - 1;
+                    x += sf[k] * a->xlocs[g];
+                    z += sf[k] * a->zlocs[g];
+                }
+                double complex carg = cexp(CMPLX(0.0, -kappax * x - kappaz * z));
+                const double p[3] = {pu, pv, pw};
+                for (int i = 0; i < NDIM; i++) {
+                    int d = lm[i];
+                    if (d <= 0) continue;
+                    init[d - 1] = 1;
+                    ue[d - 1] = CMPLX(p[i] * creal(carg), p[i] * cimag(carg));
+                }
+            }
+    int ierr = 0;
+    for (int i = 0; i < a->ndof; i++)
+        if (!init[i]) ierr = 1;
+    free(xi); free(wt); free(init);
     return ierr;
 }
 

@@ -199,6 +199,8 @@ void orc_cadjlam_rhs(int ndof, int nfreq, int nrec, int ifreq, int isrc, double 
                      double azmod, int norm, int irestp, float epss, float dtmax, int ibphase,
                      int lunwrap, const double *yrec, const int *mrdof, const float complex *est,
                      const float complex *obs, const float *wghts, float complex *rhs, int *iwarn);
+int orc_fill_uwave(const orc_asm *a, double kappax, double kappaz, double baz, double aoi,
+                   double complex *ue);
 int orc_rec_space1d(int nrec, const double *xrec, double *dx);
 int orc_laplace_1dv2(int nrec, int ldwght, const double *dx, const float complex *obs, double *covd);
 int orc_perm_obs(int nrec, const float complex *obs, int *iperm);

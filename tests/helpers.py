@@ -64,3 +64,19 @@ def pairs(nfreq=2, nky=3, fmin=0.02, fmax=0.1, vs_min=3460.0):
 
 
 TWOPI = 2 * math.pi
+
+
+def ps_inv(one):
+    """GRADPTRS's PS structure from the single-variable one: each column
+    copied forward for the second variable, MYGRAD = 2 (INPINV - 1) + IVINV
+    (gengmask.f90:1078-1098), without the NNPG*NCON*NEE bound on ICSC_FDIST
+    that stops the reference once about 60% of the anchors are inverted."""
+    cols = [one.icsc[one.jcsc[c] - 1:one.jcsc[c + 1] - 1] for c in range(one.nnpgl)]
+    icsc = np.concatenate([r for r in cols for _ in range(2)]).astype(np.int32)
+    lens = np.repeat([len(r) for r in cols], 2)
+    jcsc = np.concatenate([[1], 1 + np.cumsum(lens)]).astype(np.int32)
+    mygrad = np.empty(2 * one.nnpgl, np.int32)
+    mygrad[0::2] = 2 * one.mygrad - 1
+    mygrad[1::2] = 2 * one.mygrad
+    return oracle.Inv(one.nnpinv, one.ncon, 2 * one.nnpgl, int(icsc.size), one.mbufrhs, 2 * one.na35,
+                      one.maskg, one.mconn, mygrad, jcsc, icsc, one.elem_wts)

@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 from fem25_amd import configs as cf
-from tests.helpers import TWOPI, csr, mesh_case, oracle_S, oracle_graph, pairs, relerr
+from tests.helpers import TWOPI, csr, mesh_case, oracle_S, oracle_graph, pairs, ps_inv, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -600,6 +600,43 @@ def test_qpmfree_element_products(name, cinv):
     assert relerr(p1, r1) < 2e-5
     assert relerr(p2, r2) < 2e-5
     assert np.count_nonzero(p1) == np.count_nonzero(r1) > 0
+
+
+@pytest.mark.parametrize("name", ["tiny_layered", "pml", "p6"])
+def test_qpmfree_element_products_joint_ps(name):
+    """The joint P-S inversion (CINVTYPE 'PS', NVINV = 2): f25_inv_setup's
+    structure is the single-variable one with each column copied forward
+    (GRADPTRS), and CADJ_NU_ALPHA's right-hand sides and P3 and MET225_MOD's
+    P1, P2 on the GPU match the oracle's PS restatement (2e-5 norm-wise, the
+    reference's single-precision sums); CFMAT_DIST refuses PS as VFORCE25
+    does."""
+    mi = _mi(name)
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    inv = mi.inv_setup(b"PS")
+    io = ps_inv(oracle.inv_setup(m, mod, g, pml))
+    for k, ko in (("maskg", "maskg"), ("mygrad", "mygrad"), ("jcsc_fdist", "jcsc"), ("icsc_fdist", "icsc"),
+                  ("elem_wts", "elem_wts")):
+        np.testing.assert_array_equal(getattr(inv, k), getattr(io, ko), err_msg=k)
+    rng = np.random.default_rng(97)
+    freq, py = 0.07, 1.1e-4
+    cvec = lambda: (rng.standard_normal(g.ndof) + 1j * rng.standard_normal(g.ndof)).astype(np.complex64)  # noqa: E731
+    u, lam, nu, alpha = cvec(), cvec(), cvec(), cvec()
+    v = rng.standard_normal(io.na35).astype(np.float32)
+    print(f"qpmfree PS {name}: ndof={g.ndof} nnpgl={io.nnpgl}", flush=True)
+    bnu, alp, p3 = mi.qp_nu_alpha(freq, py, v, lam, u)
+    rb, ra, r3 = oracle.qp_nu_alpha(m, mod, g, io, freq, py, v, lam, u, cinvtype=b"PS", pml=pml)
+    assert relerr(alp, ra) < 2e-5
+    assert relerr(bnu, rb) < 2e-5
+    assert relerr(p3, r3) < 2e-5
+    p1, p2 = mi.met225(freq, py, lam, nu, alpha, u)
+    r1, r2 = oracle.met225(m, mod, g, io, freq, py, lam, nu, alpha, u, cinvtype=b"PS", pml=pml)
+    assert relerr(p1, r1) < 2e-5
+    assert relerr(p2, r2) < 2e-5
+    assert np.count_nonzero(p1[0::2]) > 0 and np.count_nonzero(p1[1::2]) > 0
+    from fem25_amd import api
+    _, code = api.cfmat_dist(inv.nz_fdist, g.ndof, freq, py, u, mi, inv)
+    assert code == 7          # F25_ERR_INVTYPE
 
 
 def test_gnmfree_batched_uneven():
