@@ -849,8 +849,8 @@ static int launch_apply_v(f25_mesh *m, const double *om, const double *kv, int n
         const size_t nwg = (size_t)((n + 7) / 8) * 8 * ngrp;
         if (nwg > 0x7fffffff) FAIL(F25_ERR_ARG, "too many pairs for one launch");
         hipLaunchKernelGGL((k_apply_v<NL, NRT>), dim3((unsigned)nwg), dim3(64 * 3 * NRT), 0, m->st,
-                           m->d_order + m->cptr[c], n, vcf, m->d_lm, m->d_first, npair, om, kv, x, y, zpad,
-                           sink);
+                           m->d_order + m->cptr[c], n, vcf, m->d_lm, m->d_lm, m->d_first, npair, om, kv, x, y,
+                           zpad, sink);
     }
     HIPCHK(hipGetLastError());
     return 0;

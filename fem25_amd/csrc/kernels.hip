@@ -821,7 +821,10 @@ __device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, cons
 // workgroups per CU.  (Measured at 200x100 elements, p4, 128 pairs: two
 // pair groups per workgroup, 6 waves, fit only two workgroups per CU and
 // ran 0.69 instead of 0.63 ms per launch; 13-row tiles at four waves per
-// SIMD ran 0.72 ms.)
+// SIMD ran 0.72 ms.)  lm maps the element's nodes to the DOFs the input is
+// read from, lm_out to the DOFs its rows go to (-1: read zero / store to
+// the sink): the same array for the operator, DOF-class masks for the
+// Bielak effective force.
 template <int NL, int NRT>
 __global__ __launch_bounds__((64 * 3 * NRT), (VShape<NL, NRT>::MINW)) void k_apply_v(
         const int *__restrict__ order, int nel, const double *__restrict__ vcf, const int *__restrict__ lm,
