@@ -93,6 +93,10 @@ EXPORTS = {
                                            C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, dp, dp, ip, C.c_void_p,
                                            C.c_void_p, fp, C.c_void_p, ip]),
     "f25_rec_space1d": (C.c_int, [C.c_int, dp, dp]),
+    "f25_fill_uwave_dev": (C.c_int, [C.c_void_p, C.c_int, dp, dp, dp, dp, C.c_void_p]),
+    "f25_peff_batch_dev": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p]),
+    "f25_bielak_rhs_dev": (C.c_int, [C.c_void_p, C.c_int, dp, dp, dp, dp, dp, dp, C.c_void_p, C.c_void_p,
+                                     C.c_int]),
     "f25_gn_fv": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp, C.c_void_p]),
     "f25_gn_fhw": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, C.c_void_p, dp]),
     "f25_gn_diag": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_void_p, dp]),

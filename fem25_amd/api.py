@@ -187,6 +187,47 @@ class MeshInfo:
               "f25_csr_apply_dev")
 
     # -- batched element-block product ----------------------------------------
+    def fill_uwave_dev(self, kappax, kappaz, baz, aoi, ue) -> None:
+        """FILL_UWAVE_QMESH for a batch of plane waves into ue, a torch
+        complex128 tensor [ndof, nwave] on the device."""
+        _ready(ue)
+        kx, kz, bz, ai = (np.ascontiguousarray(np.atleast_1d(a), np.float64) for a in (kappax, kappaz, baz, aoi))
+        if not (kx.shape == kz.shape == bz.shape == ai.shape) or kx.ndim != 1:
+            raise ValueError("kappax, kappaz, baz, aoi must be 1-D arrays of one length")
+        import torch
+        _expect(ue, torch.complex128, self.ndof * kx.size, "ue")
+        check(_lib.load().f25_fill_uwave_dev(self.h, kx.size, _p(kx, dp), _p(kz, dp), _p(bz, dp), _p(ai, dp),
+                                             C.c_void_p(ue.data_ptr())), "f25_fill_uwave_dev")
+
+    def peff_batch_dev(self, omega, ky, ue, peff) -> None:
+        """ASMB_PEFF for every (omega, ky) pair of the batch: ue, peff torch
+        complex128 tensors [ndof, npair] on the device."""
+        import torch
+        _ready(ue, peff)
+        om = np.ascontiguousarray(omega, np.float64)
+        kv = np.ascontiguousarray(ky, np.float64)
+        if om.shape != kv.shape or om.ndim != 1:
+            raise ValueError("omega and ky must be 1-D arrays of one length")
+        _expect(ue, torch.complex128, self.ndof * om.size, "ue")
+        _expect(peff, torch.complex128, self.ndof * om.size, "peff")
+        check(_lib.load().f25_peff_batch_dev(self.h, om.size, _p(om, dp), _p(kv, dp), C.c_void_p(ue.data_ptr()),
+                                             C.c_void_p(peff.data_ptr())), "f25_peff_batch_dev")
+
+    def bielak_rhs_dev(self, omega, ky, kappax, kappaz, baz, aoi, ue, peff, graph: bool = False) -> None:
+        """Plane-wave sources of a batch, one per pair: ue[:, b] and its
+        effective force peff[:, b] (torch complex128 [ndof, npair])."""
+        import torch
+        _ready(ue, peff)
+        arrs = [np.ascontiguousarray(np.atleast_1d(a), np.float64) for a in (omega, ky, kappax, kappaz, baz, aoi)]
+        if any(a.shape != arrs[0].shape for a in arrs) or arrs[0].ndim != 1:
+            raise ValueError("the per-pair arrays must be 1-D arrays of one length")
+        npair = arrs[0].size
+        _expect(ue, torch.complex128, self.ndof * npair, "ue")
+        _expect(peff, torch.complex128, self.ndof * npair, "peff")
+        check(_lib.load().f25_bielak_rhs_dev(self.h, npair, *[_p(a, dp) for a in arrs], C.c_void_p(ue.data_ptr()),
+                                             C.c_void_p(peff.data_ptr()), _lib.F25_APPLY_GRAPH if graph else 0),
+              "f25_bielak_rhs_dev")
+
     def apply_batch_dev(self, omega, ky, x, y, mode: int = _lib.F25_APPLY_F64) -> None:
         """x, y: torch tensors [ndof, npair] (complex128, or complex64 for F32)."""
         import torch

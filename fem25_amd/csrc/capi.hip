@@ -162,6 +162,27 @@ struct f25_mesh_s {
     // (omega, ky) staging, grown on demand
     double *d_omky = nullptr;
     int omky_cap = 0;
+    // Bielak sources in the batch: the ring of elements holding both 'B' and
+    // 'E' DOFs, by colour, with DOF maps selecting the input and output
+    // classes of ASMB_PEFF's two terms, and the DOF positions of
+    // FILL_UWAVE_QMESH; built on first use
+    struct {
+        bool built = false, uw = false;
+        std::vector<int> cptr;
+        int *d_order = nullptr;
+        int *d_lm[4] = {};                       // E in, B out, B in, E out
+        unsigned long long *d_first0 = nullptr;  // every row adds
+        int *d_brows = nullptr;
+        int nb = 0;
+        double2 *d_xz = nullptr;
+        signed char *d_comp = nullptr;
+        double *d_pw = nullptr;
+        int pw_cap = 0;
+        // captured plane waves + effective forces and what they were captured for
+        hipGraphExec_t exec = nullptr;
+        const void *ue = nullptr, *peff = nullptr, *om = nullptr, *pw = nullptr, *pad = nullptr, *vcf = nullptr;
+        int npair = 0;
+    } br;
 };
 
 extern "C" const char *f25_last_error(void) { return g_err.c_str(); }
@@ -439,6 +460,10 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc);
     dfree(m->d_ew);
     dfree(m->d_omky);
+    dfree(m->br.d_order); dfree(m->br.d_first0); dfree(m->br.d_brows); dfree(m->br.d_xz); dfree(m->br.d_comp);
+    dfree(m->br.d_pw);
+    for (auto &q : m->br.d_lm) dfree(q);
+    if (m->br.exec) (void)hipGraphExecDestroy(m->br.exec);
     dfree(m->pa.d_mats); dfree(m->pa.d_om); dfree(m->pa.d_grp); dfree(m->pa.d_gpair);
     if (m->blas) (void)rocblas_destroy_handle(m->blas);
     if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -903,10 +928,12 @@ extern "C" int f25_set_apply_tiles(f25_mesh *m, int nrt) {
     if (!m) FAIL(F25_ERR_ARG, "null handle");
     if (nrt != 0 && !apply_nrt_ok(m->tab.nl, nrt)) FAIL(F25_ERR_ARG, "row tile count not compiled for this order");
     HIPCHK(hipSetDevice(m->dev));
-    if (m->ag.exec) {   // a captured product holds the previous kernel
+    if (m->ag.exec || m->br.exec) {   // a captured product holds the previous kernel
         HIPCHK(hipStreamSynchronize(m->st));
-        (void)hipGraphExecDestroy(m->ag.exec);
+        if (m->ag.exec) (void)hipGraphExecDestroy(m->ag.exec);
+        if (m->br.exec) (void)hipGraphExecDestroy(m->br.exec);
         m->ag.exec = nullptr;
+        m->br.exec = nullptr;
     }
     const int old = m->apply_nrt;
     m->apply_nrt = nrt;
@@ -1209,6 +1236,285 @@ static int transpose_dev(hipStream_t s, const T *in, T *out, int rows, int cols)
     if (nt > 0x7fffffff) FAIL(F25_ERR_ARG, "transpose too large");
     hipLaunchKernelGGL(k_transpose_c<T>, dim3((unsigned)nt), dim3(16, 16), 0, s, in, out, rows, cols);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// ---- Bielak sources in the batch ------------------------------------------
+// The ring: elements with a 'B' and an 'E' DOF (only they couple the two
+// classes, asmble.f90:356-363), coloured; pass 0 reads the 'E' DOFs and
+// writes the 'B' rows, pass 1 reads 'B' and writes 'E' (the DOF maps hold -1
+// elsewhere: read zero, store to the sink).
+static int bielak_prepare(f25_mesh *m) {
+    auto &B = m->br;
+    if (B.built) return 0;
+    const int nelem = m->d.nelem, nen = m->tab.nen, nee = 3 * nen;
+    const std::vector<char> &cnp = m->g.cnp;
+    std::vector<int> ring, sub;
+    for (int e = 0; e < nelem; ++e) {
+        bool hb = false, he = false;
+        for (int j = 0; j < nee; ++j) {
+            const int d = m->g.lm[(size_t)e * nee + j];
+            if (d <= 0) continue;
+            hb |= cnp[d - 1] == 'B';
+            he |= cnp[d - 1] == 'E';
+        }
+        if (!(hb && he)) continue;
+        if (m->cdom[e] == 'A') FAIL(F25_ERR_ARG, "a Bielak ring element lies in the PML");
+        ring.push_back(e);
+        sub.insert(sub.end(), m->ieng.begin() + 4 * e, m->ieng.begin() + 4 * e + 4);
+    }
+    std::vector<int> order, ptr;
+    if (!ring.empty()) colour_elements((int)ring.size(), m->d.nnpg, sub.data(), order, ptr);
+    else ptr.assign(1, 0);
+    std::vector<int> ord(ring.size());
+    for (size_t i = 0; i < ring.size(); ++i) ord[i] = ring[order[i]];
+    const char cls[4] = {'E', 'B', 'B', 'E'};
+    std::vector<int> lmv[4];
+    for (int k = 0; k < 4; ++k) lmv[k].assign((size_t)nelem * nee, -1);
+    std::vector<char> isb(m->g.ndof, 0);
+    for (int e : ring)
+        for (int j = 0; j < nee; ++j) {
+            const int d = m->g.lm[(size_t)e * nee + j];
+            if (d <= 0) continue;
+            for (int k = 0; k < 4; ++k)
+                if (cnp[d - 1] == cls[k]) lmv[k][(size_t)e * nee + j] = d - 1;
+            if (cnp[d - 1] == 'B') isb[d - 1] = 1;
+        }
+    std::vector<int> brows;
+    for (int d = 0; d < m->g.ndof; ++d)
+        if (isb[d]) brows.push_back(d);
+    TRY(upload(&B.d_order, ord.data(), ord.size(), m->st));
+    for (int k = 0; k < 4; ++k) TRY(upload(&B.d_lm[k], lmv[k].data(), lmv[k].size(), m->st));
+    TRY(dalloc(&B.d_first0, (size_t)nelem));
+    fill_async(B.d_first0, (size_t)nelem, 0ull, m->st);
+    TRY(upload(&B.d_brows, brows.data(), brows.size(), m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    B.nb = (int)brows.size();
+    B.cptr = ptr;
+    B.built = true;
+    return 0;
+}
+
+template <int NL, int NRT>
+static int launch_ring(f25_mesh *m, int pass, const double *om, const double *kv, int npair, const double2 *x,
+                       double2 *y) {
+    auto &B = m->br;
+    const double2 *zpad = reinterpret_cast<const double2 *>(m->d_vpad);
+    double2 *sink = reinterpret_cast<double2 *>(m->d_vpad) + m->vpad_cap;
+    const int ngrp = (npair + 63) / 64;
+    for (size_t c = 0; c + 1 < B.cptr.size(); ++c) {
+        const int n = B.cptr[c + 1] - B.cptr[c];
+        if (!n) continue;
+        const size_t nwg = (size_t)((n + 7) / 8) * 8 * ngrp;
+        if (nwg > 0x7fffffff) FAIL(F25_ERR_ARG, "too many pairs for one launch");
+        hipLaunchKernelGGL((k_apply_v<NL, NRT>), dim3((unsigned)nwg), dim3(64 * 3 * NRT), 0, m->st,
+                           B.d_order + B.cptr[c], n, m->d_vcf, B.d_lm[2 * pass], B.d_lm[2 * pass + 1], B.d_first0,
+                           npair, om, kv, x, y, zpad, sink);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int ring_pass(f25_mesh *m, int pass, const double *om, const double *kv, int npair, const double2 *x,
+                     double2 *y) {
+    switch (m->tab.nl * 8 + apply_nrt(m)) {
+#define F25_RV(N, T) case N * 8 + T: return launch_ring<N, T>(m, pass, om, kv, npair, x, y);
+        F25_RV(3, 1) F25_RV(3, 2) F25_RV(4, 1) F25_RV(4, 2) F25_RV(5, 1) F25_RV(5, 2) F25_RV(5, 3)
+        F25_RV(6, 2) F25_RV(6, 3) F25_RV(6, 4) F25_RV(7, 2) F25_RV(7, 3) F25_RV(7, 4)
+#undef F25_RV
+    }
+    FAIL(F25_ERR_ARG, "unsupported order");
+}
+
+// the effective force's launches (bielak_prepare and ensure_vpad first):
+// zero fill, pass 0, 'B' rows negated, pass 1; no host synchronisation
+static int peff_launches(f25_mesh *m, int npair, const double *om, const double *kv, const double2 *u,
+                         double2 *y) {
+    auto &B = m->br;
+    fill_async(y, (size_t)m->g.ndof * npair, make_double2(0.0, 0.0), m->st);
+    TRY(ring_pass(m, 0, om, kv, npair, u, y));
+    if (B.nb) {
+        const int64_t tot = (int64_t)B.nb * npair;
+        const int nb = (int)std::min<int64_t>((tot + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_negate_rows, dim3(nb), dim3(256), 0, m->st, B.nb, npair, B.d_brows, y);
+    }
+    TRY(ring_pass(m, 1, om, kv, npair, u, y));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// ASMB_PEFF (asmble.f90:318-377) for npair (omega, ky) pairs at once, FP64:
+// peff = 0 outside the 'B' and 'E' rows, -S_BE u_E on 'B' rows, +S_EB u_E on
+// 'E' rows, S(omega, ky) from the element blocks.
+extern "C" int f25_peff_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
+                                  const void *ue_dev, void *peff_dev) {
+    if (!m || npair <= 0 || !omega || !ky || !ue_dev || !peff_dev || ue_dev == peff_dev)
+        FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    if (m->disp) FAIL(F25_ERR_ARG, "dispersive medium: assemble per pair and use f25_asmb_peff");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(bielak_prepare(m));
+    TRY(stage_omky(m, npair, omega, ky));
+    TRY(ensure_vpad(m, npair));
+    const double *om = m->d_omky, *kv = m->d_omky + m->omky_cap;
+    HIPCHK(hipEventRecord(m->ev0, m->st));
+    TRY(peff_launches(m, npair, om, kv, (const double2 *)ue_dev, (double2 *)peff_dev));
+    HIPCHK(hipEventRecord(m->ev1, m->st));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(m->st));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, m->ev0, m->ev1) == hipSuccess) record_ms("peff_batch", ms);
+    return 0;
+}
+
+// FILL_UWAVE_QMESH's DOF positions: each DOF at the GLL point of the first
+// element (element, eta, xi order) holding it, placed by CSF2DND
+static int uwave_prepare(f25_mesh *m) {
+    auto &B = m->br;
+    if (B.uw) return 0;
+    const int nl = m->tab.nl, nen = m->tab.nen, ndof = m->g.ndof;
+    std::vector<double2> xz(ndof, make_double2(0.0, 0.0));
+    std::vector<signed char> comp(ndof, 0);
+    std::vector<char> init(ndof, 0);
+    for (int e = 0; e < m->d.nelem; ++e)
+        for (int ie = 0; ie < nl; ++ie)
+            for (int ix = 0; ix < nl; ++ix) {
+                const int *lm = &m->g.lm[((size_t)e * nen + ie * nl + ix) * 3];
+                bool seen = false;
+                for (int i = 0; i < 3; ++i) seen |= lm[i] > 0 && init[lm[i] - 1];
+                if (seen) continue;
+                double x, z;
+                gll_xz(m->d, e, m->tab.xipts[ix], m->tab.xipts[ie], x, z);
+                for (int i = 0; i < 3; ++i)
+                    if (lm[i] > 0) {
+                        init[lm[i] - 1] = 1;
+                        xz[lm[i] - 1] = make_double2(x, z);
+                        comp[lm[i] - 1] = (signed char)i;
+                    }
+            }
+    for (int d = 0; d < ndof; ++d)
+        if (!init[d]) FAIL(F25_ERR_GRAPH, "a DOF belongs to no element");
+    TRY(upload(&B.d_xz, xz.data(), xz.size(), m->st));
+    TRY(upload(&B.d_comp, comp.data(), comp.size(), m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    B.uw = true;
+    return 0;
+}
+
+// the plane waves' parameters (kx, kz, three polarisation factors) staged
+// on the device; the factors as FILL_UWAVE_QMESH forms them in FP64
+static int uwave_params(f25_mesh *m, int nwave, const double *kappax, const double *kappaz, const double *baz,
+                        const double *aoi) {
+    auto &B = m->br;
+    const double pi180 = 0.017453292519943295;
+    std::vector<double> pw(5 * (size_t)nwave);
+    for (int w = 0; w < nwave; ++w) {
+        const double cosaz = std::cos(baz[w] * pi180), sinaz = std::sin(baz[w] * pi180);
+        const double cosi = std::cos(aoi[w] * pi180), sini = std::sin(aoi[w] * pi180);
+        pw[5 * w + 0] = kappax[w];
+        pw[5 * w + 1] = kappaz[w];
+        pw[5 * w + 2] = cosaz * sini;
+        pw[5 * w + 3] = sinaz * sini;
+        pw[5 * w + 4] = cosi;
+    }
+    if (5 * nwave > B.pw_cap) {
+        dfree(B.d_pw);
+        B.pw_cap = 0;
+        TRY(dalloc(&B.d_pw, 5 * (size_t)nwave));
+        B.pw_cap = 5 * nwave;
+    }
+    HIPCHK(hipMemcpyAsync(B.d_pw, pw.data(), pw.size() * sizeof(double), hipMemcpyHostToDevice, m->st));
+    return 0;
+}
+
+static int uwave_launch(f25_mesh *m, int nwave, double2 *ue) {
+    auto &B = m->br;
+    const int64_t tot = (int64_t)m->g.ndof * nwave;
+    const int nb = (int)std::min<int64_t>((tot + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(k_uwave, dim3(nb), dim3(256), 0, m->st, (int64_t)m->g.ndof, nwave, B.d_xz, B.d_comp, B.d_pw,
+                       ue);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int f25_fill_uwave_dev(f25_mesh *m, int nwave, const double *kappax, const double *kappaz,
+                                  const double *baz, const double *aoi, void *ue_dev) {
+    if (!m || nwave <= 0 || !kappax || !kappaz || !baz || !aoi || !ue_dev) FAIL(F25_ERR_ARG, "invalid arguments");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(uwave_prepare(m));
+    TRY(uwave_params(m, nwave, kappax, kappaz, baz, aoi));
+    HIPCHK(hipEventRecord(m->ev0, m->st));
+    TRY(uwave_launch(m, nwave, (double2 *)ue_dev));
+    HIPCHK(hipEventRecord(m->ev1, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, m->ev0, m->ev1) == hipSuccess) record_ms("uwave", ms);
+    return 0;
+}
+
+// Plane-wave sources of a batch: pair b's wavefield u_E (FILL_UWAVE_QMESH
+// with kappax[b], kappaz[b], baz[b], aoi[b]) and its effective force at
+// (omega[b], ky[b]).  flags & F25_APPLY_GRAPH: the launches are captured in
+// a HIP graph on the first call and replayed while npair and the buffers
+// stay the same (the parameters are staged before each replay).
+extern "C" int f25_bielak_rhs_dev(f25_mesh *m, int npair, const double *omega, const double *ky,
+                                  const double *kappax, const double *kappaz, const double *baz,
+                                  const double *aoi, void *ue_dev, void *peff_dev, int flags) {
+    if (!m || npair <= 0 || !omega || !ky || !kappax || !kappaz || !baz || !aoi || !ue_dev || !peff_dev ||
+        ue_dev == peff_dev || (flags & ~F25_APPLY_GRAPH))
+        FAIL(F25_ERR_ARG, "invalid arguments");
+    if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
+    if (m->disp) FAIL(F25_ERR_ARG, "dispersive medium: assemble per pair and use f25_asmb_peff");
+    HIPCHK(hipSetDevice(m->dev));
+    TRY(bielak_prepare(m));
+    TRY(uwave_prepare(m));
+    auto &B = m->br;
+    if (B.exec && (B.npair != npair || B.ue != ue_dev || B.peff != peff_dev)) {
+        HIPCHK(hipStreamSynchronize(m->st));
+        (void)hipGraphExecDestroy(B.exec);
+        B.exec = nullptr;
+    }
+    TRY(stage_omky(m, npair, omega, ky));
+    TRY(ensure_vpad(m, npair));
+    TRY(uwave_params(m, npair, kappax, kappaz, baz, aoi));
+    const double *om = m->d_omky, *kv = m->d_omky + m->omky_cap;
+    // staging buffers may have moved since the capture
+    if (B.exec && (B.om != om || B.pw != B.d_pw || B.pad != m->d_vpad || B.vcf != m->d_vcf)) {
+        HIPCHK(hipStreamSynchronize(m->st));
+        (void)hipGraphExecDestroy(B.exec);
+        B.exec = nullptr;
+    }
+    HIPCHK(hipEventRecord(m->ev0, m->st));
+    if (!(flags & F25_APPLY_GRAPH)) {
+        TRY(uwave_launch(m, npair, (double2 *)ue_dev));
+        TRY(peff_launches(m, npair, om, kv, (const double2 *)ue_dev, (double2 *)peff_dev));
+    } else {
+        if (!B.exec) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
+            int r = uwave_launch(m, npair, (double2 *)ue_dev);
+            if (!r) r = peff_launches(m, npair, om, kv, (const double2 *)ue_dev, (double2 *)peff_dev);
+            hipError_t e = hipStreamEndCapture(m->st, &g);
+            if (r) {
+                if (g) (void)hipGraphDestroy(g);
+                return r;
+            }
+            if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
+            e = hipGraphInstantiate(&B.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                B.exec = nullptr;
+                FAIL(F25_ERR_HIP, hipGetErrorString(e));
+            }
+            B.ue = ue_dev; B.peff = peff_dev; B.npair = npair; B.om = om; B.pw = B.d_pw; B.pad = m->d_vpad;
+            B.vcf = m->d_vcf;
+        }
+        HIPCHK(hipGraphLaunch(B.exec, m->st));
+    }
+    HIPCHK(hipEventRecord(m->ev1, m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, m->ev0, m->ev1) == hipSuccess) record_ms("bielak_rhs", ms);
     return 0;
 }
 

@@ -35,6 +35,8 @@ struct Graph {
     bool have_csr = false;
 };
 int build_numbering(const f25_mesh_desc &d, const Tables &t, Graph &g);   // IEN, ID, LM, CNP
+// position of GLL point (xi, eta) of element e by CSF2DND (csf2d.f:135)
+void gll_xz(const f25_mesh_desc &d, int e, double xi, double eta, double &x, double &z);
 int build_csr(const f25_mesh_desc &d, const Tables &t, Graph &g);         // IRPTR/JCPTR
 
 // Inversion bookkeeping (GENGMASK, GRADPTRS, ELEM_WTS, one rank).

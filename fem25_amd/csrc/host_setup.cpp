@@ -187,7 +187,7 @@ void make_tables(int nord, Tables &t) {
 
 // ---------------------------------------------------------------------------
 // Bilinear position of GLL point (xi, eta) in element e (CSF2DND, csf2d.f:135).
-static inline void gll_xz(const f25_mesh_desc &d, int e, double xi, double eta, double &x, double &z) {
+void gll_xz(const f25_mesh_desc &d, int e, double xi, double eta, double &x, double &z) {
     const double eps = 2.2204e-16;
     double sf[4] = {.25 * (1.0 - xi) * (1.0 - eta), .25 * (1.0 + xi) * (1.0 - eta),
                     .25 * (1.0 + xi) * (1.0 + eta), .25 * (1.0 - xi) * (1.0 + eta)};

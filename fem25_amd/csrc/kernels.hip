@@ -2942,6 +2942,41 @@ __global__ __launch_bounds__(256) void k_cadjlam_covd(
 }
 
 // ---------------------------------------------------------------------------
+// Bielak plane-wave sources for a batch.  FILL_UWAVE_QMESH
+// (disp25.f90:710-908): u[d][w] = pol_w(comp d) exp(-i (kx_w x_d + kz_w z_d))
+// with (x_d, z_d) the DOF's node as the first element holding it places it
+// (the host's table); the phase products rounded as the reference rounds
+// them (no contraction).  pw[w] = (kx, kz, pu, pv, pw): polarisation
+// (cos baz sin aoi, sin baz sin aoi, cos aoi).  [dof][wave] layout.
+__global__ __launch_bounds__(256) void k_uwave(int64_t ndof, int nw, const double2 *__restrict__ xz,
+                                               const signed char *__restrict__ comp,
+                                               const double *__restrict__ pw, double2 *__restrict__ ue) {
+    const int64_t tot = ndof * nw;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = i / nw;
+        const int w = (int)(i - d * nw);
+        const double *p = pw + 5 * (size_t)w;
+        const double2 c = xz[d];
+        const double t = __dsub_rn(__dmul_rn(-p[0], c.x), __dmul_rn(p[1], c.y));
+        double sn, cs;
+        sincos(t, &sn, &cs);
+        const double a = p[2 + comp[d]];
+        ue[i] = make_double2(__dmul_rn(a, cs), __dmul_rn(a, sn));
+    }
+}
+
+// rows[k] (0-based DOFs) of a [dof][pair] vector negated
+__global__ __launch_bounds__(256) void k_negate_rows(int nrow, int npair, const int *__restrict__ rows,
+                                                     double2 *__restrict__ y) {
+    const int64_t tot = (int64_t)nrow * npair;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / npair), p = (int)(i - (int64_t)r * npair);
+        double2 *v = y + (size_t)rows[r] * npair + p;
+        *v = make_double2(-v->x, -v->y);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // small utilities
 template <typename T>
 __global__ void k_fill(T *p, size_t n, T v) {

@@ -129,6 +129,31 @@ int f25_csr_apply_dev(f25_mesh *m, int precision, const void *s_dev, const void 
  * PEFF(row 'B') -= S u_E over 'E' columns, PEFF(row 'E') += S u_E over 'B'
  * columns.  cdist, ue, peff are COMPLEX*8 host arrays. */
 int f25_asmb_peff(f25_mesh *m, const void *cdist, const void *ue, void *peff);
+/* The Bielak sources of a batch, FP64, device memory, [dof][pair] complex128
+ * (pair fastest, the batched product's layout):
+ *   f25_fill_uwave_dev: FILL_UWAVE_QMESH (disp25.f90:710-908) for nwave
+ *     plane waves, u = (cos baz sin aoi, sin baz sin aoi, cos aoi)
+ *     exp(-i (kappax x + kappaz z)) at every DOF (angles in degrees, the
+ *     DOF placed by the first element holding it, as the reference does);
+ *   f25_peff_batch_dev: ASMB_PEFF's effective force for npair (omega, ky)
+ *     pairs with ue_dev[:, pair] the wavefield of that pair's source: -S u_E
+ *     over 'E' columns on 'B' rows, +S u_E over 'B' columns on 'E' rows, zero
+ *     elsewhere, S(omega, ky) from the element blocks (the batched product
+ *     on the ring of elements holding both classes).  Not for dispersive
+ *     media (F25_ERR_ARG; f25_asmb_peff takes their CDIST per pair). */
+int f25_fill_uwave_dev(f25_mesh *m, int nwave, const double *kappax, const double *kappaz, const double *baz,
+                       const double *aoi, void *ue_dev);
+int f25_peff_batch_dev(f25_mesh *m, int npair, const double *omega, const double *ky, const void *ue_dev,
+                       void *peff_dev);
+/* Both for a batch of plane-wave sources, one per pair: ue_dev[:, b] the
+ * wave (kappax[b], kappaz[b], baz[b], aoi[b]), peff_dev[:, b] its effective
+ * force at (omega[b], ky[b]) (the forward right-hand sides of
+ * funcgradh.f90:207-233).  flags F25_APPLY_GRAPH: the launches are captured
+ * in a HIP graph on the first call and replayed while npair and the two
+ * buffers stay the same. */
+int f25_bielak_rhs_dev(f25_mesh *m, int npair, const double *omega, const double *ky, const double *kappax,
+                       const double *kappaz, const double *baz, const double *aoi, void *ue_dev, void *peff_dev,
+                       int flags);
 
 /* ---- batched matrix-free product ---------------------------------------- */
 

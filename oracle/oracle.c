@@ -1814,6 +1814,7 @@ int orc_fill_uwave(const orc_asm *a, double kappax, double kappaz, double baz, d
     double *xi = (double *)malloc(sizeof(double) * nl), *wt = (double *)malloc(sizeof(double) * nl);
     char *init = (char *)calloc(a->ndof > 0 ? a->ndof : 1, 1);
     orc_gll(nl, xi, wt);
+    if (nl % 2) xi[(nl - 1) / 2] = 0.0;   /* GENPTS, dshl.f:395-396 */
     const double cosaz = cos(baz * pi180), sinaz = sin(baz * pi180);
     const double cosi = cos(aoi * pi180), sini = sin(aoi * pi180);
     const double pu = cosaz * sini, pv = sinaz * sini, pw = cosi;

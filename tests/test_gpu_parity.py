@@ -639,6 +639,112 @@ def test_qpmfree_element_products_joint_ps(name):
     assert code == 7          # F25_ERR_INVTYPE
 
 
+def _peff_fp64(m, mod, g, pml, freq, py, ue):
+    """ASMB_PEFF in FP64 from the oracle's assembled operator (pinned to the
+    compiled ASMBLE_DRIVER): -S_BE u_E on 'B' rows, +S_EB u_E on 'E' rows."""
+    S = csr(g, oracle.asmble_z(m, mod, g, freq, py, pml)).tocoo()
+    cnp = np.frombuffer(g.cnp, dtype="S1")
+    sign = (np.where((cnp[S.row] == b"B") & (cnp[S.col] == b"E"), -1.0, 0.0)
+            + np.where((cnp[S.row] == b"E") & (cnp[S.col] == b"B"), 1.0, 0.0))
+    import scipy.sparse as sps
+    return sps.csr_matrix((S.data * sign, (S.row, S.col)), shape=S.shape) @ ue
+
+
+@pytest.mark.parametrize("name", ["pml", "p2_pml"])
+def test_bielak_sources_batched(name, torch_dev):
+    """The Bielak sources of a batch on the GPU: FILL_UWAVE_QMESH for four
+    plane waves (two back-azimuths, P-type polarisation) against the oracle
+    (itself bit-exact with the compiled routine) to 1e-15; ASMB_PEFF's
+    effective force for six (omega, ky) pairs at once against the FP64
+    assembled operator restricted by DOF classes (1e-12 per pair), and equal
+    to -M_B S M_E u + M_E S M_B u from the full batched product (1e-13)."""
+    torch = torch_dev
+    mi = _mi(name)
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    n = g.ndof
+    waves = [(2.1e-4, 3.3e-4, 0.0, 20.0), (-1.5e-4, 2.2e-4, 30.0, 35.0), (1.0e-4, 4.0e-4, 0.0, 10.0),
+             (3.0e-4, 1.0e-4, -45.0, 60.0)]
+    kx, kz, bz, ai = (np.array([w[i] for w in waves]) for i in range(4))
+    ue = torch.empty(n * len(waves), dtype=torch.complex128, device="cuda")
+    print(f"bielak {name}: ndof={n} waves={len(waves)}", flush=True)
+    mi.fill_uwave_dev(kx, kz, bz, ai, ue)
+    got = ue.cpu().numpy().reshape(n, len(waves))
+    for w, (a, b, c, d) in enumerate(waves):
+        ref = oracle.fill_uwave(m, g, a, b, c, d, model=mod, pml=pml)
+        assert np.abs(got[:, w] - ref).max() <= 1e-15, w
+    # six pairs: each of the first three waves at two (omega, ky)
+    freqs = np.array([0.05, 0.05, 0.08, 0.08, 0.11, 0.11])
+    pys = np.array([0.0, 1.2e-4, 0.0, 0.8e-4, 0.5e-4, 0.0])
+    om = TWOPI * freqs
+    ky = om * pys
+    src = [0, 1, 1, 2, 0, 2]
+    u = torch.stack([ue.view(n, len(waves))[:, k] for k in src], dim=1).contiguous()
+    peff = torch.empty_like(u)
+    mi.peff_batch_dev(om, ky, u, peff)
+    pe = peff.cpu().numpy()
+    uh = u.cpu().numpy()
+    for b in range(len(src)):
+        want = _peff_fp64(m, mod, g, pml, freqs[b], pys[b], uh[:, b])
+        assert relerr(pe[:, b], want) < 1e-12, b
+    cnp = np.frombuffer(g.cnp, dtype="S1")
+    mb = torch.from_numpy((cnp == b"B").astype(np.float64)).cuda()[:, None]
+    me = torch.from_numpy((cnp == b"E").astype(np.float64)).cuda()[:, None]
+    y1, y2 = torch.empty_like(u), torch.empty_like(u)
+    mi.apply_batch_dev(om, ky, (u * me).contiguous(), y1)
+    mi.apply_batch_dev(om, ky, (u * mb).contiguous(), y2)
+    alt = (-mb * y1 + me * y2).cpu().numpy()
+    assert relerr(pe, alt) < 1e-13
+    assert np.count_nonzero(pe[cnp == b"B"]) > 0 and np.count_nonzero(pe[cnp == b"E"]) > 0
+    assert not pe[(cnp != b"B") & (cnp != b"E")].any()
+
+
+def test_bielak_rhs_graph_replay(torch_dev):
+    """The plane-wave sources of a batch in one call (FILL_UWAVE_QMESH per
+    pair, then ASMB_PEFF): the eager launches equal the separate entry
+    points bit for bit, and the HIP-graph replays equal the eager result on
+    every replay, with eager products and synchronisations in between and
+    with new wave and frequency parameters staged into the same graph."""
+    torch = torch_dev
+    name = "pml"
+    mi = _mi(name)
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    n = g.ndof
+    rng = np.random.default_rng(98)
+    npair = 5
+
+    def params():
+        f = rng.uniform(0.03, 0.12, npair)
+        om = TWOPI * f
+        ky = om * rng.uniform(0.0, 1.5e-4, npair)
+        kx = om * rng.uniform(-1.0, 1.0, npair) / 6000.0
+        kz = om * rng.uniform(0.2, 1.0, npair) / 6000.0
+        return om, ky, kx, kz, rng.uniform(-60, 60, npair), rng.uniform(5, 60, npair)
+
+    ue = torch.empty(n * npair, dtype=torch.complex128, device="cuda")
+    pe = torch.empty_like(ue)
+    ue2, pe2 = torch.empty_like(ue), torch.empty_like(ue)
+    x = torch.randn(n * npair, dtype=torch.complex128, device="cuda")
+    y = torch.empty_like(x)
+    for trial in range(2):
+        om, ky, kx, kz, bz, ai = params()
+        print(f"bielak graph trial {trial}: ndof={n} npair={npair}", flush=True)
+        mi.bielak_rhs_dev(om, ky, kx, kz, bz, ai, ue, pe)
+        mi.fill_uwave_dev(kx, kz, bz, ai, ue2)
+        mi.peff_batch_dev(om, ky, ue2, pe2)
+        assert torch.equal(ue, ue2) and torch.equal(pe, pe2)
+        want_u, want_p = ue.clone(), pe.clone()
+        for rep in range(3):
+            ue.zero_()
+            pe.fill_(complex(float("nan"), 0.0))
+            mi.apply_batch_dev(om, ky, x, y)     # eager work between replays
+            torch.cuda.synchronize()
+            mi.bielak_rhs_dev(om, ky, kx, kz, bz, ai, ue, pe, graph=True)
+            assert torch.equal(ue, want_u), (trial, rep)
+            assert torch.equal(pe, want_p), (trial, rep)
+
+
 def test_gnmfree_batched_uneven():
     """GNMFREE's sum_j alpha_j Re(J_j^H J_j p) over 12 Jacobians with 20..120
     observations each, against the oracle restatement (FP64 both)."""

@@ -941,3 +941,44 @@ def test_covd_entry_point_argument_checks():
         assert L.f25_rec_space1d(4, p(bad, dp), p(out, dp)) == _lib.F25_ERR_ARG
         assert not out.any()
     assert L.f25_rec_space1d(1, p(x, dp), p(out, dp)) == _lib.F25_ERR_ARG
+
+
+# -- Bielak plane-wave sources ------------------------------------------------------
+
+@needs_ref
+@pytest.mark.parametrize("case", ["pml", "p2_pml", "p6"])
+def test_fill_uwave_vs_reference_and_closed_form(case):
+    """FILL_UWAVE_QMESH: the restatement bit for bit against the compiled
+    routine (back-azimuths 0 and 30 degrees, P-type polarisation), and the
+    plane wave pol * exp(-i (kx x + kz z)) at the GLL nodes placed by the
+    bilinear map, computed here independently (from the last element holding
+    a node, where the routine takes the first: the phase kx x + kz z of up
+    to ~10 rad moves by a few ulps), to 4e-15."""
+    m, mod, pml = mesh_case(case)
+    g = oracle_graph(case)
+    nl = m.nord + 1
+    xi, _ = oracle.gll(nl)
+    xyz = np.full((g.ndof, 2), np.nan)
+    comp = np.zeros(g.ndof, int)
+    lm = np.asarray(g.lm).reshape(m.nelem, nl * nl, 3)
+    for e in range(m.nelem):
+        cx = m.xlocs[m.ieng[e] - 1]
+        cz = m.zlocs[m.ieng[e] - 1]
+        for ie in range(nl):
+            for ix in range(nl):
+                a, b = xi[ix], xi[ie]
+                sf = 0.25 * np.array([(1 - a) * (1 - b), (1 + a) * (1 - b), (1 + a) * (1 + b), (1 - a) * (1 + b)])
+                for c in range(3):
+                    d = lm[e, ie * nl + ix, c]
+                    if d > 0:
+                        xyz[d - 1] = (sf @ cx, sf @ cz)
+                        comp[d - 1] = c
+    assert not np.isnan(xyz).any()
+    for kx, kz, baz, aoi in ((2.1e-4, 3.3e-4, 0.0, 20.0), (-1.5e-4, 2.2e-4, 30.0, 35.0)):
+        a = oracle.fill_uwave(m, g, kx, kz, baz, aoi, model=mod, pml=pml)
+        b = oracle.fill_uwave(m, g, kx, kz, baz, aoi, reference=True)
+        np.testing.assert_array_equal(a, b)
+        pol = np.array([np.cos(np.radians(baz)) * np.sin(np.radians(aoi)),
+                        np.sin(np.radians(baz)) * np.sin(np.radians(aoi)), np.cos(np.radians(aoi))])
+        want = pol[comp] * np.exp(-1j * (kx * xyz[:, 0] + kz * xyz[:, 1]))
+        assert np.abs(a - want).max() <= 4e-15
