@@ -129,6 +129,7 @@ def main() -> int:
     extra = {}
     if not args.skip_extras:
         extra = extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g)
+        extra["bielak_rhs"] = bielak_extra(L, torch)
         log(json.dumps(extra))
 
     traffic, traffic_src = profiled_traffic(args.config)
@@ -379,9 +380,55 @@ def extras(mi, mesh, model, L, torch, x, y, omega, ky, freq, py, npair, gdofs, g
     k_ms, wall = timed(lambda: _lib.check(L.f25_cadjlam_rhs_dev(*args), "cadjlam"), L, b"cadjlam", 3)
     kern["cadjlam_rhs"] = {"kernel_ms": round(k_ms, 3), "pairs": nf * ns,
                            "roofline": roof("hbm", nf * ns * ndof * 8 / (k_ms * 1e-3), "GB/s")}
+    # the same with the data covariance (LAPLACE_1DV2 + BPRESID_COVD4),
+    # element lengths from the receivers' spacing along the line
+    xr = np.sort(ar.uniform(0.0, 8e4, nr)) + np.arange(nr)
+    dx = np.zeros(nr - 1)
+    _lib.check(L.f25_rec_space1d(nr, xr.ctypes.data_as(_lib.dp), dx.ctypes.data_as(_lib.dp)), "rec_space1d")
+    cargs = (ndof, nf, nr, ns, nf * ns, ifq.ctypes.data_as(_lib.ip), isr.ctypes.data_as(_lib.ip),
+             fq.ctypes.data_as(_lib.dp), pyy.ctypes.data_as(_lib.dp), 20.0, 3, 0.0, 1, 0, 1,
+             dx.ctypes.data_as(_lib.dp), yr.ctypes.data_as(_lib.dp), mr.ctypes.data_as(_lib.ip),
+             C.c_void_p(ee.data_ptr()), C.c_void_p(oo.data_ptr()), C.cast(C.c_void_p(ww.data_ptr()), _lib.fp),
+             C.c_void_p(rr.data_ptr()), C.byref(iw))
+    k_ms, wall = timed(lambda: _lib.check(L.f25_cadjlam_rhs_covd_dev(*cargs), "cadjlam_covd"), L,
+                       b"cadjlam_covd", 3)
+    kern["cadjlam_rhs_covd"] = {"kernel_ms": round(k_ms, 3), "pairs": nf * ns,
+                                "roofline": roof("hbm", nf * ns * ndof * 8 / (k_ms * 1e-3), "GB/s")}
     del ee, oo, ww, rr
     out["kernels"] = kern
     return out
+
+
+def bielak_extra(L, torch):
+    """The Bielak sources of a batch on the configuration-4 mesh (200 x 100
+    elements of order 4, two PML rings and one Bielak ring, 963,603 DOFs),
+    10 frequencies x 24 ky = 240 pairs, one plane wave per pair: the wave
+    (FILL_UWAVE_QMESH) and its effective force (ASMB_PEFF), eager and as one
+    HIP graph.  HBM: the two dense [dof][pair] complex128 results written
+    (the ring's product is ~1.5% of a full one)."""
+    import math
+    from fem25_amd import configs as cf
+    from fem25_amd.api import MeshInfo
+    m = cf.structured_mesh(200, 100, 4, npml=2, nbielak=1)
+    mi = MeshInfo(m, cf.PML(rcoeff=1e-3, kappa=1.0, fcent=0.05))
+    mi.set_model(cf.layered_model(m, seed=2))
+    om, ky, f, py = cf.freq_ky_batch(10, 24, 0.02, 0.1, 6300.0 / math.sqrt(3.0))
+    n, npair = mi.ndof, om.size
+    rng = np.random.default_rng(12)
+    aoi = rng.uniform(10.0, 40.0, npair)
+    baz = rng.uniform(-30.0, 30.0, npair)
+    s = np.sin(np.radians(aoi)) / 6000.0
+    kx, kz = om * s, om * np.cos(np.radians(aoi)) / 6000.0
+    ue = torch.empty(n * npair, dtype=torch.complex128, device="cuda")
+    pe = torch.empty_like(ue)
+    res = {"workload": "200x100 p4, 2 PML + 1 Bielak rings, 240 pairs", "ndof": n}
+    for name, graph in (("eager", False), ("graph", True)):
+        k_ms, wall = timed(lambda: mi.bielak_rhs_dev(om, ky, kx, kz, baz, aoi, ue, pe, graph=graph), L,
+                           b"bielak_rhs", 5)
+        res[name] = {"ms": round(k_ms, 3), "wall_ms": round(wall, 3),
+                     "roofline": roof("hbm", 2 * 16.0 * n * npair / (k_ms * 1e-3), "GB/s")}
+    mi.close()
+    return res
 
 
 def only_kernel(args, mi, mesh, L, torch, omega, ky, freq, py, g):
