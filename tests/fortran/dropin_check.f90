@@ -97,9 +97,10 @@
       inv%NNPGL = HDR(7)
       inv%NZ_FDIST = NZF
       inv%NA35 = NA35
-      inv%NVINV = 1
       inv%VPVS = VPVS
       inv%CINVTYPE = CH(2)//CH(3)
+      inv%NVINV = 1
+      IF (inv%CINVTYPE == 'PS' .OR. inv%CINVTYPE == 'ps') inv%NVINV = 2
       ALLOCATE(inv%MCONN(NNPG,inv%NCON), inv%MASKG(NNPG), inv%MYGRAD(inv%NNPGL))
       ALLOCATE(inv%JCSC_FDIST(inv%NNPGL+1), inv%ICSC_FDIST(NZF), inv%ELEM_WTS(NELEM))
       READ(10) inv%MCONN

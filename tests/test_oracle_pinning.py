@@ -982,3 +982,63 @@ def test_fill_uwave_vs_reference_and_closed_form(case):
                         np.sin(np.radians(baz)) * np.sin(np.radians(aoi)), np.cos(np.radians(aoi))])
         want = pol[comp] * np.exp(-1j * (kx * xyz[:, 0] + kz * xyz[:, 1]))
         assert np.abs(a - want).max() <= 4e-15
+
+
+@pytest.mark.parametrize("kind", [b"PP", b"SS", b"PS"])
+def test_qp_p3_p1_p2_on_anchors_with_several_elements(kind):
+    """CADJ_NU_ALPHA's P3 and MET225_MOD's P1, P2 on anchors held by two or
+    four elements: the reference clears TEMP, T1 and T2 per element, so the
+    column is the last element of MCONN(INPG,:) alone.  Checked against
+    that element's operator assembled on its own (a one-element mesh with
+    the same anchors, orc_asmble_z, lambda/mu the anchor's derivative field,
+    rho = 0): P = -w Re sum over the element's DOFs of conj(S_e x) y."""
+    name = "tiny_layered"
+    m, mod, pml = mesh_case(name)
+    g = oracle_graph(name)
+    one = oracle.inv_setup(m, mod, g, pml)
+    io = ps_inv(one) if kind == b"PS" else one
+    nv = 2 if kind == b"PS" else 1
+    rng = np.random.default_rng(99)
+    freq, py = 0.07, 1.1e-4
+    cv = lambda: (rng.standard_normal(g.ndof) + 1j * rng.standard_normal(g.ndof)).astype(np.complex64)  # noqa: E731
+    u, lam, nu, alpha = cv(), cv(), cv(), cv()
+    v = rng.standard_normal(io.na35).astype(np.float32)
+    _, _, p3 = oracle.qp_nu_alpha(m, mod, g, io, freq, py, v, lam, u, cinvtype=kind, pml=pml)
+    p1, p2 = oracle.met225(m, mod, g, io, freq, py, lam, nu, alpha, u, cinvtype=kind, pml=pml)
+    nl = m.nord + 1
+    lm = np.asarray(g.lm).reshape(m.nelem, nl * nl, 3)
+    cnt = np.bincount(m.ieng.ravel() - 1, minlength=m.nnpg)
+    anchors = [a for a in range(m.nnpg) if one.maskg[a] > 0 and cnt[a] >= 2][:6]
+    assert len(anchors) >= 3
+    lam_, mu_, rho_ = mod.ecoeff[:, 0], mod.ecoeff[:, 1], mod.dens
+    vp, vs = np.sqrt((lam_ + 2 * mu_) / rho_), np.sqrt(mu_ / rho_)
+    for a in anchors:
+        e = one.mconn[:, a][one.mconn[:, a] > 0][-1] - 1
+        ie = m.ieng[e] - 1
+        m1 = cf.Mesh(nord=m.nord, nx=1, nz=1, ieng=np.array([[1, 2, 3, 4]], np.int32), xlocs=m.xlocs[ie].copy(),
+                     zlocs=m.zlocs[ie].copy(), cdomain=b"I", cnnpg=b"II" * 4, xd=np.zeros(4), zd=np.zeros(4))
+        g1 = oracle.graph(m1)
+        lm1 = np.asarray(g1.lm).reshape(1, nl * nl, 3)
+        k = int(np.nonzero(ie == a)[0][0])
+        dofs = lm[e].ravel() - 1
+        dofs1 = lm1[0].ravel() - 1
+        assert np.all(dofs >= 0) and np.all(dofs1 >= 0)
+        for iv in range(nv):
+            p_var = (kind == b"PP") or (kind == b"PS" and iv == 0)
+            fl, fm = np.zeros(4), np.zeros(4)
+            if p_var:
+                fl[k] = 2 * vp[a]
+            else:
+                fl[k], fm[k] = -4 * vs[a], 2 * vs[a]
+            mod1 = cf.Model(dens=np.zeros(4), ecoeff=np.asfortranarray(np.stack([fl, fm], axis=1)),
+                            vp=np.zeros(4), vs=np.zeros(4))
+            S = csr(g1, oracle.asmble_z(m1, mod1, g1, freq, py, cf.PML())).toarray()
+            Se = np.zeros((g1.ndof, g1.ndof), np.complex128)
+            Se[np.ix_(dofs1, dofs1)] = S[np.ix_(dofs1, dofs1)]
+            col = (one.maskg[a] - 1) * nv + iv
+            w = float(np.float32(io.elem_wts[e]))
+            for got, x, y, wv in ((p3[col], u, lam, w * float(v[col])), (p1[col], u, nu, w), (p2[col], alpha, lam, w)):
+                t = Se @ x[dofs].astype(np.complex128)
+                terms = np.conj(t) * y[dofs]
+                want = -wv * np.real(terms.sum())
+                assert abs(got - want) <= 2e-5 * abs(wv) * np.abs(terms).sum() + 1e-30, (kind, a, iv, got, want)
