@@ -28,6 +28,8 @@ def c5():
     mi.set_model(mod)
     yield m, mi
     mi.close()
+    import torch
+    torch.cuda.empty_cache()   # the 27 GB vectors stay cached by torch otherwise
 
 
 def test_config5_product_against_assembled_operator(c5):
