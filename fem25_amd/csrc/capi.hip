@@ -157,6 +157,7 @@ struct f25_mesh_s {
     int ncon_anchor = 0;
     // inversion (CFMAT/ADJFLAM)
     int *d_mygrad = nullptr, *d_jcsc = nullptr, *d_icsc = nullptr;
+    int *d_dab_pos = nullptr;   // [column][row of its last element]: position in ICSC_FDIST, -1 fixed
     double *d_ew = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // (omega, ky) staging, grown on demand
@@ -457,7 +458,7 @@ extern "C" int f25_mesh_destroy(f25_mesh *m) {
     dfree(m->d_jcptr); dfree(m->d_comp); dfree(m->d_v0); dfree(m->d_v1); dfree(m->d_v2);
     dfree(m->d_kind); dfree(m->d_cbase);
     dfree(m->d_part); dfree(m->d_mconn); dfree(m->d_pmap); dfree(m->d_fixa); dfree(m->d_qp); dfree(m->d_qs);
-    dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc);
+    dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_dab_pos);
     dfree(m->d_ew);
     dfree(m->d_omky);
     dfree(m->br.d_order); dfree(m->br.d_first0); dfree(m->br.d_brows); dfree(m->br.d_xz); dfree(m->br.d_comp);
@@ -1649,7 +1650,7 @@ extern "C" int f25_gradient(f25_mesh *m, int ntrip, const double *omega, const d
 // ---------------------------------------------------------------------------
 static int inv_upload(f25_mesh *m) {
     m->have_inv = false;
-    dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_ew);
+    dfree(m->d_mygrad); dfree(m->d_jcsc); dfree(m->d_icsc); dfree(m->d_ew); dfree(m->d_dab_pos);
     TRY(upload(&m->d_mygrad, m->inv.mygrad.data(), m->inv.mygrad.size(), m->st));
     TRY(upload(&m->d_jcsc, m->inv.jcsc.data(), m->inv.jcsc.size(), m->st));
     TRY(upload(&m->d_icsc, m->inv.icsc.data(), m->inv.icsc.size(), m->st));
@@ -2137,6 +2138,32 @@ static DabKernel dab_kernel_t(int nl) {
 // DABMKISOB25 form (imaginary ky coupling) or DKM_VPVS form (real)
 static DabKernel dab_kernel(int nl, bool imag = true) { return imag ? dab_kernel_t<true>(nl) : dab_kernel_t<false>(nl); }
 
+// Positions of each column's element rows in its CSC pattern, for the
+// column items of k_dab_sf (modes 1 and 2): the last element of the
+// column's anchor, as the kernels take it.  A row absent from the pattern
+// is the error the reference reports ('cant find inzrow', qpmfree.f90:819).
+static int ensure_dab_pos(f25_mesh *m) {
+    if (m->d_dab_pos) return 0;
+    const InvInfo &inv = m->inv;
+    const int nee = 3 * m->tab.nen;
+    std::vector<int> ce, ck;
+    TRY(last_elements(m, ce, ck));
+    std::vector<int> pos((size_t)inv.nnpgl * nee, -1);
+    for (int c = 0; c < inv.nnpgl; ++c) {
+        const int *b = inv.icsc.data() + (inv.jcsc[c] - 1), *en = inv.icsc.data() + (inv.jcsc[c + 1] - 1);
+        for (int j = 0; j < nee; ++j) {
+            const int d = m->g.lm[(size_t)ce[c] * nee + j];
+            if (d <= 0) continue;
+            const int *f = std::lower_bound(b, en, d);
+            if (f == en || *f != d) FAIL(F25_ERR_PATTERN, "an element row is missing from its column's CSC pattern");
+            pos[(size_t)c * nee + j] = (int)(f - inv.icsc.data());
+        }
+    }
+    TRY(upload(&m->d_dab_pos, pos.data(), pos.size(), m->st));
+    HIPCHK(hipStreamSynchronize(m->st));
+    return 0;
+}
+
 static int dab_prepare(f25_mesh *m) {
     if (!m->have_inv) FAIL(F25_ERR_STATE, "f25_inv_setup has not been called");
     if (!m->have_model) FAIL(F25_ERR_STATE, "f25_set_model has not been called");
@@ -2148,7 +2175,7 @@ static int dab_prepare(f25_mesh *m) {
         for (int im = 0; im < 2; ++im)
             HIPCHK(hipFuncSetAttribute((const void *)dab_kernel(m->tab.nl, im == 1),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)dab_lds(m)));
-    return 0;
+    return ensure_dab_pos(m);
 }
 
 // per-column items (mode 1): the last element, coefficient on its corner
@@ -2191,7 +2218,7 @@ static int dab_launch_columns(f25_mesh *m, double kv, int nvec, const std::vecto
         hipLaunchKernelGGL(dab_kernel(m->tab.nl), dim3(inv.nnpgl), dim3(dab_threads(m)), dab_lds(m), m->st, 1,
                            inv.nnpgl, nvec, kp, ks, kv, de, dc, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng,
                            m->d_xl, m->d_zl, m->d_lm, x0, x1, (double2 *)nullptr, (double2 *)nullptr, z0, z1,
-                           m->d_mygrad, m->d_jcsc, m->d_icsc, p0, p1, (float2 *)nullptr, m->d_err);
+                           m->d_mygrad, m->d_jcsc, m->d_dab_pos, p0, p1, (float2 *)nullptr, m->d_err);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }
     }
@@ -2243,7 +2270,7 @@ extern "C" int f25_cfmat_dist(f25_mesh *m, double freq, double py, const void *w
                            kp, ks, kv,
                            de, dc, m->d_t1d, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, dw,
                            (const float2 *)nullptr, (double2 *)nullptr, (double2 *)nullptr, (const float2 *)nullptr,
-                           (const float2 *)nullptr, m->d_mygrad, m->d_jcsc, m->d_icsc, (float *)nullptr,
+                           (const float2 *)nullptr, m->d_mygrad, m->d_jcsc, m->d_dab_pos, (float *)nullptr,
                            (float *)nullptr, df, m->d_err);
         (void)hipEventRecord(m->ev1, m->st);
         hipError_t e = hipGetLastError();
@@ -2317,7 +2344,7 @@ extern "C" int f25_qp_nu_alpha(f25_mesh *m, double freq, double py, const float 
                                kp, ks, kv, de + o, dc + 8 * (size_t)o,
                                m->d_t1d, m->d_wq, m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_lm, du, dl, ya, yb,
                                (const float2 *)nullptr, (const float2 *)nullptr, m->d_mygrad, m->d_jcsc,
-                               m->d_icsc, (float *)nullptr, (float *)nullptr, (float2 *)nullptr, m->d_err);
+                               m->d_dab_pos, (float *)nullptr, (float *)nullptr, (float2 *)nullptr, m->d_err);
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { g_err = hipGetErrorString(e); r = F25_ERR_HIP; }

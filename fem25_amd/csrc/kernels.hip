@@ -2161,21 +2161,22 @@ __global__ void k_adjflam(int nnpgl, const int *__restrict__ mygrad, const int *
 //          p_v(MYGRAD(c)) = -sum over the element's rows Re(conj(M x_v) z_v);
 //   mode 2 (item = local column c): column c of F in CSC order, COMPLEX*8,
 //          zero outside the element's rows.
-// Every row of modes 1 and 2 is checked against column c's CSC pattern.
+// The rows of modes 1 and 2 come with their positions in column c's CSC
+// pattern (rpos[c][row], built and checked on the host once per inversion
+// structure: a per-row search of the pattern in the kernel was a chain of
+// dependent loads).
 // Fluxes per (vector, q), 8 complex: comp 1 N, Nx, Nz; comp 2 N, Nx, Nz;
 // comp 3 N, Nz (comp 3's Nx flux equals comp 1's Nz flux).
 // Row outputs of one item (rows are threads 0..nee-1, row = -1 for
 // the others): mode 0 adds into y, mode 1 contracts with z into p(MYGRAD),
-// mode 2 writes column it of F; every row of modes 1 and 2 is looked up in
-// column it's CSC pattern (err 3 if absent).
+// mode 2 writes column it of F at the rows' CSC positions (err 3 if the
+// table has none).
 __device__ inline void dab_epilogue(int mode, int nvec, int it, int row, const int *__restrict__ L,
-                                    const double2 *val, const int *__restrict__ jcsc, const int *__restrict__ icsc,
+                                    const double2 *val, const int *__restrict__ rpos,
                                     const int *__restrict__ mygrad, double2 *__restrict__ y0,
                                     double2 *__restrict__ y1, const float2 *__restrict__ z0,
                                     const float2 *__restrict__ z1, float *__restrict__ p0, float *__restrict__ p1,
                                     float2 *__restrict__ fout, int *__restrict__ err, double (*red)[8]) {
-    int i1 = 0, i2 = 0;
-    if (mode != 0) { i1 = jcsc[it] - 1; i2 = jcsc[it + 1] - 1; }
     double acc[2] = {0.0, 0.0};
     const int dof = row >= 0 ? L[row] : -1;
     if (dof >= 0) {
@@ -2183,12 +2184,10 @@ __device__ inline void dab_epilogue(int mode, int nvec, int it, int row, const i
             y0[dof].x += val[0].x; y0[dof].y += val[0].y;
             if (nvec > 1) { y1[dof].x += val[1].x; y1[dof].y += val[1].y; }
         } else {
-            int lo = i1, hi = i2 - 1, pos = -1, key = dof + 1;
-            while (lo <= hi) {
-                const int mid = (lo + hi) >> 1;
-                if (icsc[mid] == key) { pos = mid; break; }
-                if (icsc[mid] < key) lo = mid + 1; else hi = mid - 1;
-            }
+            // the row's position in column it's CSC pattern, from the host's
+            // table (rpos: this item's rows), which has checked that every
+            // row is in the pattern
+            const int pos = rpos[row];
             if (pos < 0) {
                 *err = 3;
             } else if (mode == 2) {
@@ -2241,7 +2240,7 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
                                                 const float2 *__restrict__ x1, double2 *__restrict__ y0,
                                                 double2 *__restrict__ y1, const float2 *__restrict__ z0,
                                                 const float2 *__restrict__ z1, const int *__restrict__ mygrad,
-                                                const int *__restrict__ jcsc, const int *__restrict__ icsc,
+                                                const int *__restrict__ jcsc, const int *__restrict__ rpos,
                                                 float *__restrict__ p0, float *__restrict__ p1,
                                                 float2 *__restrict__ fout, int *__restrict__ err) {
     constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen, QQ = Q1 * Q1;
@@ -2399,7 +2398,8 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
             if (v == 0) val[0] = r; else val[1] = r;
         }
     }
-    dab_epilogue(mode, nvec, it, orow, L, val, jcsc, icsc, mygrad, y0, y1, z0, z1, p0, p1, fout, err, red);
+    dab_epilogue(mode, nvec, it, orow, L, val, mode ? rpos + (size_t)it * nee : nullptr, mygrad, y0, y1, z0, z1, p0,
+                 p1, fout, err, red);
 }
 
 // ---------------------------------------------------------------------------
