@@ -2109,12 +2109,13 @@ static int ensure_dab_part(f25_mesh *m) {
 }
 
 // LDS of k_dab_sf (independent of the number of vectors): the element's
-// vectors (room for two), the geometry per quadrature point, T1/T2 (whose
-// space the projection's first-pass sums reuse) and the projection fields
-// A, B, C
+// vectors (room for two), T1/T2 (whose space the projection's first-pass
+// sums reuse), the projection fields A, B, C at the quadrature points and
+// the 1-D tables v, v'  (65.4 KB at p6: two workgroups per CU)
 static size_t dab_lds(const f25_mesh *m) {
     const int nee = 3 * m->tab.nen, nq = m->tab.nq, q1 = m->tab.nq1, n1 = m->tab.nl;
-    return (size_t)2 * nee * sizeof(double2) + ((size_t)3 * q1 * n1 * 3 + 9 * (size_t)nq) * sizeof(double2);
+    return (size_t)2 * nee * sizeof(double2) + ((size_t)3 * q1 * n1 * 3 + 9 * (size_t)nq) * sizeof(double2) +
+           (size_t)2 * n1 * q1 * sizeof(double);
 }
 // threads of a k_dab_sf workgroup: five waves at p6 (Q1 = 17)
 static int dab_threads(const f25_mesh *m) { return m->tab.nq1 > 16 ? 320 : 256; }

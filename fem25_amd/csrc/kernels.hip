@@ -1507,6 +1507,19 @@ __device__ __forceinline__ void sf_row_sums(double2 &U, double2 &UX, double2 &UE
      ...);
 }
 
+// The same sums with the lane's 1-D values read from an LDS table instead
+// of registers: sv[j Q1] = v_j(xi), sv[(NL + j) Q1] = v'_j(xi) at the
+// lane's xi point.
+template <int NL, int Q1, int... J>
+__device__ __forceinline__ void sf_row_sums_lds(double2 &U, double2 &UX, double2 &UE, const double2 &T1,
+                                                const double2 &T2, const double *sv,
+                                                std::integer_sequence<int, J...>) {
+    ((fmac_bcast<J>(U.x, T1.x, sv[J * Q1]), fmac_bcast<J>(U.y, T1.y, sv[J * Q1]),
+      fmac_bcast<J>(UX.x, T1.x, sv[(NL + J) * Q1]), fmac_bcast<J>(UX.y, T1.y, sv[(NL + J) * Q1]),
+      fmac_bcast<J>(UE.x, T2.x, sv[J * Q1]), fmac_bcast<J>(UE.y, T2.y, sv[J * Q1])),
+     ...);
+}
+
 // R = double: FP64 fields and arithmetic.  R = float: complex64 fields, the
 // interpolation and the densities of each triple in FP32, their sums over
 // triples, quadrature points and corners in FP64 (the FP32 gradient); its
@@ -2228,7 +2241,11 @@ __device__ inline void dab_epilogue(int mode, int nvec, int it, int row, const i
 //   sum_eta v_j(eta) (PA + PC)[eta][i] + v'_j(eta) PB[eta][i],
 //   PA = sum_xi v'_i(xi) A,  PB = sum_xi v_i(xi) B,  PC = sum_xi v_i(xi) C.
 // About 4x fewer multiply-adds than the direct sums over the element
-// nodes and the quadrature points.
+// nodes and the quadrature points.  The 1-D tables v, v' sit in LDS, which
+// keeps the kernel at 120 registers: four waves per SIMD, so two five-wave
+// workgroups fit per CU at p6 (a workgroup holds ceil(waves / 4) slots on
+// every SIMD; at three waves per SIMD one fits) and four four-wave ones at
+// p4.
 template <int NL, bool IMAG>
 __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mode, int nitems, int nvec, double kp, double ks,
                                                 double kv, const int *__restrict__ item_elem,
@@ -2257,6 +2274,7 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
     // the point stage, PA.. first written in the projection after it
     double2 *sP = sT;
     double2 *sH = sT + 3 * Q1 * N1 * 3;                     // [3 comp][3: A, B, C][QQ]
+    double *sb = reinterpret_cast<double *>(sH + 9 * QQ);    // [2][N1][Q1]: v_k(q), v'_k(q)
     const int it = blockIdx.x, tid = threadIdx.x;
     if (it >= nitems) return;
     const int e = item_elem[it];
@@ -2265,7 +2283,8 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
     const double c0 = ic[0], c1 = ic[1], c2 = ic[2], c3 = ic[3], s0 = ic[4], s1 = ic[5], s2 = ic[6], s3 = ic[7];
     const int *ie = ieng + 4 * e;
     const int *L = lm + (size_t)e * nee;
-    const double *v1 = t1d, *d1 = t1d + N1 * Q1;   // v1[k * Q1 + q]
+    for (int i = tid; i < 2 * N1 * Q1; i += NT) sb[i] = t1d[i];
+    const double *v1 = sb, *d1 = sb + N1 * Q1;   // v1[k * Q1 + q]
     if (mode == 2)
         for (int r = jcsc[it] - 1 + tid; r < jcsc[it + 1] - 1; r += NT) fout[r] = make_float2(0.f, 0.f);
     for (int i = tid; i < nee; i += NT) {
@@ -2283,8 +2302,8 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
     const bool act = (row16 < Q1 && x16 < Q1) || ovl;
     const int qe = ovl ? tid - 16 * Q1 : min(row16, Q1 - 1), qx = ovl ? 16 : min(x16, Q1 - 1);
     // the inverse Jacobian and the collocated density D at this lane's point,
-    // in registers (no LDS table: a smaller workgroup footprint keeps more
-    // workgroups per CU)
+    // in registers (an LDS table of the points would keep fewer workgroups
+    // per CU)
     double rx, sx, rz, sz, D, Ds;
     {
         Geo g;
@@ -2294,9 +2313,6 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
         Ds = wq[qx] * wq[qe] * g.det * (s0 * g.sf[0] + s1 * g.sf[1] + s2 * g.sf[2] + s3 * g.sf[3]);
     }
     const int ni = min(x16, N1 - 1);
-    double vx[N1], dx[N1];
-#pragma unroll
-    for (int i = 0; i < N1; ++i) { vx[i] = v1[i * Q1 + qx]; dx[i] = d1[i * Q1 + qx]; }
     const double k2 = kv * kv;
     // output row of this thread: node (j, i) = (a / N1, a % N1), component cc
     const int orow = tid < nee ? tid : -1;
@@ -2327,15 +2343,16 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
                 const double2 T1 = o[0], T2 = o[1];
                 double2 u0 = make_double2(0.0, 0.0), uxi = u0, uet = u0;
                 if (!ovl) {
-                    sf_row_sums<NL, 0>(u0, uxi, uet, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+                    sf_row_sums_lds<NL, Q1>(u0, uxi, uet, T1, T2, v1 + qx, std::make_integer_sequence<int, N1>());
                 } else {   // the xi = 16 column reads the row's T1, T2 from LDS
                     const double2 *r = o - 2 * ni;
 #pragma unroll
                     for (int j = 0; j < N1; ++j) {
                         const double2 a1 = r[2 * j], a2 = r[2 * j + 1];
-                        u0.x += vx[j] * a1.x; u0.y += vx[j] * a1.y;
-                        uxi.x += dx[j] * a1.x; uxi.y += dx[j] * a1.y;
-                        uet.x += vx[j] * a2.x; uet.y += vx[j] * a2.y;
+                        const double vj = v1[j * Q1 + qx], dj = d1[j * Q1 + qx];
+                        u0.x += vj * a1.x; u0.y += vj * a1.y;
+                        uxi.x += dj * a1.x; uxi.y += dj * a1.y;
+                        uet.x += vj * a2.x; uet.y += vj * a2.y;
                     }
                 }
                 U[c] = u0;
