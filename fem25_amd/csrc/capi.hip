@@ -1545,6 +1545,18 @@ extern "C" int f25_apply_batch(f25_mesh *m, int npair, const double *omega, cons
 }
 
 // ---------------------------------------------------------------------------
+// threads of a k_grad_sf workgroup (its launch shape for the order)
+template <typename R>
+static int grad_threads(int nl) {
+    switch (nl) {
+        case 3: return GradOcc<3, R>::NT;
+        case 4: return GradOcc<4, R>::NT;
+        case 5: return GradOcc<5, R>::NT;
+        case 6: return GradOcc<6, R>::NT;
+        default: return GradOcc<7, R>::NT;
+    }
+}
+
 template <typename R>
 static int grad_launch(f25_mesh *m, int ntrip, const double *d_om, const double *d_ky, const void *u, const void *l) {
     typedef typename CxOf<R>::T C2;
@@ -1553,16 +1565,16 @@ static int grad_launch(f25_mesh *m, int ntrip, const double *d_om, const double 
     typedef void (*GradSf)(const double *, const double *, const double *, const int *, const double *,
                            const double *, const char *, const int *, int, int64_t, const double *,
                            const double *, const C2 *, const C2 *, double *, int *);
-    // trips per pass: with two, four workgroups fit per CU at p4 (LDS and
-    // registers; 200x100, 128 triples: 9.4 ms)
-    constexpr int TS = 2;
+    // trips per pass (GradOcc::T): with two, four workgroups fit per CU at
+    // p4 (LDS and registers; 200x100, 128 triples: 9.4 ms)
     GradSf ksf = nullptr;
+    int TS = 0;
     switch (m->tab.nl) {
-        case 3: ksf = k_grad_sf<3, TS, R>; break;
-        case 4: ksf = k_grad_sf<4, TS, R>; break;
-        case 5: ksf = k_grad_sf<5, TS, R>; break;
-        case 6: ksf = k_grad_sf<6, TS, R>; break;
-        case 7: ksf = k_grad_sf<7, TS, R>; break;
+        case 3: ksf = k_grad_sf<3, GradOcc<3, R>::T, R>; TS = GradOcc<3, R>::T; break;
+        case 4: ksf = k_grad_sf<4, GradOcc<4, R>::T, R>; TS = GradOcc<4, R>::T; break;
+        case 5: ksf = k_grad_sf<5, GradOcc<5, R>::T, R>; TS = GradOcc<5, R>::T; break;
+        case 6: ksf = k_grad_sf<6, GradOcc<6, R>::T, R>; TS = GradOcc<6, R>::T; break;
+        case 7: ksf = k_grad_sf<7, GradOcc<7, R>::T, R>; TS = GradOcc<7, R>::T; break;
         default: break;
     }
     if (!ksf) FAIL(F25_ERR_ARG, "unsupported order");
@@ -1574,8 +1586,7 @@ static int grad_launch(f25_mesh *m, int ntrip, const double *d_om, const double 
     if (lsf > 64 * 1024)
         HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
     HIPCHK(hipEventRecord(m->ev0, m->st));
-    // Q1 = 17 (p6) adds a column of 17 lanes to the 17 rows: five waves
-    hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(q1 > 16 ? 320 : 256), lsf, m->st, m->d_t1d, m->d_wq,
+    hipLaunchKernelGGL(ksf, dim3(m->d.nelem), dim3(grad_threads<R>(nl)), lsf, m->st, m->d_t1d, m->d_wq,
                        m->d_xq, m->d_ieng, m->d_xl, m->d_zl, m->d_cdom, m->d_lm, ntrip, (int64_t)m->g.ndof,
                        d_om, d_ky, (const C2 *)u, (const C2 *)l, m->d_part, m->d_err);
     return 0;

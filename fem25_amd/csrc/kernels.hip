@@ -1497,7 +1497,9 @@ __global__ void k_peff(int ndofl, const int *__restrict__ mydofs, const int64_t 
 // symmetrised form before the adjoint field's components are formed one at
 // a time, and the quadrature weights are recomputed for the projection
 // instead of kept in LDS: 128 registers and 37 KB of LDS per workgroup at
-// p4, four workgroups per CU.
+// p4, four workgroups per CU.  At p6 the FP64 form takes two points per
+// lane and one triple per pass: three waves, 168 registers and 29.5 KB of
+// LDS, three workgroups per CU.
 template <int NL, int I, int... J>
 __device__ __forceinline__ void sf_row_sums(double2 &U, double2 &UX, double2 &UE, const double2 &T1,
                                             const double2 &T2, const double *vx, const double *dx,
@@ -1528,16 +1530,50 @@ template <typename R> struct CxOf;
 template <> struct CxOf<double> { typedef double2 T; };
 template <> struct CxOf<float> { typedef float2 T; };
 
-// Launch bounds: FP64 four workgroups per CU up to p4 (three waves per
-// SIMD above), FP32 five waves per SIMD (its registers allow it).
+// a[p] and a[p] += v for a register array of one or two entries and a
+// run-time p, by selects (an indexed register array would be placed in
+// scratch memory)
+template <typename V, int N>
+__device__ __forceinline__ V sel(const V (&a)[N], int p) {
+    static_assert(N == 1 || N == 2, "one or two points per lane");
+    if constexpr (N == 1) return a[0]; else return p ? a[1] : a[0];
+}
+template <int N>
+__device__ __forceinline__ void add_sel(double (&a)[N], int p, double v) {
+    static_assert(N == 1 || N == 2, "one or two points per lane");
+    if constexpr (N == 1) {
+        a[0] += v;
+    } else {
+        const double a0 = a[0] + v, a1 = a[1] + v;
+        a[0] = p ? a[0] : a0;
+        a[1] = p ? a1 : a[1];
+    }
+}
+
+// Launch shape.  One point per lane: rows of 16 lanes over the eta points
+// (at Q1 = 17 plus a column of 17 lanes for xi point 16), four waves up to
+// p5 and five at p6.  The FP64 form at p6 takes two points per lane: row r
+// (r < H = 9) holds eta points r and r + H, and nine lanes after the rows
+// hold the xi = 16 column's pairs; three waves.  A workgroup's waves are
+// placed with ceil(waves / 4) of them on every SIMD, so at the FP64 form's
+// three waves per SIMD one five-wave workgroup fits per CU, three
+// three-wave ones.  Up to p4 FP64 keeps four workgroups per CU; FP32 five
+// waves per SIMD (its registers allow it).
 template <int NL, typename R>
 struct GradOcc {
-    static constexpr int NT = 2 * NL + 3 > 16 ? 320 : 256;
-    static constexpr int MINB = std::is_same<R, float>::value ? 20 * 64 / NT : (NL <= 5 ? 4 : 3);
+    static constexpr int Q1 = 2 * NL + 3;
+    static constexpr bool F64 = std::is_same<R, double>::value;
+    static constexpr int PPL = (F64 && Q1 > 16) ? 2 : 1;   // quadrature points per lane
+    static constexpr int H = (Q1 + PPL - 1) / PPL;         // eta rows per point slot
+    static constexpr int NT = PPL == 2 ? 192 : (Q1 > 16 ? 320 : 256);
+    // waves per SIMD the registers must allow
+    static constexpr int WPE = !F64 ? 5 : (NL <= 5 ? 4 : 3);
+    // triples per pass
+    static constexpr int T = PPL == 2 ? 1 : 2;
 };
 
 template <int NL, int T, typename R = double>
-__global__ __launch_bounds__((GradOcc<NL, R>::NT), (GradOcc<NL, R>::MINB)) void k_grad_sf(
+__global__ __launch_bounds__((GradOcc<NL, R>::NT)) __attribute__((amdgpu_waves_per_eu(GradOcc<NL, R>::WPE))) void k_grad_sf(
     const double *__restrict__ t1d, const double *__restrict__ wq, const double *__restrict__ xq,
     const int *__restrict__ ieng, const double *__restrict__ xl, const double *__restrict__ zl,
     const char *__restrict__ cdom, const int *__restrict__ lm, int ntrip, int64_t ndof,
@@ -1546,10 +1582,11 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT), (GradOcc<NL, R>::MINB)) void 
     typedef typename CxOf<R>::T C2;
     constexpr bool F64 = std::is_same<R, double>::value;
     constexpr int N1 = NL, Q1 = 2 * NL + 3, nen = NL * NL, nee = 3 * nen;
-    // Q1 = 17 (p6): rows 0..16 of 16 lanes hold xi points 0..15 and the 17
-    // lanes after them hold xi point 16 of eta rows 0..16 (five waves)
+    // Q1 = 17 (p6): rows of 16 lanes hold xi points 0..15 of H eta points
+    // each (PPL per lane) and the H lanes after them xi point 16
     constexpr bool OVF = Q1 > 16;
-    constexpr int NT = OVF ? 320 : 256;
+    constexpr int PPL = GradOcc<NL, R>::PPL, H = GradOcc<NL, R>::H, NT = GradOcc<NL, R>::NT;
+    static_assert(16 * H + (OVF ? H : 0) <= NT, "lanes for every point slot");
     constexpr int NW = T * 2 * nee;            // element values gathered per pass
     constexpr int NG = (NW + NT - 1) / NT;     // of them per thread
     static_assert(Q1 <= 17, "one row of 16 lanes per eta point, plus one column");
@@ -1603,21 +1640,40 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT), (GradOcc<NL, R>::MINB)) void 
     // second stage: row = eta point, lane = xi point (clamped when idle: an
     // idle lane computes a copy and contributes nothing)
     const int row = tid >> 4, x16 = tid & 15;
-    const bool ovl = OVF && tid >= 16 * Q1 && tid < 17 * Q1;   // a lane of the xi = 16 column
-    const bool act = (row < Q1 && x16 < Q1) || ovl;
-    const int qe = ovl ? tid - 16 * Q1 : min(row, Q1 - 1), qx = ovl ? 16 : min(x16, Q1 - 1);
-    const int ni = min(x16, N1 - 1);
-    R vx[N1], dx[N1];
+    const bool ovl = OVF && tid >= 16 * H && tid < 16 * H + H;   // a lane of the xi = 16 column
+    const int e0 = ovl ? tid - 16 * H : min(row, H - 1), qx = ovl ? 16 : min(x16, Q1 - 1);
+    // the lane's points: eta e0 + p H (clamped copies where idle)
+    int qe[PPL];
+    bool act[PPL];
 #pragma unroll
-    for (int i = 0; i < N1; ++i) { vx[i] = (R)t1d[i * Q1 + qx]; dx[i] = (R)t1d[(N1 + i) * Q1 + qx]; }
-    // FP64: the inverse Jacobian at this lane's point, kept in registers
-    R rx = R(0), sx = R(0), rz = R(0), sz = R(0);
-    if constexpr (F64) {
-        Geo g;
-        if (geo_at(xq[qx], xq[qe], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
-        rx = g.dzdeta / g.det; sx = -g.dzdxi / g.det; rz = -g.dxdeta / g.det; sz = g.dxdxi / g.det;
+    for (int p = 0; p < PPL; ++p) {
+        qe[p] = min(e0 + p * H, Q1 - 1);
+        act[p] = ((row < H && x16 < Q1) || ovl) && e0 + p * H < Q1;
     }
-    double Dl = 0.0, Dm = 0.0, Dr = 0.0;
+    const int ni = min(x16, N1 - 1);
+    // the 1-D values at the lane's xi point: in registers, or with two
+    // points per lane read from the LDS table s1 (registers are that form's
+    // occupancy limit)
+    constexpr bool XL = PPL == 2;
+    R vx[N1], dx[N1];
+    if constexpr (!XL) {
+#pragma unroll
+        for (int i = 0; i < N1; ++i) { vx[i] = (R)t1d[i * Q1 + qx]; dx[i] = (R)t1d[(N1 + i) * Q1 + qx]; }
+    }
+    // FP64: the inverse Jacobian at the lane's points, kept in registers
+    R rx[PPL], sx[PPL], rz[PPL], sz[PPL];
+#pragma unroll
+    for (int p = 0; p < PPL; ++p) {
+        rx[p] = sx[p] = rz[p] = sz[p] = R(0);
+        if constexpr (F64) {
+            Geo g;
+            if (geo_at(xq[qx], xq[qe[p]], ie, xl, zl, g)) { *err = 2; g.det = 1.0; }
+            rx[p] = g.dzdeta / g.det; sx[p] = -g.dzdxi / g.det; rz[p] = -g.dxdeta / g.det; sz[p] = g.dxdxi / g.det;
+        }
+    }
+    double Dl[PPL], Dm[PPL], Dr[PPL];
+#pragma unroll
+    for (int p = 0; p < PPL; ++p) Dl[p] = Dm[p] = Dr[p] = 0.0;
     for (int t0 = 0; t0 < ntrip; t0 += T) {
         // no barrier before these writes: every wave passed the previous
         // pass's second barrier after its last read of sw, and the pass's
@@ -1684,16 +1740,22 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT), (GradOcc<NL, R>::MINB)) void 
         __syncthreads();
         if (t0 + T < ntrip) gather(t0 + T);
         if constexpr (!F64) {
-            const R *G = sG + 4 * (qx + Q1 * qe);
-            rx = G[0]; sx = G[1]; rz = G[2]; sz = G[3];
+            const R *G = sG + 4 * (qx + Q1 * qe[0]);
+            rx[0] = G[0]; sx[0] = G[1]; rz[0] = G[2]; sz[0] = G[3];
         }
 #pragma unroll
         for (int tt = 0; tt < T; ++tt) {
-            if (t0 + tt >= ntrip) break;
+          if (t0 + tt >= ntrip) break;
+          // the lane's points one after the other (a loop, not unrolled: two
+          // points' live values at once would exceed the registers)
+#pragma unroll 1
+          for (int p = 0; p < PPL; ++p) {
             const R k = kt[tt], w = wt[tt];
+            const int qp = sel(qe, p);
+            const R rxp = sel(rx, p), sxp = sel(sx, p), rzp = sel(rz, p), szp = sel(sz, p);
             // gradient of component c of field f at this point: [d/dx, d/dy, d/dz]
             auto grad = [&](int f, int c, C2 &U, C2 *Gc) {
-                const C2 *o = sT + (size_t)((((tt * 2 + f) * 3 + c) * Q1 + qe) * N1 + ni) * 2;
+                const C2 *o = sT + (size_t)((((tt * 2 + f) * 3 + c) * Q1 + qp) * N1 + ni) * 2;
                 const C2 T1 = o[0], T2 = o[1];
                 C2 UXI = {R(0), R(0)}, UET = UXI;
                 U = UXI;
@@ -1702,7 +1764,12 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT), (GradOcc<NL, R>::MINB)) void 
                 // from LDS
                 bool bc = !ovl;
                 if constexpr (F64) {
-                    if (bc) sf_row_sums<NL, 0>(U, UXI, UET, T1, T2, vx, dx, std::make_integer_sequence<int, N1>());
+                    if (bc && XL)
+                        sf_row_sums_lds<NL, Q1>(U, UXI, UET, T1, T2, (const double *)s1 + qx,
+                                                std::make_integer_sequence<int, N1>());
+                    else if (bc)
+                        sf_row_sums<NL, 0>(U, UXI, UET, T1, T2, (const double *)vx, (const double *)dx,
+                                           std::make_integer_sequence<int, N1>());
                 } else {
                     bc = false;
                 }
@@ -1711,14 +1778,15 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT), (GradOcc<NL, R>::MINB)) void 
 #pragma unroll
                     for (int j = 0; j < N1; ++j) {
                         const C2 a1 = r[2 * j], a2 = r[2 * j + 1];
-                        U.x += vx[j] * a1.x; U.y += vx[j] * a1.y;
-                        UXI.x += dx[j] * a1.x; UXI.y += dx[j] * a1.y;
-                        UET.x += vx[j] * a2.x; UET.y += vx[j] * a2.y;
+                        const R vj = XL ? s1[j * Q1 + qx] : vx[j], dj = XL ? s1[(N1 + j) * Q1 + qx] : dx[j];
+                        U.x += vj * a1.x; U.y += vj * a1.y;
+                        UXI.x += dj * a1.x; UXI.y += dj * a1.y;
+                        UET.x += vj * a2.x; UET.y += vj * a2.y;
                     }
                 }
-                Gc[0] = C2{rx * UXI.x + sx * UET.x, rx * UXI.y + sx * UET.y};
+                Gc[0] = C2{rxp * UXI.x + sxp * UET.x, rxp * UXI.y + sxp * UET.y};
                 Gc[1] = C2{k * U.y, -k * U.x};   // d/dy = -i ky
-                Gc[2] = C2{rz * UXI.x + sz * UET.x, rz * UXI.y + sz * UET.y};
+                Gc[2] = C2{rzp * UXI.x + szp * UET.x, rzp * UXI.y + szp * UET.y};
             };
             // forward field: its values, divergence and symmetrised gradient
             // S[i][j] = G_ij + G_ji; then the adjoint field's components one
@@ -1743,27 +1811,33 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT), (GradOcc<NL, R>::MINB)) void 
                 for (int j = 0; j < 3; ++j) sm2 += GV[j].x * S[c][j].x + GV[j].y * S[c][j].y;
                 vu += V.x * Uv[c].x + V.y * Uv[c].y;
             }
-            Dl += (double)(dv.x * du.x + dv.y * du.y);
-            Dm += (double)sm2;
-            Dr += -(double)w * (double)w * (double)vu;
+            add_sel(Dl, p, (double)(dv.x * du.x + dv.y * du.y));
+            add_sel(Dm, p, (double)sm2);
+            add_sel(Dr, p, -(double)w * (double)w * (double)vu);
+          }
         }
     }
     // projection on the anchor hat functions (weights recomputed here, as the
     // quadrature loop above computes them) and block reduction of the 12 sums
-    double fw[4] = {0.0, 0.0, 0.0, 0.0};
-    if (act) {
-        Geo g;
-        if (geo_at(xq[qx], xq[qe], ie, xl, zl, g)) g.det = 1.0;
-        const double wq2 = wq[qx] * wq[qe] * g.det;
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) fw[kk] = wq2 * g.sf[kk];
-    }
     double g12[12];
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-        g12[kk * 3 + 0] = fw[kk] * Dl;
-        g12[kk * 3 + 1] = fw[kk] * Dm;
-        g12[kk * 3 + 2] = fw[kk] * Dr;
+    for (int i = 0; i < 12; ++i) g12[i] = 0.0;
+#pragma unroll
+    for (int p = 0; p < PPL; ++p) {
+        double fw[4] = {0.0, 0.0, 0.0, 0.0};
+        if (act[p]) {
+            Geo g;
+            if (geo_at(xq[qx], xq[qe[p]], ie, xl, zl, g)) g.det = 1.0;
+            const double wq2 = wq[qx] * wq[qe[p]] * g.det;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) fw[kk] = wq2 * g.sf[kk];
+        }
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            g12[kk * 3 + 0] += fw[kk] * Dl[p];
+            g12[kk * 3 + 1] += fw[kk] * Dm[p];
+            g12[kk * 3 + 2] += fw[kk] * Dr[p];
+        }
     }
 #pragma unroll
     for (int i = 0; i < 12; ++i)
