@@ -862,19 +862,27 @@ static int dispatch_apply(f25_mesh *m, const double *om, const double *kv, int n
     FAIL(F25_ERR_ARG, "unsupported order");
 }
 
+// workgroups of one k_apply_v launch over n elements: per XCD (eight), its
+// ceil(n / 8) elements times the pair groups times WPT waves, four waves
+// per workgroup
+template <int NL, int NRT>
+static size_t apply_v_grid(int n, int npair) {
+    const size_t ngrp = (size_t)(npair + 63) / 64, wpt = VShape<NL, NRT>::WPT;
+    return 8 * (((size_t)(n + 7) / 8 * ngrp * wpt + 3) / 4);
+}
+
 // FP64 product (k_apply_v): one launch per element colour
 template <int NL, int NRT>
 static int launch_apply_v(f25_mesh *m, const double *om, const double *kv, int npair, const double2 *x,
                           double2 *y, const double *vcf) {
     const double2 *zpad = reinterpret_cast<const double2 *>(m->d_vpad);
     double2 *sink = reinterpret_cast<double2 *>(m->d_vpad) + m->vpad_cap;
-    const int ngrp = (npair + 63) / 64;
     for (size_t c = 0; c + 1 < m->cptr.size(); ++c) {
         int n = m->cptr[c + 1] - m->cptr[c];
         if (!n) continue;
-        const size_t nwg = (size_t)((n + 7) / 8) * 8 * ngrp;
+        const size_t nwg = apply_v_grid<NL, NRT>(n, npair);
         if (nwg > 0x7fffffff) FAIL(F25_ERR_ARG, "too many pairs for one launch");
-        hipLaunchKernelGGL((k_apply_v<NL, NRT>), dim3((unsigned)nwg), dim3(64 * 3 * NRT), 0, m->st,
+        hipLaunchKernelGGL((k_apply_v<NL, NRT>), dim3((unsigned)nwg), dim3(VShape<NL, NRT>::NT), 0, m->st,
                            m->d_order + m->cptr[c], n, vcf, m->d_lm, m->d_lm, m->d_first, npair, om, kv, x, y,
                            zpad, sink);
     }
@@ -1302,13 +1310,12 @@ static int launch_ring(f25_mesh *m, int pass, const double *om, const double *kv
     auto &B = m->br;
     const double2 *zpad = reinterpret_cast<const double2 *>(m->d_vpad);
     double2 *sink = reinterpret_cast<double2 *>(m->d_vpad) + m->vpad_cap;
-    const int ngrp = (npair + 63) / 64;
     for (size_t c = 0; c + 1 < B.cptr.size(); ++c) {
         const int n = B.cptr[c + 1] - B.cptr[c];
         if (!n) continue;
-        const size_t nwg = (size_t)((n + 7) / 8) * 8 * ngrp;
+        const size_t nwg = apply_v_grid<NL, NRT>(n, npair);
         if (nwg > 0x7fffffff) FAIL(F25_ERR_ARG, "too many pairs for one launch");
-        hipLaunchKernelGGL((k_apply_v<NL, NRT>), dim3((unsigned)nwg), dim3(64 * 3 * NRT), 0, m->st,
+        hipLaunchKernelGGL((k_apply_v<NL, NRT>), dim3((unsigned)nwg), dim3(VShape<NL, NRT>::NT), 0, m->st,
                            B.d_order + B.cptr[c], n, m->d_vcf, B.d_lm[2 * pass], B.d_lm[2 * pass + 1], B.d_first0,
                            npair, om, kv, x, y, zpad, sink);
     }

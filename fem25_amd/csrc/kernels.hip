@@ -613,6 +613,12 @@ struct VShape {
     static constexpr int CW = (5 * RT + 31) / 32 * 32;          // values per node
     static constexpr int NQ = CW / 32;                          // loads per node
     static constexpr int MINW = RT <= 13 ? 4 : 3;               // waves per SIMD the registers allow
+    // a task (element, group of 64 pairs) is 3 NRT independent waves; a
+    // workgroup is four waves of consecutive tasks (a workgroup holds
+    // ceil(waves / 4) wave slots on each of the CU's four SIMDs: four-wave
+    // workgroups fill every slot the registers allow)
+    static constexpr int WPT = 3 * NRT;
+    static constexpr int NT = 256;
 };
 
 // Sub-blocks of output component c (indices into kSub[c]) that act on input
@@ -812,21 +818,24 @@ __device__ __forceinline__ void apply_v_tile(const double *__restrict__ cf, cons
     }
 }
 
-// One workgroup = one element of the colour and one group of 64 pairs: a
-// wave per (component, row tile), 3 NRT waves, all reading the same x rows
-// at about the same time (one fetch from HBM serves them).  The pair groups
-// of one element read the same coefficients: workgroup index
-// i = (e / 8) * 8 ngrp + 8 g + e % 8 deals them to one XCD (workgroups go
-// to the 8 XCDs round robin).  168 registers: three waves per SIMD, four
-// workgroups per CU.  (Measured at 200x100 elements, p4, 128 pairs: two
-// pair groups per workgroup, 6 waves, fit only two workgroups per CU and
-// ran 0.69 instead of 0.63 ms per launch; 13-row tiles at four waves per
-// SIMD ran 0.72 ms.)  lm maps the element's nodes to the DOFs the input is
+// A task = one element of the colour and one group of 64 pairs: a wave per
+// (component, row tile), 3 NRT waves that share nothing but the x rows
+// they read at about the same time (one fetch from HBM serves them).  The
+// pair groups of one element read the same coefficients: all tasks of an
+// element go to one XCD (workgroups go to the 8 XCDs round robin).  A
+// workgroup is four consecutive waves of its XCD's task list, whatever
+// tasks they belong to: a workgroup holds ceil(waves / 4) wave slots on
+// each of the CU's four SIMDs, so four-wave workgroups fill every slot the
+// registers allow (twelve waves per CU at 164 registers), where one
+// six-wave task per workgroup (p6, two row tiles) left one workgroup, six
+// waves, per CU: 400x200 p6, 192 pairs, 12.1 -> 10.5 ms per launch.
+// (Measured at 200x100 elements, p4, 128 pairs: 13-row tiles at four
+// waves per SIMD ran 0.72 ms against 0.63.)  lm maps the element's nodes to the DOFs the input is
 // read from, lm_out to the DOFs its rows go to (-1: read zero / store to
 // the sink): the same array for the operator, DOF-class masks for the
 // Bielak effective force.
 template <int NL, int NRT>
-__global__ __launch_bounds__((64 * 3 * NRT), (VShape<NL, NRT>::MINW)) void k_apply_v(
+__global__ __launch_bounds__((VShape<NL, NRT>::NT)) __attribute__((amdgpu_waves_per_eu(VShape<NL, NRT>::MINW))) void k_apply_v(
         const int *__restrict__ order, int nel, const double *__restrict__ vcf, const int *__restrict__ lm,
         const int *__restrict__ lm_out, const unsigned long long *__restrict__ first, int npair,
         const double *__restrict__ omega,
@@ -837,11 +846,16 @@ __global__ __launch_bounds__((64 * 3 * NRT), (VShape<NL, NRT>::MINW)) void k_app
     constexpr int NEN = V::NEN;
     constexpr size_t TS = (size_t)NEN * V::CW;   // values per (element, component, row tile)
     const int ngrp = (npair + 63) >> 6;
-    const int i = blockIdx.x;
-    const int ei = (i / (8 * ngrp)) * 8 + i % 8, g = (i / 8) % ngrp;
+    // workgroup j runs on XCD j % 8 and takes that XCD's waves
+    // 4 (j / 8) .. + 3: wave wx is wave wx % WPT of task tau = wx / WPT,
+    // tau = (element block) ngrp + group, element 8 block + j % 8: all pair
+    // groups of an element on one XCD
+    const int j = blockIdx.x;
+    const int wx = (j / 8) * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave of this XCD
+    const int tau = wx / V::WPT, ct = wx % V::WPT;
+    const int ei = (tau / ngrp) * 8 + j % 8, g = tau % ngrp;
     if (ei >= nel) return;
     const int e = order[ei];
-    const int ct = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     const unsigned long long fmask = first[e];
     const int *lme = lm + (size_t)e * 3 * NEN, *lmo = lm_out + (size_t)e * 3 * NEN;
