@@ -2335,7 +2335,7 @@ __device__ inline void dab_epilogue(int mode, int nvec, int it, int row, const i
 // every SIMD; at three waves per SIMD one fits) and four four-wave ones at
 // p4.
 template <int NL, bool IMAG>
-__global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mode, int nitems, int nvec, double kp, double ks,
+__global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) __attribute__((amdgpu_waves_per_eu(4))) void k_dab_sf(int mode, int nitems, int nvec, double kp, double ks,
                                                 double kv, const int *__restrict__ item_elem,
                                                 const double *__restrict__ item_coef,
                                                 const double *__restrict__ t1d, const double *__restrict__ wq,
@@ -2353,6 +2353,10 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
     // lanes after them xi point 16 of eta rows 0..16 (five waves)
     constexpr bool OVF = Q1 > 16;
     constexpr int NT = OVF ? 320 : 256;
+    // first-pass and projection items: (c, eta point, i) when 3 Q1 N1 fit
+    // the threads, else (c, eta pair qh and qh + QH, i)
+    constexpr int QH = 3 * Q1 * N1 <= NT ? Q1 : (Q1 + 1) / 2;
+    static_assert(3 * QH * N1 <= NT, "one round of first-pass and projection items");
     static_assert(Q1 <= 17, "one row of 16 lanes per eta point, plus one column");
     extern __shared__ double sm[];
     __shared__ double red[2][8];
@@ -2408,18 +2412,28 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
     double2 val[2] = {};
     for (int v = 0; v < nvec; ++v) {
         __syncthreads();   // su written; the previous vector is done with sT, sH, sP
-        for (int k = tid; k < 3 * Q1 * N1; k += NT) {
-            const int i = k % N1, qy = (k / N1) % Q1, c = k / (N1 * Q1);
-            double2 t1 = make_double2(0.0, 0.0), t2 = t1;
+        // one item per (c, eta point pair qh, qh + QH, i): a single round of
+        // the workgroup's threads also at p6 (3 Q1 N1 = 357 > 320)
+        for (int it = tid; it < 3 * QH * N1; it += NT) {
+            const int i = it % N1, qh = (it / N1) % QH, c = it / (N1 * QH);
+            double2 a[N1];
 #pragma unroll
-            for (int j = 0; j < N1; ++j) {
-                const double2 a = su[v * nee + (j * N1 + i) * 3 + c];
-                const double vv = v1[j * Q1 + qy], dd = d1[j * Q1 + qy];
-                t1.x += vv * a.x; t1.y += vv * a.y;
-                t2.x += dd * a.x; t2.y += dd * a.y;
+            for (int j = 0; j < N1; ++j) a[j] = su[v * nee + (j * N1 + i) * 3 + c];
+#pragma unroll
+            for (int h = 0; h < Q1 / QH + (Q1 % QH ? 1 : 0); ++h) {
+                const int qy = qh + h * QH;
+                if (qy >= Q1) break;
+                double2 t1 = make_double2(0.0, 0.0), t2 = t1;
+#pragma unroll
+                for (int j = 0; j < N1; ++j) {
+                    const double vv = v1[j * Q1 + qy], dd = d1[j * Q1 + qy];
+                    t1.x += vv * a[j].x; t1.y += vv * a[j].y;
+                    t2.x += dd * a[j].x; t2.y += dd * a[j].y;
+                }
+                const int kk = (c * Q1 + qy) * N1 + i;
+                sT[kk * 2] = t1;
+                sT[kk * 2 + 1] = t2;
             }
-            sT[k * 2] = t1;
-            sT[k * 2 + 1] = t2;
         }
         __syncthreads();
         {
@@ -2476,20 +2490,39 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256)) void k_dab_sf(int mo
             }
         }
         __syncthreads();
-        for (int k = tid; k < 3 * Q1 * N1; k += NT) {
-            const int i = k % N1, qy = (k / N1) % Q1, c = k / (N1 * Q1);
-            const double2 *A = sH + (c * 3 + 0) * QQ + qy * Q1, *B = sH + (c * 3 + 1) * QQ + qy * Q1,
-                          *Cc = sH + (c * 3 + 2) * QQ + qy * Q1;
-            double2 pa = make_double2(0.0, 0.0), pb = pa, pc = pa;
-            for (int x = 0; x < Q1; ++x) {
+        for (int it = tid; it < 3 * QH * N1; it += NT) {
+            const int i = it % N1, qh = (it / N1) % QH, c = it / (N1 * QH);
+            constexpr int NH = Q1 / QH + (Q1 % QH ? 1 : 0);
+            double2 pa[NH], pb[NH], pc[NH];
+#pragma unroll
+            for (int h = 0; h < NH; ++h) pa[h] = pb[h] = pc[h] = make_double2(0.0, 0.0);
+            auto xstep = [&](int x) {
                 const double vv = v1[i * Q1 + x], dd = d1[i * Q1 + x];
-                pa.x += dd * A[x].x; pa.y += dd * A[x].y;
-                pb.x += vv * B[x].x; pb.y += vv * B[x].y;
-                pc.x += vv * Cc[x].x; pc.y += vv * Cc[x].y;
+#pragma unroll
+                for (int h = 0; h < NH; ++h) {
+                    const int qy = min(qh + h * QH, Q1 - 1);
+                    const double2 A = sH[(c * 3 + 0) * QQ + qy * Q1 + x], B = sH[(c * 3 + 1) * QQ + qy * Q1 + x],
+                                  Cc = sH[(c * 3 + 2) * QQ + qy * Q1 + x];
+                    pa[h].x += dd * A.x; pa[h].y += dd * A.y;
+                    pb[h].x += vv * B.x; pb[h].y += vv * B.y;
+                    pc[h].x += vv * Cc.x; pc[h].y += vv * Cc.y;
+                }
+            };
+            if constexpr (NH > 1) {   // two eta points: a rolled loop keeps the registers at 127
+#pragma unroll 1
+                for (int x = 0; x < Q1; ++x) xstep(x);
+            } else {
+                for (int x = 0; x < Q1; ++x) xstep(x);
             }
-            sP[k * 3] = pa;
-            sP[k * 3 + 1] = pb;
-            sP[k * 3 + 2] = pc;
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                const int qy = qh + h * QH;
+                if (qy >= Q1) break;
+                const int kk = (c * Q1 + qy) * N1 + i;
+                sP[kk * 3] = pa[h];
+                sP[kk * 3 + 1] = pb[h];
+                sP[kk * 3 + 2] = pc[h];
+            }
         }
         __syncthreads();
         if (orow >= 0) {
