@@ -3071,7 +3071,7 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
     const int ntsf = q1 > 16 ? 320 : 256;
     const size_t lsf = (size_t)(2 * nl * q1 + 2 * q1) * sizeof(double) +
                        (size_t)std::max(nee + 6 * q1 * nl, 28 * q1 * nl) * sizeof(double2) +
-                       (q1 > 16 ? (size_t)17 * q1 * sizeof(double2) : 0);
+                       (q1 > 16 ? (size_t)17 * q1 * sizeof(double2) + (size_t)2 * nl * q1 * sizeof(double) : 0);
     const int nt2 = 256;
     const size_t lds2 = (size_t)m->ncon_anchor * nee * 3 * sizeof(double2);
     for (int k = 0; k < 2 && !r; ++k) {

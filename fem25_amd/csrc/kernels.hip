@@ -1989,6 +1989,10 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     double2 *sT = su + nee;                                   // [3][Q1][N1][2] T1, T2
     double2 *sP = su;   // [14][2][Q1][N1], written once su and sT are read
     double2 *sF = su + REG;   // [17][Q1] point fields of the xi = 16 column (Q1 = 17)
+    // Q1 = 17: the derivative row-sum coefficients h_cx v'_bi in LDS
+    // [2: cx][N1][Q1] (121 instead of 160 registers: two five-wave
+    // workgroups per CU instead of one; below p6 they stay in registers)
+    double *sc = reinterpret_cast<double *>(sF + (OVF ? 17 * Q1 : 0));
     const int e = blockIdx.x, tid = threadIdx.x;
     double2 *o = out + (size_t)e * 12 * nee;
     if (cdom[e] == 'A') {   // PML elements are outside the non-PML gradient path
@@ -2010,6 +2014,11 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     const double w = wq[qx] * wq[qe] * g.det;
     for (int i = tid; i < 2 * N1 * Q1; i += NT) s1[i] = t1d[i];
     for (int i = tid; i < 2 * Q1; i += NT) sh[i] = 0.5 * (i < Q1 ? 1.0 - xq[i] : 1.0 + xq[i - Q1]);
+    if constexpr (OVF)
+        for (int i = tid; i < 2 * N1 * Q1; i += NT) {
+            const int q = i % Q1, bi = (i / Q1) % N1, cx = i / (Q1 * N1);
+            sc[i] = 0.5 * (cx ? 1.0 + xq[q] : 1.0 - xq[q]) * t1d[(N1 + bi) * Q1 + q];
+        }
     for (int i = tid; i < nee; i += NT) {
         const int d = L[i];
         su[i] = d >= 0 ? u[d] : make_double2(0.0, 0.0);
@@ -2086,13 +2095,14 @@ __global__ __launch_bounds__((2 * NL + 3 > 16 ? 320 : 256), (NL <= 5 ? 4 : 3)) v
     }
     // output slot of this lane in stage 1: (cx, bi), lanes 2 N1.. clamped
     const int slot = min(x16, 2 * N1 - 1), cx = slot / N1, bi = slot % N1;
-    double cv[Q1], cd[Q1];
+    double cv[Q1], cdr[OVF ? 1 : Q1];
 #pragma unroll
     for (int q = 0; q < Q1; ++q) {
         const double h = sh[cx * Q1 + q];
         cv[q] = h * s1[bi * Q1 + q];
-        cd[q] = h * s1[(N1 + bi) * Q1 + q];
+        if constexpr (!OVF) cdr[q] = h * s1[(N1 + bi) * Q1 + q];
     }
+    const double *cd = OVF ? sc + (cx * N1 + bi) * Q1 : cdr;
     __syncthreads();   // su and sT are read: sP may overwrite them
     const bool wr = row < Q1 && x16 < 2 * N1;
     const auto seq = std::make_integer_sequence<int, QD>();
