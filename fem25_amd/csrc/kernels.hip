@@ -1513,7 +1513,7 @@ __global__ void k_peff(int ndofl, const int *__restrict__ mydofs, const int64_t 
 // instead of kept in LDS: 128 registers and 37 KB of LDS per workgroup at
 // p4, four workgroups per CU.  At p6 the FP64 form takes two points per
 // lane and one triple per pass: three waves, 168 registers and 29.5 KB of
-// LDS, three workgroups per CU.
+// LDS, four workgroups (twelve waves) per CU.
 template <int NL, int I, int... J>
 __device__ __forceinline__ void sf_row_sums(double2 &U, double2 &UX, double2 &UE, const double2 &T1,
                                             const double2 &T2, const double *vx, const double *dx,
@@ -1568,11 +1568,11 @@ __device__ __forceinline__ void add_sel(double (&a)[N], int p, double v) {
 // (at Q1 = 17 plus a column of 17 lanes for xi point 16), four waves up to
 // p5 and five at p6.  The FP64 form at p6 takes two points per lane: row r
 // (r < H = 9) holds eta points r and r + H, and nine lanes after the rows
-// hold the xi = 16 column's pairs; three waves.  A workgroup's waves are
-// placed with ceil(waves / 4) of them on every SIMD, so at the FP64 form's
-// three waves per SIMD one five-wave workgroup fits per CU, three
-// three-wave ones.  Up to p4 FP64 keeps four workgroups per CU; FP32 five
-// waves per SIMD (its registers allow it).
+// hold the xi = 16 column's pairs; three waves.  Measured residency at the
+// FP64 form's three waves per SIMD: one five-wave workgroup per CU (a
+// workgroup of four or more waves holds ceil(waves / 4) slots on every
+// SIMD), four three-wave ones.  Up to p4 FP64 keeps four workgroups per
+// CU; FP32 five waves per SIMD (its registers allow it).
 template <int NL, typename R>
 struct GradOcc {
     static constexpr int Q1 = 2 * NL + 3;
