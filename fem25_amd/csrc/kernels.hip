@@ -1566,9 +1566,10 @@ __device__ __forceinline__ void add_sel(double (&a)[N], int p, double v) {
 
 // Launch shape.  One point per lane: rows of 16 lanes over the eta points
 // (at Q1 = 17 plus a column of 17 lanes for xi point 16), four waves up to
-// p5 and five at p6.  The FP64 form at p6 takes two points per lane: row r
-// (r < H = 9) holds eta points r and r + H, and nine lanes after the rows
-// hold the xi = 16 column's pairs; three waves.  Measured residency at the
+// p5 and five at p6.  The FP64 form at p6 takes two points per lane in its
+// first two waves (row r < 8 holds eta points r and r + 9) and one in the
+// third (eta 8, then the xi = 16 column), which therefore runs the point
+// stage once per triple instead of twice; three waves.  Measured residency at the
 // FP64 form's three waves per SIMD: one five-wave workgroup per CU (a
 // workgroup of four or more waves holds ceil(waves / 4) slots on every
 // SIMD), four three-wave ones.  Up to p4 FP64 keeps four workgroups per
@@ -1654,15 +1655,32 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT)) __attribute__((amdgpu_waves_p
     // second stage: row = eta point, lane = xi point (clamped when idle: an
     // idle lane computes a copy and contributes nothing)
     const int row = tid >> 4, x16 = tid & 15;
-    const bool ovl = OVF && tid >= 16 * H && tid < 16 * H + H;   // a lane of the xi = 16 column
-    const int e0 = ovl ? tid - 16 * H : min(row, H - 1), qx = ovl ? 16 : min(x16, Q1 - 1);
-    // the lane's points: eta e0 + p H (clamped copies where idle)
-    int qe[PPL];
-    bool act[PPL];
-#pragma unroll
-    for (int p = 0; p < PPL; ++p) {
-        qe[p] = min(e0 + p * H, Q1 - 1);
-        act[p] = ((row < H && x16 < Q1) || ovl) && e0 + p * H < Q1;
+    int qe[PPL], qx;
+    bool act[PPL], ovl;
+    // points of this lane's wave: PPL, or one for the last wave of the
+    // two-point form
+    int npw = PPL;
+    if constexpr (PPL == 1) {
+        // rows 0..Q1-1: eta = row; at Q1 = 17 the lanes 16 Q1 .. 17 Q1 - 1
+        // hold xi = 16 of eta 0..16 (clamped copies where idle)
+        ovl = OVF && tid >= 16 * Q1 && tid < 17 * Q1;
+        qe[0] = ovl ? tid - 16 * Q1 : min(row, Q1 - 1);
+        qx = ovl ? 16 : min(x16, Q1 - 1);
+        act[0] = (row < Q1 && x16 < Q1) || ovl;
+    } else {
+        // Q1 = 17, three waves: rows 0..7 (waves 0, 1) hold eta r and r + 9,
+        // xi 0..15; the last wave one point per lane: row 8 eta 8, then
+        // 17 lanes the xi = 16 column (eta 0..16), the rest idle
+        static_assert(PPL == 2 && Q1 == 17 && NT == 192, "the two-point layout is the p6 one");
+        const bool two = tid < 128;
+        ovl = tid >= 144 && tid < 161;
+        const int eb = two ? row : (ovl ? tid - 144 : 8);
+        qe[0] = eb;
+        qe[1] = two ? row + 9 : eb;
+        qx = ovl ? 16 : x16;
+        act[0] = two || row == 8 || ovl;
+        act[1] = two;
+        npw = two ? 2 : 1;
     }
     const int ni = min(x16, N1 - 1);
     // the 1-D values at the lane's xi point: in registers, or with two
@@ -1763,7 +1781,7 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT)) __attribute__((amdgpu_waves_p
           // the lane's points one after the other (a loop, not unrolled: two
           // points' live values at once would exceed the registers)
 #pragma unroll 1
-          for (int p = 0; p < PPL; ++p) {
+          for (int p = 0; p < npw; ++p) {
             const R k = kt[tt], w = wt[tt];
             const int qp = sel(qe, p);
             const R rxp = sel(rx, p), sxp = sel(sx, p), rzp = sel(rz, p), szp = sel(sz, p);
