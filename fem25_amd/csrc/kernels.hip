@@ -1579,7 +1579,6 @@ struct GradOcc {
     static constexpr int Q1 = 2 * NL + 3;
     static constexpr bool F64 = std::is_same<R, double>::value;
     static constexpr int PPL = (F64 && Q1 > 16) ? 2 : 1;   // quadrature points per lane
-    static constexpr int H = (Q1 + PPL - 1) / PPL;         // eta rows per point slot
     static constexpr int NT = PPL == 2 ? 192 : (Q1 > 16 ? 320 : 256);
     // waves per SIMD the registers must allow
     static constexpr int WPE = !F64 ? 5 : (NL <= 5 ? 4 : 3);
@@ -1600,8 +1599,8 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT)) __attribute__((amdgpu_waves_p
     // Q1 = 17 (p6): rows of 16 lanes hold xi points 0..15 of H eta points
     // each (PPL per lane) and the H lanes after them xi point 16
     constexpr bool OVF = Q1 > 16;
-    constexpr int PPL = GradOcc<NL, R>::PPL, H = GradOcc<NL, R>::H, NT = GradOcc<NL, R>::NT;
-    static_assert(16 * H + (OVF ? H : 0) <= NT, "lanes for every point slot");
+    constexpr int PPL = GradOcc<NL, R>::PPL, NT = GradOcc<NL, R>::NT;
+    static_assert(PPL == 2 || 16 * Q1 + (OVF ? Q1 : 0) <= NT, "a lane for every point");
     constexpr int NW = T * 2 * nee;            // element values gathered per pass
     constexpr int NG = (NW + NT - 1) / NT;     // of them per thread
     static_assert(Q1 <= 17, "one row of 16 lanes per eta point, plus one column");
