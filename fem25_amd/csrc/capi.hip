@@ -1552,6 +1552,12 @@ extern "C" int f25_apply_batch(f25_mesh *m, int npair, const double *omega, cons
 }
 
 // ---------------------------------------------------------------------------
+// quadrature points per lane of k_grad_sf for the order
+template <typename R>
+static int grad_ppl(int nl) {
+    return nl == 7 ? GradOcc<7, R>::PPL : 1;
+}
+
 // threads of a k_grad_sf workgroup (its launch shape for the order)
 template <typename R>
 static int grad_threads(int nl) {
@@ -1587,9 +1593,11 @@ static int grad_launch(f25_mesh *m, int ntrip, const double *d_om, const double 
     if (!ksf) FAIL(F25_ERR_ARG, "unsupported order");
     const int nl = m->tab.nl, q1 = m->tab.nq1;
     // (the FP32 form adds the inverse Jacobian at the points, 4 values each)
+    // (the FP64 two-point form adds the interleaved 1-D table, 2 N1 Q1 doubles)
     const size_t lsf = (size_t)TS * 2 * 3 * nen * sizeof(C2) + (size_t)TS * 2 * 3 * q1 * nl * 2 * sizeof(C2) +
                        (size_t)2 * nl * q1 * sizeof(R) +
-                       (std::is_same<R, float>::value ? (size_t)4 * q1 * q1 * sizeof(R) : 0);
+                       (std::is_same<R, float>::value ? (size_t)4 * q1 * q1 * sizeof(R) : 0) +
+                       (grad_ppl<R>(nl) == 2 ? (size_t)2 * nl * q1 * sizeof(double) : 0);
     if (lsf > 64 * 1024)
         HIPCHK(hipFuncSetAttribute((const void *)ksf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lsf));
     HIPCHK(hipEventRecord(m->ev0, m->st));

@@ -1536,6 +1536,17 @@ __device__ __forceinline__ void sf_row_sums_lds(double2 &U, double2 &UX, double2
      ...);
 }
 
+// The same sums from an interleaved LDS table: si[J] = (v_J, v'_J) at the
+// lane's xi point (one 16-byte read per node instead of two 8-byte ones).
+template <int... J>
+__device__ __forceinline__ void sf_row_sums_lds2(double2 &U, double2 &UX, double2 &UE, const double2 &T1,
+                                                 const double2 &T2, const double2 *si,
+                                                 std::integer_sequence<int, J...>) {
+    ((fmac_bcast<J>(U.x, T1.x, si[J].x), fmac_bcast<J>(U.y, T1.y, si[J].x), fmac_bcast<J>(UX.x, T1.x, si[J].y),
+      fmac_bcast<J>(UX.y, T1.y, si[J].y), fmac_bcast<J>(UE.x, T2.x, si[J].x), fmac_bcast<J>(UE.y, T2.y, si[J].x)),
+     ...);
+}
+
 // R = double: FP64 fields and arithmetic.  R = float: complex64 fields, the
 // interpolation and the densities of each triple in FP32, their sums over
 // triples, quadrature points and corners in FP64 (the FP32 gradient); its
@@ -1609,6 +1620,10 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT)) __attribute__((amdgpu_waves_p
     C2 *sT = sw + NW;                                                  // [T][2][3][Q1][N1][2]
     R *s1 = reinterpret_cast<R *>(sT + T * 2 * 3 * Q1 * N1 * 2);       // [2][N1][Q1]
     R *sG = s1 + 2 * N1 * Q1;                                          // FP32: [Q1*Q1][4]
+    // two points per lane (FP64, p6): the 1-D values also interleaved,
+    // si[q N1 + j] = (v_j(q), v'_j(q))
+    constexpr bool XL = PPL == 2;
+    double2 *si = reinterpret_cast<double2 *>(s1 + 2 * N1 * Q1);
     __shared__ double red[NT / 64][12];
     const int e = blockIdx.x, tid = threadIdx.x;
     if (cdom[e] == 'A') {   // PML elements are outside the non-PML gradient path
@@ -1618,6 +1633,11 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT)) __attribute__((amdgpu_waves_p
     const int *ie = ieng + 4 * e;
     const int *L = lm + (size_t)e * nee;
     for (int i = tid; i < 2 * N1 * Q1; i += NT) s1[i] = (R)t1d[i];   // v_j(eta_q), then v'_j(eta_q)
+    if constexpr (XL)
+        for (int i = tid; i < N1 * Q1; i += NT) {
+            const int q = i / N1, j = i % N1;
+            si[i] = make_double2(t1d[j * Q1 + q], t1d[(N1 + j) * Q1 + q]);
+        }
     if constexpr (!F64) {
         // FP32: the inverse Jacobian at every point in LDS (registers are the
         // FP32 form's occupancy limit; the FP64 form keeps its point's
@@ -1683,9 +1703,8 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT)) __attribute__((amdgpu_waves_p
     }
     const int ni = min(x16, N1 - 1);
     // the 1-D values at the lane's xi point: in registers, or with two
-    // points per lane read from the LDS table s1 (registers are that form's
+    // points per lane read from the LDS table si (registers are that form's
     // occupancy limit)
-    constexpr bool XL = PPL == 2;
     R vx[N1], dx[N1];
     if constexpr (!XL) {
 #pragma unroll
@@ -1734,7 +1753,13 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT)) __attribute__((amdgpu_waves_p
                     C2 t1 = {R(0), R(0)}, t2 = t1;
 #pragma unroll
                     for (int j = 0; j < N1; ++j) {
-                        const R vv = s1[j * Q1 + qy], dd = s1[(N1 + j) * Q1 + qy];
+                        R vv, dd;
+                        if constexpr (XL) {
+                            const double2 vd = si[qy * N1 + j];
+                            vv = vd.x; dd = vd.y;
+                        } else {
+                            vv = s1[j * Q1 + qy]; dd = s1[(N1 + j) * Q1 + qy];
+                        }
                         t1.x += vv * a[j].x; t1.y += vv * a[j].y;
                         t2.x += dd * a[j].x; t2.y += dd * a[j].y;
                     }
@@ -1796,8 +1821,7 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT)) __attribute__((amdgpu_waves_p
                 bool bc = !ovl;
                 if constexpr (F64) {
                     if (bc && XL)
-                        sf_row_sums_lds<NL, Q1>(U, UXI, UET, T1, T2, (const double *)s1 + qx,
-                                                std::make_integer_sequence<int, N1>());
+                        sf_row_sums_lds2(U, UXI, UET, T1, T2, si + qx * N1, std::make_integer_sequence<int, N1>());
                     else if (bc)
                         sf_row_sums<NL, 0>(U, UXI, UET, T1, T2, (const double *)vx, (const double *)dx,
                                            std::make_integer_sequence<int, N1>());
@@ -1809,7 +1833,7 @@ __global__ __launch_bounds__((GradOcc<NL, R>::NT)) __attribute__((amdgpu_waves_p
 #pragma unroll
                     for (int j = 0; j < N1; ++j) {
                         const C2 a1 = r[2 * j], a2 = r[2 * j + 1];
-                        const R vj = XL ? s1[j * Q1 + qx] : vx[j], dj = XL ? s1[(N1 + j) * Q1 + qx] : dx[j];
+                        const R vj = XL ? si[qx * N1 + j].x : vx[j], dj = XL ? si[qx * N1 + j].y : dx[j];
                         U.x += vj * a1.x; U.y += vj * a1.y;
                         UXI.x += dj * a1.x; UXI.y += dj * a1.y;
                         UET.x += vj * a2.x; UET.y += vj * a2.y;
