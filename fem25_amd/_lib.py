@@ -106,6 +106,7 @@ EXPORTS = {
     "f25_last_error": (C.c_char_p, []),
     "f25_last_kernel_ms": (C.c_double, [C.c_char_p]),
     "f25_selftest_mfma": (C.c_int, [dp]),
+    "f25_selftest_residency": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
 }
 
 _lib = None

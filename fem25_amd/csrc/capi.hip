@@ -3112,6 +3112,61 @@ extern "C" int f25_gn_diag(f25_mesh *m, int ntrip, const double *omega, const do
     return r;
 }
 
+// Workgroup residency: the most workgroups of nt threads, holding vgprs
+// registers per lane and lds bytes of LDS, resident at once on one CU
+// (median over the CUs of the per-CU maximum), from the realtime stamps of
+// 8 workgroups per CU that each spin ~40,000 cycles.
+extern "C" int f25_selftest_residency(int nt, int vgprs, int lds, int *groups_per_cu) {
+    if (!groups_per_cu) FAIL(F25_ERR_ARG, "null result pointer");
+    if (nt != 192 && nt != 256 && nt != 320 && nt != 384 && nt != 512)
+        FAIL(F25_ERR_ARG, "threads per workgroup: 192, 256, 320, 384 or 512");
+    if (vgprs != 128 && vgprs != 168) FAIL(F25_ERR_ARG, "registers per lane: 128 or 168");
+    if (lds < 0 || lds > 160 * 1024) FAIL(F25_ERR_ARG, "LDS bytes: 0 .. 160 KiB");
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const void *k = nullptr;
+#define F25_PICK(NT)                                                                        \
+    if (nt == NT) k = vgprs == 128 ? (const void *)k_resid_probe_128<NT> : (const void *)k_resid_probe_168<NT>;
+    F25_PICK(192) F25_PICK(256) F25_PICK(320) F25_PICK(384) F25_PICK(512)
+#undef F25_PICK
+    const int nwg = 8 * ncu;
+    unsigned long long *d = nullptr;
+    TRY(dalloc(&d, (size_t)3 * nwg));
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e == hipSuccess) e = hipMemset(d, 0, (size_t)3 * nwg * sizeof(unsigned long long));
+    unsigned long long spin = 40000;
+    int nw = nwg;
+    void *args[] = {&d, &nw, &spin};
+    if (e == hipSuccess) e = hipLaunchKernel(k, dim3(nwg), dim3(nt), args, (size_t)lds, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    std::vector<unsigned long long> h((size_t)3 * nwg);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), d, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    dfree(d);
+    if (e != hipSuccess) FAIL(F25_ERR_HIP, hipGetErrorString(e));
+    // per CU (XCC_ID, then HW_ID bits 8..15: CU, shader array, engine): the
+    // largest number of workgroups alive at one instant
+    std::map<unsigned long long, std::vector<std::pair<unsigned long long, int>>> ev;
+    for (int w = 0; w < nwg; ++w) {
+        const unsigned long long *o = &h[(size_t)3 * w];
+        if (!o[0] || o[1] < o[0]) FAIL(F25_ERR_HIP, "residency probe: a workgroup left no record");
+        const unsigned long long cu = ((o[2] >> 32) & 0xf) << 8 | ((o[2] >> 8) & 0xff);
+        ev[cu].push_back({o[0], 1});
+        ev[cu].push_back({o[1], -1});
+    }
+    std::vector<int> mx;
+    for (auto &kv : ev) {
+        auto &v = kv.second;
+        std::sort(v.begin(), v.end());   // at equal stamps an end (-1) sorts first
+        int cur = 0, m = 0;
+        for (auto &x : v) { cur += x.second; m = std::max(m, cur); }
+        mx.push_back(m);
+    }
+    std::sort(mx.begin(), mx.end());
+    *groups_per_cu = mx[mx.size() / 2];
+    return 0;
+}
+
 // MFMA operand/result layout check (used by the self test entry point)
 extern "C" int f25_selftest_mfma(double *maxerr) {
     double *dd = nullptr;

@@ -3217,4 +3217,35 @@ __global__ void k_mfma_probe(double *outd, float *outf) {
     }
 }
 
+// Workgroup residency probe: every wave spins a bounded number of shader
+// cycles, then lane 0 records the workgroup's realtime start and end and
+// its hardware location (HW_ID: CU, shader array, engine; XCC_ID) as
+// rec[3 wg + 0..2] (wave 0 only).  The register count is forced by an
+// inline-assembly clobber of the highest register (VG - 1), so the kernel
+// holds exactly the VGPR budget of the kernels it stands for.
+#define F25_RESID_PROBE(VG, VGM1)                                                                   \
+    template <int NT>                                                                               \
+    __global__ __launch_bounds__(NT) void k_resid_probe_##VG(unsigned long long *rec, int nwg,     \
+                                                             unsigned long long spin) {             \
+        const unsigned long long r0 = __builtin_amdgcn_s_memrealtime();                             \
+        const unsigned long long t0 = __builtin_amdgcn_s_memtime();                                 \
+        unsigned long long t = t0;                                                                  \
+        for (int it = 0; t - t0 < spin && it < (1 << 22); ++it) {                                   \
+            __builtin_amdgcn_s_sleep(1);                                                            \
+            t = __builtin_amdgcn_s_memtime();                                                       \
+        }                                                                                           \
+        asm volatile("; holds the register budget" ::: #VGM1);                                      \
+        const unsigned long long r1 = __builtin_amdgcn_s_memrealtime();                             \
+        if (threadIdx.x == 0 && (int)blockIdx.x < nwg) {                                            \
+            unsigned long long *o = rec + 3 * (size_t)blockIdx.x;                                   \
+            o[0] = r0;                                                                              \
+            o[1] = r1;                                                                              \
+            o[2] = (unsigned long long)__builtin_amdgcn_s_getreg((31 << 11) | 4) |                  \
+                   ((unsigned long long)__builtin_amdgcn_s_getreg((31 << 11) | 20) << 32);          \
+        }                                                                                           \
+    }
+F25_RESID_PROBE(128, v127)
+F25_RESID_PROBE(168, v167)
+#undef F25_RESID_PROBE
+
 }  // namespace f25k

@@ -444,6 +444,15 @@ const char *f25_last_error(void);
  * the operand/result lane layout the batched product relies on; *maxerr is
  * the largest deviation (0 when the layout holds). */
 int f25_selftest_mfma(double *maxerr);
+/* Workgroup residency on the current device: *groups_per_cu = the most
+ * workgroups of nt threads (192, 256, 320, 384 or 512) holding vgprs
+ * registers per lane (128 or 168) and lds bytes of LDS that are resident
+ * at once on one CU (median over the CUs), measured with spinning probe
+ * workgroups.  The launch shapes of the gradient, element-derivative and
+ * product kernels are chosen for what it reports (a workgroup of four or
+ * more waves holds ceil(waves / 4) wave slots on every SIMD of its CU).
+ * No counterpart in the reference (a diagnostic of this library). */
+int f25_selftest_residency(int nt, int vgprs, int lds, int *groups_per_cu);
 /* Duration (ms) of the kernels of the most recent call of the named family
  * ("apply", "grad", "asmble", "cfmat", "gn_diag", "pchadj", "gnmfree"),
  * measured with HIP events on the stream the kernels ran on; -1 if
