@@ -2603,11 +2603,14 @@ __global__ __launch_bounds__(256) void k_pchadj(int nnpgl, int nrhs, const int *
                                                  const float2 *__restrict__ gt, const float2 *__restrict__ f,
                                                  float2 *__restrict__ jac, float *__restrict__ hdiag) {
     // a lane owns right-hand sides r0 + lane + 64 j (j < 3) of a 192-wide
-    // pass; the column's nonzeros are walked two at a time so six rows of G
-    // are in flight per lane.  Entries of F that are exactly zero (a column
-    // of CFMAT_DIST holds only its last element's rows, about a third of the
-    // pattern) skip their row of G: the sums are unchanged for finite G.
-    constexpr int R = 3;
+    // pass.  The column's nonzeros are read 64 at a time, one per lane, with
+    // their rows (coalesced); a ballot keeps those whose F entry is not
+    // exactly zero (a column of CFMAT_DIST holds only its last element's
+    // rows, about a third of the pattern: the sums are unchanged for finite
+    // G), and the wave walks them four at a time in pattern order, their
+    // rows and values broadcast from the lane that read them: twelve rows
+    // of G in flight per lane, no dependent scalar loads in the loop.
+    constexpr int R = 3, U = 4;
     const int c = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
     if (c >= nnpgl) return;
@@ -2618,35 +2621,45 @@ __global__ __launch_bounds__(256) void k_pchadj(int nnpgl, int nrhs, const int *
         bool ok[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) { ar[j] = ai[j] = 0.0; ok[j] = r0 + lane + 64 * j < nrhs; }
-        int i = i1;
-        for (; i + 1 < i2; i += 2) {
-            const float2 fa = f[i], fb = f[i + 1];
-            const int ra = (fa.x != 0.f || fa.y != 0.f) ? icsc[i] - 1 : -1;
-            const int rb = (fb.x != 0.f || fb.y != 0.f) ? icsc[i + 1] - 1 : -1;
-            float2 ga[R], gb[R];
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const int r = r0 + lane + 64 * j;
-                ga[j] = (ok[j] && ra >= 0) ? gt[(size_t)ra * nrhs + r] : make_float2(0.f, 0.f);
-                gb[j] = (ok[j] && rb >= 0) ? gt[(size_t)rb * nrhs + r] : make_float2(0.f, 0.f);
+        for (int base = i1; base < i2; base += 64) {
+            const int i = base + lane;
+            float2 fl = make_float2(0.f, 0.f);
+            int rl = -1;
+            if (i < i2) {
+                fl = f[i];
+                if (fl.x != 0.f || fl.y != 0.f) rl = icsc[i] - 1;
             }
+            unsigned long long m = __ballot(rl >= 0);
+            while (m) {
+                int ru[U];
+                float fx[U], fy[U];
 #pragma unroll
-            for (int j = 0; j < R; ++j) {
-                ar[j] += (double)ga[j].x * fa.x - (double)ga[j].y * fa.y;
-                ai[j] += (double)ga[j].x * fa.y + (double)ga[j].y * fa.x;
-                ar[j] += (double)gb[j].x * fb.x - (double)gb[j].y * fb.y;
-                ai[j] += (double)gb[j].x * fb.y + (double)gb[j].y * fb.x;
-            }
-        }
-        if (i < i2) {
-            const float2 fa = f[i];
-            const int ra = (fa.x != 0.f || fa.y != 0.f) ? icsc[i] - 1 : -1;
+                for (int u = 0; u < U; ++u) {
+                    ru[u] = -1;
+                    fx[u] = fy[u] = 0.f;
+                    if (m) {
+                        const int k = __ffsll((long long)m) - 1;
+                        m &= m - 1;
+                        ru[u] = __builtin_amdgcn_readlane(rl, k);
+                        fx[u] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fl.x), k));
+                        fy[u] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fl.y), k));
+                    }
+                }
+                float2 g[U][R];
 #pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const int r = r0 + lane + 64 * j;
-                const float2 ga = (ok[j] && ra >= 0) ? gt[(size_t)ra * nrhs + r] : make_float2(0.f, 0.f);
-                ar[j] += (double)ga.x * fa.x - (double)ga.y * fa.y;
-                ai[j] += (double)ga.x * fa.y + (double)ga.y * fa.x;
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int j = 0; j < R; ++j)
+                        g[u][j] = (ok[j] && ru[u] >= 0) ? gt[(size_t)ru[u] * nrhs + r0 + lane + 64 * j]
+                                                        : make_float2(0.f, 0.f);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        if (ru[u] < 0) continue;
+                        ar[j] += (double)g[u][j].x * fx[u] - (double)g[u][j].y * fy[u];
+                        ai[j] += (double)g[u][j].x * fy[u] + (double)g[u][j].y * fx[u];
+                    }
             }
         }
 #pragma unroll
