@@ -48,7 +48,8 @@ def main() -> int:
     ap.add_argument("--config", default="c5p6")
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--skip-extras", action="store_true", help="only the timed FP64 product")
-    ap.add_argument("--only", choices=("grad", "grad32", "gn_diag", "cfmat", "qp", "met225"), default=None,
+    ap.add_argument("--only", choices=("grad", "grad32", "gn_diag", "cfmat", "qp", "met225", "pchadj"),
+                    default=None,
                     help="time one secondary kernel family alone (for profiling); prints its own JSON line")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one plain device copy of the input (known bytes) for counter scaling")
@@ -457,6 +458,19 @@ def only_kernel(args, mi, mesh, L, torch, omega, ky, freq, py, g):
         wave, lamv, nuv, alv = cv(), cv(), cv(), cv()
         if fam == "cfmat":
             fn, lib_fam = (lambda: cfmat_host(mi, inv, float(freq[0]), float(py[0]), wave)), b"cfmat"
+        elif fam == "pchadj":
+            # J = G^T F for 60 receivers x 3 components, F from CFMAT_DIST
+            import ctypes as C
+            from fem25_amd import _lib
+            fm, _ = cfmat_host(mi, inv, float(freq[0]), float(py[0]), wave)
+            nrhs = 180
+            gt = torch.randn(ndof, nrhs, dtype=torch.complex64, device="cuda", generator=g)
+            fd = torch.from_numpy(fm).cuda()
+            jac = torch.empty(inv.nnpgl, nrhs, dtype=torch.complex64, device="cuda")
+            hd = torch.zeros(inv.na35, dtype=torch.float32, device="cuda")
+            pargs = (mi.h, nrhs, C.c_void_p(gt.data_ptr()), C.c_void_p(fd.data_ptr()),
+                     C.c_void_p(jac.data_ptr()), C.c_void_p(hd.data_ptr()))
+            fn, lib_fam = (lambda: _lib.check(L.f25_pchadj_jac_dev(*pargs), "pchadj")), b"pchadj"
         elif fam == "qp":
             vq = rng.standard_normal(inv.na35).astype(np.float32)
             fn, lib_fam = (lambda: mi.qp_nu_alpha(float(freq[0]), float(py[0]), vq, lamv, wave)), b"qp"

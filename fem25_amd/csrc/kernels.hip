@@ -2607,10 +2607,13 @@ __global__ __launch_bounds__(256) void k_pchadj(int nnpgl, int nrhs, const int *
     // their rows (coalesced); a ballot keeps those whose F entry is not
     // exactly zero (a column of CFMAT_DIST holds only its last element's
     // rows, about a third of the pattern: the sums are unchanged for finite
-    // G), and the wave walks them four at a time in pattern order, their
-    // rows and values broadcast from the lane that read them: twelve rows
-    // of G in flight per lane, no dependent scalar loads in the loop.
-    constexpr int R = 3, U = 4;
+    // G), and the wave walks them two at a time in pattern order, their
+    // rows and values broadcast from the lane that read them: six rows of G
+    // in flight per lane, no dependent scalar loads in the loop, 62
+    // registers (eight waves per SIMD; four at a time took 98 registers and
+    // ran 5.20 against 3.50 ms, eight 6.54 ms: 400x200 p6, 180 right-hand
+    // sides).
+    constexpr int R = 3, U = 2;
     const int c = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
     if (c >= nnpgl) return;
