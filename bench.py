@@ -48,7 +48,7 @@ def main() -> int:
     ap.add_argument("--config", default="c5p6")
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--skip-extras", action="store_true", help="only the timed FP64 product")
-    ap.add_argument("--only", choices=("grad", "grad32", "gn_diag", "cfmat", "qp", "met225", "pchadj"),
+    ap.add_argument("--only", choices=("grad", "grad32", "gn_diag", "cfmat", "qp", "met225", "pchadj", "gnmfree"),
                     default=None,
                     help="time one secondary kernel family alone (for profiling); prints its own JSON line")
     ap.add_argument("--copy-calib", action="store_true",
@@ -452,15 +452,26 @@ def only_kernel(args, mi, mesh, L, torch, omega, ky, freq, py, g):
             del u, lam
             fn, lib_fam = (lambda: mi.gn_diag(omega[:8], ky[:8], uh)), b"gn_diag"
     else:
+        import ctypes as C
         inv = mi.inv_setup(b"PP")
         rng = np.random.default_rng(7)
         cv = lambda: (rng.standard_normal(ndof) + 1j * rng.standard_normal(ndof)).astype(np.complex64)  # noqa: E731
         wave, lamv, nuv, alv = cv(), cv(), cv(), cv()
         if fam == "cfmat":
             fn, lib_fam = (lambda: cfmat_host(mi, inv, float(freq[0]), float(py[0]), wave)), b"cfmat"
+        elif fam == "gnmfree":
+            # GNMFREE over 10 frequencies x 12 sources with 20..120 receivers each
+            from fem25_amd import _lib
+            nobs = (3 * rng.integers(20, 121, size=120)).astype(np.int32)
+            alpha = np.ones(120)
+            jt = torch.randn(int(nobs.sum()) * inv.nnpgl, dtype=torch.complex64, device="cuda", generator=g)
+            pv = torch.randn(inv.na35, dtype=torch.float64, device="cuda", generator=g)
+            zv = torch.zeros(inv.na35, dtype=torch.float64, device="cuda")
+            gargs = (mi.h, 120, nobs.ctypes.data_as(_lib.ip), alpha.ctypes.data_as(_lib.dp),
+                     C.c_void_p(jt.data_ptr()), C.c_void_p(pv.data_ptr()), C.c_void_p(zv.data_ptr()))
+            fn, lib_fam = (lambda: _lib.check(L.f25_gnmfree_dev(*gargs), "gnmfree")), b"gnmfree"
         elif fam == "pchadj":
             # J = G^T F for 60 receivers x 3 components, F from CFMAT_DIST
-            import ctypes as C
             from fem25_amd import _lib
             fm, _ = cfmat_host(mi, inv, float(freq[0]), float(py[0]), wave)
             nrhs = 180
